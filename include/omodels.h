@@ -1,0 +1,201 @@
+/*
+ * omodels.h -- C ABI of the MI355X-native opm-models hot path.
+ *
+ * The library keeps the three interfaces the reference's Newton loop sits on:
+ *   - Linearizer            (opm/models/discretization/common/tpfalinearizer.hh:116-379)
+ *   - LinearSolverBackend   (opm/simulators/linalg/parallelbasebackend.hh:137-290)
+ *   - NewtonMethod          (opm/models/nonlinear/newtonmethod.hh:112-438,
+ *                            opm/models/blackoil/blackoilnewtonmethod.hh:54-448)
+ * plus the time loop that calls them (opm/models/utils/simulator.hh:643-814,
+ * opm/models/discretization/common/fvbaseproblem.hh:512-606).
+ *
+ * Conventions (drop-in layouts of the reference):
+ *   - cells are numbered i + nx*(j + ny*k) (YaspGrid lexicographic, x fastest);
+ *   - the Jacobian is block-CSR with sorted column indices per row and
+ *     row-major 3x3 (or 2x2) blocks (Dune::BCRSMatrix<MatrixBlock>);
+ *   - residual, solution and update vectors are cell-major with the
+ *     equation / primary-variable index fastest (Dune::BlockVector);
+ *   - black-oil equations: 0 oil, 1 water, 2 gas component;
+ *     primary variables: 0 Sw, 1 po, 2 Sg or Rs (meaning byte 0 = Sg, 1 = Rs).
+ *
+ * Error convention: functions return 0 on success, 1 when a linear solve did
+ * not converge, 2 for a recoverable numerical problem (the reference's
+ * NumericalProblem, which makes the time loop halve the step), and a negative
+ * value for fatal errors.  omodels_last_error() returns a thread-local message.
+ *
+ * Threading: one handle per HIP stream; calls on one handle are not
+ * reentrant; different handles may be used from different threads.  Host
+ * arrays are borrowed for the duration of a call only.
+ */
+#ifndef OMODELS_H
+#define OMODELS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Problem description (host arrays, borrowed).  Same layout as ob_problem in
+ * the oracle.  Replaces the reference's Problem/Vanguard pair for this path:
+ * ReservoirProblem (tests/problems/reservoirproblem.hh:215-374) provides the
+ * PVT tables and StructuredGridVanguard the grid. */
+typedef struct omodels_blackoil_problem {
+    int32_t nx, ny, nz, n_src;
+    double dx, dy, dz;
+    double gravity;          /* z component of gravity; z points up */
+    const double* perm;      /* [3][N] kx, ky, kz [m^2] */
+    const double* poro;      /* [N] */
+    int32_t n_oil, n_gas;
+    const double* oil_tab;   /* [4][n_oil]: p, Rs_sat, 1/Bo_sat, mu_o_sat */
+    const double* gas_tab;   /* [3][n_gas]: p, 1/Bg, mu_g */
+    double oil_comp, water_pref, water_comp, water_mu;
+    double rho_ref[3];       /* water, oil, gas surface densities */
+    const int32_t* src_cell;
+    const int32_t* src_type; /* 0: water injector (mass rate), 1: BHP producer */
+    const double* src_a;     /* injector: kg/s; producer: well index [m^3] */
+    const double* src_b;     /* producer: bottom-hole pressure [Pa] */
+} omodels_blackoil_problem;
+
+/* Synthetic black-oil generator used by the benchmark configurations. */
+typedef struct omodels_blackoil_spec {
+    int32_t nx, ny, nz;
+    int32_t field;           /* 0: i.i.d. log-normal kx; 1: channelised Gaussian field */
+    double dx, dy, dz;
+    double perm_mean_md;     /* field 0: geometric mean kx [mD] */
+    double perm_sigma;       /* sigma of ln kx (within each facies for field 1) */
+    double kz_ratio;         /* kz = kz_ratio * kx, ky = kx */
+    double chan_perm_md, back_perm_md, chan_fraction, corr_len;  /* field 1 */
+    double poro_min, poro_max;
+    double p_top;            /* pressure of the top layer [Pa] */
+    double sw_init;
+    int32_t wells;           /* 0: corner injector/producer, 1: five-spot, 2: random pair */
+    int32_t pad;
+    double inj_rate;         /* total water injection [kg/s] */
+    double prod_bhp;         /* producer bottom-hole pressure [Pa] */
+    uint64_t seed;
+} omodels_blackoil_spec;
+
+/* Fill the PVT tables of tests/problems/reservoirproblem.hh:223-327
+ * (oil_tab [4][9], gas_tab [3][10]) and the fluid constants. */
+int omodels_blackoil_default_fluid(double* oil_tab, int32_t* n_oil, double* gas_tab,
+                                   int32_t* n_gas, double* consts /* [7] */);
+/* Generate permeability, porosity, hydrostatic initial state and wells.
+ * perm [3N], poro [N], pv [3N], meaning [N], sources up to src_cap. */
+int omodels_blackoil_generate(const omodels_blackoil_spec* spec, double* perm, double* poro,
+                              double* pv, uint8_t* meaning, int32_t src_cap, int32_t* n_src,
+                              int32_t* src_cell, int32_t* src_type, double* src_a,
+                              double* src_b);
+
+/* ------------------------------------------------------------------------ */
+/* Parameters, named after the reference's run-time parameters. */
+typedef struct omodels_params {
+    double newton_tol;          /* NewtonTolerance (newtonmethodparameters.hh:52) */
+    int32_t newton_max_iter;    /* NewtonMaxIterations (:36) */
+    int32_t newton_target_iter; /* NewtonTargetIterations (:45) */
+    double newton_max_error;    /* NewtonMaxError (:32) */
+    double lin_tol;             /* LinearSolverTolerance (linalgparameters.hh:63) */
+    double lin_abs_tol;         /* LinearSolverAbsTolerance (<0: newton_tol/100) */
+    double lin_max_error;       /* LinearSolverMaxError */
+    int32_t lin_max_iter;       /* LinearSolverMaxIterations */
+    int32_t precond;            /* 0: block ILU0 (PreconditionerWrapperILU), 1: block Jacobi */
+    double prec_relax;          /* PreconditionerRelaxation */
+    int32_t prec_order;         /* PreconditionerOrder */
+    int32_t max_dt_divisions;   /* MaxTimeStepDivisions (fvbaseparameters.hh:112) */
+    double dp_max_rel;          /* DpMaxRel (blackoilnewtonmethodparameters.hh:34) */
+    double ds_max;              /* DsMax */
+    double privar_osc_thresh;   /* PriVarOscilationThreshold */
+    double max_dt;              /* MaxTimeStepSize */
+    double min_dt;              /* MinTimeStepSize */
+    int32_t precision;          /* 0: FP64 solve; 1: FP32 Jacobian/ILU/inner BiCGStab
+                                   inside FP64 flexible outer refinement */
+    int32_t verbose;
+} omodels_params;
+
+void omodels_default_params(omodels_params* prm);
+const char* omodels_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Model: the device-resident state of one reservoir (FvBaseDiscretization's
+ * solution history and storage cache, fvbasediscretization.hh:655-857). */
+typedef struct omodels_model omodels_model;
+
+omodels_model* omodels_model_create(const omodels_blackoil_problem* prob);
+void omodels_model_destroy(omodels_model* m);
+int omodels_model_num_cells(const omodels_model* m, int64_t* n, int64_t* nnzb);
+/* solution(0) <- host; also solution(1) when both_levels != 0 */
+int omodels_model_set_solution(omodels_model* m, const double* pv, const uint8_t* meaning,
+                               int both_levels);
+int omodels_model_get_solution(omodels_model* m, double* pv, uint8_t* meaning);
+/* storage cache of time level 1 <- host (or NULL: recompute from solution(1)) */
+int omodels_model_set_storage1(omodels_model* m, const double* storage1);
+int omodels_model_get_storage1(omodels_model* m, double* storage1);
+
+/* ------------------------------------------------------------------------ */
+/* Linearizer -- replaces TpfaLinearizer (tpfalinearizer.hh:75-951). */
+/* init(): createMatrix_ (tpfalinearizer.hh:414-525).  Exports the pattern
+ * (row_ptr [N+1], col_idx [NNZB]), diagMatAddress_ as block offsets
+ * (diag_off [N]) and per neighbour info in intersection order
+ * (nb_ptr [N+1], nb_cell, nb_off = offset of A[J][I], nb_trans, nb_dzg). */
+int omodels_linearizer_pattern(omodels_model* m, int32_t* row_ptr, int32_t* col_idx,
+                               int32_t* diag_off, int32_t* nb_ptr, int32_t* nb_cell,
+                               int32_t* nb_off, double* nb_trans, double* nb_dzg);
+/* linearizeDomain() (tpfalinearizer.hh:189-245, linearize_ :711-866) at the
+ * current solution(0); iteration_index 0 refreshes the storage cache of
+ * time level 1 as recycleFirstIterationStorage() does (:786-812). */
+int omodels_linearizer_linearize_domain(omodels_model* m, double dt, int iteration_index);
+/* jacobian() / residual() host views (Dune layouts) */
+int omodels_linearizer_get_jacobian(omodels_model* m, double* values);
+int omodels_linearizer_get_residual(omodels_model* m, double* residual);
+/* eraseMatrix(): release the device Jacobian (re-created on next linearize) */
+int omodels_linearizer_erase_matrix(omodels_model* m);
+
+/* ------------------------------------------------------------------------ */
+/* Linear solver backend -- replaces ParallelBiCGStabSolverBackend
+ * (parallelbicgstabbackend.hh:91-172 over parallelbasebackend.hh:107-386). */
+typedef struct omodels_solver omodels_solver;
+omodels_solver* omodels_solver_create(omodels_model* m, const omodels_params* prm);
+void omodels_solver_destroy(omodels_solver* s);
+/* prepare(J, r) + setResidual(r) with the model's linearizer output, or with
+ * host arrays when values/residual are non-NULL (BCRS layout of the model). */
+int omodels_solver_prepare(omodels_solver* s, const double* values, const double* residual);
+int omodels_solver_get_residual(omodels_solver* s, double* residual);
+/* setMatrix + solve(x) (:252-284); x host [N*bs] may be NULL (kept on device) */
+int omodels_solver_solve(omodels_solver* s, double* x);
+int omodels_solver_iterations(const omodels_solver* s);
+/* Block ILU0 factor of the prepared matrix (diagonal blocks inverted), for
+ * parity checks of PreconditionerWrapperILU (istlpreconditionerwrappers.hh:151-190). */
+int omodels_solver_get_ilu(omodels_solver* s, double* lu_values);
+/* Preconditioner apply v = M^-1 d on host arrays. */
+int omodels_solver_apply_precond(omodels_solver* s, const double* d, double* v);
+
+/* ------------------------------------------------------------------------ */
+/* Newton method + time loop -- replaces BlackOilNewtonMethod::apply and
+ * FvBaseProblem::timeIntegration / Simulator::run for n_steps time steps. */
+typedef struct omodels_iter_log {
+    int32_t step, attempt, iter, lin_iters;
+    int32_t num_switched, lin_converged;
+    double error, dt;
+} omodels_iter_log;
+
+typedef struct omodels_run_stats {
+    int32_t newton_iters, linearizations, linear_iters, failed_attempts;
+    double t_linearize, t_solve, t_update, t_total;   /* seconds, host wall clock */
+} omodels_run_stats;
+
+/* Apply the Newton update of the black-oil model to the device state
+ * (BlackOilNewtonMethod::updatePrimaryVariables_, blackoilnewtonmethod.hh:226-427)
+ * with a host update vector; used for parity checks of the update kernel. */
+int omodels_newton_update(omodels_model* m, const omodels_params* prm, const double* delta,
+                          int32_t* num_switched);
+int omodels_model_get_switched(omodels_model* m, uint8_t* was_switched);
+
+int omodels_simulate(omodels_model* m, const omodels_params* prm, int n_steps, double dt0,
+                     double* pv_hist, double* dt_hist, omodels_iter_log* log, int log_cap,
+                     int32_t* log_len, omodels_run_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

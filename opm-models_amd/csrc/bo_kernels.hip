@@ -1,0 +1,335 @@
+// bo_kernels.hip -- black-oil assembly, storage cache and Newton-update
+// kernels for gfx950.
+//
+// Assembly (TpfaLinearizer::linearize_, tpfalinearizer.hh:711-866) runs one
+// thread per cell.  A cell owns r[I], the diagonal block A[I][I] and the
+// column blocks A[J][I] of its neighbours (the reference's single-writer rule,
+// tpfalinearizer.hh:765-770), so every Jacobian block is written exactly once
+// per linearization and no clearing pass or atomics are needed.
+//
+// The kernel is tiled over (x, y) columns of the grid and marches in z: the
+// value-only intensive quantities of a 2D tile plus its one-cell halo are
+// staged in LDS for the current plane, and the planes below and above come
+// from registers of the same thread, so each cell's neighbour quantities are
+// evaluated about (1 + halo/tile) times instead of six times.
+#include "omodels_internal.h"
+
+namespace omodels {
+
+// ---------------------------------------------------------------------------
+// Assembly.  Block = TX x TY threads, one (i, j) column each, k = 0..nz-1.
+constexpr int kTX = 32;
+constexpr int kTY = 8;
+
+struct IqLds {
+    double p, rho[3], lam[3], invB[3], Rs;
+};
+
+__device__ __forceinline__ void toLds(const IqV& q, IqLds& l)
+{
+    l.p = q.p;
+    for (int a = 0; a < 3; ++a) { l.rho[a] = q.rho[a]; l.lam[a] = q.lam[a]; l.invB[a] = q.invB[a]; }
+    l.Rs = q.Rs;
+}
+__device__ __forceinline__ void fromLds(const IqLds& l, IqV& q)
+{
+    q.p = l.p;
+    for (int a = 0; a < 3; ++a) { q.rho[a] = l.rho[a]; q.lam[a] = l.lam[a]; q.invB[a] = l.invB[a]; }
+    q.Rs = l.Rs;
+}
+
+__device__ __forceinline__ void loadIqV(const BoFluid& f, const double* __restrict__ pv,
+                                        const uint8_t* __restrict__ mean, long c, IqV& q)
+{
+    iqValues(f, pv[3 * c], pv[3 * c + 1], pv[3 * c + 2], mean[c], q);
+}
+
+// Accumulate one face of cell I into its residual / diagonal registers and
+// store the column block A[J][I] = 0 - dF/dx_I.
+__device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const IqV& qj, double trans,
+                                          double area, double dZg, bool upInTie, double r[3],
+                                          double D[9], double* __restrict__ Jout)
+{
+    E3 fl[3];
+    faceFlux(f, qi, qj, trans, area, dZg, upInTie, fl);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        r[e] += fl[e].v;
+        D[3 * e + 0] += fl[e].d0;
+        D[3 * e + 1] += fl[e].d1;
+        D[3 * e + 2] += fl[e].d2;
+    }
+    double blk[9];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        blk[3 * e + 0] = 0.0 + (-fl[e].d0);
+        blk[3 * e + 1] = 0.0 + (-fl[e].d1);
+        blk[3 * e + 2] = 0.0 + (-fl[e].d2);
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) Jout[t] = blk[t];
+}
+
+__global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
+    const double* __restrict__ pv, const uint8_t* __restrict__ mean,
+    const double* __restrict__ poro, const double* __restrict__ trans,
+    const double* __restrict__ st1, double storefac, double* __restrict__ res,
+    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff)
+{
+    __shared__ IqLds plane[kTY + 2][kTX + 2];
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int i = blockIdx.x * kTX + tx;
+    const int j = blockIdx.y * kTY + ty;
+    const bool own = (i < G.nx) && (j < G.ny);
+    const long nxy = (long)G.nx * G.ny;
+    const long N = nxy * G.nz;
+
+    // halo cells of this tile handled by this thread (perimeter of the tile)
+    // index h in [0, 2*(kTX+kTY)+4): enumerate the ring of the (kTX+2)x(kTY+2) box
+    const int ringLen = 2 * (kTX + 2) + 2 * kTY;
+
+    IqV below, cur, above;     // own column: k-1, k, k+1 (values)
+    bool haveBelow = false;
+    if (own) loadIqV(f, pv, mean, (long)i + (long)G.nx * j, cur);
+
+    for (int k = 0; k < G.nz; ++k) {
+        const long I = (long)i + (long)G.nx * j + nxy * k;
+        // stage plane k: interior from registers, ring computed here
+        if (own) toLds(cur, plane[ty + 1][tx + 1]);
+        for (int h = threadIdx.x; h < ringLen; h += kTX * kTY) {
+            int li, lj;
+            if (h < kTX + 2) { li = h; lj = 0; }
+            else if (h < 2 * (kTX + 2)) { li = h - (kTX + 2); lj = kTY + 1; }
+            else if (h < 2 * (kTX + 2) + kTY) { li = 0; lj = 1 + (h - 2 * (kTX + 2)); }
+            else { li = kTX + 1; lj = 1 + (h - 2 * (kTX + 2) - kTY); }
+            const int gi = blockIdx.x * kTX + li - 1, gj = blockIdx.y * kTY + lj - 1;
+            if (gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny) {
+                IqV q;
+                loadIqV(f, pv, mean, (long)gi + (long)G.nx * gj + nxy * k, q);
+                toLds(q, plane[lj][li]);
+            }
+        }
+        // the plane above, for the +z face (own column only)
+        if (own && k + 1 < G.nz) loadIqV(f, pv, mean, I + nxy, above);
+        __syncthreads();
+
+        if (own) {
+            IqE qi;
+            const double sw = pv[3 * I], po = pv[3 * I + 1], x2 = pv[3 * I + 2];
+            const int mI = mean[I];
+            iqEval(f, sw, po, x2, mI, qi);
+            double r[3] = {0.0, 0.0, 0.0};
+            double D[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) D[t] = 0.0;
+            const double zI = G.dz * ((double)k + 0.5);
+            // intersection order: -x, +x, -y, +y, -z, +z
+            if (i > 0) {
+                IqV q; fromLds(plane[ty + 1][tx], q);
+                applyFace(f, qi, q, trans[I - 1], G.area[0], (zI - zI) * G.g, false, r, D,
+                          jac + 9L * nbOff[0 * N + I]);
+            }
+            if (i + 1 < G.nx) {
+                IqV q; fromLds(plane[ty + 1][tx + 2], q);
+                applyFace(f, qi, q, trans[I], G.area[0], (zI - zI) * G.g, true, r, D,
+                          jac + 9L * nbOff[1 * N + I]);
+            }
+            if (j > 0) {
+                IqV q; fromLds(plane[ty][tx + 1], q);
+                applyFace(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zI) * G.g, false, r, D,
+                          jac + 9L * nbOff[2 * N + I]);
+            }
+            if (j + 1 < G.ny) {
+                IqV q; fromLds(plane[ty + 2][tx + 1], q);
+                applyFace(f, qi, q, trans[N + I], G.area[1], (zI - zI) * G.g, true, r, D,
+                          jac + 9L * nbOff[3 * N + I]);
+            }
+            if (haveBelow) {
+                const double zJ = G.dz * ((double)(k - 1) + 0.5);
+                applyFace(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zJ) * G.g, false, r, D,
+                          jac + 9L * nbOff[4 * N + I]);
+            }
+            if (k + 1 < G.nz) {
+                const double zJ = G.dz * ((double)(k + 1) + 0.5);
+                applyFace(f, qi, above, trans[2 * N + I], G.area[2], (zI - zJ) * G.g, true, r, D,
+                          jac + 9L * nbOff[5 * N + I]);
+            }
+            // accumulation term (tpfalinearizer.hh:776-825)
+            {
+                E3 st[3];
+                storageEval(f, qi, poro[I], st);
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    double rv = st[e].v;
+                    rv -= st1[3 * I + e];
+                    rv *= storefac;
+                    r[e] += rv;
+                    D[3 * e + 0] += st[e].d0 * storefac;
+                    D[3 * e + 1] += st[e].d1 * storefac;
+                    D[3 * e + 2] += st[e].d2 * storefac;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 3; ++e) res[3 * I + e] = r[e];
+            double* Dout = jac + 9L * diagOff[I];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) Dout[t] = D[t];
+        }
+        __syncthreads();
+        below = cur;
+        haveBelow = true;
+        cur = above;
+    }
+}
+
+// Cell-wise well sources (tpfalinearizer.hh:827-841), added after the flux
+// and accumulation terms exactly as the reference's per-cell order does.
+__global__ void k_bo_sources(BoFluid f, BoGrid G, const double* __restrict__ pv,
+                             const uint8_t* __restrict__ mean, int nsrc, const int* __restrict__ cell,
+                             const int* __restrict__ type, const double* __restrict__ a,
+                             const double* __restrict__ b, double* __restrict__ res,
+                             double* __restrict__ jac, const int* __restrict__ diagOff)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsrc) return;
+    const long I = cell[s];
+    IqE qi;
+    iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
+    E3 q[3];
+    sourceEval(f, qi, type[s], a[s], b[s], G.V, q);
+    double* D = jac + 9L * diagOff[I];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const E3 t = q[e] * (-G.V);
+        res[3 * I + e] += t.v;
+        D[3 * e + 0] += t.d0;
+        D[3 * e + 1] += t.d1;
+        D[3 * e + 2] += t.d2;
+    }
+}
+
+// Storage of the current solution: the cache of time level 1 at the first
+// Newton iteration (recycleFirstIterationStorage, tpfalinearizer.hh:786-812).
+__global__ void k_bo_storage(BoFluid f, long N, const double* __restrict__ pv,
+                             const uint8_t* __restrict__ mean, const double* __restrict__ poro,
+                             double* __restrict__ st)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    double s[3];
+    storageValues(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], poro[I], s);
+    st[3 * I] = s[0]; st[3 * I + 1] = s[1]; st[3 * I + 2] = s[2];
+}
+
+// ---------------------------------------------------------------------------
+// Newton update: BlackOilNewtonMethod::updatePrimaryVariables_
+// (blackoilnewtonmethod.hh:226-427) and adaptPrimaryVariables
+// (blackoilprimaryvariables.hh:549-909) for the Sw/Po/{Sg,Rs} meanings.
+__global__ void k_bo_update(BoFluid f, long N, double* __restrict__ pv, uint8_t* __restrict__ mean,
+                            uint8_t* __restrict__ wasSw, const double* __restrict__ delta,
+                            NewtonUpdateParams prm, int* __restrict__ counters)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const double d0 = delta[3 * I], d1 = delta[3 * I + 1], d2 = delta[3 * I + 2];
+    if (!isfinite(d0) || !isfinite(d1) || !isfinite(d2)) atomicOr(&counters[1], 1);
+    const double cur[3] = {pv[3 * I], pv[3 * I + 1], pv[3 * I + 2]};
+    const int cm = mean[I];
+    double dSw = 0.0, dSo = 0.0, dSg = 0.0;
+    dSw = d0;
+    dSo -= dSw;
+    if (cm == kMeanSg) { dSg = d2; dSo -= dSg; }
+    double maxSat = fmax(fabs(dSg), fabs(dSo));
+    maxSat = fmax(maxSat, fabs(dSw));
+    double satAlpha = 1.0;
+    if (maxSat > prm.dsMax) satAlpha = prm.dsMax / maxSat;
+    double nx[3];
+    {
+        double dl = d0 * satAlpha;
+        nx[0] = cur[0] - dl;
+    }
+    {
+        double dl = d1;
+        if (fabs(dl) > prm.dpMaxRel * cur[1]) {
+            const double sgn = (double)((dl > 0.0) - (dl < 0.0));
+            dl = sgn * prm.dpMaxRel * cur[1];
+        }
+        double v = cur[1] - dl;
+        v = v < -1e99 ? -1e99 : (v > 1e99 ? 1e99 : v);
+        nx[1] = v;
+    }
+    {
+        double dl = d2;
+        if (cm == kMeanSg) dl *= satAlpha;
+        else if (dl > cur[2]) dl = cur[2];
+        nx[2] = cur[2] - dl;
+    }
+    // adaptPrimaryVariables
+    const double eps = wasSw[I] ? prm.oscThresh : 0.0;
+    int m = cm;
+    bool changed = false;
+    const double sw = nx[0];
+    const double sg = (m == kMeanSg) ? nx[2] : 0.0;
+    if (sw >= 1.0) {
+        nx[0] = fmin(1.0, sw);
+        nx[2] = 0.0;
+        changed = (m != kMeanSg);
+        m = kMeanSg;
+    } else if (m == kMeanSg) {
+        const double s = 1.0 - sw;
+        if (sg < -eps && s > 0.0) {
+            m = kMeanRs;
+            const double rsSat = tabEval(f.rsSat, nx[1]);
+            nx[2] = fmin(1.79769313486231570815e+308 / 2, rsSat);
+            changed = true;
+        }
+    } else {
+        const double rsSat = tabEval(f.rsSat, nx[1]);
+        if (nx[2] > fmin(1.79769313486231570815e+308 / 2, rsSat * (1.0 + eps))) {
+            m = kMeanSg;
+            nx[2] = 0.0;
+            changed = true;
+        }
+    }
+    pv[3 * I] = nx[0]; pv[3 * I + 1] = nx[1]; pv[3 * I + 2] = nx[2];
+    mean[I] = (uint8_t)m;
+    wasSw[I] = changed ? 1 : 0;
+    // per-wave vote, one atomic per wavefront
+    const unsigned long long ball = __ballot(changed);
+    if ((threadIdx.x % warpSize) == 0 && ball) atomicAdd(&counters[0], __popcll(ball));
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+                      const double* poro, const double* trans, const double* st1, double storefac,
+                      double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s)
+{
+    dim3 grid((G.nx + kTX - 1) / kTX, (G.ny + kTY - 1) / kTY);
+    hipLaunchKernelGGL(k_bo_assemble, grid, dim3(kTX * kTY), 0, s, f, G, pv, mean, poro, trans, st1,
+                       storefac, res, jac, diagOff, nbOff);
+}
+
+void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+                     int nsrc, const int* cell, const int* type, const double* a, const double* b,
+                     double* res, double* jac, const int* diagOff, hipStream_t s)
+{
+    if (nsrc <= 0) return;
+    hipLaunchKernelGGL(k_bo_sources, dim3((nsrc + 63) / 64), dim3(64), 0, s, f, G, pv, mean, nsrc,
+                       cell, type, a, b, res, jac, diagOff);
+}
+
+void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
+                     const double* poro, double* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bo_storage, dim3((N + 255) / 256), dim3(256), 0, s, f, N, pv, mean, poro, st);
+}
+
+void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,
+                    const double* delta, const NewtonUpdateParams& prm, int* counters, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bo_update, dim3((N + 255) / 256), dim3(256), 0, s, f, N, pv, mean, wasSw,
+                       delta, prm, counters);
+}
+
+}  // namespace omodels

@@ -1,0 +1,344 @@
+// capi.cpp -- extern "C" boundary of libomodels_amd.so (declared in
+// include/omodels.h).  No C++ exception crosses this boundary: every entry
+// point maps NumericalProblem to 2, other errors to a negative code, and keeps
+// the message in a thread-local string.
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+
+#include "omodels_host.h"
+
+using namespace omodels;
+
+namespace {
+thread_local std::string g_lastError;
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const NumericalProblem& e) {
+        g_lastError = e.what();
+        return 2;
+    } catch (const std::invalid_argument& e) {
+        g_lastError = e.what();
+        return -2;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return -1;
+    } catch (...) {
+        g_lastError = "unknown error";
+        return -1;
+    }
+}
+}  // namespace
+
+struct omodels_model {
+    std::unique_ptr<Model> m;
+};
+struct omodels_solver {
+    omodels_model* owner;
+    std::unique_ptr<BiCGStabBackend> s;
+};
+
+extern "C" {
+
+const char* omodels_last_error(void) { return g_lastError.c_str(); }
+
+void omodels_default_params(omodels_params* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->newton_tol = 1e-6;           // ReservoirProblem's NewtonTolerance (reservoirproblem.hh:401)
+    p->newton_max_iter = 20;
+    p->newton_target_iter = 10;
+    p->newton_max_error = 1e100;
+    p->lin_tol = 1e-3;
+    p->lin_abs_tol = -1.0;
+    p->lin_max_error = 1e7;
+    p->lin_max_iter = 1000;
+    p->precond = 0;
+    p->prec_relax = 1.0;
+    p->prec_order = 0;
+    p->max_dt_divisions = 10;
+    p->dp_max_rel = 0.3;
+    p->ds_max = 0.2;
+    p->privar_osc_thresh = 1e-5;
+    p->max_dt = std::numeric_limits<double>::infinity();
+    p->min_dt = 0.0;
+    p->precision = 0;
+    p->verbose = 0;
+}
+
+int omodels_blackoil_default_fluid(double* oil_tab, int32_t* n_oil, double* gas_tab, int32_t* n_gas,
+                                   double* consts)
+{
+    return guarded([&] {
+        std::vector<double> oil, gas;
+        int no, ng;
+        defaultFluid(oil, no, gas, ng, consts);
+        std::copy(oil.begin(), oil.end(), oil_tab);
+        std::copy(gas.begin(), gas.end(), gas_tab);
+        *n_oil = no; *n_gas = ng;
+        return 0;
+    });
+}
+
+int omodels_blackoil_generate(const omodels_blackoil_spec* spec, double* perm, double* poro, double* pv,
+                              uint8_t* meaning, int32_t src_cap, int32_t* n_src, int32_t* src_cell,
+                              int32_t* src_type, double* src_a, double* src_b)
+{
+    return guarded([&] {
+        std::vector<double> k, ph, x, a, b;
+        std::vector<uint8_t> mn;
+        std::vector<int32_t> c, t;
+        generate(*spec, k, ph, x, mn, c, t, a, b);
+        if ((long)c.size() > src_cap) throw std::invalid_argument("source capacity too small");
+        std::copy(k.begin(), k.end(), perm);
+        std::copy(ph.begin(), ph.end(), poro);
+        std::copy(x.begin(), x.end(), pv);
+        std::copy(mn.begin(), mn.end(), meaning);
+        std::copy(c.begin(), c.end(), src_cell);
+        std::copy(t.begin(), t.end(), src_type);
+        std::copy(a.begin(), a.end(), src_a);
+        std::copy(b.begin(), b.end(), src_b);
+        *n_src = (int32_t)c.size();
+        return 0;
+    });
+}
+
+omodels_model* omodels_model_create(const omodels_blackoil_problem* prob)
+{
+    omodels_model* h = nullptr;
+    int rc = guarded([&] {
+        auto m = std::make_unique<Model>(prob);
+        h = new omodels_model{std::move(m)};
+        return 0;
+    });
+    return rc == 0 ? h : nullptr;
+}
+
+void omodels_model_destroy(omodels_model* m) { delete m; }
+
+int omodels_model_num_cells(const omodels_model* m, int64_t* n, int64_t* nnzb)
+{
+    if (!m) return -2;
+    *n = m->m->N; *nnzb = m->m->nnzb;
+    return 0;
+}
+
+int omodels_model_set_solution(omodels_model* h, const double* pv, const uint8_t* meaning, int both_levels)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.pv0.upload(pv, 3 * m.N, m.stream);
+        m.mean0.upload(meaning, m.N, m.stream);
+        if (both_levels) m.advanceTimeLevel();
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_model_get_solution(omodels_model* h, double* pv, uint8_t* meaning)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.pv0.download(pv, 3 * m.N, m.stream);
+        if (meaning) m.mean0.download(meaning, m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_model_set_storage1(omodels_model* h, const double* st)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (st) m.st1.upload(st, 3 * m.N, m.stream);
+        else launchBoStorage(m.fluid, m.N, m.pv1.p, m.mean1.p, m.poro.p, m.st1.p, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_model_get_storage1(omodels_model* h, double* st)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.st1.download(st, 3 * m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_linearizer_pattern(omodels_model* h, int32_t* row_ptr, int32_t* col_idx, int32_t* diag_off,
+                               int32_t* nb_ptr, int32_t* nb_cell, int32_t* nb_off, double* nb_trans,
+                               double* nb_dzg)
+{
+    return guarded([&] {
+        const Model& m = *h->m;
+        const HostTopology& T = m.topo;
+        const long N = T.N;
+        if (row_ptr) std::copy(T.rowPtr.begin(), T.rowPtr.end(), row_ptr);
+        if (col_idx) std::copy(T.colIdx.begin(), T.colIdx.end(), col_idx);
+        if (diag_off) std::copy(T.diagOff.begin(), T.diagOff.end(), diag_off);
+        // neighbour info in intersection order (-x,+x,-y,+y,-z,+z)
+        long k = 0;
+        if (nb_ptr) nb_ptr[0] = 0;
+        const long nxy = (long)T.nx * T.ny;
+        for (long I = 0; I < N; ++I) {
+            const long kk = I / nxy;
+            for (int d = 0; d < 6; ++d) {
+                const int off = T.nbOff[(long)d * N + I];
+                if (off < 0) continue;
+                const long J = d == 0 ? I - 1 : d == 1 ? I + 1 : d == 2 ? I - T.nx : d == 3 ? I + T.nx
+                             : d == 4 ? I - nxy : I + nxy;
+                const int ax = d / 2;
+                const long face = (d & 1) ? I : J;
+                if (nb_cell) nb_cell[k] = (int32_t)J;
+                if (nb_off) nb_off[k] = off;
+                if (nb_trans) nb_trans[k] = T.trans[(long)ax * N + face];
+                if (nb_dzg) {
+                    const long kj = J / nxy;
+                    const double zI = m.G.dz * ((double)kk + 0.5), zJ = m.G.dz * ((double)kj + 0.5);
+                    nb_dzg[k] = (zI - zJ) * m.G.g;
+                }
+                ++k;
+            }
+            if (nb_ptr) nb_ptr[I + 1] = (int32_t)k;
+        }
+        return 0;
+    });
+}
+
+int omodels_linearizer_linearize_domain(omodels_model* h, double dt, int iteration_index)
+{
+    return guarded([&] {
+        if (!(dt > 0)) throw std::invalid_argument("dt must be positive");
+        h->m->linearizeDomain(dt, iteration_index);
+        h->m->sync();
+        return 0;
+    });
+}
+
+int omodels_linearizer_get_jacobian(omodels_model* h, double* values)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.jac.download(values, 9 * (size_t)m.nnzb, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_linearizer_get_residual(omodels_model* h, double* r)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.res.download(r, 3 * (size_t)m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_linearizer_erase_matrix(omodels_model* h)
+{
+    return guarded([&] { h->m->eraseMatrix(); return 0; });
+}
+
+omodels_solver* omodels_solver_create(omodels_model* h, const omodels_params* prm)
+{
+    omodels_solver* s = nullptr;
+    int rc = guarded([&] {
+        s = new omodels_solver{h, std::make_unique<BiCGStabBackend>(*h->m, *prm)};
+        return 0;
+    });
+    return rc == 0 ? s : nullptr;
+}
+
+void omodels_solver_destroy(omodels_solver* s) { delete s; }
+
+int omodels_solver_prepare(omodels_solver* s, const double* values, const double* residual)
+{
+    return guarded([&] { s->s->prepare(values, residual); return 0; });
+}
+
+int omodels_solver_get_residual(omodels_solver* s, double* residual)
+{
+    return guarded([&] {
+        Model& m = *s->owner->m;
+        OM_HIP(hipMemcpyAsync(residual, m.res.p, 3 * m.N * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_solver_solve(omodels_solver* s, double* x)
+{
+    return guarded([&] {
+        const int rc = s->s->solve();
+        Model& m = *s->owner->m;
+        if (x) {
+            OM_HIP(hipMemcpyAsync(x, s->s->solution(), 3 * m.N * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+            m.sync();
+        }
+        return rc;
+    });
+}
+
+int omodels_solver_iterations(const omodels_solver* s) { return s->s->iterations(); }
+
+int omodels_solver_get_ilu(omodels_solver* s, double* lu)
+{
+    return guarded([&] { s->s->downloadIlu(lu); return 0; });
+}
+
+int omodels_solver_apply_precond(omodels_solver* s, const double* d, double* v)
+{
+    return guarded([&] { s->s->applyPrecond(d, v); return 0; });
+}
+
+int omodels_newton_update(omodels_model* h, const omodels_params* prm, const double* delta, int32_t* num_switched)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        DevBuf<double> d;
+        d.upload(delta, 3 * m.N, m.stream);
+        BlackOilNewtonMethod nm(m, *prm);
+        int nsw = 0;
+        const int rc = nm.update(d.p, &nsw);
+        *num_switched = nsw;
+        return rc;
+    });
+}
+
+int omodels_model_get_switched(omodels_model* h, uint8_t* was_switched)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.wasSw.download(was_switched, m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_simulate(omodels_model* h, const omodels_params* prm, int n_steps, double dt0, double* pv_hist,
+                     double* dt_hist, omodels_iter_log* log, int log_cap, int32_t* log_len,
+                     omodels_run_stats* stats)
+{
+    return guarded([&] {
+        std::vector<omodels_iter_log> lg;
+        omodels_run_stats st{};
+        const int rc = simulate(*h->m, *prm, n_steps, dt0, pv_hist, dt_hist, &lg, &st);
+        if (log) {
+            const int n = std::min<int>(log_cap, (int)lg.size());
+            std::copy(lg.begin(), lg.begin() + n, log);
+        }
+        if (log_len) *log_len = (int32_t)lg.size();
+        if (stats) *stats = st;
+        return rc;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,429 @@
+// host_model.cpp -- Model / TpfaLinearizer, BiCGStab backend and the
+// black-oil Newton method with its time loop, all driving the gfx950
+// kernels on one HIP stream.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+#include "omodels_host.h"
+
+namespace omodels {
+
+void hipCheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+// ---------------------------------------------------------------------------
+Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
+{
+    if (!p || p->nx < 1 || p->ny < 1 || p->nz < 1) throw std::invalid_argument("invalid grid");
+    if (!p->perm || !p->poro || !p->oil_tab || !p->gas_tab) throw std::invalid_argument("missing problem arrays");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("no HIP device available");
+    if (s) stream = s;
+    else { OM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); ownStream = true; }
+    fluid = makeFluid(p);
+    buildTopology(p, topo);
+    N = topo.N;
+    nnzb = topo.rowPtr[N];
+    G.nx = p->nx; G.ny = p->ny; G.nz = p->nz;
+    G.dx = p->dx; G.dy = p->dy; G.dz = p->dz; G.g = p->gravity;
+    G.V = p->dx * p->dy * p->dz;
+    G.area[0] = p->dy * p->dz; G.area[1] = p->dx * p->dz; G.area[2] = p->dx * p->dy;
+
+    trans.upload(topo.trans.data(), topo.trans.size(), stream);
+    poro.upload(p->poro, N, stream);
+    rowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
+    colIdx.upload(topo.colIdx.data(), topo.colIdx.size(), stream);
+    diagOff.upload(topo.diagOff.data(), topo.diagOff.size(), stream);
+    nbOff.upload(topo.nbOff.data(), topo.nbOff.size(), stream);
+    levels.nLevels = (int)topo.levelPtr.size() - 1;
+    levels.ptrHost.assign(topo.levelPtr.begin(), topo.levelPtr.end());
+    levels.rows.upload(topo.levelRows.data(), topo.levelRows.size(), stream);
+    nsrc = p->n_src;
+    if (nsrc > 0) {
+        for (int k = 0; k < nsrc; ++k)
+            if (p->src_cell[k] < 0 || p->src_cell[k] >= N) throw std::invalid_argument("source cell out of range");
+        srcCell.upload(p->src_cell, nsrc, stream);
+        srcType.upload(p->src_type, nsrc, stream);
+        srcA.upload(p->src_a, nsrc, stream);
+        srcB.upload(p->src_b, nsrc, stream);
+    }
+    pv0.alloc(3 * N); pv1.alloc(3 * N); st1.alloc(3 * N); res.alloc(3 * N);
+    mean0.alloc(N); mean1.alloc(N); wasSw.alloc(N);
+    OM_HIP(hipMemsetAsync(wasSw.p, 0, N, stream));
+    OM_HIP(hipMemsetAsync(st1.p, 0, 3 * N * sizeof(double), stream));
+    counters.alloc(4);
+    partial.alloc(2 * kRedBlocks);
+    scal.alloc(8);
+    OM_HIP(hipHostMalloc((void**)&hostScal, 8 * sizeof(double), hipHostMallocDefault));
+    ensureJacobian();
+    sync();
+}
+
+Model::~Model()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (hostScal) (void)hipHostFree(hostScal);
+    if (ownStream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Model::sync() { OM_HIP(hipStreamSynchronize(stream)); }
+
+void Model::ensureJacobian()
+{
+    if (!jac.p) jac.alloc(9 * (size_t)nnzb);
+}
+
+// TpfaLinearizer::linearizeDomain (tpfalinearizer.hh:189-245) at solution(0)
+void Model::linearizeDomain(double dt, int iterationIndex)
+{
+    ensureJacobian();
+    if (iterationIndex == 0)
+        launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream);
+    const double storefac = G.V / dt;
+    launchBoAssemble(fluid, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
+                     nbOff.p, stream);
+    launchBoSources(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, res.p, jac.p,
+                    diagOff.p, stream);
+    OM_HIP(hipGetLastError());
+}
+
+double Model::maxAbsResidual()
+{
+    launchMaxAbs<double>(3 * N, res.p, partial.p, scal.p, stream);
+    OM_HIP(hipMemcpyAsync(hostScal, scal.p, sizeof(double), hipMemcpyDeviceToHost, stream));
+    sync();
+    return hostScal[0];
+}
+
+void Model::advanceTimeLevel()
+{
+    OM_HIP(hipMemcpyAsync(pv1.p, pv0.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    OM_HIP(hipMemcpyAsync(mean1.p, mean0.p, N, hipMemcpyDeviceToDevice, stream));
+}
+
+void Model::restoreTimeLevel()
+{
+    OM_HIP(hipMemcpyAsync(pv0.p, pv1.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    OM_HIP(hipMemcpyAsync(mean0.p, mean1.p, N, hipMemcpyDeviceToDevice, stream));
+}
+
+// ---------------------------------------------------------------------------
+BiCGStabBackend::BiCGStabBackend(Model& m, const omodels_params& prm) : m_(m), prm_(prm)
+{
+    const long n = 3 * m.N;
+    b64_.alloc(n); x64_.alloc(n);
+    ks_.alloc(1);
+    fail_.alloc(1);
+    OM_HIP(hipHostMalloc((void**)&ksHost_, sizeof(KrylovScalars), hipHostMallocDefault));
+    if (prm_.precision == 0) {
+        for (auto& w : wd_) w.alloc(n);
+        r64_.alloc(n);
+    } else {
+        for (auto& w : wf_) w.alloc(n);
+        r64_.alloc(n);
+        bf_.alloc(n); xf_.alloc(n);
+    }
+}
+
+BiCGStabBackend::~BiCGStabBackend()
+{
+    if (ksHost_) (void)hipHostFree(ksHost_);
+}
+
+// prepare(): ParallelBaseBackend::prepare/setResidual/setMatrix
+// (parallelbasebackend.hh:182-250) + preparePreconditioner_ (:311-333):
+// the solver-owned copy in the solver scalar and the preconditioner factors.
+void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidual)
+{
+    hipStream_t s = m_.stream;
+    const long n = 3 * m_.N, nnzb = m_.nnzb;
+    if (hostValues) { Acopy_.upload(hostValues, 9 * nnzb, s); A64_ = Acopy_.p; }
+    else A64_ = m_.jac.p;
+    if (hostResidual) b64_.upload(hostResidual, n, s);
+    else OM_HIP(hipMemcpyAsync(b64_.p, m_.res.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
+    const int N = (int)m_.N;
+    if (prm_.precond == 0) {
+        // ILU(0) factorization in FP64 (pivot blocks stay FP64 in both forms)
+        lu64_.alloc(9 * nnzb);
+        OM_HIP(hipMemcpyAsync(lu64_.p, A64_, 9 * nnzb * sizeof(double), hipMemcpyDeviceToDevice, s));
+        launchIlu0Factor<double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu64_.p, m_.levels, fail_.p, s);
+        dinv64_.alloc(9 * (size_t)N);
+        launchExtractInvDiag<double>(N, 3, m_.diagOff.p, lu64_.p, dinv64_.p, s);
+        if (prm_.precision == 1) {
+            luf_.alloc(9 * nnzb);
+            launchCopyConvertD<float>(9 * nnzb, lu64_.p, luf_.p, s);
+        }
+    } else {
+        dinv64_.alloc(9 * (size_t)N);
+        launchJacobiInvert<double>(N, 3, m_.diagOff.p, A64_, dinv64_.p, fail_.p, s);
+    }
+    if (prm_.precision == 1) {
+        Af_.alloc(9 * nnzb);
+        launchCopyConvertD<float>(9 * nnzb, A64_, Af_.p, s);
+    }
+    OM_HIP(hipGetLastError());
+    int f = 0;
+    OM_HIP(hipMemcpyAsync(&f, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    m_.sync();
+    prepared_ = (f == 0);
+}
+
+template <class S>
+void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v)
+{
+    hipStream_t s = m_.stream;
+    const int N = (int)m_.N;
+    if (prm_.precond == 0)
+        launchIlu0Apply<S, double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv, d, v,
+                                   m_.levels, s);
+    else {
+        (void)lu;
+        // block Jacobi with FP64 inverted pivots: v = w D^-1 d
+        launchJacobiApplyMixed(N, (const double*)dinv, d, v, prm_.prec_relax, s);
+    }
+}
+
+// BiCGStabSolver::apply (bicgstabsolver.hh:120-330) with CombinedCriterion;
+// one host synchronisation per iteration (the convergence test).
+template <class S>
+int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x, double tol,
+                              double absTol, int maxIter, int* its)
+{
+    hipStream_t s = m_.stream;
+    const long m = 3 * m_.N;
+    const int N = (int)m_.N;
+    S* r; S* p; S* v; S* y; S* z;
+    if constexpr (sizeof(S) == 8) { r = (S*)wd_[0].p; p = (S*)wd_[1].p; v = (S*)wd_[2].p; y = (S*)wd_[3].p; z = (S*)wd_[4].p; }
+    else { r = (S*)wf_[0].p; p = (S*)wf_[1].p; v = (S*)wf_[2].p; y = (S*)wf_[3].p; z = (S*)wf_[4].p; }
+    OM_HIP(hipMemsetAsync(x, 0, m * sizeof(S), s));
+    OM_HIP(hipMemcpyAsync(r, b, m * sizeof(S), hipMemcpyDeviceToDevice, s));
+    // setInitial
+    launchMaxAbs<S>(m, r, m_.partial.p, m_.scal.p, s);
+    OM_HIP(hipMemcpyAsync(m_.hostScal, m_.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    m_.sync();
+    const double minS = (double)std::numeric_limits<S>::min() * 1e10;
+    const double init = std::max(m_.hostScal[0], minS);
+    *its = 0;
+    auto converged = [&](double res) { return res / std::max(1e-20, init) <= tol || res <= absTol; };
+    if (converged(init)) return 0;
+    KrylovScalars k0{};
+    k0.rho = 1; k0.alpha = 1; k0.omega = 1; k0.init_norm = init;
+    k0.fail_code = 0; k0.fail_tag = -1; k0.stag_half = 0; k0.stag_full_tag = -1;
+    OM_HIP(hipMemcpyAsync(ks_.p, &k0, sizeof(k0), hipMemcpyHostToDevice, s));
+    OM_HIP(hipMemsetAsync(v, 0, m * sizeof(S), s));
+    OM_HIP(hipMemsetAsync(p, 0, m * sizeof(S), s));
+    for (int it = 0; it < maxIter; ++it) {
+        const int tag = 2 * it;
+        launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, tag, s);
+        launchUpdateP<S>(m, r, v, p, ks_.p, s);
+        precApply<S>(lu, dinv, p, y);
+        launchSpmv<S>(N, 3, m_.rowPtr.p, m_.colIdx.p, A, y, v, s);
+        launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, tag, s);
+        launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
+        OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+        m_.sync();
+        const KrylovScalars& K = *ksHost_;
+        if (it > 0 && K.fail_code && K.fail_tag == tag - 1) { *its = it - 1; return 2; }
+        if (it > 0 && K.stag_full_tag == tag - 1) { *its = it - 1; return 1; }
+        if (K.fail_code) { *its = it; return 2; }
+        *its = it;
+        if (converged(K.resnorm)) return 0;
+        if (K.stag_half || K.resnorm > prm_.lin_max_error) return 1;
+        precApply<S>(lu, dinv, r, z);
+        launchSpmv<S>(N, 3, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
+        launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, tag + 1, s);
+        launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, tag + 1, s);
+    }
+    OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+    m_.sync();
+    *its = maxIter;
+    if (ksHost_->fail_code) return 2;
+    return 1;
+}
+
+int BiCGStabBackend::solve()
+{
+    if (!prepared_) return 2;   // preconditioner could not be created (NumericalProblem)
+    hipStream_t s = m_.stream;
+    const long m = 3 * m_.N;
+    const double absTol = prm_.lin_abs_tol < 0 ? prm_.newton_tol / 100.0 : prm_.lin_abs_tol;
+    iters_ = 0;
+    if (prm_.precision == 0) {
+        int its = 0;
+        int rc = bicgstab<double>(A64_, lu64_.p, dinv64_.p, b64_.p, x64_.p, prm_.lin_tol, absTol,
+                                  prm_.lin_max_iter, &its);
+        iters_ = its;
+        return rc;
+    }
+    // FP32 inner BiCGStab inside FP64 iterative refinement: the outer residual
+    // r = b - A x and its norm are FP64; the inner solve works on FP32 copies
+    // of J and of the ILU factors (with FP64 pivots).
+    OM_HIP(hipMemsetAsync(x64_.p, 0, m * sizeof(double), s));
+    OM_HIP(hipMemcpyAsync(r64_.p, b64_.p, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    launchMaxAbs<double>(m, r64_.p, m_.partial.p, m_.scal.p, s);
+    OM_HIP(hipMemcpyAsync(m_.hostScal, m_.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    m_.sync();
+    const double init = std::max(m_.hostScal[0], std::numeric_limits<double>::min() * 1e10);
+    double res = init;
+    auto converged = [&](double r) { return r / std::max(1e-20, init) <= prm_.lin_tol || r <= absTol; };
+    if (converged(res)) return 0;
+    const int maxOuter = 8;
+    int total = 0;
+    for (int outer = 0; outer < maxOuter; ++outer) {
+        launchCopyConvertD<float>(m, r64_.p, bf_.p, s);
+        const double innerTol = std::min(0.5, std::max(1e-6, prm_.lin_tol * init / res));
+        int its = 0;
+        int rc = bicgstab<float>(Af_.p, luf_.p, dinv64_.p, bf_.p, xf_.p, innerTol, absTol,
+                                 prm_.lin_max_iter - total, &its);
+        total += its;
+        if (rc == 2) { iters_ = total; return 2; }
+        launchAddFromFloat(m, xf_.p, x64_.p, s);
+        launchResidualUpdate((int)m_.N, 3, m_.rowPtr.p, m_.colIdx.p, A64_, x64_.p, b64_.p, r64_.p, s);
+        launchMaxAbs<double>(m, r64_.p, m_.partial.p, m_.scal.p, s);
+        OM_HIP(hipMemcpyAsync(m_.hostScal, m_.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        m_.sync();
+        res = m_.hostScal[0];
+        if (!std::isfinite(res)) { iters_ = total; return 2; }
+        if (converged(res)) { iters_ = total; return 0; }
+        if (rc == 1 || res > prm_.lin_max_error || total >= prm_.lin_max_iter) { iters_ = total; return 1; }
+    }
+    iters_ = total;
+    return 1;
+}
+
+void BiCGStabBackend::downloadIlu(double* hostValues)
+{
+    lu64_.download(hostValues, 9 * (size_t)m_.nnzb, m_.stream);
+    m_.sync();
+}
+
+void BiCGStabBackend::applyPrecond(const double* hostD, double* hostV)
+{
+    const long m = 3 * m_.N;
+    DevBuf<double> d, v;
+    d.upload(hostD, m, m_.stream);
+    v.alloc(m);
+    precApply<double>(lu64_.p, dinv64_.p, d.p, v.p);
+    v.download(hostV, m, m_.stream);
+    m_.sync();
+}
+
+// ---------------------------------------------------------------------------
+int BlackOilNewtonMethod::update(const double* delta, int* numSwitched)
+{
+    hipStream_t s = m_.stream;
+    NewtonUpdateParams up{prm_.dp_max_rel, prm_.ds_max, prm_.privar_osc_thresh};
+    OM_HIP(hipMemsetAsync(m_.counters.p, 0, 4 * sizeof(int), s));
+    launchBoUpdate(m_.fluid, m_.N, m_.pv0.p, m_.mean0.p, m_.wasSw.p, delta, up, m_.counters.p, s);
+    int c[4];
+    OM_HIP(hipMemcpyAsync(c, m_.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    m_.sync();
+    *numSwitched = c[0];
+    return c[1] ? 2 : 0;
+}
+
+double BlackOilNewtonMethod::suggestTimeStepSize(double oldDt) const
+{
+    const int tgt = prm_.newton_target_iter;
+    if (numIter_ > tgt) {
+        const double percent = double(numIter_ - tgt) / tgt;
+        return std::max(prm_.min_dt, oldDt / (1.0 + percent));
+    }
+    const double percent = double(tgt - numIter_) / tgt;
+    return std::max(prm_.min_dt, oldDt * (1.0 + percent / 1.2));
+}
+
+// NewtonMethod::apply (newtonmethod.hh:228-438) with the black-oil hooks.
+bool BlackOilNewtonMethod::apply(double dt, int step, int attempt, std::vector<omodels_iter_log>* log,
+                                 omodels_run_stats* st)
+{
+    using clk = std::chrono::steady_clock;
+    numIter_ = 0;
+    error_ = 1e100; lastError_ = 1e100;
+    auto converged = [&]() { return error_ <= prm_.newton_tol; };
+    auto proceed = [&]() {
+        if (numIter_ < 1) return true;
+        if (converged()) return false;
+        if (numIter_ >= prm_.newton_max_iter) return error_ * 4.0 < lastError_;
+        return true;
+    };
+    while (proceed()) {
+        auto t0 = clk::now();
+        m_.linearizeDomain(dt, numIter_);
+        lastError_ = error_;
+        error_ = m_.maxAbsResidual();           // preSolve_ (:572-607)
+        auto t1 = clk::now();
+        if (st) { st->t_linearize += std::chrono::duration<double>(t1 - t0).count(); st->linearizations++; }
+        if (!(error_ <= prm_.newton_max_error)) return false;
+        if (!proceed()) {
+            if (log) log->push_back({step, attempt, numIter_, 0, 0, 1, error_, dt});
+            ++numIter_;
+            break;
+        }
+        solver_.prepare();
+        const int rc = solver_.solve();
+        auto t2 = clk::now();
+        if (st) { st->t_solve += std::chrono::duration<double>(t2 - t1).count(); st->linear_iters += solver_.iterations(); }
+        if (rc != 0) {
+            if (log) log->push_back({step, attempt, numIter_, solver_.iterations(), 0, 0, error_, dt});
+            numIter_ = 2 * prm_.newton_target_iter;     // failed_()
+            return false;
+        }
+        int nsw = 0;
+        const int urc = update(solver_.solution(), &nsw);
+        auto t3 = clk::now();
+        if (st) { st->t_update += std::chrono::duration<double>(t3 - t2).count(); st->newton_iters++; }
+        if (log) log->push_back({step, attempt, numIter_, solver_.iterations(), nsw, 1, error_, dt});
+        if (prm_.verbose)
+            printf("step %d attempt %d iter %d error %.6e lin %d switched %d\n", step, attempt, numIter_, error_,
+                   solver_.iterations(), nsw);
+        if (urc != 0) { numIter_ = 2 * prm_.newton_target_iter; return false; }
+        ++numIter_;
+    }
+    if (!converged()) { numIter_ = 2 * prm_.newton_target_iter; return false; }
+    return true;
+}
+
+// Simulator::run / FvBaseProblem::timeIntegration with step halving.
+int simulate(Model& m, const omodels_params& prm, int nSteps, double dt0, double* pvHist, double* dtHist,
+             std::vector<omodels_iter_log>* log, omodels_run_stats* stats)
+{
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    BlackOilNewtonMethod newton(m, prm);
+    double dt = dt0;
+    m.advanceTimeLevel();
+    for (int step = 0; step < nSteps; ++step) {
+        bool ok = false;
+        for (int attempt = 0; attempt < prm.max_dt_divisions; ++attempt) {
+            if (newton.apply(dt, step, attempt, log, stats)) { ok = true; break; }
+            if (stats) stats->failed_attempts++;
+            m.restoreTimeLevel();
+            double ndt = dt / 2.0;
+            if (dt < prm.min_dt * (1 + 1e-9)) break;
+            if (ndt < prm.min_dt) ndt = prm.min_dt;
+            dt = ndt;
+        }
+        if (!ok) { m.sync(); return 3; }
+        if (dtHist) dtHist[step] = dt;
+        m.advanceTimeLevel();
+        if (pvHist) {
+            OM_HIP(hipMemcpyAsync(pvHist + (long)step * 3 * m.N, m.pv0.p, 3 * m.N * sizeof(double),
+                                  hipMemcpyDeviceToHost, m.stream));
+        }
+        dt = std::min(prm.max_dt, newton.suggestTimeStepSize(dt));
+    }
+    m.sync();
+    if (stats) stats->t_total = std::chrono::duration<double>(clk::now() - t0).count();
+    return 0;
+}
+
+}  // namespace omodels

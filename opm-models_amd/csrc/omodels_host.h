@@ -1,0 +1,129 @@
+// omodels_host.h -- host-side C++ classes behind the C ABI.  They mirror the
+// reference's interfaces for the hot path:
+//   Model                 FvBaseDiscretization state (solution levels, storage cache)
+//   TpfaLinearizer        tpfalinearizer.hh:75-951
+//   BiCGStabBackend       parallelbicgstabbackend.hh:91-172 / parallelbasebackend.hh:107-386
+//   BlackOilNewtonMethod  newtonmethod.hh:90-877 / blackoilnewtonmethod.hh:54-448
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/omodels.h"
+#include "omodels_internal.h"
+
+namespace omodels {
+
+struct HostTopology {
+    int nx = 0, ny = 0, nz = 0;
+    long N = 0;
+    std::vector<int32_t> rowPtr, colIdx, diagOff, nbOff;   // nbOff [6][N], -1 = no neighbour
+    std::vector<double> trans;                             // [3][N] faces to +x, +y, +z
+    std::vector<int32_t> levelPtr, levelRows;
+};
+
+void defaultFluid(std::vector<double>& oil, int& nOil, std::vector<double>& gas, int& nGas, double c[7]);
+BoFluid makeFluid(const omodels_blackoil_problem* p);
+void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::vector<double>& poro,
+              std::vector<double>& pv, std::vector<uint8_t>& mean, std::vector<int32_t>& srcCell,
+              std::vector<int32_t>& srcType, std::vector<double>& srcA, std::vector<double>& srcB);
+void buildTopology(const omodels_blackoil_problem* p, HostTopology& T);
+
+// ---------------------------------------------------------------------------
+class Model {
+public:
+    explicit Model(const omodels_blackoil_problem* prob, hipStream_t stream = nullptr);
+    ~Model();
+
+    long N = 0, nnzb = 0;
+    BoGrid G{};
+    BoFluid fluid{};
+    HostTopology topo;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+
+    // device state
+    DevBuf<double> trans, poro, pv0, pv1, st1, res, jac;
+    DevBuf<uint8_t> mean0, mean1, wasSw;
+    DevBuf<int> rowPtr, colIdx, diagOff, nbOff;
+    DevBuf<int> srcCell, srcType;
+    DevBuf<double> srcA, srcB;
+    LevelSchedule levels;
+    int nsrc = 0;
+    DevBuf<int> counters;        // [0] switched, [1] non-finite update
+    DevBuf<double> partial;      // reduction partials
+    DevBuf<double> scal;         // device scalars (norms)
+    double* hostScal = nullptr;  // pinned mirror
+
+    // TpfaLinearizer
+    void linearizeDomain(double dt, int iterationIndex);
+    void ensureJacobian();
+    void eraseMatrix() { jac.release(); }
+    // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)
+    void advanceTimeLevel();
+    void restoreTimeLevel();
+    void sync();
+    double maxAbsResidual();
+};
+
+struct SolverStats {
+    int iterations = 0;
+};
+
+class BiCGStabBackend {
+public:
+    BiCGStabBackend(Model& m, const omodels_params& prm);
+    ~BiCGStabBackend();
+    // prepare(J, r) + setResidual(r): uses the model's Jacobian / residual
+    void prepare(const double* hostValues = nullptr, const double* hostResidual = nullptr);
+    // solve(x); returns 0 converged, 1 not converged, 2 numerical problem
+    int solve();
+    int iterations() const { return iters_; }
+    const double* solution() const { return x64_.p; }
+    void downloadIlu(double* hostValues);
+    void applyPrecond(const double* hostD, double* hostV);
+
+private:
+    template <class S> int bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x,
+                                    double tol, double absTol, int maxIter, int* its);
+    template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v);
+
+    Model& m_;
+    omodels_params prm_;
+    int iters_ = 0;
+    int bs_ = 3;
+    const double* A64_ = nullptr;        // FP64 matrix used by the solver
+    DevBuf<double> Acopy_;               // host-provided matrix
+    DevBuf<double> b64_, x64_, r64_;
+    DevBuf<double> lu64_, dinv64_;
+    DevBuf<float> Af_, luf_, bf_, xf_;
+    // Krylov work vectors (either precision)
+    DevBuf<double> wd_[5];
+    DevBuf<float> wf_[5];
+    DevBuf<KrylovScalars> ks_;
+    KrylovScalars* ksHost_ = nullptr;
+    DevBuf<int> fail_;
+    bool prepared_ = false;
+};
+
+class BlackOilNewtonMethod {
+public:
+    BlackOilNewtonMethod(Model& m, const omodels_params& prm) : m_(m), prm_(prm), solver_(m, prm) {}
+    // NewtonMethod::apply (newtonmethod.hh:228-438); true when converged
+    bool apply(double dt, int step, int attempt, std::vector<omodels_iter_log>* log, omodels_run_stats* st);
+    int numIterations() const { return numIter_; }
+    double suggestTimeStepSize(double oldDt) const;
+    int update(const double* delta, int* numSwitched);
+
+private:
+    Model& m_;
+    omodels_params prm_;
+    BiCGStabBackend solver_;
+    int numIter_ = 0;
+    double error_ = 1e100, lastError_ = 1e100;
+};
+
+int simulate(Model& m, const omodels_params& prm, int nSteps, double dt0, double* pvHist, double* dtHist,
+             std::vector<omodels_iter_log>* log, omodels_run_stats* stats);
+
+}  // namespace omodels

@@ -1,0 +1,129 @@
+// omodels_internal.h -- shared declarations of the MI355X host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bo_physics.h"
+
+namespace omodels {
+
+// Thrown for the reference's NumericalProblem (recoverable: the time loop
+// halves the step, fvbaseproblem.hh:512-557).
+struct NumericalProblem : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hipCheck(hipError_t e, const char* what);
+#define OM_HIP(x) ::omodels::hipCheck((x), #x)
+
+struct BoGrid {
+    int nx, ny, nz, pad;
+    double dx, dy, dz, g, V;
+    double area[3];       // face areas of x, y and z faces
+};
+
+struct NewtonUpdateParams {
+    double dpMaxRel, dsMax, oscThresh;
+};
+
+// Device buffer owning a hipMalloc'ed array.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr; n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        release();
+        if (count) OM_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count) OM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* h, size_t count, hipStream_t s) const
+    {
+        if (count) OM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// ---- black-oil kernels (bo_kernels.hip) ----
+void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+                      const double* poro, const double* trans, const double* st1, double storefac,
+                      double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s);
+void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+                     int nsrc, const int* cell, const int* type, const double* a, const double* b,
+                     double* res, double* jac, const int* diagOff, hipStream_t s);
+void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
+                     const double* poro, double* st, hipStream_t s);
+void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,
+                    const double* delta, const NewtonUpdateParams& prm, int* counters, hipStream_t s);
+
+// ---- linear algebra kernels (linalg_kernels.hip) ----
+// Scalars of one BiCGStab run kept on the device (no host round trip per dot).
+struct KrylovScalars {
+    double rho, alpha, omega, beta, rho_i, denom;
+    double resnorm, init_norm;
+    int fail_code;        // 0 ok, 2 breakdown (NumericalProblem)
+    int fail_tag;         // half-iteration index of the first breakdown
+    int stag_half;        // change indicator all zero at the last half-step check
+    int stag_full_tag;    // half-iteration index of a full-step stagnation, -1 if none
+};
+
+constexpr int kRedBlocks = 1024;     // partial sums per reduction (fixed => deterministic)
+
+template <class S> void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a,
+                                   const S* x, S* y, hipStream_t st);
+template <class S> void launchCopyConvertD(long m, const double* x, S* y, hipStream_t st);
+
+// ILU0 with level scheduling; lvlPtr/lvlRows on the device, lvlPtrHost on the host
+struct LevelSchedule {
+    int nLevels = 0;
+    std::vector<int> ptrHost;     // [nLevels+1]
+    DevBuf<int> rows;             // rows ordered by level
+};
+template <class S> int launchIlu0Factor(int n, int bs, const int* rp, const int* ci, const int* diag,
+                                        S* lu, const LevelSchedule& L, int* dFail, hipStream_t st);
+template <class S, class P> void launchIlu0Apply(int n, int bs, const int* rp, const int* ci,
+                                                 const int* diag, const S* lu, const P* dinv,
+                                                 const S* d, S* v, const LevelSchedule& L, hipStream_t st);
+template <class S> void launchJacobiInvert(int n, int bs, const int* diag, const S* a, S* dinv,
+                                           int* dFail, hipStream_t st);
+template <class S> void launchJacobiApplyMixed(int n, const double* dinv, const S* d, S* v, double w,
+                                               hipStream_t st);
+template <class S> void launchExtractInvDiag(int n, int bs, const int* diag, const S* lu, double* dinv, hipStream_t st);
+
+// BiCGStab steps (see bicgstab in host_solver.cpp for the sequence)
+template <class S> void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks,
+                                       int op, int tag, hipStream_t st);
+template <class S> void launchUpdateP(long m, const S* r, const S* v, S* p, const KrylovScalars* ks,
+                                      hipStream_t st);
+template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* r, double* partial,
+                                       KrylovScalars* ks, hipStream_t st);
+template <class S> void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial,
+                                        KrylovScalars* ks, int tag, hipStream_t st);
+template <class S> void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks,
+                                        int tag, hipStream_t st);
+template <class S> void launchMaxAbs(long m, const S* x, double* partial, double* out, hipStream_t st);
+void launchResidualUpdate(int n, int bs, const int* rp, const int* ci, const double* a, const double* x,
+                          const double* b, double* r, hipStream_t st);
+void launchAxpyD(long m, double alpha, const double* x, double* y, hipStream_t st);
+void launchAddFromFloat(long m, const float* d, double* x, hipStream_t st);
+
+}  // namespace omodels

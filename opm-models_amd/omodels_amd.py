@@ -1,0 +1,345 @@
+"""ctypes binding of libomodels_amd.so (the C ABI declared in include/omodels.h).
+
+Host arrays are numpy arrays; every call returns plain numpy data.  The
+library must have been built (python opm-models_amd/build_lib.py); there is
+no fallback implementation: a missing library raises at load time.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libomodels_amd.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class BlackOilProblem(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_src", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("gravity", C.c_double),
+        ("perm", _dp), ("poro", _dp),
+        ("n_oil", C.c_int32), ("n_gas", C.c_int32),
+        ("oil_tab", _dp), ("gas_tab", _dp),
+        ("oil_comp", C.c_double), ("water_pref", C.c_double), ("water_comp", C.c_double),
+        ("water_mu", C.c_double), ("rho_ref", C.c_double * 3),
+        ("src_cell", _ip), ("src_type", _ip), ("src_a", _dp), ("src_b", _dp),
+    ]
+
+
+class BlackOilSpec(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("field", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("perm_mean_md", C.c_double), ("perm_sigma", C.c_double), ("kz_ratio", C.c_double),
+        ("chan_perm_md", C.c_double), ("back_perm_md", C.c_double), ("chan_fraction", C.c_double),
+        ("corr_len", C.c_double),
+        ("poro_min", C.c_double), ("poro_max", C.c_double),
+        ("p_top", C.c_double), ("sw_init", C.c_double),
+        ("wells", C.c_int32), ("pad", C.c_int32),
+        ("inj_rate", C.c_double), ("prod_bhp", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("newton_tol", C.c_double), ("newton_max_iter", C.c_int32), ("newton_target_iter", C.c_int32),
+        ("newton_max_error", C.c_double), ("lin_tol", C.c_double), ("lin_abs_tol", C.c_double),
+        ("lin_max_error", C.c_double), ("lin_max_iter", C.c_int32), ("precond", C.c_int32),
+        ("prec_relax", C.c_double), ("prec_order", C.c_int32), ("max_dt_divisions", C.c_int32),
+        ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
+        ("max_dt", C.c_double), ("min_dt", C.c_double), ("precision", C.c_int32), ("verbose", C.c_int32),
+    ]
+
+
+class IterLog(C.Structure):
+    _fields_ = [("step", C.c_int32), ("attempt", C.c_int32), ("iter", C.c_int32), ("lin_iters", C.c_int32),
+                ("num_switched", C.c_int32), ("lin_converged", C.c_int32), ("error", C.c_double),
+                ("dt", C.c_double)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int32), ("linearizations", C.c_int32), ("linear_iters", C.c_int32),
+                ("failed_attempts", C.c_int32), ("t_linearize", C.c_double), ("t_solve", C.c_double),
+                ("t_update", C.c_double), ("t_total", C.c_double)]
+
+
+# every symbol declared in include/omodels.h, with its ctypes signature
+SIGNATURES = {
+    "omodels_blackoil_default_fluid": (C.c_int, [_dp, _ip, _dp, _ip, _dp]),
+    "omodels_blackoil_generate": (C.c_int, [C.POINTER(BlackOilSpec), _dp, _dp, _dp, _u8p, C.c_int32, _ip, _ip,
+                                            _ip, _dp, _dp]),
+    "omodels_default_params": (None, [C.POINTER(Params)]),
+    "omodels_last_error": (C.c_char_p, []),
+    "omodels_model_create": (C.c_void_p, [C.POINTER(BlackOilProblem)]),
+    "omodels_model_destroy": (None, [C.c_void_p]),
+    "omodels_model_num_cells": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "omodels_model_set_solution": (C.c_int, [C.c_void_p, _dp, _u8p, C.c_int]),
+    "omodels_model_get_solution": (C.c_int, [C.c_void_p, _dp, _u8p]),
+    "omodels_model_set_storage1": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_model_get_storage1": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_linearizer_pattern": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]),
+    "omodels_linearizer_linearize_domain": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "omodels_linearizer_get_jacobian": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_linearizer_get_residual": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_linearizer_erase_matrix": (C.c_int, [C.c_void_p]),
+    "omodels_solver_create": (C.c_void_p, [C.c_void_p, C.POINTER(Params)]),
+    "omodels_solver_destroy": (None, [C.c_void_p]),
+    "omodels_solver_prepare": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_solver_get_residual": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_solver_solve": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_solver_iterations": (C.c_int, [C.c_void_p]),
+    "omodels_solver_get_ilu": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
+    "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
+    "omodels_simulate": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_double, _dp, _dp,
+                                   C.POINTER(IterLog), C.c_int, _ip, C.POINTER(RunStats)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library once (raises OSError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libomodels_amd.so is not built: run opm-models_amd/build_lib.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a, t=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+class OmodelsError(RuntimeError):
+    pass
+
+
+def check(rc, what):
+    if rc < 0:
+        raise OmodelsError("%s failed (%d): %s" % (what, rc, lib().omodels_last_error().decode()))
+    return rc
+
+
+def default_params(**kw):
+    p = Params()
+    lib().omodels_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_fluid():
+    oil = np.zeros(4 * 12)
+    gas = np.zeros(3 * 12)
+    no = C.c_int32()
+    ng = C.c_int32()
+    consts = np.zeros(7)
+    check(lib().omodels_blackoil_default_fluid(ptr(oil), C.byref(no), ptr(gas), C.byref(ng), ptr(consts)),
+          "default_fluid")
+    return oil[:4 * no.value].copy(), no.value, gas[:3 * ng.value].copy(), ng.value, consts
+
+
+def make_spec(nx, ny, nz, dx=50.0, dy=50.0, dz=5.0, field=0, perm_mean_md=100.0, perm_sigma=1.0,
+              kz_ratio=0.1, chan_perm_md=1000.0, back_perm_md=1.0, chan_fraction=0.3, corr_len=0.0,
+              poro_min=0.15, poro_max=0.30, p_top=200e5, sw_init=0.2, wells=0, inj_rate=None,
+              prod_bhp=150e5, seed=1234):
+    s = BlackOilSpec()
+    s.nx, s.ny, s.nz, s.field = nx, ny, nz, field
+    s.dx, s.dy, s.dz = dx, dy, dz
+    s.perm_mean_md, s.perm_sigma, s.kz_ratio = perm_mean_md, perm_sigma, kz_ratio
+    s.chan_perm_md, s.back_perm_md, s.chan_fraction, s.corr_len = chan_perm_md, back_perm_md, chan_fraction, corr_len
+    s.poro_min, s.poro_max, s.p_top, s.sw_init = poro_min, poro_max, p_top, sw_init
+    s.wells = wells
+    if inj_rate is None:
+        # 0.01% of the pore volume of water per day
+        inj_rate = 1e-4 * (nx * dx * ny * dy * nz * dz) * 0.225 * 1037.0 / 86400.0
+    s.inj_rate, s.prod_bhp, s.seed = inj_rate, prod_bhp, seed
+    return s
+
+
+class BlackOilCase:
+    """Host arrays of one black-oil problem plus the ctypes descriptor."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        n = spec.nx * spec.ny * spec.nz
+        self.N = n
+        self.perm = np.zeros(3 * n)
+        self.poro = np.zeros(n)
+        self.pv0 = np.zeros(3 * n)
+        self.mean0 = np.zeros(n, dtype=np.uint8)
+        cap = 8 * spec.nz + 8
+        self.src_cell = np.zeros(cap, dtype=np.int32)
+        self.src_type = np.zeros(cap, dtype=np.int32)
+        self.src_a = np.zeros(cap)
+        self.src_b = np.zeros(cap)
+        ns = C.c_int32()
+        check(lib().omodels_blackoil_generate(C.byref(spec), ptr(self.perm), ptr(self.poro), ptr(self.pv0),
+                                              ptr(self.mean0, _u8p), cap, C.byref(ns), ptr(self.src_cell, _ip),
+                                              ptr(self.src_type, _ip), ptr(self.src_a), ptr(self.src_b)),
+              "generate")
+        k = ns.value
+        self.src_cell, self.src_type = self.src_cell[:k].copy(), self.src_type[:k].copy()
+        self.src_a, self.src_b = self.src_a[:k].copy(), self.src_b[:k].copy()
+        self.oil, self.n_oil, self.gas, self.n_gas, self.consts = default_fluid()
+        self.prob = self.descriptor(BlackOilProblem)
+
+    def descriptor(self, cls):
+        s = self.spec
+        p = cls()
+        p.nx, p.ny, p.nz, p.n_src = s.nx, s.ny, s.nz, len(self.src_cell)
+        p.dx, p.dy, p.dz, p.gravity = s.dx, s.dy, s.dz, -9.81
+        p.perm, p.poro = ptr(self.perm), ptr(self.poro)
+        p.n_oil, p.n_gas = self.n_oil, self.n_gas
+        p.oil_tab, p.gas_tab = ptr(self.oil), ptr(self.gas)
+        c = self.consts
+        p.oil_comp, p.water_pref, p.water_comp, p.water_mu = c[0], c[1], c[2], c[3]
+        p.rho_ref[0], p.rho_ref[1], p.rho_ref[2] = c[4], c[5], c[6]
+        p.src_cell, p.src_type = ptr(self.src_cell, _ip), ptr(self.src_type, _ip)
+        p.src_a, p.src_b = ptr(self.src_a), ptr(self.src_b)
+        return p
+
+
+class Model:
+    """Device-resident reservoir state with the linearizer interface."""
+
+    def __init__(self, case):
+        self.case = case
+        h = lib().omodels_model_create(C.byref(case.prob))
+        if not h:
+            raise OmodelsError("model_create failed: " + lib().omodels_last_error().decode())
+        self.h = C.c_void_p(h)
+        n = C.c_int64()
+        z = C.c_int64()
+        lib().omodels_model_num_cells(self.h, C.byref(n), C.byref(z))
+        self.N, self.nnzb = n.value, z.value
+
+    def close(self):
+        if self.h:
+            lib().omodels_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solution(self, pv, mean, both_levels=True):
+        pv = np.ascontiguousarray(pv, dtype=np.float64)
+        mean = np.ascontiguousarray(mean, dtype=np.uint8)
+        check(lib().omodels_model_set_solution(self.h, ptr(pv), ptr(mean, _u8p), int(both_levels)), "set_solution")
+
+    def get_solution(self):
+        pv = np.zeros(3 * self.N)
+        m = np.zeros(self.N, dtype=np.uint8)
+        check(lib().omodels_model_get_solution(self.h, ptr(pv), ptr(m, _u8p)), "get_solution")
+        return pv, m
+
+    def set_storage1(self, st=None):
+        check(lib().omodels_model_set_storage1(self.h, ptr(st) if st is not None else None), "set_storage1")
+
+    def get_storage1(self):
+        st = np.zeros(3 * self.N)
+        check(lib().omodels_model_get_storage1(self.h, ptr(st)), "get_storage1")
+        return st
+
+    def pattern(self):
+        N, z = self.N, self.nnzb
+        nf = z - N
+        rp = np.zeros(N + 1, np.int32)
+        ci = np.zeros(z, np.int32)
+        dg = np.zeros(N, np.int32)
+        nbp = np.zeros(N + 1, np.int32)
+        nbc = np.zeros(nf, np.int32)
+        nbo = np.zeros(nf, np.int32)
+        nbt = np.zeros(nf)
+        nbz = np.zeros(nf)
+        check(lib().omodels_linearizer_pattern(self.h, ptr(rp, _ip), ptr(ci, _ip), ptr(dg, _ip), ptr(nbp, _ip),
+                                               ptr(nbc, _ip), ptr(nbo, _ip), ptr(nbt), ptr(nbz)), "pattern")
+        return dict(row_ptr=rp, col_idx=ci, diag_off=dg, nb_ptr=nbp, nb_cell=nbc, nb_off=nbo, nb_trans=nbt,
+                    nb_dzg=nbz)
+
+    def linearize(self, dt, iteration_index=1):
+        check(lib().omodels_linearizer_linearize_domain(self.h, dt, iteration_index), "linearize_domain")
+        J = np.zeros(9 * self.nnzb)
+        r = np.zeros(3 * self.N)
+        check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
+        check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
+        return J, r
+
+    def newton_update(self, delta, params):
+        delta = np.ascontiguousarray(delta, dtype=np.float64)
+        ns = C.c_int32()
+        rc = check(lib().omodels_newton_update(self.h, C.byref(params), ptr(delta), C.byref(ns)), "newton_update")
+        sw = np.zeros(self.N, np.uint8)
+        check(lib().omodels_model_get_switched(self.h, ptr(sw, _u8p)), "get_switched")
+        return rc, ns.value, sw
+
+    def simulate(self, params, n_steps, dt0, keep_history=True, log_cap=4096):
+        hist = np.zeros((n_steps, 3 * self.N)) if keep_history else None
+        dts = np.zeros(n_steps)
+        log = (IterLog * log_cap)()
+        nlog = C.c_int32()
+        st = RunStats()
+        rc = check(lib().omodels_simulate(self.h, C.byref(params), n_steps, dt0,
+                                          ptr(hist) if hist is not None else None, ptr(dts), log, log_cap,
+                                          C.byref(nlog), C.byref(st)), "simulate")
+        recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
+                for l in log[:min(nlog.value, log_cap)]]
+        return rc, hist, dts, recs, st
+
+
+class Solver:
+    """BiCGStab backend bound to a model."""
+
+    def __init__(self, model, params):
+        self.model = model
+        h = lib().omodels_solver_create(model.h, C.byref(params))
+        if not h:
+            raise OmodelsError("solver_create failed: " + lib().omodels_last_error().decode())
+        self.h = C.c_void_p(h)
+
+    def close(self):
+        if self.h:
+            lib().omodels_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, values=None, residual=None):
+        check(lib().omodels_solver_prepare(self.h, ptr(values), ptr(residual)), "solver_prepare")
+
+    def solve(self):
+        x = np.zeros(3 * self.model.N)
+        rc = check(lib().omodels_solver_solve(self.h, ptr(x)), "solver_solve")
+        return rc, x, lib().omodels_solver_iterations(self.h)
+
+    def ilu(self):
+        lu = np.zeros(9 * self.model.nnzb)
+        check(lib().omodels_solver_get_ilu(self.h, ptr(lu)), "get_ilu")
+        return lu
+
+    def apply_precond(self, d):
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        v = np.zeros_like(d)
+        check(lib().omodels_solver_apply_precond(self.h, ptr(d), ptr(v)), "apply_precond")
+        return v

@@ -1,0 +1,1051 @@
+/*
+ * blackoil.cpp -- CPU restatement (the oracle) of opm-models' fully-implicit
+ * black-oil TPFA path: TpfaLinearizer, BlackOilLocalResidualTPFA,
+ * BlackOilIntensiveQuantities, BiCGStab + block ILU0 / block Jacobi,
+ * NewtonMethod + BlackOilNewtonMethod and the time-step control.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker / CPU baseline, never by the
+ * product library.  Written as plain, sequential-by-default C++ so that each
+ * function reads against the reference file:line it restates.
+ *
+ * Choices for code that lives outside opm-models (opm-common, dune-istl,
+ * opm-simulators; absent here, "parity unpinned" against them):
+ *  - DenseAd: value + N derivatives; a/b = a * (1/b), (a/b)' = (a' - (a/b) b') / b
+ *    computed with the reciprocal; min/max keep the AD operand at ties.
+ *  - LinearMaterial (tests/problems/reservoirproblem.hh:84-100,340-350):
+ *    kr_a = max(min(S_a, 1), 0), pc = 0 (pcMinSat = pcMaxSat = 0).  The
+ *    relperm hook (multiphasebaseproblem.hh:257) is filled as flow does.
+ *  - PVT: piecewise-linear tables in p of 1/B and mu (LiveOil saturated,
+ *    DryGas), linear extrapolation outside the table; undersaturated oil:
+ *    pb = Rs_sat^{-1}(Rs), 1/Bo = 1/Bo_sat(pb) * (1 + c_o (p - pb)),
+ *    mu_o = mu_o_sat(pb).  Water: 1/Bw = 1 + X (1 + X/2), X = c_w (p - p_ref),
+ *    constant mu_w (ConstantCompressibilityWater with c_v = 0, Bw_ref = 1).
+ *  - Transmissibility: flow's half-transmissibility A K |n.d| / |d|^2 with the
+ *    harmonic combination of transfluxmodule.hh:425-456.
+ *  - Pressure difference: transfluxmodule.hh:254-303 (no threshold pressure).
+ *  - 3x3/2x2 block inverse by the adjugate; det == 0 -> NumericalProblem.
+ *
+ * Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <chrono>
+#include <limits>
+#include <set>
+#include <stdexcept>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* ------------------------------------------------------------------ */
+/* Forward-mode AD scalar (restates DenseAd::Evaluation semantics as used by
+ * fvbaseadlocallinearizer.hh:62-72 and blackoilprimaryvariables.hh:217-233). */
+template <int N>
+struct Ev {
+    double v;
+    double d[N];
+    Ev() : v(0.0) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+    Ev(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+    static Ev var(double x, int idx) { Ev r(x); r.d[idx] = 1.0; return r; }
+};
+template <int N> inline Ev<N> operator+(const Ev<N>& a, const Ev<N>& b)
+{ Ev<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> inline Ev<N> operator-(const Ev<N>& a, const Ev<N>& b)
+{ Ev<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> inline Ev<N> operator*(const Ev<N>& a, const Ev<N>& b)
+{ Ev<N> r; r.v = a.v * b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> inline Ev<N> operator/(const Ev<N>& a, const Ev<N>& b)
+{
+    Ev<N> r; const double inv = 1.0 / b.v; r.v = a.v * inv;
+    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
+    return r;
+}
+template <int N> inline Ev<N> operator+(const Ev<N>& a, double s)
+{ Ev<N> r = a; r.v = a.v + s; return r; }
+template <int N> inline Ev<N> operator-(const Ev<N>& a, double s)
+{ Ev<N> r = a; r.v = a.v - s; return r; }
+template <int N> inline Ev<N> operator-(double s, const Ev<N>& a)
+{ Ev<N> r; r.v = s - a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> inline Ev<N> operator*(const Ev<N>& a, double s)
+{ Ev<N> r; r.v = a.v * s; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * s; return r; }
+template <int N> inline Ev<N> operator*(double s, const Ev<N>& a) { return a * s; }
+template <int N> inline Ev<N> ev_min(const Ev<N>& a, double c) { return a.v <= c ? a : Ev<N>(c); }
+template <int N> inline Ev<N> ev_max(const Ev<N>& a, double c) { return a.v >= c ? a : Ev<N>(c); }
+
+typedef Ev<3> E3;
+
+/* ------------------------------------------------------------------ */
+/* Piecewise-linear table (Tabulated1DFunction with extrapolation). */
+struct Table {
+    std::vector<double> x, y, slope;
+    void set(const double* xs, const double* ys, int n)
+    {
+        x.assign(xs, xs + n); y.assign(ys, ys + n); slope.resize(n - 1);
+        for (int i = 0; i + 1 < n; ++i) slope[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+    }
+    int seg(double xv) const
+    {
+        int s = 0;
+        for (int i = 1; i + 1 < (int)x.size(); ++i) if (xv >= x[i]) s = i;
+        return s;
+    }
+    double eval(double xv) const { int s = seg(xv); return y[s] + slope[s] * (xv - x[s]); }
+    template <int N> Ev<N> eval(const Ev<N>& xv) const
+    {
+        int s = seg(xv.v); Ev<N> r;
+        r.v = y[s] + slope[s] * (xv.v - x[s]);
+        for (int i = 0; i < N; ++i) r.d[i] = slope[s] * xv.d[i];
+        return r;
+    }
+};
+
+enum { W = 0, O = 1, G = 2 };                 /* phases (BlackOilFluidSystem) */
+enum { CO = 0, CW = 1, CG = 2 };              /* components == equations */
+static const int compOfPhase[3] = {CW, CO, CG};
+enum { MEAN_SG = 0, MEAN_RS = 1 };            /* GasMeaning used here */
+
+struct Pvt {
+    Table rsSat, pOfRs, invBoSat, muoSat, invBg, mug;
+    double co, pref, cw, muw, rho[3];
+    void init(const ob_problem* p)
+    {
+        const int n = p->n_oil; const double* t = p->oil_tab;
+        rsSat.set(t, t + n, n); pOfRs.set(t + n, t, n);
+        invBoSat.set(t, t + 2 * n, n); muoSat.set(t, t + 3 * n, n);
+        const int m = p->n_gas; const double* g = p->gas_tab;
+        invBg.set(g, g + m, m); mug.set(g, g + 2 * m, m);
+        co = p->oil_comp; pref = p->water_pref; cw = p->water_comp; muw = p->water_mu;
+        for (int i = 0; i < 3; ++i) rho[i] = p->rho_ref[i];
+    }
+};
+
+/* Intensive quantities of one cell; T = E3 (own cell, with derivatives) or
+ * double (neighbour, value only).  Restates
+ * blackoilintensivequantities.hh:179-516 for the Sw/Po/{Sg,Rs} meanings. */
+template <class T>
+struct IQ {
+    T S[3], p[3], kr[3], invB[3], mu[3], lam[3], rho[3], Rs, phi;
+};
+
+inline double valueOf(double x) { return x; }
+inline double valueOf(const E3& x) { return x.v; }
+inline double dmax(double a, double c) { return a >= c ? a : c; }
+inline double dmin(double a, double c) { return a <= c ? a : c; }
+inline E3 tmin(const E3& a, double c) { return ev_min(a, c); }
+inline E3 tmax(const E3& a, double c) { return ev_max(a, c); }
+inline double tmin(double a, double c) { return dmin(a, c); }
+inline double tmax(double a, double c) { return dmax(a, c); }
+
+template <class T> T makeVar(double x, int idx);
+template <> inline E3 makeVar<E3>(double x, int idx) { return E3::var(x, idx); }
+template <> inline double makeVar<double>(double x, int) { return x; }
+template <class T> inline T tdiv(const T& a, const T& b);
+template <> inline E3 tdiv<E3>(const E3& a, const E3& b) { return a / b; }
+template <> inline double tdiv<double>(const double& a, const double& b) { return a * (1.0 / b); }
+
+template <class T>
+void computeIQ(const Pvt& pvt, const double* pv, int meaning, double poro, IQ<T>& q)
+{
+    const T Sw = makeVar<T>(pv[0], 0);
+    const T Sg = (meaning == MEAN_SG) ? makeVar<T>(pv[2], 2) : T(0.0);
+    const T So = (1.0 - Sw) - Sg;
+    q.S[W] = Sw; q.S[O] = So; q.S[G] = Sg;
+    const T po = makeVar<T>(pv[1], 1);
+    /* LinearMaterial: pc == 0, hence p_a = po + (pc_a - pc_o) = po */
+    for (int a = 0; a < 3; ++a) q.p[a] = po;
+    for (int a = 0; a < 3; ++a) q.kr[a] = tmax(tmin(q.S[a], 1.0), 0.0);
+    /* dissolved gas: blackoilintensivequantities.hh:309-323 */
+    if (meaning == MEAN_RS) q.Rs = makeVar<T>(pv[2], 2);
+    else q.Rs = pvt.rsSat.eval(po);
+    /* water: ConstantCompressibilityWater */
+    {
+        T X = (q.p[W] - pvt.pref) * pvt.cw;
+        T u = X * 0.5 + 1.0;
+        q.invB[W] = X * u + 1.0;
+        q.mu[W] = T(pvt.muw);
+    }
+    /* oil: LiveOil */
+    if (meaning == MEAN_SG) {
+        q.invB[O] = pvt.invBoSat.eval(q.p[O]);
+        q.mu[O] = pvt.muoSat.eval(q.p[O]);
+    } else {
+        T pb = pvt.pOfRs.eval(q.Rs);
+        T f = (q.p[O] - pb) * pvt.co + 1.0;
+        q.invB[O] = pvt.invBoSat.eval(pb) * f;
+        q.mu[O] = pvt.muoSat.eval(pb);
+    }
+    /* gas: DryGas */
+    q.invB[G] = pvt.invBg.eval(q.p[G]);
+    q.mu[G] = pvt.mug.eval(q.p[G]);
+    /* mobilities: blackoilintensivequantities.hh:366-397 */
+    for (int a = 0; a < 3; ++a) q.lam[a] = tdiv<T>(q.kr[a], q.mu[a]);
+    /* densities: blackoilintensivequantities.hh:400-442 */
+    q.rho[W] = q.invB[W] * pvt.rho[W];
+    q.rho[G] = q.invB[G] * pvt.rho[G];
+    q.rho[O] = q.invB[O] * pvt.rho[O] + (q.invB[O] * q.Rs) * pvt.rho[G];
+    /* porosity: no rock compressibility, multiplier 1 (blackoilproblem.hh) */
+    q.phi = T(poro);
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid helpers (YaspGrid order, EcfvStencil intersection order). */
+struct Grid {
+    int nx, ny, nz; long N;
+    double dx, dy, dz, g;
+    const ob_problem* P;
+    explicit Grid(const ob_problem* p)
+        : nx(p->nx), ny(p->ny), nz(p->nz), N((long)p->nx * p->ny * p->nz),
+          dx(p->dx), dy(p->dy), dz(p->dz), g(p->gravity), P(p) {}
+    double volume() const { return dx * dy * dz; }
+    double zc(long I) const { long k = I / ((long)nx * ny); return dz * (k + 0.5); }
+    /* neighbour of I across intersection `dir` (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z), -1 if boundary */
+    long nb(long I, int dir) const
+    {
+        long i = I % nx, j = (I / nx) % ny, k = I / ((long)nx * ny);
+        switch (dir) {
+        case 0: return i > 0 ? I - 1 : -1;
+        case 1: return i + 1 < nx ? I + 1 : -1;
+        case 2: return j > 0 ? I - nx : -1;
+        case 3: return j + 1 < ny ? I + nx : -1;
+        case 4: return k > 0 ? I - (long)nx * ny : -1;
+        default: return k + 1 < nz ? I + (long)nx * ny : -1;
+        }
+    }
+    double area(int dir) const
+    {
+        int ax = dir / 2;
+        return ax == 0 ? dy * dz : (ax == 1 ? dx * dz : dx * dy);
+    }
+    double h(int dir) const { int ax = dir / 2; return ax == 0 ? dx : (ax == 1 ? dy : dz); }
+    /* transfluxmodule.hh:425-456 with flow's area factor */
+    double trans(long I, long J, int dir) const
+    {
+        const int ax = dir / 2;
+        const double A = area(dir), ndot = 0.5 * h(dir), dist2 = ndot * ndot;
+        const double K0 = P->perm[(long)ax * N + I], K1 = P->perm[(long)ax * N + J];
+        const double T0 = A * K0 * ndot / dist2;
+        const double T1 = A * K1 * ndot / dist2;
+        return T0 * T1 / (T0 + T1);
+    }
+};
+
+struct NbInfo { long J; int off; double trans, area, dZg; int dir; };
+
+struct Pattern {
+    std::vector<int32_t> rowPtr, colIdx, diagOff, nbPtr;
+    std::vector<NbInfo> nb;
+};
+
+/* TpfaLinearizer::createMatrix_ (tpfalinearizer.hh:414-525) with
+ * IstlSparseMatrixAdapter::reserve (istlsparsematrixadapter.hh:76-101). */
+void buildPattern(const Grid& G, Pattern& P)
+{
+    const long N = G.N;
+    std::vector<std::set<long>> sp(N);
+    P.nbPtr.assign(N + 1, 0);
+    P.nb.clear();
+    for (long I = 0; I < N; ++I) {
+        sp[I].insert(I);
+        for (int dir = 0; dir < 6; ++dir) {
+            long J = G.nb(I, dir);
+            if (J < 0) continue;
+            sp[I].insert(J);
+            NbInfo n;
+            n.J = J; n.off = -1; n.trans = G.trans(I, J, dir); n.area = G.area(dir);
+            n.dZg = (G.zc(I) - G.zc(J)) * G.g; n.dir = dir;
+            P.nb.push_back(n);
+        }
+        P.nbPtr[I + 1] = (int32_t)P.nb.size();
+    }
+    P.rowPtr.assign(N + 1, 0);
+    for (long I = 0; I < N; ++I) P.rowPtr[I + 1] = P.rowPtr[I] + (int32_t)sp[I].size();
+    P.colIdx.resize(P.rowPtr[N]);
+    for (long I = 0; I < N; ++I) {
+        int k = P.rowPtr[I];
+        for (long c : sp[I]) P.colIdx[k++] = (int32_t)c;
+    }
+    auto blockAddress = [&](long row, long col) -> int32_t {
+        const int32_t* b = P.colIdx.data() + P.rowPtr[row];
+        const int32_t* e = P.colIdx.data() + P.rowPtr[row + 1];
+        const int32_t* it = std::lower_bound(b, e, (int32_t)col);
+        if (it == e || *it != col) throw std::logic_error("missing block");
+        return (int32_t)(it - P.colIdx.data());
+    };
+    P.diagOff.resize(N);
+    for (long I = 0; I < N; ++I) {
+        P.diagOff[I] = blockAddress(I, I);
+        for (int k = P.nbPtr[I]; k < P.nbPtr[I + 1]; ++k) P.nb[k].off = blockAddress(P.nb[k].J, I);
+    }
+}
+
+/* BlackOilLocalResidualTPFA::computeStorage (blackoillocalresidualtpfa.hh:151-227)
+ * incl. adaptMassConservationQuantities_ (:824-851). */
+template <class T>
+void storageOf(const Pvt& pvt, const IQ<T>& q, T st[3])
+{
+    for (int e = 0; e < 3; ++e) st[e] = T(0.0);
+    for (int a = 0; a < 3; ++a) {
+        T sv = (q.S[a] * q.invB[a]) * q.phi;
+        st[compOfPhase[a]] = st[compOfPhase[a]] + sv;
+        if (a == O) st[CG] = st[CG] + q.Rs * sv;
+    }
+    st[CW] = st[CW] * pvt.rho[W];
+    st[CG] = st[CG] * pvt.rho[G];
+    st[CO] = st[CO] * pvt.rho[O];
+}
+
+/* calculateFluxes_ (blackoillocalresidualtpfa.hh:325-504) with the
+ * pressure difference of transfluxmodule.hh:254-303 and evalPhaseFluxes_
+ * (:752-811), followed by the caller's multiplication with the face area
+ * (tpfalinearizer.hh:758). */
+void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, long J,
+              const NbInfo& nbi, double Vin, double Vex, E3 flux[3])
+{
+    for (int e = 0; e < 3; ++e) flux[e] = E3(0.0);
+    for (int a = 0; a < 3; ++a) {
+        if (in.lam[a].v <= 0.0 && ex.lam[a] <= 0.0) continue;
+        E3 rhoAvg = (in.rho[a] + ex.rho[a]) * 0.5;
+        E3 pEx = rhoAvg * nbi.dZg + ex.p[a];
+        E3 dp = pEx - in.p[a];
+        bool upIn;
+        if (dp.v > 0.0) upIn = false;
+        else if (dp.v < 0.0) upIn = true;
+        else if (Vin > Vex) upIn = true;
+        else if (Vin < Vex) upIn = false;
+        else upIn = (I < J);
+        const double tf = (-nbi.trans) / nbi.area;
+        E3 darcy, svf;
+        if (upIn) {
+            darcy = (dp * in.lam[a]) * tf;
+            svf = in.invB[a] * darcy;
+        } else {
+            darcy = dp * (ex.lam[a] * tf);
+            svf = darcy * ex.invB[a];
+        }
+        flux[compOfPhase[a]] = flux[compOfPhase[a]] + svf * pvt.rho[a];
+        if (a == O) {
+            E3 gs = upIn ? (in.Rs * svf) : (svf * ex.Rs);
+            flux[CG] = flux[CG] + gs * pvt.rho[G];
+        }
+    }
+    for (int e = 0; e < 3; ++e) flux[e] = flux[e] * nbi.area;
+}
+
+/* ReservoirProblem-style source(): injectors add water mass, producers are
+ * bottom-hole-pressure controlled.  Rate per unit volume. */
+void sourceOf(const Pvt& pvt, const IQ<E3>& q, int type, double a, double b, double V, E3 src[3])
+{
+    for (int e = 0; e < 3; ++e) src[e] = E3(0.0);
+    if (type == 0) {
+        src[CW] = E3(a / V);
+        return;
+    }
+    E3 drive = q.p[O] - b;
+    if (drive.v <= 0.0) return;
+    for (int ph = 0; ph < 3; ++ph) {
+        E3 qs = ((q.lam[ph] * q.invB[ph]) * drive) * (-a);
+        src[compOfPhase[ph]] = src[compOfPhase[ph]] + (qs * pvt.rho[ph]) * (1.0 / V);
+        if (ph == O) src[CG] = src[CG] + ((q.Rs * qs) * pvt.rho[G]) * (1.0 / V);
+    }
+}
+
+struct Problem {
+    const ob_problem* P;
+    Grid G;
+    Pvt pvt;
+    Pattern pat;
+    std::vector<int> srcOfCell;
+    explicit Problem(const ob_problem* p) : P(p), G(p)
+    {
+        pvt.init(p);
+        buildPattern(G, pat);
+        srcOfCell.assign(G.N, -1);
+        for (int s = 0; s < p->n_src; ++s) srcOfCell[p->src_cell[s]] = s;
+    }
+};
+
+void computeStorageAll(const Problem& pr, const double* pv, const uint8_t* mean, double* st)
+{
+    for (long I = 0; I < pr.G.N; ++I) {
+        IQ<double> q;
+        computeIQ<double>(pr.pvt, pv + 3 * I, mean[I], pr.P->poro[I], q);
+        double s[3];
+        storageOf<double>(pr.pvt, q, s);
+        for (int e = 0; e < 3; ++e) st[3 * I + e] = s[e];
+    }
+}
+
+/* TpfaLinearizer::linearize_ (tpfalinearizer.hh:711-866): resetSystem_ then
+ * one pass over cells, each cell owning r[I], A[I][I] and the column blocks
+ * A[J][I] of its neighbours. */
+void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const double* st1,
+               double dt, double* res, double* jac, int nthreads)
+{
+    const long N = pr.G.N;
+    const long nnzb = pr.pat.rowPtr[N];
+    std::fill(res, res + 3 * N, 0.0);
+    std::fill(jac, jac + 9 * nnzb, 0.0);
+    const double V = pr.G.volume();
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (long I = 0; I < N; ++I) {
+        IQ<E3> qi;
+        computeIQ<E3>(pr.pvt, pv + 3 * I, mean[I], pr.P->poro[I], qi);
+        double* r = res + 3 * I;
+        double* D = jac + 9L * pr.pat.diagOff[I];
+        for (int k = pr.pat.nbPtr[I]; k < pr.pat.nbPtr[I + 1]; ++k) {
+            const NbInfo& nbi = pr.pat.nb[k];
+            IQ<double> qj;
+            computeIQ<double>(pr.pvt, pv + 3 * nbi.J, mean[nbi.J], pr.P->poro[nbi.J], qj);
+            E3 f[3];
+            faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f);
+            double* O_ = jac + 9L * nbi.off;
+            for (int e = 0; e < 3; ++e) {
+                r[e] += f[e].v;
+                for (int pvi = 0; pvi < 3; ++pvi) {
+                    D[3 * e + pvi] += f[e].d[pvi];
+                    O_[3 * e + pvi] += -f[e].d[pvi];
+                }
+            }
+        }
+        /* accumulation (:776-825), storage cache of the step start */
+        {
+            E3 s[3];
+            storageOf<E3>(pr.pvt, qi, s);
+            const double storefac = V / dt;
+            for (int e = 0; e < 3; ++e) {
+                double rv = s[e].v;
+                rv -= st1[3 * I + e];
+                rv *= storefac;
+                r[e] += rv;
+                for (int pvi = 0; pvi < 3; ++pvi) D[3 * e + pvi] += s[e].d[pvi] * storefac;
+            }
+        }
+        /* cell-wise sources (:827-841) */
+        const int s = pr.srcOfCell[I];
+        if (s >= 0) {
+            E3 q[3];
+            sourceOf(pr.pvt, qi, pr.P->src_type[s], pr.P->src_a[s], pr.P->src_b[s], V, q);
+            for (int e = 0; e < 3; ++e) {
+                E3 t = q[e] * (-V);
+                r[e] += t.v;
+                for (int pvi = 0; pvi < 3; ++pvi) D[3 * e + pvi] += t.d[pvi];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Block helpers, bs = 2 or 3, row-major (Dune FieldMatrix). */
+template <class S>
+bool invertBlock(int bs, S* a)
+{
+    if (bs == 2) {
+        S det = a[0] * a[3] - a[1] * a[2];
+        if (!(det != S(0)) || !std::isfinite((double)det)) return false;
+        S r = S(1) / det;
+        S i00 = a[3] * r, i01 = -a[1] * r, i10 = -a[2] * r, i11 = a[0] * r;
+        a[0] = i00; a[1] = i01; a[2] = i10; a[3] = i11;
+        return true;
+    }
+    S c00 = a[4] * a[8] - a[5] * a[7];
+    S c01 = a[5] * a[6] - a[3] * a[8];
+    S c02 = a[3] * a[7] - a[4] * a[6];
+    S det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (!(det != S(0)) || !std::isfinite((double)det)) return false;
+    S r = S(1) / det;
+    S c10 = a[2] * a[7] - a[1] * a[8];
+    S c11 = a[0] * a[8] - a[2] * a[6];
+    S c12 = a[1] * a[6] - a[0] * a[7];
+    S c20 = a[1] * a[5] - a[2] * a[4];
+    S c21 = a[2] * a[3] - a[0] * a[5];
+    S c22 = a[0] * a[4] - a[1] * a[3];
+    a[0] = c00 * r; a[1] = c10 * r; a[2] = c20 * r;
+    a[3] = c01 * r; a[4] = c11 * r; a[5] = c21 * r;
+    a[6] = c02 * r; a[7] = c12 * r; a[8] = c22 * r;
+    return true;
+}
+/* C = A * B */
+template <class S>
+void matmul(int bs, const S* A, const S* B, S* C)
+{
+    for (int i = 0; i < bs; ++i)
+        for (int j = 0; j < bs; ++j) {
+            S s = A[i * bs] * B[j];
+            for (int k = 1; k < bs; ++k) s += A[i * bs + k] * B[k * bs + j];
+            C[i * bs + j] = s;
+        }
+}
+/* y += sign * A x  (in the order y_i = y_i + sign * sum_k A_ik x_k) */
+template <class S>
+void matvecAcc(int bs, const S* A, const S* x, S* y, S sign)
+{
+    for (int i = 0; i < bs; ++i) {
+        S s = A[i * bs] * x[0];
+        for (int k = 1; k < bs; ++k) s += A[i * bs + k] * x[k];
+        y[i] += sign * s;
+    }
+}
+
+/* Block ILU(0): dune-istl bilu0_decomposition as wrapped by
+ * PreconditionerWrapperILU (istlpreconditionerwrappers.hh:151-190); diagonal
+ * blocks are stored inverted. */
+template <class S>
+int ilu0Factor(int n, int bs, const int32_t* rp, const int32_t* ci, S* v)
+{
+    const int bb = bs * bs;
+    std::vector<S> tmp(bb);
+    for (int i = 0; i < n; ++i) {
+        int ij = rp[i];
+        for (; ij < rp[i + 1] && ci[ij] < i; ++ij) {
+            const int j = ci[ij];
+            /* A_jj (inverted) */
+            int jj = rp[j];
+            while (ci[jj] != j) ++jj;
+            matmul(bs, v + (long)bb * ij, v + (long)bb * jj, tmp.data());
+            std::copy(tmp.begin(), tmp.end(), v + (long)bb * ij);
+            int ik = ij + 1, jk = jj + 1;
+            while (ik < rp[i + 1] && jk < rp[j + 1]) {
+                if (ci[ik] == ci[jk]) {
+                    matmul(bs, v + (long)bb * ij, v + (long)bb * jk, tmp.data());
+                    for (int t = 0; t < bb; ++t) v[(long)bb * ik + t] -= tmp[t];
+                    ++ik; ++jk;
+                } else if (ci[ik] < ci[jk]) ++ik;
+                else ++jk;
+            }
+        }
+        if (ij >= rp[i + 1] || ci[ij] != i) return -1;
+        if (!invertBlock(bs, v + (long)bb * ij)) return 2;
+    }
+    return 0;
+}
+
+/* bilu_backsolve: forward with unit lower, backward with inverted diagonal,
+ * then v *= relax (SeqILU::apply). */
+template <class S>
+void ilu0Apply(int n, int bs, const int32_t* rp, const int32_t* ci, const S* lu, const S* d,
+               S* v, S relax)
+{
+    const int bb = bs * bs;
+    for (int i = 0; i < n; ++i) {
+        S rhs[3];
+        for (int e = 0; e < bs; ++e) rhs[e] = d[bs * i + e];
+        for (int k = rp[i]; k < rp[i + 1] && ci[k] < i; ++k)
+            matvecAcc(bs, lu + (long)bb * k, v + (long)bs * ci[k], rhs, S(-1));
+        for (int e = 0; e < bs; ++e) v[bs * i + e] = rhs[e];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        S rhs[3];
+        for (int e = 0; e < bs; ++e) rhs[e] = v[bs * i + e];
+        int diag = -1;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            if (ci[k] == i) diag = k;
+            else if (ci[k] > i) matvecAcc(bs, lu + (long)bb * k, v + (long)bs * ci[k], rhs, S(-1));
+        }
+        S out[3] = {0, 0, 0};
+        matvecAcc(bs, lu + (long)bb * diag, rhs, out, S(1));
+        for (int e = 0; e < bs; ++e) v[bs * i + e] = out[e];
+    }
+    if (relax != S(1))
+        for (long k = 0; k < (long)n * bs; ++k) v[k] *= relax;
+}
+
+template <class S>
+void spmv(int n, int bs, const int32_t* rp, const int32_t* ci, const S* a, const S* x, S* y)
+{
+    const int bb = bs * bs;
+    for (int i = 0; i < n; ++i) {
+        S acc[3] = {0, 0, 0};
+        for (int k = rp[i]; k < rp[i + 1]; ++k) matvecAcc(bs, a + (long)bb * k, x + (long)bs * ci[k], acc, S(1));
+        for (int e = 0; e < bs; ++e) y[bs * i + e] = acc[e];
+    }
+}
+
+template <class S>
+S dot(const std::vector<S>& x, const std::vector<S>& y)
+{
+    S s = 0;
+    for (size_t i = 0; i < x.size(); ++i) s += x[i] * y[i];
+    return s;
+}
+
+/* CombinedCriterion (combinedcriterion.hh:54-236). */
+template <class S>
+struct Criterion {
+    S tol, absTol, maxErr, res, init, last; bool stag;
+    void updateErrors(const std::vector<S>& change, const std::vector<S>& r)
+    {
+        last = res; res = 0; stag = true;
+        for (size_t i = 0; i < r.size(); ++i) {
+            res = std::max<S>(res, std::abs(r[i]));
+            if (stag && change[i] != S(0)) stag = false;
+        }
+    }
+    void setInitial(const std::vector<S>& x, const std::vector<S>& r)
+    {
+        updateErrors(x, r);
+        stag = false;
+        res = std::max<S>(res, std::numeric_limits<S>::min() * S(1e10));
+        init = res; last = res;
+    }
+    bool converged() const { return res / std::max<S>(S(1e-20), init) <= tol || res <= absTol; }
+    bool failed() const { return !converged() && (stag || res > maxErr); }
+};
+
+struct Precond {
+    int kind, bs, n; double relax; int order;
+    const int32_t *rp, *ci;
+};
+
+/* BiCGStabSolver::apply (bicgstabsolver.hh:120-330) with the aliasing of the
+ * reference (h == x, s == r, t == y).  Returns 0 converged, 1 not converged,
+ * 2 breakdown (NumericalProblem). */
+template <class S>
+int bicgstab(int n, int bs, const int32_t* rp, const int32_t* ci, const S* A, const S* M,
+             const std::vector<S>& b, std::vector<S>& x, const Precond& pc, S tol, S absTol,
+             S maxErr, int maxIter, int* itersOut)
+{
+    const long m = (long)n * bs;
+    const S breakdownEps = std::numeric_limits<S>::min() * S(1e10);
+    auto precApply = [&](std::vector<S>& out, const std::vector<S>& in) {
+        if (pc.kind == 0) ilu0Apply<S>(n, bs, rp, ci, M, in.data(), out.data(), (S)pc.relax);
+        else {
+            /* block Jacobi: out = relax * D^{-1} in (M holds the inverted diagonals) */
+            const int bb = bs * bs;
+            for (int i = 0; i < n; ++i) {
+                S o[3] = {0, 0, 0};
+                matvecAcc(bs, M + (long)bb * i, in.data() + (long)bs * i, o, S(1));
+                for (int e = 0; e < bs; ++e) out[bs * i + e] = (S)pc.relax * o[e];
+            }
+        }
+    };
+    x.assign(m, S(0));
+    std::vector<S> r = b;
+    Criterion<S> crit{tol, absTol, maxErr, 0, 0, 0, false};
+    crit.setInitial(x, r);
+    int it = 0;
+    *itersOut = 0;
+    if (crit.converged()) return 0;
+    const std::vector<S>& r0hat = b;
+    S rho = 1, alpha = 1, omega = 1;
+    std::vector<S> v(m, S(0)), p(m, S(0)), y(m, S(0)), z(m, S(0));
+    for (; it < maxIter; ++it) {
+        *itersOut = it;
+        S rho_i = dot(r0hat, r);
+        if (std::abs(rho) <= breakdownEps || std::abs(omega) <= breakdownEps) return 2;
+        S beta = (rho_i / rho) * (alpha / omega);
+        rho = rho_i;
+        for (long i = 0; i < m; ++i) {
+            S tmp = v[i];
+            tmp *= omega;
+            tmp -= p[i];
+            tmp *= -beta;
+            p[i] = r[i];
+            p[i] += tmp;
+        }
+        precApply(y, p);
+        spmv<S>(n, bs, rp, ci, A, y.data(), v.data());
+        S denom = dot(r0hat, v);
+        if (std::abs(denom) <= breakdownEps) return 2;
+        alpha = rho_i / denom;
+        if (std::abs(alpha) <= breakdownEps) return 2;
+        for (long i = 0; i < m; ++i) {
+            S tmp = y[i];
+            tmp *= alpha;
+            tmp += x[i];
+            x[i] = tmp;
+            tmp = v[i];
+            tmp *= alpha;
+            r[i] -= tmp;
+        }
+        crit.updateErrors(y, r);
+        if (crit.converged()) { *itersOut = it; return 0; }
+        if (crit.failed()) { *itersOut = it; return 1; }
+        precApply(z, r);
+        spmv<S>(n, bs, rp, ci, A, z.data(), y.data());
+        denom = dot(y, y);
+        if (std::abs(denom) <= breakdownEps) return 2;
+        omega = dot(y, r) / denom;
+        if (std::abs(omega) <= breakdownEps) return 2;
+        for (long i = 0; i < m; ++i) x[i] += omega * z[i];
+        crit.updateErrors(z, r);
+        if (crit.converged()) { *itersOut = it; return 0; }
+        if (crit.failed()) { *itersOut = it; return 1; }
+        for (long i = 0; i < m; ++i) r[i] += -omega * y[i];
+    }
+    *itersOut = it;
+    return 1;
+}
+
+/* ParallelBaseBackend::solve (parallelbasebackend.hh:252-284) +
+ * ParallelBiCGStabSolverBackend::prepareSolver_ (parallelbicgstabbackend.hh:131-160):
+ * copy (and convert to the solver scalar, overlappingbcrsmatrix.hh:220-262),
+ * prepare the preconditioner, run, copy back. */
+template <class S>
+int backendSolve(int n, int bs, const int32_t* rp, const int32_t* ci, const double* J,
+                 const double* b, double* xOut, const ob_params* prm, int* iters)
+{
+    const int bb = bs * bs;
+    const long nnzb = rp[n];
+    std::vector<S> A(J, J + (long)bb * nnzb);
+    std::vector<S> M;
+    if (prm->precond == 0) {
+        M = A;
+        int rc = ilu0Factor<S>(n, bs, rp, ci, M.data());
+        if (rc != 0) return 2;
+    } else {
+        M.resize((long)bb * n);
+        for (int i = 0; i < n; ++i) {
+            int k = rp[i];
+            while (ci[k] != i) ++k;
+            std::copy(A.begin() + (long)bb * k, A.begin() + (long)bb * (k + 1), M.begin() + (long)bb * i);
+            if (!invertBlock(bs, M.data() + (long)bb * i)) return 2;
+        }
+    }
+    std::vector<S> bv(b, b + (long)n * bs), x;
+    Precond pc{prm->precond, bs, n, prm->prec_relax, prm->prec_order, rp, ci};
+    double absTol = prm->lin_abs_tol < 0 ? prm->newton_tol / 100.0 : prm->lin_abs_tol;
+    int rc = bicgstab<S>(n, bs, rp, ci, A.data(), M.data(), bv, x, pc, (S)prm->lin_tol, (S)absTol,
+                         (S)prm->lin_max_error, prm->lin_max_iter, iters);
+    for (long k = 0; k < (long)n * bs; ++k) xOut[k] = (double)x[k];
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Newton update: NewtonMethod::update_ (newtonmethod.hh:665-710) ->
+ * BlackOilNewtonMethod::updatePrimaryVariables_ (blackoilnewtonmethod.hh:226-427)
+ * -> BlackOilPrimaryVariables::adaptPrimaryVariables (blackoilprimaryvariables.hh:549-909). */
+bool adaptPrimary(const Pvt& pvt, double* x, uint8_t& mean, double eps)
+{
+    const double swMaximum = 1.0, waterOnlyThreshold = 1.0;
+    double sw = x[0];
+    double sg = (mean == MEAN_SG) ? x[2] : 0.0;
+    if (sw >= waterOnlyThreshold) {
+        x[0] = std::min(swMaximum, sw);
+        x[2] = 0.0;
+        bool changed = mean != MEAN_SG;
+        mean = MEAN_SG;
+        return changed;
+    }
+    bool changed = false;
+    if (mean == MEAN_SG) {
+        double s = 1.0 - sw;
+        if (sg < -eps && s > 0.0) {
+            const double po = x[1];
+            mean = MEAN_RS;
+            double rsSat = pvt.rsSat.eval(po);
+            const double rsMax = std::numeric_limits<double>::max() / 2;
+            x[2] = std::min(rsMax, rsSat);
+            changed = true;
+        }
+    } else {
+        const double po = x[1];
+        double rsSat = pvt.rsSat.eval(po);
+        const double rsMax = std::numeric_limits<double>::max() / 2;
+        double rs = x[2];
+        if (rs > std::min(rsMax, rsSat * (1.0 + eps))) {
+            mean = MEAN_SG;
+            x[2] = 0.0;
+            changed = true;
+        }
+    }
+    return changed;
+}
+
+void updatePrimary(const Pvt& pvt, const double* cur, uint8_t curMean, const double* upd,
+                   double* next, uint8_t& nextMean, uint8_t& wasSw, const ob_params* prm)
+{
+    double dSw = 0.0, dSo = 0.0, dSg = 0.0;
+    dSw = upd[0];
+    dSo -= dSw;
+    if (curMean == MEAN_SG) { dSg = upd[2]; dSo -= dSg; }
+    double maxSatDelta = std::max(std::abs(dSg), std::abs(dSo));
+    maxSatDelta = std::max(maxSatDelta, std::abs(dSw));
+    double satAlpha = 1.0;
+    if (maxSatDelta > prm->ds_max) satAlpha = prm->ds_max / maxSatDelta;
+    for (int pvi = 0; pvi < 3; ++pvi) {
+        double delta = upd[pvi];
+        if (pvi == 1) {
+            if (std::abs(delta) > prm->dp_max_rel * cur[pvi]) {
+                double sgn = (delta > 0) - (delta < 0);
+                delta = sgn * prm->dp_max_rel * cur[pvi];
+            }
+        } else if (pvi == 0) {
+            delta *= satAlpha;
+        } else {
+            if (curMean == MEAN_SG) delta *= satAlpha;
+            else if (delta > cur[2]) delta = cur[2];
+        }
+        next[pvi] = cur[pvi] - delta;
+        if (pvi == 1) next[pvi] = std::clamp(next[pvi], -1e99, 1e99);
+    }
+    nextMean = curMean;
+    const double eps = wasSw ? prm->privar_osc_thresh : 0.0;
+    wasSw = adaptPrimary(pvt, next, nextMean, eps) ? 1 : 0;
+}
+
+} // namespace
+
+/* ====================================================================== */
+extern "C" {
+
+void ob_default_params(ob_params* p)
+{
+    p->newton_tol = 1e-6; p->newton_max_iter = 20; p->newton_target_iter = 10;
+    p->newton_max_error = 1e100; p->lin_tol = 1e-3; p->lin_abs_tol = -1.0;
+    p->lin_max_error = 1e7; p->lin_max_iter = 1000; p->precond = 0; p->prec_relax = 1.0;
+    p->prec_order = 0; p->max_dt_divisions = 10; p->dp_max_rel = 0.3; p->ds_max = 0.2;
+    p->privar_osc_thresh = 1e-5; p->max_dt = std::numeric_limits<double>::infinity();
+    p->min_dt = 0.0; p->solver_float = 0; p->verbose = 0;
+}
+
+int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2)
+{
+    try {
+        Grid G(p);
+        long f2 = 0;
+        for (long I = 0; I < G.N; ++I)
+            for (int d = 0; d < 6; ++d) if (G.nb(I, d) >= 0) ++f2;
+        *nfaces2 = f2; *nnzb = G.N + f2;
+        return 0;
+    } catch (...) { return -1; }
+}
+
+int ob_build_pattern(const ob_problem* p, int32_t* row_ptr, int32_t* col_idx, int32_t* diag_off,
+                     int32_t* nb_ptr, int32_t* nb_cell, int32_t* nb_off, double* nb_trans,
+                     double* nb_area, double* nb_dzg, int32_t* nb_dir)
+{
+    try {
+        Grid G(p);
+        Pattern P;
+        buildPattern(G, P);
+        std::copy(P.rowPtr.begin(), P.rowPtr.end(), row_ptr);
+        std::copy(P.colIdx.begin(), P.colIdx.end(), col_idx);
+        std::copy(P.diagOff.begin(), P.diagOff.end(), diag_off);
+        std::copy(P.nbPtr.begin(), P.nbPtr.end(), nb_ptr);
+        for (size_t k = 0; k < P.nb.size(); ++k) {
+            nb_cell[k] = (int32_t)P.nb[k].J; nb_off[k] = P.nb[k].off;
+            nb_trans[k] = P.nb[k].trans; nb_area[k] = P.nb[k].area;
+            nb_dzg[k] = P.nb[k].dZg; nb_dir[k] = P.nb[k].dir;
+        }
+        return 0;
+    } catch (...) { return -1; }
+}
+
+int ob_storage(const ob_problem* p, const double* pv, const uint8_t* meaning, double* storage)
+{
+    Grid G(p);
+    Pvt pvt; pvt.init(p);
+    for (long I = 0; I < G.N; ++I) {
+        IQ<double> q;
+        computeIQ<double>(pvt, pv + 3 * I, meaning[I], p->poro[I], q);
+        storageOf<double>(pvt, q, storage + 3 * I);
+    }
+    return 0;
+}
+
+int ob_intensive(const ob_problem* p, const double* pv3, int meaning, int cell, double* out)
+{
+    Pvt pvt; pvt.init(p);
+    IQ<E3> q;
+    computeIQ<E3>(pvt, pv3, meaning, p->poro[cell], q);
+    const E3* f[13] = {&q.p[0], &q.rho[W], &q.rho[O], &q.rho[G], &q.lam[W], &q.lam[O], &q.lam[G],
+                       &q.invB[W], &q.invB[O], &q.invB[G], &q.Rs, &q.phi, &q.S[O]};
+    for (int k = 0; k < 13; ++k) {
+        out[4 * k] = f[k]->v;
+        for (int i = 0; i < 3; ++i) out[4 * k + 1 + i] = f[k]->d[i];
+    }
+    return 0;
+}
+
+int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
+                 const double* storage1, double dt, double* residual, double* jac, int nthreads)
+{
+    try {
+        Problem pr(p);
+        linearize(pr, pv, meaning, storage1, dt, residual, jac, nthreads);
+        return 0;
+    } catch (...) { return -1; }
+}
+
+int ob_ilu0_factor(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, double* vals)
+{
+    return ilu0Factor<double>(n, bs, row_ptr, col_idx, vals);
+}
+
+int ob_ilu0_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
+                  const double* lu, const double* d, double* v, double relax)
+{
+    ilu0Apply<double>(n, bs, row_ptr, col_idx, lu, d, v, relax);
+    return 0;
+}
+
+int ob_spmv(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+            const double* x, double* y)
+{
+    spmv<double>(n, bs, row_ptr, col_idx, vals, x, y);
+    return 0;
+}
+
+int ob_bicgstab(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                const double* b, double* x, const ob_params* prm, int32_t* iterations)
+{
+    int it = 0, rc;
+    if (prm->solver_float) rc = backendSolve<float>(n, bs, row_ptr, col_idx, vals, b, x, prm, &it);
+    else rc = backendSolve<double>(n, bs, row_ptr, col_idx, vals, b, x, prm, &it);
+    *iterations = it;
+    return rc;
+}
+
+int ob_update_primary(const ob_problem* p, const double* cur, const uint8_t* cur_meaning,
+                      const double* delta, double* next, uint8_t* next_meaning,
+                      uint8_t* was_switched, const ob_params* prm, int32_t* num_switched)
+{
+    Pvt pvt; pvt.init(p);
+    const long N = (long)p->nx * p->ny * p->nz;
+    for (long k = 0; k < 3 * N; ++k)
+        if (!std::isfinite(delta[k])) return 2;
+    int sw = 0;
+    for (long I = 0; I < N; ++I) {
+        updatePrimary(pvt, cur + 3 * I, cur_meaning[I], delta + 3 * I, next + 3 * I,
+                      next_meaning[I], was_switched[I], prm);
+        sw += was_switched[I];
+    }
+    *num_switched = sw;
+    return 0;
+}
+
+/* Simulator::run (simulator.hh:697-810) -> FvBaseProblem::timeIntegration
+ * (fvbaseproblem.hh:512-557) -> NewtonMethod::apply (newtonmethod.hh:228-438),
+ * dt control by suggestTimeStepSize (newtonmethod.hh:448-465). */
+int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps, double dt0,
+                const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
+                int log_cap, int32_t* log_len, double* timers, int nthreads)
+{
+    using clk = std::chrono::steady_clock;
+    try {
+        Problem pr(p);
+        const long N = pr.G.N;
+        const int32_t* rp = pr.pat.rowPtr.data();
+        const int32_t* ci = pr.pat.colIdx.data();
+        const long nnzb = rp[N];
+        std::vector<double> sol1(pv, pv + 3 * N), st1(3 * N), res(3 * N), jac(9 * nnzb), upd(3 * N),
+            cur(3 * N);
+        std::vector<uint8_t> mean1(meaning, meaning + N), curMean(N), wasSw(N, 0);
+        double tLin = 0, tSol = 0, tUpd = 0;
+        auto t0 = clk::now();
+        int nlog = 0;
+        double dt = dt0;
+        for (int step = 0; step < n_steps; ++step) {
+            bool stepOk = false;
+            int numIter = 0;
+            for (int attempt = 0; attempt < prm->max_dt_divisions; ++attempt) {
+                /* NewtonMethod::apply */
+                numIter = 0;
+                double error = 1e100, lastError = 1e100;
+                bool failed = false;
+                auto converged = [&]() { return error <= prm->newton_tol; };
+                auto proceed = [&]() {
+                    if (numIter < 1) return true;
+                    if (converged()) return false;
+                    if (numIter >= prm->newton_max_iter) return error * 4.0 < lastError;
+                    return true;
+                };
+                while (proceed()) {
+                    std::copy(pv, pv + 3 * N, cur.begin());
+                    std::copy(meaning, meaning + N, curMean.begin());
+                    auto a = clk::now();
+                    if (numIter == 0) computeStorageAll(pr, cur.data(), curMean.data(), st1.data());
+                    linearize(pr, cur.data(), curMean.data(), st1.data(), dt, res.data(), jac.data(), nthreads);
+                    auto b = clk::now();
+                    tLin += std::chrono::duration<double>(b - a).count();
+                    /* preSolve_ */
+                    lastError = error;
+                    error = 0;
+                    for (long k = 0; k < 3 * N; ++k) error = std::max(std::abs(res[k]), error);
+                    if (error > prm->newton_max_error) { failed = true; break; }
+                    if (!proceed()) {
+                        if (nlog < log_cap && log) log[nlog++] = {step, attempt, numIter, 0, 0, 1, error, dt};
+                        ++numIter;
+                        break;
+                    }
+                    auto c = clk::now();
+                    int it = 0;
+                    int rc;
+                    if (prm->solver_float) rc = backendSolve<float>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it);
+                    else rc = backendSolve<double>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it);
+                    auto d = clk::now();
+                    tSol += std::chrono::duration<double>(d - c).count();
+                    if (rc != 0) {
+                        if (nlog < log_cap && log) log[nlog++] = {step, attempt, numIter, it, 0, 0, error, dt};
+                        failed = true;
+                        break;
+                    }
+                    /* update_ */
+                    bool finite = true;
+                    for (long k = 0; k < 3 * N; ++k) if (!std::isfinite(upd[k])) { finite = false; break; }
+                    if (!finite) { failed = true; break; }
+                    int nsw = 0;
+                    for (long I = 0; I < N; ++I) {
+                        updatePrimary(pr.pvt, cur.data() + 3 * I, curMean[I], upd.data() + 3 * I,
+                                      pv + 3 * I, meaning[I], wasSw[I], prm);
+                        nsw += wasSw[I];
+                    }
+                    tUpd += std::chrono::duration<double>(clk::now() - d).count();
+                    if (nlog < log_cap && log) log[nlog++] = {step, attempt, numIter, it, nsw, 1, error, dt};
+                    if (prm->verbose)
+                        printf("step %d attempt %d iter %d error %.6e lin %d switched %d\n", step, attempt,
+                               numIter, error, it, nsw);
+                    ++numIter;
+                }
+                if (!failed && converged()) { stepOk = true; break; }
+                /* failed_: numIterations = 2 * target; updateFailed: restore */
+                numIter = 2 * prm->newton_target_iter;
+                std::copy(sol1.begin(), sol1.end(), pv);
+                std::copy(mean1.begin(), mean1.end(), meaning);
+                double ndt = dt / 2.0;
+                if (dt < prm->min_dt * (1 + 1e-9)) break;
+                if (ndt < prm->min_dt) ndt = prm->min_dt;
+                dt = ndt;
+            }
+            if (!stepOk) { if (log_len) *log_len = nlog; return 3; }
+            if (dt_hist) dt_hist[step] = dt;
+            /* advanceTimeLevel */
+            std::copy(pv, pv + 3 * N, sol1.begin());
+            std::copy(meaning, meaning + N, mean1.begin());
+            if (pv_hist) std::copy(pv, pv + 3 * N, pv_hist + (long)step * 3 * N);
+            /* suggestTimeStepSize */
+            const int tgt = prm->newton_target_iter;
+            double nd;
+            if (numIter > tgt) {
+                double percent = double(numIter - tgt) / tgt;
+                nd = std::max(prm->min_dt, dt / (1.0 + percent));
+            } else {
+                double percent = double(tgt - numIter) / tgt;
+                nd = std::max(prm->min_dt, dt * (1.0 + percent / 1.2));
+            }
+            dt = std::min(prm->max_dt, nd);
+        }
+        if (log_len) *log_len = nlog;
+        if (timers) {
+            timers[0] = tLin; timers[1] = tSol; timers[2] = tUpd;
+            timers[3] = std::chrono::duration<double>(clk::now() - t0).count();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        fprintf(stderr, "ob_simulate: %s\n", e.what());
+        return -1;
+    }
+}
+
+} // extern "C"

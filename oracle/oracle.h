@@ -1,0 +1,129 @@
+/*
+ * oracle.h -- CPU restatement of the opm-models fully-implicit TPFA hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product library.  It is linked or loaded only by tests/, by
+ * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, and only
+ * as the checker or as the timed CPU baseline -- never as a product path.
+ *
+ * Every function states the reference file:line it restates (paths relative
+ * to the opm-models source tree).  The reference itself cannot be compiled
+ * here (dune-*, opm-common, opm-simulators are absent), so this restatement
+ * is pinned by closed forms, invariants, finite differences and brute force
+ * in tests/test_oracle_*.py.
+ */
+#ifndef OMODELS_ORACLE_H
+#define OMODELS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Structured Cartesian black-oil problem.  Cell index I = i + nx*(j + ny*k),
+ * x fastest (YaspGrid lexicographic order).  z points up; gravity is the z
+ * component of the gravity vector (-9.81 for the usual downward gravity,
+ * opm/models/common/multiphasebaseproblem.hh:398-403).
+ * Same memory layout as omodels_blackoil_problem in include/omodels.h. */
+typedef struct ob_problem {
+    int32_t nx, ny, nz, n_src;
+    double dx, dy, dz;
+    double gravity;
+    const double* perm;      /* [3][N] kx, ky, kz in m^2 (SoA) */
+    const double* poro;      /* [N] reference porosity */
+    int32_t n_oil, n_gas;
+    const double* oil_tab;   /* [4][n_oil]: p, Rs_sat, 1/Bo_sat, mu_o_sat */
+    const double* gas_tab;   /* [3][n_gas]: p, 1/Bg, mu_g */
+    double oil_comp;         /* undersaturated oil compressibility [1/Pa] */
+    double water_pref;       /* water reference pressure [Pa] (Bw(pref) = 1) */
+    double water_comp;       /* water compressibility [1/Pa] */
+    double water_mu;         /* water viscosity [Pa s] */
+    double rho_ref[3];       /* surface densities: water, oil, gas [kg/m^3] */
+    const int32_t* src_cell; /* [n_src] */
+    const int32_t* src_type; /* 0: mass-rate water injector, 1: BHP producer */
+    const double* src_a;     /* injector: water mass rate [kg/s]; producer: WI [m^3] */
+    const double* src_b;     /* producer: bottom-hole pressure [Pa] */
+} ob_problem;
+
+/* Newton / linear-solver parameters; defaults are the reference's
+ * (newtonmethodparameters.hh:30-60, blackoilnewtonmethodparameters.hh:33-63,
+ * linalgparameters.hh:35-78, fvbaseparameters.hh:112). */
+typedef struct ob_params {
+    double newton_tol;          /* NewtonTolerance */
+    int32_t newton_max_iter;    /* NewtonMaxIterations (20) */
+    int32_t newton_target_iter; /* NewtonTargetIterations (10) */
+    double newton_max_error;    /* NewtonMaxError (1e100) */
+    double lin_tol;             /* LinearSolverTolerance (1e-3) */
+    double lin_abs_tol;         /* LinearSolverAbsTolerance (<0: newton_tol/100) */
+    double lin_max_error;       /* LinearSolverMaxError (1e7) */
+    int32_t lin_max_iter;       /* LinearSolverMaxIterations (1000) */
+    int32_t precond;            /* 0: block ILU0, 1: block Jacobi */
+    double prec_relax;          /* PreconditionerRelaxation (1.0) */
+    int32_t prec_order;         /* PreconditionerOrder: Jacobi sweeps (0 in the reference) */
+    int32_t max_dt_divisions;   /* MaxTimeStepDivisions (10) */
+    double dp_max_rel;          /* DpMaxRel (0.3) */
+    double ds_max;              /* DsMax (0.2) */
+    double privar_osc_thresh;   /* PriVarOscilationThreshold (1e-5) */
+    double max_dt;              /* MaxTimeStepSize */
+    double min_dt;              /* MinTimeStepSize (0) */
+    int32_t solver_float;       /* 1: linear solve in float (LinearSolverScalar=float) */
+    int32_t verbose;
+} ob_params;
+
+/* Per-Newton-iteration record, the tuple the reference prints at
+ * newtonmethod.hh:778-782 plus the linear iteration count. */
+typedef struct ob_iter_log {
+    int32_t step, attempt, iter, lin_iters;
+    int32_t num_switched, lin_converged;
+    double error, dt;
+} ob_iter_log;
+
+/* ---- A1: pattern, neighbour info, block addresses ---- */
+int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2);
+int ob_build_pattern(const ob_problem* p, int32_t* row_ptr, int32_t* col_idx,
+                     int32_t* diag_off, int32_t* nb_ptr, int32_t* nb_cell,
+                     int32_t* nb_off, double* nb_trans, double* nb_area,
+                     double* nb_dzg, int32_t* nb_dir);
+
+/* ---- A2-A7: linearizeDomain ---- */
+int ob_storage(const ob_problem* p, const double* pv, const uint8_t* meaning,
+               double* storage /* [N][3] */);
+int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
+                 const double* storage1, double dt, double* residual /* [N][3] */,
+                 double* jac /* [NNZB][3][3] */, int nthreads);
+/* Intensive quantities of one cell (values and derivatives); used by tests
+ * to pin the PVT and the AD propagation. out: [13][4] = (p, rho_w, rho_o,
+ * rho_g, lam_w, lam_o, lam_g, invB_w, invB_o, invB_g, Rs, phi, So) x (v,d0,d1,d2) */
+int ob_intensive(const ob_problem* p, const double* pv3, int meaning, int cell,
+                 double* out);
+
+/* ---- A10-A14: BCRS linear algebra ---- */
+int ob_ilu0_factor(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
+                   double* vals);
+int ob_ilu0_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
+                  const double* lu, const double* d, double* v, double relax);
+int ob_spmv(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
+            const double* vals, const double* x, double* y);
+int ob_bicgstab(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
+                const double* vals, const double* b, double* x,
+                const ob_params* prm, int32_t* iterations);
+
+/* ---- A15-A16: Newton update of one iterate ---- */
+int ob_update_primary(const ob_problem* p, const double* cur, const uint8_t* cur_meaning,
+                      const double* delta, double* next, uint8_t* next_meaning,
+                      uint8_t* was_switched, const ob_params* prm, int32_t* num_switched);
+
+/* ---- A15-A18: full run (time loop + Newton + backend) ---- */
+int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps,
+                double dt0, const ob_params* prm, double* pv_hist /* [n_steps][N][3] or NULL */,
+                double* dt_hist /* [n_steps] */, ob_iter_log* log, int log_cap,
+                int32_t* log_len, double* timers /* [4] linearize, solve, update, total */,
+                int nthreads);
+
+void ob_default_params(ob_params* prm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,192 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the parity checker and CPU baseline.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class ObProblem(C.Structure):
+    """Same layout as ob_problem (oracle.h) and omodels_blackoil_problem."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_src", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("gravity", C.c_double),
+        ("perm", _dp), ("poro", _dp),
+        ("n_oil", C.c_int32), ("n_gas", C.c_int32),
+        ("oil_tab", _dp), ("gas_tab", _dp),
+        ("oil_comp", C.c_double), ("water_pref", C.c_double), ("water_comp", C.c_double),
+        ("water_mu", C.c_double), ("rho_ref", C.c_double * 3),
+        ("src_cell", _ip), ("src_type", _ip), ("src_a", _dp), ("src_b", _dp),
+    ]
+
+
+class ObParams(C.Structure):
+    _fields_ = [
+        ("newton_tol", C.c_double), ("newton_max_iter", C.c_int32), ("newton_target_iter", C.c_int32),
+        ("newton_max_error", C.c_double), ("lin_tol", C.c_double), ("lin_abs_tol", C.c_double),
+        ("lin_max_error", C.c_double), ("lin_max_iter", C.c_int32), ("precond", C.c_int32),
+        ("prec_relax", C.c_double), ("prec_order", C.c_int32), ("max_dt_divisions", C.c_int32),
+        ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
+        ("max_dt", C.c_double), ("min_dt", C.c_double), ("solver_float", C.c_int32), ("verbose", C.c_int32),
+    ]
+
+
+class ObIterLog(C.Structure):
+    _fields_ = [("step", C.c_int32), ("attempt", C.c_int32), ("iter", C.c_int32), ("lin_iters", C.c_int32),
+                ("num_switched", C.c_int32), ("lin_converged", C.c_int32), ("error", C.c_double),
+                ("dt", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        L.ob_default_params.argtypes = [C.POINTER(ObParams)]
+        L.ob_pattern_size.argtypes = [C.POINTER(ObProblem), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.ob_build_pattern.argtypes = [C.POINTER(ObProblem), _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip]
+        L.ob_storage.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp]
+        L.ob_linearize.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, _dp, _dp, C.c_int]
+        L.ob_intensive.argtypes = [C.POINTER(ObProblem), _dp, C.c_int, C.c_int, _dp]
+        L.ob_ilu0_factor.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp]
+        L.ob_ilu0_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
+        L.ob_spmv.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.ob_bicgstab.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
+        L.ob_update_primary.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp, _u8p, _u8p,
+                                        C.POINTER(ObParams), _ip]
+        L.ob_simulate.argtypes = [C.POINTER(ObProblem), _dp, _u8p, C.c_int, C.c_double, C.POINTER(ObParams),
+                                  _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def P(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def params(**kw):
+    p = ObParams()
+    lib().ob_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def problem_from_case(case):
+    """ob_problem view of a BlackOilCase built by the product's generator."""
+    return case.descriptor(ObProblem)
+
+
+def pattern(prob):
+    nz = C.c_int64()
+    f2 = C.c_int64()
+    lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
+    N = prob.nx * prob.ny * prob.nz
+    rp = np.zeros(N + 1, np.int32)
+    ci = np.zeros(nz.value, np.int32)
+    dg = np.zeros(N, np.int32)
+    nbp = np.zeros(N + 1, np.int32)
+    nbc = np.zeros(f2.value, np.int32)
+    nbo = np.zeros(f2.value, np.int32)
+    nbt = np.zeros(f2.value)
+    nba = np.zeros(f2.value)
+    nbz = np.zeros(f2.value)
+    nbd = np.zeros(f2.value, np.int32)
+    rc = lib().ob_build_pattern(C.byref(prob), P(rp, _ip), P(ci, _ip), P(dg, _ip), P(nbp, _ip), P(nbc, _ip),
+                                P(nbo, _ip), P(nbt), P(nba), P(nbz), P(nbd, _ip))
+    assert rc == 0
+    return dict(row_ptr=rp, col_idx=ci, diag_off=dg, nb_ptr=nbp, nb_cell=nbc, nb_off=nbo, nb_trans=nbt,
+                nb_area=nba, nb_dzg=nbz, nb_dir=nbd)
+
+
+def storage(prob, pv, mean):
+    N = prob.nx * prob.ny * prob.nz
+    st = np.zeros(3 * N)
+    lib().ob_storage(C.byref(prob), P(pv), P(mean, _u8p), P(st))
+    return st
+
+
+def linearize(prob, pv, mean, st1, dt, nthreads=1):
+    N = prob.nx * prob.ny * prob.nz
+    nz = C.c_int64()
+    f2 = C.c_int64()
+    lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
+    r = np.zeros(3 * N)
+    J = np.zeros(9 * nz.value)
+    rc = lib().ob_linearize(C.byref(prob), P(pv), P(mean, _u8p), P(st1), dt, P(r), P(J), nthreads)
+    assert rc == 0
+    return J, r
+
+
+def intensive(prob, pv3, meaning, cell=0):
+    out = np.zeros(13 * 4)
+    pv3 = np.ascontiguousarray(pv3, dtype=np.float64)
+    lib().ob_intensive(C.byref(prob), P(pv3), int(meaning), int(cell), P(out))
+    names = ["p", "rho_w", "rho_o", "rho_g", "lam_w", "lam_o", "lam_g", "invB_w", "invB_o", "invB_g", "Rs",
+             "phi", "So"]
+    o = out.reshape(13, 4)
+    return {n: o[k] for k, n in enumerate(names)}
+
+
+def ilu0(n, bs, rp, ci, vals):
+    lu = np.array(vals, dtype=np.float64, copy=True)
+    rc = lib().ob_ilu0_factor(n, bs, P(rp, _ip), P(ci, _ip), P(lu))
+    return rc, lu
+
+
+def ilu0_apply(n, bs, rp, ci, lu, d, relax=1.0):
+    v = np.zeros_like(d)
+    lib().ob_ilu0_apply(n, bs, P(rp, _ip), P(ci, _ip), P(lu), P(d), P(v), relax)
+    return v
+
+
+def spmv(n, bs, rp, ci, vals, x):
+    y = np.zeros(n * bs)
+    lib().ob_spmv(n, bs, P(rp, _ip), P(ci, _ip), P(vals), P(x), P(y))
+    return y
+
+
+def bicgstab(n, bs, rp, ci, vals, b, prm):
+    x = np.zeros(n * bs)
+    its = C.c_int32()
+    rc = lib().ob_bicgstab(n, bs, P(rp, _ip), P(ci, _ip), P(vals), P(b), P(x), C.byref(prm), C.byref(its))
+    return rc, x, its.value
+
+
+def update_primary(prob, cur, cur_mean, delta, was_switched, prm):
+    N = prob.nx * prob.ny * prob.nz
+    nxt = np.zeros(3 * N)
+    nm = np.zeros(N, np.uint8)
+    ws = np.array(was_switched, dtype=np.uint8, copy=True)
+    ns = C.c_int32()
+    rc = lib().ob_update_primary(C.byref(prob), P(cur), P(cur_mean, _u8p), P(delta), P(nxt), P(nm, _u8p),
+                                 P(ws, _u8p), C.byref(prm), C.byref(ns))
+    return rc, nxt, nm, ws, ns.value
+
+
+def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep_history=True):
+    N = prob.nx * prob.ny * prob.nz
+    pv = np.array(pv0, dtype=np.float64, copy=True)
+    mean = np.array(mean0, dtype=np.uint8, copy=True)
+    hist = np.zeros((n_steps, 3 * N)) if keep_history else None
+    dts = np.zeros(n_steps)
+    log = (ObIterLog * log_cap)()
+    nlog = C.c_int32()
+    timers = np.zeros(4)
+    rc = lib().ob_simulate(C.byref(prob), P(pv), P(mean, _u8p), n_steps, dt0, C.byref(prm), P(hist), P(dts), log,
+                           log_cap, C.byref(nlog), P(timers), nthreads)
+    recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
+            for l in log[:min(nlog.value, log_cap)]]
+    return rc, pv, mean, hist, dts, recs, timers
