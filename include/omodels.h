@@ -111,6 +111,8 @@ typedef struct omodels_params {
     int32_t precision;          /* 0: FP64 solve; 1: FP32 Jacobian/ILU/inner BiCGStab
                                    inside FP64 flexible outer refinement */
     int32_t verbose;
+    int32_t ilu_schedule;       /* 0: persistent tile-wavefront sweeps; 1: one launch per level */
+    int32_t pad;
 } omodels_params;
 
 void omodels_default_params(omodels_params* prm);
@@ -194,6 +196,14 @@ int omodels_model_get_switched(omodels_model* m, uint8_t* was_switched);
 int omodels_simulate(omodels_model* m, const omodels_params* prm, int n_steps, double dt0,
                      double* pv_hist, double* dt_hist, omodels_iter_log* log, int log_cap,
                      int32_t* log_len, omodels_run_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing with HIP events recorded on the model's stream around each
+ * kernel class: 0 assembly, 1 SpMV, 2 preconditioner apply (ILU0 sweep pair),
+ * 3 ILU0 factorization, 4 other.  The reference's counterpart is its phase
+ * timers (newtonmethod.hh:240-262, timer.hh:48-215). */
+int omodels_model_set_profiling(omodels_model* m, int enable);
+int omodels_model_kernel_times(omodels_model* m, double* ms, int64_t* counts, int n);
 
 #ifdef __cplusplus
 }

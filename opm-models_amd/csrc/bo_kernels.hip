@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
-    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff)
+    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
@@ -88,11 +88,17 @@ __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
     // index h in [0, 2*(kTX+kTY)+4): enumerate the ring of the (kTX+2)x(kTY+2) box
     const int ringLen = 2 * (kTX + 2) + 2 * kTY;
 
+    // this block marches over planes [k0, k1)
+    const int k0 = blockIdx.z * kChunk;
+    const int k1 = min(G.nz, k0 + kChunk);
     IqV below, cur, above;     // own column: k-1, k, k+1 (values)
     bool haveBelow = false;
-    if (own) loadIqV(f, pv, mean, (long)i + (long)G.nx * j, cur);
+    if (own) {
+        loadIqV(f, pv, mean, (long)i + (long)G.nx * j + nxy * k0, cur);
+        if (k0 > 0) { loadIqV(f, pv, mean, (long)i + (long)G.nx * j + nxy * (k0 - 1), below); haveBelow = true; }
+    }
 
-    for (int k = 0; k < G.nz; ++k) {
+    for (int k = k0; k < k1; ++k) {
         const long I = (long)i + (long)G.nx * j + nxy * k;
         // stage plane k: interior from registers, ring computed here
         if (own) toLds(cur, plane[ty + 1][tx + 1]);
@@ -305,9 +311,14 @@ void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s)
 {
-    dim3 grid((G.nx + kTX - 1) / kTX, (G.ny + kTY - 1) / kTY);
-    hipLaunchKernelGGL(k_bo_assemble, grid, dim3(kTX * kTY), 0, s, f, G, pv, mean, poro, trans, st1,
-                       storefac, res, jac, diagOff, nbOff);
+    const int gx = (G.nx + kTX - 1) / kTX, gy = (G.ny + kTY - 1) / kTY;
+    // enough workgroups to fill the chip: split the z march into chunks
+    int gz = (2048 + gx * gy - 1) / (gx * gy);
+    gz = gz < 1 ? 1 : (gz > G.nz ? G.nz : gz);
+    const int kChunk = (G.nz + gz - 1) / gz;
+    gz = (G.nz + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_bo_assemble, dim3(gx, gy, gz), dim3(kTX * kTY), 0, s, f, G, pv, mean, poro, trans, st1,
+                       storefac, res, jac, diagOff, nbOff, kChunk);
 }
 
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,

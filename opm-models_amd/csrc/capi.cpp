@@ -341,4 +341,23 @@ int omodels_simulate(omodels_model* h, const omodels_params* prm, int n_steps, d
     });
 }
 
+int omodels_model_set_profiling(omodels_model* h, int enable)
+{
+    return guarded([&] {
+        h->m->timer.reset();
+        h->m->timer.enabled = enable != 0;
+        return 0;
+    });
+}
+
+int omodels_model_kernel_times(omodels_model* h, double* ms, int64_t* counts, int n)
+{
+    return guarded([&] {
+        KernelTimer& t = h->m->timer;
+        t.collect();
+        for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.ms[i]; counts[i] = t.count[i]; }
+        return 0;
+    });
+}
+
 }  // extern "C"

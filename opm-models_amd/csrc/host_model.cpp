@@ -87,8 +87,10 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     if (iterationIndex == 0)
         launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream);
     const double storefac = G.V / dt;
+    timer.begin(kClsAssemble, stream);
     launchBoAssemble(fluid, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
                      nbOff.p, stream);
+    timer.end(stream);
     launchBoSources(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, res.p, jac.p,
                     diagOff.p, stream);
     OM_HIP(hipGetLastError());
@@ -122,6 +124,8 @@ BiCGStabBackend::BiCGStabBackend(Model& m, const omodels_params& prm) : m_(m), p
     ks_.alloc(1);
     fail_.alloc(1);
     OM_HIP(hipHostMalloc((void**)&ksHost_, sizeof(KrylovScalars), hipHostMallocDefault));
+    if (prm_.precond == 0 && prm_.ilu_schedule == 0)
+        tiled_ = std::make_unique<TiledIlu>(m.G.nx, m.G.ny, m.G.nz, m.stream);
     if (prm_.precision == 0) {
         for (auto& w : wd_) w.alloc(n);
         r64_.alloc(n);
@@ -154,9 +158,15 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
         // ILU(0) factorization in FP64 (pivot blocks stay FP64 in both forms)
         lu64_.alloc(9 * nnzb);
         OM_HIP(hipMemcpyAsync(lu64_.p, A64_, 9 * nnzb * sizeof(double), hipMemcpyDeviceToDevice, s));
-        launchIlu0Factor<double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu64_.p, m_.levels, fail_.p, s);
         dinv64_.alloc(9 * (size_t)N);
-        launchExtractInvDiag<double>(N, 3, m_.diagOff.p, lu64_.p, dinv64_.p, s);
+        m_.timer.begin(kClsIluFactor, s);
+        if (tiled_) {
+            tiled_->factor(m_.rowPtr.p, m_.diagOff.p, lu64_.p, dinv64_.p, fail_.p, s);
+        } else {
+            launchIlu0Factor<double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu64_.p, m_.levels, fail_.p, s);
+            launchExtractInvDiag<double>(N, 3, m_.diagOff.p, lu64_.p, dinv64_.p, s);
+        }
+        m_.timer.end(s);
         if (prm_.precision == 1) {
             luf_.alloc(9 * nnzb);
             launchCopyConvertD<float>(9 * nnzb, lu64_.p, luf_.p, s);
@@ -173,6 +183,7 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     int f = 0;
     OM_HIP(hipMemcpyAsync(&f, fail_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     m_.sync();
+    if (tiled_) tiled_->checkError(s);
     prepared_ = (f == 0);
 }
 
@@ -181,10 +192,15 @@ void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v)
 {
     hipStream_t s = m_.stream;
     const int N = (int)m_.N;
-    if (prm_.precond == 0)
-        launchIlu0Apply<S, double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv, d, v,
-                                   m_.levels, s);
-    else {
+    if (prm_.precond == 0) {
+        m_.timer.begin(kClsIluApply, s);
+        if (tiled_)
+            tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s);
+        else
+            launchIlu0Apply<S, double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv, d, v,
+                                       m_.levels, s);
+        m_.timer.end(s);
+    } else {
         (void)lu;
         // block Jacobi with FP64 inverted pivots: v = w D^-1 d
         launchJacobiApplyMixed(N, (const double*)dinv, d, v, prm_.prec_relax, s);
@@ -225,7 +241,9 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, tag, s);
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
         precApply<S>(lu, dinv, p, y);
+        m_.timer.begin(kClsSpmv, s);
         launchSpmv<S>(N, 3, m_.rowPtr.p, m_.colIdx.p, A, y, v, s);
+        m_.timer.end(s);
         launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, tag, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
         OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
@@ -238,7 +256,9 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         if (converged(K.resnorm)) return 0;
         if (K.stag_half || K.resnorm > prm_.lin_max_error) return 1;
         precApply<S>(lu, dinv, r, z);
+        m_.timer.begin(kClsSpmv, s);
         launchSpmv<S>(N, 3, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
+        m_.timer.end(s);
         launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, tag + 1, s);
         launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, tag + 1, s);
     }
@@ -250,6 +270,13 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
 }
 
 int BiCGStabBackend::solve()
+{
+    const int rc = solveImpl();
+    if (tiled_) tiled_->checkError(m_.stream);
+    return rc;
+}
+
+int BiCGStabBackend::solveImpl()
 {
     if (!prepared_) return 2;   // preconditioner could not be created (NumericalProblem)
     hipStream_t s = m_.stream;
@@ -422,6 +449,7 @@ int simulate(Model& m, const omodels_params& prm, int nSteps, double dt0, double
         dt = std::min(prm.max_dt, newton.suggestTimeStepSize(dt));
     }
     m.sync();
+    m.timer.collect();
     if (stats) stats->t_total = std::chrono::duration<double>(clk::now() - t0).count();
     return 0;
 }

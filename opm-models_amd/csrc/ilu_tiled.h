@@ -1,0 +1,37 @@
+// ilu_tiled.h -- persistent tile-wavefront ILU(0) kernels for the 7-point
+// structured pattern (see ilu_tiled.hip).
+#pragma once
+
+#include "omodels_internal.h"
+
+namespace omodels {
+
+struct TileGeom {
+    int nx, ny, nz;
+    int BX, BY, BZ;     // tile shape, BX*BY*BZ <= 256
+    int TX, TY, TZ, T;  // tiles per axis and in total
+};
+
+class TiledIlu {
+public:
+    TiledIlu(int nx, int ny, int nz, hipStream_t s);
+    // in-place factorization of lu (a copy of A); inverted pivots also row-wise in dinv
+    void factor(const int* rp, const int* diag, double* lu, double* dinv, int* fail, hipStream_t s);
+    // v = (LU)^-1 d
+    template <class S, class P>
+    void apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s);
+    // throws if a dependency wait ran into its bound since construction
+    void checkError(hipStream_t s);
+    const TileGeom& geom() const { return g; }
+    int gridSize() const { return grid; }
+
+private:
+    TileGeom g{};
+    DevBuf<int> order, done, err;
+    DevBuf<unsigned long long> counter;
+    long long sweeps = 0;
+    int epoch = 0;
+    int grid = 0, maxResident = 0;
+};
+
+}  // namespace omodels

@@ -11,6 +11,7 @@
 
 #include "../../include/omodels.h"
 #include "omodels_internal.h"
+#include "ilu_tiled.h"
 
 namespace omodels {
 
@@ -54,6 +55,7 @@ public:
     DevBuf<double> partial;      // reduction partials
     DevBuf<double> scal;         // device scalars (norms)
     double* hostScal = nullptr;  // pinned mirror
+    KernelTimer timer;
 
     // TpfaLinearizer
     void linearizeDomain(double dt, int iterationIndex);
@@ -87,6 +89,7 @@ private:
     template <class S> int bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x,
                                     double tol, double absTol, int maxIter, int* its);
     template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v);
+    int solveImpl();
 
     Model& m_;
     omodels_params prm_;
@@ -103,6 +106,7 @@ private:
     DevBuf<KrylovScalars> ks_;
     KrylovScalars* ksHost_ = nullptr;
     DevBuf<int> fail_;
+    std::unique_ptr<TiledIlu> tiled_;
     bool prepared_ = false;
 };
 

@@ -63,6 +63,56 @@ struct DevBuf {
     }
 };
 
+// Optional HIP-event timing of kernel classes on the stream they run on.
+enum KernelClass : int { kClsAssemble = 0, kClsSpmv, kClsIluApply, kClsIluFactor, kClsOther, kNumClasses };
+struct KernelTimer {
+    bool enabled = false;
+    struct Pair { hipEvent_t a, b; int cls; };
+    std::vector<Pair> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[kNumClasses] = {0, 0, 0, 0, 0};
+    long count[kNumClasses] = {0, 0, 0, 0, 0};
+    hipEvent_t cur = nullptr;
+    int curCls = 0;
+    hipEvent_t get()
+    {
+        if (pool.empty()) { hipEvent_t e; OM_HIP(hipEventCreate(&e)); return e; }
+        hipEvent_t e = pool.back(); pool.pop_back(); return e;
+    }
+    void begin(int cls, hipStream_t s)
+    {
+        if (!enabled) return;
+        cur = get(); curCls = cls;
+        OM_HIP(hipEventRecord(cur, s));
+    }
+    void end(hipStream_t s)
+    {
+        if (!enabled || !cur) return;
+        hipEvent_t b = get();
+        OM_HIP(hipEventRecord(b, s));
+        pending.push_back({cur, b, curCls});
+        cur = nullptr;
+        if (pending.size() > 8192) collect();
+    }
+    void collect()
+    {
+        for (auto& p : pending) {
+            OM_HIP(hipEventSynchronize(p.b));
+            float t = 0.f;
+            OM_HIP(hipEventElapsedTime(&t, p.a, p.b));
+            ms[p.cls] += t; count[p.cls] += 1;
+            pool.push_back(p.a); pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() { collect(); for (int i = 0; i < kNumClasses; ++i) { ms[i] = 0; count[i] = 0; } }
+    ~KernelTimer()
+    {
+        for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 // ---- black-oil kernels (bo_kernels.hip) ----
 void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,

@@ -52,6 +52,7 @@ class Params(C.Structure):
         ("prec_relax", C.c_double), ("prec_order", C.c_int32), ("max_dt_divisions", C.c_int32),
         ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
         ("max_dt", C.c_double), ("min_dt", C.c_double), ("precision", C.c_int32), ("verbose", C.c_int32),
+        ("ilu_schedule", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -96,6 +97,8 @@ SIGNATURES = {
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
     "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
+    "omodels_model_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "omodels_model_kernel_times": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64), C.c_int]),
     "omodels_simulate": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_double, _dp, _dp,
                                    C.POINTER(IterLog), C.c_int, _ip, C.POINTER(RunStats)]),
 }
@@ -281,6 +284,18 @@ class Model:
         check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
         check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
         return J, r
+
+    def set_profiling(self, enable=True):
+        check(lib().omodels_model_set_profiling(self.h, int(enable)), "set_profiling")
+
+    KERNEL_CLASSES = ("assemble", "spmv", "ilu_apply", "ilu_factor", "other")
+
+    def kernel_times(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, np.int64)
+        check(lib().omodels_model_kernel_times(self.h, ptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), 5),
+              "kernel_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.KERNEL_CLASSES)}
 
     def newton_update(self, delta, params):
         delta = np.ascontiguousarray(delta, dtype=np.float64)

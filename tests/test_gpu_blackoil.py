@@ -83,20 +83,26 @@ def _system(dims, seed=1):
     return c, J, r, pat
 
 
-@pytest.mark.parametrize("dims", [(5, 1, 1), (6, 5, 1), (7, 5, 4), (20, 16, 6)])
-def test_ilu0_factor_and_apply_parity(dims):
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("dims", [(5, 1, 1), (6, 5, 1), (1, 3, 9), (7, 5, 4), (20, 16, 6), (37, 11, 9)])
+def test_ilu0_factor_and_apply_parity(dims, schedule):
+    """Both schedules (persistent tile wavefront, one launch per level)
+    reproduce the sequential ILU(0) factor and sweeps bit for bit."""
     c, J, r, pat = _system(dims)
     m = om.Model(c)
-    s = om.Solver(m, om.default_params())
+    s = om.Solver(m, om.default_params(ilu_schedule=schedule))
     s.prepare(J, r)
     lu = s.ilu()
     rc, luo = orc.ilu0(c.N, 3, pat["row_ptr"], pat["col_idx"], J)
     assert rc == 0
     assert block_rel_err(lu, luo) <= 1e-12
+    assert np.array_equal(lu.view(np.uint64), luo.view(np.uint64))
     d = np.random.default_rng(0).standard_normal(3 * c.N)
-    v = s.apply_precond(d)
-    vo = orc.ilu0_apply(c.N, 3, pat["row_ptr"], pat["col_idx"], luo, d)
-    assert rel_err(v, vo) <= 1e-12
+    for rep in range(3):
+        v = s.apply_precond(d)
+        vo = orc.ilu0_apply(c.N, 3, pat["row_ptr"], pat["col_idx"], luo, d)
+        assert rel_err(v, vo) <= 1e-12
+        assert np.array_equal(v.view(np.uint64), vo.view(np.uint64))
 
 
 @pytest.mark.parametrize("precond", [0, 1])
