@@ -198,6 +198,32 @@ int omodels_simulate(omodels_model* m, const omodels_params* prm, int n_steps, d
                      int32_t* log_len, omodels_run_stats* stats);
 
 /* ------------------------------------------------------------------------ */
+/* Ensemble engine and request loop.  No counterpart in the reference (which
+ * runs one reservoir per process); each member follows the reference's
+ * Newton/time-step control (newtonmethod.hh:228-465, fvbaseproblem.hh:512-606)
+ * while the kernels of all members run in the same launches.  Requests are
+ * admitted in arrival order when their device memory fits hbm_budget_bytes
+ * and at most max_batch run at once; a request that arrives while others run
+ * joins at the next time-step boundary.  All members share one fluid. */
+typedef struct omodels_server omodels_server;
+omodels_server* omodels_server_create(const omodels_params* prm, double hbm_budget_bytes, int max_batch);
+void omodels_server_destroy(omodels_server* s);
+/* copies the problem; returns the request id (>= 0) */
+int omodels_server_submit(omodels_server* s, const omodels_blackoil_problem* p, const double* pv,
+                          const uint8_t* meaning, int n_steps, double dt0, double arrival_s);
+/* runs until every submitted request is done (arrival times are seconds
+ * after the start of this call) */
+int omodels_server_run(omodels_server* s);
+/* final state, times [arrival, admission, completion] (s), per-step dt and
+ * statistics of one request; returns its status (0 ok, 3 time step failed) */
+int omodels_server_result(omodels_server* s, int id, double* pv, uint8_t* meaning, double* times,
+                          double* dt_hist, omodels_run_stats* stats);
+int omodels_server_log(omodels_server* s, int id, omodels_iter_log* log, int cap, int32_t* len);
+int omodels_server_summary(omodels_server* s, double* wall_s, int64_t* batch_steps, int32_t* max_running);
+int omodels_server_set_profiling(omodels_server* s, int enable);
+int omodels_server_kernel_times(omodels_server* s, double* ms, int64_t* counts, int n);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing with HIP events recorded on the model's stream around each
  * kernel class: 0 assembly, 1 SpMV, 2 preconditioner apply (ILU0 sweep pair),
  * 3 ILU0 factorization, 4 other.  The reference's counterpart is its phase

@@ -70,16 +70,18 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
     for (int t = 0; t < 9; ++t) Jout[t] = blk[t];
 }
 
-__global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
+// One (x, y) tile of columns over the planes of one z chunk.  Shared by the
+// single-reservoir kernel and the ensemble kernel (per-instance pointers).
+__device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
-    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk)
+    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
+    int bx, int by, int bz, IqLds (*plane)[kTX + 2])
 {
-    __shared__ IqLds plane[kTY + 2][kTX + 2];
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-    const int i = blockIdx.x * kTX + tx;
-    const int j = blockIdx.y * kTY + ty;
+    const int i = bx * kTX + tx;
+    const int j = by * kTY + ty;
     const bool own = (i < G.nx) && (j < G.ny);
     const long nxy = (long)G.nx * G.ny;
     const long N = nxy * G.nz;
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
     const int ringLen = 2 * (kTX + 2) + 2 * kTY;
 
     // this block marches over planes [k0, k1)
-    const int k0 = blockIdx.z * kChunk;
+    const int k0 = bz * kChunk;
     const int k1 = min(G.nz, k0 + kChunk);
     IqV below, cur, above;     // own column: k-1, k, k+1 (values)
     bool haveBelow = false;
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
             else if (h < 2 * (kTX + 2)) { li = h - (kTX + 2); lj = kTY + 1; }
             else if (h < 2 * (kTX + 2) + kTY) { li = 0; lj = 1 + (h - 2 * (kTX + 2)); }
             else { li = kTX + 1; lj = 1 + (h - 2 * (kTX + 2) - kTY); }
-            const int gi = blockIdx.x * kTX + li - 1, gj = blockIdx.y * kTY + lj - 1;
+            const int gi = bx * kTX + li - 1, gj = by * kTY + lj - 1;
             if (gi >= 0 && gi < G.nx && gj >= 0 && gj < G.ny) {
                 IqV q;
                 loadIqV(f, pv, mean, (long)gi + (long)G.nx * gj + nxy * k, q);
@@ -186,6 +188,17 @@ __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
         haveBelow = true;
         cur = above;
     }
+}
+
+__global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
+    const double* __restrict__ pv, const uint8_t* __restrict__ mean,
+    const double* __restrict__ poro, const double* __restrict__ trans,
+    const double* __restrict__ st1, double storefac, double* __restrict__ res,
+    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk)
+{
+    __shared__ IqLds plane[kTY + 2][kTX + 2];
+    assembleTile(f, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, blockIdx.x,
+                 blockIdx.y, blockIdx.z, plane);
 }
 
 // Cell-wise well sources (tpfalinearizer.hh:827-841), added after the flux
@@ -341,6 +354,171 @@ void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t
 {
     hipLaunchKernelGGL(k_bo_update, dim3((N + 255) / 256), dim3(256), 0, s, f, N, pv, mean, wasSw,
                        delta, prm, counters);
+}
+
+}  // namespace omodels
+
+// ===========================================================================
+// Ensemble kernels: every block finds its instance through a table, so one
+// launch serves all realizations of a batch (per-instance pointers, grids and
+// time steps in EnsInst).
+#include "ensemble.h"
+
+namespace omodels {
+
+__global__ __launch_bounds__(kTX * kTY) void k_e_assemble(BoFluid f, const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ active)
+{
+    __shared__ IqLds plane[kTY + 2][kTX + 2];
+    const int2 t = table[blockIdx.x];
+    if (!active[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const int bx = t.y % I.gx, by = (t.y / I.gx) % I.gy, bz = t.y / (I.gx * I.gy);
+    assembleTile(f, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac, I.diagOff, I.nbOff,
+                 I.kChunk, bx, by, bz, plane);
+}
+
+__global__ void k_e_sources(BoFluid f, const EnsInst* __restrict__ insts, const int2* __restrict__ table, int n,
+                            const int* __restrict__ active)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int2 t = table[q];
+    if (!active[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = I.srcCell[t.y];
+    IqE qi;
+    iqEval(f, I.pv0[3 * c], I.pv0[3 * c + 1], I.pv0[3 * c + 2], I.mean0[c], qi);
+    E3 s[3];
+    sourceEval(f, qi, I.srcType[t.y], I.srcA[t.y], I.srcB[t.y], I.G.V, s);
+    double* D = I.jac + 9L * I.diagOff[c];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const E3 v = s[e] * (-I.G.V);
+        I.res[3 * c + e] += v.v;
+        D[3 * e + 0] += v.d0;
+        D[3 * e + 1] += v.d1;
+        D[3 * e + 2] += v.d2;
+    }
+}
+
+// storage cache of time level 1 for the instances starting a Newton solve
+__global__ __launch_bounds__(kEnsChunk) void k_e_storage(BoFluid f, const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ refresh)
+{
+    const int2 t = table[blockIdx.x];
+    if (!refresh[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kEnsChunk + threadIdx.x;
+    if (c >= I.N) return;
+    double s[3];
+    storageValues(f, I.pv0[3 * c], I.pv0[3 * c + 1], I.pv0[3 * c + 2], I.mean0[c], I.poro[c], s);
+    I.st1[3 * c] = s[0]; I.st1[3 * c + 1] = s[1]; I.st1[3 * c + 2] = s[2];
+}
+
+// Newton update of the instances that solved (same cell rule as k_bo_update)
+__global__ __launch_bounds__(kEnsChunk) void k_e_update(BoFluid f, const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ active, NewtonUpdateParams prm)
+{
+    const int2 t = table[blockIdx.x];
+    if (!active[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kEnsChunk + threadIdx.x;
+    const bool valid = c < I.N;
+    bool changed = false;
+    if (valid) {
+        const double* delta = I.dv[kVx];
+        double* pv = I.pv0;
+        const double d0 = delta[3 * c], d1 = delta[3 * c + 1], d2 = delta[3 * c + 2];
+        if (!isfinite(d0) || !isfinite(d1) || !isfinite(d2)) atomicOr(&I.counters[1], 1);
+        const double cur[3] = {pv[3 * c], pv[3 * c + 1], pv[3 * c + 2]};
+        const int cm = I.mean0[c];
+        double dSw = 0.0, dSo = 0.0, dSg = 0.0;
+        dSw = d0;
+        dSo -= dSw;
+        if (cm == kMeanSg) { dSg = d2; dSo -= dSg; }
+        double maxSat = fmax(fabs(dSg), fabs(dSo));
+        maxSat = fmax(maxSat, fabs(dSw));
+        double satAlpha = 1.0;
+        if (maxSat > prm.dsMax) satAlpha = prm.dsMax / maxSat;
+        double nx[3];
+        nx[0] = cur[0] - d0 * satAlpha;
+        {
+            double dl = d1;
+            if (fabs(dl) > prm.dpMaxRel * cur[1]) {
+                const double sgn = (double)((dl > 0.0) - (dl < 0.0));
+                dl = sgn * prm.dpMaxRel * cur[1];
+            }
+            double v = cur[1] - dl;
+            v = v < -1e99 ? -1e99 : (v > 1e99 ? 1e99 : v);
+            nx[1] = v;
+        }
+        {
+            double dl = d2;
+            if (cm == kMeanSg) dl *= satAlpha;
+            else if (dl > cur[2]) dl = cur[2];
+            nx[2] = cur[2] - dl;
+        }
+        const double eps = I.wasSw[c] ? prm.oscThresh : 0.0;
+        int m = cm;
+        const double sw = nx[0];
+        const double sg = (m == kMeanSg) ? nx[2] : 0.0;
+        if (sw >= 1.0) {
+            nx[0] = fmin(1.0, sw);
+            nx[2] = 0.0;
+            changed = (m != kMeanSg);
+            m = kMeanSg;
+        } else if (m == kMeanSg) {
+            const double s = 1.0 - sw;
+            if (sg < -eps && s > 0.0) {
+                m = kMeanRs;
+                nx[2] = fmin(1.79769313486231570815e+308 / 2, tabEval(f.rsSat, nx[1]));
+                changed = true;
+            }
+        } else {
+            const double rsSat = tabEval(f.rsSat, nx[1]);
+            if (nx[2] > fmin(1.79769313486231570815e+308 / 2, rsSat * (1.0 + eps))) {
+                m = kMeanSg;
+                nx[2] = 0.0;
+                changed = true;
+            }
+        }
+        pv[3 * c] = nx[0]; pv[3 * c + 1] = nx[1]; pv[3 * c + 2] = nx[2];
+        I.mean0[c] = (uint8_t)m;
+        I.wasSw[c] = changed ? 1 : 0;
+    }
+    const unsigned long long ball = __ballot(changed);
+    if ((threadIdx.x % warpSize) == 0 && ball) atomicAdd(&I.counters[0], __popcll(ball));
+}
+
+void launchEnsAssemble(const BoFluid& f, const EnsInst* insts, const int2* table, int nBlocks, const int* active,
+                       const int2* srcTable, int nSrc, hipStream_t s)
+{
+    if (nBlocks > 0) hipLaunchKernelGGL(k_e_assemble, dim3(nBlocks), dim3(kTX * kTY), 0, s, f, insts, table, active);
+    if (nSrc > 0)
+        hipLaunchKernelGGL(k_e_sources, dim3((nSrc + 63) / 64), dim3(64), 0, s, f, insts, srcTable, nSrc, active);
+}
+
+void launchEnsStorage(const BoFluid& f, const EnsInst* insts, const int2* table, int nChunks, const int* refresh,
+                      hipStream_t s)
+{
+    if (nChunks > 0) hipLaunchKernelGGL(k_e_storage, dim3(nChunks), dim3(kEnsChunk), 0, s, f, insts, table, refresh);
+}
+
+void launchEnsUpdate(const BoFluid& f, const EnsInst* insts, const int2* table, int nChunks, const int* active,
+                     const NewtonUpdateParams& prm, hipStream_t s)
+{
+    if (nChunks > 0) hipLaunchKernelGGL(k_e_update, dim3(nChunks), dim3(kEnsChunk), 0, s, f, insts, table, active, prm);
+}
+
+void assemblyGrid(const BoGrid& G, int& gx, int& gy, int& gz, int& kChunk, int targetBlocks)
+{
+    gx = (G.nx + kTX - 1) / kTX;
+    gy = (G.ny + kTY - 1) / kTY;
+    gz = (targetBlocks + gx * gy - 1) / (gx * gy);
+    gz = gz < 1 ? 1 : (gz > G.nz ? G.nz : gz);
+    kChunk = (G.nz + gz - 1) / gz;
+    gz = (G.nz + kChunk - 1) / kChunk;
 }
 
 }  // namespace omodels

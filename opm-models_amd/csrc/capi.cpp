@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "ensemble_host.h"
 #include "omodels_host.h"
 
 using namespace omodels;
@@ -37,6 +38,9 @@ int guarded(F&& f)
 
 struct omodels_model {
     std::unique_ptr<Model> m;
+};
+struct omodels_server {
+    std::unique_ptr<Server> s;
 };
 struct omodels_solver {
     omodels_model* owner;
@@ -338,6 +342,84 @@ int omodels_simulate(omodels_model* h, const omodels_params* prm, int n_steps, d
         if (log_len) *log_len = (int32_t)lg.size();
         if (stats) *stats = st;
         return rc;
+    });
+}
+
+omodels_server* omodels_server_create(const omodels_params* prm, double hbm_budget_bytes, int max_batch)
+{
+    omodels_server* h = nullptr;
+    int rc = guarded([&] {
+        h = new omodels_server{std::make_unique<Server>(*prm, hbm_budget_bytes, max_batch)};
+        return 0;
+    });
+    return rc == 0 ? h : nullptr;
+}
+
+void omodels_server_destroy(omodels_server* s) { delete s; }
+
+int omodels_server_submit(omodels_server* s, const omodels_blackoil_problem* p, const double* pv,
+                          const uint8_t* meaning, int n_steps, double dt0, double arrival_s)
+{
+    return guarded([&] { return s->s->submit(p, pv, meaning, n_steps, dt0, arrival_s); });
+}
+
+int omodels_server_run(omodels_server* s)
+{
+    return guarded([&] { s->s->run(); return 0; });
+}
+
+int omodels_server_result(omodels_server* s, int id, double* pv, uint8_t* meaning, double* times, double* dt_hist,
+                          omodels_run_stats* stats)
+{
+    return guarded([&] {
+        const Server::Request& r = s->s->request(id);
+        if (r.status < 0) throw std::invalid_argument("request has not run");
+        if (pv) std::copy(r.result.begin(), r.result.end(), pv);
+        if (meaning) std::copy(r.resultMean.begin(), r.resultMean.end(), meaning);
+        if (times) { times[0] = r.arrival; times[1] = r.admit; times[2] = r.finish; }
+        if (dt_hist) std::copy(r.dtHist.begin(), r.dtHist.end(), dt_hist);
+        if (stats) *stats = r.stats;
+        return r.status;
+    });
+}
+
+int omodels_server_log(omodels_server* s, int id, omodels_iter_log* log, int cap, int32_t* len)
+{
+    return guarded([&] {
+        const Server::Request& r = s->s->request(id);
+        const int n = std::min<int>(cap, (int)r.log.size());
+        if (log) std::copy(r.log.begin(), r.log.begin() + n, log);
+        if (len) *len = (int32_t)r.log.size();
+        return 0;
+    });
+}
+
+int omodels_server_summary(omodels_server* s, double* wall_s, int64_t* batch_steps, int32_t* max_running)
+{
+    return guarded([&] {
+        if (wall_s) *wall_s = s->s->wall();
+        if (batch_steps) *batch_steps = s->s->batchSteps();
+        if (max_running) *max_running = s->s->maxRunning();
+        return 0;
+    });
+}
+
+int omodels_server_set_profiling(omodels_server* s, int enable)
+{
+    return guarded([&] {
+        s->s->ensemble().timer.reset();
+        s->s->ensemble().timer.enabled = enable != 0;
+        return 0;
+    });
+}
+
+int omodels_server_kernel_times(omodels_server* s, double* ms, int64_t* counts, int n)
+{
+    return guarded([&] {
+        KernelTimer& t = s->s->ensemble().timer;
+        t.collect();
+        for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.ms[i]; counts[i] = t.count[i]; }
+        return 0;
     });
 }
 

@@ -1,0 +1,486 @@
+// ens_kernels.hip -- BiCGStab vector steps, SpMV and norms for an ensemble of
+// reservoirs in single launches.  Blocks are cell chunks (kEnsChunk cells of
+// one instance, table chunk -> (instance, chunk)); per-instance reductions
+// sum the chunk partials of that instance in a fixed order, so an instance
+// gets bit-identical scalars whether it is solved alone or in a batch.
+// Instances whose solve has finished are masked by their `active` flag.
+#include "ensemble.h"
+#include "ens_ops.h"
+
+namespace omodels {
+
+namespace {
+
+constexpr int kT = kEnsChunk;
+
+template <class S> __device__ __forceinline__ S* vec(const EnsInst& I, int id);
+template <> __device__ __forceinline__ double* vec<double>(const EnsInst& I, int id) { return I.dv[id]; }
+template <> __device__ __forceinline__ float* vec<float>(const EnsInst& I, int id) { return I.fv[id]; }
+
+__device__ __forceinline__ double waveSum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double waveMax(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
+}
+template <bool IsMax>
+__device__ __forceinline__ double blockReduce(double v)
+{
+    __shared__ double sm[kT / 64];
+    v = IsMax ? waveMax(v) : waveSum(v);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sm[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = sm[0];
+        for (int q = 1; q < kT / 64; ++q) r = IsMax ? fmax(r, sm[q]) : r + sm[q];
+    }
+    __syncthreads();
+    return r;
+}
+// sum / max of the partials of instance b (chunks [first[b], first[b+1]))
+template <bool IsMax>
+__device__ __forceinline__ double instReduce(const double* __restrict__ part, int c0, int c1)
+{
+    double v = 0.0;
+    for (int k = c0 + (int)threadIdx.x; k < c1; k += kT) v = IsMax ? fmax(v, part[k]) : v + part[k];
+    return blockReduce<IsMax>(v);
+}
+
+__device__ __forceinline__ void markFail(KrylovScalars* ks, int tag)
+{
+    if (!ks->fail_code) { ks->fail_code = 2; ks->fail_tag = tag; }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_copy(const EnsInst* __restrict__ insts, const int2* __restrict__ table,
+                                               const int* __restrict__ act, int dst, int src)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    if (c >= I.N) return;
+    S* y = vec<S>(I, dst);
+    if (src < 0) { y[3 * c] = S(0); y[3 * c + 1] = S(0); y[3 * c + 2] = S(0); }
+    else {
+        const S* x = vec<S>(I, src);
+        y[3 * c] = x[3 * c]; y[3 * c + 1] = x[3 * c + 1]; y[3 * c + 2] = x[3 * c + 2];
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_dot_partial(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act, int ia, int ib, double* __restrict__ part)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    double v = 0.0;
+    if (c < I.N) {
+        const S* a = vec<S>(I, ia);
+        const S* b = vec<S>(I, ib);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) v += (double)a[3 * c + e] * (double)b[3 * c + e];
+    }
+    v = blockReduce<false>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// op 0: rho_i and beta; op 1: denom and alpha (bicgstabsolver.hh:197-236)
+__global__ __launch_bounds__(kT) void k_e_dot_final(const int* __restrict__ first, const int* __restrict__ act,
+    const double* __restrict__ part, KrylovScalars* ksAll, int op, double eps, int tag)
+{
+    const int b = blockIdx.x;
+    if (!act[b]) return;
+    const double s = instReduce<false>(part, first[b], first[b + 1]);
+    if (threadIdx.x != 0) return;
+    KrylovScalars* ks = ksAll + b;
+    if (op == 0) {
+        ks->rho_i = s;
+        if (fabs(ks->rho) <= eps || fabs(ks->omega) <= eps) markFail(ks, tag);
+        ks->beta = (s / ks->rho) * (ks->alpha / ks->omega);
+        ks->rho = s;
+    } else {
+        ks->denom = s;
+        if (fabs(s) <= eps) markFail(ks, tag);
+        ks->alpha = ks->rho / s;
+        if (fabs(ks->alpha) <= eps) markFail(ks, tag);
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_update_p(const EnsInst* __restrict__ insts, const int2* __restrict__ table,
+                                                   const int* __restrict__ act, const KrylovScalars* __restrict__ ks)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    if (c >= I.N) return;
+    const S omega = (S)ks[t.x].omega, beta = (S)ks[t.x].beta;
+    const S* r = vec<S>(I, kVr);
+    const S* v = vec<S>(I, kVv);
+    S* p = vec<S>(I, kVp);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const long i = 3 * c + e;
+        S tmp = v[i];
+        tmp *= omega;
+        tmp -= p[i];
+        tmp *= -beta;
+        S pn = r[i];
+        pn += tmp;
+        p[i] = pn;
+    }
+}
+
+// x += alpha y; r -= alpha v; partials of max|r| and of (y != 0)
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_update_xr(const EnsInst* __restrict__ insts, const int2* __restrict__ table,
+    const int* __restrict__ act, const KrylovScalars* __restrict__ ks, double* __restrict__ part, int nChunks)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    double mx = 0.0, nz = 0.0;
+    if (c < I.N) {
+        const S alpha = (S)ks[t.x].alpha;
+        const S* y = vec<S>(I, kVy);
+        const S* v = vec<S>(I, kVv);
+        S* x = vec<S>(I, kVx);
+        S* r = vec<S>(I, kVr);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const long i = 3 * c + e;
+            const S yi = y[i];
+            S tmp = yi;
+            tmp *= alpha;
+            tmp += x[i];
+            x[i] = tmp;
+            tmp = v[i];
+            tmp *= alpha;
+            S ri = r[i];
+            ri -= tmp;
+            r[i] = ri;
+            mx = fmax(mx, fabs((double)ri));
+            if (yi != S(0)) nz = 1.0;
+        }
+    }
+    mx = blockReduce<true>(mx);
+    nz = blockReduce<true>(nz);
+    if (threadIdx.x == 0) { part[blockIdx.x] = mx; part[nChunks + blockIdx.x] = nz; }
+}
+
+__global__ __launch_bounds__(kT) void k_e_crit_final(const int* __restrict__ first, const int* __restrict__ act,
+    const double* __restrict__ part, int nChunks, KrylovScalars* ksAll)
+{
+    const int b = blockIdx.x;
+    if (!act[b]) return;
+    const double mx = instReduce<true>(part, first[b], first[b + 1]);
+    const double nz = instReduce<true>(part + nChunks, first[b], first[b + 1]);
+    if (threadIdx.x != 0) return;
+    ksAll[b].resnorm = mx;
+    ksAll[b].stag_half = nz == 0.0 ? 1 : 0;
+}
+
+// <t,t>, <t,s> with t = y, s = r
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_dot2_partial(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act, double* __restrict__ part, int nChunks)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    double tt = 0.0, ts = 0.0;
+    if (c < I.N) {
+        const S* y = vec<S>(I, kVy);
+        const S* r = vec<S>(I, kVr);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const double ti = (double)y[3 * c + e];
+            tt += ti * ti;
+            ts += ti * (double)r[3 * c + e];
+        }
+    }
+    tt = blockReduce<false>(tt);
+    ts = blockReduce<false>(ts);
+    if (threadIdx.x == 0) { part[blockIdx.x] = tt; part[nChunks + blockIdx.x] = ts; }
+}
+
+__global__ __launch_bounds__(kT) void k_e_dot2_final(const int* __restrict__ first, const int* __restrict__ act,
+    const double* __restrict__ part, int nChunks, KrylovScalars* ksAll, double eps, int tag)
+{
+    const int b = blockIdx.x;
+    if (!act[b]) return;
+    const double tt = instReduce<false>(part, first[b], first[b + 1]);
+    const double ts = instReduce<false>(part + nChunks, first[b], first[b + 1]);
+    if (threadIdx.x != 0) return;
+    KrylovScalars* ks = ksAll + b;
+    if (fabs(tt) <= eps) markFail(ks, tag);
+    ks->omega = ts / tt;
+    if (fabs(ks->omega) <= eps) markFail(ks, tag);
+}
+
+// x += omega z; r += -omega y; partial of (z != 0)
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_update_xr2(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act, const KrylovScalars* __restrict__ ks,
+    double* __restrict__ part)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    double nz = 0.0;
+    if (c < I.N) {
+        const S omega = (S)ks[t.x].omega;
+        const S* z = vec<S>(I, kVz);
+        const S* y = vec<S>(I, kVy);
+        S* x = vec<S>(I, kVx);
+        S* r = vec<S>(I, kVr);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const long i = 3 * c + e;
+            const S zi = z[i];
+            x[i] += omega * zi;
+            if (zi != S(0)) nz = 1.0;
+            r[i] += -omega * y[i];
+        }
+    }
+    nz = blockReduce<true>(nz);
+    if (threadIdx.x == 0) part[blockIdx.x] = nz;
+}
+
+__global__ __launch_bounds__(kT) void k_e_stag_final(const int* __restrict__ first, const int* __restrict__ act,
+    const double* __restrict__ part, KrylovScalars* ksAll, int tag)
+{
+    const int b = blockIdx.x;
+    if (!act[b]) return;
+    const double nz = instReduce<true>(part, first[b], first[b + 1]);
+    if (threadIdx.x != 0) return;
+    if (nz == 0.0 && ksAll[b].stag_full_tag < 0) ksAll[b].stag_full_tag = tag;
+}
+
+// y = A x, A = FP64 Jacobian (useFloat = 0) or its FP32 copy
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_spmv(const EnsInst* __restrict__ insts, const int2* __restrict__ table,
+                                               const int* __restrict__ act, int ix, int iy)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long i = (long)t.y * kT + threadIdx.x;
+    if (i >= I.N) return;
+    const S* a;
+    if constexpr (sizeof(S) == 8) a = (const S*)I.jac; else a = (const S*)I.Af;
+    const S* x = vec<S>(I, ix);
+    S* y = vec<S>(I, iy);
+    S acc[3] = {S(0), S(0), S(0)};
+    const int k1 = I.rowPtr[i + 1];
+    for (int k = I.rowPtr[i]; k < k1; ++k) {
+        const long c = I.colIdx[k];
+        const S* blk = a + 9L * k;
+        const S x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            S s = blk[e * 3] * x0;
+            s += blk[e * 3 + 1] * x1;
+            s += blk[e * 3 + 2] * x2;
+            acc[e] += s;
+        }
+    }
+    y[3 * i] = acc[0]; y[3 * i + 1] = acc[1]; y[3 * i + 2] = acc[2];
+}
+
+// max |vec| per instance -> out[b]
+template <class S>
+__global__ __launch_bounds__(kT) void k_e_maxabs_partial(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act, int id, int useRes, double* __restrict__ part)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    double mx = 0.0;
+    if (c < I.N) {
+        const S* x = useRes ? (const S*)I.res : vec<S>(I, id);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) mx = fmax(mx, fabs((double)x[3 * c + e]));
+    }
+    mx = blockReduce<true>(mx);
+    if (threadIdx.x == 0) part[blockIdx.x] = mx;
+}
+__global__ __launch_bounds__(kT) void k_e_max_final(const int* __restrict__ first, const int* __restrict__ act,
+                                                    const double* __restrict__ part, double* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    if (!act[b]) return;
+    const double mx = instReduce<true>(part, first[b], first[b + 1]);
+    if (threadIdx.x == 0) out[b] = mx;
+}
+
+// lu64 = J (row blocks of the chunk); with toFloat also Af = float(J)
+__global__ __launch_bounds__(kT) void k_e_copy_matrix(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act, int toFloat)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long i = (long)t.y * kT + threadIdx.x;
+    if (i >= I.N) return;
+    for (long k = 9L * I.rowPtr[i]; k < 9L * I.rowPtr[i + 1]; ++k) {
+        const double a = I.jac[k];
+        if (toFloat) I.Af[k] = (float)a;
+        else I.lu64[k] = a;
+    }
+}
+// luf = float(lu64)
+__global__ __launch_bounds__(kT) void k_e_factor_to_float(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long i = (long)t.y * kT + threadIdx.x;
+    if (i >= I.N) return;
+    for (long k = 9L * I.rowPtr[i]; k < 9L * I.rowPtr[i + 1]; ++k) I.luf[k] = (float)I.lu64[k];
+}
+
+// mixed precision: fv[b] = float(r64); then x64 += double(fv[x]); r64 = b64 - J x64
+__global__ __launch_bounds__(kT) void k_e_rhs_to_float(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    if (c >= I.N) return;
+    for (int e = 0; e < 3; ++e) I.fv[kVb][3 * c + e] = (float)I.dv[kVr64][3 * c + e];
+}
+__global__ __launch_bounds__(kT) void k_e_add_correction(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long c = (long)t.y * kT + threadIdx.x;
+    if (c >= I.N) return;
+    for (int e = 0; e < 3; ++e) I.dv[kVx][3 * c + e] += (double)I.fv[kVx][3 * c + e];
+}
+__global__ __launch_bounds__(kT) void k_e_residual64(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act)
+{
+    const int2 t = table[blockIdx.x];
+    if (!act[t.x]) return;
+    const EnsInst& I = insts[t.x];
+    const long i = (long)t.y * kT + threadIdx.x;
+    if (i >= I.N) return;
+    const double* x = I.dv[kVx];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = I.rowPtr[i]; k < I.rowPtr[i + 1]; ++k) {
+        const long c = I.colIdx[k];
+        const double* blk = I.jac + 9L * k;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            double s = blk[e * 3] * x[3 * c];
+            s += blk[e * 3 + 1] * x[3 * c + 1];
+            s += blk[e * 3 + 2] * x[3 * c + 2];
+            acc[e] += s;
+        }
+    }
+    for (int e = 0; e < 3; ++e) I.dv[kVr64][3 * i + e] = I.res[3 * i + e] - acc[e];
+}
+
+// ---------------------------------------------------------------------------
+EnsOps::EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* first, int nInst, double* part,
+               KrylovScalars* ks, hipStream_t s)
+    : insts_(insts), table_(table), nChunks_(nChunks), first_(first), nInst_(nInst), part_(part), ks_(ks), s_(s)
+{}
+
+#define EGRID dim3(nChunks_), dim3(kT), 0, s_
+#define IGRID dim3(nInst_), dim3(kT), 0, s_
+
+template <class S> void EnsOps::copy(const int* act, int dst, int src)
+{ if (nChunks_) hipLaunchKernelGGL((k_e_copy<S>), EGRID, insts_, table_, act, dst, src); }
+
+template <class S> void EnsOps::dot(const int* act, int ia, int ib, int op, int tag)
+{
+    if (!nChunks_) return;
+    hipLaunchKernelGGL((k_e_dot_partial<S>), EGRID, insts_, table_, act, ia, ib, part_);
+    const double eps = (double)std::numeric_limits<S>::min() * 1e10;
+    hipLaunchKernelGGL(k_e_dot_final, IGRID, first_, act, part_, ks_, op, eps, tag);
+}
+
+template <class S> void EnsOps::updateP(const int* act)
+{ if (nChunks_) hipLaunchKernelGGL((k_e_update_p<S>), EGRID, insts_, table_, act, ks_); }
+
+template <class S> void EnsOps::updateXR(const int* act)
+{
+    if (!nChunks_) return;
+    hipLaunchKernelGGL((k_e_update_xr<S>), EGRID, insts_, table_, act, ks_, part_, nChunks_);
+    hipLaunchKernelGGL(k_e_crit_final, IGRID, first_, act, part_, nChunks_, ks_);
+}
+
+template <class S> void EnsOps::dot2(const int* act, int tag)
+{
+    if (!nChunks_) return;
+    hipLaunchKernelGGL((k_e_dot2_partial<S>), EGRID, insts_, table_, act, part_, nChunks_);
+    const double eps = (double)std::numeric_limits<S>::min() * 1e10;
+    hipLaunchKernelGGL(k_e_dot2_final, IGRID, first_, act, part_, nChunks_, ks_, eps, tag);
+}
+
+template <class S> void EnsOps::updateXR2(const int* act, int tag)
+{
+    if (!nChunks_) return;
+    hipLaunchKernelGGL((k_e_update_xr2<S>), EGRID, insts_, table_, act, ks_, part_);
+    hipLaunchKernelGGL(k_e_stag_final, IGRID, first_, act, part_, ks_, tag);
+}
+
+template <class S> void EnsOps::spmv(const int* act, int ix, int iy)
+{ if (nChunks_) hipLaunchKernelGGL((k_e_spmv<S>), EGRID, insts_, table_, act, ix, iy); }
+
+template <class S> void EnsOps::maxAbs(const int* act, int id, int useRes, double* out)
+{
+    if (!nChunks_) return;
+    hipLaunchKernelGGL((k_e_maxabs_partial<S>), EGRID, insts_, table_, act, id, useRes, part_);
+    hipLaunchKernelGGL(k_e_max_final, IGRID, first_, act, part_, out);
+}
+
+void EnsOps::copyMatrix(const int* act, int toFloat)
+{ if (nChunks_) hipLaunchKernelGGL(k_e_copy_matrix, EGRID, insts_, table_, act, toFloat); }
+void EnsOps::factorToFloat(const int* act)
+{ if (nChunks_) hipLaunchKernelGGL(k_e_factor_to_float, EGRID, insts_, table_, act); }
+void EnsOps::rhsToFloat(const int* act)
+{ if (nChunks_) hipLaunchKernelGGL(k_e_rhs_to_float, EGRID, insts_, table_, act); }
+void EnsOps::addCorrection(const int* act)
+{ if (nChunks_) hipLaunchKernelGGL(k_e_add_correction, EGRID, insts_, table_, act); }
+void EnsOps::residual64(const int* act)
+{ if (nChunks_) hipLaunchKernelGGL(k_e_residual64, EGRID, insts_, table_, act); }
+
+#define EINST(S)                                                              \
+    template void EnsOps::copy<S>(const int*, int, int);                     \
+    template void EnsOps::dot<S>(const int*, int, int, int, int);            \
+    template void EnsOps::updateP<S>(const int*);                            \
+    template void EnsOps::updateXR<S>(const int*);                           \
+    template void EnsOps::dot2<S>(const int*, int);                          \
+    template void EnsOps::updateXR2<S>(const int*, int);                     \
+    template void EnsOps::spmv<S>(const int*, int, int);                     \
+    template void EnsOps::maxAbs<S>(const int*, int, int, double*);
+EINST(double)
+EINST(float)
+
+}  // namespace omodels

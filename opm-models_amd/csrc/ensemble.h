@@ -1,0 +1,94 @@
+// ensemble.h -- per-instance device descriptors of the ensemble engine: many
+// reservoir realizations of different sizes assembled and solved by the same
+// launches, each kernel block finding its instance through a table.
+#pragma once
+
+#include <memory>
+#include <utility>
+
+#include "omodels_internal.h"
+#include "ilu_tiled.h"
+
+namespace omodels {
+
+constexpr int kEnsChunk = 256;      // cells per block of the cell-parallel ensemble kernels
+
+// vectors of the Krylov solver (per instance, FP64 and FP32 sets)
+enum EnsVec : int { kVb = 0, kVx, kVr, kVp, kVv, kVy, kVz, kVr64, kNumVec };
+
+struct EnsInst {
+    BoGrid G;
+    int N, nnzb, nsrc, kChunk;
+    int gx, gy, gz, nAsm;            // assembly grid of this instance
+    double storefac;                 // cell volume / dt of the current solve
+    const double* trans;
+    const double* poro;
+    double* pv0;
+    uint8_t* mean0;
+    uint8_t* wasSw;
+    double* st1;
+    double* res;
+    double* jac;
+    const int* rowPtr;
+    const int* colIdx;
+    const int* diagOff;
+    const int* nbOff;
+    const int* srcCell;
+    const int* srcType;
+    const double* srcA;
+    const double* srcB;
+    double* lu64;
+    double* dinv64;
+    float* Af;
+    float* luf;
+    double* dv[kNumVec];
+    float* fv[kNumVec];
+    int* counters;                   // [0] switched, [1] non-finite update
+};
+
+// one tile of one instance, in ticket order
+struct TileTask {
+    int inst, tile;
+};
+
+// per-instance view of the tiled ILU kernels
+struct IluInst {
+    TileGeom g;
+    const int* rp;
+    const int* diag;
+    const void* lu;       // S*
+    void* luw;            // factorization output (double*)
+    const double* dinv;
+    double* dinvw;
+    const void* d;        // S*
+    void* v;              // S*
+    int* done;            // flags of this instance's tiles
+    const int* active;    // nullptr: always active
+};
+
+// Multi-instance tiled ILU: factor / sweeps over all tiles of all instances,
+// ordered by tile level so that dependencies hold smaller tickets.
+class TiledIluSet {
+public:
+    explicit TiledIluSet(hipStream_t s);
+    // (re)build the ticket order for instances with the given geometries
+    void setInstances(const std::vector<TileGeom>& geoms, hipStream_t s);
+    void factor(const std::vector<IluInst>& insts, int* fail, hipStream_t s);
+    template <class S>
+    void apply(const std::vector<IluInst>& insts, hipStream_t s);
+    void checkError(hipStream_t s);
+    int totalTiles() const { return T_; }
+    const std::vector<int>& tileOffsets() const { return tileOff_; }
+
+private:
+    const IluInst* upload(const std::vector<IluInst>& insts, hipStream_t s);
+    DevBuf<TileTask> order_;
+    std::vector<std::pair<std::vector<IluInst>, std::unique_ptr<DevBuf<IluInst>>>> dcache_;
+    DevBuf<int> done_, err_;
+    DevBuf<unsigned long long> counter_;
+    std::vector<int> tileOff_;
+    long long ticketBase_ = 0;   // counter value at the start of the next launch
+    int epoch_ = 0, T_ = 0, grid_ = 0, maxResident_ = 0;
+};
+
+}  // namespace omodels

@@ -1,0 +1,126 @@
+// ensemble_host.h -- host classes of the ensemble engine and of the serving
+// loop (see ensemble.cpp).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/omodels.h"
+#include "ensemble.h"
+#include "omodels_host.h"
+
+namespace omodels {
+
+class Ensemble {
+public:
+    static constexpr int kMaxMembers = 1024;
+
+    struct Member {
+        int id = 0;
+        HostTopology topo;
+        long N = 0, nnzb = 0;
+        BoGrid G{};
+        TileGeom tg{};
+        int gx = 0, gy = 0, gz = 0, kChunk = 1, nsrc = 0;
+        DevBuf<double> trans, poro, pv0, pv1, st1, res, jac, lu64, dinv64, srcA, srcB;
+        DevBuf<uint8_t> mean0, mean1, wasSw;
+        DevBuf<int> rowPtr, colIdx, diagOff, nbOff, srcCell, srcType, counters;
+        DevBuf<float> Af, luf;
+        DevBuf<double> dv[kNumVec];
+        DevBuf<float> fv[kNumVec];
+        // time stepping and Newton state
+        int stepsTotal = 0, stepsDone = 0, attempt = 0, numIter = 0, error = 0;
+        double dt = 0.0, errorN = 1e100, lastErrorN = 1e100;
+        omodels_run_stats stats{};
+        std::vector<omodels_iter_log> log;
+        std::vector<double> dtHist;
+    };
+
+    explicit Ensemble(const omodels_params& prm, hipStream_t s = nullptr);
+    ~Ensemble();
+    Ensemble(const Ensemble&) = delete;
+    Ensemble& operator=(const Ensemble&) = delete;
+
+    // device bytes one member of this problem's size needs (admission control)
+    static size_t memoryEstimate(const omodels_blackoil_problem* p, int precision);
+    int add(const omodels_blackoil_problem* p, const double* pv, const uint8_t* mean, int nSteps, double dt0);
+    void remove(int id);
+    Member* find(int id);
+    void download(int id, double* pv, uint8_t* mean);
+    // one lockstep time step of every member with steps left
+    void timeStep();
+    bool allDone() const;
+    int size() const { return (int)members_.size(); }
+    KernelTimer timer;
+
+private:
+    void rebuild();
+    void refreshStorefac();
+    void setFlags(DevBuf<int>& dst, const std::vector<int>& flags);
+    std::vector<IluInst> iluInsts(const int* act, bool floatFactors, int idD, int idV) const;
+    void linearSolve(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its);
+    template <class S>
+    void bicgstab(const std::vector<int>& set, const std::vector<double>& tol, double absTol, int maxIter,
+                  std::vector<int>& rc, std::vector<int>& its);
+
+    omodels_params prm_;
+    hipStream_t stream_ = nullptr;
+    bool ownStream_ = false;
+    BoFluid fluid_{};
+    bool fluidSet_ = false;
+    std::vector<std::unique_ptr<Member>> members_;
+    int nextId_ = 0;
+    bool dirty_ = true;
+    std::unique_ptr<TiledIluSet> ilu_;
+    DevBuf<EnsInst> desc_;
+    std::vector<EnsInst> hostDesc_;
+    DevBuf<int2> asmTable_, chunkTable_, srcTable_;
+    DevBuf<int> first_, actLin_, actRefresh_, actSolve_, actInner_, fail_;
+    int nAsm_ = 0, nChunks_ = 0, nSrc_ = 0;
+    DevBuf<double> partial_, maxOut_;
+    DevBuf<KrylovScalars> ks_;
+    KrylovScalars* hostKs_ = nullptr;
+    double* hostD_ = nullptr;
+    int* hostI_ = nullptr;
+};
+
+// Request loop with HBM-budget admission and continuous batching.
+class Server {
+public:
+    struct Request {
+        int id = 0;
+        omodels_blackoil_problem prob{};
+        std::vector<double> perm, poro, oil, gas, srcA, srcB, pv, result, dtHist;
+        std::vector<int32_t> srcCell, srcType;
+        std::vector<uint8_t> mean, resultMean;
+        int nSteps = 0, status = -1;
+        double dt0 = 0.0, arrival = 0.0, admit = -1.0, finish = -1.0;
+        size_t bytes = 0;
+        omodels_run_stats stats{};
+        std::vector<omodels_iter_log> log;
+        omodels_blackoil_problem view() const;
+    };
+
+    Server(const omodels_params& prm, double budgetBytes, int maxBatch);
+    int submit(const omodels_blackoil_problem* p, const double* pv, const uint8_t* mean, int nSteps, double dt0,
+               double arrival);
+    void run();
+    const Request& request(int id) const { return reqs_.at(id); }
+    int numRequests() const { return (int)reqs_.size(); }
+    double wall() const { return wall_; }
+    long batchSteps() const { return steps_; }
+    int maxRunning() const { return maxRunning_; }
+    Ensemble& ensemble() { return ens_; }
+
+private:
+    omodels_params prm_;
+    double budget_;
+    int maxBatch_;
+    Ensemble ens_;
+    std::vector<Request> reqs_;
+    double wall_ = 0.0;
+    long steps_ = 0;
+    int maxRunning_ = 0;
+};
+
+}  // namespace omodels

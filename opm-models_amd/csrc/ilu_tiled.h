@@ -12,26 +12,29 @@ struct TileGeom {
     int TX, TY, TZ, T;  // tiles per axis and in total
 };
 
+TileGeom makeTileGeom(int nx, int ny, int nz);
+
+class TiledIluSet;
+
+// One reservoir (an ensemble of one instance).
 class TiledIlu {
 public:
     TiledIlu(int nx, int ny, int nz, hipStream_t s);
+    ~TiledIlu();
+    TiledIlu(const TiledIlu&) = delete;
+    TiledIlu& operator=(const TiledIlu&) = delete;
     // in-place factorization of lu (a copy of A); inverted pivots also row-wise in dinv
     void factor(const int* rp, const int* diag, double* lu, double* dinv, int* fail, hipStream_t s);
     // v = (LU)^-1 d
     template <class S, class P>
     void apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s);
-    // throws if a dependency wait ran into its bound since construction
+    // throws if a dependency wait ran into its bound
     void checkError(hipStream_t s);
     const TileGeom& geom() const { return g; }
-    int gridSize() const { return grid; }
 
 private:
     TileGeom g{};
-    DevBuf<int> order, done, err;
-    DevBuf<unsigned long long> counter;
-    long long sweeps = 0;
-    int epoch = 0;
-    int grid = 0, maxResident = 0;
+    TiledIluSet* set_;
 };
 
 }  // namespace omodels

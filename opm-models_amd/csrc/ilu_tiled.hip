@@ -1,22 +1,27 @@
 // ilu_tiled.hip -- block ILU(0) factorization and triangular sweeps of the
-// 7-point structured-grid pattern as persistent, tile-wavefront kernels.
+// 7-point structured-grid pattern as persistent, tile-wavefront kernels, for
+// one reservoir or for an ensemble of reservoirs in the same launch.
 //
 // Semantics are those of the level-scheduled kernels (dune-istl SeqILU<.,.,.,0>,
 // natural ordering, istlpreconditionerwrappers.hh:151-190): same rows, same
 // per-row operation order.  Only the schedule differs.
 //
-// Schedule: the grid is cut into BX x BY x BZ tiles, one workgroup of 256
+// Schedule: each grid is cut into BX x BY x BZ tiles, one workgroup of 256
 // threads per tile and one thread per cell.  A tile depends on its -x, -y and
-// -z neighbour tiles (+x, +y, +z for the backward sweep); tiles are handed
-// out by an atomic ticket in order of tile level (TI + TJ + TK), so every
-// dependency of a tile holds a smaller ticket.  The kernel is launched
-// cooperatively (all workgroups co-resident), hence the tile with the lowest
-// unfinished ticket can always proceed and every wait ends; waits are also
-// bounded and report an error instead of spinning forever.  Inside a tile the
-// BX+BY+BZ-2 local levels are separated by workgroup barriers and exchange
-// values through LDS; values across tile faces go through global memory,
-// published with an agent-scope release and read after an agent-scope acquire.
-#include "omodels_internal.h"
+// -z neighbour tiles (+x, +y, +z for the backward sweep) of the same
+// instance; tiles of all instances are handed out by one atomic ticket in
+// order of tile level (TI + TJ + TK), so every dependency of a tile holds a
+// smaller ticket.  The kernels are launched cooperatively (all workgroups
+// co-resident), hence the tile with the lowest unfinished ticket can always
+// proceed and every wait ends; waits are also bounded and report an error
+// instead of spinning forever.  Inside a tile the BX+BY+BZ-2 local levels are
+// separated by workgroup barriers and exchange values through LDS; values
+// across tile faces go through global memory, published with an agent-scope
+// release and read once per tile after an agent-scope acquire.
+#include <algorithm>
+#include <cstring>
+
+#include "ensemble.h"
 #include "ilu_tiled.h"
 
 namespace omodels {
@@ -24,11 +29,11 @@ namespace omodels {
 namespace {
 
 constexpr int kTileThreads = 256;
-constexpr long kSpinLimit = 1L << 22;   // about a few seconds of polling
+constexpr long kSpinLimit = 1L << 22;   // a few seconds of polling
 
 // Polling uses relaxed agent-scope loads (no cache maintenance per poll); the
-// single acquire fence after the last dependency is seen invalidates the
-// stale cache lines once per tile.
+// single acquire fence after the last dependency is seen invalidates stale
+// cache lines once per tile.
 __device__ __forceinline__ int loadRelaxed(const int* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -117,39 +122,45 @@ __device__ __forceinline__ void matvecSub(const S* blk, const S* x, S rhs[3])
     }
 }
 
+__device__ __forceinline__ bool isActive(const IluInst& I) { return I.active == nullptr || *I.active != 0; }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Forward sweep: v = L^-1 d (unit lower), lower blocks in column order -z,-y,-x.
 template <class S>
-__global__ __launch_bounds__(kTileThreads) void k_ilu_fwd_tiled(TileGeom g, const int* __restrict__ order,
-    const int* __restrict__ rp, const S* __restrict__ lu, const S* __restrict__ d, S* __restrict__ v,
-    int* __restrict__ done, unsigned long long* counter, long long base, int epoch, int* err)
+__global__ __launch_bounds__(kTileThreads) void k_ilu_fwd_set(const IluInst* __restrict__ insts,
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
 {
     __shared__ S lv[kTileThreads * 3];
     __shared__ int sTicket;
-    const long nxy = (long)g.nx * g.ny;
     for (;;) {
         const int t = nextTicket(counter, base, &sTicket);
-        if (t >= g.T) break;
-        const int tile = order[t];
+        if (t < 0 || t >= T) break;
+        const TileTask task = order[t];
+        const IluInst I = insts[task.inst];
+        if (!isActive(I)) { publish(I.done, task.tile, epoch); continue; }
+        const TileGeom& g = I.g;
+        const S* __restrict__ lu = (const S*)I.lu;
+        const S* __restrict__ d = (const S*)I.d;
+        S* __restrict__ v = (S*)I.v;
+        const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
-        tileOf(g, tile, TI, TJ, TK);
+        tileOf(g, task.tile, TI, TJ, TK);
         const CellCtx c = cellOf(g, TI, TJ, TK);
         // prefetch the row: rhs and the lower blocks (independent of the sweep)
         S rhs[3] = {S(0), S(0), S(0)};
         S Bz[9], By[9], Bx[9];
         const bool hz = c.k > 0, hy = c.j > 0, hx = c.i > 0;
         if (c.valid) {
-            const long r0 = rp[c.I];
 #pragma unroll
             for (int e = 0; e < 3; ++e) rhs[e] = d[3 * c.I + e];
-            long o = r0;
+            long o = I.rp[c.I];
             if (hz) { for (int q = 0; q < 9; ++q) Bz[q] = lu[9 * o + q]; ++o; }
             if (hy) { for (int q = 0; q < 9; ++q) By[q] = lu[9 * o + q]; ++o; }
             if (hx) { for (int q = 0; q < 9; ++q) Bx[q] = lu[9 * o + q]; }
         }
-        waitDeps(g, TI, TJ, TK, -1, done, epoch, err);
+        waitDeps(g, TI, TJ, TK, -1, I.done, epoch, err);
         // values across the tile faces, all loaded in one round trip
         S xz[3], xy[3], xx[3];
         const bool fz = c.valid && hz && c.lk == 0, fy = c.valid && hy && c.lj == 0, fx = c.valid && hx && c.li == 0;
@@ -181,41 +192,46 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_fwd_tiled(TileGeom g, cons
             }
             __syncthreads();
         }
-        publish(done, tile, epoch);
+        publish(I.done, task.tile, epoch);
     }
 }
 
-// Backward sweep: v = U^-1 v with the inverted pivots; upper blocks +x,+y,+z.
-template <class S, class P>
-__global__ __launch_bounds__(kTileThreads) void k_ilu_bwd_tiled(TileGeom g, const int* __restrict__ order,
-    const int* __restrict__ diag, const S* __restrict__ lu, const P* __restrict__ dinv, S* __restrict__ v,
-    int* __restrict__ done, unsigned long long* counter, long long base, int epoch, int* err)
+// Backward sweep: v = U^-1 v with the inverted (FP64) pivots; upper blocks +x,+y,+z.
+template <class S>
+__global__ __launch_bounds__(kTileThreads) void k_ilu_bwd_set(const IluInst* __restrict__ insts,
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
 {
     __shared__ S lv[kTileThreads * 3];
     __shared__ int sTicket;
-    const long nxy = (long)g.nx * g.ny;
     for (;;) {
         const int t = nextTicket(counter, base, &sTicket);
-        if (t >= g.T) break;
-        const int tile = order[g.T - 1 - t];
+        if (t < 0 || t >= T) break;
+        const TileTask task = order[T - 1 - t];
+        const IluInst I = insts[task.inst];
+        if (!isActive(I)) { publish(I.done, task.tile, epoch); continue; }
+        const TileGeom& g = I.g;
+        const S* __restrict__ lu = (const S*)I.lu;
+        const double* __restrict__ dinv = I.dinv;
+        S* __restrict__ v = (S*)I.v;
+        const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
-        tileOf(g, tile, TI, TJ, TK);
+        tileOf(g, task.tile, TI, TJ, TK);
         const CellCtx c = cellOf(g, TI, TJ, TK);
         S rhs[3] = {S(0), S(0), S(0)};
         S Bx[9], By[9], Bz[9];
-        P D[9];
+        double D[9];
         const bool hx = c.i + 1 < g.nx, hy = c.j + 1 < g.ny, hz = c.k + 1 < g.nz;
         if (c.valid) {
 #pragma unroll
             for (int e = 0; e < 3; ++e) rhs[e] = v[3 * c.I + e];
-            long o = diag[c.I] + 1;
+            long o = I.diag[c.I] + 1;
             if (hx) { for (int q = 0; q < 9; ++q) Bx[q] = lu[9 * o + q]; ++o; }
             if (hy) { for (int q = 0; q < 9; ++q) By[q] = lu[9 * o + q]; ++o; }
             if (hz) { for (int q = 0; q < 9; ++q) Bz[q] = lu[9 * o + q]; }
 #pragma unroll
             for (int q = 0; q < 9; ++q) D[q] = dinv[9 * c.I + q];
         }
-        waitDeps(g, TI, TJ, TK, +1, done, epoch, err);
+        waitDeps(g, TI, TJ, TK, +1, I.done, epoch, err);
         S xx[3], xy[3], xz[3];
         const bool fx = c.valid && hx && c.li + 1 == g.BX, fy = c.valid && hy && c.lj + 1 == g.BY;
         const bool fz = c.valid && hz && c.lk + 1 == g.BZ;
@@ -245,9 +261,9 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_bwd_tiled(TileGeom g, cons
                 S out[3];
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
-                    P s = D[e * 3] * (P)rhs[0];
-                    s += D[e * 3 + 1] * (P)rhs[1];
-                    s += D[e * 3 + 2] * (P)rhs[2];
+                    double s = D[e * 3] * (double)rhs[0];
+                    s += D[e * 3 + 1] * (double)rhs[1];
+                    s += D[e * 3 + 2] * (double)rhs[2];
                     out[e] = S(0) + S(1) * (S)s;
                 }
 #pragma unroll
@@ -255,15 +271,17 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_bwd_tiled(TileGeom g, cons
             }
             __syncthreads();
         }
-        publish(done, tile, epoch);
+        publish(I.done, task.tile, epoch);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Factorization: for the 7-point pattern ILU(0) only changes the lower blocks
 // (L_IJ = A_IJ A_JJ^-1) and the pivot (A_II -= L_IJ A_JI for J = -z,-y,-x in
-// that order), exactly the updates of bilu0_decomposition on this pattern.
-// Inverted pivots are also written row-wise to dinv.
+// that order), exactly the updates of bilu0_decomposition on this pattern
+// (two adjacent cells of a structured grid have no common neighbour, so no
+// other entry of the pattern is touched).  Inverted pivots are also written
+// row-wise to dinv.
 __device__ __forceinline__ void mat3mul(const double* A, const double* B, double* C)
 {
 #pragma unroll
@@ -297,48 +315,53 @@ __device__ __forceinline__ bool inv3(double* a)
     return true;
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_ilu_factor_tiled(TileGeom g, const int* __restrict__ order,
-    const int* __restrict__ rp, const int* __restrict__ diag, double* __restrict__ lu, double* __restrict__ dinv,
-    int* __restrict__ done, unsigned long long* counter, long long base, int epoch, int* err, int* fail)
+__global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* __restrict__ insts,
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
+    int* fail)
 {
     __shared__ double ld[kTileThreads * 9];
     __shared__ int sTicket;
-    const long nxy = (long)g.nx * g.ny;
     for (;;) {
         const int t = nextTicket(counter, base, &sTicket);
-        if (t >= g.T) break;
-        const int tile = order[t];
+        if (t < 0 || t >= T) break;
+        const TileTask task = order[t];
+        const IluInst I = insts[task.inst];
+        if (!isActive(I)) { publish(I.done, task.tile, epoch); continue; }
+        const TileGeom& g = I.g;
+        double* __restrict__ lu = (double*)I.luw;
+        double* __restrict__ dinv = I.dinvw;
+        const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
-        tileOf(g, tile, TI, TJ, TK);
+        tileOf(g, task.tile, TI, TJ, TK);
         const CellCtx c = cellOf(g, TI, TJ, TK);
         const bool hz = c.k > 0, hy = c.j > 0, hx = c.i > 0;
         double D[9], Az[9], Ay[9], Ax[9], Uz[9], Uy[9], Ux[9];
         long oz = 0, oy = 0, ox = 0, od = 0;
         if (c.valid) {
-            long o = rp[c.I];
+            long o = I.rp[c.I];
             if (hz) { oz = o++; }
             if (hy) { oy = o++; }
             if (hx) { ox = o++; }
-            od = diag[c.I];
+            od = I.diag[c.I];
             for (int q = 0; q < 9; ++q) D[q] = lu[9 * od + q];
             // A_JI: the block of neighbour row J that points back to I
             if (hz) {
                 const long J = c.I - nxy;
-                const long oJ = diag[J] + 1 + (c.i + 1 < g.nx) + (c.j + 1 < g.ny);
+                const long oJ = I.diag[J] + 1 + (c.i + 1 < g.nx) + (c.j + 1 < g.ny);
                 for (int q = 0; q < 9; ++q) { Az[q] = lu[9 * oz + q]; Uz[q] = lu[9 * oJ + q]; }
             }
             if (hy) {
                 const long J = c.I - g.nx;
-                const long oJ = diag[J] + 1 + (c.i + 1 < g.nx);
+                const long oJ = I.diag[J] + 1 + (c.i + 1 < g.nx);
                 for (int q = 0; q < 9; ++q) { Ay[q] = lu[9 * oy + q]; Uy[q] = lu[9 * oJ + q]; }
             }
             if (hx) {
                 const long J = c.I - 1;
-                const long oJ = diag[J] + 1;
+                const long oJ = I.diag[J] + 1;
                 for (int q = 0; q < 9; ++q) { Ax[q] = lu[9 * ox + q]; Ux[q] = lu[9 * oJ + q]; }
             }
         }
-        waitDeps(g, TI, TJ, TK, -1, done, epoch, err);
+        waitDeps(g, TI, TJ, TK, -1, I.done, epoch, err);
         // pivots of the neighbours across the tile faces, one round trip
         const bool fz = c.valid && hz && c.lk == 0, fy = c.valid && hy && c.lj == 0, fx = c.valid && hx && c.li == 0;
         double Pz[9], Py[9], Px[9];
@@ -349,30 +372,30 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_tiled(TileGeom g, c
         const int nLev = g.BX + g.BY + g.BZ - 2;
         for (int lev = 0; lev <= nLev; ++lev) {
             if (c.valid && myLev == lev) {
-                double Dj[9], L[9], T[9];
+                double Dj[9], L[9], Tm[9];
                 if (hz) {
                     if (!fz) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - g.BX * g.BY) + q]; }
                     else { for (int q = 0; q < 9; ++q) Dj[q] = Pz[q]; }
                     mat3mul(Az, Dj, L);
                     for (int q = 0; q < 9; ++q) lu[9 * oz + q] = L[q];
-                    mat3mul(L, Uz, T);
-                    for (int q = 0; q < 9; ++q) D[q] -= T[q];
+                    mat3mul(L, Uz, Tm);
+                    for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
                 }
                 if (hy) {
                     if (!fy) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - g.BX) + q]; }
                     else { for (int q = 0; q < 9; ++q) Dj[q] = Py[q]; }
                     mat3mul(Ay, Dj, L);
                     for (int q = 0; q < 9; ++q) lu[9 * oy + q] = L[q];
-                    mat3mul(L, Uy, T);
-                    for (int q = 0; q < 9; ++q) D[q] -= T[q];
+                    mat3mul(L, Uy, Tm);
+                    for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
                 }
                 if (hx) {
                     if (!fx) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - 1) + q]; }
                     else { for (int q = 0; q < 9; ++q) Dj[q] = Px[q]; }
                     mat3mul(Ax, Dj, L);
                     for (int q = 0; q < 9; ++q) lu[9 * ox + q] = L[q];
-                    mat3mul(L, Ux, T);
-                    for (int q = 0; q < 9; ++q) D[q] -= T[q];
+                    mat3mul(L, Ux, Tm);
+                    for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
                 }
                 if (!inv3(D)) atomicOr(fail, 2);
                 for (int q = 0; q < 9; ++q) {
@@ -383,12 +406,12 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_tiled(TileGeom g, c
             }
             __syncthreads();
         }
-        publish(done, tile, epoch);
+        publish(I.done, task.tile, epoch);
     }
 }
 
 // ---------------------------------------------------------------------------
-TiledIlu::TiledIlu(int nx, int ny, int nz, hipStream_t s)
+TileGeom makeTileGeom(int nx, int ny, int nz)
 {
     // tile shape: 256 cells, as cubic as the grid allows
     int bz = nz >= 4 ? 4 : (nz >= 2 ? 2 : 1);
@@ -398,105 +421,162 @@ TiledIlu::TiledIlu(int nx, int ny, int nz, hipStream_t s)
     int by = rest / bx;
     if (nx < bx) { bx = 1; while (bx * 2 <= nx) bx *= 2; by = rest / bx; }
     if (ny < by) { by = 1; while (by * 2 <= ny) by *= 2; }
+    TileGeom g;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.BX = bx; g.BY = by; g.BZ = bz;
     g.TX = (nx + bx - 1) / bx; g.TY = (ny + by - 1) / by; g.TZ = (nz + bz - 1) / bz;
     g.T = g.TX * g.TY * g.TZ;
-    // tiles in order of tile level
-    std::vector<int> ord(g.T);
-    std::vector<int> cnt(g.TX + g.TY + g.TZ, 0);
-    for (int t = 0; t < g.T; ++t) {
-        const int TI = t % g.TX, TJ = (t / g.TX) % g.TY, TK = t / (g.TX * g.TY);
-        cnt[TI + TJ + TK + 1]++;
-    }
-    for (size_t l = 1; l < cnt.size(); ++l) cnt[l] += cnt[l - 1];
-    for (int t = 0; t < g.T; ++t) {
-        const int TI = t % g.TX, TJ = (t / g.TX) % g.TY, TK = t / (g.TX * g.TY);
-        ord[cnt[TI + TJ + TK]++] = t;
-    }
-    order.upload(ord.data(), ord.size(), s);
-    done.alloc(g.T);
-    OM_HIP(hipMemsetAsync(done.p, 0, g.T * sizeof(int), s));
-    counter.alloc(1);
-    OM_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned long long), s));
-    err.alloc(1);
-    OM_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    return g;
+}
+
+TiledIluSet::TiledIluSet(hipStream_t s)
+{
+    counter_.alloc(1);
+    OM_HIP(hipMemsetAsync(counter_.p, 0, sizeof(unsigned long long), s));
+    err_.alloc(1);
+    OM_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
     int dev = 0;
     OM_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     OM_HIP(hipGetDeviceProperties(&prop, dev));
-    int perCU = 0;
-    OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)k_ilu_bwd_tiled<double, double>,
-                                                        kTileThreads, 0));
-    int perCU2 = 0;
-    OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU2, (const void*)k_ilu_factor_tiled, kTileThreads, 0));
-    int perCU3 = 0;
-    OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU3, (const void*)k_ilu_fwd_tiled<double>,
-                                                        kTileThreads, 0));
-    int perCU4 = 0;
-    OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU4, (const void*)k_ilu_bwd_tiled<float, double>,
-                                                        kTileThreads, 0));
-    int perCU5 = 0;
-    OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU5, (const void*)k_ilu_fwd_tiled<float>,
-                                                        kTileThreads, 0));
-    perCU = std::min(std::min(std::min(perCU, perCU2), std::min(perCU3, perCU4)), perCU5);
+    const void* ks[] = {(const void*)k_ilu_factor_set, (const void*)k_ilu_fwd_set<double>,
+                        (const void*)k_ilu_bwd_set<double>, (const void*)k_ilu_fwd_set<float>,
+                        (const void*)k_ilu_bwd_set<float>};
+    int perCU = 1 << 20;
+    for (const void* k : ks) {
+        int n = 0;
+        OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kTileThreads, 0));
+        perCU = std::min(perCU, n);
+    }
     if (perCU < 1) throw std::runtime_error("tiled ILU kernels cannot be resident");
-    maxResident = perCU * prop.multiProcessorCount;
-    grid = std::min(g.T, maxResident);
+    maxResident_ = perCU * prop.multiProcessorCount;
 }
 
-void TiledIlu::checkError(hipStream_t s)
+void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s)
+{
+    // tickets: all tiles of all instances ordered by tile level, then instance
+    std::vector<std::pair<int, TileTask>> tasks;
+    tileOff_.assign(geoms.size() + 1, 0);
+    for (size_t b = 0; b < geoms.size(); ++b) {
+        const TileGeom& g = geoms[b];
+        tileOff_[b + 1] = tileOff_[b] + g.T;
+        for (int t = 0; t < g.T; ++t) {
+            const int TI = t % g.TX, TJ = (t / g.TX) % g.TY, TK = t / (g.TX * g.TY);
+            tasks.push_back({TI + TJ + TK, TileTask{(int)b, t}});
+        }
+    }
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const std::pair<int, TileTask>& a, const std::pair<int, TileTask>& b) { return a.first < b.first; });
+    std::vector<TileTask> ord(tasks.size());
+    for (size_t k = 0; k < tasks.size(); ++k) ord[k] = tasks[k].second;
+    OM_HIP(hipStreamSynchronize(s));
+    T_ = (int)ord.size();
+    order_.upload(ord.data(), ord.size(), s);
+    done_.alloc(std::max(T_, 1));
+    OM_HIP(hipMemsetAsync(done_.p, 0, std::max(T_, 1) * sizeof(int), s));
+    // flags restart from zero: restart the epoch so that no stale value matches
+    epoch_ = 0;
+    grid_ = std::min(T_, maxResident_);
+    dcache_.clear();
+}
+
+const IluInst* TiledIluSet::upload(const std::vector<IluInst>& insts, hipStream_t s)
+{
+    std::vector<IluInst> v(insts);
+    for (size_t b = 0; b < v.size(); ++b) v[b].done = done_.p + tileOff_[b];
+    // descriptor arrays stay resident: a solver alternates between a few
+    // (input, output) vector pairs, each gets its own device copy once
+    for (auto& e : dcache_)
+        if (e.first.size() == v.size() && std::memcmp(e.first.data(), v.data(), v.size() * sizeof(IluInst)) == 0)
+            return e.second->p;
+    if (dcache_.size() >= 16) {
+        OM_HIP(hipStreamSynchronize(s));
+        dcache_.clear();
+    }
+    auto buf = std::make_unique<DevBuf<IluInst>>();
+    buf->upload(v.data(), v.size(), s);
+    dcache_.emplace_back(v, std::move(buf));
+    return dcache_.back().second->p;
+}
+
+void TiledIluSet::checkError(hipStream_t s)
 {
     int e = 0;
-    OM_HIP(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    OM_HIP(hipMemcpyAsync(&e, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     OM_HIP(hipStreamSynchronize(s));
     if (e) throw std::runtime_error("tiled ILU sweep: dependency wait exceeded its bound");
 }
 
-template <class F>
-static void coopLaunch(F kernel, int grid, void** args, hipStream_t s)
+void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream_t s)
 {
-    OM_HIP(hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(kTileThreads), args, 0, s));
+    if (T_ == 0) return;
+    const IluInst* di = upload(insts, s);
+    long long b = ticketBase_;
+    ticketBase_ += T_ + grid_;   // every launch draws exactly T + grid tickets
+    int ep = ++epoch_;
+    const TileTask* od = order_.p;
+    int T = T_;
+    unsigned long long* ct = counter_.p;
+    int* er = err_.p;
+    void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&fail};
+    OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_factor_set, dim3(grid_), dim3(kTileThreads), args, 0, s));
 }
+
+template <class S>
+void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
+{
+    if (T_ == 0) return;
+    const IluInst* di = upload(insts, s);
+    const TileTask* od = order_.p;
+    int T = T_;
+    unsigned long long* ct = counter_.p;
+    int* er = err_.p;
+    {
+        long long b = ticketBase_;
+        ticketBase_ += T_ + grid_;   // every launch draws exactly T + grid tickets
+        int ep = ++epoch_;
+        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er};
+        OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_fwd_set<S>, dim3(grid_), dim3(kTileThreads), args, 0, s));
+    }
+    {
+        long long b = ticketBase_;
+        ticketBase_ += T_ + grid_;   // every launch draws exactly T + grid tickets
+        int ep = ++epoch_;
+        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er};
+        OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_bwd_set<S>, dim3(grid_), dim3(kTileThreads), args, 0, s));
+    }
+}
+template void TiledIluSet::apply<double>(const std::vector<IluInst>&, hipStream_t);
+template void TiledIluSet::apply<float>(const std::vector<IluInst>&, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// single reservoir: an ensemble of one
+TiledIlu::TiledIlu(int nx, int ny, int nz, hipStream_t s) : set_(new TiledIluSet(s))
+{
+    g = makeTileGeom(nx, ny, nz);
+    set_->setInstances({g}, s);
+}
+
+TiledIlu::~TiledIlu() { delete set_; }
+
+void TiledIlu::checkError(hipStream_t s) { set_->checkError(s); }
 
 void TiledIlu::factor(const int* rp, const int* diag, double* lu, double* dinvOut, int* fail, hipStream_t s)
 {
-    const long long base = sweeps * (long long)(g.T + grid);
-    ++sweeps;
-    int ep = ++epoch;
-    TileGeom gg = g;
-    const int* ord = order.p;
-    int* dn = done.p;
-    unsigned long long* ct = counter.p;
-    int* er = err.p;
-    long long b = base;
-    void* args[] = {&gg, (void*)&ord, (void*)&rp, (void*)&diag, (void*)&lu, (void*)&dinvOut, &dn, &ct, &b, &ep, &er,
-                    (void*)&fail};
-    coopLaunch(k_ilu_factor_tiled, grid, args, s);
+    IluInst I{};
+    I.g = g; I.rp = rp; I.diag = diag; I.lu = lu; I.luw = lu; I.dinv = dinvOut; I.dinvw = dinvOut;
+    I.active = nullptr;
+    set_->factor({I}, fail, s);
 }
 
 template <class S, class P>
 void TiledIlu::apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s)
 {
-    TileGeom gg = g;
-    const int* ord = order.p;
-    int* dn = done.p;
-    unsigned long long* ct = counter.p;
-    int* er = err.p;
-    {
-        long long b = sweeps * (long long)(g.T + grid);
-        ++sweeps;
-        int ep = ++epoch;
-        void* args[] = {&gg, (void*)&ord, (void*)&rp, (void*)&lu, (void*)&d, (void*)&v, &dn, &ct, &b, &ep, &er};
-        coopLaunch(k_ilu_fwd_tiled<S>, grid, args, s);
-    }
-    {
-        long long b = sweeps * (long long)(g.T + grid);
-        ++sweeps;
-        int ep = ++epoch;
-        void* args[] = {&gg, (void*)&ord, (void*)&diag, (void*)&lu, (void*)&dinv, (void*)&v, &dn, &ct, &b, &ep, &er};
-        coopLaunch(k_ilu_bwd_tiled<S, P>, grid, args, s);
-    }
+    static_assert(sizeof(P) == 8, "pivots are kept in FP64");
+    IluInst I{};
+    I.g = g; I.rp = rp; I.diag = diag; I.lu = lu; I.dinv = (const double*)dinv; I.d = d; I.v = v;
+    I.active = nullptr;
+    set_->apply<S>({I}, s);
 }
 template void TiledIlu::apply<double, double>(const int*, const int*, const double*, const double*, const double*,
                                               double*, hipStream_t);

@@ -97,6 +97,17 @@ SIGNATURES = {
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
     "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
+    "omodels_server_create": (C.c_void_p, [C.POINTER(Params), C.c_double, C.c_int]),
+    "omodels_server_destroy": (None, [C.c_void_p]),
+    "omodels_server_submit": (C.c_int, [C.c_void_p, C.POINTER(BlackOilProblem), _dp, _u8p, C.c_int, C.c_double,
+                                        C.c_double]),
+    "omodels_server_run": (C.c_int, [C.c_void_p]),
+    "omodels_server_result": (C.c_int, [C.c_void_p, C.c_int, _dp, _u8p, _dp, _dp, C.POINTER(RunStats)]),
+    "omodels_server_log": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(IterLog), C.c_int, _ip]),
+    "omodels_server_summary": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int32)]),
+    "omodels_server_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "omodels_server_kernel_times": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64), C.c_int]),
     "omodels_model_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "omodels_model_kernel_times": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64), C.c_int]),
     "omodels_simulate": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int, C.c_double, _dp, _dp,
@@ -358,3 +369,72 @@ class Solver:
         v = np.zeros_like(d)
         check(lib().omodels_solver_apply_precond(self.h, ptr(d), ptr(v)), "apply_precond")
         return v
+
+
+class Server:
+    """Ensemble engine behind a request loop (continuous batching)."""
+
+    def __init__(self, params, hbm_budget_bytes=64e9, max_batch=256):
+        h = lib().omodels_server_create(C.byref(params), float(hbm_budget_bytes), int(max_batch))
+        if not h:
+            raise OmodelsError("server_create failed: " + lib().omodels_last_error().decode())
+        self.h = C.c_void_p(h)
+        self.cases = {}
+
+    def close(self):
+        if self.h:
+            lib().omodels_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, case, n_steps, dt0=86400.0, arrival=0.0, pv=None, mean=None):
+        pv = case.pv0 if pv is None else np.ascontiguousarray(pv, dtype=np.float64)
+        mean = case.mean0 if mean is None else np.ascontiguousarray(mean, dtype=np.uint8)
+        rid = check(lib().omodels_server_submit(self.h, C.byref(case.prob), ptr(pv), ptr(mean, _u8p), int(n_steps),
+                                                float(dt0), float(arrival)), "server_submit")
+        self.cases[rid] = (case, n_steps)
+        return rid
+
+    def run(self):
+        check(lib().omodels_server_run(self.h), "server_run")
+
+    def result(self, rid):
+        case, n_steps = self.cases[rid]
+        pv = np.zeros(3 * case.N)
+        mean = np.zeros(case.N, np.uint8)
+        times = np.zeros(3)
+        dts = np.zeros(n_steps)
+        st = RunStats()
+        status = check(lib().omodels_server_result(self.h, rid, ptr(pv), ptr(mean, _u8p), ptr(times), ptr(dts),
+                                                   C.byref(st)), "server_result")
+        return dict(status=status, pv=pv, mean=mean, arrival=times[0], admit=times[1], finish=times[2], dts=dts,
+                    stats=st)
+
+    def log(self, rid, cap=4096):
+        buf = (IterLog * cap)()
+        n = C.c_int32()
+        check(lib().omodels_server_log(self.h, rid, buf, cap, C.byref(n)), "server_log")
+        return [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
+                for l in buf[:min(n.value, cap)]]
+
+    def summary(self):
+        w = C.c_double()
+        bs = C.c_int64()
+        mr = C.c_int32()
+        check(lib().omodels_server_summary(self.h, C.byref(w), C.byref(bs), C.byref(mr)), "server_summary")
+        return dict(wall=w.value, batch_steps=bs.value, max_running=mr.value)
+
+    def set_profiling(self, enable=True):
+        check(lib().omodels_server_set_profiling(self.h, int(enable)), "server_set_profiling")
+
+    def kernel_times(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, np.int64)
+        check(lib().omodels_server_kernel_times(self.h, ptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), 5),
+              "server_kernel_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(Model.KERNEL_CLASSES)}
