@@ -334,18 +334,27 @@ __global__ __launch_bounds__(kT) void k_e_max_final(const int* __restrict__ firs
 }
 
 // lu64 = J (row blocks of the chunk); with toFloat also Af = float(J)
+// The block values of a chunk's rows are one contiguous range; it is copied
+// with consecutive threads on consecutive doubles.
+__device__ __forceinline__ void chunkValueRange(const EnsInst& I, int chunk, long& lo, long& hi)
+{
+    const long r0 = (long)chunk * kT;
+    const long r1 = r0 + kT < I.N ? r0 + kT : I.N;
+    lo = 9L * I.rowPtr[r0];
+    hi = 9L * I.rowPtr[r1];
+}
 __global__ __launch_bounds__(kT) void k_e_copy_matrix(const EnsInst* __restrict__ insts,
     const int2* __restrict__ table, const int* __restrict__ act, int toFloat)
 {
     const int2 t = table[blockIdx.x];
     if (!act[t.x]) return;
     const EnsInst& I = insts[t.x];
-    const long i = (long)t.y * kT + threadIdx.x;
-    if (i >= I.N) return;
-    for (long k = 9L * I.rowPtr[i]; k < 9L * I.rowPtr[i + 1]; ++k) {
-        const double a = I.jac[k];
-        if (toFloat) I.Af[k] = (float)a;
-        else I.lu64[k] = a;
+    long lo, hi;
+    chunkValueRange(I, t.y, lo, hi);
+    if (toFloat) {
+        for (long k = lo + threadIdx.x; k < hi; k += kT) I.Af[k] = (float)I.jac[k];
+    } else {
+        for (long k = lo + threadIdx.x; k < hi; k += kT) I.lu64[k] = I.jac[k];
     }
 }
 // luf = float(lu64)
@@ -355,9 +364,9 @@ __global__ __launch_bounds__(kT) void k_e_factor_to_float(const EnsInst* __restr
     const int2 t = table[blockIdx.x];
     if (!act[t.x]) return;
     const EnsInst& I = insts[t.x];
-    const long i = (long)t.y * kT + threadIdx.x;
-    if (i >= I.N) return;
-    for (long k = 9L * I.rowPtr[i]; k < 9L * I.rowPtr[i + 1]; ++k) I.luf[k] = (float)I.lu64[k];
+    long lo, hi;
+    chunkValueRange(I, t.y, lo, hi);
+    for (long k = lo + threadIdx.x; k < hi; k += kT) I.luf[k] = (float)I.lu64[k];
 }
 
 // mixed precision: fv[b] = float(r64); then x64 += double(fv[x]); r64 = b64 - J x64

@@ -88,7 +88,8 @@ private:
     DevBuf<unsigned long long> counter_;
     std::vector<int> tileOff_;
     long long ticketBase_ = 0;   // counter value at the start of the next launch
-    int epoch_ = 0, T_ = 0, grid_ = 0, maxResident_ = 0;
+    int epoch_ = 0, T_ = 0, gridFactor_ = 0, residentFactor_ = 0;
+    int residentSweep_[2][2] = {};   // [FP64, FP32][forward, backward]
 };
 
 }  // namespace omodels

@@ -15,9 +15,11 @@
 // co-resident), hence the tile with the lowest unfinished ticket can always
 // proceed and every wait ends; waits are also bounded and report an error
 // instead of spinning forever.  Inside a tile the BX+BY+BZ-2 local levels are
-// separated by workgroup barriers and exchange values through LDS; values
-// across tile faces go through global memory, published with an agent-scope
-// release and read once per tile after an agent-scope acquire.
+// separated by workgroup barriers and exchange values through LDS only; the
+// tile's results are stored to global memory once, after its last level.
+// Values across tile faces go through global memory, written through to the
+// device-coherent level before the tile's flag is published and read after
+// the flag with L2-bypassing loads.
 #include <algorithm>
 #include <cstring>
 
@@ -29,19 +31,39 @@ namespace omodels {
 namespace {
 
 constexpr int kTileThreads = 256;
+constexpr int kPadMax = 704;            // bound of (BX+1)(BY+1)(BZ+1) over all tile shapes
 constexpr long kSpinLimit = 1L << 22;   // a few seconds of polling
 
-// Polling uses relaxed agent-scope loads (no cache maintenance per poll); the
-// single acquire fence after the last dependency is seen invalidates stale
-// cache lines once per tile.
+// Cross-tile exchange.  Every value that another tile reads while this kernel
+// runs (sweep outputs, inverted pivots) and every done-flag is written and read
+// with device-scope relaxed atomics: on gfx950 these are write-through stores
+// and L2-bypassing loads (the sc1 forms), coherent across the XCDs without any
+// cache-wide maintenance.  A tile publishes its flag only after each of its
+// threads has seen its own write-through stores acknowledged (vmcnt(0)) and
+// the workgroup has met at a barrier; a dependent tile issues its exchange
+// loads only after its poll has returned the flag.  The agent-scope
+// release/acquire pair that would express the same order in the language
+// model costs a write-back and an invalidation of the whole L2 of the XCD per
+// tile (buffer_wbl2 / buffer_inv), which dominated the sweep time; it is not
+// needed because no exchanged value is ever accessed non-atomically while the
+// kernel runs.
 __device__ __forceinline__ int loadRelaxed(const int* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void storeRelease(int* p, int v)
+template <class S>
+__device__ __forceinline__ S ldCoh(const S* p)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<S*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+template <class S>
+__device__ __forceinline__ void stCoh(S* p, S v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every vector memory operation of this wave has completed
+// (s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void waitStores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 struct CellCtx {
     int li, lj, lk;        // local coordinates in the tile
@@ -90,36 +112,30 @@ __device__ __forceinline__ void waitDeps(const TileGeom& g, int TI, int TJ, int 
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ int nextTicket(unsigned long long* counter, long long base, int* sTicket)
+// Tickets: thread 0 draws the next ticket while the current tile is worked
+// on and hands it to the workgroup at the end of the tile.  Holding one drawn
+// ticket while finishing a smaller one keeps the progress argument intact:
+// the holder of the smallest unfinished ticket is always working on it.
+__device__ __forceinline__ int drawTicket(unsigned long long* counter, long long base)
 {
-    if (threadIdx.x == 0) *sTicket = (int)((long long)atomicAdd(counter, 1ULL) - base);
+    return (int)((long long)atomicAdd(counter, 1ULL) - base);
+}
+__device__ __forceinline__ int shareTicket(int t, int* sTicket)
+{
+    if (threadIdx.x == 0) *sTicket = t;
     __syncthreads();
-    const int t = *sTicket;
-    __syncthreads();
-    return t;
+    return *sTicket;
 }
 
 __device__ __forceinline__ void publish(int* done, int tile, int epoch)
 {
+    waitStores();
     __syncthreads();
-    if (threadIdx.x == 0) storeRelease(done + tile, epoch);
-}
-
-template <class S>
-__device__ __forceinline__ void matvecSub(const S* blk, const S* x, S rhs[3])
-{
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        S s = blk[e * 3] * x[0];
-        s += blk[e * 3 + 1] * x[1];
-        s += blk[e * 3 + 2] * x[2];
-        rhs[e] += S(-1) * s;
-    }
+    if (threadIdx.x == 0) stCoh(done + tile, epoch);
 }
 
 __device__ __forceinline__ bool isActive(const IluInst& I) { return I.active == nullptr || *I.active != 0; }
@@ -127,151 +143,233 @@ __device__ __forceinline__ bool isActive(const IluInst& I) { return I.active == 
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Sweeps.  A sweep workgroup has four lanes per tile cell, the cells taken in
+// level order: lanes 4r..4r+3 handle the r-th cell of the tile's level
+// sequence, one lane per block row (the fourth lane repeats the third row and
+// stores nothing).  Each lane keeps its rows of the cell's blocks and its
+// right-hand side in registers for the whole tile; a level is then one
+// contiguous range of lanes (one or two waves) that reads the neighbour values
+// from LDS, does its row's products and writes its result to LDS.  Values of
+// neighbour tiles are staged as a ghost layer in LDS after their flags are
+// seen.  Per row the operations and their order are those of dune's block mmv.
+constexpr int kSweepThreads = 4 * kTileThreads;
+
+__device__ __forceinline__ int tri3(int m) { return m < 0 ? 0 : (m + 1) * (m + 2) * (m + 3) / 6; }
+// number of cells (a, b, c) of a BX x BY x BZ box with a + b + c < l
+__device__ __forceinline__ int cellsBelow(int l, int BX, int BY, int BZ)
+{
+    const int m = l - 1;
+    return tri3(m) - tri3(m - BX) - tri3(m - BY) - tri3(m - BZ) + tri3(m - BX - BY) + tri3(m - BX - BZ) +
+           tri3(m - BY - BZ) - tri3(m - BX - BY - BZ);
+}
+// position of (a, b, c) among the cells of its level a + b + c, ordered by (c, b)
+__device__ __forceinline__ int rankInLevel(int a, int b, int c, int BX, int BY)
+{
+    const int l = a + b + c;
+    int r = 0;
+    for (int cp = 0; cp < c; ++cp) {
+        const int lo = max(0, l - cp - BX + 1), hi = min(BY - 1, l - cp);
+        r += max(0, hi - lo + 1);
+    }
+    return r + b - max(0, l - c - BX + 1);
+}
+
+// one word per cell: local coordinates and which neighbours exist; -1 outside the grid
+__device__ __forceinline__ int packCell(int li, int lj, int lk, bool n0, bool n1, bool n2)
+{
+    return li | (lj << 8) | (lk << 16) | ((int)n0 << 27) | ((int)n1 << 28) | ((int)n2 << 29);
+}
+
+// value of lane Q of this lane's quad (all four lanes of the quad are active)
+template <int Q>
+__device__ __forceinline__ double quadLane(double v)
+{
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int Q>
+__device__ __forceinline__ float quadLane(float v)
+{
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+template <class S>
+__device__ __forceinline__ S rowDot(const S* b, const S* x)
+{
+    S s = b[0] * x[0];
+    s += b[1] * x[1];
+    s += b[2] * x[2];
+    return s;
+}
+
 // Forward sweep: v = L^-1 d (unit lower), lower blocks in column order -z,-y,-x.
 template <class S>
-__global__ __launch_bounds__(kTileThreads) void k_ilu_fwd_set(const IluInst* __restrict__ insts,
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
 {
-    __shared__ S lv[kTileThreads * 3];
+    __shared__ S sV[kPadMax * 3];          // results, ghosts at local -1
+    __shared__ int sOrder[kTileThreads];   // cells in level order
     __shared__ int sTicket;
+    const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
+    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, base) : 0, &sTicket);
     for (;;) {
-        const int t = nextTicket(counter, base, &sTicket);
         if (t < 0 || t >= T) break;
+        int next = 0;
+        if (threadIdx.x == 0) next = drawTicket(counter, base);
         const TileTask task = order[t];
         const IluInst I = insts[task.inst];
-        if (!isActive(I)) { publish(I.done, task.tile, epoch); continue; }
+        if (!isActive(I)) {
+            publish(I.done, task.tile, epoch);
+            t = shareTicket(next, &sTicket);
+            continue;
+        }
         const TileGeom& g = I.g;
+        const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
         const S* __restrict__ lu = (const S*)I.lu;
         const S* __restrict__ d = (const S*)I.d;
         S* __restrict__ v = (S*)I.v;
         const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
         tileOf(g, task.tile, TI, TJ, TK);
-        const CellCtx c = cellOf(g, TI, TJ, TK);
-        // prefetch the row: rhs and the lower blocks (independent of the sweep)
-        S rhs[3] = {S(0), S(0), S(0)};
-        S Bz[9], By[9], Bx[9];
-        const bool hz = c.k > 0, hy = c.j > 0, hx = c.i > 0;
-        if (c.valid) {
-#pragma unroll
-            for (int e = 0; e < 3; ++e) rhs[e] = d[3 * c.I + e];
-            long o = I.rp[c.I];
-            if (hz) { for (int q = 0; q < 9; ++q) Bz[q] = lu[9 * o + q]; ++o; }
-            if (hy) { for (int q = 0; q < 9; ++q) By[q] = lu[9 * o + q]; ++o; }
-            if (hx) { for (int q = 0; q < 9; ++q) Bx[q] = lu[9 * o + q]; }
+        if (threadIdx.x < BX * BY * BZ) {
+            const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
+            const int i = TI * BX + li, j = TJ * BY + lj, k = TK * BZ + lk;
+            sOrder[cellsBelow(li + lj + lk, BX, BY, BZ) + rankInLevel(li, lj, lk, BX, BY)] =
+                (i < g.nx && j < g.ny && k < g.nz) ? packCell(li, lj, lk, k > 0, j > 0, i > 0) : -1;
+        }
+        __syncthreads();
+        const int word = pos < BX * BY * BZ ? sOrder[pos] : -1;
+        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
+        const bool hz = word >= 0 && (word & (1 << 27)), hy = word >= 0 && (word & (1 << 28));
+        const bool hx = word >= 0 && (word & (1 << 29));
+        const long Ic = (long)(TI * BX + li) + (long)g.nx * ((long)(TJ * BY + lj) + (long)g.ny * (TK * BZ + lk));
+        const int p = (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
+        const int myLev = word >= 0 ? li + lj + lk : -1;
+        // this lane's row of the cell's lower blocks and its right-hand side
+        S Lz[3] = {}, Ly[3] = {}, Lx[3] = {};
+        S r = S(0);
+        if (word >= 0) {
+            r = d[3 * Ic + e];
+            long o = I.rp[Ic];
+            if (hz) { for (int w = 0; w < 3; ++w) Lz[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (hy) { for (int w = 0; w < 3; ++w) Ly[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (hx) { for (int w = 0; w < 3; ++w) Lx[w] = lu[9 * o + 3 * e + w]; }
         }
         waitDeps(g, TI, TJ, TK, -1, I.done, epoch, err);
-        // values across the tile faces, all loaded in one round trip
-        S xz[3], xy[3], xx[3];
-        const bool fz = c.valid && hz && c.lk == 0, fy = c.valid && hy && c.lj == 0, fx = c.valid && hx && c.li == 0;
-        if (fz) { for (int e = 0; e < 3; ++e) xz[e] = v[3 * (c.I - nxy) + e]; }
-        if (fy) { for (int e = 0; e < 3; ++e) xy[e] = v[3 * (c.I - g.nx) + e]; }
-        if (fx) { for (int e = 0; e < 3; ++e) xx[e] = v[3 * (c.I - 1) + e]; }
-        const int myLev = c.li + c.lj + c.lk;
-        const int nLev = g.BX + g.BY + g.BZ - 2;
+        // ghosts: values of the neighbour tiles, written before their flags
+        if (q < 3) {
+            if (hz && lk == 0) sV[3 * (p - PXY) + e] = ldCoh(v + 3 * (Ic - nxy) + e);
+            if (hy && lj == 0) sV[3 * (p - PX) + e] = ldCoh(v + 3 * (Ic - g.nx) + e);
+            if (hx && li == 0) sV[3 * (p - 1) + e] = ldCoh(v + 3 * (Ic - 1) + e);
+        }
+        __syncthreads();
+        const int nLev = BX + BY + BZ - 2;
         for (int lev = 0; lev <= nLev; ++lev) {
-            if (c.valid && myLev == lev) {
-                S x[3];
-                if (hz) {
-                    if (!fz) { for (int e = 0; e < 3; ++e) x[e] = lv[3 * (threadIdx.x - g.BX * g.BY) + e]; }
-                    else { for (int e = 0; e < 3; ++e) x[e] = xz[e]; }
-                    matvecSub(Bz, x, rhs);
-                }
-                if (hy) {
-                    if (!fy) { for (int e = 0; e < 3; ++e) x[e] = lv[3 * (threadIdx.x - g.BX) + e]; }
-                    else { for (int e = 0; e < 3; ++e) x[e] = xy[e]; }
-                    matvecSub(By, x, rhs);
-                }
-                if (hx) {
-                    if (!fx) { for (int e = 0; e < 3; ++e) x[e] = lv[3 * (threadIdx.x - 1) + e]; }
-                    else { for (int e = 0; e < 3; ++e) x[e] = xx[e]; }
-                    matvecSub(Bx, x, rhs);
-                }
-#pragma unroll
-                for (int e = 0; e < 3; ++e) { lv[3 * threadIdx.x + e] = rhs[e]; v[3 * c.I + e] = rhs[e]; }
+            if (myLev == lev) {
+                const S sz = rowDot(Lz, sV + 3 * (p - PXY));
+                const S sy = rowDot(Ly, sV + 3 * (p - PX));
+                const S sx = rowDot(Lx, sV + 3 * (p - 1));
+                if (hz) r += S(-1) * sz;
+                if (hy) r += S(-1) * sy;
+                if (hx) r += S(-1) * sx;
+                if (q < 3) sV[3 * p + e] = r;
             }
             __syncthreads();
         }
+        if (word >= 0 && q < 3) stCoh(v + 3 * Ic + e, r);
         publish(I.done, task.tile, epoch);
+        t = shareTicket(next, &sTicket);
     }
 }
 
 // Backward sweep: v = U^-1 v with the inverted (FP64) pivots; upper blocks +x,+y,+z.
 template <class S>
-__global__ __launch_bounds__(kTileThreads) void k_ilu_bwd_set(const IluInst* __restrict__ insts,
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
 {
-    __shared__ S lv[kTileThreads * 3];
+    __shared__ S sV[kPadMax * 3];          // results, ghosts at local +B
+    __shared__ int sOrder[kTileThreads];
     __shared__ int sTicket;
+    const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
+    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, base) : 0, &sTicket);
     for (;;) {
-        const int t = nextTicket(counter, base, &sTicket);
         if (t < 0 || t >= T) break;
+        int next = 0;
+        if (threadIdx.x == 0) next = drawTicket(counter, base);
         const TileTask task = order[T - 1 - t];
         const IluInst I = insts[task.inst];
-        if (!isActive(I)) { publish(I.done, task.tile, epoch); continue; }
+        if (!isActive(I)) {
+            publish(I.done, task.tile, epoch);
+            t = shareTicket(next, &sTicket);
+            continue;
+        }
         const TileGeom& g = I.g;
+        const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
         const S* __restrict__ lu = (const S*)I.lu;
         const double* __restrict__ dinv = I.dinv;
         S* __restrict__ v = (S*)I.v;
         const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
         tileOf(g, task.tile, TI, TJ, TK);
-        const CellCtx c = cellOf(g, TI, TJ, TK);
-        S rhs[3] = {S(0), S(0), S(0)};
-        S Bx[9], By[9], Bz[9];
-        double D[9];
-        const bool hx = c.i + 1 < g.nx, hy = c.j + 1 < g.ny, hz = c.k + 1 < g.nz;
-        if (c.valid) {
-#pragma unroll
-            for (int e = 0; e < 3; ++e) rhs[e] = v[3 * c.I + e];
-            long o = I.diag[c.I] + 1;
-            if (hx) { for (int q = 0; q < 9; ++q) Bx[q] = lu[9 * o + q]; ++o; }
-            if (hy) { for (int q = 0; q < 9; ++q) By[q] = lu[9 * o + q]; ++o; }
-            if (hz) { for (int q = 0; q < 9; ++q) Bz[q] = lu[9 * o + q]; }
-#pragma unroll
-            for (int q = 0; q < 9; ++q) D[q] = dinv[9 * c.I + q];
+        if (threadIdx.x < BX * BY * BZ) {
+            const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
+            const int i = TI * BX + li, j = TJ * BY + lj, k = TK * BZ + lk;
+            const int a = BX - 1 - li, b = BY - 1 - lj, c = BZ - 1 - lk;
+            sOrder[cellsBelow(a + b + c, BX, BY, BZ) + rankInLevel(a, b, c, BX, BY)] =
+                (i < g.nx && j < g.ny && k < g.nz)
+                    ? packCell(li, lj, lk, i + 1 < g.nx, j + 1 < g.ny, k + 1 < g.nz) : -1;
+        }
+        __syncthreads();
+        const int word = pos < BX * BY * BZ ? sOrder[pos] : -1;
+        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
+        const bool ux = word >= 0 && (word & (1 << 27)), uy = word >= 0 && (word & (1 << 28));
+        const bool uz = word >= 0 && (word & (1 << 29));
+        const long Ic = (long)(TI * BX + li) + (long)g.nx * ((long)(TJ * BY + lj) + (long)g.ny * (TK * BZ + lk));
+        const int p = li + PX * lj + PXY * lk;
+        const int myLev = word >= 0 ? (BX - 1 - li) + (BY - 1 - lj) + (BZ - 1 - lk) : -1;
+        S Ux[3] = {}, Uy[3] = {}, Uz[3] = {};
+        double D[3] = {};
+        S r = S(0);
+        if (word >= 0) {
+            r = v[3 * Ic + e];
+            long o = I.diag[Ic] + 1;
+            if (ux) { for (int w = 0; w < 3; ++w) Ux[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (uy) { for (int w = 0; w < 3; ++w) Uy[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (uz) { for (int w = 0; w < 3; ++w) Uz[w] = lu[9 * o + 3 * e + w]; }
+            for (int w = 0; w < 3; ++w) D[w] = dinv[9 * Ic + 3 * e + w];
         }
         waitDeps(g, TI, TJ, TK, +1, I.done, epoch, err);
-        S xx[3], xy[3], xz[3];
-        const bool fx = c.valid && hx && c.li + 1 == g.BX, fy = c.valid && hy && c.lj + 1 == g.BY;
-        const bool fz = c.valid && hz && c.lk + 1 == g.BZ;
-        if (fx) { for (int e = 0; e < 3; ++e) xx[e] = v[3 * (c.I + 1) + e]; }
-        if (fy) { for (int e = 0; e < 3; ++e) xy[e] = v[3 * (c.I + g.nx) + e]; }
-        if (fz) { for (int e = 0; e < 3; ++e) xz[e] = v[3 * (c.I + nxy) + e]; }
-        const int myLev = (g.BX - 1 - c.li) + (g.BY - 1 - c.lj) + (g.BZ - 1 - c.lk);
-        const int nLev = g.BX + g.BY + g.BZ - 2;
+        if (q < 3) {
+            if (ux && li + 1 == BX) sV[3 * (p + 1) + e] = ldCoh(v + 3 * (Ic + 1) + e);
+            if (uy && lj + 1 == BY) sV[3 * (p + PX) + e] = ldCoh(v + 3 * (Ic + g.nx) + e);
+            if (uz && lk + 1 == BZ) sV[3 * (p + PXY) + e] = ldCoh(v + 3 * (Ic + nxy) + e);
+        }
+        __syncthreads();
+        const int nLev = BX + BY + BZ - 2;
+        S out = S(0);
         for (int lev = 0; lev <= nLev; ++lev) {
-            if (c.valid && myLev == lev) {
-                S x[3];
-                if (hx) {
-                    if (!fx) { for (int e = 0; e < 3; ++e) x[e] = lv[3 * (threadIdx.x + 1) + e]; }
-                    else { for (int e = 0; e < 3; ++e) x[e] = xx[e]; }
-                    matvecSub(Bx, x, rhs);
-                }
-                if (hy) {
-                    if (!fy) { for (int e = 0; e < 3; ++e) x[e] = lv[3 * (threadIdx.x + g.BX) + e]; }
-                    else { for (int e = 0; e < 3; ++e) x[e] = xy[e]; }
-                    matvecSub(By, x, rhs);
-                }
-                if (hz) {
-                    if (!fz) { for (int e = 0; e < 3; ++e) x[e] = lv[3 * (threadIdx.x + g.BX * g.BY) + e]; }
-                    else { for (int e = 0; e < 3; ++e) x[e] = xz[e]; }
-                    matvecSub(Bz, x, rhs);
-                }
-                S out[3];
-#pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    double s = D[e * 3] * (double)rhs[0];
-                    s += D[e * 3 + 1] * (double)rhs[1];
-                    s += D[e * 3 + 2] * (double)rhs[2];
-                    out[e] = S(0) + S(1) * (S)s;
-                }
-#pragma unroll
-                for (int e = 0; e < 3; ++e) { lv[3 * threadIdx.x + e] = out[e]; v[3 * c.I + e] = out[e]; }
+            if (myLev == lev) {
+                const S sx = rowDot(Ux, sV + 3 * (p + 1));
+                const S sy = rowDot(Uy, sV + 3 * (p + PX));
+                const S sz = rowDot(Uz, sV + 3 * (p + PXY));
+                if (ux) r += S(-1) * sx;
+                if (uy) r += S(-1) * sy;
+                if (uz) r += S(-1) * sz;
+                const S r0 = quadLane<0>(r), r1 = quadLane<1>(r), r2 = quadLane<2>(r);
+                double w = D[0] * (double)r0;
+                w += D[1] * (double)r1;
+                w += D[2] * (double)r2;
+                out = S(0) + S(1) * (S)w;
+                if (q < 3) sV[3 * p + e] = out;
             }
             __syncthreads();
         }
+        if (word >= 0 && q < 3) stCoh(v + 3 * Ic + e, out);
         publish(I.done, task.tile, epoch);
+        t = shareTicket(next, &sTicket);
     }
 }
 
@@ -321,12 +419,18 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
 {
     __shared__ double ld[kTileThreads * 9];
     __shared__ int sTicket;
+    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, base) : 0, &sTicket);
     for (;;) {
-        const int t = nextTicket(counter, base, &sTicket);
         if (t < 0 || t >= T) break;
+        int next = 0;
+        if (threadIdx.x == 0) next = drawTicket(counter, base);
         const TileTask task = order[t];
         const IluInst I = insts[task.inst];
-        if (!isActive(I)) { publish(I.done, task.tile, epoch); continue; }
+        if (!isActive(I)) {
+            publish(I.done, task.tile, epoch);
+            t = shareTicket(next, &sTicket);
+            continue;
+        }
         const TileGeom& g = I.g;
         double* __restrict__ lu = (double*)I.luw;
         double* __restrict__ dinv = I.dinvw;
@@ -365,9 +469,9 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
         // pivots of the neighbours across the tile faces, one round trip
         const bool fz = c.valid && hz && c.lk == 0, fy = c.valid && hy && c.lj == 0, fx = c.valid && hx && c.li == 0;
         double Pz[9], Py[9], Px[9];
-        if (fz) { for (int q = 0; q < 9; ++q) Pz[q] = dinv[9 * (c.I - nxy) + q]; }
-        if (fy) { for (int q = 0; q < 9; ++q) Py[q] = dinv[9 * (c.I - g.nx) + q]; }
-        if (fx) { for (int q = 0; q < 9; ++q) Px[q] = dinv[9 * (c.I - 1) + q]; }
+        if (fz) { for (int q = 0; q < 9; ++q) Pz[q] = ldCoh(dinv + 9 * (c.I - nxy) + q); }
+        if (fy) { for (int q = 0; q < 9; ++q) Py[q] = ldCoh(dinv + 9 * (c.I - g.nx) + q); }
+        if (fx) { for (int q = 0; q < 9; ++q) Px[q] = ldCoh(dinv + 9 * (c.I - 1) + q); }
         const int myLev = c.li + c.lj + c.lk;
         const int nLev = g.BX + g.BY + g.BZ - 2;
         for (int lev = 0; lev <= nLev; ++lev) {
@@ -377,7 +481,7 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
                     if (!fz) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - g.BX * g.BY) + q]; }
                     else { for (int q = 0; q < 9; ++q) Dj[q] = Pz[q]; }
                     mat3mul(Az, Dj, L);
-                    for (int q = 0; q < 9; ++q) lu[9 * oz + q] = L[q];
+                    for (int q = 0; q < 9; ++q) Az[q] = L[q];
                     mat3mul(L, Uz, Tm);
                     for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
                 }
@@ -385,7 +489,7 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
                     if (!fy) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - g.BX) + q]; }
                     else { for (int q = 0; q < 9; ++q) Dj[q] = Py[q]; }
                     mat3mul(Ay, Dj, L);
-                    for (int q = 0; q < 9; ++q) lu[9 * oy + q] = L[q];
+                    for (int q = 0; q < 9; ++q) Ay[q] = L[q];
                     mat3mul(L, Uy, Tm);
                     for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
                 }
@@ -393,20 +497,29 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
                     if (!fx) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - 1) + q]; }
                     else { for (int q = 0; q < 9; ++q) Dj[q] = Px[q]; }
                     mat3mul(Ax, Dj, L);
-                    for (int q = 0; q < 9; ++q) lu[9 * ox + q] = L[q];
+                    for (int q = 0; q < 9; ++q) Ax[q] = L[q];
                     mat3mul(L, Ux, Tm);
                     for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
                 }
                 if (!inv3(D)) atomicOr(fail, 2);
-                for (int q = 0; q < 9; ++q) {
-                    ld[9 * threadIdx.x + q] = D[q];
-                    lu[9 * od + q] = D[q];
-                    dinv[9 * c.I + q] = D[q];
-                }
+                for (int q = 0; q < 9; ++q) ld[9 * threadIdx.x + q] = D[q];
             }
             __syncthreads();
         }
+        // the tile's outputs leave the workgroup once, after the level loop (a
+        // global store inside the loop would make every level barrier wait for
+        // its acknowledgement)
+        if (c.valid) {
+            if (hz) { for (int q = 0; q < 9; ++q) lu[9 * oz + q] = Az[q]; }
+            if (hy) { for (int q = 0; q < 9; ++q) lu[9 * oy + q] = Ay[q]; }
+            if (hx) { for (int q = 0; q < 9; ++q) lu[9 * ox + q] = Ax[q]; }
+            for (int q = 0; q < 9; ++q) {
+                lu[9 * od + q] = D[q];
+                stCoh(dinv + 9 * c.I + q, D[q]);
+            }
+        }
         publish(I.done, task.tile, epoch);
+        t = shareTicket(next, &sTicket);
     }
 }
 
@@ -421,6 +534,12 @@ TileGeom makeTileGeom(int nx, int ny, int nz)
     int by = rest / bx;
     if (nx < bx) { bx = 1; while (bx * 2 <= nx) bx *= 2; by = rest / bx; }
     if (ny < by) { by = 1; while (by * 2 <= ny) by *= 2; }
+    // the sweeps stage a tile plus one ghost layer in LDS
+    while ((bx + 1) * (by + 1) * (bz + 1) > kPadMax) {
+        if (by >= bx && by >= bz) by /= 2;
+        else if (bx >= bz) bx /= 2;
+        else bz /= 2;
+    }
     TileGeom g;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.BX = bx; g.BY = by; g.BZ = bz;
@@ -439,17 +558,23 @@ TiledIluSet::TiledIluSet(hipStream_t s)
     OM_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     OM_HIP(hipGetDeviceProperties(&prop, dev));
-    const void* ks[] = {(const void*)k_ilu_factor_set, (const void*)k_ilu_fwd_set<double>,
-                        (const void*)k_ilu_bwd_set<double>, (const void*)k_ilu_fwd_set<float>,
-                        (const void*)k_ilu_bwd_set<float>};
-    int perCU = 1 << 20;
-    for (const void* k : ks) {
-        int n = 0;
-        OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kTileThreads, 0));
-        perCU = std::min(perCU, n);
-    }
-    if (perCU < 1) throw std::runtime_error("tiled ILU kernels cannot be resident");
-    maxResident_ = perCU * prop.multiProcessorCount;
+    // co-resident workgroups of the factorization and of the sweeps (the
+    // factorization holds far more registers per thread)
+    auto resident = [&](std::initializer_list<const void*> ks, int threads) {
+        int perCU = 1 << 20;
+        for (const void* k : ks) {
+            int n = 0;
+            OM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0));
+            perCU = std::min(perCU, n);
+        }
+        if (perCU < 1) throw std::runtime_error("tiled ILU kernels cannot be resident");
+        return perCU * prop.multiProcessorCount;
+    };
+    residentFactor_ = resident({(const void*)k_ilu_factor_set}, kTileThreads);
+    residentSweep_[0][0] = resident({(const void*)k_ilu_fwd_set<double>}, kSweepThreads);
+    residentSweep_[0][1] = resident({(const void*)k_ilu_bwd_set<double>}, kSweepThreads);
+    residentSweep_[1][0] = resident({(const void*)k_ilu_fwd_set<float>}, kSweepThreads);
+    residentSweep_[1][1] = resident({(const void*)k_ilu_bwd_set<float>}, kSweepThreads);
 }
 
 void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s)
@@ -476,7 +601,7 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
     OM_HIP(hipMemsetAsync(done_.p, 0, std::max(T_, 1) * sizeof(int), s));
     // flags restart from zero: restart the epoch so that no stale value matches
     epoch_ = 0;
-    grid_ = std::min(T_, maxResident_);
+    gridFactor_ = std::min(T_, residentFactor_);
     dcache_.clear();
 }
 
@@ -512,14 +637,14 @@ void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream
     if (T_ == 0) return;
     const IluInst* di = upload(insts, s);
     long long b = ticketBase_;
-    ticketBase_ += T_ + grid_;   // every launch draws exactly T + grid tickets
+    ticketBase_ += T_ + gridFactor_;   // every launch draws exactly T + grid tickets
     int ep = ++epoch_;
     const TileTask* od = order_.p;
     int T = T_;
     unsigned long long* ct = counter_.p;
     int* er = err_.p;
     void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&fail};
-    OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_factor_set, dim3(grid_), dim3(kTileThreads), args, 0, s));
+    OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_factor_set, dim3(gridFactor_), dim3(kTileThreads), args, 0, s));
 }
 
 template <class S>
@@ -531,19 +656,15 @@ void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
     int T = T_;
     unsigned long long* ct = counter_.p;
     int* er = err_.p;
-    {
+    const int prec = sizeof(S) == 4 ? 1 : 0;
+    const void* kernels[2] = {(const void*)k_ilu_fwd_set<S>, (const void*)k_ilu_bwd_set<S>};
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        const int grid = std::min(T_, residentSweep_[prec][sweep]);
         long long b = ticketBase_;
-        ticketBase_ += T_ + grid_;   // every launch draws exactly T + grid tickets
+        ticketBase_ += T_ + grid;   // every launch draws exactly T + grid tickets
         int ep = ++epoch_;
         void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er};
-        OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_fwd_set<S>, dim3(grid_), dim3(kTileThreads), args, 0, s));
-    }
-    {
-        long long b = ticketBase_;
-        ticketBase_ += T_ + grid_;   // every launch draws exactly T + grid tickets
-        int ep = ++epoch_;
-        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er};
-        OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_bwd_set<S>, dim3(grid_), dim3(kTileThreads), args, 0, s));
+        OM_HIP(hipLaunchCooperativeKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
     }
 }
 template void TiledIluSet::apply<double>(const std::vector<IluInst>&, hipStream_t);
