@@ -11,8 +11,18 @@ A "step" is one implicit time step of one day (the Newton loop until
 convergence); value = Newton iterations per second over the timed steps.
 Kernel durations come from HIP events recorded on the library's stream.
 
+The same run also serves an ensemble request stream (the serving workload):
+black-oil realizations of 8k-250k cells with their own seeded permeability
+(sigma 0.5-2.0) and random well placement, Poisson arrivals, 10 time steps of
+at most one day each (halved on a failed Newton solve), admitted by HBM budget into the continuous-batching server and
+solved with FP32 inner / FP64 outer BiCGStab.  It reports solves/s and p50/p99
+latency (completion - arrival), and the plain FP64 single-reservoir path on the
+first requests of the same stream for comparison (solves/s, and the pressure
+difference of the served results against it).
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--config 500k|c2|c3]
-                  [--precision 0|1] [--no-cpu]
+                  [--precision 0|1] [--no-cpu] [--no-serve]
+                  [--serve-requests R] [--serve-rate REQ_PER_S] [--serve-plain P]
 """
 import argparse
 import json
@@ -32,6 +42,18 @@ CONFIGS = {
 }
 
 
+# HBM bytes per launch at the 500k workload, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, then WRITE_SIZE) scaled by what the same counters read for a
+# device copy of 256 MiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the commands and
+# per-kernel figures are in profiles/blackoil_500k_hbm_traffic.md.
+HBM_TRAFFIC_500K = {
+    "ilu_apply": 500703641,    # forward + backward sweep
+    "spmv": 630945894,
+    "assemble": 482937651,
+    "ilu_factor": 651998413,
+}
+
+
 def algorithmic_bytes(N, F, nnzb, n=3, s=8):
     """Per-launch algorithmic HBM bytes of each kernel class."""
     return {
@@ -44,6 +66,108 @@ def algorithmic_bytes(N, F, nnzb, n=3, s=8):
     }
 
 
+def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77):
+    """Poisson stream of uneven black-oil realizations through the server."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n_req):
+        n = rng.uniform(8e3, 250e3)
+        nz = int(rng.integers(5, 21))
+        nxy = max(4, int(round(np.sqrt(n / nz))))
+        spec = om.make_spec(nxy, nxy, nz, seed=10007 + k, wells=2, perm_sigma=float(rng.uniform(0.5, 2.0)))
+        cases.append(om.BlackOilCase(spec))
+    gaps = rng.exponential(1.0 / rate, n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    dt = 86400.0
+    # the reference's time-step control with steps of at most one day: a step
+    # whose Newton solve fails is retried at half the size
+    prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1)
+    budget = 64 * 2.0 ** 30
+    srv = om.Server(prm, budget, 64)
+    for c, a in zip(cases, arrivals):
+        srv.submit(c, n_steps, dt, float(a))
+    t0 = time.time()
+    srv.run()
+    wall = time.time() - t0
+    res = [srv.result(i) for i in range(n_req)]
+    lat = np.array([r["finish"] - r["arrival"] for r in res])
+    summ = srv.summary()
+    srv.close()
+    out = {
+        "requests": n_req, "steps_per_request": n_steps, "max_dt_days": 1.0,
+        "cells_min": int(min(c.N for c in cases)), "cells_max": int(max(c.N for c in cases)),
+        "cells_total": int(sum(c.N for c in cases)),
+        "arrival_rate_per_s": rate, "precision": "fp32-inner/fp64-outer", "newton_tol": 1e-7,
+        "hbm_budget_GiB": 64, "max_batch": 64, "max_running": summ["max_running"],
+        "batch_steps": summ["batch_steps"],
+        "wall_s": round(wall, 3), "solves_per_s": round(n_req / wall, 4),
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+        "failed": int(sum(1 for r in res if r["status"] != 0)),
+        "newton_iters": int(sum(r["stats"].newton_iters for r in res)),
+        "linear_iters": int(sum(r["stats"].linear_iters for r in res)),
+    }
+    if n_plain > 0:
+        # the plain FP64 drop-in path, one reservoir at a time, same requests
+        # and the same time-step control
+        pprm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=0)
+        t0 = time.time()
+        finals = []
+        for k in range(n_plain):
+            m = om.Model(cases[k])
+            m.set_solution(cases[k].pv0, cases[k].mean0)
+            rc, _, dts, _, _ = m.simulate(pprm, n_steps, dt, keep_history=False)
+            if rc != 0:
+                raise SystemExit("plain FP64 run of request %d failed rc=%d" % (k, rc))
+            finals.append((m.get_solution()[0], dts))
+            m.close()
+        pw = time.time() - t0
+        # accuracy at equal times: where the two runs chose different step
+        # sizes, the FP64 path is replayed with the served run's steps
+        errs, replayed = [], 0
+        for k in range(n_plain):
+            pv, dts = finals[k]
+            if not np.array_equal(dts, res[k]["dts"]):
+                replayed += 1
+                m = om.Model(cases[k])
+                m.set_solution(cases[k].pv0, cases[k].mean0)
+                for h in res[k]["dts"]:
+                    rc, _, _, _, _ = m.simulate(om.default_params(max_dt=h, min_dt=h, newton_tol=1e-7), 1, h,
+                                                keep_history=False)
+                    if rc != 0:
+                        raise SystemExit("replay of request %d failed rc=%d" % (k, rc))
+                pv = m.get_solution()[0]
+                m.close()
+            pp, ps = pv[1::3], res[k]["pv"][1::3]
+            errs.append(float(np.max(np.abs(ps - pp) / np.abs(pp))))
+        out["plain_fp64"] = {"requests": n_plain, "wall_s": round(pw, 3), "solves_per_s": round(n_plain / pw, 4)}
+        out["speedup_vs_plain_fp64"] = round(out["solves_per_s"] / (n_plain / pw), 3)
+        out["max_rel_pressure_diff_vs_plain_fp64"] = max(errs)
+        out["plain_replayed_with_served_steps"] = replayed
+    return out
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of a known size (the runtime's blit kernel):
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    n = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(C.byref(buf), C.c_size_t(n)) != 0:
+            raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +177,13 @@ def main():
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one device copy of MIB MiB (reference for HBM byte counters)")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-steps", type=int, default=10)
+    ap.add_argument("--serve-plain", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the reservoir would be split "
@@ -61,6 +192,9 @@ def main():
 
     import numpy as np
     import omodels_amd as om
+
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
 
     cfg = CONFIGS[args.config]
     spec = om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024)
@@ -102,7 +236,8 @@ def main():
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     asm = kernels.get("assemble")
     roof = {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(kernels[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None}
+            "unit": "GB/s", "frac": round(kernels[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": HBM_TRAFFIC_500K.get(dom) if args.config == "500k" and args.precision == 0 else None}
 
     line = {
         "metric": "Newton steps/s & assembly Mcells/s at 500k blackoil; ensemble p50/p99 latency",
@@ -149,6 +284,9 @@ def main():
             "wall_s": round(cw, 2), "newton_iters": cn,
             "assembly_mcells_per_s": round(N * (cn + args.cpu_steps) / max(tim[0], 1e-9) / 1e6, 2),
         }
+    if not args.no_serve:
+        line["ensemble"] = serve_workload(om, np, args.serve_requests, args.serve_rate, args.serve_steps,
+                                          min(args.serve_plain, args.serve_requests))
     print(json.dumps(line))
 
 
