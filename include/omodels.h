@@ -109,7 +109,8 @@ typedef struct omodels_params {
     double max_dt;              /* MaxTimeStepSize */
     double min_dt;              /* MinTimeStepSize */
     int32_t precision;          /* 0: FP64 solve; 1: FP32 Jacobian/ILU/inner BiCGStab
-                                   inside FP64 flexible outer refinement */
+                                   inside FP64 flexible outer refinement; 2: FP32 solve
+                                   (LinearSolverScalar = float, lens_immiscible_ecfv_ad.hh:53) */
     int32_t verbose;
     int32_t ilu_schedule;       /* 0: persistent tile-wavefront sweeps; 1: one launch per level */
     int32_t pad;
@@ -230,6 +231,56 @@ int omodels_server_kernel_times(omodels_server* s, double* ms, int64_t* counts, 
  * timers (newtonmethod.hh:240-262, timer.hh:48-215). */
 int omodels_model_set_profiling(omodels_model* m, int enable);
 int omodels_model_kernel_times(omodels_model* m, double* ms, int64_t* counts, int n);
+
+/* ------------------------------------------------------------------------ */
+/* Two-phase immiscible element path (config 1: the lens problem).
+ * Replaces, for ImmiscibleModel + EcfvDiscretization + AutoDiffLocalLinearizer
+ * (tests/lens_immiscible_ecfv_ad.hh:37-55): FvBaseLinearizer::linearize_ /
+ * linearizeElement_ (fvbaselinearizer.hh:510-626), FvBaseAdLocalLinearizer
+ * (fvbaseadlocallinearizer.hh:173-292), FvBaseLocalResidual::eval
+ * (fvbaselocalresidual.hh:158-626), the problem of tests/problems/lensproblem.hh:227-627
+ * and the Newton / time loop (newtonmethod.hh:228-465, fvbaseproblem.hh:512-606,
+ * simulator.hh:481-512,697-810).  2-D structured grid, cell I = i + nx*j,
+ * primary variables (p_w [Pa], S_w) per cell, 2x2 row-major blocks. */
+typedef struct omodels_lens_problem {
+    int32_t nx, ny;
+    double lx, ly;                                   /* domain size [m] */
+    double lens_x0, lens_y0, lens_x1, lens_y1;       /* lens box [m] (lensproblem.hh:130-148) */
+    double k_lens, k_outer;                          /* intrinsic permeability [m^2] */
+    double porosity;
+    double swr_lens, snr_lens, alpha_lens, n_lens;   /* regularized Van Genuchten, lens */
+    double swr_outer, snr_outer, alpha_outer, n_outer;
+    double rho_w, rho_n, mu_w, mu_n;                 /* incompressible liquids */
+    double gravity;                                  /* y component [m/s^2] */
+    double inflow_rate;                              /* DNAPL inflow on the top strip [kg/(m^2 s)] */
+    double p_atm;                                    /* pressure at the top of the free-flow sides [Pa] */
+    double eps;                                      /* geometric tolerance [m] */
+} omodels_lens_problem;
+
+typedef struct omodels_lens omodels_lens;
+
+/* the defaults of LensProblem (lensproblem.hh:254-262,301-320,370-373) */
+void omodels_lens_default_problem(omodels_lens_problem* p);
+omodels_lens* omodels_lens_create(const omodels_lens_problem* p);
+void omodels_lens_destroy(omodels_lens* m);
+int omodels_lens_num_cells(const omodels_lens* m, int64_t* n, int64_t* nnzb);
+/* createMatrix_ (fvbaselinearizer.hh:382-420): sorted columns; diag optional */
+int omodels_lens_pattern(omodels_lens* m, int32_t* row_ptr, int32_t* col_idx, int32_t* diag);
+/* LensProblem::initial (lensproblem.hh:550-582): water-saturated, hydrostatic */
+int omodels_lens_initial(omodels_lens* m, double* pv);
+int omodels_lens_set_solution(omodels_lens* m, const double* pv, int both_levels);
+int omodels_lens_get_solution(omodels_lens* m, double* pv);
+int omodels_lens_set_storage1(omodels_lens* m, const double* storage1);
+/* linearize() at the current solution with the storage cache of the
+ * previous level (computed at iteration 0 from the current solution) */
+int omodels_lens_linearize(omodels_lens* m, double dt, int iteration_index);
+int omodels_lens_get_jacobian(omodels_lens* m, double* values);
+int omodels_lens_get_residual(omodels_lens* m, double* residual);
+/* Simulator::run until t_end from dt0; t_hist / dt_hist have max_steps entries */
+int omodels_lens_simulate(omodels_lens* m, const omodels_params* prm, double t_end, double dt0,
+                          int max_steps, double* t_hist, double* dt_hist, int32_t* n_steps,
+                          omodels_iter_log* log, int log_cap, int32_t* log_len,
+                          omodels_run_stats* stats);
 
 #ifdef __cplusplus
 }

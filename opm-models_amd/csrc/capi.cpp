@@ -8,6 +8,7 @@
 #include <string>
 
 #include "ensemble_host.h"
+#include "lens_host.h"
 #include "omodels_host.h"
 
 using namespace omodels;
@@ -41,6 +42,9 @@ struct omodels_model {
 };
 struct omodels_server {
     std::unique_ptr<Server> s;
+};
+struct omodels_lens {
+    std::unique_ptr<LensModel> m;
 };
 struct omodels_solver {
     omodels_model* owner;
@@ -439,6 +443,142 @@ int omodels_model_kernel_times(omodels_model* h, double* ms, int64_t* counts, in
         t.collect();
         for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.ms[i]; counts[i] = t.count[i]; }
         return 0;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// two-phase immiscible element path (lens problem)
+
+void omodels_lens_default_problem(omodels_lens_problem* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->nx = 48; p->ny = 32;                         // CellsX/Y (lensproblem.hh:301-302)
+    p->lx = 6.0; p->ly = 4.0;                       // DomainSizeX/Y (:303-304)
+    p->lens_x0 = 1.0; p->lens_y0 = 2.0; p->lens_x1 = 4.0; p->lens_y1 = 3.0;   // (:130-148)
+    p->k_lens = 9.05e-12; p->k_outer = 4.6e-10;     // (:260-261)
+    p->porosity = 0.4;                              // (:370-373)
+    p->swr_lens = 0.18; p->snr_lens = 0.0; p->alpha_lens = 0.00045; p->n_lens = 7.3;      // (:246-258)
+    p->swr_outer = 0.05; p->snr_outer = 0.0; p->alpha_outer = 0.0037; p->n_outer = 4.7;
+    p->rho_w = 1000.0; p->rho_n = 1460.0;           // SimpleH2O / DNAPL liquid densities
+    p->mu_w = 1.0e-3; p->mu_n = 5.7e-4;             // and viscosities
+    p->gravity = -9.81;                             // (multiphasebaseproblem.hh:398-403)
+    p->inflow_rate = 0.04;                          // (:525-530)
+    p->p_atm = 1e5;
+    p->eps = 3e-6;                                  // (:232)
+}
+
+omodels_lens* omodels_lens_create(const omodels_lens_problem* p)
+{
+    omodels_lens* h = nullptr;
+    int rc = guarded([&] {
+        h = new omodels_lens{std::make_unique<LensModel>(p)};
+        return 0;
+    });
+    return rc == 0 ? h : nullptr;
+}
+
+void omodels_lens_destroy(omodels_lens* m) { delete m; }
+
+int omodels_lens_num_cells(const omodels_lens* h, int64_t* n, int64_t* nnzb)
+{
+    if (!h) return -2;
+    *n = h->m->N; *nnzb = h->m->nnzb;
+    return 0;
+}
+
+int omodels_lens_pattern(omodels_lens* h, int32_t* row_ptr, int32_t* col_idx, int32_t* diag)
+{
+    return guarded([&] {
+        const LensModel& m = *h->m;
+        std::copy(m.hRowPtr.begin(), m.hRowPtr.end(), row_ptr);
+        std::copy(m.hColIdx.begin(), m.hColIdx.end(), col_idx);
+        if (diag) std::copy(m.hDiag.begin(), m.hDiag.end(), diag);
+        return 0;
+    });
+}
+
+int omodels_lens_initial(omodels_lens* h, double* pv)
+{
+    return guarded([&] { h->m->initial(pv); return 0; });
+}
+
+int omodels_lens_set_solution(omodels_lens* h, const double* pv, int both_levels)
+{
+    return guarded([&] {
+        LensModel& m = *h->m;
+        m.pv0.upload(pv, 2 * m.N, m.stream);
+        if (both_levels) m.advanceTimeLevel();
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_lens_get_solution(omodels_lens* h, double* pv)
+{
+    return guarded([&] {
+        LensModel& m = *h->m;
+        m.pv0.download(pv, 2 * m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_lens_set_storage1(omodels_lens* h, const double* st)
+{
+    return guarded([&] {
+        LensModel& m = *h->m;
+        m.st1.upload(st, 2 * m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_lens_linearize(omodels_lens* h, double dt, int iteration_index)
+{
+    return guarded([&] {
+        h->m->linearizeDomain(dt, iteration_index);
+        h->m->sync();
+        return 0;
+    });
+}
+
+int omodels_lens_get_jacobian(omodels_lens* h, double* values)
+{
+    return guarded([&] {
+        LensModel& m = *h->m;
+        m.jac.download(values, 4 * (size_t)m.nnzb, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_lens_get_residual(omodels_lens* h, double* residual)
+{
+    return guarded([&] {
+        LensModel& m = *h->m;
+        m.res.download(residual, 2 * m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_lens_simulate(omodels_lens* h, const omodels_params* prm, double t_end, double dt0, int max_steps,
+                          double* t_hist, double* dt_hist, int32_t* n_steps, omodels_iter_log* log, int log_cap,
+                          int32_t* log_len, omodels_run_stats* stats)
+{
+    return guarded([&] {
+        std::vector<omodels_iter_log> lg;
+        omodels_run_stats st{};
+        int ns = 0;
+        const int rc = simulateLens(*h->m, *prm, t_end, dt0, max_steps, t_hist, dt_hist, &ns, &lg, &st);
+        if (n_steps) *n_steps = ns;
+        if (log) {
+            const int n = std::min<int>(log_cap, (int)lg.size());
+            std::copy(lg.begin(), lg.begin() + n, log);
+        }
+        if (log_len) *log_len = (int32_t)lg.size();
+        if (stats) *stats = st;
+        return rc;
     });
 }
 

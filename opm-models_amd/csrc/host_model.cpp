@@ -23,14 +23,13 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
 {
     if (!p || p->nx < 1 || p->ny < 1 || p->nz < 1) throw std::invalid_argument("invalid grid");
     if (!p->perm || !p->poro || !p->oil_tab || !p->gas_tab) throw std::invalid_argument("missing problem arrays");
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("no HIP device available");
-    if (s) stream = s;
-    else { OM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); ownStream = true; }
+    initSystem(s);
     fluid = makeFluid(p);
     buildTopology(p, topo);
     N = topo.N;
     nnzb = topo.rowPtr[N];
+    bs = 3;
+    gx = p->nx; gy = p->ny; gz = p->nz;
     G.nx = p->nx; G.ny = p->ny; G.nz = p->nz;
     G.dx = p->dx; G.dy = p->dy; G.dz = p->dz; G.g = p->gravity;
     G.V = p->dx * p->dy * p->dz;
@@ -59,21 +58,29 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
     OM_HIP(hipMemsetAsync(wasSw.p, 0, N, stream));
     OM_HIP(hipMemsetAsync(st1.p, 0, 3 * N * sizeof(double), stream));
     counters.alloc(4);
-    partial.alloc(2 * kRedBlocks);
-    scal.alloc(8);
-    OM_HIP(hipHostMalloc((void**)&hostScal, 8 * sizeof(double), hipHostMallocDefault));
     ensureJacobian();
     sync();
 }
 
-Model::~Model()
+void LinearSystem::initSystem(hipStream_t s)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("no HIP device available");
+    if (s) stream = s;
+    else { OM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); ownStream = true; }
+    partial.alloc(2 * kRedBlocks);
+    scal.alloc(8);
+    OM_HIP(hipHostMalloc((void**)&hostScal, 8 * sizeof(double), hipHostMallocDefault));
+}
+
+LinearSystem::~LinearSystem()
 {
     if (stream) (void)hipStreamSynchronize(stream);
     if (hostScal) (void)hipHostFree(hostScal);
     if (ownStream && stream) (void)hipStreamDestroy(stream);
 }
 
-void Model::sync() { OM_HIP(hipStreamSynchronize(stream)); }
+void LinearSystem::sync() { OM_HIP(hipStreamSynchronize(stream)); }
 
 void Model::ensureJacobian()
 {
@@ -96,9 +103,9 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     OM_HIP(hipGetLastError());
 }
 
-double Model::maxAbsResidual()
+double LinearSystem::maxAbsResidual()
 {
-    launchMaxAbs<double>(3 * N, res.p, partial.p, scal.p, stream);
+    launchMaxAbs<double>((long)bs * N, res.p, partial.p, scal.p, stream);
     OM_HIP(hipMemcpyAsync(hostScal, scal.p, sizeof(double), hipMemcpyDeviceToHost, stream));
     sync();
     return hostScal[0];
@@ -117,15 +124,18 @@ void Model::restoreTimeLevel()
 }
 
 // ---------------------------------------------------------------------------
-BiCGStabBackend::BiCGStabBackend(Model& m, const omodels_params& prm) : m_(m), prm_(prm)
+BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m_(m), prm_(prm)
 {
-    const long n = 3 * m.N;
+    const long n = (long)m.bs * m.N;
+    if (prm_.precond != 0 && m.bs != 3) throw std::invalid_argument("block Jacobi is built for 3x3 blocks");
     b64_.alloc(n); x64_.alloc(n);
     ks_.alloc(1);
     fail_.alloc(1);
     OM_HIP(hipHostMalloc((void**)&ksHost_, sizeof(KrylovScalars), hipHostMallocDefault));
-    if (prm_.precond == 0 && prm_.ilu_schedule == 0)
-        tiled_ = std::make_unique<TiledIlu>(m.G.nx, m.G.ny, m.G.nz, m.stream);
+    if (prm_.precision < 0 || prm_.precision > 2) throw std::invalid_argument("precision must be 0, 1 or 2");
+    if (prm_.precision == 2 && prm_.precond != 0) throw std::invalid_argument("the FP32 solve uses block ILU(0)");
+    if (prm_.precond == 0 && prm_.ilu_schedule == 0 && m.bs == 3 && prm_.precision != 2)
+        tiled_ = std::make_unique<TiledIlu>(m.gx, m.gy, m.gz, m.stream);
     if (prm_.precision == 0) {
         for (auto& w : wd_) w.alloc(n);
         r64_.alloc(n);
@@ -147,37 +157,51 @@ BiCGStabBackend::~BiCGStabBackend()
 void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidual)
 {
     hipStream_t s = m_.stream;
-    const long n = 3 * m_.N, nnzb = m_.nnzb;
-    if (hostValues) { Acopy_.upload(hostValues, 9 * nnzb, s); A64_ = Acopy_.p; }
+    const int bs = m_.bs, bb = bs * bs;
+    const long n = (long)bs * m_.N, nnzb = m_.nnzb;
+    if (hostValues) { Acopy_.upload(hostValues, bb * nnzb, s); A64_ = Acopy_.p; }
     else A64_ = m_.jac.p;
     if (hostResidual) b64_.upload(hostResidual, n, s);
     else OM_HIP(hipMemcpyAsync(b64_.p, m_.res.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
     const int N = (int)m_.N;
-    if (prm_.precond == 0) {
+    if (prm_.precision == 2) {
+        // the reference's LinearSolverScalar = float: matrix, factorization and
+        // Krylov solve all in FP32 (the inverted FP32 pivots are kept as FP64
+        // numbers, which represents them exactly)
+        Af_.alloc(bb * nnzb);
+        launchCopyConvertD<float>(bb * nnzb, A64_, Af_.p, s);
+        luf_.alloc(bb * nnzb);
+        launchCopyConvertD<float>(bb * nnzb, A64_, luf_.p, s);
+        dinv64_.alloc(bb * (size_t)N);
+        m_.timer.begin(kClsIluFactor, s);
+        launchIlu0Factor<float>(N, bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, luf_.p, m_.levels, fail_.p, s);
+        launchExtractInvDiag<float>(N, bs, m_.diagOff.p, luf_.p, dinv64_.p, s);
+        m_.timer.end(s);
+    } else if (prm_.precond == 0) {
         // ILU(0) factorization in FP64 (pivot blocks stay FP64 in both forms)
-        lu64_.alloc(9 * nnzb);
-        OM_HIP(hipMemcpyAsync(lu64_.p, A64_, 9 * nnzb * sizeof(double), hipMemcpyDeviceToDevice, s));
-        dinv64_.alloc(9 * (size_t)N);
+        lu64_.alloc(bb * nnzb);
+        OM_HIP(hipMemcpyAsync(lu64_.p, A64_, bb * nnzb * sizeof(double), hipMemcpyDeviceToDevice, s));
+        dinv64_.alloc(bb * (size_t)N);
         m_.timer.begin(kClsIluFactor, s);
         if (tiled_) {
             tiled_->factor(m_.rowPtr.p, m_.diagOff.p, lu64_.p, dinv64_.p, fail_.p, s);
         } else {
-            launchIlu0Factor<double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu64_.p, m_.levels, fail_.p, s);
-            launchExtractInvDiag<double>(N, 3, m_.diagOff.p, lu64_.p, dinv64_.p, s);
+            launchIlu0Factor<double>(N, bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu64_.p, m_.levels, fail_.p, s);
+            launchExtractInvDiag<double>(N, bs, m_.diagOff.p, lu64_.p, dinv64_.p, s);
         }
         m_.timer.end(s);
         if (prm_.precision == 1) {
-            luf_.alloc(9 * nnzb);
-            launchCopyConvertD<float>(9 * nnzb, lu64_.p, luf_.p, s);
+            luf_.alloc(bb * nnzb);
+            launchCopyConvertD<float>(bb * nnzb, lu64_.p, luf_.p, s);
         }
     } else {
-        dinv64_.alloc(9 * (size_t)N);
-        launchJacobiInvert<double>(N, 3, m_.diagOff.p, A64_, dinv64_.p, fail_.p, s);
+        dinv64_.alloc(bb * (size_t)N);
+        launchJacobiInvert<double>(N, bs, m_.diagOff.p, A64_, dinv64_.p, fail_.p, s);
     }
     if (prm_.precision == 1) {
-        Af_.alloc(9 * nnzb);
-        launchCopyConvertD<float>(9 * nnzb, A64_, Af_.p, s);
+        Af_.alloc(bb * nnzb);
+        launchCopyConvertD<float>(bb * nnzb, A64_, Af_.p, s);
     }
     OM_HIP(hipGetLastError());
     int f = 0;
@@ -197,8 +221,8 @@ void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v)
         if (tiled_)
             tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s);
         else
-            launchIlu0Apply<S, double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv, d, v,
-                                       m_.levels, s);
+            launchIlu0Apply<S, double>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv,
+                                       d, v, m_.levels, s);
         m_.timer.end(s);
     } else {
         (void)lu;
@@ -214,7 +238,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
                               double absTol, int maxIter, int* its)
 {
     hipStream_t s = m_.stream;
-    const long m = 3 * m_.N;
+    const long m = (long)m_.bs * m_.N;
     const int N = (int)m_.N;
     S* r; S* p; S* v; S* y; S* z;
     if constexpr (sizeof(S) == 8) { r = (S*)wd_[0].p; p = (S*)wd_[1].p; v = (S*)wd_[2].p; y = (S*)wd_[3].p; z = (S*)wd_[4].p; }
@@ -242,7 +266,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
         precApply<S>(lu, dinv, p, y);
         m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, 3, m_.rowPtr.p, m_.colIdx.p, A, y, v, s);
+        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s);
         m_.timer.end(s);
         launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, tag, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
@@ -257,7 +281,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         if (K.stag_half || K.resnorm > prm_.lin_max_error) return 1;
         precApply<S>(lu, dinv, r, z);
         m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, 3, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
+        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
         m_.timer.end(s);
         launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, tag + 1, s);
         launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, tag + 1, s);
@@ -280,7 +304,7 @@ int BiCGStabBackend::solveImpl()
 {
     if (!prepared_) return 2;   // preconditioner could not be created (NumericalProblem)
     hipStream_t s = m_.stream;
-    const long m = 3 * m_.N;
+    const long m = (long)m_.bs * m_.N;
     const double absTol = prm_.lin_abs_tol < 0 ? prm_.newton_tol / 100.0 : prm_.lin_abs_tol;
     iters_ = 0;
     if (prm_.precision == 0) {
@@ -288,6 +312,16 @@ int BiCGStabBackend::solveImpl()
         int rc = bicgstab<double>(A64_, lu64_.p, dinv64_.p, b64_.p, x64_.p, prm_.lin_tol, absTol,
                                   prm_.lin_max_iter, &its);
         iters_ = its;
+        return rc;
+    }
+    if (prm_.precision == 2) {
+        launchCopyConvertD<float>(m, b64_.p, bf_.p, s);
+        int its = 0;
+        int rc = bicgstab<float>(Af_.p, luf_.p, dinv64_.p, bf_.p, xf_.p, prm_.lin_tol, absTol, prm_.lin_max_iter,
+                                 &its);
+        iters_ = its;
+        OM_HIP(hipMemsetAsync(x64_.p, 0, m * sizeof(double), s));
+        launchAddFromFloat(m, xf_.p, x64_.p, s);
         return rc;
     }
     // FP32 inner BiCGStab inside FP64 iterative refinement: the outer residual
@@ -313,7 +347,7 @@ int BiCGStabBackend::solveImpl()
         total += its;
         if (rc == 2) { iters_ = total; return 2; }
         launchAddFromFloat(m, xf_.p, x64_.p, s);
-        launchResidualUpdate((int)m_.N, 3, m_.rowPtr.p, m_.colIdx.p, A64_, x64_.p, b64_.p, r64_.p, s);
+        launchResidualUpdate((int)m_.N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A64_, x64_.p, b64_.p, r64_.p, s);
         launchMaxAbs<double>(m, r64_.p, m_.partial.p, m_.scal.p, s);
         OM_HIP(hipMemcpyAsync(m_.hostScal, m_.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
         m_.sync();
@@ -328,13 +362,13 @@ int BiCGStabBackend::solveImpl()
 
 void BiCGStabBackend::downloadIlu(double* hostValues)
 {
-    lu64_.download(hostValues, 9 * (size_t)m_.nnzb, m_.stream);
+    lu64_.download(hostValues, (size_t)m_.bs * m_.bs * m_.nnzb, m_.stream);
     m_.sync();
 }
 
 void BiCGStabBackend::applyPrecond(const double* hostD, double* hostV)
 {
-    const long m = 3 * m_.N;
+    const long m = (long)m_.bs * m_.N;
     DevBuf<double> d, v;
     d.upload(hostD, m, m_.stream);
     v.alloc(m);

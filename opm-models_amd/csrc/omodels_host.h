@@ -31,31 +31,51 @@ void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::v
 void buildTopology(const omodels_blackoil_problem* p, HostTopology& T);
 
 // ---------------------------------------------------------------------------
-class Model {
-public:
-    explicit Model(const omodels_blackoil_problem* prob, hipStream_t stream = nullptr);
-    ~Model();
-
+// The linearized system as the linear-solver backend sees it (the
+// linearizer's jacobian() and residual(), tpfalinearizer.hh:893-896): block
+// CSR pattern, FP64 values and right-hand side on the device, plus the level
+// schedule of the pattern and the stream everything runs on.
+struct LinearSystem {
     long N = 0, nnzb = 0;
-    BoGrid G{};
-    BoFluid fluid{};
-    HostTopology topo;
+    int bs = 3;                  // block size (3 black-oil, 2 two-phase)
+    int gx = 0, gy = 0, gz = 0;  // structured grid sizes (tile-wavefront ILU, bs 3)
     hipStream_t stream = nullptr;
     bool ownStream = false;
-
-    // device state
-    DevBuf<double> trans, poro, pv0, pv1, st1, res, jac;
-    DevBuf<uint8_t> mean0, mean1, wasSw;
-    DevBuf<int> rowPtr, colIdx, diagOff, nbOff;
-    DevBuf<int> srcCell, srcType;
-    DevBuf<double> srcA, srcB;
+    DevBuf<double> res, jac;
+    DevBuf<int> rowPtr, colIdx, diagOff;
     LevelSchedule levels;
-    int nsrc = 0;
-    DevBuf<int> counters;        // [0] switched, [1] non-finite update
     DevBuf<double> partial;      // reduction partials
     DevBuf<double> scal;         // device scalars (norms)
     double* hostScal = nullptr;  // pinned mirror
     KernelTimer timer;
+
+    LinearSystem() = default;
+    LinearSystem(const LinearSystem&) = delete;
+    LinearSystem& operator=(const LinearSystem&) = delete;
+    virtual ~LinearSystem();
+    void sync();
+    double maxAbsResidual();
+
+protected:
+    void initSystem(hipStream_t s);   // stream and reduction scratch
+};
+
+class Model : public LinearSystem {
+public:
+    explicit Model(const omodels_blackoil_problem* prob, hipStream_t stream = nullptr);
+
+    BoGrid G{};
+    BoFluid fluid{};
+    HostTopology topo;
+
+    // device state
+    DevBuf<double> trans, poro, pv0, pv1, st1;
+    DevBuf<uint8_t> mean0, mean1, wasSw;
+    DevBuf<int> nbOff;
+    DevBuf<int> srcCell, srcType;
+    DevBuf<double> srcA, srcB;
+    int nsrc = 0;
+    DevBuf<int> counters;        // [0] switched, [1] non-finite update
 
     // TpfaLinearizer
     void linearizeDomain(double dt, int iterationIndex);
@@ -64,8 +84,6 @@ public:
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)
     void advanceTimeLevel();
     void restoreTimeLevel();
-    void sync();
-    double maxAbsResidual();
 };
 
 struct SolverStats {
@@ -74,7 +92,7 @@ struct SolverStats {
 
 class BiCGStabBackend {
 public:
-    BiCGStabBackend(Model& m, const omodels_params& prm);
+    BiCGStabBackend(LinearSystem& m, const omodels_params& prm);
     ~BiCGStabBackend();
     // prepare(J, r) + setResidual(r): uses the model's Jacobian / residual
     void prepare(const double* hostValues = nullptr, const double* hostResidual = nullptr);
@@ -91,10 +109,9 @@ private:
     template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v);
     int solveImpl();
 
-    Model& m_;
+    LinearSystem& m_;
     omodels_params prm_;
     int iters_ = 0;
-    int bs_ = 3;
     const double* A64_ = nullptr;        // FP64 matrix used by the solver
     DevBuf<double> Acopy_;               // host-provided matrix
     DevBuf<double> b64_, x64_, r64_;

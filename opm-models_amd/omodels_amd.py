@@ -68,6 +68,20 @@ class RunStats(C.Structure):
                 ("t_update", C.c_double), ("t_total", C.c_double)]
 
 
+class LensProblem(C.Structure):
+    """omodels_lens_problem: the two-phase immiscible lens problem."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("lx", C.c_double), ("ly", C.c_double),
+        ("lens_x0", C.c_double), ("lens_y0", C.c_double), ("lens_x1", C.c_double), ("lens_y1", C.c_double),
+        ("k_lens", C.c_double), ("k_outer", C.c_double), ("porosity", C.c_double),
+        ("swr_lens", C.c_double), ("snr_lens", C.c_double), ("alpha_lens", C.c_double), ("n_lens", C.c_double),
+        ("swr_outer", C.c_double), ("snr_outer", C.c_double), ("alpha_outer", C.c_double),
+        ("n_outer", C.c_double),
+        ("rho_w", C.c_double), ("rho_n", C.c_double), ("mu_w", C.c_double), ("mu_n", C.c_double),
+        ("gravity", C.c_double), ("inflow_rate", C.c_double), ("p_atm", C.c_double), ("eps", C.c_double),
+    ]
+
+
 # every symbol declared in include/omodels.h, with its ctypes signature
 SIGNATURES = {
     "omodels_blackoil_default_fluid": (C.c_int, [_dp, _ip, _dp, _ip, _dp]),
@@ -97,6 +111,20 @@ SIGNATURES = {
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
     "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
+    "omodels_lens_default_problem": (None, [C.POINTER(LensProblem)]),
+    "omodels_lens_create": (C.c_void_p, [C.POINTER(LensProblem)]),
+    "omodels_lens_destroy": (None, [C.c_void_p]),
+    "omodels_lens_num_cells": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "omodels_lens_pattern": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "omodels_lens_initial": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_lens_set_solution": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "omodels_lens_get_solution": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_lens_set_storage1": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_lens_linearize": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "omodels_lens_get_jacobian": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_lens_get_residual": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_lens_simulate": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                        _ip, C.POINTER(IterLog), C.c_int, _ip, C.POINTER(RunStats)]),
     "omodels_server_create": (C.c_void_p, [C.POINTER(Params), C.c_double, C.c_int]),
     "omodels_server_destroy": (None, [C.c_void_p]),
     "omodels_server_submit": (C.c_int, [C.c_void_p, C.POINTER(BlackOilProblem), _dp, _u8p, C.c_int, C.c_double,
@@ -438,3 +466,88 @@ class Server:
         check(lib().omodels_server_kernel_times(self.h, ptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), 5),
               "server_kernel_times")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(Model.KERNEL_CLASSES)}
+
+
+# ---------------------------------------------------------------------------
+# two-phase immiscible element path (the lens problem)
+
+def lens_problem(**kw):
+    p = LensProblem()
+    lib().omodels_lens_default_problem(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class LensModel:
+    """Device-resident lens problem with the linearizer interface and the Newton / time loop."""
+
+    def __init__(self, prob=None):
+        self.prob = prob if prob is not None else lens_problem()
+        h = lib().omodels_lens_create(C.byref(self.prob))
+        if not h:
+            raise OmodelsError("lens_create failed: " + lib().omodels_last_error().decode())
+        self.h = C.c_void_p(h)
+        n = C.c_int64()
+        z = C.c_int64()
+        lib().omodels_lens_num_cells(self.h, C.byref(n), C.byref(z))
+        self.N, self.nnzb = n.value, z.value
+
+    def close(self):
+        if self.h:
+            lib().omodels_lens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pattern(self):
+        rp = np.zeros(self.N + 1, np.int32)
+        ci = np.zeros(self.nnzb, np.int32)
+        dg = np.zeros(self.N, np.int32)
+        check(lib().omodels_lens_pattern(self.h, ptr(rp, _ip), ptr(ci, _ip), ptr(dg, _ip)), "lens_pattern")
+        return rp, ci, dg
+
+    def initial(self):
+        pv = np.zeros(2 * self.N)
+        check(lib().omodels_lens_initial(self.h, ptr(pv)), "lens_initial")
+        return pv
+
+    def set_solution(self, pv, both_levels=True):
+        pv = np.ascontiguousarray(pv, dtype=np.float64)
+        check(lib().omodels_lens_set_solution(self.h, ptr(pv), int(both_levels)), "lens_set_solution")
+
+    def get_solution(self):
+        pv = np.zeros(2 * self.N)
+        check(lib().omodels_lens_get_solution(self.h, ptr(pv)), "lens_get_solution")
+        return pv
+
+    def set_storage1(self, st):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        check(lib().omodels_lens_set_storage1(self.h, ptr(st)), "lens_set_storage1")
+
+    def linearize(self, dt, iteration_index=1):
+        check(lib().omodels_lens_linearize(self.h, float(dt), int(iteration_index)), "lens_linearize")
+        r = np.zeros(2 * self.N)
+        J = np.zeros(4 * self.nnzb)
+        check(lib().omodels_lens_get_residual(self.h, ptr(r)), "lens_get_residual")
+        check(lib().omodels_lens_get_jacobian(self.h, ptr(J)), "lens_get_jacobian")
+        return r, J
+
+    def simulate(self, params, t_end, dt0, max_steps=10000, log_cap=100000):
+        th = np.zeros(max_steps)
+        dth = np.zeros(max_steps)
+        ns = C.c_int32()
+        log = (IterLog * log_cap)()
+        nlog = C.c_int32()
+        st = RunStats()
+        rc = check(lib().omodels_lens_simulate(self.h, C.byref(params), float(t_end), float(dt0), int(max_steps),
+                                               ptr(th), ptr(dth), C.byref(ns), log, log_cap, C.byref(nlog),
+                                               C.byref(st)), "lens_simulate")
+        recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
+                for l in log[:min(nlog.value, log_cap)]]
+        k = ns.value
+        return rc, th[:k], dth[:k], recs, st

@@ -29,6 +29,7 @@
  * Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
  */
 #include "oracle.h"
+#include "ad.h"
 
 #include <algorithm>
 #include <cmath>
@@ -45,41 +46,7 @@
 
 namespace {
 
-/* ------------------------------------------------------------------ */
-/* Forward-mode AD scalar (restates DenseAd::Evaluation semantics as used by
- * fvbaseadlocallinearizer.hh:62-72 and blackoilprimaryvariables.hh:217-233). */
-template <int N>
-struct Ev {
-    double v;
-    double d[N];
-    Ev() : v(0.0) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
-    Ev(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
-    static Ev var(double x, int idx) { Ev r(x); r.d[idx] = 1.0; return r; }
-};
-template <int N> inline Ev<N> operator+(const Ev<N>& a, const Ev<N>& b)
-{ Ev<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
-template <int N> inline Ev<N> operator-(const Ev<N>& a, const Ev<N>& b)
-{ Ev<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
-template <int N> inline Ev<N> operator*(const Ev<N>& a, const Ev<N>& b)
-{ Ev<N> r; r.v = a.v * b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
-template <int N> inline Ev<N> operator/(const Ev<N>& a, const Ev<N>& b)
-{
-    Ev<N> r; const double inv = 1.0 / b.v; r.v = a.v * inv;
-    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
-    return r;
-}
-template <int N> inline Ev<N> operator+(const Ev<N>& a, double s)
-{ Ev<N> r = a; r.v = a.v + s; return r; }
-template <int N> inline Ev<N> operator-(const Ev<N>& a, double s)
-{ Ev<N> r = a; r.v = a.v - s; return r; }
-template <int N> inline Ev<N> operator-(double s, const Ev<N>& a)
-{ Ev<N> r; r.v = s - a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
-template <int N> inline Ev<N> operator*(const Ev<N>& a, double s)
-{ Ev<N> r; r.v = a.v * s; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * s; return r; }
-template <int N> inline Ev<N> operator*(double s, const Ev<N>& a) { return a * s; }
-template <int N> inline Ev<N> ev_min(const Ev<N>& a, double c) { return a.v <= c ? a : Ev<N>(c); }
-template <int N> inline Ev<N> ev_max(const Ev<N>& a, double c) { return a.v >= c ? a : Ev<N>(c); }
-
+using namespace oracle_ad;
 typedef Ev<3> E3;
 
 /* ------------------------------------------------------------------ */

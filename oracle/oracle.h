@@ -123,6 +123,37 @@ int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps,
 
 void ob_default_params(ob_params* prm);
 
+/* Two-phase immiscible lens problem on a 2-D structured grid (config 1,
+ * tests/problems/lensproblem.hh).  Cell index I = i + nx*j, x fastest; y up.
+ * Primary variables per cell: p_w [Pa], S_w.  Same layout as
+ * omodels_lens_problem in include/omodels.h. */
+typedef struct ol_problem {
+    int32_t nx, ny;
+    double lx, ly;                                   /* domain size [m] */
+    double lens_x0, lens_y0, lens_x1, lens_y1;       /* lens box [m] */
+    double k_lens, k_outer;                          /* intrinsic permeability [m^2] */
+    double porosity;
+    double swr_lens, snr_lens, alpha_lens, n_lens;   /* regularized Van Genuchten, lens */
+    double swr_outer, snr_outer, alpha_outer, n_outer;
+    double rho_w, rho_n, mu_w, mu_n;                 /* incompressible liquids */
+    double gravity;                                  /* y component [m/s^2] */
+    double inflow_rate;                              /* DNAPL inflow on the top strip [kg/(m^2 s)] */
+    double p_atm;                                    /* reference pressure of the free-flow sides [Pa] */
+    double eps;                                      /* geometric tolerance [m] */
+} ol_problem;
+
+void ol_default_problem(ol_problem* p);
+int ol_num_cells(const ol_problem* p, int64_t* n, int64_t* nnzb);
+int ol_pattern(const ol_problem* p, int32_t* row_ptr, int32_t* col_idx, int32_t* diag);
+int ol_initial(const ol_problem* p, double* pv);
+int ol_storage(const ol_problem* p, const double* pv, double* storage);
+int ol_linearize(const ol_problem* p, const double* pv, const double* storage1, double dt,
+                 double* residual, double* jac);
+int ol_material(const ol_problem* p, int lens, double sw, double* out);
+int ol_simulate(const ol_problem* p, double* pv, double t_end, double dt0, const ob_params* prm,
+                int max_steps, double* t_hist, double* dt_hist, int32_t* n_steps,
+                ob_iter_log* log, int log_cap, int32_t* log_len);
+
 #ifdef __cplusplus
 }
 #endif

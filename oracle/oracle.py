@@ -47,6 +47,20 @@ class ObIterLog(C.Structure):
                 ("dt", C.c_double)]
 
 
+class OlProblem(C.Structure):
+    """Same layout as ol_problem (oracle.h) and omodels_lens_problem."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("lx", C.c_double), ("ly", C.c_double),
+        ("lens_x0", C.c_double), ("lens_y0", C.c_double), ("lens_x1", C.c_double), ("lens_y1", C.c_double),
+        ("k_lens", C.c_double), ("k_outer", C.c_double), ("porosity", C.c_double),
+        ("swr_lens", C.c_double), ("snr_lens", C.c_double), ("alpha_lens", C.c_double), ("n_lens", C.c_double),
+        ("swr_outer", C.c_double), ("snr_outer", C.c_double), ("alpha_outer", C.c_double),
+        ("n_outer", C.c_double),
+        ("rho_w", C.c_double), ("rho_n", C.c_double), ("mu_w", C.c_double), ("mu_n", C.c_double),
+        ("gravity", C.c_double), ("inflow_rate", C.c_double), ("p_atm", C.c_double), ("eps", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -68,6 +82,15 @@ def lib():
                                         C.POINTER(ObParams), _ip]
         L.ob_simulate.argtypes = [C.POINTER(ObProblem), _dp, _u8p, C.c_int, C.c_double, C.POINTER(ObParams),
                                   _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp, C.c_int]
+        L.ol_default_problem.argtypes = [C.POINTER(OlProblem)]
+        L.ol_num_cells.argtypes = [C.POINTER(OlProblem), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.ol_pattern.argtypes = [C.POINTER(OlProblem), _ip, _ip, _ip]
+        L.ol_initial.argtypes = [C.POINTER(OlProblem), _dp]
+        L.ol_storage.argtypes = [C.POINTER(OlProblem), _dp, _dp]
+        L.ol_linearize.argtypes = [C.POINTER(OlProblem), _dp, _dp, C.c_double, _dp, _dp]
+        L.ol_material.argtypes = [C.POINTER(OlProblem), C.c_int, C.c_double, _dp]
+        L.ol_simulate.argtypes = [C.POINTER(OlProblem), _dp, C.c_double, C.c_double, C.POINTER(ObParams),
+                                  C.c_int, _dp, _dp, _ip, C.POINTER(ObIterLog), C.c_int, _ip]
         _lib = L
     return _lib
 
@@ -190,3 +213,79 @@ def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep
     recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
             for l in log[:min(nlog.value, log_cap)]]
     return rc, pv, mean, hist, dts, recs, timers
+
+
+# ---------------------------------------------------------------------------
+# lens problem (two-phase immiscible element path)
+
+def lens_problem(**kw):
+    p = OlProblem()
+    lib().ol_default_problem(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def lens_sizes(p):
+    n, nnzb = C.c_int64(), C.c_int64()
+    if lib().ol_num_cells(C.byref(p), C.byref(n), C.byref(nnzb)) != 0:
+        raise RuntimeError("ol_num_cells failed")
+    return n.value, nnzb.value
+
+
+def lens_pattern(p):
+    n, nnzb = lens_sizes(p)
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnzb, np.int32)
+    dg = np.zeros(n, np.int32)
+    if lib().ol_pattern(C.byref(p), P(rp, _ip), P(ci, _ip), P(dg, _ip)) != 0:
+        raise RuntimeError("ol_pattern failed")
+    return rp, ci, dg
+
+
+def lens_initial(p):
+    n, _ = lens_sizes(p)
+    pv = np.zeros(2 * n)
+    lib().ol_initial(C.byref(p), P(pv))
+    return pv
+
+
+def lens_storage(p, pv):
+    n, _ = lens_sizes(p)
+    st = np.zeros(2 * n)
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    if lib().ol_storage(C.byref(p), P(pv), P(st)) != 0:
+        raise RuntimeError("ol_storage failed")
+    return st
+
+
+def lens_linearize(p, pv, st1, dt):
+    n, nnzb = lens_sizes(p)
+    res = np.zeros(2 * n)
+    jac = np.zeros(4 * nnzb)
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    st1 = np.ascontiguousarray(st1, dtype=np.float64)
+    if lib().ol_linearize(C.byref(p), P(pv), P(st1), float(dt), P(res), P(jac)) != 0:
+        raise RuntimeError("ol_linearize failed")
+    return res, jac
+
+
+def lens_material(p, lens, sw):
+    out = np.zeros(6)
+    lib().ol_material(C.byref(p), int(lens), float(sw), P(out))
+    return dict(pc=out[0], dpc=out[1], krw=out[2], dkrw=out[3], krn=out[4], dkrn=out[5])
+
+
+def lens_simulate(p, pv, t_end, dt0, prm, max_steps=10000, log_cap=100000):
+    pv = np.array(pv, dtype=np.float64)
+    th = np.zeros(max_steps)
+    dth = np.zeros(max_steps)
+    ns = C.c_int32()
+    log = (ObIterLog * log_cap)()
+    nlog = C.c_int32()
+    rc = lib().ol_simulate(C.byref(p), P(pv), float(t_end), float(dt0), C.byref(prm), int(max_steps), P(th),
+                           P(dth), C.byref(ns), log, log_cap, C.byref(nlog))
+    recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
+            for l in log[:min(nlog.value, log_cap)]]
+    k = ns.value
+    return rc, pv, th[:k], dth[:k], recs
