@@ -105,10 +105,12 @@ __device__ __forceinline__ void waitDeps(const TileGeom& g, int TI, int TJ, int 
         if (ti >= 0 && ti < g.TX) deps[nd++] = ti + g.TX * (TJ + g.TY * TK);
         if (tj >= 0 && tj < g.TY) deps[nd++] = TI + g.TX * (tj + g.TY * TK);
         if (tk >= 0 && tk < g.TZ) deps[nd++] = TI + g.TX * (TJ + g.TY * tk);
-        for (int q = 0; q < nd; ++q) {
+        // once any wait of this launch has run out (err set), the remaining
+        // tiles stop waiting too, so a broken protocol ends within one limit
+        for (int q = 0; q < nd && !loadRelaxed(err); ++q) {
             long spins = 0;
             while (loadRelaxed(done + deps[q]) != epoch) {
-                if (++spins > kSpinLimit) { atomicOr(err, 1); break; }
+                if (++spins > kSpinLimit || ((spins & 1023) == 0 && loadRelaxed(err))) { atomicOr(err, 1); break; }
                 __builtin_amdgcn_s_sleep(2);
             }
         }

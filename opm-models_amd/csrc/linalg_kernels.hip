@@ -93,12 +93,73 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp,
     for (int e = 0; e < BS; ++e) y[(long)BS * i + e] = acc[e];
 }
 
+// Workgroup w runs on XCD w % 8 (round-robin dispatch); renumber the
+// workgroups so that each XCD walks one contiguous range of rows and the
+// x values of neighbouring rows (+-1, +-nx, +-nx*ny) are reused in its own L2.
+__device__ __forceinline__ int xcdContiguousBlock(int w, int G)
+{
+    const int per = G / 8, rem = G % 8, xcd = w % 8, k = w / 8;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+// SpMV for 3x3 blocks: three lanes per block row, lane e forms row e of each
+// block product, so a wave reads each row's blocks as contiguous 24-byte
+// pieces; per row the operation order is that of k_spmv.
+template <class S>
+__global__ __launch_bounds__(256) void k_spmv3(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                               const S* __restrict__ a, const S* __restrict__ x,
+                                               S* __restrict__ y)
+{
+    const long g = (long)xcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = (int)(g / 3), e = (int)(g - 3L * (g / 3));
+    if (i >= n) return;
+    // a 7-point row has at most 7 blocks: the fixed bound lets every load of
+    // the row be issued before the dependent sums
+    const int k0 = rp[i], nb = rp[i + 1] - k0;
+    S v[7][3], xv[7][3];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        if (j < nb) {
+            const long c = ci[k0 + j];
+            const S* row = a + 9L * (k0 + j) + 3 * e;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                v[j][q] = row[q];
+                xv[j][q] = x[3 * c + q];
+            }
+        }
+    }
+    S acc = S(0);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        if (j < nb) {
+            S s = v[j][0] * xv[j][0];
+            s += v[j][1] * xv[j][1];
+            s += v[j][2] * xv[j][2];
+            acc += s;
+        }
+    }
+    for (int k = k0 + 7; k < k0 + nb; ++k) {   // longer rows (other patterns), same order
+        const long c = ci[k];
+        const S* row = a + 9L * k + 3 * e;
+        S s = row[0] * x[3 * c];
+        s += row[1] * x[3 * c + 1];
+        s += row[2] * x[3 * c + 2];
+        acc += s;
+    }
+    y[3L * i + e] = acc;
+}
+
 template <class S>
 void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a, const S* x, S* y, hipStream_t st)
 {
-    dim3 g((n + 255) / 256), b(256);
-    if (bs == 3) hipLaunchKernelGGL((k_spmv<3, S>), g, b, 0, st, n, rp, ci, a, x, y);
-    else hipLaunchKernelGGL((k_spmv<2, S>), g, b, 0, st, n, rp, ci, a, x, y);
+    if (bs == 3) {
+        const long lanes = 3L * n;
+        hipLaunchKernelGGL((k_spmv3<S>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, n, rp, ci, a, x, y);
+    } else {
+        dim3 g((n + 255) / 256), b(256);
+        hipLaunchKernelGGL((k_spmv<2, S>), g, b, 0, st, n, rp, ci, a, x, y);
+    }
 }
 template void launchSpmv<double>(int, int, const int*, const int*, const double*, const double*, double*, hipStream_t);
 template void launchSpmv<float>(int, int, const int*, const int*, const float*, const float*, float*, hipStream_t);
