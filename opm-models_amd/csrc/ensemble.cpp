@@ -36,6 +36,7 @@ size_t Ensemble::memoryEstimate(const omodels_blackoil_problem* p, int precision
     if (precision == 1) b += nnzb * 9 * 4 * 2;                // FP32 J and factor
     b += N * 3 * 8 * (kNumVec + 5) + N * 3 * 4 * kNumVec;      // vectors and state
     b += N * (4 * 8 + 6 * 4 + 3);                            // topology, transmissibilities, flags
+    b += N * 9 * 8 + (precision == 1 ? N * 9 * 4 : 0);       // exchange slots of the ILU sweeps
     return b;
 }
 

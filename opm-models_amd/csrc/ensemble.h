@@ -62,8 +62,9 @@ struct IluInst {
     double* dinvw;
     const void* d;        // S*
     void* v;              // S*
-    int* done;            // flags of this instance's tiles
+    int* done;            // flags of this instance's tiles (factorization)
     const int* active;    // nullptr: always active
+    void* xch[3];         // sweep exchange slots toward the x, y, z neighbour tiles (S*, [N][3] each)
 };
 
 // Multi-instance tiled ILU: factor / sweeps over all tiles of all instances,
@@ -81,8 +82,13 @@ public:
     const std::vector<int>& tileOffsets() const { return tileOff_; }
 
 private:
+    template <class S>
     const IluInst* upload(const std::vector<IluInst>& insts, hipStream_t s);
+    void resetExchange(hipStream_t s);
     DevBuf<TileTask> order_;
+    std::vector<long> cellOff_;          // first cell of each instance in the exchange arrays
+    DevBuf<double> xch64_;               // [9 * cells]: three [N][3] slot arrays per instance
+    DevBuf<float> xch32_;
     std::vector<std::pair<std::vector<IluInst>, std::unique_ptr<DevBuf<IluInst>>>> dcache_;
     DevBuf<int> done_, err_;
     DevBuf<unsigned long long> counter_;

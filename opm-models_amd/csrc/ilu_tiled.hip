@@ -61,6 +61,39 @@ __device__ __forceinline__ void stCoh(S* p, S v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Sweep exchange.  A value that a neighbour tile needs is written to its own
+// slot of a per-direction exchange array, which holds a signalling-NaN
+// sentinel while empty; the consumer polls the slot (one aligned 8- or
+// 4-byte load, atomic) until it differs from the sentinel, takes the value
+// and writes the sentinel back for the next sweep.  Arithmetic never yields a
+// signalling NaN; a value that is one (a copied input) is sent as a quiet NaN.
+constexpr unsigned long long kEmpty64 = 0x7FF4A5A5A5A5A5A5ULL;
+constexpr unsigned int kEmpty32 = 0x7FA5A5A5U;
+__device__ __forceinline__ bool isEmpty(double v) { return (unsigned long long)__double_as_longlong(v) == kEmpty64; }
+__device__ __forceinline__ bool isEmpty(float v) { return (unsigned int)__float_as_int(v) == kEmpty32; }
+template <class S> __device__ __forceinline__ S emptySlot();
+template <> __device__ __forceinline__ double emptySlot<double>() { return __longlong_as_double((long long)kEmpty64); }
+template <> __device__ __forceinline__ float emptySlot<float>() { return __int_as_float((int)kEmpty32); }
+template <class S>
+__device__ __forceinline__ void sendSlot(S* slot, S v)
+{
+    stCoh(slot, isEmpty(v) ? S(NAN) : v);
+}
+// wait for a neighbour's value; bounded, and cut short once the launch has an error
+template <class S>
+__device__ __forceinline__ S receiveSlot(S* slot, int* err)
+{
+    long spins = 0;
+    S v = ldCoh(slot);
+    while (isEmpty(v)) {
+        if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) { atomicOr(err, 1); break; }
+        __builtin_amdgcn_s_sleep(1);
+        v = ldCoh(slot);
+    }
+    stCoh(slot, emptySlot<S>());
+    return v;
+}
+
 // wait until every vector memory operation of this wave has completed
 // (s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maxima)
 __device__ __forceinline__ void waitStores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
@@ -212,7 +245,7 @@ template <class S>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
 {
-    __shared__ S sV[kPadMax * 3];          // results, ghosts at local -1
+    __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
     __shared__ int sOrder[kTileThreads];   // cells in level order
     __shared__ int sTicket;
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
@@ -224,7 +257,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __
         const TileTask task = order[t];
         const IluInst I = insts[task.inst];
         if (!isActive(I)) {
-            publish(I.done, task.tile, epoch);
             t = shareTicket(next, &sTicket);
             continue;
         }
@@ -260,12 +292,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __
             if (hy) { for (int w = 0; w < 3; ++w) Ly[w] = lu[9 * o + 3 * e + w]; ++o; }
             if (hx) { for (int w = 0; w < 3; ++w) Lx[w] = lu[9 * o + 3 * e + w]; }
         }
-        waitDeps(g, TI, TJ, TK, -1, I.done, epoch, err);
-        // ghosts: values of the neighbour tiles, written before their flags
+        // ghosts: the neighbour tiles' face values, received through the exchange
+        // slots in one batch before the tile's levels (receiving them level by
+        // level would put a memory round trip on every level)
+        S* const xx = (S*)I.xch[0];
+        S* const xy = (S*)I.xch[1];
+        S* const xz = (S*)I.xch[2];
         if (q < 3) {
-            if (hz && lk == 0) sV[3 * (p - PXY) + e] = ldCoh(v + 3 * (Ic - nxy) + e);
-            if (hy && lj == 0) sV[3 * (p - PX) + e] = ldCoh(v + 3 * (Ic - g.nx) + e);
-            if (hx && li == 0) sV[3 * (p - 1) + e] = ldCoh(v + 3 * (Ic - 1) + e);
+            if (hz && lk == 0) sV[3 * (p - PXY) + e] = receiveSlot(xz + 3 * (Ic - nxy) + e, err);
+            if (hy && lj == 0) sV[3 * (p - PX) + e] = receiveSlot(xy + 3 * (Ic - g.nx) + e, err);
+            if (hx && li == 0) sV[3 * (p - 1) + e] = receiveSlot(xx + 3 * (Ic - 1) + e, err);
         }
         __syncthreads();
         const int nLev = BX + BY + BZ - 2;
@@ -281,8 +317,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __
             }
             __syncthreads();
         }
-        if (word >= 0 && q < 3) stCoh(v + 3 * Ic + e, r);
-        publish(I.done, task.tile, epoch);
+        if (word >= 0 && q < 3) {
+            v[3 * Ic + e] = r;
+            if (li + 1 == BX && TI * BX + li + 1 < g.nx) sendSlot(xx + 3 * Ic + e, r);
+            if (lj + 1 == BY && TJ * BY + lj + 1 < g.ny) sendSlot(xy + 3 * Ic + e, r);
+            if (lk + 1 == BZ && TK * BZ + lk + 1 < g.nz) sendSlot(xz + 3 * Ic + e, r);
+        }
+        __syncthreads();
         t = shareTicket(next, &sTicket);
     }
 }
@@ -292,7 +333,7 @@ template <class S>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
 {
-    __shared__ S sV[kPadMax * 3];          // results, ghosts at local +B
+    __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
     __shared__ int sOrder[kTileThreads];
     __shared__ int sTicket;
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
@@ -304,7 +345,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __
         const TileTask task = order[T - 1 - t];
         const IluInst I = insts[task.inst];
         if (!isActive(I)) {
-            publish(I.done, task.tile, epoch);
             t = shareTicket(next, &sTicket);
             continue;
         }
@@ -343,11 +383,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __
             if (uz) { for (int w = 0; w < 3; ++w) Uz[w] = lu[9 * o + 3 * e + w]; }
             for (int w = 0; w < 3; ++w) D[w] = dinv[9 * Ic + 3 * e + w];
         }
-        waitDeps(g, TI, TJ, TK, +1, I.done, epoch, err);
+        S* const xx = (S*)I.xch[0];
+        S* const xy = (S*)I.xch[1];
+        S* const xz = (S*)I.xch[2];
         if (q < 3) {
-            if (ux && li + 1 == BX) sV[3 * (p + 1) + e] = ldCoh(v + 3 * (Ic + 1) + e);
-            if (uy && lj + 1 == BY) sV[3 * (p + PX) + e] = ldCoh(v + 3 * (Ic + g.nx) + e);
-            if (uz && lk + 1 == BZ) sV[3 * (p + PXY) + e] = ldCoh(v + 3 * (Ic + nxy) + e);
+            if (ux && li + 1 == BX) sV[3 * (p + 1) + e] = receiveSlot(xx + 3 * (Ic + 1) + e, err);
+            if (uy && lj + 1 == BY) sV[3 * (p + PX) + e] = receiveSlot(xy + 3 * (Ic + g.nx) + e, err);
+            if (uz && lk + 1 == BZ) sV[3 * (p + PXY) + e] = receiveSlot(xz + 3 * (Ic + nxy) + e, err);
         }
         __syncthreads();
         const int nLev = BX + BY + BZ - 2;
@@ -369,8 +411,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __
             }
             __syncthreads();
         }
-        if (word >= 0 && q < 3) stCoh(v + 3 * Ic + e, out);
-        publish(I.done, task.tile, epoch);
+        if (word >= 0 && q < 3) {
+            v[3 * Ic + e] = out;
+            if (li == 0 && TI * BX + li > 0) sendSlot(xx + 3 * Ic + e, out);
+            if (lj == 0 && TJ * BY + lj > 0) sendSlot(xy + 3 * Ic + e, out);
+            if (lk == 0 && TK * BZ + lk > 0) sendSlot(xz + 3 * Ic + e, out);
+        }
+        __syncthreads();
         t = shareTicket(next, &sTicket);
     }
 }
@@ -526,6 +573,18 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
 }
 
 // ---------------------------------------------------------------------------
+// empty exchange slots
+__global__ void k_fill_empty64(unsigned long long* p, long n)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = kEmpty64;
+}
+__global__ void k_fill_empty32(unsigned int* p, long n)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = kEmpty32;
+}
+
 TileGeom makeTileGeom(int nx, int ny, int nz)
 {
     // tile shape: 256 cells, as cubic as the grid allows
@@ -596,6 +655,9 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
                      [](const std::pair<int, TileTask>& a, const std::pair<int, TileTask>& b) { return a.first < b.first; });
     std::vector<TileTask> ord(tasks.size());
     for (size_t k = 0; k < tasks.size(); ++k) ord[k] = tasks[k].second;
+    cellOff_.assign(geoms.size() + 1, 0);
+    for (size_t b = 0; b < geoms.size(); ++b)
+        cellOff_[b + 1] = cellOff_[b] + (long)geoms[b].nx * geoms[b].ny * geoms[b].nz;
     OM_HIP(hipStreamSynchronize(s));
     T_ = (int)ord.size();
     order_.upload(ord.data(), ord.size(), s);
@@ -603,14 +665,46 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
     OM_HIP(hipMemsetAsync(done_.p, 0, std::max(T_, 1) * sizeof(int), s));
     // flags restart from zero: restart the epoch so that no stale value matches
     epoch_ = 0;
+    // exchange slots of the sweeps: FP64 now, FP32 on first use
+    xch64_.alloc(std::max<long>(9 * cellOff_.back(), 1));
+    xch32_.release();
+    resetExchange(s);
     gridFactor_ = std::min(T_, residentFactor_);
     dcache_.clear();
 }
 
+void TiledIluSet::resetExchange(hipStream_t s)
+{
+    if (xch64_.p && xch64_.n)
+        hipLaunchKernelGGL(k_fill_empty64, dim3((unsigned)((xch64_.n + 255) / 256)), dim3(256), 0, s,
+                           (unsigned long long*)xch64_.p, (long)xch64_.n);
+    if (xch32_.p && xch32_.n)
+        hipLaunchKernelGGL(k_fill_empty32, dim3((unsigned)((xch32_.n + 255) / 256)), dim3(256), 0, s,
+                           (unsigned int*)xch32_.p, (long)xch32_.n);
+    OM_HIP(hipGetLastError());
+}
+
+template <class S>
 const IluInst* TiledIluSet::upload(const std::vector<IluInst>& insts, hipStream_t s)
 {
     std::vector<IluInst> v(insts);
-    for (size_t b = 0; b < v.size(); ++b) v[b].done = done_.p + tileOff_[b];
+    S* xbase = nullptr;
+    if constexpr (sizeof(S) == 8) {
+        xbase = (S*)xch64_.p;
+    } else {
+        if (!xch32_.p) {
+            xch32_.alloc(std::max<long>(9 * cellOff_.back(), 1));
+            hipLaunchKernelGGL(k_fill_empty32, dim3((unsigned)((xch32_.n + 255) / 256)), dim3(256), 0, s,
+                               (unsigned int*)xch32_.p, (long)xch32_.n);
+            OM_HIP(hipGetLastError());
+        }
+        xbase = (S*)xch32_.p;
+    }
+    for (size_t b = 0; b < v.size(); ++b) {
+        v[b].done = done_.p + tileOff_[b];
+        const long nb = cellOff_[b + 1] - cellOff_[b];
+        for (int d = 0; d < 3; ++d) v[b].xch[d] = xbase + 9 * cellOff_[b] + 3L * d * nb;
+    }
     // descriptor arrays stay resident: a solver alternates between a few
     // (input, output) vector pairs, each gets its own device copy once
     for (auto& e : dcache_)
@@ -631,13 +725,19 @@ void TiledIluSet::checkError(hipStream_t s)
     int e = 0;
     OM_HIP(hipMemcpyAsync(&e, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     OM_HIP(hipStreamSynchronize(s));
-    if (e) throw std::runtime_error("tiled ILU sweep: dependency wait exceeded its bound");
+    if (e) {
+        // a launch that ran out of its waits leaves slots filled: empty them and the flag
+        resetExchange(s);
+        OM_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
+        OM_HIP(hipStreamSynchronize(s));
+        throw std::runtime_error("tiled ILU sweep: dependency wait exceeded its bound");
+    }
 }
 
 void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream_t s)
 {
     if (T_ == 0) return;
-    const IluInst* di = upload(insts, s);
+    const IluInst* di = upload<double>(insts, s);
     long long b = ticketBase_;
     ticketBase_ += T_ + gridFactor_;   // every launch draws exactly T + grid tickets
     int ep = ++epoch_;
@@ -653,7 +753,7 @@ template <class S>
 void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
 {
     if (T_ == 0) return;
-    const IluInst* di = upload(insts, s);
+    const IluInst* di = upload<S>(insts, s);
     const TileTask* od = order_.p;
     int T = T_;
     unsigned long long* ct = counter_.p;
