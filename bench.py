@@ -200,7 +200,7 @@ def main():
     spec = om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024)
     case = om.BlackOilCase(spec)
     dt = 86400.0
-    prm = om.default_params(max_dt=dt, min_dt=dt, precision=args.precision)
+    prm = om.default_params(max_dt=dt, min_dt=dt, precision=args.precision, autotune=1)
 
     t_setup = time.time()
     model = om.Model(case)

@@ -113,7 +113,9 @@ typedef struct omodels_params {
                                    (LinearSolverScalar = float, lens_immiscible_ecfv_ad.hh:53) */
     int32_t verbose;
     int32_t ilu_schedule;       /* 0: persistent tile-wavefront sweeps; 1: one launch per level */
-    int32_t pad;
+    int32_t autotune;           /* 1: time the ILU tile shapes (and the level schedule) on the first
+                                   factorization of a grid size and keep the fastest; results do
+                                   not depend on the choice */
 } omodels_params;
 
 void omodels_default_params(omodels_params* prm);

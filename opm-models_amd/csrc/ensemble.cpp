@@ -84,7 +84,10 @@ int Ensemble::add(const omodels_blackoil_problem* p, const double* pv, const uin
     M.G.dx = p->dx; M.G.dy = p->dy; M.G.dz = p->dz; M.G.g = p->gravity;
     M.G.V = p->dx * p->dy * p->dz;
     M.G.area[0] = p->dy * p->dz; M.G.area[1] = p->dx * p->dz; M.G.area[2] = p->dx * p->dy;
-    M.tg = makeTileGeom(p->nx, p->ny, p->nz);
+    TunedIlu tuned;
+    if (prm_.autotune && lookupTunedIlu(p->nx, p->ny, p->nz, prm_.precision, tuned) && !tuned.level)
+        M.tg = tuned.shape;
+    else M.tg = makeTileGeom(p->nx, p->ny, p->nz);
     assemblyGrid(M.G, M.gx, M.gy, M.gz, M.kChunk, 512);
     hipStream_t s = stream_;
     const long N = M.N;

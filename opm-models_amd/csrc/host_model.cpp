@@ -134,8 +134,15 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
     OM_HIP(hipHostMalloc((void**)&ksHost_, sizeof(KrylovScalars), hipHostMallocDefault));
     if (prm_.precision < 0 || prm_.precision > 2) throw std::invalid_argument("precision must be 0, 1 or 2");
     if (prm_.precision == 2 && prm_.precond != 0) throw std::invalid_argument("the FP32 solve uses block ILU(0)");
-    if (prm_.precond == 0 && prm_.ilu_schedule == 0 && m.bs == 3 && prm_.precision != 2)
-        tiled_ = std::make_unique<TiledIlu>(m.gx, m.gy, m.gz, m.stream);
+    if (prm_.precond == 0 && prm_.ilu_schedule == 0 && m.bs == 3 && prm_.precision != 2) {
+        TunedIlu t;
+        if (prm_.autotune && lookupTunedIlu(m.gx, m.gy, m.gz, prm_.precision, t)) {
+            tuned_ = true;
+            if (!t.level) tiled_ = std::make_unique<TiledIlu>(t.shape, m.stream);
+        } else {
+            tiled_ = std::make_unique<TiledIlu>(m.gx, m.gy, m.gz, m.stream);
+        }
+    }
     if (prm_.precision == 0) {
         for (auto& w : wd_) w.alloc(n);
         r64_.alloc(n);
@@ -154,6 +161,91 @@ BiCGStabBackend::~BiCGStabBackend()
 // prepare(): ParallelBaseBackend::prepare/setResidual/setMatrix
 // (parallelbasebackend.hh:182-250) + preparePreconditioner_ (:311-333):
 // the solver-owned copy in the solver scalar and the preconditioner factors.
+// Autotuner: time the factorization and applications of every tile shape
+// (and of the level-scheduled kernels) on the current matrix, in two
+// interleaved rounds, and keep the fastest by factor + 50 applications (a
+// factorization serves about that many preconditioner applications in these
+// solves), using the median of five applications per round and the better of
+// the two rounds.  The default shape is kept unless another one is more than
+// 5% faster.  Every candidate applies each row's operations in the same
+// order, so the choice never changes results.
+void BiCGStabBackend::tuneIlu()
+{
+    tuned_ = true;
+    hipStream_t s = m_.stream;
+    const int N = (int)m_.N;
+    const long nnzb = m_.nnzb, n = 3L * N;
+    DevBuf<double> lu, dinv, v64;
+    DevBuf<float> luf, df, vf;
+    lu.alloc(9 * nnzb); dinv.alloc(9 * (size_t)N); v64.alloc(n);
+    if (prm_.precision == 1) { luf.alloc(9 * nnzb); df.alloc(n); vf.alloc(n); launchCopyConvertD<float>(n, b64_.p, df.p, s); }
+    constexpr int kApply = 5;
+    hipEvent_t ev[kApply + 3];
+    for (auto& e : ev) OM_HIP(hipEventCreate(&e));
+    auto run = [&](TiledIlu* t) {
+        OM_HIP(hipMemcpyAsync(lu.p, A64_, 9 * nnzb * sizeof(double), hipMemcpyDeviceToDevice, s));
+        OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
+        OM_HIP(hipEventRecord(ev[0], s));
+        if (t) t->factor(m_.rowPtr.p, m_.diagOff.p, lu.p, dinv.p, fail_.p, s);
+        else {
+            launchIlu0Factor<double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu.p, m_.levels, fail_.p, s);
+            launchExtractInvDiag<double>(N, 3, m_.diagOff.p, lu.p, dinv.p, s);
+        }
+        OM_HIP(hipEventRecord(ev[1], s));
+        if (prm_.precision == 1) launchCopyConvertD<float>(9 * nnzb, lu.p, luf.p, s);
+        for (int r = 0; r <= kApply; ++r) {   // application 0 warms up
+            if (prm_.precision == 1) {
+                if (t) t->apply<float, double>(m_.rowPtr.p, m_.diagOff.p, luf.p, dinv.p, df.p, vf.p, s);
+                else launchIlu0Apply<float, double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, luf.p, dinv.p,
+                                                    df.p, vf.p, m_.levels, s);
+            } else {
+                if (t) t->apply<double, double>(m_.rowPtr.p, m_.diagOff.p, lu.p, dinv.p, b64_.p, v64.p, s);
+                else launchIlu0Apply<double, double>(N, 3, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu.p, dinv.p,
+                                                      b64_.p, v64.p, m_.levels, s);
+            }
+            OM_HIP(hipEventRecord(ev[2 + r], s));
+        }
+        OM_HIP(hipEventSynchronize(ev[2 + kApply]));
+        if (t) t->checkError(s);
+        float fms = 0.f;
+        OM_HIP(hipEventElapsedTime(&fms, ev[0], ev[1]));
+        std::vector<double> a(kApply);
+        for (int r = 0; r < kApply; ++r) {
+            float ms = 0.f;
+            OM_HIP(hipEventElapsedTime(&ms, ev[2 + r], ev[3 + r]));
+            a[r] = ms;
+        }
+        std::sort(a.begin(), a.end());
+        return (double)fms + 50.0 * a[kApply / 2];
+    };
+    const std::vector<TileGeom> shapes = tileShapeCandidates(m_.gx, m_.gy, m_.gz);
+    std::vector<double> cost(shapes.size() + 1, 1e300);   // last entry: level schedule
+    for (int round = 0; round < 2; ++round) {
+        for (size_t c = 0; c < shapes.size(); ++c) {
+            TiledIlu t(shapes[c], s);
+            cost[c] = std::min(cost[c], run(&t));
+        }
+        cost.back() = std::min(cost.back(), run(nullptr));
+    }
+    for (auto& e : ev) OM_HIP(hipEventDestroy(e));
+    size_t best = 0;   // candidate 0 is the default shape
+    for (size_t c = 1; c < cost.size(); ++c)
+        if (cost[c] < cost[best]) best = c;
+    if (cost[0] <= 1.05 * cost[best]) best = 0;
+    if (prm_.verbose) {
+        for (size_t c = 0; c < shapes.size(); ++c)
+            printf("autotune ILU tile %dx%dx%d: %.3f ms\n", shapes[c].BX, shapes[c].BY, shapes[c].BZ, cost[c]);
+        printf("autotune ILU level schedule: %.3f ms; chosen: %s\n", cost.back(),
+               best == shapes.size() ? "level" : "tiles");
+    }
+    TunedIlu choice;
+    choice.level = best == shapes.size();
+    if (!choice.level) choice.shape = shapes[best];
+    storeTunedIlu(m_.gx, m_.gy, m_.gz, prm_.precision, choice);
+    if (choice.level) tiled_.reset();
+    else tiled_ = std::make_unique<TiledIlu>(choice.shape, s);
+}
+
 void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidual)
 {
     hipStream_t s = m_.stream;
@@ -163,6 +255,7 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     else A64_ = m_.jac.p;
     if (hostResidual) b64_.upload(hostResidual, n, s);
     else OM_HIP(hipMemcpyAsync(b64_.p, m_.res.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (prm_.autotune && tiled_ && !tuned_) tuneIlu();
     OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
     const int N = (int)m_.N;
     if (prm_.precision == 2) {

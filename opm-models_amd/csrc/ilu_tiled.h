@@ -2,6 +2,8 @@
 // structured pattern (see ilu_tiled.hip).
 #pragma once
 
+#include <vector>
+
 #include "omodels_internal.h"
 
 namespace omodels {
@@ -13,13 +15,28 @@ struct TileGeom {
 };
 
 TileGeom makeTileGeom(int nx, int ny, int nz);
+// a grid cut into tiles of the given shape (0 = the default shape)
+TileGeom makeTileGeom(int nx, int ny, int nz, int bx, int by, int bz);
+// tile shapes the autotuner tries for a grid (valid for the sweep kernels)
+std::vector<TileGeom> tileShapeCandidates(int nx, int ny, int nz);
 
 class TiledIluSet;
+
+// Tile shapes chosen by the autotuner, per grid size and solver precision
+// (shared by every model and ensemble of the process).  `level` = the
+// level-scheduled kernels were fastest.
+struct TunedIlu {
+    TileGeom shape{};
+    bool level = false;
+};
+bool lookupTunedIlu(int nx, int ny, int nz, int precision, TunedIlu& out);
+void storeTunedIlu(int nx, int ny, int nz, int precision, const TunedIlu& t);
 
 // One reservoir (an ensemble of one instance).
 class TiledIlu {
 public:
     TiledIlu(int nx, int ny, int nz, hipStream_t s);
+    TiledIlu(const TileGeom& shape, hipStream_t s);
     ~TiledIlu();
     TiledIlu(const TiledIlu&) = delete;
     TiledIlu& operator=(const TiledIlu&) = delete;

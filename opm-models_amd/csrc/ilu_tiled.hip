@@ -21,6 +21,10 @@
 // device-coherent level before the tile's flag is published and read after
 // the flag with L2-bypassing loads.
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
+#include <stdexcept>
 #include <cstring>
 
 #include "ensemble.h"
@@ -585,6 +589,36 @@ __global__ void k_fill_empty32(unsigned int* p, long n)
     if (k < n) p[k] = kEmpty32;
 }
 
+TileGeom makeTileGeom(int nx, int ny, int nz, int bx, int by, int bz)
+{
+    if (bx <= 0 || by <= 0 || bz <= 0) return makeTileGeom(nx, ny, nz);
+    if (bx * by * bz > kTileThreads || (bx + 1) * (by + 1) * (bz + 1) > kPadMax)
+        throw std::invalid_argument("tile shape exceeds the sweep kernels' limits");
+    TileGeom g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.BX = bx; g.BY = by; g.BZ = bz;
+    g.TX = (nx + bx - 1) / bx; g.TY = (ny + by - 1) / by; g.TZ = (nz + bz - 1) / bz;
+    g.T = g.TX * g.TY * g.TZ;
+    return g;
+}
+
+std::vector<TileGeom> tileShapeCandidates(int nx, int ny, int nz)
+{
+    static const int shapes[][3] = {{8, 8, 4}, {16, 4, 4}, {4, 16, 4}, {8, 4, 8}, {4, 8, 8}, {16, 8, 2},
+                                    {8, 16, 2}, {16, 16, 1}, {32, 8, 1}, {8, 32, 1}, {4, 4, 16}, {2, 8, 16}};
+    std::vector<TileGeom> out;
+    out.push_back(makeTileGeom(nx, ny, nz));
+    for (const auto& sh : shapes) {
+        // a tile no larger than the grid in any direction (powers of two below it)
+        if (sh[0] > 2 * nx || sh[1] > 2 * ny || sh[2] > 2 * nz) continue;
+        const TileGeom g = makeTileGeom(nx, ny, nz, sh[0], sh[1], sh[2]);
+        bool dup = false;
+        for (const auto& o : out) dup = dup || (o.BX == g.BX && o.BY == g.BY && o.BZ == g.BZ);
+        if (!dup) out.push_back(g);
+    }
+    return out;
+}
+
 TileGeom makeTileGeom(int nx, int ny, int nz)
 {
     // tile shape: 256 cells, as cubic as the grid allows
@@ -773,10 +807,37 @@ template void TiledIluSet::apply<double>(const std::vector<IluInst>&, hipStream_
 template void TiledIluSet::apply<float>(const std::vector<IluInst>&, hipStream_t);
 
 // ---------------------------------------------------------------------------
+namespace {
+std::mutex gTunedMutex;
+std::map<std::array<int, 4>, TunedIlu> gTuned;
+}  // namespace
+
+bool lookupTunedIlu(int nx, int ny, int nz, int precision, TunedIlu& out)
+{
+    std::lock_guard<std::mutex> lock(gTunedMutex);
+    auto it = gTuned.find({nx, ny, nz, precision});
+    if (it == gTuned.end()) return false;
+    out = it->second;
+    return true;
+}
+
+void storeTunedIlu(int nx, int ny, int nz, int precision, const TunedIlu& t)
+{
+    std::lock_guard<std::mutex> lock(gTunedMutex);
+    gTuned[{nx, ny, nz, precision}] = t;
+}
+
+// ---------------------------------------------------------------------------
 // single reservoir: an ensemble of one
 TiledIlu::TiledIlu(int nx, int ny, int nz, hipStream_t s) : set_(new TiledIluSet(s))
 {
     g = makeTileGeom(nx, ny, nz);
+    set_->setInstances({g}, s);
+}
+
+TiledIlu::TiledIlu(const TileGeom& shape, hipStream_t s) : set_(new TiledIluSet(s))
+{
+    g = shape;
     set_->setInstances({g}, s);
 }
 

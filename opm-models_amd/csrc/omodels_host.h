@@ -108,6 +108,7 @@ private:
                                     double tol, double absTol, int maxIter, int* its);
     template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v);
     int solveImpl();
+    void tuneIlu();   // pick the tile shape / schedule of the ILU(0) by timing them
 
     LinearSystem& m_;
     omodels_params prm_;
@@ -125,6 +126,7 @@ private:
     DevBuf<int> fail_;
     std::unique_ptr<TiledIlu> tiled_;
     bool prepared_ = false;
+    bool tuned_ = false;
 };
 
 class BlackOilNewtonMethod {

@@ -83,14 +83,19 @@ def _system(dims, seed=1):
     return c, J, r, pat
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, "tuned"])
 @pytest.mark.parametrize("dims", [(5, 1, 1), (6, 5, 1), (1, 3, 9), (7, 5, 4), (20, 16, 6), (37, 11, 9)])
 def test_ilu0_factor_and_apply_parity(dims, schedule):
-    """Both schedules (persistent tile wavefront, one launch per level)
-    reproduce the sequential ILU(0) factor and sweeps bit for bit."""
+    """Both schedules (persistent tile wavefront, one launch per level), and
+    whatever tile shape or schedule the autotuner picks, reproduce the
+    sequential ILU(0) factor and sweeps bit for bit."""
     c, J, r, pat = _system(dims)
     m = om.Model(c)
-    s = om.Solver(m, om.default_params(ilu_schedule=schedule))
+    if schedule == "tuned":
+        prm = om.default_params(ilu_schedule=0, autotune=1)
+    else:
+        prm = om.default_params(ilu_schedule=schedule)
+    s = om.Solver(m, prm)
     s.prepare(J, r)
     lu = s.ilu()
     rc, luo = orc.ilu0(c.N, 3, pat["row_ptr"], pat["col_idx"], J)
@@ -206,3 +211,18 @@ def test_simulate_config2_reference_settings():
     assert same >= 5
     ep, es, ex = _cmp_states(hist[:same], histo[:same])
     assert ep[:5].max() <= 1e-7, ep
+
+
+def test_autotuned_solve_is_identical_to_default():
+    """The autotuner only changes the schedule: solutions agree bit for bit."""
+    c, J, r, pat = _system((40, 40, 10), seed=9)
+    m = om.Model(c)
+    xs = []
+    for tune in (0, 1):
+        s = om.Solver(m, om.default_params(autotune=tune))
+        s.prepare(J, r)
+        rc, x, its = s.solve()
+        assert rc == 0
+        xs.append((x, its))
+    assert xs[0][1] == xs[1][1]
+    assert np.array_equal(xs[0][0].view(np.uint64), xs[1][0].view(np.uint64))
