@@ -116,6 +116,9 @@ typedef struct omodels_params {
     int32_t autotune;           /* 1: time the ILU tile shapes (and the level schedule) on the first
                                    factorization of a grid size and keep the fastest; results do
                                    not depend on the choice */
+    int32_t graph;              /* 1: each BiCGStab iteration is replayed from a captured hipGraph
+                                   (one launch per iteration); results do not depend on it */
+    int32_t pad;
 } omodels_params;
 
 void omodels_default_params(omodels_params* prm);

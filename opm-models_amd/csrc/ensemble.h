@@ -78,6 +78,12 @@ public:
     template <class S>
     void apply(const std::vector<IluInst>& insts, hipStream_t s);
     void checkError(hipStream_t s);
+    // make the device descriptor of these instances resident (before a stream
+    // capture, which must not record the upload); generation() changes
+    // whenever resident descriptors are dropped
+    template <class S>
+    void prime(const std::vector<IluInst>& insts, hipStream_t s) { (void)upload<S>(insts, s); }
+    long generation() const { return generation_; }
     int totalTiles() const { return T_; }
     const std::vector<int>& tileOffsets() const { return tileOff_; }
 
@@ -92,8 +98,9 @@ private:
     std::vector<std::pair<std::vector<IluInst>, std::unique_ptr<DevBuf<IluInst>>>> dcache_;
     DevBuf<int> done_, err_;
     DevBuf<unsigned long long> counter_;
+    DevBuf<unsigned int> exits_;
     std::vector<int> tileOff_;
-    long long ticketBase_ = 0;   // counter value at the start of the next launch
+    long generation_ = 0;
     int epoch_ = 0, T_ = 0, gridFactor_ = 0, residentFactor_ = 0;
     int residentSweep_[2][2] = {};   // [FP64, FP32][forward, backward]
 };

@@ -324,8 +324,41 @@ void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v)
     }
 }
 
+void BiCGStabBackend::IterGraphs::reset()
+{
+    if (first) (void)hipGraphExecDestroy(first);
+    if (iter) (void)hipGraphExecDestroy(iter);
+    first = iter = nullptr;
+    generation = -1;
+}
+
+namespace {
+// record the work `body` puts on stream s into an executable graph
+template <class F>
+hipGraphExec_t captureGraph(hipStream_t s, F&& body)
+{
+    OM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        body();
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    OM_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    OM_HIP(rc);
+    return e;
+}
+}  // namespace
+
 // BiCGStabSolver::apply (bicgstabsolver.hh:120-330) with CombinedCriterion;
-// one host synchronisation per iteration (the convergence test).
+// one host synchronisation per iteration (the convergence test).  The device
+// scalars carry the iteration's tag base (2 it): the reductions of the first
+// half tag their failures with 2 it, those of the second half with 2 it + 1.
 template <class S>
 int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x, double tol,
                               double absTol, int maxIter, int* its)
@@ -350,20 +383,77 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     KrylovScalars k0{};
     k0.rho = 1; k0.alpha = 1; k0.omega = 1; k0.init_norm = init;
     k0.fail_code = 0; k0.fail_tag = -1; k0.stag_half = 0; k0.stag_full_tag = -1;
+    k0.tag_base = 0;
     OM_HIP(hipMemcpyAsync(ks_.p, &k0, sizeof(k0), hipMemcpyHostToDevice, s));
     OM_HIP(hipMemsetAsync(v, 0, m * sizeof(S), s));
     OM_HIP(hipMemsetAsync(p, 0, m * sizeof(S), s));
-    for (int it = 0; it < maxIter; ++it) {
-        const int tag = 2 * it;
-        launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, tag, s);
+
+    // first half: rho, p, y = M^-1 p, v = A y, alpha, x/r update, scalars to the host
+    auto firstHalf = [&]() {
+        launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, 0, s);
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
         precApply<S>(lu, dinv, p, y);
         m_.timer.begin(kClsSpmv, s);
         launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s);
         m_.timer.end(s);
-        launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, tag, s);
+        launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, 0, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
         OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+    };
+    // second half: z = M^-1 r, y = A z, omega, x/r update, next tag base
+    auto secondHalf = [&]() {
+        precApply<S>(lu, dinv, r, z);
+        m_.timer.begin(kClsSpmv, s);
+        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
+        m_.timer.end(s);
+        launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, 1, s);
+        launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, s);
+        launchAdvanceTag(ks_.p, 2, s);
+    };
+
+    IterGraphs* gr = nullptr;
+    if (prm_.graph) {
+        gr = &graphs_[sizeof(S) == 4 ? 1 : 0];
+        const std::array<const void*, 12> key = {A, lu, dinv, b, x, r, p, v, y, z, m_.partial.p, ks_.p};
+        // the tiled ILU's device descriptors of both applications must be
+        // resident before a capture (an upload cannot be recorded); a cache
+        // turnover between the two primes is followed by one more pass
+        if (tiled_ && prm_.precond == 0) {
+            for (int pass = 0; pass < 2; ++pass) {
+                const long g0 = tiled_->generation();
+                tiled_->primeApply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, p, y, s);
+                tiled_->primeApply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, r, z, s);
+                if (tiled_->generation() == g0) break;
+            }
+        }
+        const long gen = tiled_ ? tiled_->generation() : 0;
+        if (gr->key != key || gr->generation != gen || !gr->first) {
+            gr->reset();
+            m_.sync();
+            const bool timed = m_.timer.enabled;
+            m_.timer.enabled = false;   // event records stay out of the graphs
+            try {
+                gr->first = captureGraph(s, firstHalf);
+                gr->iter = captureGraph(s, [&]() { secondHalf(); firstHalf(); });
+            } catch (...) {
+                m_.timer.enabled = timed;
+                gr->reset();
+                throw;
+            }
+            m_.timer.enabled = timed;
+            gr->key = key;
+            gr->generation = gen;
+        }
+    }
+
+    for (int it = 0; it < maxIter; ++it) {
+        const int tag = 2 * it;
+        if (gr) {
+            OM_HIP(hipGraphLaunch(it == 0 ? gr->first : gr->iter, s));
+        } else {
+            if (it > 0) secondHalf();
+            firstHalf();
+        }
         m_.sync();
         const KrylovScalars& K = *ksHost_;
         if (it > 0 && K.fail_code && K.fail_tag == tag - 1) { *its = it - 1; return 2; }
@@ -372,13 +462,8 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         *its = it;
         if (converged(K.resnorm)) return 0;
         if (K.stag_half || K.resnorm > prm_.lin_max_error) return 1;
-        precApply<S>(lu, dinv, r, z);
-        m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
-        m_.timer.end(s);
-        launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, tag + 1, s);
-        launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, tag + 1, s);
     }
+    if (maxIter > 0) secondHalf();
     OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
     m_.sync();
     *its = maxIter;

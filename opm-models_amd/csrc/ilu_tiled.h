@@ -45,6 +45,10 @@ public:
     // v = (LU)^-1 d
     template <class S, class P>
     void apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s);
+    // upload the descriptor of this application ahead of a stream capture
+    template <class S, class P>
+    void primeApply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s);
+    long generation() const;
     // throws if a dependency wait ran into its bound
     void checkError(hipStream_t s);
     const TileGeom& geom() const { return g; }

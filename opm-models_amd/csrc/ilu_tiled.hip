@@ -170,6 +170,17 @@ __device__ __forceinline__ int shareTicket(int t, int* sTicket)
     return *sTicket;
 }
 
+// The last workgroup to leave a launch resets the ticket counter, so every
+// launch (eager or replayed from a captured graph) draws tickets from zero:
+// all workgroups have drawn their final ticket before they leave.
+__device__ __forceinline__ void leaveLaunch(unsigned long long* counter, unsigned int* exits)
+{
+    if (threadIdx.x == 0 && atomicAdd(exits, 1u) == gridDim.x - 1) {
+        atomicExch(counter, 0ULL);
+        atomicExch(exits, 0u);
+    }
+}
+
 __device__ __forceinline__ void publish(int* done, int tile, int epoch)
 {
     waitStores();
@@ -247,7 +258,8 @@ __device__ __forceinline__ S rowDot(const S* b, const S* x)
 // Forward sweep: v = L^-1 d (unit lower), lower blocks in column order -z,-y,-x.
 template <class S>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
+    unsigned int* exits)
 {
     __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
     __shared__ int sOrder[kTileThreads];   // cells in level order
@@ -330,12 +342,14 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __
         __syncthreads();
         t = shareTicket(next, &sTicket);
     }
+    leaveLaunch(counter, exits);
 }
 
 // Backward sweep: v = U^-1 v with the inverted (FP64) pivots; upper blocks +x,+y,+z.
 template <class S>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err)
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
+    unsigned int* exits)
 {
     __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
     __shared__ int sOrder[kTileThreads];
@@ -424,6 +438,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __
         __syncthreads();
         t = shareTicket(next, &sTicket);
     }
+    leaveLaunch(counter, exits);
 }
 
 // ---------------------------------------------------------------------------
@@ -468,7 +483,7 @@ __device__ __forceinline__ bool inv3(double* a)
 
 __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
-    int* fail)
+    int* fail, unsigned int* exits)
 {
     __shared__ double ld[kTileThreads * 9];
     __shared__ int sTicket;
@@ -574,6 +589,7 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
         publish(I.done, task.tile, epoch);
         t = shareTicket(next, &sTicket);
     }
+    leaveLaunch(counter, exits);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,6 +663,8 @@ TiledIluSet::TiledIluSet(hipStream_t s)
 {
     counter_.alloc(1);
     OM_HIP(hipMemsetAsync(counter_.p, 0, sizeof(unsigned long long), s));
+    exits_.alloc(1);
+    OM_HIP(hipMemsetAsync(exits_.p, 0, sizeof(unsigned int), s));
     err_.alloc(1);
     OM_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
     int dev = 0;
@@ -705,6 +723,7 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
     resetExchange(s);
     gridFactor_ = std::min(T_, residentFactor_);
     dcache_.clear();
+    ++generation_;
 }
 
 void TiledIluSet::resetExchange(hipStream_t s)
@@ -747,6 +766,7 @@ const IluInst* TiledIluSet::upload(const std::vector<IluInst>& insts, hipStream_
     if (dcache_.size() >= 16) {
         OM_HIP(hipStreamSynchronize(s));
         dcache_.clear();
+        ++generation_;
     }
     auto buf = std::make_unique<DevBuf<IluInst>>();
     buf->upload(v.data(), v.size(), s);
@@ -772,14 +792,14 @@ void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream
 {
     if (T_ == 0) return;
     const IluInst* di = upload<double>(insts, s);
-    long long b = ticketBase_;
-    ticketBase_ += T_ + gridFactor_;   // every launch draws exactly T + grid tickets
+    long long b = 0;
     int ep = ++epoch_;
     const TileTask* od = order_.p;
     int T = T_;
     unsigned long long* ct = counter_.p;
     int* er = err_.p;
-    void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&fail};
+    unsigned int* ex = exits_.p;
+    void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&fail, (void*)&ex};
     OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_factor_set, dim3(gridFactor_), dim3(kTileThreads), args, 0, s));
 }
 
@@ -794,12 +814,12 @@ void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
     int* er = err_.p;
     const int prec = sizeof(S) == 4 ? 1 : 0;
     const void* kernels[2] = {(const void*)k_ilu_fwd_set<S>, (const void*)k_ilu_bwd_set<S>};
+    unsigned int* ex = exits_.p;
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int grid = std::min(T_, residentSweep_[prec][sweep]);
-        long long b = ticketBase_;
-        ticketBase_ += T_ + grid;   // every launch draws exactly T + grid tickets
-        int ep = ++epoch_;
-        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er};
+        long long b = 0;
+        int ep = 0;   // the sweeps exchange values through the slots, not the epoch flags
+        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&ex};
         OM_HIP(hipLaunchCooperativeKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
     }
 }
@@ -862,6 +882,21 @@ void TiledIlu::apply(const int* rp, const int* diag, const S* lu, const P* dinv,
     I.active = nullptr;
     set_->apply<S>({I}, s);
 }
+template <class S, class P>
+void TiledIlu::primeApply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s)
+{
+    IluInst I{};
+    I.g = g; I.rp = rp; I.diag = diag; I.lu = lu; I.dinv = (const double*)dinv; I.d = d; I.v = v;
+    I.active = nullptr;
+    set_->prime<S>({I}, s);
+}
+template void TiledIlu::primeApply<double, double>(const int*, const int*, const double*, const double*,
+                                                   const double*, double*, hipStream_t);
+template void TiledIlu::primeApply<float, double>(const int*, const int*, const float*, const double*, const float*,
+                                                  float*, hipStream_t);
+
+long TiledIlu::generation() const { return set_->generation(); }
+
 template void TiledIlu::apply<double, double>(const int*, const int*, const double*, const double*, const double*,
                                               double*, hipStream_t);
 template void TiledIlu::apply<float, double>(const int*, const int*, const float*, const double*, const float*,

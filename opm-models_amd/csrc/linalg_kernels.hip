@@ -474,10 +474,11 @@ __device__ __forceinline__ void markFail(KrylovScalars* ks, int tag)
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restrict__ part, KrylovScalars* ks,
-                                                           int op, double eps, int tag)
+                                                           int op, double eps, int tagOff)
 {
     const double s = finalReduce<false>(part);
     if (threadIdx.x != 0) return;
+    const int tag = ks->tag_base + tagOff;
     if (op == 0) {
         ks->rho_i = s;
         if (fabs(ks->rho) <= eps || fabs(ks->omega) <= eps) markFail(ks, tag);
@@ -492,12 +493,22 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restr
 }
 
 template <class S>
-void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks, int op, int tag,
+void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks, int op, int tagOff,
                     hipStream_t st)
 {
     hipLaunchKernelGGL((k_dot_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, x, y, partial);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, op, eps, tag);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, op, eps, tagOff);
+}
+
+// next iteration: the half-iteration tags move on by two
+__global__ void k_advance_tag(KrylovScalars* ks, int delta)
+{
+    if (threadIdx.x == 0) ks->tag_base += delta;
+}
+void launchAdvanceTag(KrylovScalars* ks, int delta, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance_tag, dim3(1), dim3(64), 0, st, ks, delta);
 }
 template void launchDotFinal<double>(long, const double*, const double*, double*, KrylovScalars*, int, int, hipStream_t);
 template void launchDotFinal<float>(long, const float*, const float*, double*, KrylovScalars*, int, int, hipStream_t);
@@ -586,21 +597,22 @@ __global__ __launch_bounds__(kRedThreads) void k_dot2_partial(long m, const S* _
     if (threadIdx.x == 0) { part[blockIdx.x] = tt; part[kRedBlocks + blockIdx.x] = ts; }
 }
 __global__ __launch_bounds__(kRedThreads) void k_dot2_final(const double* __restrict__ part, KrylovScalars* ks,
-                                                            double eps, int tag)
+                                                            double eps, int tagOff)
 {
     const double tt = finalReduce<false>(part);
     const double ts = finalReduce<false>(part + kRedBlocks);
     if (threadIdx.x != 0) return;
+    const int tag = ks->tag_base + tagOff;
     if (fabs(tt) <= eps) markFail(ks, tag);
     ks->omega = ts / tt;
     if (fabs(ks->omega) <= eps) markFail(ks, tag);
 }
 template <class S>
-void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks, int tag, hipStream_t st)
+void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks, int tagOff, hipStream_t st)
 {
     hipLaunchKernelGGL((k_dot2_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, t, s, partial);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, eps, tag);
+    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, eps, tagOff);
 }
 template void launchDot2Final<double>(long, const double*, const double*, double*, KrylovScalars*, int, hipStream_t);
 template void launchDot2Final<float>(long, const float*, const float*, double*, KrylovScalars*, int, hipStream_t);
@@ -621,18 +633,19 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr2(long m, const S* __r
     nz = blockReduce<true>(nz);
     if (threadIdx.x == 0) part[blockIdx.x] = nz;
 }
-__global__ __launch_bounds__(kRedThreads) void k_stag_final(const double* __restrict__ part, KrylovScalars* ks, int tag)
+__global__ __launch_bounds__(kRedThreads) void k_stag_final(const double* __restrict__ part, KrylovScalars* ks,
+                                                            int tagOff)
 {
     const double nz = finalReduce<true>(part);
     if (threadIdx.x != 0) return;
-    if (nz == 0.0 && ks->stag_full_tag < 0) ks->stag_full_tag = tag;
+    if (nz == 0.0 && ks->stag_full_tag < 0) ks->stag_full_tag = ks->tag_base + tagOff;
 }
 template <class S>
-void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial, KrylovScalars* ks, int tag,
+void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial, KrylovScalars* ks, int tagOff,
                      hipStream_t st)
 {
     hipLaunchKernelGGL((k_update_xr2<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, z, t, x, r, partial, ks);
-    hipLaunchKernelGGL(k_stag_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, tag);
+    hipLaunchKernelGGL(k_stag_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, tagOff);
 }
 template void launchUpdateXR2<double>(long, const double*, const double*, double*, double*, double*, KrylovScalars*, int, hipStream_t);
 template void launchUpdateXR2<float>(long, const float*, const float*, float*, float*, double*, KrylovScalars*, int, hipStream_t);

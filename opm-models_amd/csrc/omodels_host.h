@@ -6,6 +6,7 @@
 //   BlackOilNewtonMethod  newtonmethod.hh:90-877 / blackoilnewtonmethod.hh:54-448
 #pragma once
 
+#include <array>
 #include <memory>
 #include <vector>
 
@@ -109,6 +110,20 @@ private:
     template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v);
     int solveImpl();
     void tuneIlu();   // pick the tile shape / schedule of the ILU(0) by timing them
+
+    // One BiCGStab iteration as replayed hipGraphs: `first` holds the first
+    // half of iteration 0 (up to the scalars copied to the host), `iter` the
+    // second half of iteration it-1 followed by the first half of iteration
+    // it.  Captured for one set of buffers (key) and one generation of the
+    // tiled ILU's resident descriptors; recaptured when either changes.
+    struct IterGraphs {
+        std::array<const void*, 12> key{};
+        long generation = -1;
+        hipGraphExec_t first = nullptr, iter = nullptr;
+        void reset();
+        ~IterGraphs() { reset(); }
+    };
+    IterGraphs graphs_[2];   // FP64, FP32 Krylov vectors
 
     LinearSystem& m_;
     omodels_params prm_;

@@ -134,6 +134,8 @@ struct KrylovScalars {
     int fail_tag;         // half-iteration index of the first breakdown
     int stag_half;        // change indicator all zero at the last half-step check
     int stag_full_tag;    // half-iteration index of a full-step stagnation, -1 if none
+    int tag_base;         // half-iteration index of the current iteration's first half (2 it)
+    int pad_;
 };
 
 constexpr int kRedBlocks = 1024;     // partial sums per reduction (fixed => deterministic)
@@ -160,6 +162,7 @@ template <class S> void launchJacobiApplyMixed(int n, const double* dinv, const 
 template <class S> void launchExtractInvDiag(int n, int bs, const int* diag, const S* lu, double* dinv, hipStream_t st);
 
 // BiCGStab steps (see bicgstab in host_solver.cpp for the sequence)
+// tagOff: 0 for the first half of an iteration, 1 for the second (tag = ks->tag_base + tagOff)
 template <class S> void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks,
                                        int op, int tag, hipStream_t st);
 template <class S> void launchUpdateP(long m, const S* r, const S* v, S* p, const KrylovScalars* ks,
@@ -168,6 +171,7 @@ template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* 
                                        KrylovScalars* ks, hipStream_t st);
 template <class S> void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial,
                                         KrylovScalars* ks, int tag, hipStream_t st);
+void launchAdvanceTag(KrylovScalars* ks, int delta, hipStream_t st);
 template <class S> void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks,
                                         int tag, hipStream_t st);
 template <class S> void launchMaxAbs(long m, const S* x, double* partial, double* out, hipStream_t st);

@@ -226,3 +226,50 @@ def test_autotuned_solve_is_identical_to_default():
         xs.append((x, its))
     assert xs[0][1] == xs[1][1]
     assert np.array_equal(xs[0][0].view(np.uint64), xs[1][0].view(np.uint64))
+
+
+GRAPH_CONFIGS = [
+    dict(),                                   # FP64, tiled ILU(0)
+    dict(precision=1),                        # FP32 inner BiCGStab in FP64 refinement
+    dict(precision=2),                        # FP32 solve, level-scheduled ILU(0)
+    dict(ilu_schedule=1),                     # FP64, one launch per ILU level
+    dict(precond=1, prec_relax=0.9),          # block Jacobi
+]
+
+
+@pytest.mark.parametrize("cfg", GRAPH_CONFIGS, ids=lambda c: ",".join(f"{k}={v}" for k, v in c.items()) or "fp64")
+def test_graph_replay_is_identical_to_eager(cfg):
+    """Replayed BiCGStab iterations give the eager solution bit for bit, on
+    several solves in a row through the same graphs (the second system has
+    other values in the same buffers) with eager work in between."""
+    c = case(40, 40, 10)
+    m = om.Model(c)
+    systems = [_system((40, 40, 10), seed=9)[1:3], _system((40, 40, 10), seed=21)[1:3]]
+    eager = om.Solver(m, om.default_params(**cfg))
+    graph = om.Solver(m, om.default_params(graph=1, **cfg))
+    for rep in range(2):
+        for J, r in systems:
+            eager.prepare(J, r)
+            rc0, x0, it0 = eager.solve()
+            graph.prepare(J, r)
+            rc1, x1, it1 = graph.solve()
+            print("rep", rep, "rc", rc0, rc1, "its", it0, it1, flush=True)
+            assert (rc0, it0) == (rc1, it1)
+            assert it0 > 2
+            assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+
+
+def test_graph_newton_run_is_identical_to_eager():
+    """A Newton run with every linear iteration replayed from graphs follows
+    the eager run step for step with bit-identical states."""
+    c = case(40, 40, 10, seed=1234)
+    out = []
+    for g in (0, 1):
+        m = om.Model(c)
+        m.set_solution(c.pv0, c.mean0)
+        rc, hist, dts, log, st = m.simulate(om.default_params(graph=g, precision=1), 6, 86400.0)
+        assert rc == 0
+        out.append((hist, dts, [l[:4] for l in log]))
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))

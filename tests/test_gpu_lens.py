@@ -105,3 +105,20 @@ def test_simulate_matches_oracle(precision, solver_float):
     v = (pg.lx / pg.nx) * (pg.ly / pg.ny)
     mass = np.sum((1.0 - s) * pg.porosity * pg.rho_n * v)
     assert mass == pytest.approx(pg.inflow_rate * 1.0 * 3000.0, rel=1e-7)
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_graph_run_is_identical_to_eager(precision):
+    """The lens run with its BiCGStab iterations replayed from graphs matches
+    the eager run bit for bit."""
+    pg, _ = problems()
+    out = []
+    for g in (0, 1):
+        m = om.LensModel(pg)
+        rc, th, dth, log, st = m.simulate(om.default_params(newton_tol=1e-8, precision=precision, graph=g),
+                                          3000.0, 250.0)
+        assert rc == 0
+        out.append((m.get_solution(), dth, st.linear_iters))
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))
