@@ -64,7 +64,8 @@ struct IluInst {
     void* v;              // S*
     int* done;            // flags of this instance's tiles (factorization)
     const int* active;    // nullptr: always active
-    void* xch[3];         // sweep exchange slots toward the x, y, z neighbour tiles (S*, [N][3] each)
+    void* xch;            // sweep exchange slots toward the x, y, z neighbour tiles: three [N][3]
+                          // arrays of S back to back (N = cells of the instance, 9 N < 2^31)
 };
 
 // Multi-instance tiled ILU: factor / sweeps over all tiles of all instances,

@@ -10,7 +10,7 @@ namespace omodels {
 
 struct TileGeom {
     int nx, ny, nz;
-    int BX, BY, BZ;     // tile shape, BX*BY*BZ <= 256
+    int BX, BY, BZ;     // tile shape, BX*BY*BZ <= 240
     int TX, TY, TZ, T;  // tiles per axis and in total
 };
 

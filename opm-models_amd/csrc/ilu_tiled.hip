@@ -6,21 +6,25 @@
 // natural ordering, istlpreconditionerwrappers.hh:151-190): same rows, same
 // per-row operation order.  Only the schedule differs.
 //
-// Schedule: each grid is cut into BX x BY x BZ tiles, one workgroup of 256
-// threads per tile and one thread per cell.  A tile depends on its -x, -y and
-// -z neighbour tiles (+x, +y, +z for the backward sweep) of the same
-// instance; tiles of all instances are handed out by one atomic ticket in
-// order of tile level (TI + TJ + TK), so every dependency of a tile holds a
-// smaller ticket.  The kernels are launched cooperatively (all workgroups
-// co-resident), hence the tile with the lowest unfinished ticket can always
-// proceed and every wait ends; waits are also bounded and report an error
-// instead of spinning forever.  Inside a tile the BX+BY+BZ-2 local levels are
-// separated by workgroup barriers and exchange values through LDS only; the
-// tile's results are stored to global memory once, after its last level.
-// Values across tile faces go through global memory, written through to the
-// device-coherent level before the tile's flag is published and read after
-// the flag with L2-bypassing loads.
+// Schedule: each grid is cut into BX x BY x BZ tiles (at most 240 cells).  A
+// tile depends on its -x, -y and -z neighbour tiles (+x, +y, +z for the
+// backward sweep) of the same instance; tiles of all instances are handed out
+// by one atomic ticket in order of tile level (TI + TJ + TK), so every
+// dependency of a tile holds a smaller ticket.  The kernels are launched
+// cooperatively (all workgroups co-resident), hence the tile with the lowest
+// unfinished ticket can always proceed and every wait ends; waits are also
+// bounded and report an error instead of spinning forever.
+//
+// The factorization works one thread per cell: the tile's BX+BY+BZ-2 local
+// levels are separated by workgroup barriers, and a tile publishes a done flag
+// after its values are written through to the device-coherent level.  The
+// sweeps, applied twice per solver iteration, stream instead: each face value
+// goes to its neighbour tile as soon as it is computed, and inside a tile the
+// levels follow each other through per-level completion counts in LDS rather
+// than workgroup barriers, so a tile runs a few cell levels behind its
+// upstream tiles instead of a whole tile behind them.
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <map>
 #include <mutex>
@@ -51,19 +55,27 @@ constexpr long kSpinLimit = 1L << 22;   // a few seconds of polling
 // tile (buffer_wbl2 / buffer_inv), which dominated the sweep time; it is not
 // needed because no exchanged value is ever accessed non-atomically while the
 // kernel runs.
+//
+// Every global-memory access of these kernels goes through global-address-space
+// pointers: generic (flat) accesses would also count against the LDS wait
+// counter, so each LDS wait would wait for the wave's global stores as well.
+template <class S> using GS = __attribute__((address_space(1))) S;
+template <class S> __device__ __forceinline__ GS<S>* toGlobal(S* p) { return (GS<S>*)p; }
+template <class S> __device__ __forceinline__ const GS<S>* toGlobal(const S* p) { return (const GS<S>*)p; }
+
 __device__ __forceinline__ int loadRelaxed(const int* p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<GS<int>*>(toGlobal(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class S>
 __device__ __forceinline__ S ldCoh(const S* p)
 {
-    return __hip_atomic_load(const_cast<S*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<GS<S>*>(toGlobal(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class S>
 __device__ __forceinline__ void stCoh(S* p, S v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(toGlobal(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Sweep exchange.  A value that a neighbour tile needs is written to its own
 // slot of a per-direction exchange array, which holds a signalling-NaN
@@ -193,16 +205,31 @@ __device__ __forceinline__ bool isActive(const IluInst& I) { return I.active == 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Sweeps.  A sweep workgroup has four lanes per tile cell, the cells taken in
-// level order: lanes 4r..4r+3 handle the r-th cell of the tile's level
-// sequence, one lane per block row (the fourth lane repeats the third row and
-// stores nothing).  Each lane keeps its rows of the cell's blocks and its
-// right-hand side in registers for the whole tile; a level is then one
-// contiguous range of lanes (one or two waves) that reads the neighbour values
-// from LDS, does its row's products and writes its result to LDS.  Values of
-// neighbour tiles are staged as a ghost layer in LDS after their flags are
-// seen.  Per row the operations and their order are those of dune's block mmv.
-constexpr int kSweepThreads = 4 * kTileThreads;
+// Sweeps.  A sweep workgroup has 16 waves.  Waves 0..14 compute, four lanes
+// per tile cell (at most kStreamCells cells) with the cells taken in level
+// order: lanes 4r..4r+3 handle the r-th cell of the tile's level sequence,
+// one lane per block row (the fourth lane repeats the third row and stores
+// nothing), and keep their rows of the cell's blocks and the right-hand side
+// in registers.  Wave 15 receives the neighbour tiles' face values.
+//
+// Inside a tile the sweep is a dataflow, as it is across tiles: every value
+// slot of the tile's LDS image (its cells and the ghost layer) starts as the
+// exchange sentinel; a lane computes its cell once the nine values of its
+// three dependency cells have left the sentinel, then writes its row's result
+// to LDS and, for a cell on a face, straight into the exchange slot of the
+// dependent tile.  The receiving wave polls all exchange slots of the tile's
+// ghost layer together and moves every value that has arrived into LDS.  A
+// tile thus starts as soon as its first ghost values arrive and follows its
+// upstream tiles cell level by cell level, with no workgroup barrier between
+// levels.  Per row the operations and their order are those of dune's block
+// mmv.
+constexpr int kSweepThreads = 1024;
+constexpr int kStreamCells = 240;                // compute cells per tile: 15 waves of 16 cells
+constexpr int kRecvWave = kStreamCells / 16;     // the receiving wave
+constexpr int kRecvPerLane = 8;                  // ghost values per receiving lane
+constexpr int kMaxTileLevels = 256;              // cell levels of a tile, BX + BY + BZ - 2 < 256
+// s_waitcnt lgkmcnt(0) with the vector-memory and export counters left at their maxima
+constexpr int kWaitLds = 0xC07F;
 
 __device__ __forceinline__ int tri3(int m) { return m < 0 ? 0 : (m + 1) * (m + 2) * (m + 3) / 6; }
 // number of cells (a, b, c) of a BX x BY x BZ box with a + b + c < l
@@ -224,10 +251,11 @@ __device__ __forceinline__ int rankInLevel(int a, int b, int c, int BX, int BY)
     return r + b - max(0, l - c - BX + 1);
 }
 
-// one word per cell: local coordinates and which neighbours exist; -1 outside the grid
-__device__ __forceinline__ int packCell(int li, int lj, int lk, bool n0, bool n1, bool n2)
+// one word per tile position: local coordinates, which dependency neighbours
+// exist, and whether the cell lies inside the grid
+__device__ __forceinline__ int packCell(int li, int lj, int lk, bool n0, bool n1, bool n2, bool valid)
 {
-    return li | (lj << 8) | (lk << 16) | ((int)n0 << 27) | ((int)n1 << 28) | ((int)n2 << 29);
+    return li | (lj << 8) | (lk << 16) | ((int)n0 << 27) | ((int)n1 << 28) | ((int)n2 << 29) | ((int)valid << 30);
 }
 
 // value of lane Q of this lane's quad (all four lanes of the quad are active)
@@ -255,112 +283,38 @@ __device__ __forceinline__ S rowDot(const S* b, const S* x)
     return s;
 }
 
-// Forward sweep: v = L^-1 d (unit lower), lower blocks in column order -z,-y,-x.
+// LDS values polled by other waves: single aligned accesses the compiler keeps
 template <class S>
-__global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_set(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
-    unsigned int* exits)
+__device__ __forceinline__ S ldsLoad(const S* p)
 {
-    __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
-    __shared__ int sOrder[kTileThreads];   // cells in level order
-    __shared__ int sTicket;
-    const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
-    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, base) : 0, &sTicket);
-    for (;;) {
-        if (t < 0 || t >= T) break;
-        int next = 0;
-        if (threadIdx.x == 0) next = drawTicket(counter, base);
-        const TileTask task = order[t];
-        const IluInst I = insts[task.inst];
-        if (!isActive(I)) {
-            t = shareTicket(next, &sTicket);
-            continue;
-        }
-        const TileGeom& g = I.g;
-        const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
-        const S* __restrict__ lu = (const S*)I.lu;
-        const S* __restrict__ d = (const S*)I.d;
-        S* __restrict__ v = (S*)I.v;
-        const long nxy = (long)g.nx * g.ny;
-        int TI, TJ, TK;
-        tileOf(g, task.tile, TI, TJ, TK);
-        if (threadIdx.x < BX * BY * BZ) {
-            const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
-            const int i = TI * BX + li, j = TJ * BY + lj, k = TK * BZ + lk;
-            sOrder[cellsBelow(li + lj + lk, BX, BY, BZ) + rankInLevel(li, lj, lk, BX, BY)] =
-                (i < g.nx && j < g.ny && k < g.nz) ? packCell(li, lj, lk, k > 0, j > 0, i > 0) : -1;
-        }
-        __syncthreads();
-        const int word = pos < BX * BY * BZ ? sOrder[pos] : -1;
-        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
-        const bool hz = word >= 0 && (word & (1 << 27)), hy = word >= 0 && (word & (1 << 28));
-        const bool hx = word >= 0 && (word & (1 << 29));
-        const long Ic = (long)(TI * BX + li) + (long)g.nx * ((long)(TJ * BY + lj) + (long)g.ny * (TK * BZ + lk));
-        const int p = (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
-        const int myLev = word >= 0 ? li + lj + lk : -1;
-        // this lane's row of the cell's lower blocks and its right-hand side
-        S Lz[3] = {}, Ly[3] = {}, Lx[3] = {};
-        S r = S(0);
-        if (word >= 0) {
-            r = d[3 * Ic + e];
-            long o = I.rp[Ic];
-            if (hz) { for (int w = 0; w < 3; ++w) Lz[w] = lu[9 * o + 3 * e + w]; ++o; }
-            if (hy) { for (int w = 0; w < 3; ++w) Ly[w] = lu[9 * o + 3 * e + w]; ++o; }
-            if (hx) { for (int w = 0; w < 3; ++w) Lx[w] = lu[9 * o + 3 * e + w]; }
-        }
-        // ghosts: the neighbour tiles' face values, received through the exchange
-        // slots in one batch before the tile's levels (receiving them level by
-        // level would put a memory round trip on every level)
-        S* const xx = (S*)I.xch[0];
-        S* const xy = (S*)I.xch[1];
-        S* const xz = (S*)I.xch[2];
-        if (q < 3) {
-            if (hz && lk == 0) sV[3 * (p - PXY) + e] = receiveSlot(xz + 3 * (Ic - nxy) + e, err);
-            if (hy && lj == 0) sV[3 * (p - PX) + e] = receiveSlot(xy + 3 * (Ic - g.nx) + e, err);
-            if (hx && li == 0) sV[3 * (p - 1) + e] = receiveSlot(xx + 3 * (Ic - 1) + e, err);
-        }
-        __syncthreads();
-        const int nLev = BX + BY + BZ - 2;
-        for (int lev = 0; lev <= nLev; ++lev) {
-            if (myLev == lev) {
-                const S sz = rowDot(Lz, sV + 3 * (p - PXY));
-                const S sy = rowDot(Ly, sV + 3 * (p - PX));
-                const S sx = rowDot(Lx, sV + 3 * (p - 1));
-                if (hz) r += S(-1) * sz;
-                if (hy) r += S(-1) * sy;
-                if (hx) r += S(-1) * sx;
-                if (q < 3) sV[3 * p + e] = r;
-            }
-            __syncthreads();
-        }
-        if (word >= 0 && q < 3) {
-            v[3 * Ic + e] = r;
-            if (li + 1 == BX && TI * BX + li + 1 < g.nx) sendSlot(xx + 3 * Ic + e, r);
-            if (lj + 1 == BY && TJ * BY + lj + 1 < g.ny) sendSlot(xy + 3 * Ic + e, r);
-            if (lk + 1 == BZ && TK * BZ + lk + 1 < g.nz) sendSlot(xz + 3 * Ic + e, r);
-        }
-        __syncthreads();
-        t = shareTicket(next, &sTicket);
-    }
-    leaveLaunch(counter, exits);
+    return __hip_atomic_load(const_cast<S*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class S>
+__device__ __forceinline__ void ldsStore(S* p, S v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Backward sweep: v = U^-1 v with the inverted (FP64) pivots; upper blocks +x,+y,+z.
-template <class S>
-__global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
-    unsigned int* exits)
+// Forward sweep (Bwd = false): v = L^-1 d (unit lower), lower blocks in column
+// order -z, -y, -x.  Backward sweep (Bwd = true): v = U^-1 v with the inverted
+// (FP64) pivots, upper blocks in column order +x, +y, +z.
+template <class S, bool Bwd>
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* __restrict__ insts,
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
 {
-    __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
-    __shared__ int sOrder[kTileThreads];
+    __shared__ S sV[kPadMax * 3];             // the tile's cells and its ghost layer (padded layout)
+    __shared__ int sOrder[kStreamCells];      // tile positions in level order
+    __shared__ int sDone[kMaxTileLevels];     // per cell level: compute waves that have finished it
+    __shared__ int sWaves[kMaxTileLevels];    // per cell level: compute waves holding part of it
     __shared__ int sTicket;
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
-    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, base) : 0, &sTicket);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
     for (;;) {
         if (t < 0 || t >= T) break;
         int next = 0;
-        if (threadIdx.x == 0) next = drawTicket(counter, base);
-        const TileTask task = order[T - 1 - t];
+        if (threadIdx.x == 0) next = drawTicket(counter, 0);
+        const TileTask task = order[Bwd ? T - 1 - t : t];
         const IluInst I = insts[task.inst];
         if (!isActive(I)) {
             t = shareTicket(next, &sTicket);
@@ -368,72 +322,226 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_set(const IluInst* __
         }
         const TileGeom& g = I.g;
         const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
-        const S* __restrict__ lu = (const S*)I.lu;
-        const double* __restrict__ dinv = I.dinv;
-        S* __restrict__ v = (S*)I.v;
+        const int ncell = BX * BY * BZ;
         const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
         tileOf(g, task.tile, TI, TJ, TK);
-        if (threadIdx.x < BX * BY * BZ) {
+        const int i0 = TI * BX, j0 = TJ * BY, k0 = TK * BZ;
+        // every value slot starts empty; tile positions in level order
+        for (int k = threadIdx.x; k < 3 * PXY * (BZ + 1); k += kSweepThreads) sV[k] = emptySlot<S>();
+        if (threadIdx.x < BX + BY + BZ - 2 + 1) {
+            const int l = threadIdx.x;
+            sDone[l] = 0;
+            sWaves[l] = (cellsBelow(l + 1, BX, BY, BZ) - 1) / 16 - cellsBelow(l, BX, BY, BZ) / 16 + 1;
+        }
+        if (threadIdx.x < ncell) {
             const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
-            const int i = TI * BX + li, j = TJ * BY + lj, k = TK * BZ + lk;
-            const int a = BX - 1 - li, b = BY - 1 - lj, c = BZ - 1 - lk;
+            const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+            const int a = Bwd ? BX - 1 - li : li, b = Bwd ? BY - 1 - lj : lj, c = Bwd ? BZ - 1 - lk : lk;
+            // dependency neighbours in column order: -z, -y, -x forward; +x, +y, +z backward
+            const bool n0 = Bwd ? i + 1 < g.nx : k > 0;
+            const bool n1 = Bwd ? j + 1 < g.ny : j > 0;
+            const bool n2 = Bwd ? k + 1 < g.nz : i > 0;
             sOrder[cellsBelow(a + b + c, BX, BY, BZ) + rankInLevel(a, b, c, BX, BY)] =
-                (i < g.nx && j < g.ny && k < g.nz)
-                    ? packCell(li, lj, lk, i + 1 < g.nx, j + 1 < g.ny, k + 1 < g.nz) : -1;
+                packCell(li, lj, lk, n0, n1, n2, i < g.nx && j < g.ny && k < g.nz);
         }
         __syncthreads();
-        const int word = pos < BX * BY * BZ ? sOrder[pos] : -1;
-        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
-        const bool ux = word >= 0 && (word & (1 << 27)), uy = word >= 0 && (word & (1 << 28));
-        const bool uz = word >= 0 && (word & (1 << 29));
-        const long Ic = (long)(TI * BX + li) + (long)g.nx * ((long)(TJ * BY + lj) + (long)g.ny * (TK * BZ + lk));
-        const int p = li + PX * lj + PXY * lk;
-        const int myLev = word >= 0 ? (BX - 1 - li) + (BY - 1 - lj) + (BZ - 1 - lk) : -1;
-        S Ux[3] = {}, Uy[3] = {}, Uz[3] = {};
-        double D[3] = {};
-        S r = S(0);
-        if (word >= 0) {
-            r = v[3 * Ic + e];
-            long o = I.diag[Ic] + 1;
-            if (ux) { for (int w = 0; w < 3; ++w) Ux[w] = lu[9 * o + 3 * e + w]; ++o; }
-            if (uy) { for (int w = 0; w < 3; ++w) Uy[w] = lu[9 * o + 3 * e + w]; ++o; }
-            if (uz) { for (int w = 0; w < 3; ++w) Uz[w] = lu[9 * o + 3 * e + w]; }
-            for (int w = 0; w < 3; ++w) D[w] = dinv[9 * Ic + 3 * e + w];
-        }
-        S* const xx = (S*)I.xch[0];
-        S* const xy = (S*)I.xch[1];
-        S* const xz = (S*)I.xch[2];
-        if (q < 3) {
-            if (ux && li + 1 == BX) sV[3 * (p + 1) + e] = receiveSlot(xx + 3 * (Ic + 1) + e, err);
-            if (uy && lj + 1 == BY) sV[3 * (p + PX) + e] = receiveSlot(xy + 3 * (Ic + g.nx) + e, err);
-            if (uz && lk + 1 == BZ) sV[3 * (p + PXY) + e] = receiveSlot(xz + 3 * (Ic + nxy) + e, err);
-        }
-        __syncthreads();
-        const int nLev = BX + BY + BZ - 2;
-        S out = S(0);
-        for (int lev = 0; lev <= nLev; ++lev) {
-            if (myLev == lev) {
-                const S sx = rowDot(Ux, sV + 3 * (p + 1));
-                const S sy = rowDot(Uy, sV + 3 * (p + PX));
-                const S sz = rowDot(Uz, sV + 3 * (p + PXY));
-                if (ux) r += S(-1) * sx;
-                if (uy) r += S(-1) * sy;
-                if (uz) r += S(-1) * sz;
-                const S r0 = quadLane<0>(r), r1 = quadLane<1>(r), r2 = quadLane<2>(r);
-                double w = D[0] * (double)r0;
-                w += D[1] * (double)r1;
-                w += D[2] * (double)r2;
-                out = S(0) + S(1) * (S)w;
-                if (q < 3) sV[3 * p + e] = out;
+        // exchange slots: direction d of cell I at xch[3 d N + 3 I + component]
+        S* const xch = (S*)I.xch;
+        const long N = nxy * g.nz;
+        if (wave == kRecvWave) {
+            // this lane's ghost values: gv = lane + 64 m runs over the tile's x-,
+            // y- and z-face cells (three values each) whose neighbour exists
+            const int nfx = BY * BZ, nfy = BX * BZ, nfz = BX * BY;
+            const int nG = 3 * (nfx + nfy + nfz);
+            // src: element offset of the slot in xch; dst: LDS index
+            int src[kRecvPerLane], dst[kRecvPerLane];
+            unsigned mask = 0;
+#pragma clang loop unroll(full)
+            for (int m = 0; m < kRecvPerLane; ++m) {
+                const int gv = lane + 64 * m;
+                src[m] = 0;
+                dst[m] = 0;
+                if (gv < nG) {
+                    const int comp = gv % 3, f = gv / 3;
+                    int dir, li, lj, lk;
+                    if (f < nfx) {
+                        dir = 0; lj = f % BY; lk = f / BY; li = Bwd ? BX - 1 : 0;
+                    } else if (f < nfx + nfy) {
+                        dir = 1; li = (f - nfx) % BX; lk = (f - nfx) / BX; lj = Bwd ? BY - 1 : 0;
+                    } else {
+                        dir = 2; li = (f - nfx - nfy) % BX; lj = (f - nfx - nfy) / BX; lk = Bwd ? BZ - 1 : 0;
+                    }
+                    const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+                    bool has = i < g.nx && j < g.ny && k < g.nz;
+                    if (dir == 0) has = has && (Bwd ? i + 1 < g.nx : i > 0);
+                    if (dir == 1) has = has && (Bwd ? j + 1 < g.ny : j > 0);
+                    if (dir == 2) has = has && (Bwd ? k + 1 < g.nz : k > 0);
+                    if (has) {
+                        const long Ic = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+                        const long stride = dir == 0 ? 1 : (dir == 1 ? (long)g.nx : nxy);
+                        const int pTile = Bwd ? li + PX * lj + PXY * lk : (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
+                        const int pStride = dir == 0 ? 1 : (dir == 1 ? PX : PXY);
+                        src[m] = (int)(3 * dir * N + 3 * (Bwd ? Ic + stride : Ic - stride) + comp);
+                        dst[m] = 3 * (Bwd ? pTile + pStride : pTile - pStride) + comp;
+                        mask |= 1u << m;
+                    }
+                }
             }
-            __syncthreads();
-        }
-        if (word >= 0 && q < 3) {
-            v[3 * Ic + e] = out;
-            if (li == 0 && TI * BX + li > 0) sendSlot(xx + 3 * Ic + e, out);
-            if (lj == 0 && TJ * BY + lj > 0) sendSlot(xy + 3 * Ic + e, out);
-            if (lk == 0 && TK * BZ + lk > 0) sendSlot(xz + 3 * Ic + e, out);
+            // continuous polling: every slot always has one load in flight; a load is
+            // re-issued as soon as its result has been looked at, so an arriving value
+            // is seen one memory latency after it lands.  All loads are unconditional
+            // (finished or unused slots re-read a harmless address) and the loop has
+            // no stores, so each wait covers exactly the one load it needs.
+            const unsigned all = mask;
+            S val[kRecvPerLane];
+#pragma clang loop unroll(full)
+            for (int m = 0; m < kRecvPerLane; ++m) val[m] = ldCoh(xch + src[m]);
+            int failed = loadRelaxed(err);
+            long spins = 0;
+            while (mask) {
+#pragma clang loop unroll(full)
+                for (int m = 0; m < kRecvPerLane; ++m) {
+                    const S x = val[m];
+                    val[m] = ldCoh(xch + src[m]);
+                    if (((mask >> m) & 1u) && !isEmpty(x)) {
+                        ldsStore(sV + dst[m], x);
+                        mask &= ~(1u << m);
+                    }
+                }
+                const int f = failed;
+                failed = loadRelaxed(err);
+                if (f || ++spins > kSpinLimit) {
+                    if (!f) atomicOr(err, 1);
+                    break;
+                }
+            }
+            // empty the slots taken (the next sweep's values will arrive in them)
+#pragma clang loop unroll(full)
+            for (int m = 0; m < kRecvPerLane; ++m)
+                if ((all >> m) & 1u) stCoh(xch + src[m], emptySlot<S>());
+        } else if (16 * wave < ncell) {
+            const int word = pos < ncell ? sOrder[pos] : 0;
+            const bool valid = pos < ncell && (word & (1 << 30));
+            const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
+            const bool h0 = valid && (word & (1 << 27)), h1 = valid && (word & (1 << 28));
+            const bool h2 = valid && (word & (1 << 29));
+            const long Ic = (long)(i0 + li) + (long)g.nx * ((long)(j0 + lj) + (long)g.ny * (k0 + lk));
+            const int p = Bwd ? li + PX * lj + PXY * lk : (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
+            const int d0 = Bwd ? p + 1 : p - PXY, d1 = Bwd ? p + PX : p - PX, d2 = Bwd ? p + PXY : p - 1;
+            auto levelOf = [&](int w) {
+                const int a = w & 0xFF, b = (w >> 8) & 0xFF, c = (w >> 16) & 0xFF;
+                return Bwd ? (BX - 1 - a) + (BY - 1 - b) + (BZ - 1 - c) : a + b + c;
+            };
+            const GS<S>* __restrict__ lu = toGlobal((const S*)I.lu);
+            GS<S>* __restrict__ v = toGlobal((S*)I.v);
+            // this lane's row of the cell's off-diagonal blocks (and pivot) and its right-hand side
+            S B0[3] = {}, B1[3] = {}, B2[3] = {};
+            double D[3] = {};
+            S r = S(0);
+            if (valid) {
+                long o;
+                if (Bwd) {
+                    r = v[3 * Ic + e];
+                    o = toGlobal(I.diag)[Ic] + 1;
+                    for (int w = 0; w < 3; ++w) D[w] = toGlobal(I.dinv)[9 * Ic + 3 * e + w];
+                } else {
+                    r = toGlobal((const S*)I.d)[3 * Ic + e];
+                    o = toGlobal(I.rp)[Ic];
+                }
+                if (h0) { for (int w = 0; w < 3; ++w) B0[w] = lu[9 * o + 3 * e + w]; ++o; }
+                if (h1) { for (int w = 0; w < 3; ++w) B1[w] = lu[9 * o + 3 * e + w]; ++o; }
+                if (h2) { for (int w = 0; w < 3; ++w) B2[w] = lu[9 * o + 3 * e + w]; }
+            }
+            // dependencies that are values of a neighbour tile (the only ones that
+            // can still be missing once their level is complete), and the faces
+            // this lane's row is sent across
+            const bool g0 = h0 && (Bwd ? li + 1 == BX : lk == 0);
+            const bool g1 = h1 && (Bwd ? lj + 1 == BY : lj == 0);
+            const bool g2 = h2 && (Bwd ? lk + 1 == BZ : li == 0);
+            const bool ghost = g0 || g1 || g2;
+            const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+            const bool row = valid && q < 3;
+            const bool s0 = row && (Bwd ? li == 0 && i > 0 : li + 1 == BX && i + 1 < g.nx);
+            const bool s1 = row && (Bwd ? lj == 0 && j > 0 : lj + 1 == BY && j + 1 < g.ny);
+            const bool s2 = row && (Bwd ? lk == 0 && k > 0 : lk + 1 == BZ && k + 1 < g.nz);
+            S* const own = xch + 3 * Ic + e;
+            // the rows are in registers before the levels start: the loop below then
+            // has only stores in flight, which none of its waits has to cover
+            waitStores();
+            const int myLev = valid ? levelOf(word) : -1;
+            const int lastLev = levelOf(sOrder[min(16 * wave + 15, ncell - 1)]);
+            S res = S(0);
+            for (int lev = levelOf(sOrder[16 * wave]); lev <= lastLev; ++lev) {
+                // level lev-1 complete: every wave holding part of it has counted it
+                // (one broadcast LDS word per poll)
+                if (lev > 0) {
+                    const int waves = sWaves[lev - 1];
+                    long spins = 0;
+                    while (ldsLoad(sDone + lev - 1) < waves) {
+                        if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
+                            atomicOr(err, 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                if (myLev == lev) {
+                    S x0[3], x1[3], x2[3];
+                    for (int w = 0; w < 3; ++w) {
+                        x0[w] = ldsLoad(sV + 3 * d0 + w);
+                        x1[w] = ldsLoad(sV + 3 * d1 + w);
+                        x2[w] = ldsLoad(sV + 3 * d2 + w);
+                    }
+                    if (ghost) {
+                        long spins = 0;
+                        for (;;) {
+                            unsigned empty = 0;
+                            for (int w = 0; w < 3; ++w)
+                                empty |= (unsigned)(g0 && isEmpty(x0[w])) | (unsigned)(g1 && isEmpty(x1[w])) |
+                                         (unsigned)(g2 && isEmpty(x2[w]));
+                            if (!empty) break;
+                            if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
+                                atomicOr(err, 1);
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                            for (int w = 0; w < 3; ++w) {
+                                if (g0) x0[w] = ldsLoad(sV + 3 * d0 + w);
+                                if (g1) x1[w] = ldsLoad(sV + 3 * d1 + w);
+                                if (g2) x2[w] = ldsLoad(sV + 3 * d2 + w);
+                            }
+                        }
+                    }
+                    const S t0 = rowDot(B0, x0), t1 = rowDot(B1, x1), t2 = rowDot(B2, x2);
+                    if (h0) r += S(-1) * t0;
+                    if (h1) r += S(-1) * t1;
+                    if (h2) r += S(-1) * t2;
+                    S out = r;
+                    if (Bwd) {
+                        const S r0 = quadLane<0>(r), r1 = quadLane<1>(r), r2 = quadLane<2>(r);
+                        double w = D[0] * (double)r0;
+                        w += D[1] * (double)r1;
+                        w += D[2] * (double)r2;
+                        out = S(0) + S(1) * (S)w;
+                    }
+                    res = out;
+                    if (q < 3) {
+                        ldsStore(sV + 3 * p + e, isEmpty(out) ? S(NAN) : out);
+                        if (s0) sendSlot(own, out);
+                        if (s1) sendSlot(own + 3 * N, out);
+                        if (s2) sendSlot(own + 6 * N, out);
+                    }
+                }
+                // count the level once this wave's results of it are in LDS
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_s_waitcnt(kWaitLds);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane == 0) __hip_atomic_fetch_add(sDone + lev, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            // the tile's own results leave after its levels, behind the face values
+            // the neighbour tiles are waiting for
+            if (row) v[3 * Ic + e] = res;
         }
         __syncthreads();
         t = shareTicket(next, &sTicket);
@@ -605,11 +713,20 @@ __global__ void k_fill_empty32(unsigned int* p, long n)
     if (k < n) p[k] = kEmpty32;
 }
 
+namespace {
+// what the sweep kernels can hold: cells per tile, the tile and its ghost
+// layer in LDS, and the ghost values of the receiving wave
+bool tileFits(int bx, int by, int bz)
+{
+    return bx * by * bz <= kStreamCells && (bx + 1) * (by + 1) * (bz + 1) <= kPadMax &&
+           3 * (by * bz + bx * bz + bx * by) <= 64 * kRecvPerLane && bx + by + bz - 2 < kMaxTileLevels;
+}
+}  // namespace
+
 TileGeom makeTileGeom(int nx, int ny, int nz, int bx, int by, int bz)
 {
     if (bx <= 0 || by <= 0 || bz <= 0) return makeTileGeom(nx, ny, nz);
-    if (bx * by * bz > kTileThreads || (bx + 1) * (by + 1) * (bz + 1) > kPadMax)
-        throw std::invalid_argument("tile shape exceeds the sweep kernels' limits");
+    if (!tileFits(bx, by, bz)) throw std::invalid_argument("tile shape exceeds the sweep kernels' limits");
     TileGeom g;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.BX = bx; g.BY = by; g.BZ = bz;
@@ -620,14 +737,16 @@ TileGeom makeTileGeom(int nx, int ny, int nz, int bx, int by, int bz)
 
 std::vector<TileGeom> tileShapeCandidates(int nx, int ny, int nz)
 {
-    static const int shapes[][3] = {{8, 8, 4}, {16, 4, 4}, {4, 16, 4}, {8, 4, 8}, {4, 8, 8}, {16, 8, 2},
-                                    {8, 16, 2}, {16, 16, 1}, {32, 8, 1}, {8, 32, 1}, {4, 4, 16}, {2, 8, 16}};
+    static const int shapes[][3] = {{6, 8, 5}, {8, 6, 5}, {8, 8, 3}, {10, 6, 4}, {6, 10, 4}, {12, 5, 4},
+                                    {6, 6, 6}, {16, 5, 3}, {5, 16, 3}, {4, 6, 10}, {12, 12, 1}, {16, 8, 1}};
     std::vector<TileGeom> out;
     out.push_back(makeTileGeom(nx, ny, nz));
     for (const auto& sh : shapes) {
-        // a tile no larger than the grid in any direction (powers of two below it)
+        // no tile much larger than the grid in any direction
         if (sh[0] > 2 * nx || sh[1] > 2 * ny || sh[2] > 2 * nz) continue;
-        const TileGeom g = makeTileGeom(nx, ny, nz, sh[0], sh[1], sh[2]);
+        const int bx = std::min(sh[0], nx), by = std::min(sh[1], ny), bz = std::min(sh[2], nz);
+        if (!tileFits(bx, by, bz)) continue;
+        const TileGeom g = makeTileGeom(nx, ny, nz, bx, by, bz);
         bool dup = false;
         for (const auto& o : out) dup = dup || (o.BX == g.BX && o.BY == g.BY && o.BZ == g.BZ);
         if (!dup) out.push_back(g);
@@ -637,26 +756,19 @@ std::vector<TileGeom> tileShapeCandidates(int nx, int ny, int nz)
 
 TileGeom makeTileGeom(int nx, int ny, int nz)
 {
-    // tile shape: 256 cells, as cubic as the grid allows
-    int bz = nz >= 4 ? 4 : (nz >= 2 ? 2 : 1);
-    int rest = kTileThreads / bz;
-    int bx = 1;
-    while (bx * bx < rest) bx *= 2;
-    int by = rest / bx;
-    if (nx < bx) { bx = 1; while (bx * 2 <= nx) bx *= 2; by = rest / bx; }
-    if (ny < by) { by = 1; while (by * 2 <= ny) by *= 2; }
-    // the sweeps stage a tile plus one ghost layer in LDS
-    while ((bx + 1) * (by + 1) * (bz + 1) > kPadMax) {
-        if (by >= bx && by >= bz) by /= 2;
-        else if (bx >= bz) bx /= 2;
-        else bz /= 2;
+    // tile shape: about kStreamCells cells, as cubic as the grid allows
+    const int bz = std::min(nz, 5);
+    const int rest = kStreamCells / bz;
+    int bx = std::min(nx, std::max(1, (int)std::sqrt((double)rest)));
+    int by = std::min(ny, rest / bx);
+    bx = std::min(nx, rest / by);
+    int cz = bz;
+    while (!tileFits(bx, by, cz)) {
+        if (by >= bx && by >= cz) --by;
+        else if (bx >= cz) --bx;
+        else --cz;
     }
-    TileGeom g;
-    g.nx = nx; g.ny = ny; g.nz = nz;
-    g.BX = bx; g.BY = by; g.BZ = bz;
-    g.TX = (nx + bx - 1) / bx; g.TY = (ny + by - 1) / by; g.TZ = (nz + bz - 1) / bz;
-    g.T = g.TX * g.TY * g.TZ;
-    return g;
+    return makeTileGeom(nx, ny, nz, bx, by, cz);
 }
 
 TiledIluSet::TiledIluSet(hipStream_t s)
@@ -684,10 +796,10 @@ TiledIluSet::TiledIluSet(hipStream_t s)
         return perCU * prop.multiProcessorCount;
     };
     residentFactor_ = resident({(const void*)k_ilu_factor_set}, kTileThreads);
-    residentSweep_[0][0] = resident({(const void*)k_ilu_fwd_set<double>}, kSweepThreads);
-    residentSweep_[0][1] = resident({(const void*)k_ilu_bwd_set<double>}, kSweepThreads);
-    residentSweep_[1][0] = resident({(const void*)k_ilu_fwd_set<float>}, kSweepThreads);
-    residentSweep_[1][1] = resident({(const void*)k_ilu_bwd_set<float>}, kSweepThreads);
+    residentSweep_[0][0] = resident({(const void*)k_ilu_sweep_set<double, false>}, kSweepThreads);
+    residentSweep_[0][1] = resident({(const void*)k_ilu_sweep_set<double, true>}, kSweepThreads);
+    residentSweep_[1][0] = resident({(const void*)k_ilu_sweep_set<float, false>}, kSweepThreads);
+    residentSweep_[1][1] = resident({(const void*)k_ilu_sweep_set<float, true>}, kSweepThreads);
 }
 
 void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s)
@@ -708,8 +820,12 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
     std::vector<TileTask> ord(tasks.size());
     for (size_t k = 0; k < tasks.size(); ++k) ord[k] = tasks[k].second;
     cellOff_.assign(geoms.size() + 1, 0);
-    for (size_t b = 0; b < geoms.size(); ++b)
-        cellOff_[b + 1] = cellOff_[b] + (long)geoms[b].nx * geoms[b].ny * geoms[b].nz;
+    for (size_t b = 0; b < geoms.size(); ++b) {
+        const long n = (long)geoms[b].nx * geoms[b].ny * geoms[b].nz;
+        // the sweeps address an instance's exchange slots with 32-bit offsets
+        if (9 * n >= (1L << 31)) throw std::invalid_argument("tiled ILU: more than 238 million cells in one grid");
+        cellOff_[b + 1] = cellOff_[b] + n;
+    }
     OM_HIP(hipStreamSynchronize(s));
     T_ = (int)ord.size();
     order_.upload(ord.data(), ord.size(), s);
@@ -755,8 +871,7 @@ const IluInst* TiledIluSet::upload(const std::vector<IluInst>& insts, hipStream_
     }
     for (size_t b = 0; b < v.size(); ++b) {
         v[b].done = done_.p + tileOff_[b];
-        const long nb = cellOff_[b + 1] - cellOff_[b];
-        for (int d = 0; d < 3; ++d) v[b].xch[d] = xbase + 9 * cellOff_[b] + 3L * d * nb;
+        v[b].xch = xbase + 9 * cellOff_[b];
     }
     // descriptor arrays stay resident: a solver alternates between a few
     // (input, output) vector pairs, each gets its own device copy once
@@ -813,13 +928,11 @@ void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
     unsigned long long* ct = counter_.p;
     int* er = err_.p;
     const int prec = sizeof(S) == 4 ? 1 : 0;
-    const void* kernels[2] = {(const void*)k_ilu_fwd_set<S>, (const void*)k_ilu_bwd_set<S>};
+    const void* kernels[2] = {(const void*)k_ilu_sweep_set<S, false>, (const void*)k_ilu_sweep_set<S, true>};
     unsigned int* ex = exits_.p;
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int grid = std::min(T_, residentSweep_[prec][sweep]);
-        long long b = 0;
-        int ep = 0;   // the sweeps exchange values through the slots, not the epoch flags
-        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&ex};
+        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex};
         OM_HIP(hipLaunchCooperativeKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
     }
 }
