@@ -47,10 +47,10 @@ CONFIGS = {
 # device copy of 256 MiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the commands and
 # per-kernel figures are in profiles/blackoil_500k_hbm_traffic.md.
 HBM_TRAFFIC_500K = {
-    "ilu_apply": 500703641,    # forward + backward sweep
-    "spmv": 630945894,
-    "assemble": 482937651,
-    "ilu_factor": 651998413,
+    "ilu_apply": 598595686,    # forward + backward sweep
+    "spmv": 347883724,
+    "assemble": 482900480,
+    "ilu_factor": 662758604,
 }
 
 
@@ -168,6 +168,29 @@ def copy_calibration(mib):
     hip.hipFree(b)
 
 
+def beside_runs(om, config, dt, steps=2):
+    """Newton iterations/s of workloads reported beside the headline: the
+    432k-cell black-oil grid (120x120x30) in FP64, and the headline grid with
+    the FP32 inner solve inside FP64 refinement.  One untimed step each first."""
+    out = {}
+    runs = [("c3_fp64", "c3", 0), ("%s_fp32_inner" % config, config, 1)]
+    for key, cfg_name, precision in runs:
+        cfg = CONFIGS[cfg_name]
+        case = om.BlackOilCase(om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024))
+        prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, autotune=1)
+        model = om.Model(case)
+        model.set_solution(case.pv0, case.mean0)
+        rc, _, _, _, _ = model.simulate(prm, 1, dt, keep_history=False)
+        t0 = time.time()
+        rc2, _, _, _, st = model.simulate(prm, steps, dt, keep_history=False)
+        wall = time.time() - t0
+        out[key] = {"workload": cfg["label"], "newton_iters_per_s": round(st.newton_iters / wall, 3),
+                    "newton_iters": st.newton_iters, "linear_iters_per_newton": round(st.linear_iters / max(st.newton_iters, 1), 2),
+                    "steps": steps, "ok": rc == 0 and rc2 == 0}
+        model.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +201,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 432k workload and the mixed-precision run reported beside the headline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (reference for HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -265,6 +290,8 @@ def main():
         "setup_s": round(t_setup, 3),
     }
 
+    if not args.no_extra:
+        line["beside"] = beside_runs(om, args.config, dt)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc

@@ -112,7 +112,10 @@ typedef struct omodels_params {
                                    inside FP64 flexible outer refinement; 2: FP32 solve
                                    (LinearSolverScalar = float, lens_immiscible_ecfv_ad.hh:53) */
     int32_t verbose;
-    int32_t ilu_schedule;       /* 0: persistent tile-wavefront sweeps; 1: one launch per level */
+    int32_t ilu_schedule;       /* 0: persistent tile-wavefront kernels, sweep form chosen from the
+                                   tiles per tile level (or by the autotuner); 1: one launch per
+                                   level; 2: tiles with streaming sweeps; 3: tiles with batched
+                                   sweeps.  Results do not depend on the choice */
     int32_t autotune;           /* 1: time the ILU tile shapes (and the level schedule) on the first
                                    factorization of a grid size and keep the fastest; results do
                                    not depend on the choice */

@@ -47,6 +47,8 @@ Ensemble::Ensemble(const omodels_params& prm, hipStream_t s) : prm_(prm)
     if (s) stream_ = s;
     else { OM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)); ownStream_ = true; }
     ilu_ = std::make_unique<TiledIluSet>(stream_);
+    // members always use the tiled kernels; ilu_schedule 2 / 3 force the streaming / batched sweeps
+    ilu_->setSweepMode(prm_.ilu_schedule == 2 ? 1 : (prm_.ilu_schedule == 3 ? 2 : 0));
     OM_HIP(hipHostMalloc((void**)&hostKs_, sizeof(KrylovScalars) * kMaxMembers, hipHostMallocDefault));
     OM_HIP(hipHostMalloc((void**)&hostD_, sizeof(double) * kMaxMembers, hipHostMallocDefault));
     OM_HIP(hipHostMalloc((void**)&hostI_, sizeof(int) * 2 * kMaxMembers, hipHostMallocDefault));

@@ -85,6 +85,9 @@ public:
     template <class S>
     void prime(const std::vector<IluInst>& insts, hipStream_t s) { (void)upload<S>(insts, s); }
     long generation() const { return generation_; }
+    // sweep form: 0 chosen from the tiles per tile level, 1 streaming, 2 batched
+    void setSweepMode(int mode) { mode_ = mode; }
+    bool streaming() const { return mode_ == 1 || (mode_ == 0 && autoStream_); }
     int totalTiles() const { return T_; }
     const std::vector<int>& tileOffsets() const { return tileOff_; }
 
@@ -103,7 +106,10 @@ private:
     std::vector<int> tileOff_;
     long generation_ = 0;
     int epoch_ = 0, T_ = 0, gridFactor_ = 0, residentFactor_ = 0;
-    int residentSweep_[2][2] = {};   // [FP64, FP32][forward, backward]
+    int residentSweep_[2][2] = {};   // streaming sweeps: [FP64, FP32][forward, backward]
+    int residentBatch_[2][2] = {};   // batched sweeps
+    int mode_ = 0;
+    bool autoStream_ = true;
 };
 
 }  // namespace omodels

@@ -134,13 +134,16 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
     OM_HIP(hipHostMalloc((void**)&ksHost_, sizeof(KrylovScalars), hipHostMallocDefault));
     if (prm_.precision < 0 || prm_.precision > 2) throw std::invalid_argument("precision must be 0, 1 or 2");
     if (prm_.precision == 2 && prm_.precond != 0) throw std::invalid_argument("the FP32 solve uses block ILU(0)");
-    if (prm_.precond == 0 && prm_.ilu_schedule == 0 && m.bs == 3 && prm_.precision != 2) {
+    if (prm_.ilu_schedule < 0 || prm_.ilu_schedule > 3) throw std::invalid_argument("ilu_schedule must be 0..3");
+    const bool tiles = prm_.ilu_schedule != 1;
+    if (prm_.precond == 0 && tiles && m.bs == 3 && prm_.precision != 2) {
         TunedIlu t;
-        if (prm_.autotune && lookupTunedIlu(m.gx, m.gy, m.gz, prm_.precision, t)) {
+        if (prm_.autotune && prm_.ilu_schedule == 0 && lookupTunedIlu(m.gx, m.gy, m.gz, prm_.precision, t)) {
             tuned_ = true;
-            if (!t.level) tiled_ = std::make_unique<TiledIlu>(t.shape, m.stream);
+            if (!t.level) tiled_ = std::make_unique<TiledIlu>(t.shape, m.stream, t.sweep);
         } else {
-            tiled_ = std::make_unique<TiledIlu>(m.gx, m.gy, m.gz, m.stream);
+            const int mode = prm_.ilu_schedule == 2 ? 1 : (prm_.ilu_schedule == 3 ? 2 : 0);
+            tiled_ = std::make_unique<TiledIlu>(makeTileGeom(m.gx, m.gy, m.gz), m.stream, mode);
         }
     }
     if (prm_.precision == 0) {
@@ -218,32 +221,51 @@ void BiCGStabBackend::tuneIlu()
         std::sort(a.begin(), a.end());
         return (double)fms + 50.0 * a[kApply / 2];
     };
+    // candidates: every tile shape in both sweep forms (streaming, batched), then
+    // the level schedule; candidate 0 is the default shape in the form the
+    // default rule picks for it
     const std::vector<TileGeom> shapes = tileShapeCandidates(m_.gx, m_.gy, m_.gz);
-    std::vector<double> cost(shapes.size() + 1, 1e300);   // last entry: level schedule
+    struct Cand { int shape, sweep; };
+    std::vector<Cand> cands;
+    {
+        TiledIlu probe(shapes[0], s);
+        const int first = probe.streaming() ? 1 : 2;
+        cands.push_back({0, first});
+        cands.push_back({0, 3 - first});
+    }
+    for (size_t c = 1; c < shapes.size(); ++c) {
+        cands.push_back({(int)c, 1});
+        cands.push_back({(int)c, 2});
+    }
+    std::vector<double> cost(cands.size() + 1, 1e300);   // last entry: level schedule
     for (int round = 0; round < 2; ++round) {
-        for (size_t c = 0; c < shapes.size(); ++c) {
-            TiledIlu t(shapes[c], s);
+        for (size_t c = 0; c < cands.size(); ++c) {
+            TiledIlu t(shapes[cands[c].shape], s, cands[c].sweep);
             cost[c] = std::min(cost[c], run(&t));
         }
         cost.back() = std::min(cost.back(), run(nullptr));
     }
     for (auto& e : ev) OM_HIP(hipEventDestroy(e));
-    size_t best = 0;   // candidate 0 is the default shape
+    size_t best = 0;
     for (size_t c = 1; c < cost.size(); ++c)
         if (cost[c] < cost[best]) best = c;
     if (cost[0] <= 1.05 * cost[best]) best = 0;
     if (prm_.verbose) {
-        for (size_t c = 0; c < shapes.size(); ++c)
-            printf("autotune ILU tile %dx%dx%d: %.3f ms\n", shapes[c].BX, shapes[c].BY, shapes[c].BZ, cost[c]);
+        for (size_t c = 0; c < cands.size(); ++c)
+            printf("autotune ILU tile %dx%dx%d %s: %.3f ms\n", shapes[cands[c].shape].BX, shapes[cands[c].shape].BY,
+                   shapes[cands[c].shape].BZ, cands[c].sweep == 1 ? "streaming" : "batched", cost[c]);
         printf("autotune ILU level schedule: %.3f ms; chosen: %s\n", cost.back(),
-               best == shapes.size() ? "level" : "tiles");
+               best == cands.size() ? "level" : "tiles");
     }
     TunedIlu choice;
-    choice.level = best == shapes.size();
-    if (!choice.level) choice.shape = shapes[best];
+    choice.level = best == cands.size();
+    if (!choice.level) {
+        choice.shape = shapes[cands[best].shape];
+        choice.sweep = cands[best].sweep;
+    }
     storeTunedIlu(m_.gx, m_.gy, m_.gz, prm_.precision, choice);
     if (choice.level) tiled_.reset();
-    else tiled_ = std::make_unique<TiledIlu>(choice.shape, s);
+    else tiled_ = std::make_unique<TiledIlu>(choice.shape, s, choice.sweep);
 }
 
 void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidual)
@@ -255,7 +277,7 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     else A64_ = m_.jac.p;
     if (hostResidual) b64_.upload(hostResidual, n, s);
     else OM_HIP(hipMemcpyAsync(b64_.p, m_.res.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (prm_.autotune && tiled_ && !tuned_) tuneIlu();
+    if (prm_.autotune && prm_.ilu_schedule == 0 && tiled_ && !tuned_) tuneIlu();
     OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
     const int N = (int)m_.N;
     if (prm_.precision == 2) {
@@ -407,8 +429,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
         m_.timer.end(s);
         launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, 1, s);
-        launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, s);
-        launchAdvanceTag(ks_.p, 2, s);
+        launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, 2, s);
     };
 
     IterGraphs* gr = nullptr;

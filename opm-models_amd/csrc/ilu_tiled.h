@@ -28,6 +28,7 @@ class TiledIluSet;
 struct TunedIlu {
     TileGeom shape{};
     bool level = false;
+    int sweep = 0;      // sweep form of the tiled kernels: 1 streaming, 2 batched
 };
 bool lookupTunedIlu(int nx, int ny, int nz, int precision, TunedIlu& out);
 void storeTunedIlu(int nx, int ny, int nz, int precision, const TunedIlu& t);
@@ -36,7 +37,10 @@ void storeTunedIlu(int nx, int ny, int nz, int precision, const TunedIlu& t);
 class TiledIlu {
 public:
     TiledIlu(int nx, int ny, int nz, hipStream_t s);
-    TiledIlu(const TileGeom& shape, hipStream_t s);
+    // sweepMode: 0 chosen from the tiles per tile level, 1 streaming, 2 batched
+    TiledIlu(const TileGeom& shape, hipStream_t s, int sweepMode = 0);
+    void setSweepMode(int mode);
+    bool streaming() const;
     ~TiledIlu();
     TiledIlu(const TiledIlu&) = delete;
     TiledIlu& operator=(const TiledIlu&) = delete;

@@ -549,6 +549,198 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
     leaveLaunch(counter, exits);
 }
 
+// Batched sweeps: the same lane layout without a receiving wave.  A tile waits
+// for all of its neighbour tiles' face values, stages them as the LDS ghost
+// layer, runs its levels separated by workgroup barriers and sends its own
+// face values after its last level.  A level costs less than in the streaming
+// sweeps (one barrier instead of per-level counts), a tile hop more (a tile
+// starts when its upstream tiles have finished); with many more tiles per tile
+// level than resident workgroups (large ensembles) the sweeps are throughput-
+// rather than latency-bound and this form is the faster one.
+// Forward sweep: v = L^-1 d (unit lower), lower blocks in column order -z,-y,-x.
+template <class S>
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_batch(const IluInst* __restrict__ insts,
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
+{
+    __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
+    __shared__ int sOrder[kStreamCells];   // cells in level order
+    __shared__ int sTicket;
+    const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
+    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
+    for (;;) {
+        if (t < 0 || t >= T) break;
+        int next = 0;
+        if (threadIdx.x == 0) next = drawTicket(counter, 0);
+        const TileTask task = order[t];
+        const IluInst I = insts[task.inst];
+        if (!isActive(I)) {
+            t = shareTicket(next, &sTicket);
+            continue;
+        }
+        const TileGeom& g = I.g;
+        const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
+        const GS<S>* __restrict__ lu = toGlobal((const S*)I.lu);
+        const GS<S>* __restrict__ d = toGlobal((const S*)I.d);
+        GS<S>* __restrict__ v = toGlobal((S*)I.v);
+        const long nxy = (long)g.nx * g.ny;
+        int TI, TJ, TK;
+        tileOf(g, task.tile, TI, TJ, TK);
+        if (threadIdx.x < BX * BY * BZ) {
+            const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
+            const int i = TI * BX + li, j = TJ * BY + lj, k = TK * BZ + lk;
+            sOrder[cellsBelow(li + lj + lk, BX, BY, BZ) + rankInLevel(li, lj, lk, BX, BY)] =
+                (i < g.nx && j < g.ny && k < g.nz) ? packCell(li, lj, lk, k > 0, j > 0, i > 0, true) : -1;
+        }
+        __syncthreads();
+        const int word = pos < BX * BY * BZ ? sOrder[pos] : -1;
+        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
+        const bool hz = word >= 0 && (word & (1 << 27)), hy = word >= 0 && (word & (1 << 28));
+        const bool hx = word >= 0 && (word & (1 << 29));
+        const long Ic = (long)(TI * BX + li) + (long)g.nx * ((long)(TJ * BY + lj) + (long)g.ny * (TK * BZ + lk));
+        const int p = (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
+        const int myLev = word >= 0 ? li + lj + lk : -1;
+        // this lane's row of the cell's lower blocks and its right-hand side
+        S Lz[3] = {}, Ly[3] = {}, Lx[3] = {};
+        S r = S(0);
+        if (word >= 0) {
+            r = d[3 * Ic + e];
+            long o = toGlobal(I.rp)[Ic];
+            if (hz) { for (int w = 0; w < 3; ++w) Lz[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (hy) { for (int w = 0; w < 3; ++w) Ly[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (hx) { for (int w = 0; w < 3; ++w) Lx[w] = lu[9 * o + 3 * e + w]; }
+        }
+        // ghosts: the neighbour tiles' face values, received through the exchange
+        // slots in one batch before the tile's levels (receiving them level by
+        // level would put a memory round trip on every level)
+        S* const xx = (S*)I.xch;
+        S* const xy = xx + 3 * nxy * g.nz;
+        S* const xz = xx + 6 * nxy * g.nz;
+        if (q < 3) {
+            if (hz && lk == 0) sV[3 * (p - PXY) + e] = receiveSlot(xz + 3 * (Ic - nxy) + e, err);
+            if (hy && lj == 0) sV[3 * (p - PX) + e] = receiveSlot(xy + 3 * (Ic - g.nx) + e, err);
+            if (hx && li == 0) sV[3 * (p - 1) + e] = receiveSlot(xx + 3 * (Ic - 1) + e, err);
+        }
+        __syncthreads();
+        const int nLev = BX + BY + BZ - 2;
+        for (int lev = 0; lev <= nLev; ++lev) {
+            if (myLev == lev) {
+                const S sz = rowDot(Lz, sV + 3 * (p - PXY));
+                const S sy = rowDot(Ly, sV + 3 * (p - PX));
+                const S sx = rowDot(Lx, sV + 3 * (p - 1));
+                if (hz) r += S(-1) * sz;
+                if (hy) r += S(-1) * sy;
+                if (hx) r += S(-1) * sx;
+                if (q < 3) sV[3 * p + e] = r;
+            }
+            __syncthreads();
+        }
+        if (word >= 0 && q < 3) {
+            v[3 * Ic + e] = r;
+            if (li + 1 == BX && TI * BX + li + 1 < g.nx) sendSlot(xx + 3 * Ic + e, r);
+            if (lj + 1 == BY && TJ * BY + lj + 1 < g.ny) sendSlot(xy + 3 * Ic + e, r);
+            if (lk + 1 == BZ && TK * BZ + lk + 1 < g.nz) sendSlot(xz + 3 * Ic + e, r);
+        }
+        __syncthreads();
+        t = shareTicket(next, &sTicket);
+    }
+    leaveLaunch(counter, exits);
+}
+
+// Backward sweep: v = U^-1 v with the inverted (FP64) pivots; upper blocks +x,+y,+z.
+template <class S>
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_batch(const IluInst* __restrict__ insts,
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
+{
+    __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
+    __shared__ int sOrder[kStreamCells];
+    __shared__ int sTicket;
+    const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
+    int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
+    for (;;) {
+        if (t < 0 || t >= T) break;
+        int next = 0;
+        if (threadIdx.x == 0) next = drawTicket(counter, 0);
+        const TileTask task = order[T - 1 - t];
+        const IluInst I = insts[task.inst];
+        if (!isActive(I)) {
+            t = shareTicket(next, &sTicket);
+            continue;
+        }
+        const TileGeom& g = I.g;
+        const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
+        const GS<S>* __restrict__ lu = toGlobal((const S*)I.lu);
+        const GS<double>* __restrict__ dinv = toGlobal(I.dinv);
+        GS<S>* __restrict__ v = toGlobal((S*)I.v);
+        const long nxy = (long)g.nx * g.ny;
+        int TI, TJ, TK;
+        tileOf(g, task.tile, TI, TJ, TK);
+        if (threadIdx.x < BX * BY * BZ) {
+            const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
+            const int i = TI * BX + li, j = TJ * BY + lj, k = TK * BZ + lk;
+            const int a = BX - 1 - li, b = BY - 1 - lj, c = BZ - 1 - lk;
+            sOrder[cellsBelow(a + b + c, BX, BY, BZ) + rankInLevel(a, b, c, BX, BY)] =
+                (i < g.nx && j < g.ny && k < g.nz)
+                    ? packCell(li, lj, lk, i + 1 < g.nx, j + 1 < g.ny, k + 1 < g.nz, true) : -1;
+        }
+        __syncthreads();
+        const int word = pos < BX * BY * BZ ? sOrder[pos] : -1;
+        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
+        const bool ux = word >= 0 && (word & (1 << 27)), uy = word >= 0 && (word & (1 << 28));
+        const bool uz = word >= 0 && (word & (1 << 29));
+        const long Ic = (long)(TI * BX + li) + (long)g.nx * ((long)(TJ * BY + lj) + (long)g.ny * (TK * BZ + lk));
+        const int p = li + PX * lj + PXY * lk;
+        const int myLev = word >= 0 ? (BX - 1 - li) + (BY - 1 - lj) + (BZ - 1 - lk) : -1;
+        S Ux[3] = {}, Uy[3] = {}, Uz[3] = {};
+        double D[3] = {};
+        S r = S(0);
+        if (word >= 0) {
+            r = v[3 * Ic + e];
+            long o = toGlobal(I.diag)[Ic] + 1;
+            if (ux) { for (int w = 0; w < 3; ++w) Ux[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (uy) { for (int w = 0; w < 3; ++w) Uy[w] = lu[9 * o + 3 * e + w]; ++o; }
+            if (uz) { for (int w = 0; w < 3; ++w) Uz[w] = lu[9 * o + 3 * e + w]; }
+            for (int w = 0; w < 3; ++w) D[w] = dinv[9 * Ic + 3 * e + w];
+        }
+        S* const xx = (S*)I.xch;
+        S* const xy = xx + 3 * nxy * g.nz;
+        S* const xz = xx + 6 * nxy * g.nz;
+        if (q < 3) {
+            if (ux && li + 1 == BX) sV[3 * (p + 1) + e] = receiveSlot(xx + 3 * (Ic + 1) + e, err);
+            if (uy && lj + 1 == BY) sV[3 * (p + PX) + e] = receiveSlot(xy + 3 * (Ic + g.nx) + e, err);
+            if (uz && lk + 1 == BZ) sV[3 * (p + PXY) + e] = receiveSlot(xz + 3 * (Ic + nxy) + e, err);
+        }
+        __syncthreads();
+        const int nLev = BX + BY + BZ - 2;
+        S out = S(0);
+        for (int lev = 0; lev <= nLev; ++lev) {
+            if (myLev == lev) {
+                const S sx = rowDot(Ux, sV + 3 * (p + 1));
+                const S sy = rowDot(Uy, sV + 3 * (p + PX));
+                const S sz = rowDot(Uz, sV + 3 * (p + PXY));
+                if (ux) r += S(-1) * sx;
+                if (uy) r += S(-1) * sy;
+                if (uz) r += S(-1) * sz;
+                const S r0 = quadLane<0>(r), r1 = quadLane<1>(r), r2 = quadLane<2>(r);
+                double w = D[0] * (double)r0;
+                w += D[1] * (double)r1;
+                w += D[2] * (double)r2;
+                out = S(0) + S(1) * (S)w;
+                if (q < 3) sV[3 * p + e] = out;
+            }
+            __syncthreads();
+        }
+        if (word >= 0 && q < 3) {
+            v[3 * Ic + e] = out;
+            if (li == 0 && TI * BX + li > 0) sendSlot(xx + 3 * Ic + e, out);
+            if (lj == 0 && TJ * BY + lj > 0) sendSlot(xy + 3 * Ic + e, out);
+            if (lk == 0 && TK * BZ + lk > 0) sendSlot(xz + 3 * Ic + e, out);
+        }
+        __syncthreads();
+        t = shareTicket(next, &sTicket);
+    }
+    leaveLaunch(counter, exits);
+}
+
 // ---------------------------------------------------------------------------
 // Factorization: for the 7-point pattern ILU(0) only changes the lower blocks
 // (L_IJ = A_IJ A_JJ^-1) and the pivot (A_II -= L_IJ A_JI for J = -z,-y,-x in
@@ -800,6 +992,10 @@ TiledIluSet::TiledIluSet(hipStream_t s)
     residentSweep_[0][1] = resident({(const void*)k_ilu_sweep_set<double, true>}, kSweepThreads);
     residentSweep_[1][0] = resident({(const void*)k_ilu_sweep_set<float, false>}, kSweepThreads);
     residentSweep_[1][1] = resident({(const void*)k_ilu_sweep_set<float, true>}, kSweepThreads);
+    residentBatch_[0][0] = resident({(const void*)k_ilu_fwd_batch<double>}, kSweepThreads);
+    residentBatch_[0][1] = resident({(const void*)k_ilu_bwd_batch<double>}, kSweepThreads);
+    residentBatch_[1][0] = resident({(const void*)k_ilu_fwd_batch<float>}, kSweepThreads);
+    residentBatch_[1][1] = resident({(const void*)k_ilu_bwd_batch<float>}, kSweepThreads);
 }
 
 void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s)
@@ -838,6 +1034,12 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
     xch32_.release();
     resetExchange(s);
     gridFactor_ = std::min(T_, residentFactor_);
+    // streaming pays off while a tile level has fewer tiles than there are
+    // resident workgroups (the sweeps wait on the dependency chain); with more,
+    // they are throughput-bound and the batched form does less work per tile
+    int levels = 1;
+    for (const TileGeom& g : geoms) levels = std::max(levels, g.TX + g.TY + g.TZ - 2);
+    autoStream_ = (double)T_ / levels < residentSweep_[0][0];
     dcache_.clear();
     ++generation_;
 }
@@ -928,10 +1130,12 @@ void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
     unsigned long long* ct = counter_.p;
     int* er = err_.p;
     const int prec = sizeof(S) == 4 ? 1 : 0;
-    const void* kernels[2] = {(const void*)k_ilu_sweep_set<S, false>, (const void*)k_ilu_sweep_set<S, true>};
+    const bool stream = streaming();
+    const void* kernels[2] = {stream ? (const void*)k_ilu_sweep_set<S, false> : (const void*)k_ilu_fwd_batch<S>,
+                              stream ? (const void*)k_ilu_sweep_set<S, true> : (const void*)k_ilu_bwd_batch<S>};
     unsigned int* ex = exits_.p;
     for (int sweep = 0; sweep < 2; ++sweep) {
-        const int grid = std::min(T_, residentSweep_[prec][sweep]);
+        const int grid = std::min(T_, stream ? residentSweep_[prec][sweep] : residentBatch_[prec][sweep]);
         void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex};
         OM_HIP(hipLaunchCooperativeKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
     }
@@ -968,11 +1172,15 @@ TiledIlu::TiledIlu(int nx, int ny, int nz, hipStream_t s) : set_(new TiledIluSet
     set_->setInstances({g}, s);
 }
 
-TiledIlu::TiledIlu(const TileGeom& shape, hipStream_t s) : set_(new TiledIluSet(s))
+TiledIlu::TiledIlu(const TileGeom& shape, hipStream_t s, int sweepMode) : set_(new TiledIluSet(s))
 {
     g = shape;
     set_->setInstances({g}, s);
+    set_->setSweepMode(sweepMode);
 }
+
+void TiledIlu::setSweepMode(int mode) { set_->setSweepMode(mode); }
+bool TiledIlu::streaming() const { return set_->streaming(); }
 
 TiledIlu::~TiledIlu() { delete set_; }
 

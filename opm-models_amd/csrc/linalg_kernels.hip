@@ -502,14 +502,6 @@ void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScala
 }
 
 // next iteration: the half-iteration tags move on by two
-__global__ void k_advance_tag(KrylovScalars* ks, int delta)
-{
-    if (threadIdx.x == 0) ks->tag_base += delta;
-}
-void launchAdvanceTag(KrylovScalars* ks, int delta, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_advance_tag, dim3(1), dim3(64), 0, st, ks, delta);
-}
 template void launchDotFinal<double>(long, const double*, const double*, double*, KrylovScalars*, int, int, hipStream_t);
 template void launchDotFinal<float>(long, const float*, const float*, double*, KrylovScalars*, int, int, hipStream_t);
 
@@ -634,21 +626,22 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr2(long m, const S* __r
     if (threadIdx.x == 0) part[blockIdx.x] = nz;
 }
 __global__ __launch_bounds__(kRedThreads) void k_stag_final(const double* __restrict__ part, KrylovScalars* ks,
-                                                            int tagOff)
+                                                            int tagOff, int advance)
 {
     const double nz = finalReduce<true>(part);
     if (threadIdx.x != 0) return;
     if (nz == 0.0 && ks->stag_full_tag < 0) ks->stag_full_tag = ks->tag_base + tagOff;
+    ks->tag_base += advance;
 }
 template <class S>
 void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial, KrylovScalars* ks, int tagOff,
-                     hipStream_t st)
+                     int advance, hipStream_t st)
 {
     hipLaunchKernelGGL((k_update_xr2<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, z, t, x, r, partial, ks);
-    hipLaunchKernelGGL(k_stag_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, tagOff);
+    hipLaunchKernelGGL(k_stag_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, tagOff, advance);
 }
-template void launchUpdateXR2<double>(long, const double*, const double*, double*, double*, double*, KrylovScalars*, int, hipStream_t);
-template void launchUpdateXR2<float>(long, const float*, const float*, float*, float*, double*, KrylovScalars*, int, hipStream_t);
+template void launchUpdateXR2<double>(long, const double*, const double*, double*, double*, double*, KrylovScalars*, int, int, hipStream_t);
+template void launchUpdateXR2<float>(long, const float*, const float*, float*, float*, double*, KrylovScalars*, int, int, hipStream_t);
 
 // max |x| -> out (device)
 template <class S>

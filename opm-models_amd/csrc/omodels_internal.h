@@ -164,16 +164,16 @@ template <class S> void launchExtractInvDiag(int n, int bs, const int* diag, con
 // BiCGStab steps (see bicgstab in host_solver.cpp for the sequence)
 // tagOff: 0 for the first half of an iteration, 1 for the second (tag = ks->tag_base + tagOff)
 template <class S> void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks,
-                                       int op, int tag, hipStream_t st);
+                                       int op, int tagOff, hipStream_t st);
 template <class S> void launchUpdateP(long m, const S* r, const S* v, S* p, const KrylovScalars* ks,
                                       hipStream_t st);
 template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* r, double* partial,
                                        KrylovScalars* ks, hipStream_t st);
+// the second half ends the iteration: its final kernel advances tag_base by `advance`
 template <class S> void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial,
-                                        KrylovScalars* ks, int tag, hipStream_t st);
-void launchAdvanceTag(KrylovScalars* ks, int delta, hipStream_t st);
+                                        KrylovScalars* ks, int tagOff, int advance, hipStream_t st);
 template <class S> void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks,
-                                        int tag, hipStream_t st);
+                                        int tagOff, hipStream_t st);
 template <class S> void launchMaxAbs(long m, const S* x, double* partial, double* out, hipStream_t st);
 void launchResidualUpdate(int n, int bs, const int* rp, const int* ci, const double* a, const double* x,
                           const double* b, double* r, hipStream_t st);

@@ -83,12 +83,13 @@ def _system(dims, seed=1):
     return c, J, r, pat
 
 
-@pytest.mark.parametrize("schedule", [0, 1, "tuned"])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, "tuned"])
 @pytest.mark.parametrize("dims", [(5, 1, 1), (6, 5, 1), (1, 3, 9), (7, 5, 4), (20, 16, 6), (37, 11, 9)])
 def test_ilu0_factor_and_apply_parity(dims, schedule):
-    """Both schedules (persistent tile wavefront, one launch per level), and
-    whatever tile shape or schedule the autotuner picks, reproduce the
-    sequential ILU(0) factor and sweeps bit for bit."""
+    """Every schedule (tile wavefront with the sweep form chosen, streaming or
+    batched sweeps, one launch per level), and whatever tile shape, sweep form
+    or schedule the autotuner picks, reproduce the sequential ILU(0) factor and
+    sweeps bit for bit."""
     c, J, r, pat = _system(dims)
     m = om.Model(c)
     if schedule == "tuned":

@@ -45,10 +45,14 @@ def test_ensemble_members_match_the_oracle():
         assert np.array_equal(r["mean"], mo)
 
 
+@pytest.mark.parametrize("schedule", [0, 2, 3])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_member_result_is_identical_alone_and_in_the_mix(precision):
+def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule):
+    """schedule 0 lets the set choose its sweep form (one member alone and the
+    mix of four may choose differently), 2 / 3 force the streaming / batched
+    sweeps: the arithmetic is the same in every case."""
     ms = _members()
-    prm = om.default_params(precision=precision)
+    prm = om.default_params(precision=precision, ilu_schedule=schedule)
     srv, mixed = _serve(ms, prm)
     for k, c in enumerate(ms):
         srv1, alone = _serve([c], prm)
