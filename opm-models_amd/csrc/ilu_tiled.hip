@@ -10,10 +10,15 @@
 // tile depends on its -x, -y and -z neighbour tiles (+x, +y, +z for the
 // backward sweep) of the same instance; tiles of all instances are handed out
 // by one atomic ticket in order of tile level (TI + TJ + TK), so every
-// dependency of a tile holds a smaller ticket.  The kernels are launched
-// cooperatively (all workgroups co-resident), hence the tile with the lowest
-// unfinished ticket can always proceed and every wait ends; waits are also
-// bounded and report an error instead of spinning forever.
+// dependency of a tile holds a smaller ticket.  Only a running workgroup draws
+// tickets, and it draws its next ticket while working on a smaller one, so the
+// holder of the lowest unfinished ticket is always running and its
+// dependencies are finished: it proceeds, and by induction every wait ends,
+// whether or not all workgroups of the launch are resident at once.  The
+// kernels are therefore launched as ordinary kernels (a cooperative launch
+// would guarantee co-residency the schedule does not need, at the price of an
+// idle gap of about 20 us around every launch).  Waits are also bounded and
+// report an error instead of spinning forever.
 //
 // The factorization works one thread per cell: the tile's BX+BY+BZ-2 local
 // levels are separated by workgroup barriers, and a tile publishes a done flag
@@ -975,8 +980,9 @@ TiledIluSet::TiledIluSet(hipStream_t s)
     OM_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     OM_HIP(hipGetDeviceProperties(&prop, dev));
-    // co-resident workgroups of the factorization and of the sweeps (the
-    // factorization holds far more registers per thread)
+    // workgroups of the factorization and of the sweeps that fit on the device
+    // at once (the factorization holds far more registers per thread); the
+    // launches use no more, so no workgroup waits for another one to leave
     auto resident = [&](std::initializer_list<const void*> ks, int threads) {
         int perCU = 1 << 20;
         for (const void* k : ks) {
@@ -1117,7 +1123,7 @@ void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream
     int* er = err_.p;
     unsigned int* ex = exits_.p;
     void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&fail, (void*)&ex};
-    OM_HIP(hipLaunchCooperativeKernel((const void*)k_ilu_factor_set, dim3(gridFactor_), dim3(kTileThreads), args, 0, s));
+    OM_HIP(hipLaunchKernel((const void*)k_ilu_factor_set, dim3(gridFactor_), dim3(kTileThreads), args, 0, s));
 }
 
 template <class S>
@@ -1137,7 +1143,7 @@ void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int grid = std::min(T_, stream ? residentSweep_[prec][sweep] : residentBatch_[prec][sweep]);
         void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex};
-        OM_HIP(hipLaunchCooperativeKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
+        OM_HIP(hipLaunchKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
     }
 }
 template void TiledIluSet::apply<double>(const std::vector<IluInst>&, hipStream_t);
