@@ -458,13 +458,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
                 if (h1) { for (int w = 0; w < 3; ++w) B1[w] = lu[9 * o + 3 * e + w]; ++o; }
                 if (h2) { for (int w = 0; w < 3; ++w) B2[w] = lu[9 * o + 3 * e + w]; }
             }
-            // dependencies that are values of a neighbour tile (the only ones that
-            // can still be missing once their level is complete), and the faces
-            // this lane's row is sent across
-            const bool g0 = h0 && (Bwd ? li + 1 == BX : lk == 0);
-            const bool g1 = h1 && (Bwd ? lj + 1 == BY : lj == 0);
-            const bool g2 = h2 && (Bwd ? lk + 1 == BZ : li == 0);
-            const bool ghost = g0 || g1 || g2;
+            // the faces this lane's row is sent across
             const int i = i0 + li, j = j0 + lj, k = k0 + lk;
             const bool row = valid && q < 3;
             const bool s0 = row && (Bwd ? li == 0 && i > 0 : li + 1 == BX && i + 1 < g.nx);
@@ -478,12 +472,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
             const int lastLev = levelOf(sOrder[min(16 * wave + 15, ncell - 1)]);
             S res = S(0);
             for (int lev = levelOf(sOrder[16 * wave]); lev <= lastLev; ++lev) {
-                // level lev-1 complete: every wave holding part of it has counted it
-                // (one broadcast LDS word per poll)
-                if (lev > 0) {
-                    const int waves = sWaves[lev - 1];
+                // level lev-2 complete: every wave holding part of it has counted it
+                // (one broadcast LDS word per poll; the waves still further ahead
+                // wait here without loading the LDS)
+                if (lev > 1) {
+                    const int waves = sWaves[lev - 2];
                     long spins = 0;
-                    while (ldsLoad(sDone + lev - 1) < waves) {
+                    while (ldsLoad(sDone + lev - 2) < waves) {
                         if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
                             atomicOr(err, 1);
                             break;
@@ -492,30 +487,25 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
                     }
                 }
                 if (myLev == lev) {
+                    // the dependencies themselves (level lev-1 and ghost values) are
+                    // polled: a slot leaves the sentinel exactly once per tile, with its
+                    // final value, so the last step of the chain costs no count
                     S x0[3], x1[3], x2[3];
-                    for (int w = 0; w < 3; ++w) {
-                        x0[w] = ldsLoad(sV + 3 * d0 + w);
-                        x1[w] = ldsLoad(sV + 3 * d1 + w);
-                        x2[w] = ldsLoad(sV + 3 * d2 + w);
-                    }
-                    if (ghost) {
-                        long spins = 0;
-                        for (;;) {
-                            unsigned empty = 0;
-                            for (int w = 0; w < 3; ++w)
-                                empty |= (unsigned)(g0 && isEmpty(x0[w])) | (unsigned)(g1 && isEmpty(x1[w])) |
-                                         (unsigned)(g2 && isEmpty(x2[w]));
-                            if (!empty) break;
-                            if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
-                                atomicOr(err, 1);
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
-                            for (int w = 0; w < 3; ++w) {
-                                if (g0) x0[w] = ldsLoad(sV + 3 * d0 + w);
-                                if (g1) x1[w] = ldsLoad(sV + 3 * d1 + w);
-                                if (g2) x2[w] = ldsLoad(sV + 3 * d2 + w);
-                            }
+                    long spins = 0;
+                    for (;;) {
+                        for (int w = 0; w < 3; ++w) {
+                            x0[w] = ldsLoad(sV + 3 * d0 + w);
+                            x1[w] = ldsLoad(sV + 3 * d1 + w);
+                            x2[w] = ldsLoad(sV + 3 * d2 + w);
+                        }
+                        unsigned empty = 0;
+                        for (int w = 0; w < 3; ++w)
+                            empty |= (unsigned)(h0 && isEmpty(x0[w])) | (unsigned)(h1 && isEmpty(x1[w])) |
+                                     (unsigned)(h2 && isEmpty(x2[w]));
+                        if (!empty) break;
+                        if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
+                            atomicOr(err, 1);
+                            break;
                         }
                     }
                     const S t0 = rowDot(B0, x0), t1 = rowDot(B1, x1), t2 = rowDot(B2, x2);
