@@ -232,6 +232,7 @@ constexpr int kSweepThreads = 1024;
 constexpr int kStreamCells = 240;                // compute cells per tile: 15 waves of 16 cells
 constexpr int kRecvWave = kStreamCells / 16;     // the receiving wave
 constexpr int kRecvPerLane = 8;                  // ghost values per receiving lane
+constexpr int kRecvDepth = 1;                    // loads in flight per ghost value (two measured slower: the extra polls load the fabric)
 constexpr int kMaxTileLevels = 256;              // cell levels of a tile, BX + BY + BZ - 2 < 256
 // s_waitcnt lgkmcnt(0) with the vector-memory and export counters left at their maxima
 constexpr int kWaitLds = 0xC07F;
@@ -393,27 +394,32 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
                     }
                 }
             }
-            // continuous polling: every slot always has one load in flight; a load is
-            // re-issued as soon as its result has been looked at, so an arriving value
-            // is seen one memory latency after it lands.  All loads are unconditional
-            // (finished or unused slots re-read a harmless address) and the loop has
-            // no stores, so each wait covers exactly the one load it needs.
+            // continuous polling: every slot always has kRecvDepth loads in flight;
+            // a load is re-issued as soon as its result has been looked at, so an
+            // arriving value is seen about 1/kRecvDepth of a memory latency (plus
+            // the return trip) after it lands.  All loads are
+            // unconditional (finished or unused slots re-read a harmless address) and
+            // the loop has no stores, so each wait covers exactly the one load it needs.
             const unsigned all = mask;
-            S val[kRecvPerLane];
+            S val[kRecvDepth][kRecvPerLane];
 #pragma clang loop unroll(full)
-            for (int m = 0; m < kRecvPerLane; ++m) val[m] = ldCoh(xch + src[m]);
+            for (int d = 0; d < kRecvDepth; ++d)
+#pragma clang loop unroll(full)
+                for (int m = 0; m < kRecvPerLane; ++m) val[d][m] = ldCoh(xch + src[m]);
             int failed = loadRelaxed(err);
             long spins = 0;
             while (mask) {
 #pragma clang loop unroll(full)
-                for (int m = 0; m < kRecvPerLane; ++m) {
-                    const S x = val[m];
-                    val[m] = ldCoh(xch + src[m]);
-                    if (((mask >> m) & 1u) && !isEmpty(x)) {
-                        ldsStore(sV + dst[m], x);
-                        mask &= ~(1u << m);
+                for (int d = 0; d < kRecvDepth; ++d)
+#pragma clang loop unroll(full)
+                    for (int m = 0; m < kRecvPerLane; ++m) {
+                        const S x = val[d][m];
+                        val[d][m] = ldCoh(xch + src[m]);
+                        if (((mask >> m) & 1u) && !isEmpty(x)) {
+                            ldsStore(sV + dst[m], x);
+                            mask &= ~(1u << m);
+                        }
                     }
-                }
                 const int f = failed;
                 failed = loadRelaxed(err);
                 if (f || ++spins > kSpinLimit) {
