@@ -47,10 +47,10 @@ CONFIGS = {
 # device copy of 256 MiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the commands and
 # per-kernel figures are in profiles/blackoil_500k_hbm_traffic.md.
 HBM_TRAFFIC_500K = {
-    "ilu_apply": 598595686,    # forward + backward sweep
-    "spmv": 347883724,
-    "assemble": 482900480,
-    "ilu_factor": 662758604,
+    "ilu_apply": 592127569,    # forward + backward sweep
+    "spmv": 347830615,
+    "assemble": 483159271,
+    "ilu_factor": 656911064,
 }
 
 
