@@ -66,8 +66,9 @@ def algorithmic_bytes(N, F, nnzb, n=3, s=8):
     }
 
 
-def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77):
-    """Poisson stream of uneven black-oil realizations through the server."""
+def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=False):
+    """Poisson stream of uneven black-oil realizations through the server
+    (progress: the server prints a line per finished request)."""
     rng = np.random.default_rng(seed)
     cases = []
     for k in range(n_req):
@@ -81,7 +82,7 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77):
     dt = 86400.0
     # the reference's time-step control with steps of at most one day: a step
     # whose Newton solve fails is retried at half the size
-    prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1)
+    prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1, verbose=1 if progress else 0)
     budget = 64 * 2.0 ** 30
     srv = om.Server(prm, budget, 64)
     for c, a in zip(cases, arrivals):
@@ -209,6 +210,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-steps", type=int, default=10)
     ap.add_argument("--serve-plain", type=int, default=4)
+    ap.add_argument("--serve-progress", action="store_true", help="a line per finished request (long streams)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the reservoir would be split "
@@ -313,7 +315,8 @@ def main():
         }
     if not args.no_serve:
         line["ensemble"] = serve_workload(om, np, args.serve_requests, args.serve_rate, args.serve_steps,
-                                          min(args.serve_plain, args.serve_requests))
+                                          min(args.serve_plain, args.serve_requests),
+                                          progress=args.serve_progress)
     print(json.dumps(line))
 
 

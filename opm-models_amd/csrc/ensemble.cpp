@@ -15,6 +15,7 @@
 // members as soon as their last step is done, so requests that arrive while
 // others run join the next step (continuous batching).
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -605,6 +606,7 @@ void Server::run()
     size_t next = 0;
     double used = 0.0;
     std::vector<std::pair<int, int>> running;   // (request, member id)
+    int retired = 0;
     while (next < order.size() || !running.empty()) {
         // admission: arrived requests in FIFO order while memory and slots allow
         const double t = now();
@@ -644,6 +646,12 @@ void Server::run()
                 used -= r.bytes;
                 ens_.remove(m->id);
                 running.erase(running.begin() + k);
+                ++retired;
+                if (prm_.verbose) {
+                    printf("served request %d (%d of %zu) at %.1f s, latency %.2f s, status %d\n", r.id, retired,
+                           reqs_.size(), r.finish, r.finish - r.arrival, r.status);
+                    fflush(stdout);
+                }
             } else ++k;
         }
     }
