@@ -39,6 +39,14 @@ CONFIGS = {
     "500k": dict(nx=100, ny=100, nz=50, label="blackoil 100x100x50 (500k cells), 3x3 BCRS"),
     "c2": dict(nx=40, ny=40, nz=10, label="blackoil 40x40x10 (16k cells), 3x3 BCRS"),
     "c3": dict(nx=120, ny=120, nz=30, label="blackoil 120x120x30 (432k cells), 3x3 BCRS"),
+    # 2M cells with the headline's log-normal field and corner wells; the
+    # injection rate grows with the pore volume, so the reference's adaptive
+    # step control (at most one day, halved on failure) sets the steps
+    "c4": dict(nx=200, ny=200, nz=50, adaptive=True, label="blackoil 200x200x50 (2M cells), 3x3 BCRS"),
+    # channelised field (1000 mD channels in a 1 mD background, sigma 2 within
+    # each facies), five-spot, the reference's adaptive step control
+    "c4ch": dict(nx=200, ny=200, nz=50, spec=dict(field=1, perm_sigma=2.0, wells=1), adaptive=True,
+                 label="blackoil 200x200x50 (2M cells) channelised five-spot, 3x3 BCRS"),
 }
 
 
@@ -177,7 +185,7 @@ def beside_runs(om, config, dt, steps=2):
     runs = [("c3_fp64", "c3", 0), ("%s_fp32_inner" % config, config, 1)]
     for key, cfg_name, precision in runs:
         cfg = CONFIGS[cfg_name]
-        case = om.BlackOilCase(om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024))
+        case = om.BlackOilCase(om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024, **cfg.get("spec", {})))
         prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, autotune=1)
         model = om.Model(case)
         model.set_solution(case.pv0, case.mean0)
@@ -224,10 +232,11 @@ def main():
         copy_calibration(args.copy_calibration)
 
     cfg = CONFIGS[args.config]
-    spec = om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024)
+    spec = om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024, **cfg.get("spec", {}))
     case = om.BlackOilCase(spec)
     dt = 86400.0
-    prm = om.default_params(max_dt=dt, min_dt=dt, precision=args.precision, autotune=1)
+    prm = om.default_params(max_dt=dt, min_dt=0.0 if cfg.get("adaptive") else dt, precision=args.precision,
+                            autotune=1)
 
     t_setup = time.time()
     model = om.Model(case)
@@ -279,7 +288,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f64/f32",
         "data": "synthetic",
-        "config": {"workload": cfg["label"], "cells": N, "nnzb": nnzb, "dt_days": 1.0,
+        "config": {"workload": cfg["label"], "cells": N, "nnzb": nnzb,
+                   "dt_days": "adaptive, at most 1.0" if cfg.get("adaptive") else 1.0,
                    "precond": "block-ILU0", "precision": "fp64" if args.precision == 0 else "fp32-inner/fp64-outer",
                    "parallelism": "single GPU"},
         "assembly_mcells_per_s": round(N * asm["launches"] / (asm["total_ms"] * 1e-3) / 1e6, 1) if asm else None,
