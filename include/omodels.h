@@ -235,8 +235,11 @@ int omodels_server_kernel_times(omodels_server* s, double* ms, int64_t* counts, 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing with HIP events recorded on the model's stream around each
  * kernel class: 0 assembly, 1 SpMV, 2 preconditioner apply (ILU0 sweep pair),
- * 3 ILU0 factorization, 4 other.  The reference's counterpart is its phase
- * timers (newtonmethod.hh:240-262, timer.hh:48-215). */
+ * 3 ILU0 factorization, 4 other.  set_profiling(enable): 0 off, k >= 1 times
+ * every k-th launch of each class (each event record idles the stream for a
+ * few microseconds); kernel_times reports per class the launch count and the
+ * timed launches' average duration times that count.  The reference's
+ * counterpart is its phase timers (newtonmethod.hh:240-262, timer.hh:48-215). */
 int omodels_model_set_profiling(omodels_model* m, int enable);
 int omodels_model_kernel_times(omodels_model* m, double* ms, int64_t* counts, int n);
 

@@ -2,6 +2,7 @@
 // include/omodels.h).  No C++ exception crosses this boundary: every entry
 // point maps NumericalProblem to 2, other errors to a negative code, and keeps
 // the message in a thread-local string.
+#include <algorithm>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -412,7 +413,8 @@ int omodels_server_set_profiling(omodels_server* s, int enable)
 {
     return guarded([&] {
         s->s->ensemble().timer.reset();
-        s->s->ensemble().timer.enabled = enable != 0;
+        s->s->ensemble().timer.enabled = enable > 0;
+        s->s->ensemble().timer.every = std::max(1, enable);
         return 0;
     });
 }
@@ -422,7 +424,7 @@ int omodels_server_kernel_times(omodels_server* s, double* ms, int64_t* counts, 
     return guarded([&] {
         KernelTimer& t = s->s->ensemble().timer;
         t.collect();
-        for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.ms[i]; counts[i] = t.count[i]; }
+        for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.estimatedMs(i); counts[i] = t.launched[i]; }
         return 0;
     });
 }
@@ -431,7 +433,8 @@ int omodels_model_set_profiling(omodels_model* h, int enable)
 {
     return guarded([&] {
         h->m->timer.reset();
-        h->m->timer.enabled = enable != 0;
+        h->m->timer.enabled = enable > 0;
+        h->m->timer.every = std::max(1, enable);
         return 0;
     });
 }
@@ -441,7 +444,7 @@ int omodels_model_kernel_times(omodels_model* h, double* ms, int64_t* counts, in
     return guarded([&] {
         KernelTimer& t = h->m->timer;
         t.collect();
-        for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.ms[i]; counts[i] = t.count[i]; }
+        for (int i = 0; i < n && i < kNumClasses; ++i) { ms[i] = t.estimatedMs(i); counts[i] = t.launched[i]; }
         return 0;
     });
 }

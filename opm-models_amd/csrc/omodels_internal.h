@@ -63,15 +63,19 @@ struct DevBuf {
     }
 };
 
-// Optional HIP-event timing of kernel classes on the stream they run on.
+// Optional HIP-event timing of kernel classes on the stream they run on.  With
+// every = k only each k-th launch of a class is bracketed by events (an event
+// record costs the stream a few microseconds); launched counts all of them.
 enum KernelClass : int { kClsAssemble = 0, kClsSpmv, kClsIluApply, kClsIluFactor, kClsOther, kNumClasses };
 struct KernelTimer {
     bool enabled = false;
+    int every = 1;
     struct Pair { hipEvent_t a, b; int cls; };
     std::vector<Pair> pending;
     std::vector<hipEvent_t> pool;
-    double ms[kNumClasses] = {0, 0, 0, 0, 0};
-    long count[kNumClasses] = {0, 0, 0, 0, 0};
+    double ms[kNumClasses] = {0, 0, 0, 0, 0};        // summed over the timed launches
+    long count[kNumClasses] = {0, 0, 0, 0, 0};       // timed launches
+    long launched[kNumClasses] = {0, 0, 0, 0, 0};    // all launches
     hipEvent_t cur = nullptr;
     int curCls = 0;
     hipEvent_t get()
@@ -82,6 +86,7 @@ struct KernelTimer {
     void begin(int cls, hipStream_t s)
     {
         if (!enabled) return;
+        if (launched[cls]++ % every != 0) { cur = nullptr; return; }
         cur = get(); curCls = cls;
         OM_HIP(hipEventRecord(cur, s));
     }
@@ -105,7 +110,9 @@ struct KernelTimer {
         }
         pending.clear();
     }
-    void reset() { collect(); for (int i = 0; i < kNumClasses; ++i) { ms[i] = 0; count[i] = 0; } }
+    void reset() { collect(); for (int i = 0; i < kNumClasses; ++i) { ms[i] = 0; count[i] = 0; launched[i] = 0; } }
+    // total time of a class over all its launches (the timed ones' average times the launch count)
+    double estimatedMs(int cls) const { return count[cls] ? ms[cls] * (double)launched[cls] / count[cls] : 0.0; }
     ~KernelTimer()
     {
         for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }

@@ -274,3 +274,23 @@ def test_graph_newton_run_is_identical_to_eager():
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))
+
+
+def test_sampled_kernel_timing_counts_every_launch():
+    """set_profiling(k) times every k-th launch of a class and still reports
+    every launch; the averages are of the same size as with every launch timed."""
+    c = case(24, 20, 6, seed=3)
+    out = []
+    for every in (1, 3):
+        m = om.Model(c)
+        m.set_solution(c.pv0, c.mean0)
+        m.set_profiling(every)
+        rc, _, _, _, st = m.simulate(om.default_params(max_dt=86400.0, min_dt=86400.0), 2, 86400.0,
+                                     keep_history=False)
+        assert rc == 0
+        out.append(m.kernel_times())
+    for k in ("assemble", "spmv", "ilu_apply", "ilu_factor"):
+        (ms1, n1), (ms3, n3) = out[0][k], out[1][k]
+        assert n1 == n3 > 0, k
+        assert ms1 > 0 and ms3 > 0
+        assert 0.2 < (ms3 / n3) / (ms1 / n1) < 5.0, k
