@@ -877,17 +877,18 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
         }
         // the tile's outputs leave the workgroup once, after the level loop (a
         // global store inside the loop would make every level barrier wait for
-        // its acknowledgement)
+        // its acknowledgement).  Dependent tiles read only the inverted pivots:
+        // those are stored and acknowledged before the flag, the factor blocks
+        // (read by later kernels) after it, off the dependency chain.
+        if (c.valid)
+            for (int q = 0; q < 9; ++q) stCoh(dinv + 9 * c.I + q, D[q]);
+        publish(I.done, task.tile, epoch);
         if (c.valid) {
             if (hz) { for (int q = 0; q < 9; ++q) lu[9 * oz + q] = Az[q]; }
             if (hy) { for (int q = 0; q < 9; ++q) lu[9 * oy + q] = Ay[q]; }
             if (hx) { for (int q = 0; q < 9; ++q) lu[9 * ox + q] = Ax[q]; }
-            for (int q = 0; q < 9; ++q) {
-                lu[9 * od + q] = D[q];
-                stCoh(dinv + 9 * c.I + q, D[q]);
-            }
+            for (int q = 0; q < 9; ++q) lu[9 * od + q] = D[q];
         }
-        publish(I.done, task.tile, epoch);
         t = shareTicket(next, &sTicket);
     }
     leaveLaunch(counter, exits);
