@@ -43,7 +43,6 @@ namespace omodels {
 
 namespace {
 
-constexpr int kTileThreads = 256;
 constexpr int kPadMax = 704;            // bound of (BX+1)(BY+1)(BZ+1) over all tile shapes
 constexpr long kSpinLimit = 1L << 22;   // a few seconds of polling
 
@@ -119,33 +118,11 @@ __device__ __forceinline__ S receiveSlot(S* slot, int* err)
 // (s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maxima)
 __device__ __forceinline__ void waitStores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-struct CellCtx {
-    int li, lj, lk;        // local coordinates in the tile
-    int i, j, k;           // global coordinates
-    long I;
-    bool valid;
-};
-
 __device__ __forceinline__ void tileOf(const TileGeom& g, int tile, int& TI, int& TJ, int& TK)
 {
     TI = tile % g.TX;
     TJ = (tile / g.TX) % g.TY;
     TK = tile / (g.TX * g.TY);
-}
-
-__device__ __forceinline__ CellCtx cellOf(const TileGeom& g, int TI, int TJ, int TK)
-{
-    CellCtx c;
-    const int t = threadIdx.x;
-    c.li = t % g.BX;
-    c.lj = (t / g.BX) % g.BY;
-    c.lk = t / (g.BX * g.BY);
-    c.i = TI * g.BX + c.li;
-    c.j = TJ * g.BY + c.lj;
-    c.k = TK * g.BZ + c.lk;
-    c.valid = c.i < g.nx && c.j < g.ny && c.k < g.nz && c.lk < g.BZ;
-    c.I = (long)c.i + (long)g.nx * ((long)c.j + (long)g.ny * c.k);
-    return c;
 }
 
 // wait for the dependency tiles of (TI,TJ,TK): dir = -1 forward, +1 backward
@@ -782,12 +759,26 @@ __device__ __forceinline__ bool inv3(double* a)
     return true;
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* __restrict__ insts,
+// One workgroup of kSweepThreads per tile with the sweeps' lane layout: four
+// lanes per cell, cells in level order, lane e (the fourth repeats row 2)
+// forming row e of every block product of its cell.  The inverted pivots of
+// the tile's cells and of the neighbour tiles' face cells sit in LDS (padded
+// layout), as do the cells' U_JI blocks; the cell's pivot row and lower-block
+// rows stay in registers.  After the pivot update each lane gathers the whole
+// pivot block from its quad (DPP) and inverts it with inv3, so every entry
+// is computed with the operations, and in the order, of the one-thread-per-
+// cell form.  Levels are separated by workgroup barriers; a tile waits for
+// the done flags of its -x, -y, -z neighbour tiles, publishes its own once
+// its inverted pivots are acknowledged, and stores its factor blocks after.
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_factor_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, long long base, int epoch, int* err,
     int* fail, unsigned int* exits)
 {
-    __shared__ double ld[kTileThreads * 9];
+    __shared__ double sP[kPadMax * 9];            // inverted pivots (padded tile + ghost layer)
+    __shared__ double sU[kStreamCells * 27];      // per cell (level-order rank): U_JI of its -z, -y, -x neighbours
+    __shared__ int sOrder[kStreamCells];
     __shared__ int sTicket;
+    const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
     int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, base) : 0, &sTicket);
     for (;;) {
         if (t < 0 || t >= T) break;
@@ -801,93 +792,115 @@ __global__ __launch_bounds__(kTileThreads) void k_ilu_factor_set(const IluInst* 
             continue;
         }
         const TileGeom& g = I.g;
-        double* __restrict__ lu = (double*)I.luw;
-        double* __restrict__ dinv = I.dinvw;
+        const int BX = g.BX, BY = g.BY, BZ = g.BZ, PX = BX + 1, PXY = (BX + 1) * (BY + 1);
+        const int ncell = BX * BY * BZ;
         const long nxy = (long)g.nx * g.ny;
         int TI, TJ, TK;
         tileOf(g, task.tile, TI, TJ, TK);
-        const CellCtx c = cellOf(g, TI, TJ, TK);
-        const bool hz = c.k > 0, hy = c.j > 0, hx = c.i > 0;
-        double D[9], Az[9], Ay[9], Ax[9], Uz[9], Uy[9], Ux[9];
+        const int i0 = TI * BX, j0 = TJ * BY, k0 = TK * BZ;
+        if (threadIdx.x < ncell) {
+            const int li = threadIdx.x % BX, lj = (threadIdx.x / BX) % BY, lk = threadIdx.x / (BX * BY);
+            const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+            sOrder[cellsBelow(li + lj + lk, BX, BY, BZ) + rankInLevel(li, lj, lk, BX, BY)] =
+                packCell(li, lj, lk, k > 0, j > 0, i > 0, i < g.nx && j < g.ny && k < g.nz);
+        }
+        __syncthreads();
+        const int word = pos < ncell ? sOrder[pos] : 0;
+        const bool valid = pos < ncell && (word & (1 << 30));
+        const int li = word & 0xFF, lj = (word >> 8) & 0xFF, lk = (word >> 16) & 0xFF;
+        const bool hz = valid && (word & (1 << 27)), hy = valid && (word & (1 << 28));
+        const bool hx = valid && (word & (1 << 29));
+        const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+        const long Ic = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+        const int p = (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
+        const GS<int>* __restrict__ rp = toGlobal(I.rp);
+        const GS<int>* __restrict__ dg = toGlobal(I.diag);
+        double* __restrict__ lu = (double*)I.luw;
+        double* __restrict__ dinv = I.dinvw;
+        // this lane's row of the cell's pivot and lower blocks; its row of the
+        // U_JI blocks goes to LDS (every lane of the cell reads the whole block)
+        double D[3] = {}, Az[3] = {}, Ay[3] = {}, Ax[3] = {};
         long oz = 0, oy = 0, ox = 0, od = 0;
-        if (c.valid) {
-            long o = I.rp[c.I];
-            if (hz) { oz = o++; }
-            if (hy) { oy = o++; }
-            if (hx) { ox = o++; }
-            od = I.diag[c.I];
-            for (int q = 0; q < 9; ++q) D[q] = lu[9 * od + q];
-            // A_JI: the block of neighbour row J that points back to I
+        if (valid) {
+            long o = rp[Ic];
+            if (hz) oz = o++;
+            if (hy) oy = o++;
+            if (hx) ox = o++;
+            od = dg[Ic];
+            const GS<double>* __restrict__ a = toGlobal((const double*)lu);
+            for (int w = 0; w < 3; ++w) D[w] = a[9 * od + 3 * e + w];
             if (hz) {
-                const long J = c.I - nxy;
-                const long oJ = I.diag[J] + 1 + (c.i + 1 < g.nx) + (c.j + 1 < g.ny);
-                for (int q = 0; q < 9; ++q) { Az[q] = lu[9 * oz + q]; Uz[q] = lu[9 * oJ + q]; }
+                const long oJ = dg[Ic - nxy] + 1 + (i + 1 < g.nx) + (j + 1 < g.ny);
+                for (int w = 0; w < 3; ++w) { Az[w] = a[9 * oz + 3 * e + w]; if (q < 3) sU[27 * pos + 3 * e + w] = a[9 * oJ + 3 * e + w]; }
             }
             if (hy) {
-                const long J = c.I - g.nx;
-                const long oJ = I.diag[J] + 1 + (c.i + 1 < g.nx);
-                for (int q = 0; q < 9; ++q) { Ay[q] = lu[9 * oy + q]; Uy[q] = lu[9 * oJ + q]; }
+                const long oJ = dg[Ic - g.nx] + 1 + (i + 1 < g.nx);
+                for (int w = 0; w < 3; ++w) { Ay[w] = a[9 * oy + 3 * e + w]; if (q < 3) sU[27 * pos + 9 + 3 * e + w] = a[9 * oJ + 3 * e + w]; }
             }
             if (hx) {
-                const long J = c.I - 1;
-                const long oJ = I.diag[J] + 1;
-                for (int q = 0; q < 9; ++q) { Ax[q] = lu[9 * ox + q]; Ux[q] = lu[9 * oJ + q]; }
+                const long oJ = dg[Ic - 1] + 1;
+                for (int w = 0; w < 3; ++w) { Ax[w] = a[9 * ox + 3 * e + w]; if (q < 3) sU[27 * pos + 18 + 3 * e + w] = a[9 * oJ + 3 * e + w]; }
             }
         }
         waitDeps(g, TI, TJ, TK, -1, I.done, epoch, err);
-        // pivots of the neighbours across the tile faces, one round trip
-        const bool fz = c.valid && hz && c.lk == 0, fy = c.valid && hy && c.lj == 0, fx = c.valid && hx && c.li == 0;
-        double Pz[9], Py[9], Px[9];
-        if (fz) { for (int q = 0; q < 9; ++q) Pz[q] = ldCoh(dinv + 9 * (c.I - nxy) + q); }
-        if (fy) { for (int q = 0; q < 9; ++q) Py[q] = ldCoh(dinv + 9 * (c.I - g.nx) + q); }
-        if (fx) { for (int q = 0; q < 9; ++q) Px[q] = ldCoh(dinv + 9 * (c.I - 1) + q); }
-        const int myLev = c.li + c.lj + c.lk;
-        const int nLev = g.BX + g.BY + g.BZ - 2;
+        // inverted pivots of the neighbours across the tile faces, one round trip
+        if (q < 3) {
+            if (hz && lk == 0) for (int w = 0; w < 3; ++w) sP[9 * (p - PXY) + 3 * e + w] = ldCoh(dinv + 9 * (Ic - nxy) + 3 * e + w);
+            if (hy && lj == 0) for (int w = 0; w < 3; ++w) sP[9 * (p - PX) + 3 * e + w] = ldCoh(dinv + 9 * (Ic - g.nx) + 3 * e + w);
+            if (hx && li == 0) for (int w = 0; w < 3; ++w) sP[9 * (p - 1) + 3 * e + w] = ldCoh(dinv + 9 * (Ic - 1) + 3 * e + w);
+        }
+        __syncthreads();
+        const int myLev = valid ? li + lj + lk : -1;
+        const int nLev = BX + BY + BZ - 2;
         for (int lev = 0; lev <= nLev; ++lev) {
-            if (c.valid && myLev == lev) {
-                double Dj[9], L[9], Tm[9];
-                if (hz) {
-                    if (!fz) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - g.BX * g.BY) + q]; }
-                    else { for (int q = 0; q < 9; ++q) Dj[q] = Pz[q]; }
-                    mat3mul(Az, Dj, L);
-                    for (int q = 0; q < 9; ++q) Az[q] = L[q];
-                    mat3mul(L, Uz, Tm);
-                    for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
+            if (myLev == lev) {
+                // L_IJ = A_IJ Dj^-1 (row e), then A_II -= L_IJ A_JI, for J = -z, -y, -x
+                auto update = [&](double* Arow, int pn, int ub) {
+                    double L[3], Tm[3];
+                    const double* Dj = sP + 9 * pn;
+                    const double* U = sU + 27 * pos + ub;
+                    for (int c2 = 0; c2 < 3; ++c2) {
+                        double s2 = Arow[0] * Dj[c2];
+                        s2 += Arow[1] * Dj[3 + c2];
+                        s2 += Arow[2] * Dj[6 + c2];
+                        L[c2] = s2;
+                    }
+                    for (int c2 = 0; c2 < 3; ++c2) Arow[c2] = L[c2];
+                    for (int c2 = 0; c2 < 3; ++c2) {
+                        double s2 = L[0] * U[c2];
+                        s2 += L[1] * U[3 + c2];
+                        s2 += L[2] * U[6 + c2];
+                        Tm[c2] = s2;
+                    }
+                    for (int c2 = 0; c2 < 3; ++c2) D[c2] -= Tm[c2];
+                };
+                if (hz) update(Az, p - PXY, 0);
+                if (hy) update(Ay, p - PX, 9);
+                if (hx) update(Ax, p - 1, 18);
+                // the whole pivot block from the quad's three rows, inverted as one
+                double M[9];
+                for (int c2 = 0; c2 < 3; ++c2) {
+                    M[c2] = quadLane<0>(D[c2]);
+                    M[3 + c2] = quadLane<1>(D[c2]);
+                    M[6 + c2] = quadLane<2>(D[c2]);
                 }
-                if (hy) {
-                    if (!fy) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - g.BX) + q]; }
-                    else { for (int q = 0; q < 9; ++q) Dj[q] = Py[q]; }
-                    mat3mul(Ay, Dj, L);
-                    for (int q = 0; q < 9; ++q) Ay[q] = L[q];
-                    mat3mul(L, Uy, Tm);
-                    for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
-                }
-                if (hx) {
-                    if (!fx) { for (int q = 0; q < 9; ++q) Dj[q] = ld[9 * (threadIdx.x - 1) + q]; }
-                    else { for (int q = 0; q < 9; ++q) Dj[q] = Px[q]; }
-                    mat3mul(Ax, Dj, L);
-                    for (int q = 0; q < 9; ++q) Ax[q] = L[q];
-                    mat3mul(L, Ux, Tm);
-                    for (int q = 0; q < 9; ++q) D[q] -= Tm[q];
-                }
-                if (!inv3(D)) atomicOr(fail, 2);
-                for (int q = 0; q < 9; ++q) ld[9 * threadIdx.x + q] = D[q];
+                if (!inv3(M) && q == 0) atomicOr(fail, 2);
+                for (int c2 = 0; c2 < 3; ++c2) D[c2] = M[3 * e + c2];
+                if (q < 3) for (int c2 = 0; c2 < 3; ++c2) sP[9 * p + 3 * e + c2] = D[c2];
             }
             __syncthreads();
         }
-        // the tile's outputs leave the workgroup once, after the level loop (a
-        // global store inside the loop would make every level barrier wait for
-        // its acknowledgement).  Dependent tiles read only the inverted pivots:
-        // those are stored and acknowledged before the flag, the factor blocks
-        // (read by later kernels) after it, off the dependency chain.
-        if (c.valid)
-            for (int q = 0; q < 9; ++q) stCoh(dinv + 9 * c.I + q, D[q]);
+        // inverted pivots first (dependent tiles read them), then the flag, then
+        // the factor blocks, read only by later kernels
+        const bool row = valid && q < 3;
+        if (row) for (int c2 = 0; c2 < 3; ++c2) stCoh(dinv + 9 * Ic + 3 * e + c2, D[c2]);
         publish(I.done, task.tile, epoch);
-        if (c.valid) {
-            if (hz) { for (int q = 0; q < 9; ++q) lu[9 * oz + q] = Az[q]; }
-            if (hy) { for (int q = 0; q < 9; ++q) lu[9 * oy + q] = Ay[q]; }
-            if (hx) { for (int q = 0; q < 9; ++q) lu[9 * ox + q] = Ax[q]; }
-            for (int q = 0; q < 9; ++q) lu[9 * od + q] = D[q];
+        if (row) {
+            GS<double>* __restrict__ w = toGlobal(lu);
+            if (hz) for (int c2 = 0; c2 < 3; ++c2) w[9 * oz + 3 * e + c2] = Az[c2];
+            if (hy) for (int c2 = 0; c2 < 3; ++c2) w[9 * oy + 3 * e + c2] = Ay[c2];
+            if (hx) for (int c2 = 0; c2 < 3; ++c2) w[9 * ox + 3 * e + c2] = Ax[c2];
+            for (int c2 = 0; c2 < 3; ++c2) w[9 * od + 3 * e + c2] = D[c2];
         }
         t = shareTicket(next, &sTicket);
     }
@@ -990,7 +1003,7 @@ TiledIluSet::TiledIluSet(hipStream_t s)
         if (perCU < 1) throw std::runtime_error("tiled ILU kernels cannot be resident");
         return perCU * prop.multiProcessorCount;
     };
-    residentFactor_ = resident({(const void*)k_ilu_factor_set}, kTileThreads);
+    residentFactor_ = resident({(const void*)k_ilu_factor_set}, kSweepThreads);
     residentSweep_[0][0] = resident({(const void*)k_ilu_sweep_set<double, false>}, kSweepThreads);
     residentSweep_[0][1] = resident({(const void*)k_ilu_sweep_set<double, true>}, kSweepThreads);
     residentSweep_[1][0] = resident({(const void*)k_ilu_sweep_set<float, false>}, kSweepThreads);
@@ -1120,7 +1133,7 @@ void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream
     int* er = err_.p;
     unsigned int* ex = exits_.p;
     void* args[] = {(void*)&di, (void*)&od, &T, &ct, &b, &ep, &er, (void*)&fail, (void*)&ex};
-    OM_HIP(hipLaunchKernel((const void*)k_ilu_factor_set, dim3(gridFactor_), dim3(kTileThreads), args, 0, s));
+    OM_HIP(hipLaunchKernel((const void*)k_ilu_factor_set, dim3(gridFactor_), dim3(kSweepThreads), args, 0, s));
 }
 
 template <class S>
