@@ -20,10 +20,11 @@
 // idle gap of about 20 us around every launch).  Waits are also bounded and
 // report an error instead of spinning forever.
 //
-// The factorization works one thread per cell: the tile's BX+BY+BZ-2 local
-// levels are separated by workgroup barriers, and a tile publishes a done flag
-// after its values are written through to the device-coherent level.  The
-// sweeps, applied twice per solver iteration, stream instead: each face value
+// The factorization (once per Newton iteration) uses four lanes per cell with
+// the tile's BX+BY+BZ-2 local levels separated by workgroup barriers, and a
+// tile publishes a done flag after its inverted pivots are written through to
+// the device-coherent level.  The sweeps, applied twice per solver iteration,
+// come in two forms: batched like the factorization, or streaming: each face value
 // goes to its neighbour tile as soon as it is computed, and inside a tile the
 // levels follow each other through per-level completion counts in LDS rather
 // than workgroup barriers, so a tile runs a few cell levels behind its
