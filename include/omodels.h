@@ -161,6 +161,25 @@ int omodels_linearizer_get_jacobian(omodels_model* m, double* values);
 int omodels_linearizer_get_residual(omodels_model* m, double* residual);
 /* eraseMatrix(): release the device Jacobian (re-created on next linearize) */
 int omodels_linearizer_erase_matrix(omodels_model* m);
+/* SeparateSparseSourceTerms (tpfalinearizer.hh:56, default off): when on,
+ * linearize_domain leaves the problem's source terms out of the cell loop
+ * (computeSourceDense instead of computeSource, :829-833) and adds them after
+ * it as sparse blocks, as wellModel().addReservoirSourceTerms does (:842-846);
+ * the problem's source list plays the well model.  Either way each cell's
+ * source term is the last addition to its row, so both settings give the same
+ * bits.  Further terms (an external well model's) go in through
+ * add_sparse_sources after linearize_domain. */
+int omodels_linearizer_set_separate_sparse_sources(omodels_model* m, int enable);
+/* The problem's source terms at the current solution(0) as sparse blocks:
+ * cells [n_src], residual contributions [n_src][3] (already times -V, the
+ * linearizer's units) and their derivatives w.r.t. the cell's primaries
+ * [n_src][3][3] row-major; returns n_src.  NULL arrays are not written. */
+int omodels_linearizer_source_blocks(omodels_model* m, int32_t* cells, double* residual, double* diag);
+/* addReservoirSourceTerms(residual, diagMatAddress) (tpfalinearizer.hh:843-846):
+ * residual[cell] += r and A[cell][cell] += d for every entry, in the given
+ * order (a cell may appear more than once). */
+int omodels_linearizer_add_sparse_sources(omodels_model* m, int32_t n, const int32_t* cells,
+                                          const double* residual, const double* diag);
 
 /* ------------------------------------------------------------------------ */
 /* Linear solver backend -- replaces ParallelBiCGStabSolverBackend

@@ -227,6 +227,51 @@ __global__ void k_bo_sources(BoFluid f, BoGrid G, const double* __restrict__ pv,
     }
 }
 
+// The same source terms as blocks, not added: block s holds the residual
+// contribution (3 values, already times -V) and its derivatives w.r.t. the
+// cell's primaries (3x3, row-major) of source s.  With separate sparse source
+// terms these are what the built-in well model adds after the cell loop
+// (tpfalinearizer.hh:842-846) in place of the cell-wise pass (:827-841).
+__global__ void k_bo_source_blocks(BoFluid f, BoGrid G, const double* __restrict__ pv,
+                                   const uint8_t* __restrict__ mean, int nsrc, const int* __restrict__ cell,
+                                   const int* __restrict__ type, const double* __restrict__ a,
+                                   const double* __restrict__ b, double* __restrict__ outRes,
+                                   double* __restrict__ outDiag)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsrc) return;
+    const long I = cell[s];
+    IqE qi;
+    iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
+    E3 q[3];
+    sourceEval(f, qi, type[s], a[s], b[s], G.V, q);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const E3 t = q[e] * (-G.V);
+        outRes[3 * s + e] = t.v;
+        outDiag[9 * s + 3 * e + 0] = t.d0;
+        outDiag[9 * s + 3 * e + 1] = t.d1;
+        outDiag[9 * s + 3 * e + 2] = t.d2;
+    }
+}
+
+// Sparse source terms (wellModel().addReservoirSourceTerms, tpfalinearizer.hh:
+// 843-846): residual[cell] += r and A[cell][cell] += D for every entry, in the
+// given order.  One wave: lane c < 12 owns component c of every entry, so
+// entries for the same cell are added one after another.
+__global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const double* __restrict__ r,
+                                     const double* __restrict__ d, double* __restrict__ res,
+                                     double* __restrict__ jac, const int* __restrict__ diagOff)
+{
+    const int c = threadIdx.x;
+    if (c >= 12) return;
+    for (int k = 0; k < n; ++k) {
+        const long I = cell[k];
+        if (c < 3) res[3 * I + c] += r[3L * k + c];
+        else jac[9L * diagOff[I] + (c - 3)] += d[9L * k + (c - 3)];
+    }
+}
+
 // Storage of the current solution: the cache of time level 1 at the first
 // Newton iteration (recycleFirstIterationStorage, tpfalinearizer.hh:786-812).
 __global__ void k_bo_storage(BoFluid f, long N, const double* __restrict__ pv,
@@ -341,6 +386,22 @@ void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const 
     if (nsrc <= 0) return;
     hipLaunchKernelGGL(k_bo_sources, dim3((nsrc + 63) / 64), dim3(64), 0, s, f, G, pv, mean, nsrc,
                        cell, type, a, b, res, jac, diagOff);
+}
+
+void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+                          int nsrc, const int* cell, const int* type, const double* a, const double* b,
+                          double* outRes, double* outDiag, hipStream_t s)
+{
+    if (nsrc <= 0) return;
+    hipLaunchKernelGGL(k_bo_source_blocks, dim3((nsrc + 63) / 64), dim3(64), 0, s, f, G, pv, mean, nsrc,
+                       cell, type, a, b, outRes, outDiag);
+}
+
+void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
+                            double* jac, const int* diagOff, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add_sparse_sources, dim3(1), dim3(64), 0, s, n, cell, r, d, res, jac, diagOff);
 }
 
 void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,

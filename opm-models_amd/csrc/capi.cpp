@@ -256,6 +256,46 @@ int omodels_linearizer_erase_matrix(omodels_model* h)
     return guarded([&] { h->m->eraseMatrix(); return 0; });
 }
 
+int omodels_linearizer_set_separate_sparse_sources(omodels_model* h, int enable)
+{
+    return guarded([&] { h->m->separateSparseSources = enable != 0; return 0; });
+}
+
+int omodels_linearizer_source_blocks(omodels_model* h, int32_t* cells, double* residual, double* diag)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (m.nsrc <= 0) return 0;
+        m.computeSourceBlocks();
+        if (cells) m.srcCell.download(cells, m.nsrc, m.stream);
+        if (residual) m.srcRes.download(residual, 3 * (size_t)m.nsrc, m.stream);
+        if (diag) m.srcDiag.download(diag, 9 * (size_t)m.nsrc, m.stream);
+        m.sync();
+        return m.nsrc;
+    });
+}
+
+int omodels_linearizer_add_sparse_sources(omodels_model* h, int32_t n, const int32_t* cells, const double* residual,
+                                          const double* diag)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (n < 0) throw std::invalid_argument("negative number of sparse source entries");
+        if (n == 0) return 0;
+        if (!cells || !residual || !diag) throw std::invalid_argument("sparse source arrays must not be NULL");
+        for (int k = 0; k < n; ++k)
+            if (cells[k] < 0 || cells[k] >= m.N) throw std::invalid_argument("sparse source cell out of range");
+        DevBuf<int> dc;
+        DevBuf<double> dr, dd;
+        dc.upload(cells, n, m.stream);
+        dr.upload(residual, 3 * (size_t)n, m.stream);
+        dd.upload(diag, 9 * (size_t)n, m.stream);
+        m.addSparseSources(n, dc.p, dr.p, dd.p);
+        m.sync();
+        return 0;
+    });
+}
+
 omodels_solver* omodels_solver_create(omodels_model* h, const omodels_params* prm)
 {
     omodels_solver* s = nullptr;

@@ -98,9 +98,37 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     launchBoAssemble(fluid, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
                      nbOff.p, stream);
     timer.end(stream);
-    launchBoSources(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, res.p, jac.p,
-                    diagOff.p, stream);
+    // cell-wise sources (:827-841), or the same terms as sparse blocks added
+    // after the cell loop (:842-846); both add each cell's term last
+    if (!separateSparseSources)
+        launchBoSources(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, res.p, jac.p,
+                        diagOff.p, stream);
+    else
+        addBuiltinSparseSources();
     OM_HIP(hipGetLastError());
+}
+
+void Model::computeSourceBlocks()
+{
+    srcRes.alloc(3 * (size_t)std::max(nsrc, 1));
+    srcDiag.alloc(9 * (size_t)std::max(nsrc, 1));
+    launchBoSourceBlocks(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, srcRes.p,
+                         srcDiag.p, stream);
+    OM_HIP(hipGetLastError());
+}
+
+void Model::addSparseSources(int n, const int* cells, const double* r, const double* d)
+{
+    ensureJacobian();
+    launchAddSparseSources(n, cells, r, d, res.p, jac.p, diagOff.p, stream);
+    OM_HIP(hipGetLastError());
+}
+
+void Model::addBuiltinSparseSources()
+{
+    if (nsrc <= 0) return;
+    computeSourceBlocks();
+    addSparseSources(nsrc, srcCell.p, srcRes.p, srcDiag.p);
 }
 
 double LinearSystem::maxAbsResidual()

@@ -80,6 +80,18 @@ public:
 
     // TpfaLinearizer
     void linearizeDomain(double dt, int iterationIndex);
+    // SeparateSparseSourceTerms (tpfalinearizer.hh:56): when set, linearizeDomain
+    // leaves the source terms out of the cell loop and adds them after it as
+    // sparse blocks
+    bool separateSparseSources = false;
+    DevBuf<double> srcRes, srcDiag;     // built-in source terms as blocks, [nsrc][3] and [nsrc][9]
+    // the built-in source terms at solution(0) as blocks (in srcRes / srcDiag)
+    void computeSourceBlocks();
+    // wellModel().addReservoirSourceTerms (tpfalinearizer.hh:843-846): residual and
+    // diagonal blocks of n cells (device arrays) added in the given order
+    void addSparseSources(int n, const int* cells, const double* r, const double* d);
+    // the built-in source terms through the sparse path (the built-in well model)
+    void addBuiltinSparseSources();
     void ensureJacobian();
     void eraseMatrix() { jac.release(); }
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)

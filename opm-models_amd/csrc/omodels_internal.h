@@ -129,6 +129,11 @@ void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const 
                      double* res, double* jac, const int* diagOff, hipStream_t s);
 void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
                      const double* poro, double* st, hipStream_t s);
+void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+                          int nsrc, const int* cell, const int* type, const double* a, const double* b,
+                          double* outRes, double* outDiag, hipStream_t s);
+void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
+                            double* jac, const int* diagOff, hipStream_t s);
 void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,
                     const double* delta, const NewtonUpdateParams& prm, int* counters, hipStream_t s);
 

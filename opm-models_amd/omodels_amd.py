@@ -101,6 +101,9 @@ SIGNATURES = {
     "omodels_linearizer_get_jacobian": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_get_residual": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_erase_matrix": (C.c_int, [C.c_void_p]),
+    "omodels_linearizer_set_separate_sparse_sources": (C.c_int, [C.c_void_p, C.c_int]),
+    "omodels_linearizer_source_blocks": (C.c_int, [C.c_void_p, _ip, _dp, _dp]),
+    "omodels_linearizer_add_sparse_sources": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
     "omodels_solver_create": (C.c_void_p, [C.c_void_p, C.POINTER(Params)]),
     "omodels_solver_destroy": (None, [C.c_void_p]),
     "omodels_solver_prepare": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -323,6 +326,33 @@ class Model:
         check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
         check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
         return J, r
+
+    def system(self):
+        """The linearizer's current Jacobian values and residual (no relinearization)."""
+        J = np.zeros(9 * self.nnzb)
+        r = np.zeros(3 * self.N)
+        check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
+        check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
+        return J, r
+
+    def set_separate_sparse_sources(self, enable=True):
+        check(lib().omodels_linearizer_set_separate_sparse_sources(self.h, int(enable)), "set_separate_sparse_sources")
+
+    def source_blocks(self):
+        """(cells, residual [n][3], diag [n][9]) of the problem's source terms at solution(0)."""
+        n = self.case.prob.n_src
+        cells = np.zeros(max(n, 1), np.int32)
+        r = np.zeros(3 * max(n, 1))
+        d = np.zeros(9 * max(n, 1))
+        got = check(lib().omodels_linearizer_source_blocks(self.h, ptr(cells, _ip), ptr(r), ptr(d)), "source_blocks")
+        return cells[:got], r[:3 * got], d[:9 * got]
+
+    def add_sparse_sources(self, cells, residual, diag):
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        residual = np.ascontiguousarray(residual, dtype=np.float64)
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        check(lib().omodels_linearizer_add_sparse_sources(self.h, len(cells), ptr(cells, _ip), ptr(residual), ptr(diag)),
+              "add_sparse_sources")
 
     def set_profiling(self, enable=True):
         check(lib().omodels_model_set_profiling(self.h, int(enable)), "set_profiling")

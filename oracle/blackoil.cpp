@@ -835,6 +835,33 @@ int ob_intensive(const ob_problem* p, const double* pv3, int meaning, int cell, 
     return 0;
 }
 
+/* The problem's source terms as blocks: per source the residual contribution
+ * (times -V) and its derivatives w.r.t. the cell's primaries, exactly what
+ * linearize adds in its cell loop (tpfalinearizer.hh:827-841); with separate
+ * sparse source terms they are added after the loop (:842-846). */
+int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cells,
+                     double* residual, double* diag)
+{
+    try {
+        Problem pr(p);
+        const double V = pr.G.volume();
+        for (int s = 0; s < p->n_src; ++s) {
+            const long I = p->src_cell[s];
+            IQ<E3> qi;
+            computeIQ<E3>(pr.pvt, pv + 3 * I, meaning[I], p->poro[I], qi);
+            E3 q[3];
+            sourceOf(pr.pvt, qi, p->src_type[s], p->src_a[s], p->src_b[s], V, q);
+            cells[s] = (int32_t)I;
+            for (int e = 0; e < 3; ++e) {
+                const E3 t = q[e] * (-V);
+                residual[3 * s + e] = t.v;
+                for (int pvi = 0; pvi < 3; ++pvi) diag[9 * s + 3 * e + pvi] = t.d[pvi];
+            }
+        }
+        return p->n_src;
+    } catch (...) { return -1; }
+}
+
 int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  const double* storage1, double dt, double* residual, double* jac, int nthreads)
 {

@@ -92,6 +92,10 @@ int ob_storage(const ob_problem* p, const double* pv, const uint8_t* meaning,
 int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  const double* storage1, double dt, double* residual /* [N][3] */,
                  double* jac /* [NNZB][3][3] */, int nthreads);
+/* source terms as sparse blocks: cells [n_src], residual [n_src][3] (times -V),
+ * derivatives [n_src][3][3]; returns n_src */
+int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cells,
+                     double* residual, double* diag);
 /* Intensive quantities of one cell (values and derivatives); used by tests
  * to pin the PVT and the AD propagation. out: [13][4] = (p, rho_w, rho_o,
  * rho_g, lam_w, lam_o, lam_g, invB_w, invB_o, invB_g, Rs, phi, So) x (v,d0,d1,d2) */

@@ -73,6 +73,7 @@ def lib():
         L.ob_build_pattern.argtypes = [C.POINTER(ObProblem), _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip]
         L.ob_storage.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp]
         L.ob_linearize.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, _dp, _dp, C.c_int]
+        L.ob_source_blocks.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _ip, _dp, _dp]
         L.ob_intensive.argtypes = [C.POINTER(ObProblem), _dp, C.c_int, C.c_int, _dp]
         L.ob_ilu0_factor.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp]
         L.ob_ilu0_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
@@ -151,6 +152,17 @@ def linearize(prob, pv, mean, st1, dt, nthreads=1):
     rc = lib().ob_linearize(C.byref(prob), P(pv), P(mean, _u8p), P(st1), dt, P(r), P(J), nthreads)
     assert rc == 0
     return J, r
+
+
+def source_blocks(prob, pv, mean):
+    """(cells, residual [n][3], diag [n][9]) of the problem's source terms."""
+    n = prob.n_src
+    cells = np.zeros(max(n, 1), np.int32)
+    r = np.zeros(3 * max(n, 1))
+    d = np.zeros(9 * max(n, 1))
+    got = lib().ob_source_blocks(C.byref(prob), P(pv), P(mean, _u8p), P(cells, _ip), P(r), P(d))
+    assert got == n
+    return cells[:n], r[:3 * n], d[:9 * n]
 
 
 def intensive(prob, pv3, meaning, cell=0):

@@ -294,3 +294,108 @@ def test_sampled_kernel_timing_counts_every_launch():
         assert n1 == n3 > 0, k
         assert ms1 > 0 and ms3 > 0
         assert 0.2 < (ms3 / n3) / (ms1 / n1) < 5.0, k
+
+
+# ----------------------------------------------------------------------------
+# Sparse source terms: SeparateSparseSourceTerms (tpfalinearizer.hh:56) and the
+# addReservoirSourceTerms hook after the cell loop (:842-846)
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def _without_sources(c):
+    c.src_cell = c.src_cell[:0].copy()
+    c.src_type = c.src_type[:0].copy()
+    c.src_a = c.src_a[:0].copy()
+    c.src_b = c.src_b[:0].copy()
+    c.prob = c.descriptor(om.BlackOilProblem)
+    return c
+
+
+@pytest.mark.parametrize("dims,seed", [((6, 5, 3), 4), ((40, 40, 10), 5)])
+def test_source_blocks_parity(dims, seed):
+    c = case(*dims, seed=seed)
+    pv, mean = mixed_state(c, seed=seed)
+    m = om.Model(c)
+    m.set_solution(pv, mean)
+    cells, r, d = m.source_blocks()
+    co, ro, do = orc.source_blocks(orc.problem_from_case(c), pv, mean)
+    assert len(cells) == len(c.src_cell) > 0
+    assert np.array_equal(cells, co)
+    assert np.array_equal(_bits(r), _bits(ro))
+    assert np.array_equal(_bits(d), _bits(do))
+
+
+@pytest.mark.parametrize("dims,seed", [((6, 5, 3), 4), ((33, 9, 3), 2), ((40, 40, 10), 5)])
+def test_separate_sparse_sources_give_the_cell_loop_bits(dims, seed):
+    c = case(*dims, seed=seed)
+    pv, mean = mixed_state(c, seed=seed)
+    st1 = orc.storage(orc.problem_from_case(c), *mixed_state(c, seed=seed + 100))
+    J, r, Jo, ro = _linearize_both(c, pv, mean, st1, dt=43200.0)
+    m = om.Model(c)
+    m.set_solution(pv, mean)
+    m.set_storage1(st1)
+    m.set_separate_sparse_sources(True)
+    Js, rs = m.linearize(43200.0, iteration_index=1)
+    for a, b in ((J, Jo), (Js, Jo), (r, ro), (rs, ro)):
+        assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_add_sparse_sources_hook_matches_the_oracle():
+    """A problem without source terms, linearized, plus the oracle's source
+    blocks through the hook: the oracle's system with the source terms."""
+    c = case(9, 7, 4, seed=6)
+    pv, mean = mixed_state(c, seed=6)
+    prob = orc.problem_from_case(c)
+    st1 = orc.storage(prob, *mixed_state(c, seed=60))
+    cells, rb, db = orc.source_blocks(prob, pv, mean)
+    Jo, ro = orc.linearize(prob, pv, mean, st1, 86400.0)
+    bare = _without_sources(case(9, 7, 4, seed=6))
+    m = om.Model(bare)
+    m.set_solution(pv, mean)
+    m.set_storage1(st1)
+    m.linearize(86400.0, iteration_index=1)
+    m.add_sparse_sources(cells, rb, db)
+    J, r = m.system()
+    assert np.array_equal(_bits(J), _bits(Jo))
+    assert np.array_equal(_bits(r), _bits(ro))
+
+
+def test_add_sparse_sources_repeated_cells_in_order():
+    c = case(5, 4, 3, seed=8)
+    pv, mean = mixed_state(c, seed=8)
+    m = om.Model(c)
+    m.set_solution(pv, mean)
+    J0, r0 = m.linearize(3600.0, iteration_index=0)
+    rng = np.random.default_rng(8)
+    cells = np.array([3, 3, 7, 3, 0, c.N - 1, 7], np.int32)
+    rb = rng.standard_normal(3 * len(cells)) * np.abs(r0).max()
+    db = rng.standard_normal(9 * len(cells)) * np.abs(J0).max()
+    m.add_sparse_sources(cells, rb, db)
+    J, r = m.system()
+    diag = m.pattern()["diag_off"]
+    for k, I in enumerate(cells):        # one entry after another, as the hook promises
+        r0[3 * I:3 * I + 3] += rb[3 * k:3 * k + 3]
+        J0[9 * diag[I]:9 * diag[I] + 9] += db[9 * k:9 * k + 9]
+    assert np.array_equal(_bits(J), _bits(J0))
+    assert np.array_equal(_bits(r), _bits(r0))
+    # rejected entries change nothing
+    for bad in ([c.N], [-1]):
+        with pytest.raises(om.OmodelsError):
+            m.add_sparse_sources(np.array(bad, np.int32), np.zeros(3), np.zeros(9))
+    J2, r2 = m.system()
+    assert np.array_equal(_bits(J2), _bits(J)) and np.array_equal(_bits(r2), _bits(r))
+
+
+def test_simulate_with_separate_sparse_sources_is_identical():
+    c = case(20, 20, 5, seed=11)
+    runs = []
+    for sep in (False, True):
+        m = om.Model(c)
+        m.set_solution(c.pv0, c.mean0)
+        m.set_separate_sparse_sources(sep)
+        runs.append(m.simulate(om.default_params(), 5, 86400.0))
+    (rc0, h0, d0, l0, _), (rc1, h1, d1, l1, _) = runs
+    assert rc0 == 0 and rc1 == 0
+    assert l0 == l1
+    assert np.array_equal(_bits(h0), _bits(h1)) and np.array_equal(_bits(d0), _bits(d1))

@@ -235,6 +235,65 @@ def test_accumulation_term_closed_form():
 
 
 # ----------------------------------------------------------------------------
+# Sparse source terms: the blocks added after the cell loop
+def _source_setup(seed):
+    c = case(6, 5, 3, seed=seed)
+    pv, mean = mixed_state(c, seed=seed)
+    prob = orc.problem_from_case(c)
+    st1 = orc.storage(prob, *mixed_state(c, seed=seed + 20))
+    return c, pv, mean, prob, st1
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_source_blocks_are_what_the_cell_loop_adds(seed):
+    c, pv, mean, prob, st1 = _source_setup(seed)
+    J, r = orc.linearize(prob, pv, mean, st1, 86400.0)
+    bare = orc.problem_from_case(c)
+    bare.n_src = 0
+    J0, r0 = orc.linearize(bare, pv, mean, st1, 86400.0)
+    cells, rs, ds = orc.source_blocks(prob, pv, mean)
+    assert np.array_equal(cells, c.src_cell)
+    # the producers are open, so their blocks carry derivatives
+    assert all(np.abs(ds[9 * s:9 * s + 9]).max() > 0 for s in range(len(cells)) if c.src_type[s] == 1)
+    diag = orc.pattern(prob)["diag_off"]
+    for s, I in enumerate(cells):
+        r0[3 * I:3 * I + 3] += rs[3 * s:3 * s + 3]
+        J0[9 * diag[I]:9 * diag[I] + 9] += ds[9 * s:9 * s + 9]
+    assert np.array_equal(r, r0)
+    assert np.array_equal(J, J0)
+
+
+def test_source_blocks_closed_form_and_derivatives():
+    c, pv, mean, prob, st1 = _source_setup(4)
+    cells, rs, ds = orc.source_blocks(prob, pv, mean)
+    rw, ro, rg = om.default_fluid()[4][4:7]
+    for s, I in enumerate(cells):
+        R, D = rs[3 * s:3 * s + 3], ds[9 * s:9 * s + 9].reshape(3, 3)
+        a, b = c.src_a[s], c.src_b[s]
+        if c.src_type[s] == 0:
+            # water injection at a fixed rate: -rate in the water equation, no derivatives
+            assert R[1] == pytest.approx(-a, rel=1e-15) and R[0] == 0.0 and R[2] == 0.0
+            assert not D.any()
+            continue
+        # producer at fixed bottom-hole pressure: (rate of each phase at surface) * density,
+        # the gas equation also takes the oil's dissolved gas
+        q = orc.intensive(prob, pv[3 * I:3 * I + 3], mean[I], cell=I)
+        drive = q["p"][0] - b
+        assert drive > 0
+        w = a * drive * q["lam_w"][0] * q["invB_w"][0] * rw
+        o = a * drive * q["lam_o"][0] * q["invB_o"][0] * ro
+        g = a * drive * (q["lam_g"][0] * q["invB_g"][0] + q["Rs"][0] * q["lam_o"][0] * q["invB_o"][0]) * rg
+        assert R == pytest.approx([o, w, g], rel=1e-13)
+        # derivatives against central differences of the blocks themselves
+        for var, h in enumerate([1e-7, 1e-2, 1e-7 if mean[I] == 0 else 1e-5]):
+            pp, pm = pv.copy(), pv.copy()
+            pp[3 * I + var] += h
+            pm[3 * I + var] -= h
+            fd = (orc.source_blocks(prob, pp, mean)[1] - orc.source_blocks(prob, pm, mean)[1])[3 * s:3 * s + 3] / (2 * h)
+            assert np.abs(D[:, var] - fd).max() <= 2e-6 * np.abs(D).max(), (s, var, D[:, var], fd)
+
+
+# ----------------------------------------------------------------------------
 # A11-A12: linear algebra
 def _system(dims, seed=1):
     c = case(*dims, seed=seed)
