@@ -212,8 +212,6 @@ constexpr int kRecvWave = kStreamCells / 16;     // the receiving wave
 constexpr int kRecvPerLane = 8;                  // ghost values per receiving lane
 constexpr int kRecvDepth = 1;                    // loads in flight per ghost value (two measured slower: the extra polls load the fabric)
 constexpr int kMaxTileLevels = 256;              // cell levels of a tile, BX + BY + BZ - 2 < 256
-// s_waitcnt lgkmcnt(0) with the vector-memory and export counters left at their maxima
-constexpr int kWaitLds = 0xC07F;
 
 __device__ __forceinline__ int tri3(int m) { return m < 0 ? 0 : (m + 1) * (m + 2) * (m + 3) / 6; }
 // number of cells (a, b, c) of a BX x BY x BZ box with a + b + c < l
@@ -286,7 +284,8 @@ template <class S, bool Bwd>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* __restrict__ insts,
     const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
 {
-    __shared__ S sV[kPadMax * 3];             // the tile's cells and its ghost layer (padded layout)
+    __shared__ S sV[(kPadMax + 1) * 3];       // the tile's cells and its ghost layer (padded layout),
+                                              // then a zero triple read in place of absent neighbours
     __shared__ int sOrder[kStreamCells];      // tile positions in level order
     __shared__ int sDone[kMaxTileLevels];     // per cell level: compute waves that have finished it
     __shared__ int sWaves[kMaxTileLevels];    // per cell level: compute waves holding part of it
@@ -313,6 +312,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
         const int i0 = TI * BX, j0 = TJ * BY, k0 = TK * BZ;
         // every value slot starts empty; tile positions in level order
         for (int k = threadIdx.x; k < 3 * PXY * (BZ + 1); k += kSweepThreads) sV[k] = emptySlot<S>();
+        if (threadIdx.x < 3) sV[3 * kPadMax + threadIdx.x] = S(0);
         if (threadIdx.x < BX + BY + BZ - 2 + 1) {
             const int l = threadIdx.x;
             sDone[l] = 0;
@@ -417,7 +417,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
             const bool h2 = valid && (word & (1 << 29));
             const long Ic = (long)(i0 + li) + (long)g.nx * ((long)(j0 + lj) + (long)g.ny * (k0 + lk));
             const int p = Bwd ? li + PX * lj + PXY * lk : (li + 1) + PX * (lj + 1) + PXY * (lk + 1);
-            const int d0 = Bwd ? p + 1 : p - PXY, d1 = Bwd ? p + PX : p - PX, d2 = Bwd ? p + PXY : p - 1;
+            // dependency positions; an absent neighbour reads the zero triple
+            const int d0 = h0 ? (Bwd ? p + 1 : p - PXY) : kPadMax;
+            const int d1 = h1 ? (Bwd ? p + PX : p - PX) : kPadMax;
+            const int d2 = h2 ? (Bwd ? p + PXY : p - 1) : kPadMax;
             auto levelOf = [&](int w) {
                 const int a = w & 0xFF, b = (w >> 8) & 0xFF, c = (w >> 16) & 0xFF;
                 return Bwd ? (BX - 1 - a) + (BY - 1 - b) + (BZ - 1 - c) : a + b + c;
@@ -455,13 +458,17 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
             const int myLev = valid ? levelOf(word) : -1;
             const int lastLev = levelOf(sOrder[min(16 * wave + 15, ncell - 1)]);
             S res = S(0);
-            for (int lev = levelOf(sOrder[16 * wave]); lev <= lastLev; ++lev) {
-                // level lev-2 complete: every wave holding part of it has counted it
-                // (one broadcast LDS word per poll; the waves still further ahead
-                // wait here without loading the LDS)
-                if (lev > 1) {
+            const int firstLev = levelOf(sOrder[16 * wave]);
+            for (int lev = firstLev; lev <= lastLev; ++lev) {
+                // before its first level a wave waits until level lev-2 is complete
+                // (every wave holding part of it has counted it; one broadcast LDS
+                // word per poll), so the waves further ahead do not load the LDS with
+                // value polls.  The count only throttles: the value polls below carry
+                // the dependencies, and a wave's later levels follow its own results
+                // without it.
+                if (lev > 1 && lev == firstLev) {
                     const int waves = sWaves[lev - 2];
-                    long spins = 0;
+                    int spins = 0;
                     while (ldsLoad(sDone + lev - 2) < waves) {
                         if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
                             atomicOr(err, 1);
@@ -474,19 +481,19 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
                     // the dependencies themselves (level lev-1 and ghost values) are
                     // polled: a slot leaves the sentinel exactly once per tile, with its
                     // final value, so the last step of the chain costs no count
+                    // (the wave polls until all its lanes of the level are ready, so the
+                    // loop and its spin count are wave-uniform)
                     S x0[3], x1[3], x2[3];
-                    long spins = 0;
+                    int spins = 0;
                     for (;;) {
                         for (int w = 0; w < 3; ++w) {
                             x0[w] = ldsLoad(sV + 3 * d0 + w);
                             x1[w] = ldsLoad(sV + 3 * d1 + w);
                             x2[w] = ldsLoad(sV + 3 * d2 + w);
                         }
-                        unsigned empty = 0;
-                        for (int w = 0; w < 3; ++w)
-                            empty |= (unsigned)(h0 && isEmpty(x0[w])) | (unsigned)(h1 && isEmpty(x1[w])) |
-                                     (unsigned)(h2 && isEmpty(x2[w]));
-                        if (!empty) break;
+                        bool empty = false;
+                        for (int w = 0; w < 3; ++w) empty |= isEmpty(x0[w]) | isEmpty(x1[w]) | isEmpty(x2[w]);
+                        if (!__any(empty)) break;
                         if (++spins > kSpinLimit || ((spins & 255) == 0 && loadRelaxed(err))) {
                             atomicOr(err, 1);
                             break;
@@ -512,10 +519,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
                         if (s2) sendSlot(own + 6 * N, out);
                     }
                 }
-                // count the level once this wave's results of it are in LDS
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_s_waitcnt(kWaitLds);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // count the level (a throttle only, so no wait for the results first)
                 if (lane == 0) __hip_atomic_fetch_add(sDone + lev, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             // the tile's own results leave after its levels, behind the face values
