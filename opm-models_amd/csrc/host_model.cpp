@@ -159,7 +159,8 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
     b64_.alloc(n); x64_.alloc(n);
     ks_.alloc(1);
     fail_.alloc(1);
-    OM_HIP(hipHostMalloc((void**)&ksHost_, sizeof(KrylovScalars), hipHostMallocDefault));
+    OM_HIP(hipHostMalloc((void**)&ksHost_, 2 * sizeof(KrylovScalars), hipHostMallocDefault));
+    for (auto& e : ksEv_) OM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (prm_.precision < 0 || prm_.precision > 2) throw std::invalid_argument("precision must be 0, 1 or 2");
     if (prm_.precision == 2 && prm_.precond != 0) throw std::invalid_argument("the FP32 solve uses block ILU(0)");
     if (prm_.ilu_schedule < 0 || prm_.ilu_schedule > 3) throw std::invalid_argument("ilu_schedule must be 0..3");
@@ -186,6 +187,8 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
 
 BiCGStabBackend::~BiCGStabBackend()
 {
+    for (auto e : ksEv_)
+        if (e) (void)hipEventDestroy(e);
     if (ksHost_) (void)hipHostFree(ksHost_);
 }
 
@@ -355,14 +358,14 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
 }
 
 template <class S>
-void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v)
+void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run)
 {
     hipStream_t s = m_.stream;
     const int N = (int)m_.N;
     if (prm_.precond == 0) {
         m_.timer.begin(kClsIluApply, s);
         if (tiled_)
-            tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s);
+            tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s, run);
         else
             launchIlu0Apply<S, double>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv,
                                        d, v, m_.levels, s);
@@ -434,27 +437,32 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     k0.rho = 1; k0.alpha = 1; k0.omega = 1; k0.init_norm = init;
     k0.fail_code = 0; k0.fail_tag = -1; k0.stag_half = 0; k0.stag_full_tag = -1;
     k0.tag_base = 0;
+    k0.tol = tol; k0.abs_tol = absTol; k0.max_error = prm_.lin_max_error;
+    k0.run = 1; k0.stop_rc = -1; k0.stop_back = 0;
+    const int* run = reinterpret_cast<const int*>(reinterpret_cast<const char*>(ks_.p) + offsetof(KrylovScalars, run));
     OM_HIP(hipMemcpyAsync(ks_.p, &k0, sizeof(k0), hipMemcpyHostToDevice, s));
     OM_HIP(hipMemsetAsync(v, 0, m * sizeof(S), s));
     OM_HIP(hipMemsetAsync(p, 0, m * sizeof(S), s));
 
-    // first half: rho, p, y = M^-1 p, v = A y, alpha, x/r update, scalars to the host
+    // first half: rho, p, y = M^-1 p, v = A y, alpha, x/r update, convergence test.
+    // Every kernel of the solve returns at once after the test has ended it
+    // (KrylovScalars::run), so the host queues iteration it+1 before it has
+    // seen the test of iteration it, and the device never waits for the host.
     auto firstHalf = [&]() {
         launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, 0, s);
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
-        precApply<S>(lu, dinv, p, y);
+        precApply<S>(lu, dinv, p, y, run);
         m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s);
+        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s, run);
         m_.timer.end(s);
         launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, 0, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
-        OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
     };
     // second half: z = M^-1 r, y = A z, omega, x/r update, next tag base
     auto secondHalf = [&]() {
-        precApply<S>(lu, dinv, r, z);
+        precApply<S>(lu, dinv, r, z, run);
         m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s);
+        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s, run);
         m_.timer.end(s);
         launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, 1, s);
         launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, 2, s);
@@ -470,8 +478,8 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         if (tiled_ && prm_.precond == 0) {
             for (int pass = 0; pass < 2; ++pass) {
                 const long g0 = tiled_->generation();
-                tiled_->primeApply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, p, y, s);
-                tiled_->primeApply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, r, z, s);
+                tiled_->primeApply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, p, y, s, run);
+                tiled_->primeApply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, r, z, s, run);
                 if (tiled_->generation() == g0) break;
             }
         }
@@ -495,22 +503,32 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         }
     }
 
-    for (int it = 0; it < maxIter; ++it) {
-        const int tag = 2 * it;
+    // iteration it: the second half of it-1 and the first half of it, then the
+    // scalars (with the device's verdict) to host copy it % 2
+    auto enqueue = [&](int it) {
         if (gr) {
             OM_HIP(hipGraphLaunch(it == 0 ? gr->first : gr->iter, s));
         } else {
             if (it > 0) secondHalf();
             firstHalf();
         }
-        m_.sync();
-        const KrylovScalars& K = *ksHost_;
-        if (it > 0 && K.fail_code && K.fail_tag == tag - 1) { *its = it - 1; return 2; }
-        if (it > 0 && K.stag_full_tag == tag - 1) { *its = it - 1; return 1; }
-        if (K.fail_code) { *its = it; return 2; }
+        OM_HIP(hipMemcpyAsync(ksHost_ + (it & 1), ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+        OM_HIP(hipEventRecord(ksEv_[it & 1], s));
+    };
+    if (maxIter > 0) enqueue(0);
+    for (int it = 0; it < maxIter; ++it) {
+        if (it + 1 < maxIter) enqueue(it + 1);
+        OM_HIP(hipEventSynchronize(ksEv_[it & 1]));
+        const KrylovScalars K = ksHost_[it & 1];
+        if (K.run == 0) {
+            *its = it - K.stop_back;
+            if (it + 1 < maxIter && !gr) {      // the queued iteration it+1 was skipped: untime it
+                if (prm_.precond == 0) m_.timer.dropLast(kClsIluApply, 2);
+                m_.timer.dropLast(kClsSpmv, 2);
+            }
+            return K.stop_rc;
+        }
         *its = it;
-        if (converged(K.resnorm)) return 0;
-        if (K.stag_half || K.resnorm > prm_.lin_max_error) return 1;
     }
     if (maxIter > 0) secondHalf();
     OM_HIP(hipMemcpyAsync(ksHost_, ks_.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));

@@ -1216,32 +1216,34 @@ void TiledIlu::factor(const int* rp, const int* diag, double* lu, double* dinvOu
 }
 
 template <class S, class P>
-void TiledIlu::apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s)
+void TiledIlu::apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
+                     const int* active)
 {
     static_assert(sizeof(P) == 8, "pivots are kept in FP64");
     IluInst I{};
     I.g = g; I.rp = rp; I.diag = diag; I.lu = lu; I.dinv = (const double*)dinv; I.d = d; I.v = v;
-    I.active = nullptr;
+    I.active = active;
     set_->apply<S>({I}, s);
 }
 template <class S, class P>
-void TiledIlu::primeApply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s)
+void TiledIlu::primeApply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
+                          const int* active)
 {
     IluInst I{};
     I.g = g; I.rp = rp; I.diag = diag; I.lu = lu; I.dinv = (const double*)dinv; I.d = d; I.v = v;
-    I.active = nullptr;
+    I.active = active;
     set_->prime<S>({I}, s);
 }
 template void TiledIlu::primeApply<double, double>(const int*, const int*, const double*, const double*,
-                                                   const double*, double*, hipStream_t);
+                                                   const double*, double*, hipStream_t, const int*);
 template void TiledIlu::primeApply<float, double>(const int*, const int*, const float*, const double*, const float*,
-                                                  float*, hipStream_t);
+                                                  float*, hipStream_t, const int*);
 
 long TiledIlu::generation() const { return set_->generation(); }
 
 template void TiledIlu::apply<double, double>(const int*, const int*, const double*, const double*, const double*,
-                                              double*, hipStream_t);
+                                              double*, hipStream_t, const int*);
 template void TiledIlu::apply<float, double>(const int*, const int*, const float*, const double*, const float*,
-                                             float*, hipStream_t);
+                                             float*, hipStream_t, const int*);
 
 }  // namespace omodels

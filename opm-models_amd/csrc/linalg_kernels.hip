@@ -67,10 +67,10 @@ __device__ __forceinline__ bool bInvert(S* a)
 template <int BS, class S>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                               const S* __restrict__ a, const S* __restrict__ x,
-                                              S* __restrict__ y)
+                                              S* __restrict__ y, const int* __restrict__ run)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (run && !*run)) return;
     S acc[BS];
 #pragma unroll
     for (int e = 0; e < BS; ++e) acc[e] = S(0);
@@ -108,8 +108,9 @@ __device__ __forceinline__ int xcdContiguousBlock(int w, int G)
 template <class S>
 __global__ __launch_bounds__(256) void k_spmv3(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                const S* __restrict__ a, const S* __restrict__ x,
-                                               S* __restrict__ y)
+                                               S* __restrict__ y, const int* __restrict__ run)
 {
+    if (run && !*run) return;
     const long g = (long)xcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = (int)(g / 3), e = (int)(g - 3L * (g / 3));
     if (i >= n) return;
@@ -151,18 +152,22 @@ __global__ __launch_bounds__(256) void k_spmv3(int n, const int* __restrict__ rp
 }
 
 template <class S>
-void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a, const S* x, S* y, hipStream_t st)
+void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a, const S* x, S* y, hipStream_t st,
+                const int* run)
 {
     if (bs == 3) {
         const long lanes = 3L * n;
-        hipLaunchKernelGGL((k_spmv3<S>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, n, rp, ci, a, x, y);
+        hipLaunchKernelGGL((k_spmv3<S>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, n, rp, ci, a, x, y,
+                           run);
     } else {
         dim3 g((n + 255) / 256), b(256);
-        hipLaunchKernelGGL((k_spmv<2, S>), g, b, 0, st, n, rp, ci, a, x, y);
+        hipLaunchKernelGGL((k_spmv<2, S>), g, b, 0, st, n, rp, ci, a, x, y, run);
     }
 }
-template void launchSpmv<double>(int, int, const int*, const int*, const double*, const double*, double*, hipStream_t);
-template void launchSpmv<float>(int, int, const int*, const int*, const float*, const float*, float*, hipStream_t);
+template void launchSpmv<double>(int, int, const int*, const int*, const double*, const double*, double*, hipStream_t,
+                                 const int*);
+template void launchSpmv<float>(int, int, const int*, const int*, const float*, const float*, float*, hipStream_t,
+                                const int*);
 
 // ---------------------------------------------------------------------------
 // conversions
@@ -458,8 +463,10 @@ __device__ __forceinline__ double finalReduce(const double* __restrict__ partial
 
 template <class S>
 __global__ __launch_bounds__(kRedThreads) void k_dot_partial(long m, const S* __restrict__ x,
-                                                             const S* __restrict__ y, double* __restrict__ part)
+                                                             const S* __restrict__ y, double* __restrict__ part,
+                                                             const KrylovScalars* __restrict__ ks)
 {
+    if (!ks->run) return;
     double v = 0.0;
     for (long i = (long)blockIdx.x * kRedThreads + threadIdx.x; i < m; i += (long)kRedBlocks * kRedThreads)
         v += (double)x[i] * (double)y[i];
@@ -476,6 +483,7 @@ __device__ __forceinline__ void markFail(KrylovScalars* ks, int tag)
 __global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restrict__ part, KrylovScalars* ks,
                                                            int op, double eps, int tagOff)
 {
+    if (!ks->run) return;
     const double s = finalReduce<false>(part);
     if (threadIdx.x != 0) return;
     const int tag = ks->tag_base + tagOff;
@@ -496,7 +504,7 @@ template <class S>
 void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks, int op, int tagOff,
                     hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dot_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, x, y, partial);
+    hipLaunchKernelGGL((k_dot_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, x, y, partial, ks);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, op, eps, tagOff);
 }
@@ -511,7 +519,7 @@ __global__ void k_update_p(long m, const S* __restrict__ r, const S* __restrict_
                            const KrylovScalars* __restrict__ ks)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    if (i >= m || !ks->run) return;
     const S omega = (S)ks->omega, beta = (S)ks->beta;
     S tmp = v[i];
     tmp *= omega;
@@ -534,6 +542,7 @@ template <class S>
 __global__ __launch_bounds__(kRedThreads) void k_update_xr(long m, const S* __restrict__ y, const S* __restrict__ v,
     S* __restrict__ x, S* __restrict__ r, double* __restrict__ part, const KrylovScalars* __restrict__ ks)
 {
+    if (!ks->run) return;
     const S alpha = (S)ks->alpha;
     double mx = 0.0, nz = 0.0;
     for (long i = (long)blockIdx.x * kRedThreads + threadIdx.x; i < m; i += (long)kRedBlocks * kRedThreads) {
@@ -557,11 +566,28 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr(long m, const S* __re
 
 __global__ __launch_bounds__(kRedThreads) void k_crit_final(const double* __restrict__ part, KrylovScalars* ks)
 {
+    if (!ks->run) return;
     const double mx = finalReduce<true>(part);
     const double nz = finalReduce<true>(part + kRedBlocks);
     if (threadIdx.x != 0) return;
     ks->resnorm = mx;
     ks->stag_half = (nz == 0.0) ? 1 : 0;
+    // the test after the first half of iteration tag_base / 2 (bicgstabsolver.hh:
+    // 252-273) with the outcome of the previous second half (:286-318), in the
+    // order of the host loop; once it ends the solve, the queued kernels of
+    // the next iteration return at once
+    const int tag = ks->tag_base;
+    int rc = -1, back = 0;
+    if (tag > 0 && ks->fail_code && ks->fail_tag == tag - 1) { rc = 2; back = 1; }
+    else if (tag > 0 && ks->stag_full_tag == tag - 1) { rc = 1; back = 1; }
+    else if (ks->fail_code) rc = 2;
+    else if (mx / fmax(1e-20, ks->init_norm) <= ks->tol || mx <= ks->abs_tol) rc = 0;
+    else if (ks->stag_half || mx > ks->max_error) rc = 1;
+    if (rc >= 0) {
+        ks->stop_rc = rc;
+        ks->stop_back = back;
+        ks->run = 0;
+    }
 }
 
 template <class S>
@@ -576,8 +602,10 @@ template void launchUpdateXR<float>(long, const float*, const float*, float*, fl
 // <t,t> and <t,s> in one pass; omega = <t,s>/<t,t>  (:287-292)
 template <class S>
 __global__ __launch_bounds__(kRedThreads) void k_dot2_partial(long m, const S* __restrict__ t,
-                                                              const S* __restrict__ s, double* __restrict__ part)
+                                                              const S* __restrict__ s, double* __restrict__ part,
+                                                              const KrylovScalars* __restrict__ ks)
 {
+    if (!ks->run) return;
     double tt = 0.0, ts = 0.0;
     for (long i = (long)blockIdx.x * kRedThreads + threadIdx.x; i < m; i += (long)kRedBlocks * kRedThreads) {
         const double ti = (double)t[i];
@@ -591,6 +619,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dot2_partial(long m, const S* _
 __global__ __launch_bounds__(kRedThreads) void k_dot2_final(const double* __restrict__ part, KrylovScalars* ks,
                                                             double eps, int tagOff)
 {
+    if (!ks->run) return;
     const double tt = finalReduce<false>(part);
     const double ts = finalReduce<false>(part + kRedBlocks);
     if (threadIdx.x != 0) return;
@@ -602,7 +631,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dot2_final(const double* __rest
 template <class S>
 void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks, int tagOff, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dot2_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, t, s, partial);
+    hipLaunchKernelGGL((k_dot2_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, t, s, partial, ks);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
     hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, eps, tagOff);
 }
@@ -614,6 +643,7 @@ template <class S>
 __global__ __launch_bounds__(kRedThreads) void k_update_xr2(long m, const S* __restrict__ z, const S* __restrict__ t,
     S* __restrict__ x, S* __restrict__ r, double* __restrict__ part, const KrylovScalars* __restrict__ ks)
 {
+    if (!ks->run) return;
     const S omega = (S)ks->omega;
     double nz = 0.0;
     for (long i = (long)blockIdx.x * kRedThreads + threadIdx.x; i < m; i += (long)kRedBlocks * kRedThreads) {
@@ -628,6 +658,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update_xr2(long m, const S* __r
 __global__ __launch_bounds__(kRedThreads) void k_stag_final(const double* __restrict__ part, KrylovScalars* ks,
                                                             int tagOff, int advance)
 {
+    if (!ks->run) return;
     const double nz = finalReduce<true>(part);
     if (threadIdx.x != 0) return;
     if (nz == 0.0 && ks->stag_full_tag < 0) ks->stag_full_tag = ks->tag_base + tagOff;

@@ -119,12 +119,12 @@ public:
 private:
     template <class S> int bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x,
                                     double tol, double absTol, int maxIter, int* its);
-    template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v);
+    template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run = nullptr);
     int solveImpl();
     void tuneIlu();   // pick the tile shape / schedule of the ILU(0) by timing them
 
     // One BiCGStab iteration as replayed hipGraphs: `first` holds the first
-    // half of iteration 0 (up to the scalars copied to the host), `iter` the
+    // half of iteration 0 (up to the convergence test), `iter` the
     // second half of iteration it-1 followed by the first half of iteration
     // it.  Captured for one set of buffers (key) and one generation of the
     // tiled ILU's resident descriptors; recaptured when either changes.
@@ -149,7 +149,8 @@ private:
     DevBuf<double> wd_[5];
     DevBuf<float> wf_[5];
     DevBuf<KrylovScalars> ks_;
-    KrylovScalars* ksHost_ = nullptr;
+    KrylovScalars* ksHost_ = nullptr;    // two pinned copies: iteration it lands in ksHost_[it % 2]
+    hipEvent_t ksEv_[2] = {nullptr, nullptr};
     DevBuf<int> fail_;
     std::unique_ptr<TiledIlu> tiled_;
     bool prepared_ = false;

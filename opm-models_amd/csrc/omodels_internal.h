@@ -70,7 +70,7 @@ enum KernelClass : int { kClsAssemble = 0, kClsSpmv, kClsIluApply, kClsIluFactor
 struct KernelTimer {
     bool enabled = false;
     int every = 1;
-    struct Pair { hipEvent_t a, b; int cls; };
+    struct Pair { hipEvent_t a, b; int cls; long idx; };
     std::vector<Pair> pending;
     std::vector<hipEvent_t> pool;
     double ms[kNumClasses] = {0, 0, 0, 0, 0};        // summed over the timed launches
@@ -78,6 +78,7 @@ struct KernelTimer {
     long launched[kNumClasses] = {0, 0, 0, 0, 0};    // all launches
     hipEvent_t cur = nullptr;
     int curCls = 0;
+    long curIdx = 0;
     hipEvent_t get()
     {
         if (pool.empty()) { hipEvent_t e; OM_HIP(hipEventCreate(&e)); return e; }
@@ -86,7 +87,8 @@ struct KernelTimer {
     void begin(int cls, hipStream_t s)
     {
         if (!enabled) return;
-        if (launched[cls]++ % every != 0) { cur = nullptr; return; }
+        curIdx = launched[cls]++;
+        if (curIdx % every != 0) { cur = nullptr; return; }
         cur = get(); curCls = cls;
         OM_HIP(hipEventRecord(cur, s));
     }
@@ -95,7 +97,7 @@ struct KernelTimer {
         if (!enabled || !cur) return;
         hipEvent_t b = get();
         OM_HIP(hipEventRecord(b, s));
-        pending.push_back({cur, b, curCls});
+        pending.push_back({cur, b, curCls, curIdx});
         cur = nullptr;
         if (pending.size() > 8192) collect();
     }
@@ -111,6 +113,19 @@ struct KernelTimer {
         pending.clear();
     }
     void reset() { collect(); for (int i = 0; i < kNumClasses; ++i) { ms[i] = 0; count[i] = 0; launched[i] = 0; } }
+    // forget the last n launches of a class (queued work that the device skipped)
+    void dropLast(int cls, long n)
+    {
+        if (!enabled || n <= 0) return;
+        const long from = launched[cls] - n;
+        std::vector<Pair> keep;
+        for (auto& p : pending) {
+            if (p.cls == cls && p.idx >= from) { pool.push_back(p.a); pool.push_back(p.b); }
+            else keep.push_back(p);
+        }
+        pending.swap(keep);
+        launched[cls] = from;
+    }
     // total time of a class over all its launches (the timed ones' average times the launch count)
     double estimatedMs(int cls) const { return count[cls] ? ms[cls] * (double)launched[cls] / count[cls] : 0.0; }
     ~KernelTimer()
@@ -148,12 +163,20 @@ struct KrylovScalars {
     int stag_full_tag;    // half-iteration index of a full-step stagnation, -1 if none
     int tag_base;         // half-iteration index of the current iteration's first half (2 it)
     int pad_;
+    // the test after each first half, evaluated on the device so that the next
+    // iteration can be queued before the host has seen this one's result
+    double tol, abs_tol, max_error;
+    int run;              // 1 while the solve goes on; every kernel of the solve returns at once when 0
+    int stop_rc;          // the solve's return code once run is 0
+    int stop_back;        // iterations the count goes back (0, or 1 for a test of the previous second half)
+    int pad2_;
 };
 
 constexpr int kRedBlocks = 1024;     // partial sums per reduction (fixed => deterministic)
 
+// run: nullptr, or a device flag; the product is skipped while it is 0
 template <class S> void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a,
-                                   const S* x, S* y, hipStream_t st);
+                                   const S* x, S* y, hipStream_t st, const int* run = nullptr);
 template <class S> void launchCopyConvertD(long m, const double* x, S* y, hipStream_t st);
 
 // ILU0 with level scheduling; lvlPtr/lvlRows on the device, lvlPtrHost on the host

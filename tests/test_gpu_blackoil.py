@@ -294,6 +294,11 @@ def test_sampled_kernel_timing_counts_every_launch():
         assert n1 == n3 > 0, k
         assert ms1 > 0 and ms3 > 0
         assert 0.2 < (ms3 / n3) / (ms1 / n1) < 5.0, k
+    # a solve that converges after the first half of iteration i applies the
+    # preconditioner and the operator 2 i + 1 times; the iteration queued
+    # behind the test and skipped by the device is not counted
+    for k in ("spmv", "ilu_apply"):
+        assert out[0][k][1] == 2 * st.linear_iters + st.newton_iters, k
 
 
 # ----------------------------------------------------------------------------
