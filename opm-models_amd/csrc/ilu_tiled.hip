@@ -185,6 +185,16 @@ __device__ __forceinline__ void publish(int* done, int tile, int epoch)
 
 __device__ __forceinline__ bool isActive(const IluInst& I) { return I.active == nullptr || *I.active != 0; }
 
+// A launch for one instance that is inactive returns at once: no workgroup
+// draws a ticket, the last one to leave resets the counter as usual.
+__device__ __forceinline__ bool skipLaunch(const IluInst* insts, int nInst, unsigned long long* counter,
+                                           unsigned int* exits)
+{
+    if (nInst != 1 || isActive(insts[0])) return false;
+    leaveLaunch(counter, exits);
+    return true;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -282,7 +292,7 @@ __device__ __forceinline__ void ldsStore(S* p, S v)
 // (FP64) pivots, upper blocks in column order +x, +y, +z.
 template <class S, bool Bwd>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits, int nInst)
 {
     __shared__ S sV[(kPadMax + 1) * 3];       // the tile's cells and its ghost layer (padded layout),
                                               // then a zero triple read in place of absent neighbours
@@ -292,6 +302,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
     __shared__ int sTicket;
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (skipLaunch(insts, nInst, counter, exits)) return;
     int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
     for (;;) {
         if (t < 0 || t >= T) break;
@@ -543,12 +554,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
 // Forward sweep: v = L^-1 d (unit lower), lower blocks in column order -z,-y,-x.
 template <class S>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_batch(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits, int nInst)
 {
     __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
     __shared__ int sOrder[kStreamCells];   // cells in level order
     __shared__ int sTicket;
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
+    if (skipLaunch(insts, nInst, counter, exits)) return;
     int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
     for (;;) {
         if (t < 0 || t >= T) break;
@@ -632,12 +644,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_fwd_batch(const IluInst* 
 // Backward sweep: v = U^-1 v with the inverted (FP64) pivots; upper blocks +x,+y,+z.
 template <class S>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_bwd_batch(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits)
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits, int nInst)
 {
     __shared__ S sV[kPadMax * 3];          // results of the tile's cells (padded layout)
     __shared__ int sOrder[kStreamCells];
     __shared__ int sTicket;
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
+    if (skipLaunch(insts, nInst, counter, exits)) return;
     int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
     for (;;) {
         if (t < 0 || t >= T) break;
@@ -1157,7 +1170,8 @@ void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
     unsigned int* ex = exits_.p;
     for (int sweep = 0; sweep < 2; ++sweep) {
         const int grid = std::min(T_, stream ? residentSweep_[prec][sweep] : residentBatch_[prec][sweep]);
-        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex};
+        int nInst = (int)insts.size();
+        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex, &nInst};
         OM_HIP(hipLaunchKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
     }
 }
