@@ -55,10 +55,10 @@ CONFIGS = {
 # device copy of 256 MiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); the commands and
 # per-kernel figures are in profiles/blackoil_500k_hbm_traffic.md.
 HBM_TRAFFIC_500K = {
-    "ilu_apply": 592127569,    # forward + backward sweep
-    "spmv": 347830615,
-    "assemble": 483159271,
-    "ilu_factor": 656911064,
+    "ilu_apply": 585807565,    # forward + backward sweep
+    "spmv": 347363328,
+    "assemble": 482855219,
+    "ilu_factor": 634320998,
 }
 
 
