@@ -304,6 +304,19 @@ def main():
         "setup_s": round(t_setup, 3),
     }
 
+    # FLOWS / FLORES of the final state (TpfaLinearizer::updateFlowsInfo), after
+    # the timed region: kernel time from the same HIP events (class "other")
+    ms0, n0 = model.kernel_times()["other"]
+    for _ in range(8):
+        model.flows()
+    ms1, n1 = model.kernel_times()["other"]
+    if n1 > n0:
+        favg = (ms1 - ms0) / (n1 - n0)
+        # per cell: primaries, meaning, three transmissibilities, row pointer read;
+        # FLOWS and FLORES written for each neighbour entry (3 doubles each)
+        fb = N * (24 + 1 + 24 + 4) + 2 * (nnzb - N) * 24
+        line["flows"] = {"avg_us": round(favg * 1e3, 2), "bytes_per_launch": fb,
+                         "achieved_GBs": round(fb / (favg * 1e-3) / 1e9, 1)}
     if not args.no_extra:
         line["beside"] = beside_runs(om, args.config, dt)
     if not args.no_cpu:

@@ -175,6 +175,16 @@ int omodels_linearizer_set_separate_sparse_sources(omodels_model* m, int enable)
  * linearizer's units) and their derivatives w.r.t. the cell's primaries
  * [n_src][3][3] row-major; returns n_src.  NULL arrays are not written. */
 int omodels_linearizer_source_blocks(omodels_model* m, int32_t* cells, double* residual, double* diag);
+/* updateFlowsInfo (tpfalinearizer.hh:647-705) at the current solution(0), for
+ * the FLOWS and FLORES output: per cell and per neighbour in intersection order
+ * (the order of nb_ptr / nb_cell of omodels_linearizer_pattern, nnzb - N
+ * entries), flows = the values of the face's component fluxes times the face
+ * area as the residual takes them, flores = each phase's darcyFlux.value() *
+ * faceArea at its component index (blackoillocalresidualtpfa.hh:376-385);
+ * both [nnzb - N][3], either may be NULL.  The face direction of an entry is
+ * its position among the cell's existing neighbours -x, +x, -y, +y, -z, +z
+ * (the reference's dirId 0..5, ecfvstencil.hh:209-218). */
+int omodels_linearizer_flows(omodels_model* m, double* flows, double* flores);
 /* addReservoirSourceTerms(residual, diagMatAddress) (tpfalinearizer.hh:843-846):
  * residual[cell] += r and A[cell][cell] += d for every entry, in the given
  * order (a cell may appear more than once). */

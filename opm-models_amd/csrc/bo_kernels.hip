@@ -272,6 +272,45 @@ __global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const 
     }
 }
 
+// Face rates of the current solution (TpfaLinearizer::updateFlowsInfo,
+// tpfalinearizer.hh:647-705): for every cell I and every neighbour J in
+// intersection order, the values of the face's component fluxes as the
+// residual takes them (FLOWS) and the phases' reservoir-volume rates (FLORES),
+// [entry][3] with entries in the pattern's neighbour order.  One thread per
+// cell, the assembly's face evaluation.
+__global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, const uint8_t* __restrict__ mean,
+                           const double* __restrict__ trans, const int* __restrict__ rowPtr,
+                           double* __restrict__ flows, double* __restrict__ flores)
+{
+    const long nxy = (long)G.nx * G.ny;
+    const long N = nxy * G.nz;
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    const int i = (int)(I % G.nx), j = (int)((I / G.nx) % G.ny), k = (int)(I / nxy);
+    IqE qi;
+    iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
+    const double zI = G.dz * ((double)k + 0.5);
+    long e = rowPtr[I] - I;     // first neighbour entry of I
+    auto face = [&](long J, double t, double area, double zJ, bool upInTie) {
+        IqV qj;
+        iqValues(f, pv[3 * J], pv[3 * J + 1], pv[3 * J + 2], mean[J], qj);
+        E3 fl[3];
+        double darcy[3];
+        faceFluxT<true>(f, qi, qj, t, area, (zI - zJ) * G.g, upInTie, fl, darcy);
+        for (int c = 0; c < 3; ++c) {
+            if (flows) flows[3 * e + c] = fl[c].v;
+            if (flores) flores[3 * e + c] = darcy[c];
+        }
+        ++e;
+    };
+    if (i > 0) face(I - 1, trans[I - 1], G.area[0], zI, false);
+    if (i + 1 < G.nx) face(I + 1, trans[I], G.area[0], zI, true);
+    if (j > 0) face(I - G.nx, trans[N + I - G.nx], G.area[1], zI, false);
+    if (j + 1 < G.ny) face(I + G.nx, trans[N + I], G.area[1], zI, true);
+    if (k > 0) face(I - nxy, trans[2 * N + I - nxy], G.area[2], G.dz * ((double)(k - 1) + 0.5), false);
+    if (k + 1 < G.nz) face(I + nxy, trans[2 * N + I], G.area[2], G.dz * ((double)(k + 1) + 0.5), true);
+}
+
 // Storage of the current solution: the cache of time level 1 at the first
 // Newton iteration (recycleFirstIterationStorage, tpfalinearizer.hh:786-812).
 __global__ void k_bo_storage(BoFluid f, long N, const double* __restrict__ pv,
@@ -395,6 +434,15 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
     if (nsrc <= 0) return;
     hipLaunchKernelGGL(k_bo_source_blocks, dim3((nsrc + 63) / 64), dim3(64), 0, s, f, G, pv, mean, nsrc,
                        cell, type, a, b, outRes, outDiag);
+}
+
+void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
+                   const int* rowPtr, double* flows, double* flores, hipStream_t s)
+{
+    const long N = (long)G.nx * G.ny * G.nz;
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, f, G, pv, mean, trans, rowPtr,
+                       flows, flores);
 }
 
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,

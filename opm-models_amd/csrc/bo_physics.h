@@ -221,10 +221,15 @@ __device__ __forceinline__ void storageValues(const BoFluid& f, double sw, doubl
 // One face seen from its interior cell I (derivatives w.r.t. I only), times
 // the face area.  upInTie: upwind choice when the pressure difference is
 // exactly zero (equal volumes -> the smaller global index, transfluxmodule.hh:283-301).
-__device__ __forceinline__ void faceFlux(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
-                                         double area, double dZg, bool upInTie, E3 flux[3])
+// WithDarcy: also the phases' reservoir-volume rates across the face,
+// darcyFlux.value() * faceArea at the phase's component index (the FLORES
+// values, blackoillocalresidualtpfa.hh:376-385)
+template <bool WithDarcy>
+__device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
+                                          double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3])
 {
     flux[0] = ec(0.0); flux[1] = ec(0.0); flux[2] = ec(0.0);
+    if constexpr (WithDarcy) { darcyOut[0] = 0.0; darcyOut[1] = 0.0; darcyOut[2] = 0.0; }
     const double tf = (-trans) / area;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -235,20 +240,29 @@ __device__ __forceinline__ void faceFlux(const BoFluid& f, const IqE& in, const 
         const bool upIn = dp.v > 0.0 ? false : (dp.v < 0.0 ? true : upInTie);
         E3 svf;
         E3 gas;
+        double darcyV;
         if (upIn) {
             const E3 darcy = (dp * in.lam[a]) * tf;
             svf = in.invB[a] * darcy;
             if (a == kO) gas = in.Rs * svf;
+            darcyV = darcy.v;
         } else {
             const E3 darcy = dp * (ex.lam[a] * tf);
             svf = darcy * ex.invB[a];
             if (a == kO) gas = svf * ex.Rs;
+            darcyV = darcy.v;
         }
         const int c = compOfPhase(a);
         flux[c] = flux[c] + svf * f.rho[a];
         if (a == kO) flux[kCG] = flux[kCG] + gas * f.rho[kG];
+        if constexpr (WithDarcy) darcyOut[c] = darcyV * area;
     }
     flux[0] = flux[0] * area; flux[1] = flux[1] * area; flux[2] = flux[2] * area;
+}
+__device__ __forceinline__ void faceFlux(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
+                                         double area, double dZg, bool upInTie, E3 flux[3])
+{
+    faceFluxT<false>(f, in, ex, trans, area, dZg, upInTie, flux, nullptr);
 }
 
 // Well source terms per unit volume (ReservoirProblem-style problem.source()).

@@ -275,6 +275,23 @@ int omodels_linearizer_source_blocks(omodels_model* h, int32_t* cells, double* r
     });
 }
 
+int omodels_linearizer_flows(omodels_model* h, double* flows, double* flores)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        const size_t nf = (size_t)(m.nnzb - m.N);
+        if (nf == 0 || (!flows && !flores)) return 0;
+        DevBuf<double> df, dr;
+        if (flows) df.alloc(3 * nf);
+        if (flores) dr.alloc(3 * nf);
+        m.computeFlows(df.p, dr.p);
+        if (flows) df.download(flows, 3 * nf, m.stream);
+        if (flores) dr.download(flores, 3 * nf, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
 int omodels_linearizer_add_sparse_sources(omodels_model* h, int32_t n, const int32_t* cells, const double* residual,
                                           const double* diag)
 {

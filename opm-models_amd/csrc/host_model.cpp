@@ -124,6 +124,14 @@ void Model::addSparseSources(int n, const int* cells, const double* r, const dou
     OM_HIP(hipGetLastError());
 }
 
+void Model::computeFlows(double* flows, double* flores)
+{
+    timer.begin(kClsOther, stream);
+    launchBoFlows(fluid, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream);
+    timer.end(stream);
+    OM_HIP(hipGetLastError());
+}
+
 void Model::addBuiltinSparseSources()
 {
     if (nsrc <= 0) return;

@@ -92,6 +92,9 @@ public:
     void addSparseSources(int n, const int* cells, const double* r, const double* d);
     // the built-in source terms through the sparse path (the built-in well model)
     void addBuiltinSparseSources();
+    // updateFlowsInfo (tpfalinearizer.hh:647-705) at solution(0): device arrays
+    // [nnzb - N][3] in the pattern's neighbour order, either may be nullptr
+    void computeFlows(double* flows, double* flores);
     void ensureJacobian();
     void eraseMatrix() { jac.release(); }
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)

@@ -147,6 +147,8 @@ void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* 
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
                           double* outRes, double* outDiag, hipStream_t s);
+void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
+                   const int* rowPtr, double* flows, double* flores, hipStream_t s);
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
                             double* jac, const int* diagOff, hipStream_t s);
 void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,

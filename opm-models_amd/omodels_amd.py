@@ -104,6 +104,7 @@ SIGNATURES = {
     "omodels_linearizer_set_separate_sparse_sources": (C.c_int, [C.c_void_p, C.c_int]),
     "omodels_linearizer_source_blocks": (C.c_int, [C.c_void_p, _ip, _dp, _dp]),
     "omodels_linearizer_add_sparse_sources": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
+    "omodels_linearizer_flows": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_solver_create": (C.c_void_p, [C.c_void_p, C.POINTER(Params)]),
     "omodels_solver_destroy": (None, [C.c_void_p]),
     "omodels_solver_prepare": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -353,6 +354,14 @@ class Model:
         diag = np.ascontiguousarray(diag, dtype=np.float64)
         check(lib().omodels_linearizer_add_sparse_sources(self.h, len(cells), ptr(cells, _ip), ptr(residual), ptr(diag)),
               "add_sparse_sources")
+
+    def flows(self):
+        """(flows, flores), each [nnzb - N][3] in the pattern's neighbour order, at solution(0)."""
+        nf = self.nnzb - self.N
+        fl = np.zeros((nf, 3))
+        fr = np.zeros((nf, 3))
+        check(lib().omodels_linearizer_flows(self.h, ptr(fl), ptr(fr)), "flows")
+        return fl, fr
 
     def set_profiling(self, enable=True):
         check(lib().omodels_model_set_profiling(self.h, int(enable)), "set_profiling")

@@ -273,10 +273,13 @@ void storageOf(const Pvt& pvt, const IQ<T>& q, T st[3])
  * pressure difference of transfluxmodule.hh:254-303 and evalPhaseFluxes_
  * (:752-811), followed by the caller's multiplication with the face area
  * (tpfalinearizer.hh:758). */
+/* darcyOut (optional): each phase's darcyFlux.value() * faceArea at its
+ * component index, the FLORES values (blackoillocalresidualtpfa.hh:376-385) */
 void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, long J,
-              const NbInfo& nbi, double Vin, double Vex, E3 flux[3])
+              const NbInfo& nbi, double Vin, double Vex, E3 flux[3], double* darcyOut = nullptr)
 {
     for (int e = 0; e < 3; ++e) flux[e] = E3(0.0);
+    if (darcyOut) for (int e = 0; e < 3; ++e) darcyOut[e] = 0.0;
     for (int a = 0; a < 3; ++a) {
         if (in.lam[a].v <= 0.0 && ex.lam[a] <= 0.0) continue;
         E3 rhoAvg = (in.rho[a] + ex.rho[a]) * 0.5;
@@ -302,6 +305,7 @@ void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, lo
             E3 gs = upIn ? (in.Rs * svf) : (svf * ex.Rs);
             flux[CG] = flux[CG] + gs * pvt.rho[G];
         }
+        if (darcyOut) darcyOut[compOfPhase[a]] = darcy.v * nbi.area;
     }
     for (int e = 0; e < 3; ++e) flux[e] = flux[e] * nbi.area;
 }
@@ -859,6 +863,34 @@ int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meani
             }
         }
         return p->n_src;
+    } catch (...) { return -1; }
+}
+
+/* TpfaLinearizer::updateFlowsInfo (tpfalinearizer.hh:647-705): per cell and
+ * neighbour in intersection order, the values of the face's component fluxes
+ * (FLOWS) and the phases' reservoir-volume rates (FLORES), [entries][3]. */
+int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, double* flows, double* flores)
+{
+    try {
+        Problem pr(p);
+        const double V = pr.G.volume();
+        for (long I = 0; I < pr.G.N; ++I) {
+            IQ<E3> qi;
+            computeIQ<E3>(pr.pvt, pv + 3 * I, meaning[I], p->poro[I], qi);
+            for (int k = pr.pat.nbPtr[I]; k < pr.pat.nbPtr[I + 1]; ++k) {
+                const NbInfo& nbi = pr.pat.nb[k];
+                IQ<double> qj;
+                computeIQ<double>(pr.pvt, pv + 3 * nbi.J, meaning[nbi.J], p->poro[nbi.J], qj);
+                E3 f[3];
+                double d[3];
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, d);
+                for (int e = 0; e < 3; ++e) {
+                    if (flows) flows[3L * k + e] = f[e].v;
+                    if (flores) flores[3L * k + e] = d[e];
+                }
+            }
+        }
+        return 0;
     } catch (...) { return -1; }
 }
 

@@ -92,6 +92,8 @@ int ob_storage(const ob_problem* p, const double* pv, const uint8_t* meaning,
 int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  const double* storage1, double dt, double* residual /* [N][3] */,
                  double* jac /* [NNZB][3][3] */, int nthreads);
+/* face rates (FLOWS, FLORES) per neighbour entry of the pattern, [entries][3] */
+int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, double* flows, double* flores);
 /* source terms as sparse blocks: cells [n_src], residual [n_src][3] (times -V),
  * derivatives [n_src][3][3]; returns n_src */
 int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cells,

@@ -74,6 +74,7 @@ def lib():
         L.ob_storage.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp]
         L.ob_linearize.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, _dp, _dp, C.c_int]
         L.ob_source_blocks.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _ip, _dp, _dp]
+        L.ob_flows.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp]
         L.ob_intensive.argtypes = [C.POINTER(ObProblem), _dp, C.c_int, C.c_int, _dp]
         L.ob_ilu0_factor.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp]
         L.ob_ilu0_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
@@ -163,6 +164,19 @@ def source_blocks(prob, pv, mean):
     got = lib().ob_source_blocks(C.byref(prob), P(pv), P(mean, _u8p), P(cells, _ip), P(r), P(d))
     assert got == n
     return cells[:n], r[:3 * n], d[:9 * n]
+
+
+def flows(prob, pv, mean):
+    """(flows, flores), [neighbour entries][3] in the pattern's order."""
+    nz = C.c_int64()
+    f2 = C.c_int64()
+    lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
+    N = prob.nx * prob.ny * prob.nz
+    nf = nz.value - N
+    fl = np.zeros((nf, 3))
+    fr = np.zeros((nf, 3))
+    assert lib().ob_flows(C.byref(prob), P(pv), P(mean, _u8p), P(fl), P(fr)) == 0
+    return fl, fr
 
 
 def intensive(prob, pv3, meaning, cell=0):

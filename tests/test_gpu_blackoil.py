@@ -404,3 +404,18 @@ def test_simulate_with_separate_sparse_sources_is_identical():
     assert rc0 == 0 and rc1 == 0
     assert l0 == l1
     assert np.array_equal(_bits(h0), _bits(h1)) and np.array_equal(_bits(d0), _bits(d1))
+
+
+# ----------------------------------------------------------------------------
+# Face rates for output: updateFlowsInfo (tpfalinearizer.hh:647-705)
+@pytest.mark.parametrize("dims,seed", [((1, 1, 1), 1), ((3, 2, 1), 2), ((7, 5, 6), 3), ((40, 40, 10), 4)])
+def test_flows_and_flores_parity(dims, seed):
+    c = case(*dims, seed=seed)
+    pv, mean = mixed_state(c, seed=seed)
+    m = om.Model(c)
+    m.set_solution(pv, mean)
+    fl, fr = m.flows()
+    flo, fro = orc.flows(orc.problem_from_case(c), pv, mean)
+    assert fl.shape == flo.shape == (m.nnzb - m.N, 3)
+    assert np.array_equal(_bits(fl), _bits(flo))
+    assert np.array_equal(_bits(fr), _bits(fro))

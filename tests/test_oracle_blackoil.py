@@ -294,6 +294,58 @@ def test_source_blocks_closed_form_and_derivatives():
 
 
 # ----------------------------------------------------------------------------
+# Face rates for output (FLOWS / FLORES)
+def test_flows_sum_to_the_residual_in_order():
+    """Without sources or accumulation the residual of a cell is the sum of its
+    face fluxes in intersection order: the FLOWS are exactly those terms."""
+    c = _case_nowells(5, 4, 3, seed=5)
+    pv, mean = mixed_state(c, seed=5)
+    prob = orc.problem_from_case(c)
+    J, r = orc.linearize(prob, pv, mean, orc.storage(prob, pv, mean), 1e300)
+    fl, _ = orc.flows(prob, pv, mean)
+    nbp = orc.pattern(prob)["nb_ptr"]
+    acc = np.zeros(3 * c.N)
+    for I in range(c.N):
+        a = np.zeros(3)
+        for k in range(nbp[I], nbp[I + 1]):
+            a = a + fl[k]
+        acc[3 * I:3 * I + 3] = a
+    assert np.array_equal(acc.view(np.uint64), r.view(np.uint64))
+
+
+def test_flores_closed_form_and_antisymmetry():
+    c = case(4, 3, 3, seed=9)
+    pv, mean = mixed_state(c, seed=9)
+    prob = orc.problem_from_case(c)
+    fl, fr = orc.flows(prob, pv, mean)
+    pat = orc.pattern(prob)
+    nbp, nbc = pat["nb_ptr"], pat["nb_cell"]
+    q = [orc.intensive(prob, pv[3 * I:3 * I + 3], mean[I], cell=I) for I in range(c.N)]
+    comp = {"w": 1, "o": 0, "g": 2}
+    checked = 0
+    for I in range(c.N):
+        for k in range(nbp[I], nbp[I + 1]):
+            Jc = nbc[k]
+            T, A, dzg = pat["nb_trans"][k], pat["nb_area"][k], pat["nb_dzg"][k]
+            for ph in "wog":
+                lamI, lamJ = q[I]["lam_" + ph][0], q[Jc]["lam_" + ph][0]
+                want = 0.0
+                if lamI > 0 or lamJ > 0:
+                    rho = 0.5 * (q[I]["rho_" + ph][0] + q[Jc]["rho_" + ph][0])
+                    dp = rho * dzg + q[Jc]["p"][0] - q[I]["p"][0]
+                    up = lamJ if dp > 0 else lamI
+                    want = dp * up * (-T / A) * A
+                assert fr[k, comp[ph]] == pytest.approx(want, rel=1e-12, abs=1e-300)
+                checked += want != 0.0
+            # the same face seen from the neighbour carries the opposite rates
+            kb = [kk for kk in range(nbp[Jc], nbp[Jc + 1]) if nbc[kk] == I][0]
+            for arr in (fl, fr):
+                scale = np.abs(arr[k]).max() + 1e-300
+                assert np.abs(arr[k] + arr[kb]).max() <= 1e-10 * scale
+    assert checked > 0
+
+
+# ----------------------------------------------------------------------------
 # A11-A12: linear algebra
 def _system(dims, seed=1):
     c = case(*dims, seed=seed)
