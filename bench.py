@@ -317,6 +317,15 @@ def main():
         fb = N * (24 + 1 + 24 + 4) + 2 * (nnzb - N) * 24
         line["flows"] = {"avg_us": round(favg * 1e3, 2), "bytes_per_launch": fb,
                          "achieved_GBs": round(fb / (favg * 1e-3) / 1e9, 1)}
+    # the output quantities of the VTK files (18 fields per cell)
+    for _ in range(8):
+        model.output_fields()
+    ms2, n2 = model.kernel_times()["other"]
+    if n2 > n1:
+        oavg = (ms2 - ms1) / (n2 - n1)
+        ob = N * (24 + 1 + 8) + 18 * N * 8
+        line["output_fields"] = {"avg_us": round(oavg * 1e3, 2), "bytes_per_launch": ob,
+                                 "achieved_GBs": round(ob / (oavg * 1e-3) / 1e9, 1)}
     if not args.no_extra:
         line["beside"] = beside_runs(om, args.config, dt)
     if not args.no_cpu:

@@ -185,6 +185,37 @@ int omodels_linearizer_source_blocks(omodels_model* m, int32_t* cells, double* r
  * its position among the cell's existing neighbours -x, +x, -y, +y, -z, +z
  * (the reference's dirId 0..5, ecfvstencil.hh:209-218). */
 int omodels_linearizer_flows(omodels_model* m, double* flows, double* flores);
+
+/* ------------------------------------------------------------------------ */
+/* Output and restart files. */
+/* Output quantities of solution(0), [count][N] field after field, named as
+ * the reference's VTK modules name them: pressure_, saturation_, density_ and
+ * mobility_<water|oil|gas> and porosity (VtkMultiPhaseModule,
+ * vtkmultiphasemodule.hh:306-323), R_s, B_w, B_o, B_g and "primary vars
+ * meaning gas" (VtkBlackOilModule, vtkblackoilmodule.hh:276-301). */
+int omodels_output_field_count(void);
+const char* omodels_output_field_name(int32_t i);   /* NULL out of range */
+int omodels_output_fields(omodels_model* m, double* out);
+/* VTK unstructured-grid file (.vtu, ASCII) of cell fields [nfields][N] on the
+ * hexahedra of an nx x ny x nz grid with spacings dx, dy, dz (the files
+ * VtkMultiWriter writes, vtkmultiwriter.hh); host only. */
+int omodels_vtk_write(const char* path, int32_t nx, int32_t ny, int32_t nz, double dx, double dy, double dz,
+                      int32_t nfields, const char* const* names, const double* fields);
+/* the output quantities of solution(0) as a .vtu file */
+int omodels_model_write_vtk(omodels_model* m, const char* path);
+/* Restart files in the reference's text format (Restart, restart.hh:40-280;
+ * Simulator::serialize, simulator.hh:928-940; BlackOilModel::serializeEntity,
+ * blackoilmodel.hh:468-492): grid cookie, simulator section (time and time
+ * step index), one line per cell with the primaries, the gas / water /
+ * pressure meanings and the PVT region; host only.  Reading checks the grid
+ * cookie and every section and rejects meanings this model does not have. */
+int omodels_restart_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* pv,
+                          const uint8_t* meaning, double time, int32_t time_step_idx);
+int omodels_restart_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* pv, uint8_t* meaning,
+                         double* time, int32_t* time_step_idx);
+/* solution(0) to a restart file; from one into solution(0) and solution(1) */
+int omodels_model_write_restart(omodels_model* m, const char* path, double time, int32_t time_step_idx);
+int omodels_model_read_restart(omodels_model* m, const char* path, double* time, int32_t* time_step_idx);
 /* addReservoirSourceTerms(residual, diagMatAddress) (tpfalinearizer.hh:843-846):
  * residual[cell] += r and A[cell][cell] += d for every entry, in the given
  * order (a cell may appear more than once). */

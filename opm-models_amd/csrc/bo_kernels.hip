@@ -311,6 +311,38 @@ __global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, c
     if (k + 1 < G.nz) face(I + nxy, trans[2 * N + I], G.area[2], G.dz * ((double)(k + 1) + 0.5), true);
 }
 
+// Output quantities of the current solution, one field after the other
+// ([kOutFields][N]): the VtkMultiPhaseModule fields pressure_, saturation_,
+// density_ and mobility_<phase> and porosity (vtkmultiphasemodule.hh:306-323),
+// then VtkBlackOilModule's R_s, B_w, B_o, B_g and the gas meaning
+// (vtkblackoilmodule.hh:276-301).  Phases in the order water, oil, gas.
+__global__ void k_bo_output(BoFluid f, long N, const double* __restrict__ pv, const uint8_t* __restrict__ mean,
+                            const double* __restrict__ poro, double* __restrict__ out)
+{
+    const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= N) return;
+    IqE q;
+    iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], q);
+    const int ph[3] = {kW, kO, kG};
+    for (int a = 0; a < 3; ++a) {
+        out[(0 + a) * N + I] = q.p.v;                  // capillary pressure is zero
+        out[(3 + a) * N + I] = q.S[ph[a]].v;
+        out[(6 + a) * N + I] = q.rho[ph[a]].v;
+        out[(9 + a) * N + I] = q.lam[ph[a]].v;
+    }
+    out[12 * N + I] = poro[I];
+    out[13 * N + I] = q.Rs.v;
+    for (int a = 0; a < 3; ++a) out[(14 + a) * N + I] = 1.0 / q.invB[ph[a]].v;
+    out[17 * N + I] = (double)mean[I];
+}
+
+void launchBoOutput(const BoFluid& f, long N, const double* pv, const uint8_t* mean, const double* poro,
+                    double* out, hipStream_t s)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_bo_output, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, f, N, pv, mean, poro, out);
+}
+
 // Storage of the current solution: the cache of time level 1 at the first
 // Newton iteration (recycleFirstIterationStorage, tpfalinearizer.hh:786-812).
 __global__ void k_bo_storage(BoFluid f, long N, const double* __restrict__ pv,

@@ -11,6 +11,7 @@
 #include "ensemble_host.h"
 #include "lens_host.h"
 #include "omodels_host.h"
+#include "omodels_io.h"
 
 using namespace omodels;
 
@@ -288,6 +289,121 @@ int omodels_linearizer_flows(omodels_model* h, double* flows, double* flores)
         if (flows) df.download(flows, 3 * nf, m.stream);
         if (flores) dr.download(flores, 3 * nf, m.stream);
         m.sync();
+        return 0;
+    });
+}
+
+// ---- output and restart files ----
+namespace {
+const char* const kOutNames[kOutFields] = {
+    "pressure_water", "pressure_oil", "pressure_gas", "saturation_water", "saturation_oil", "saturation_gas",
+    "density_water", "density_oil", "density_gas", "mobility_water", "mobility_oil", "mobility_gas",
+    "porosity", "R_s", "B_w", "B_o", "B_g", "primary vars meaning gas"};
+
+void checkGrid(const char* path, int32_t nx, int32_t ny, int32_t nz)
+{
+    if (!path) throw std::invalid_argument("file path must not be NULL");
+    if (nx <= 0 || ny <= 0 || nz <= 0) throw std::invalid_argument("grid dimensions must be positive");
+}
+}  // namespace
+
+int omodels_output_field_count(void) { return kOutFields; }
+
+const char* omodels_output_field_name(int32_t i) { return (i >= 0 && i < kOutFields) ? kOutNames[i] : nullptr; }
+
+int omodels_output_fields(omodels_model* h, double* out)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (!out) throw std::invalid_argument("output array must not be NULL");
+        DevBuf<double> d;
+        d.alloc((size_t)kOutFields * m.N);
+        m.timer.begin(kClsOther, m.stream);
+        launchBoOutput(m.fluid, m.N, m.pv0.p, m.mean0.p, m.poro.p, d.p, m.stream);
+        m.timer.end(m.stream);
+        OM_HIP(hipGetLastError());
+        d.download(out, (size_t)kOutFields * m.N, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_restart_write(const char* path, int32_t nx, int32_t ny, int32_t nz, const double* pv,
+                          const uint8_t* meaning, double time, int32_t time_step_idx)
+{
+    return guarded([&] {
+        checkGrid(path, nx, ny, nz);
+        if (!pv || !meaning) throw std::invalid_argument("solution arrays must not be NULL");
+        writeRestart(path, nx, ny, nz, pv, meaning, time, time_step_idx);
+        return 0;
+    });
+}
+
+int omodels_restart_read(const char* path, int32_t nx, int32_t ny, int32_t nz, double* pv, uint8_t* meaning,
+                         double* time, int32_t* time_step_idx)
+{
+    return guarded([&] {
+        checkGrid(path, nx, ny, nz);
+        if (!pv || !meaning) throw std::invalid_argument("solution arrays must not be NULL");
+        int step = 0;
+        readRestart(path, nx, ny, nz, pv, meaning, time, &step);
+        if (time_step_idx) *time_step_idx = step;
+        return 0;
+    });
+}
+
+int omodels_vtk_write(const char* path, int32_t nx, int32_t ny, int32_t nz, double dx, double dy, double dz,
+                      int32_t nfields, const char* const* names, const double* fields)
+{
+    return guarded([&] {
+        checkGrid(path, nx, ny, nz);
+        if (nfields < 0 || (nfields > 0 && (!names || !fields))) throw std::invalid_argument("bad field arrays");
+        writeVtu(path, nx, ny, nz, dx, dy, dz, nfields, names, fields);
+        return 0;
+    });
+}
+
+int omodels_model_write_restart(omodels_model* h, const char* path, double time, int32_t time_step_idx)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        checkGrid(path, m.G.nx, m.G.ny, m.G.nz);
+        std::vector<double> pv(3 * (size_t)m.N);
+        std::vector<uint8_t> mean(m.N);
+        m.pv0.download(pv.data(), pv.size(), m.stream);
+        m.mean0.download(mean.data(), mean.size(), m.stream);
+        m.sync();
+        writeRestart(path, m.G.nx, m.G.ny, m.G.nz, pv.data(), mean.data(), time, time_step_idx);
+        return 0;
+    });
+}
+
+int omodels_model_read_restart(omodels_model* h, const char* path, double* time, int32_t* time_step_idx)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        checkGrid(path, m.G.nx, m.G.ny, m.G.nz);
+        std::vector<double> pv(3 * (size_t)m.N);
+        std::vector<uint8_t> mean(m.N);
+        int step = 0;
+        readRestart(path, m.G.nx, m.G.ny, m.G.nz, pv.data(), mean.data(), time, &step);
+        if (time_step_idx) *time_step_idx = step;
+        m.pv0.upload(pv.data(), pv.size(), m.stream);
+        m.mean0.upload(mean.data(), mean.size(), m.stream);
+        m.advanceTimeLevel();
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_model_write_vtk(omodels_model* h, const char* path)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        checkGrid(path, m.G.nx, m.G.ny, m.G.nz);
+        std::vector<double> out((size_t)kOutFields * m.N);
+        if (int rc = omodels_output_fields(h, out.data())) return rc;
+        writeVtu(path, m.G.nx, m.G.ny, m.G.nz, m.G.dx, m.G.dy, m.G.dz, kOutFields, kOutNames, out.data());
         return 0;
     });
 }

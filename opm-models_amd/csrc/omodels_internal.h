@@ -149,6 +149,9 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
                           double* outRes, double* outDiag, hipStream_t s);
 void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s);
+constexpr int kOutFields = 18;
+void launchBoOutput(const BoFluid& f, long N, const double* pv, const uint8_t* mean, const double* poro,
+                    double* out, hipStream_t s);
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
                             double* jac, const int* diagOff, hipStream_t s);
 void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,

@@ -105,6 +105,19 @@ SIGNATURES = {
     "omodels_linearizer_source_blocks": (C.c_int, [C.c_void_p, _ip, _dp, _dp]),
     "omodels_linearizer_add_sparse_sources": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
     "omodels_linearizer_flows": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_output_field_count": (C.c_int, []),
+    "omodels_output_field_name": (C.c_char_p, [C.c_int32]),
+    "omodels_output_fields": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_vtk_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                    C.c_int32, C.POINTER(C.c_char_p), _dp]),
+    "omodels_model_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "omodels_restart_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _dp, _u8p, C.c_double,
+                                        C.c_int32]),
+    "omodels_restart_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _dp, _u8p,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "omodels_model_write_restart": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_int32]),
+    "omodels_model_read_restart": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int32)]),
     "omodels_solver_create": (C.c_void_p, [C.c_void_p, C.POINTER(Params)]),
     "omodels_solver_destroy": (None, [C.c_void_p]),
     "omodels_solver_prepare": (C.c_int, [C.c_void_p, _dp, _dp]),
@@ -168,6 +181,47 @@ def ptr(a, t=_dp):
     if a is None:
         return None
     return a.ctypes.data_as(t)
+
+
+def output_field_names():
+    return [lib().omodels_output_field_name(i).decode() for i in range(lib().omodels_output_field_count())]
+
+
+def restart_file_name(output_dir, sim_name, t):
+    """<dir>/<name>_time=<t>_rank=0.ers, the reference's restartFileName_ (restart.hh:80-95)."""
+    d = "" if output_dir in (".", "") else output_dir.rstrip("/") + "/"
+    return "%s%s_time=%s_rank=0.ers" % (d, sim_name, "%g" % t)
+
+
+def write_restart(path, dims, pv, meaning, time, time_step_idx=0):
+    nx, ny, nz = dims
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    meaning = np.ascontiguousarray(meaning, dtype=np.uint8)
+    check(lib().omodels_restart_write(os.fsencode(path), nx, ny, nz, ptr(pv), ptr(meaning, _u8p), float(time),
+                                      int(time_step_idx)), "restart_write")
+
+
+def read_restart(path, dims):
+    """(pv [3N], meaning [N], time, time_step_idx)"""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    pv = np.zeros(3 * n)
+    meaning = np.zeros(n, np.uint8)
+    t = C.c_double()
+    k = C.c_int32()
+    check(lib().omodels_restart_read(os.fsencode(path), nx, ny, nz, ptr(pv), ptr(meaning, _u8p), C.byref(t),
+                                     C.byref(k)), "restart_read")
+    return pv, meaning, t.value, k.value
+
+
+def write_vtk(path, dims, spacing, fields):
+    """fields: {name: [N] array} written as cell data of a .vtu file."""
+    nx, ny, nz = dims
+    names = list(fields)
+    arr = np.ascontiguousarray(np.stack([np.asarray(fields[k], dtype=np.float64) for k in names]))
+    cnames = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    check(lib().omodels_vtk_write(os.fsencode(path), nx, ny, nz, *map(float, spacing), len(names), cnames, ptr(arr)),
+          "vtk_write")
 
 
 class OmodelsError(RuntimeError):
@@ -354,6 +408,27 @@ class Model:
         diag = np.ascontiguousarray(diag, dtype=np.float64)
         check(lib().omodels_linearizer_add_sparse_sources(self.h, len(cells), ptr(cells, _ip), ptr(residual), ptr(diag)),
               "add_sparse_sources")
+
+    def output_fields(self):
+        """{name: [N] array} of the output quantities of solution(0)."""
+        names = output_field_names()
+        out = np.zeros((len(names), self.N))
+        check(lib().omodels_output_fields(self.h, ptr(out)), "output_fields")
+        return {n: out[i] for i, n in enumerate(names)}
+
+    def write_vtk(self, path):
+        check(lib().omodels_model_write_vtk(self.h, os.fsencode(path)), "write_vtk")
+
+    def write_restart(self, path, time, time_step_idx=0):
+        check(lib().omodels_model_write_restart(self.h, os.fsencode(path), float(time), int(time_step_idx)),
+              "write_restart")
+
+    def read_restart(self, path):
+        """Loads solution(0) and solution(1); returns (time, time_step_idx)."""
+        t = C.c_double()
+        k = C.c_int32()
+        check(lib().omodels_model_read_restart(self.h, os.fsencode(path), C.byref(t), C.byref(k)), "read_restart")
+        return t.value, k.value
 
     def flows(self):
         """(flows, flores), each [nnzb - N][3] in the pattern's neighbour order, at solution(0)."""

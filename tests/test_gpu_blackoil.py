@@ -419,3 +419,70 @@ def test_flows_and_flores_parity(dims, seed):
     assert fl.shape == flo.shape == (m.nnzb - m.N, 3)
     assert np.array_equal(_bits(fl), _bits(flo))
     assert np.array_equal(_bits(fr), _bits(fro))
+
+
+# ----------------------------------------------------------------------------
+# Output quantities, VTK and restart files of a model
+def test_output_fields_parity():
+    c = case(7, 5, 6, seed=12)
+    pv, mean = mixed_state(c, seed=12)
+    m = om.Model(c)
+    m.set_solution(pv, mean)
+    out = m.output_fields()
+    assert list(out) == om.output_field_names()
+    prob = orc.problem_from_case(c)
+    exp = {k: np.zeros(c.N) for k in out}
+    for I in range(c.N):
+        q = orc.intensive(prob, pv[3 * I:3 * I + 3], mean[I], cell=I)
+        sw, so = pv[3 * I], q["So"][0]
+        sg = pv[3 * I + 2] if mean[I] == 0 else 0.0
+        for ph, s in (("water", sw), ("oil", so), ("gas", sg)):
+            exp["pressure_" + ph][I] = q["p"][0]
+            exp["saturation_" + ph][I] = s
+            exp["density_" + ph][I] = q["rho_" + ph[0]][0]
+            exp["mobility_" + ph][I] = q["lam_" + ph[0]][0]
+            exp["B_" + ph[0]][I] = 1.0 / q["invB_" + ph[0]][0]
+        exp["porosity"][I] = q["phi"][0]
+        exp["R_s"][I] = q["Rs"][0]
+        exp["primary vars meaning gas"][I] = mean[I]
+    for k in out:
+        assert np.array_equal(_bits(out[k]), _bits(exp[k])), k
+
+
+def test_model_vtu_holds_the_output_fields(tmp_path):
+    import xml.etree.ElementTree as ET
+    c = case(5, 4, 3, seed=13)
+    pv, mean = mixed_state(c, seed=13)
+    m = om.Model(c)
+    m.set_solution(pv, mean)
+    path = str(tmp_path / "state.vtu")
+    m.write_vtk(path)
+    piece = ET.parse(path).getroot().find("UnstructuredGrid/Piece")
+    arrays = {a.get("Name"): np.array(a.text.split(), dtype=np.float64) for a in piece.find("CellData")}
+    out = m.output_fields()
+    assert list(arrays) == list(out)
+    for k in out:
+        assert np.array_equal(_bits(arrays[k]), _bits(out[k])), k
+
+
+def test_restart_continues_the_run_exactly(tmp_path):
+    """5 one-day steps in one run == 3 steps, restart file, new model, 2 steps."""
+    c = case(12, 10, 4, seed=14)
+    prm = om.default_params(max_dt=86400.0, min_dt=86400.0)
+    whole = om.Model(c)
+    whole.set_solution(c.pv0, c.mean0)
+    rc, hist, _, log, _ = whole.simulate(prm, 5, 86400.0)
+    assert rc == 0
+    first = om.Model(c)
+    first.set_solution(c.pv0, c.mean0)
+    rc, h1, _, l1, _ = first.simulate(prm, 3, 86400.0)
+    assert rc == 0
+    path = str(tmp_path / om.restart_file_name(".", "bo", 3 * 86400.0))
+    first.write_restart(path, 3 * 86400.0, 3)
+    second = om.Model(c)
+    t, k = second.read_restart(path)
+    assert (t, k) == (3 * 86400.0, 3)
+    rc, h2, _, l2, _ = second.simulate(prm, 2, 86400.0)
+    assert rc == 0
+    assert np.array_equal(_bits(np.vstack([h1, h2])), _bits(hist))
+    assert [r[2:] for r in l1 + l2] == [r[2:] for r in log]
