@@ -71,6 +71,8 @@ def algorithmic_bytes(N, F, nnzb, n=3, s=8):
         "spmv": nnzb * (n * n * s + 4) + (N + 1) * 4 + 2 * N * n * s,
         "ilu_apply": nnzb * (n * n * s + 4) + (N + 1) * 4 + 3 * N * n * s,
         "ilu_factor": 2 * nnzb * n * n * 8 + nnzb * 4 + (N + 1) * 4,
+        # preconditioner application with the operator application in its backward sweep
+        "ilu_apply_spmv": 2 * (nnzb * (n * n * s + 4) + (N + 1) * 4) + 5 * N * n * s,
     }
 
 
@@ -259,6 +261,11 @@ def main():
     if rc != 0:
         raise SystemExit("timed run failed rc=%d" % rc)
     kt = model.kernel_times()
+    # the operator application runs inside the backward sweep of the tiled ILU
+    # (no launches of its own): the class then carries both kernels' bytes
+    fused = kt["spmv"][1] == 0 and kt["ilu_apply"][1] > 0
+    if fused:
+        kt = {("ilu_apply_spmv" if k == "ilu_apply" else k): v for k, v in kt.items() if k != "spmv"}
 
     newton = st.newton_iters
     value = newton / wall

@@ -121,7 +121,11 @@ typedef struct omodels_params {
                                    not depend on the choice */
     int32_t graph;              /* 1: each BiCGStab iteration is replayed from a captured hipGraph
                                    (one launch per iteration); results do not depend on it */
-    int32_t pad;
+    int32_t spmv_in_sweep;      /* 1: with the streaming tiled ILU of one reservoir, the operator
+                                   application after each preconditioner application runs inside
+                                   the backward sweep, tile by tile as the tiles it reads finish
+                                   (same results; no faster at 500k, where the sweeps have no idle
+                                   workgroups); 0 (default): as a kernel of its own */
 } omodels_params;
 
 void omodels_default_params(omodels_params* prm);

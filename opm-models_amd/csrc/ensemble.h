@@ -68,6 +68,18 @@ struct IluInst {
                           // arrays of S back to back (N = cells of the instance, 9 N < 2^31)
 };
 
+// The operator application that follows a preconditioner application, y = A v,
+// computed inside the backward sweep of one instance: tile T's rows once the
+// sweep has stored the results of T and of its -x, -y and -z neighbour tiles.
+struct SpmvFuse {
+    const int* rp;
+    const int* ci;
+    const void* a;              // S*, the operator's blocks
+    void* y;                    // S*, the product
+    int* tileDone;              // per tile: the launch's epoch once the tile's results are stored
+    unsigned int* epoch;        // advanced by the last workgroup leaving each launch
+};
+
 // Multi-instance tiled ILU: factor / sweeps over all tiles of all instances,
 // ordered by tile level so that dependencies hold smaller tickets.
 class TiledIluSet {
@@ -76,8 +88,10 @@ public:
     // (re)build the ticket order for instances with the given geometries
     void setInstances(const std::vector<TileGeom>& geoms, hipStream_t s);
     void factor(const std::vector<IluInst>& insts, int* fail, hipStream_t s);
+    // fuse: also y = A v in the backward sweep when the sweeps stream and there
+    // is one instance; returns whether it did
     template <class S>
-    void apply(const std::vector<IluInst>& insts, hipStream_t s);
+    bool apply(const std::vector<IluInst>& insts, hipStream_t s, const SpmvFuse* fuse = nullptr);
     void checkError(hipStream_t s);
     // make the device descriptor of these instances resident (before a stream
     // capture, which must not record the upload); generation() changes
@@ -96,6 +110,10 @@ private:
     const IluInst* upload(const std::vector<IluInst>& insts, hipStream_t s);
     void resetExchange(hipStream_t s);
     DevBuf<TileTask> order_;
+    DevBuf<TileTask> orderFused_;        // one instance: backward tiles and the operator's row blocks
+    DevBuf<int> tileDone_;
+    DevBuf<unsigned int> epochDev_;
+    int TF_ = 0;
     std::vector<long> cellOff_;          // first cell of each instance in the exchange arrays
     DevBuf<double> xch64_;               // [9 * cells]: three [N][3] slot arrays per instance
     DevBuf<float> xch32_;
@@ -108,6 +126,7 @@ private:
     int epoch_ = 0, T_ = 0, gridFactor_ = 0, residentFactor_ = 0;
     int residentSweep_[2][2] = {};   // streaming sweeps: [FP64, FP32][forward, backward]
     int residentBatch_[2][2] = {};   // batched sweeps
+    int residentFused_[2] = {};      // backward sweep with the operator application
     int mode_ = 0;
     bool autoStream_ = true;
 };

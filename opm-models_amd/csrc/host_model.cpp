@@ -366,13 +366,17 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
 }
 
 template <class S>
-void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run)
+bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run, const S* A, S* Av)
 {
     hipStream_t s = m_.stream;
     const int N = (int)m_.N;
+    bool fused = false;
     if (prm_.precond == 0) {
         m_.timer.begin(kClsIluApply, s);
-        if (tiled_)
+        if (tiled_ && A && prm_.spmv_in_sweep)
+            fused = tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s, run,
+                                             m_.colIdx.p, A, Av);
+        else if (tiled_)
             tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s, run);
         else
             launchIlu0Apply<S, double>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, lu, (const double*)dinv,
@@ -383,6 +387,7 @@ void BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
         // block Jacobi with FP64 inverted pivots: v = w D^-1 d
         launchJacobiApplyMixed(N, (const double*)dinv, d, v, prm_.prec_relax, s);
     }
+    return fused;
 }
 
 void BiCGStabBackend::IterGraphs::reset()
@@ -456,22 +461,27 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     // Every kernel of the solve returns at once after the test has ended it
     // (KrylovScalars::run), so the host queues iteration it+1 before it has
     // seen the test of iteration it, and the device never waits for the host.
+    bool fusedSpmv = false;     // the operator runs inside the preconditioner's backward sweep
     auto firstHalf = [&]() {
         launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, 0, s);
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
-        precApply<S>(lu, dinv, p, y, run);
-        m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s, run);
-        m_.timer.end(s);
+        fusedSpmv = precApply<S>(lu, dinv, p, y, run, A, v);
+        if (!fusedSpmv) {
+            m_.timer.begin(kClsSpmv, s);
+            launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s, run);
+            m_.timer.end(s);
+        }
         launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, 0, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
     };
     // second half: z = M^-1 r, y = A z, omega, x/r update, next tag base
     auto secondHalf = [&]() {
-        precApply<S>(lu, dinv, r, z, run);
-        m_.timer.begin(kClsSpmv, s);
-        launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s, run);
-        m_.timer.end(s);
+        fusedSpmv = precApply<S>(lu, dinv, r, z, run, A, y);
+        if (!fusedSpmv) {
+            m_.timer.begin(kClsSpmv, s);
+            launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s, run);
+            m_.timer.end(s);
+        }
         launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, 1, s);
         launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, 2, s);
     };
@@ -532,7 +542,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
             *its = it - K.stop_back;
             if (it + 1 < maxIter && !gr) {      // the queued iteration it+1 was skipped: untime it
                 if (prm_.precond == 0) m_.timer.dropLast(kClsIluApply, 2);
-                m_.timer.dropLast(kClsSpmv, 2);
+                if (!fusedSpmv) m_.timer.dropLast(kClsSpmv, 2);
             }
             return K.stop_rc;
         }

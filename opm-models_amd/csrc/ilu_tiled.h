@@ -46,10 +46,12 @@ public:
     TiledIlu& operator=(const TiledIlu&) = delete;
     // in-place factorization of lu (a copy of A); inverted pivots also row-wise in dinv
     void factor(const int* rp, const int* diag, double* lu, double* dinv, int* fail, hipStream_t s);
-    // v = (LU)^-1 d; active: nullptr, or a device flag, the application is skipped while it is 0
+    // v = (LU)^-1 d; active: nullptr, or a device flag, the application is skipped while it is 0.
+    // With A and y, also y = A v inside the backward sweep where the sweeps stream;
+    // returns whether it did (otherwise y is untouched)
     template <class S, class P>
-    void apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
-               const int* active = nullptr);
+    bool apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
+               const int* active = nullptr, const int* ci = nullptr, const S* A = nullptr, S* y = nullptr);
     // upload the descriptor of this application ahead of a stream capture
     template <class S, class P>
     void primeApply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,

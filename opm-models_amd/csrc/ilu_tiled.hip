@@ -287,12 +287,102 @@ __device__ __forceinline__ void ldsStore(S* p, S v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// One tile's rows of y = A v inside the backward sweep, once the sweep has
+// stored the results of the tile and of its six neighbours (their flags carry
+// this launch's epoch; a tile may finish before its +x, +y, +z neighbours have
+// stored all their results, since it only waits for their faces).  The v values of the tile and its
+// one-cell box around it are staged in LDS with device-scope loads (other
+// workgroups stored them), then lane u takes row u % 3 of the tile's cell
+// u / 3 with the blocks of the row in column order; per row the operation
+// order is that of k_spmv3, so the product is bit-identical to the separate
+// kernel's.
+template <class S>
+__device__ __forceinline__ void spmvTile(const IluInst& I, const SpmvFuse& fz, int tile, int ep, int* err, S* box)
+{
+    const TileGeom& g = I.g;
+    int TI, TJ, TK;
+    tileOf(g, tile, TI, TJ, TK);
+    if (threadIdx.x == 0) {
+        int deps[7];
+        int nd = 0;
+        deps[nd++] = tile;
+        if (TI > 0) deps[nd++] = tile - 1;
+        if (TJ > 0) deps[nd++] = tile - g.TX;
+        if (TK > 0) deps[nd++] = tile - g.TX * g.TY;
+        if (TI + 1 < g.TX) deps[nd++] = tile + 1;
+        if (TJ + 1 < g.TY) deps[nd++] = tile + g.TX;
+        if (TK + 1 < g.TZ) deps[nd++] = tile + g.TX * g.TY;
+        for (int q = 0; q < nd && !loadRelaxed(err); ++q) {
+            long spins = 0;
+            while (loadRelaxed(fz.tileDone + deps[q]) != ep) {
+                if (++spins > kSpinLimit || ((spins & 1023) == 0 && loadRelaxed(err))) { atomicOr(err, 1); break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+    __syncthreads();
+    const int BX2 = g.BX + 2, BXY2 = (g.BX + 2) * (g.BY + 2);
+    const int nbox = BXY2 * (g.BZ + 2);
+    const long nxy = (long)g.nx * g.ny;
+    const S* x = (const S*)I.v;
+    for (int u = threadIdx.x; u < 3 * nbox; u += kSweepThreads) {
+        const int b = u / 3, q = u - 3 * b;
+        const int i = TI * g.BX + b % BX2 - 1, j = TJ * g.BY + (b / BX2) % (g.BY + 2) - 1,
+                  k = TK * g.BZ + b / BXY2 - 1;
+        if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz)
+            box[u] = ldCoh(x + 3 * ((long)i + (long)g.nx * j + nxy * k) + q);
+    }
+    __syncthreads();
+    const int u = threadIdx.x;
+    if (u >= 3 * g.BX * g.BY * g.BZ) return;
+    const int c = u / 3, e = u - 3 * c;
+    const int li = c % g.BX, lj = (c / g.BX) % g.BY, lk = c / (g.BX * g.BY);
+    const int i = TI * g.BX + li, j = TJ * g.BY + lj, k = TK * g.BZ + lk;
+    if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+    const long row = (long)i + (long)g.nx * ((long)j + (long)g.ny * k);
+    const int pos = (li + 1) + BX2 * (lj + 1) + BXY2 * (lk + 1);
+    const GS<int>* __restrict__ rp = toGlobal(fz.rp);
+    const GS<int>* __restrict__ ci = toGlobal(fz.ci);
+    const GS<S>* __restrict__ a = toGlobal((const S*)fz.a);
+    const int k0 = rp[row], nb = rp[row + 1] - k0;
+    S v[7][3];
+    int off[7];
+#pragma unroll
+    for (int jb = 0; jb < 7; ++jb) {
+        off[jb] = pos;
+        if (jb < nb) {
+            const long d = (long)ci[k0 + jb] - row;     // the 7-point neighbours only
+            off[jb] = pos + (d == -nxy ? -BXY2 : d == -g.nx ? -BX2 : d == -1 ? -1 : d == 0 ? 0
+                             : d == 1 ? 1 : d == g.nx ? BX2 : BXY2);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[jb][q] = a[9L * (k0 + jb) + 3 * e + q];
+        }
+    }
+    S acc = S(0);
+#pragma unroll
+    for (int jb = 0; jb < 7; ++jb) {
+        if (jb < nb) {
+            const S* xb = box + 3 * off[jb];
+            S t = v[jb][0] * xb[0];
+            t += v[jb][1] * xb[1];
+            t += v[jb][2] * xb[2];
+            acc += t;
+        }
+    }
+    toGlobal((S*)fz.y)[3 * row + e] = acc;
+}
+
 // Forward sweep (Bwd = false): v = L^-1 d (unit lower), lower blocks in column
 // order -z, -y, -x.  Backward sweep (Bwd = true): v = U^-1 v with the inverted
 // (FP64) pivots, upper blocks in column order +x, +y, +z.
-template <class S, bool Bwd>
+// Fused (backward, one instance): the ticket order also holds the operator's
+// row blocks (inst < 0), each after every tile it reads (its own, the +x, +y,
+// +z neighbours one backward level earlier and the -x, -y, -z neighbours one
+// level later), and every tile publishes a flag once its results are stored.
+template <class S, bool Bwd, bool Fused>
 __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* __restrict__ insts,
-    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits, int nInst)
+    const TileTask* __restrict__ order, int T, unsigned long long* counter, int* err, unsigned int* exits, int nInst,
+    SpmvFuse fz)
 {
     __shared__ S sV[(kPadMax + 1) * 3];       // the tile's cells and its ghost layer (padded layout),
                                               // then a zero triple read in place of absent neighbours
@@ -303,12 +393,22 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
     const int q = threadIdx.x & 3, e = q < 3 ? q : 2, pos = threadIdx.x >> 2;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (skipLaunch(insts, nInst, counter, exits)) return;
+    // the epoch only moves when the last workgroup leaves, after every one has read it
+    const int ep = Fused ? (int)__hip_atomic_load(fz.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     int t = shareTicket(threadIdx.x == 0 ? drawTicket(counter, 0) : 0, &sTicket);
     for (;;) {
         if (t < 0 || t >= T) break;
         int next = 0;
         if (threadIdx.x == 0) next = drawTicket(counter, 0);
-        const TileTask task = order[Bwd ? T - 1 - t : t];
+        const TileTask task = order[(Bwd && !Fused) ? T - 1 - t : t];
+        if constexpr (Fused) {
+            if (task.inst < 0) {
+                spmvTile<S>(insts[0], fz, task.tile, ep, err, sV);
+                __syncthreads();
+                t = shareTicket(next, &sTicket);
+                continue;
+            }
+        }
         const IluInst I = insts[task.inst];
         if (!isActive(I)) {
             t = shareTicket(next, &sTicket);
@@ -534,13 +634,30 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_set(const IluInst* 
                 if (lane == 0) __hip_atomic_fetch_add(sDone + lev, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             // the tile's own results leave after its levels, behind the face values
-            // the neighbour tiles are waiting for
-            if (row) v[3 * Ic + e] = res;
+            // the neighbour tiles are waiting for (device-scope when other
+            // workgroups of this launch read them)
+            if (row) {
+                if constexpr (Fused) stCoh((S*)I.v + 3 * Ic + e, res);
+                else v[3 * Ic + e] = res;
+            }
+        }
+        if constexpr (Fused) {
+            waitStores();
+            __syncthreads();
+            if (threadIdx.x == 0) stCoh(fz.tileDone + task.tile, ep);
         }
         __syncthreads();
         t = shareTicket(next, &sTicket);
     }
-    leaveLaunch(counter, exits);
+    if constexpr (Fused) {
+        if (threadIdx.x == 0 && atomicAdd(exits, 1u) == gridDim.x - 1) {
+            atomicExch(counter, 0ULL);
+            atomicAdd(fz.epoch, 1u);
+            atomicExch(exits, 0u);
+        }
+    } else {
+        leaveLaunch(counter, exits);
+    }
 }
 
 // Batched sweeps: the same lane layout without a receiving wave.  A tile waits
@@ -1022,10 +1139,12 @@ TiledIluSet::TiledIluSet(hipStream_t s)
         return perCU * prop.multiProcessorCount;
     };
     residentFactor_ = resident({(const void*)k_ilu_factor_set}, kSweepThreads);
-    residentSweep_[0][0] = resident({(const void*)k_ilu_sweep_set<double, false>}, kSweepThreads);
-    residentSweep_[0][1] = resident({(const void*)k_ilu_sweep_set<double, true>}, kSweepThreads);
-    residentSweep_[1][0] = resident({(const void*)k_ilu_sweep_set<float, false>}, kSweepThreads);
-    residentSweep_[1][1] = resident({(const void*)k_ilu_sweep_set<float, true>}, kSweepThreads);
+    residentSweep_[0][0] = resident({(const void*)k_ilu_sweep_set<double, false, false>}, kSweepThreads);
+    residentSweep_[0][1] = resident({(const void*)k_ilu_sweep_set<double, true, false>}, kSweepThreads);
+    residentSweep_[1][0] = resident({(const void*)k_ilu_sweep_set<float, false, false>}, kSweepThreads);
+    residentSweep_[1][1] = resident({(const void*)k_ilu_sweep_set<float, true, false>}, kSweepThreads);
+    residentFused_[0] = resident({(const void*)k_ilu_sweep_set<double, true, true>}, kSweepThreads);
+    residentFused_[1] = resident({(const void*)k_ilu_sweep_set<float, true, true>}, kSweepThreads);
     residentBatch_[0][0] = resident({(const void*)k_ilu_fwd_batch<double>}, kSweepThreads);
     residentBatch_[0][1] = resident({(const void*)k_ilu_bwd_batch<double>}, kSweepThreads);
     residentBatch_[1][0] = resident({(const void*)k_ilu_fwd_batch<float>}, kSweepThreads);
@@ -1074,6 +1193,36 @@ void TiledIluSet::setInstances(const std::vector<TileGeom>& geoms, hipStream_t s
     int levels = 1;
     for (const TileGeom& g : geoms) levels = std::max(levels, g.TX + g.TY + g.TZ - 2);
     autoStream_ = (double)T_ / levels < residentSweep_[0][0];
+    // one instance: the backward tickets with the operator's row blocks of
+    // each tile after the tiles it reads (its own and its six neighbours, at
+    // most one backward level later) and a few more levels of slack, so that
+    // a workgroup drawing one rarely waits
+    TF_ = 0;
+    if (geoms.size() == 1) {
+        const TileGeom& g = geoms[0];
+        const int L = g.TX + g.TY + g.TZ - 3;       // backward tile levels 0..L
+        std::vector<std::vector<int>> byLevel(L + 1);
+        for (int k = T_ - 1; k >= 0; --k) {         // the backward sweep's order within a level
+            int TI = ord[k].tile % g.TX, TJ = (ord[k].tile / g.TX) % g.TY, TK = ord[k].tile / (g.TX * g.TY);
+            byLevel[(g.TX - 1 - TI) + (g.TY - 1 - TJ) + (g.TZ - 1 - TK)].push_back(ord[k].tile);
+        }
+        constexpr int kSlack = 3;
+        std::vector<TileTask> f;
+        for (int l = 0; l <= L; ++l) {
+            for (int t : byLevel[l]) f.push_back(TileTask{0, t});
+            const int sl = l - 1 - kSlack;
+            if (sl >= 0)
+                for (int t : byLevel[sl]) f.push_back(TileTask{-1, t});
+        }
+        for (int sl = std::max(0, L - kSlack); sl <= L; ++sl)
+            for (int t : byLevel[sl]) f.push_back(TileTask{-1, t});
+        TF_ = (int)f.size();
+        orderFused_.upload(f.data(), f.size(), s);
+        tileDone_.alloc(T_);
+        OM_HIP(hipMemsetAsync(tileDone_.p, 0xFF, T_ * sizeof(int), s));
+        epochDev_.alloc(1);
+        OM_HIP(hipMemsetAsync(epochDev_.p, 0, sizeof(unsigned int), s));
+    }
     dcache_.clear();
     ++generation_;
 }
@@ -1155,28 +1304,49 @@ void TiledIluSet::factor(const std::vector<IluInst>& insts, int* fail, hipStream
 }
 
 template <class S>
-void TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s)
+bool TiledIluSet::apply(const std::vector<IluInst>& insts, hipStream_t s, const SpmvFuse* fuse)
 {
-    if (T_ == 0) return;
+    if (T_ == 0) return false;
     const IluInst* di = upload<S>(insts, s);
-    const TileTask* od = order_.p;
-    int T = T_;
     unsigned long long* ct = counter_.p;
     int* er = err_.p;
     const int prec = sizeof(S) == 4 ? 1 : 0;
     const bool stream = streaming();
-    const void* kernels[2] = {stream ? (const void*)k_ilu_sweep_set<S, false> : (const void*)k_ilu_fwd_batch<S>,
-                              stream ? (const void*)k_ilu_sweep_set<S, true> : (const void*)k_ilu_bwd_batch<S>};
+    // the operator's rows stage the tile's one-cell box of v in the sweep's LDS image
+    const bool fused = fuse && stream && insts.size() == 1 && TF_ > 0 &&
+                       (insts[0].g.BX + 2) * (insts[0].g.BY + 2) * (insts[0].g.BZ + 2) <= kPadMax + 1;
     unsigned int* ex = exits_.p;
+    int nInst = (int)insts.size();
     for (int sweep = 0; sweep < 2; ++sweep) {
-        const int grid = std::min(T_, stream ? residentSweep_[prec][sweep] : residentBatch_[prec][sweep]);
-        int nInst = (int)insts.size();
-        void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex, &nInst};
-        OM_HIP(hipLaunchKernel(kernels[sweep], dim3(grid), dim3(kSweepThreads), args, 0, s));
+        const TileTask* od = order_.p;
+        int T = T_;
+        if (stream) {
+            SpmvFuse fz{};
+            const void* k = sweep == 0 ? (const void*)k_ilu_sweep_set<S, false, false>
+                                       : (const void*)k_ilu_sweep_set<S, true, false>;
+            int grid = std::min(T_, residentSweep_[prec][sweep]);
+            if (sweep == 1 && fused) {
+                fz = *fuse;
+                fz.tileDone = tileDone_.p;
+                fz.epoch = epochDev_.p;
+                od = orderFused_.p;
+                T = TF_;
+                k = (const void*)k_ilu_sweep_set<S, true, true>;
+                grid = std::min(T_, residentFused_[prec]);
+            }
+            void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex, &nInst, &fz};
+            OM_HIP(hipLaunchKernel(k, dim3(grid), dim3(kSweepThreads), args, 0, s));
+        } else {
+            const void* k = sweep == 0 ? (const void*)k_ilu_fwd_batch<S> : (const void*)k_ilu_bwd_batch<S>;
+            const int grid = std::min(T_, residentBatch_[prec][sweep]);
+            void* args[] = {(void*)&di, (void*)&od, &T, &ct, &er, (void*)&ex, &nInst};
+            OM_HIP(hipLaunchKernel(k, dim3(grid), dim3(kSweepThreads), args, 0, s));
+        }
     }
+    return fused;
 }
-template void TiledIluSet::apply<double>(const std::vector<IluInst>&, hipStream_t);
-template void TiledIluSet::apply<float>(const std::vector<IluInst>&, hipStream_t);
+template bool TiledIluSet::apply<double>(const std::vector<IluInst>&, hipStream_t, const SpmvFuse*);
+template bool TiledIluSet::apply<float>(const std::vector<IluInst>&, hipStream_t, const SpmvFuse*);
 
 // ---------------------------------------------------------------------------
 namespace {
@@ -1230,14 +1400,20 @@ void TiledIlu::factor(const int* rp, const int* diag, double* lu, double* dinvOu
 }
 
 template <class S, class P>
-void TiledIlu::apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
-                     const int* active)
+bool TiledIlu::apply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
+                     const int* active, const int* ci, const S* A, S* y)
 {
     static_assert(sizeof(P) == 8, "pivots are kept in FP64");
     IluInst I{};
     I.g = g; I.rp = rp; I.diag = diag; I.lu = lu; I.dinv = (const double*)dinv; I.d = d; I.v = v;
     I.active = active;
-    set_->apply<S>({I}, s);
+    if (!A || !y || !ci) {
+        set_->apply<S>({I}, s);
+        return false;
+    }
+    SpmvFuse fz{};
+    fz.rp = rp; fz.ci = ci; fz.a = A; fz.y = y;
+    return set_->apply<S>({I}, s, &fz);
 }
 template <class S, class P>
 void TiledIlu::primeApply(const int* rp, const int* diag, const S* lu, const P* dinv, const S* d, S* v, hipStream_t s,
@@ -1255,9 +1431,9 @@ template void TiledIlu::primeApply<float, double>(const int*, const int*, const 
 
 long TiledIlu::generation() const { return set_->generation(); }
 
-template void TiledIlu::apply<double, double>(const int*, const int*, const double*, const double*, const double*,
-                                              double*, hipStream_t, const int*);
-template void TiledIlu::apply<float, double>(const int*, const int*, const float*, const double*, const float*,
-                                             float*, hipStream_t, const int*);
+template bool TiledIlu::apply<double, double>(const int*, const int*, const double*, const double*, const double*,
+                                              double*, hipStream_t, const int*, const int*, const double*, double*);
+template bool TiledIlu::apply<float, double>(const int*, const int*, const float*, const double*, const float*,
+                                             float*, hipStream_t, const int*, const int*, const float*, float*);
 
 }  // namespace omodels

@@ -122,7 +122,10 @@ public:
 private:
     template <class S> int bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x,
                                     double tol, double absTol, int maxIter, int* its);
-    template <class S> void precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run = nullptr);
+    // v = M^-1 d; with A and Av also Av = A v inside the preconditioner where it
+    // can (returns whether it did)
+    template <class S> bool precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run = nullptr,
+                                      const S* A = nullptr, S* Av = nullptr);
     int solveImpl();
     void tuneIlu();   // pick the tile shape / schedule of the ILU(0) by timing them
 

@@ -52,7 +52,7 @@ class Params(C.Structure):
         ("prec_relax", C.c_double), ("prec_order", C.c_int32), ("max_dt_divisions", C.c_int32),
         ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
         ("max_dt", C.c_double), ("min_dt", C.c_double), ("precision", C.c_int32), ("verbose", C.c_int32),
-        ("ilu_schedule", C.c_int32), ("autotune", C.c_int32), ("graph", C.c_int32), ("pad", C.c_int32),
+        ("ilu_schedule", C.c_int32), ("autotune", C.c_int32), ("graph", C.c_int32), ("spmv_in_sweep", C.c_int32),
     ]
 
 

@@ -486,3 +486,41 @@ def test_restart_continues_the_run_exactly(tmp_path):
     assert rc == 0
     assert np.array_equal(_bits(np.vstack([h1, h2])), _bits(hist))
     assert [r[2:] for r in l1 + l2] == [r[2:] for r in log]
+
+
+# ----------------------------------------------------------------------------
+# The operator application inside the backward sweep: the same bits as the
+# separate SpMV kernel
+@pytest.mark.parametrize("dims,precision", [((7, 5, 6), 0), ((33, 9, 3), 0), ((40, 40, 10), 0), ((40, 40, 10), 1),
+                                            ((25, 13, 11), 0)])
+def test_spmv_in_sweep_is_identical_to_separate_spmv(dims, precision):
+    c, J, r, pat = _system(dims, seed=6)
+    m = om.Model(c)
+    out = []
+    for fused in (0, 1):
+        m.set_profiling(1)
+        s = om.Solver(m, om.default_params(lin_tol=1e-10, lin_max_iter=500, precision=precision,
+                                           ilu_schedule=2, spmv_in_sweep=fused))
+        s.prepare(J, r)
+        rc, x, its = s.solve()
+        out.append((rc, x, its))
+        # fused: no SpMV launches of their own
+        assert (m.kernel_times()["spmv"][1] == 0) == bool(fused)
+        m.set_profiling(0)
+    (rc1, x1, i1), (rc0, x0, i0) = out
+    assert rc1 == rc0 == 0 and i1 == i0 > 3
+    assert np.array_equal(_bits(x0), _bits(x1))
+
+
+def test_spmv_in_sweep_newton_run_and_graph_replay():
+    c = case(20, 20, 5, seed=11)
+    runs = []
+    for kw in (dict(spmv_in_sweep=0), dict(spmv_in_sweep=1), dict(spmv_in_sweep=1, graph=1)):
+        m = om.Model(c)
+        m.set_solution(c.pv0, c.mean0)
+        runs.append(m.simulate(om.default_params(**kw), 3, 86400.0))
+    base = runs[0]
+    for rc, h, d, l, _ in runs[1:]:
+        assert rc == base[0] == 0
+        assert l == base[3]
+        assert np.array_equal(_bits(h), _bits(base[1]))
