@@ -115,7 +115,8 @@ typedef struct omodels_params {
     int32_t ilu_schedule;       /* 0: persistent tile-wavefront kernels, sweep form chosen from the
                                    tiles per tile level (or by the autotuner); 1: one launch per
                                    level; 2: tiles with streaming sweeps; 3: tiles with batched
-                                   sweeps.  Results do not depend on the choice */
+                                   sweeps.  Ensembles and the server use tiles with batched sweeps
+                                   unless 2 is given.  Results do not depend on the choice */
     int32_t autotune;           /* 1: time the ILU tile shapes (and the level schedule) on the first
                                    factorization of a grid size and keep the fastest; results do
                                    not depend on the choice */

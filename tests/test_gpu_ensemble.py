@@ -48,9 +48,8 @@ def test_ensemble_members_match_the_oracle():
 @pytest.mark.parametrize("schedule", [0, 2, 3])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule):
-    """schedule 0 lets the set choose its sweep form (one member alone and the
-    mix of four may choose differently), 2 / 3 force the streaming / batched
-    sweeps: the arithmetic is the same in every case."""
+    """schedule 0 (batched sweeps for ensembles), 2 / 3 force the streaming /
+    batched sweeps: the arithmetic is the same in every case."""
     ms = _members()
     prm = om.default_params(precision=precision, ilu_schedule=schedule)
     srv, mixed = _serve(ms, prm)
