@@ -5,7 +5,7 @@
  * NewtonMethod + BlackOilNewtonMethod and the time-step control.
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py as the checker / CPU baseline, never by the
+ * cpu_baseline leg of bench.py as the parity reference / CPU baseline, never by the
  * product library.  Written as plain, sequential-by-default C++ so that each
  * function reads against the reference file:line it restates.
  *

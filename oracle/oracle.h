@@ -1,10 +1,10 @@
 /*
  * oracle.h -- CPU restatement of the opm-models fully-implicit TPFA hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
- * product library.  It is linked or loaded only by tests/, by
+ * TEST INFRASTRUCTURE ONLY.  This library is what the parity tests compare the
+ * HIP product library with.  It is linked or loaded only by tests/, by
  * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, and only
- * as the checker or as the timed CPU baseline -- never as a product path.
+ * as that reference or as the timed CPU baseline -- never as a product path.
  *
  * Every function states the reference file:line it restates (paths relative
  * to the opm-models source tree).  The reference itself cannot be compiled

@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the parity checker and CPU baseline.
+cpu_baseline leg of bench.py, as the parity reference and CPU baseline.
 """
 import ctypes as C
 import os
