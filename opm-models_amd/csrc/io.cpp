@@ -15,10 +15,7 @@
 // produces for element-centred data (vtkmultiwriter.hh).
 #include <cctype>
 #include <cfloat>
-#include <cmath>
-#include <cstdio>
 #include <fstream>
-#include <iomanip>
 #include <sstream>
 #include <stdexcept>
 #include <string>
