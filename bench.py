@@ -333,6 +333,20 @@ def main():
         ob = N * (24 + 1 + 8) + 18 * N * 8
         line["output_fields"] = {"avg_us": round(oavg * 1e3, 2), "bytes_per_launch": ob,
                                  "achieved_GBs": round(ob / (oavg * 1e-3) / 1e9, 1)}
+    # the assembly with molecular diffusion in the oil phase (the diffusive face
+    # term), at the final state: same HIP events (class "assemble")
+    ms3, n3 = model.kernel_times()["assemble"]
+    model.set_diffusion(3e-4, 2e-4)
+    for _ in range(8):
+        om.check(om.lib().omodels_linearizer_linearize_domain(model.h, dt, 1), "linearize_domain")
+    ms4, n4 = model.kernel_times()["assemble"]
+    model.set_diffusion(None)
+    if n4 > n3:
+        davg = (ms4 - ms3) / (n4 - n3)
+        db = ab["assemble"] + F * 8          # plus one diffusivity per face
+        line["assemble_diffusion"] = {"avg_us": round(davg * 1e3, 2), "bytes_per_launch": db,
+                                      "achieved_GBs": round(db / (davg * 1e-3) / 1e9, 1),
+                                      "mcells_per_s": round(N / (davg * 1e-3) / 1e6, 1)}
     if not args.no_extra:
         line["beside"] = beside_runs(om, args.config, dt)
     if not args.no_cpu:

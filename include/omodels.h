@@ -191,6 +191,28 @@ int omodels_linearizer_source_blocks(omodels_model* m, int32_t* cells, double* r
  * (the reference's dirId 0..5, ecfvstencil.hh:209-218). */
 int omodels_linearizer_flows(omodels_model* m, double* flows, double* flores);
 
+/* Molecular diffusion in the oil phase: the model with enableDiffusion, whose
+ * face term is BlackOilDiffusionModule::addDiffusiveFlux
+ * (blackoildiffusionmodule.hh:168-238, added in
+ * blackoillocalresidualtpfa.hh:469-479).  The oil component and the dissolved
+ * gas move along the Rs difference of the two cells of a face, weighted by the
+ * mean So/Bo, in surface-volume units as the reference adds them.  Water has
+ * no term (no gas in water) and the gas phase none without vaporized oil. */
+typedef struct omodels_diffusion {
+    double oil_in_oil;              /* coefficient of the oil component in the oil phase [m^2/s] */
+    double gas_in_oil;              /* coefficient of the gas component in the oil phase [m^2/s] */
+    double mw_oil, mw_gas;          /* molar masses; both > 0: mole-fraction form (DIFFC's
+                                     * diffMoleFraction, :118-121), else mass fractions */
+    const double* face_diffusivity; /* the problem's diffusivity(I, J) (tpfalinearizer.hh:462-464),
+                                     * [3][N] with the face I | I + e_axis at axis*N + I;
+                                     * NULL: the harmonic mean of A phi |n.d| / |d|^2 */
+} omodels_diffusion;
+/* Switch diffusion on for the following linearizations and flows (d != NULL)
+ * or off (d == NULL).  No reference counterpart as a call: the reference
+ * fixes enableDiffusion at compile time and reads the coefficients from the
+ * deck.  The ensemble path has no diffusion. */
+int omodels_model_set_diffusion(omodels_model* m, const omodels_diffusion* d);
+
 /* ------------------------------------------------------------------------ */
 /* Output and restart files. */
 /* Output quantities of solution(0), [count][N] field after field, named as

@@ -44,14 +44,32 @@ __device__ __forceinline__ void loadIqV(const BoFluid& f, const double* __restri
     iqValues(f, pv[3 * c], pv[3 * c + 1], pv[3 * c + 2], mean[c], q);
 }
 
+// The neighbour J's side of the diffusive term of one face: So * 1/Bo of J
+// (the value half of the own-cell saturation formula) and the face's
+// diffusivity, stored at the transmissibility's index.
+template <bool Diff>
+__device__ __forceinline__ DiffFace diffFace(const BoDiff& dif, const double* __restrict__ pv,
+                                             const uint8_t* __restrict__ mean, long J, const IqV& q, long face)
+{
+    if constexpr (!Diff) {
+        return DiffFace{0.0, 0.0};
+    } else {
+        const double sg = (mean[J] == kMeanSg) ? pv[3 * J + 2] : 0.0;
+        const double so = (1.0 - pv[3 * J]) - sg;
+        return DiffFace{so * q.invB[kO], dif.diff[face]};
+    }
+}
+
 // Accumulate one face of cell I into its residual / diagonal registers and
 // store the column block A[J][I] = 0 - dF/dx_I.
+template <bool Diff>
 __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const IqV& qj, double trans,
                                           double area, double dZg, bool upInTie, double r[3],
-                                          double D[9], double* __restrict__ Jout)
+                                          double D[9], double* __restrict__ Jout, const BoDiff& dif,
+                                          DiffFace df)
 {
     E3 fl[3];
-    faceFlux(f, qi, qj, trans, area, dZg, upInTie, fl);
+    faceFluxT<false, Diff>(f, qi, qj, trans, area, dZg, upInTie, fl, nullptr, &dif, df);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         r[e] += fl[e].v;
@@ -72,12 +90,14 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
 
 // One (x, y) tile of columns over the planes of one z chunk.  Shared by the
 // single-reservoir kernel and the ensemble kernel (per-instance pointers).
+// Diff: with the diffusive face term (the face diffusivities in dif.diff).
+template <bool Diff>
 __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
     double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
-    int bx, int by, int bz, IqLds (*plane)[kTX + 2])
+    int bx, int by, int bz, IqLds (*plane)[kTX + 2], const BoDiff& dif)
 {
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
     const int i = bx * kTX + tx;
@@ -134,33 +154,36 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
             // intersection order: -x, +x, -y, +y, -z, +z
             if (i > 0) {
                 IqV q; fromLds(plane[ty + 1][tx], q);
-                applyFace(f, qi, q, trans[I - 1], G.area[0], (zI - zI) * G.g, false, r, D,
-                          jac + 9L * nbOff[0 * N + I]);
+                applyFace<Diff>(f, qi, q, trans[I - 1], G.area[0], (zI - zI) * G.g, false, r, D,
+                                jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff>(dif, pv, mean, I - 1, q, I - 1));
             }
             if (i + 1 < G.nx) {
                 IqV q; fromLds(plane[ty + 1][tx + 2], q);
-                applyFace(f, qi, q, trans[I], G.area[0], (zI - zI) * G.g, true, r, D,
-                          jac + 9L * nbOff[1 * N + I]);
+                applyFace<Diff>(f, qi, q, trans[I], G.area[0], (zI - zI) * G.g, true, r, D,
+                                jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff>(dif, pv, mean, I + 1, q, I));
             }
             if (j > 0) {
                 IqV q; fromLds(plane[ty][tx + 1], q);
-                applyFace(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zI) * G.g, false, r, D,
-                          jac + 9L * nbOff[2 * N + I]);
+                applyFace<Diff>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zI) * G.g, false, r, D,
+                                jac + 9L * nbOff[2 * N + I], dif,
+                                diffFace<Diff>(dif, pv, mean, I - G.nx, q, N + I - G.nx));
             }
             if (j + 1 < G.ny) {
                 IqV q; fromLds(plane[ty + 2][tx + 1], q);
-                applyFace(f, qi, q, trans[N + I], G.area[1], (zI - zI) * G.g, true, r, D,
-                          jac + 9L * nbOff[3 * N + I]);
+                applyFace<Diff>(f, qi, q, trans[N + I], G.area[1], (zI - zI) * G.g, true, r, D,
+                                jac + 9L * nbOff[3 * N + I], dif, diffFace<Diff>(dif, pv, mean, I + G.nx, q, N + I));
             }
             if (haveBelow) {
                 const double zJ = G.dz * ((double)(k - 1) + 0.5);
-                applyFace(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zJ) * G.g, false, r, D,
-                          jac + 9L * nbOff[4 * N + I]);
+                applyFace<Diff>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zJ) * G.g, false, r, D,
+                                jac + 9L * nbOff[4 * N + I], dif,
+                                diffFace<Diff>(dif, pv, mean, I - nxy, below, 2 * N + I - nxy));
             }
             if (k + 1 < G.nz) {
                 const double zJ = G.dz * ((double)(k + 1) + 0.5);
-                applyFace(f, qi, above, trans[2 * N + I], G.area[2], (zI - zJ) * G.g, true, r, D,
-                          jac + 9L * nbOff[5 * N + I]);
+                applyFace<Diff>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zJ) * G.g, true, r, D,
+                                jac + 9L * nbOff[5 * N + I], dif,
+                                diffFace<Diff>(dif, pv, mean, I + nxy, above, 2 * N + I));
             }
             // accumulation term (tpfalinearizer.hh:776-825)
             {
@@ -190,15 +213,17 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     }
 }
 
+template <bool Diff>
 __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
-    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk)
+    double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
+    BoDiff dif)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
-    assembleTile(f, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, blockIdx.x,
-                 blockIdx.y, blockIdx.z, plane);
+    assembleTile<Diff>(f, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, blockIdx.x,
+                       blockIdx.y, blockIdx.z, plane, dif);
 }
 
 // Cell-wise well sources (tpfalinearizer.hh:827-841), added after the flux
@@ -280,7 +305,7 @@ __global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const 
 // cell, the assembly's face evaluation.
 __global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, const uint8_t* __restrict__ mean,
                            const double* __restrict__ trans, const int* __restrict__ rowPtr,
-                           double* __restrict__ flows, double* __restrict__ flores)
+                           double* __restrict__ flows, double* __restrict__ flores, BoDiff dif)
 {
     const long nxy = (long)G.nx * G.ny;
     const long N = nxy * G.nz;
@@ -291,24 +316,28 @@ __global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, c
     iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
     const double zI = G.dz * ((double)k + 0.5);
     long e = rowPtr[I] - I;     // first neighbour entry of I
-    auto face = [&](long J, double t, double area, double zJ, bool upInTie) {
+    auto face = [&](long J, long fi, double area, double zJ, bool upInTie) {
         IqV qj;
         iqValues(f, pv[3 * J], pv[3 * J + 1], pv[3 * J + 2], mean[J], qj);
         E3 fl[3];
         double darcy[3];
-        faceFluxT<true>(f, qi, qj, t, area, (zI - zJ) * G.g, upInTie, fl, darcy);
+        if (dif.diff)
+            faceFluxT<true, true>(f, qi, qj, trans[fi], area, (zI - zJ) * G.g, upInTie, fl, darcy, &dif,
+                                  diffFace<true>(dif, pv, mean, J, qj, fi));
+        else
+            faceFluxT<true>(f, qi, qj, trans[fi], area, (zI - zJ) * G.g, upInTie, fl, darcy);
         for (int c = 0; c < 3; ++c) {
             if (flows) flows[3 * e + c] = fl[c].v;
             if (flores) flores[3 * e + c] = darcy[c];
         }
         ++e;
     };
-    if (i > 0) face(I - 1, trans[I - 1], G.area[0], zI, false);
-    if (i + 1 < G.nx) face(I + 1, trans[I], G.area[0], zI, true);
-    if (j > 0) face(I - G.nx, trans[N + I - G.nx], G.area[1], zI, false);
-    if (j + 1 < G.ny) face(I + G.nx, trans[N + I], G.area[1], zI, true);
-    if (k > 0) face(I - nxy, trans[2 * N + I - nxy], G.area[2], G.dz * ((double)(k - 1) + 0.5), false);
-    if (k + 1 < G.nz) face(I + nxy, trans[2 * N + I], G.area[2], G.dz * ((double)(k + 1) + 0.5), true);
+    if (i > 0) face(I - 1, I - 1, G.area[0], zI, false);
+    if (i + 1 < G.nx) face(I + 1, I, G.area[0], zI, true);
+    if (j > 0) face(I - G.nx, N + I - G.nx, G.area[1], zI, false);
+    if (j + 1 < G.ny) face(I + G.nx, N + I, G.area[1], zI, true);
+    if (k > 0) face(I - nxy, 2 * N + I - nxy, G.area[2], G.dz * ((double)(k - 1) + 0.5), false);
+    if (k + 1 < G.nz) face(I + nxy, 2 * N + I, G.area[2], G.dz * ((double)(k + 1) + 0.5), true);
 }
 
 // Output quantities of the current solution, one field after the other
@@ -438,7 +467,8 @@ __global__ void k_bo_update(BoFluid f, long N, double* __restrict__ pv, uint8_t*
 // Host launchers.
 void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
-                      double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s)
+                      double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
+                      const BoDiff* dif)
 {
     const int gx = (G.nx + kTX - 1) / kTX, gy = (G.ny + kTY - 1) / kTY;
     // enough workgroups to fill the chip: split the z march into chunks
@@ -446,8 +476,12 @@ void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const
     gz = gz < 1 ? 1 : (gz > G.nz ? G.nz : gz);
     const int kChunk = (G.nz + gz - 1) / gz;
     gz = (G.nz + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(k_bo_assemble, dim3(gx, gy, gz), dim3(kTX * kTY), 0, s, f, G, pv, mean, poro, trans, st1,
-                       storefac, res, jac, diagOff, nbOff, kChunk);
+    if (dif && dif->diff)
+        hipLaunchKernelGGL(k_bo_assemble<true>, dim3(gx, gy, gz), dim3(kTX * kTY), 0, s, f, G, pv, mean, poro,
+                           trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, *dif);
+    else
+        hipLaunchKernelGGL(k_bo_assemble<false>, dim3(gx, gy, gz), dim3(kTX * kTY), 0, s, f, G, pv, mean, poro,
+                           trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, BoDiff{});
 }
 
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
@@ -469,12 +503,12 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
 }
 
 void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
-                   const int* rowPtr, double* flows, double* flores, hipStream_t s)
+                   const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif)
 {
     const long N = (long)G.nx * G.ny * G.nz;
     if (N <= 0) return;
     hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, f, G, pv, mean, trans, rowPtr,
-                       flows, flores);
+                       flows, flores, dif ? *dif : BoDiff{});
 }
 
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
@@ -515,8 +549,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_e_assemble(BoFluid f, const EnsIn
     if (!active[t.x]) return;
     const EnsInst& I = insts[t.x];
     const int bx = t.y % I.gx, by = (t.y / I.gx) % I.gy, bz = t.y / (I.gx * I.gy);
-    assembleTile(f, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac, I.diagOff, I.nbOff,
-                 I.kChunk, bx, by, bz, plane);
+    assembleTile<false>(f, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac, I.diagOff,
+                        I.nbOff, I.kChunk, bx, by, bz, plane, BoDiff{});
 }
 
 __global__ void k_e_sources(BoFluid f, const EnsInst* __restrict__ insts, const int2* __restrict__ table, int n,

@@ -75,6 +75,7 @@ __device__ __forceinline__ E3 operator+(E3 a, double s) { return E3{a.v + s, a.d
 __device__ __forceinline__ E3 operator-(E3 a, double s) { return E3{a.v - s, a.d0, a.d1, a.d2}; }
 __device__ __forceinline__ E3 operator-(double s, E3 a) { return E3{s - a.v, -a.d0, -a.d1, -a.d2}; }
 __device__ __forceinline__ E3 operator*(E3 a, double s) { return E3{a.v * s, a.d0 * s, a.d1 * s, a.d2 * s}; }
+__device__ __forceinline__ E3 operator-(E3 a) { return E3{-a.v, -a.d0, -a.d1, -a.d2}; }
 __device__ __forceinline__ E3 emin(E3 a, double c) { return a.v <= c ? a : ec(c); }
 __device__ __forceinline__ E3 emax(E3 a, double c) { return a.v >= c ? a : ec(c); }
 
@@ -218,15 +219,33 @@ __device__ __forceinline__ void storageValues(const BoFluid& f, double sw, doubl
     st[kCO] = st[kCO] * f.rho[kO];
 }
 
+// Molecular diffusion in the oil phase (BlackOilDiffusionModule).  toFrac is
+// toFractionGasOil (blackoildiffusionmodule.hh:242-248), rho_o M_g / (rho_g M_o)
+// with unit molar masses unless the mole-fraction form is asked for; effOil
+// and effGas are the effective coefficients of the oil and the gas component
+// in the oil phase: the arithmetic mean of the two cells' constant
+// coefficients (:568-590), tortuosity being disabled for black-oil (:384-394).
+// diff: the problem's diffusivity per face, [3][N] like the transmissibilities.
+struct BoDiff {
+    double toFrac, effOil, effGas;
+    const double* diff;
+};
+// the neighbour's side of one face: So * 1/Bo and the face's diffusivity
+struct DiffFace {
+    double sbEx, diffusivity;
+};
+
 // One face seen from its interior cell I (derivatives w.r.t. I only), times
 // the face area.  upInTie: upwind choice when the pressure difference is
 // exactly zero (equal volumes -> the smaller global index, transfluxmodule.hh:283-301).
 // WithDarcy: also the phases' reservoir-volume rates across the face,
 // darcyFlux.value() * faceArea at the phase's component index (the FLORES
 // values, blackoillocalresidualtpfa.hh:376-385)
-template <bool WithDarcy>
+// WithDiff: plus the diffusive term of the oil phase (see addDiffusion).
+template <bool WithDarcy, bool WithDiff = false>
 __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
-                                          double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3])
+                                          double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3],
+                                          const BoDiff* dif = nullptr, DiffFace df = DiffFace{0.0, 0.0})
 {
     flux[0] = ec(0.0); flux[1] = ec(0.0); flux[2] = ec(0.0);
     if constexpr (WithDarcy) { darcyOut[0] = 0.0; darcyOut[1] = 0.0; darcyOut[2] = 0.0; }
@@ -257,12 +276,27 @@ __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const
         if (a == kO) flux[kCG] = flux[kCG] + gas * f.rho[kG];
         if constexpr (WithDarcy) darcyOut[c] = darcyV * area;
     }
+    // BlackOilDiffusionModule::addDiffusiveFlux (blackoildiffusionmodule.hh:168-238)
+    // after the phase loop, with the face's diffusivity per area
+    // (blackoillocalresidualtpfa.hh:469-479).  Only the oil phase has a term:
+    // water is skipped (no gas in water, :180-183) and the gas phase, without
+    // vaporized oil, has diffR = 0, so its terms are exact zeros (:205-209).
+    // The terms enter in surface-volume units, as the reference adds them.
+    if constexpr (WithDiff) {
+        E3 bSAvg = in.S[kO] * in.invB[kO];
+        bSAvg = bSAvg + df.sbEx;
+        bSAvg = bSAvg * 0.5;
+        if (!(bSAvg.v < 1.0e-6)) {
+            const E3 rsAvg = (in.Rs + ex.Rs) * 0.5;
+            const E3 convFactor = ec(1.0) / (rsAvg + dif->toFrac);
+            const E3 diffR = in.Rs - ex.Rs;
+            const double tmpDiff = df.diffusivity / area;
+            // solvent (oil in oil) and solute (gas in oil)
+            flux[kCO] = flux[kCO] + (((-bSAvg * convFactor) * diffR) * tmpDiff) * dif->effOil;
+            flux[kCG] = flux[kCG] + (((bSAvg * diffR) * convFactor) * tmpDiff) * dif->effGas;
+        }
+    }
     flux[0] = flux[0] * area; flux[1] = flux[1] * area; flux[2] = flux[2] * area;
-}
-__device__ __forceinline__ void faceFlux(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
-                                         double area, double dZg, bool upInTie, E3 flux[3])
-{
-    faceFluxT<false>(f, in, ex, trans, area, dZg, upInTie, flux, nullptr);
 }
 
 // Well source terms per unit volume (ReservoirProblem-style problem.source()).

@@ -293,6 +293,14 @@ int omodels_linearizer_flows(omodels_model* h, double* flows, double* flores)
     });
 }
 
+int omodels_model_set_diffusion(omodels_model* h, const omodels_diffusion* d)
+{
+    return guarded([&] {
+        h->m->setDiffusion(d);
+        return 0;
+    });
+}
+
 // ---- output and restart files ----
 namespace {
 const char* const kOutNames[kOutFields] = {

@@ -96,7 +96,7 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     const double storefac = G.V / dt;
     timer.begin(kClsAssemble, stream);
     launchBoAssemble(fluid, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
-                     nbOff.p, stream);
+                     nbOff.p, stream, &diffusion);
     timer.end(stream);
     // cell-wise sources (:827-841), or the same terms as sparse blocks added
     // after the cell loop (:842-846); both add each cell's term last
@@ -127,9 +127,48 @@ void Model::addSparseSources(int n, const int* cells, const double* r, const dou
 void Model::computeFlows(double* flows, double* flores)
 {
     timer.begin(kClsOther, stream);
-    launchBoFlows(fluid, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream);
+    launchBoFlows(fluid, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion);
     timer.end(stream);
     OM_HIP(hipGetLastError());
+}
+
+void Model::setDiffusion(const omodels_diffusion* d)
+{
+    if (!d) { diffusion = BoDiff{}; faceDiff.release(); return; }
+    if (!(d->oil_in_oil >= 0.0) || !(d->gas_in_oil >= 0.0) || !std::isfinite(d->oil_in_oil) ||
+        !std::isfinite(d->gas_in_oil))
+        throw std::invalid_argument("diffusion coefficients must be finite and >= 0");
+    std::vector<double> fd(3 * (size_t)N, 0.0);
+    if (d->face_diffusivity) {
+        std::copy(d->face_diffusivity, d->face_diffusivity + 3 * N, fd.begin());
+    } else {
+        // harmonic mean of the half-face values A phi |n.d| / |d|^2
+        std::vector<double> phi(N);
+        poro.download(phi.data(), N, stream);
+        sync();
+        const double h[3] = {G.dx, G.dy, G.dz};
+        const long stride[3] = {1, (long)G.nx, (long)G.nx * G.ny};
+        for (long I = 0; I < N; ++I) {
+            const long ijk[3] = {I % G.nx, (I / G.nx) % G.ny, I / ((long)G.nx * G.ny)};
+            const int n[3] = {G.nx, G.ny, G.nz};
+            for (int ax = 0; ax < 3; ++ax) {
+                if (ijk[ax] + 1 >= n[ax]) continue;
+                const long J = I + stride[ax];
+                const double A = G.area[ax], ndot = 0.5 * h[ax], dist2 = ndot * ndot;
+                const double D0 = A * phi[I] * ndot / dist2;
+                const double D1 = A * phi[J] * ndot / dist2;
+                fd[(size_t)ax * N + I] = D0 * D1 / (D0 + D1);
+            }
+        }
+    }
+    faceDiff.upload(fd.data(), fd.size(), stream);
+    const bool mole = d->mw_oil > 0.0 && d->mw_gas > 0.0;
+    const double mO = mole ? d->mw_oil : 1.0, mG = mole ? d->mw_gas : 1.0;
+    diffusion.toFrac = fluid.rho[kO] * mG / (fluid.rho[kG] * mO);
+    diffusion.effOil = 0.5 * (d->oil_in_oil + d->oil_in_oil);
+    diffusion.effGas = 0.5 * (d->gas_in_oil + d->gas_in_oil);
+    diffusion.diff = faceDiff.p;
+    sync();
 }
 
 void Model::addBuiltinSparseSources()

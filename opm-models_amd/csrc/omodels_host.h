@@ -95,6 +95,12 @@ public:
     // updateFlowsInfo (tpfalinearizer.hh:647-705) at solution(0): device arrays
     // [nnzb - N][3] in the pattern's neighbour order, either may be nullptr
     void computeFlows(double* flows, double* flores);
+    // BlackOilDiffusionModule in the face term (enableDiffusion); d == nullptr
+    // switches it off.  Without face diffusivities they are derived from the
+    // porosity like the transmissibilities.
+    void setDiffusion(const omodels_diffusion* d);
+    DevBuf<double> faceDiff;     // [3][N], at the transmissibility's index
+    BoDiff diffusion{};          // diffusion.diff == nullptr: off
     void ensureJacobian();
     void eraseMatrix() { jac.release(); }
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)

@@ -30,6 +30,12 @@ class BlackOilProblem(C.Structure):
     ]
 
 
+class Diffusion(C.Structure):
+    """omodels_diffusion: molecular diffusion in the oil phase."""
+    _fields_ = [("oil_in_oil", C.c_double), ("gas_in_oil", C.c_double), ("mw_oil", C.c_double),
+                ("mw_gas", C.c_double), ("face_diffusivity", _dp)]
+
+
 class BlackOilSpec(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("field", C.c_int32),
@@ -105,6 +111,7 @@ SIGNATURES = {
     "omodels_linearizer_source_blocks": (C.c_int, [C.c_void_p, _ip, _dp, _dp]),
     "omodels_linearizer_add_sparse_sources": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
     "omodels_linearizer_flows": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_model_set_diffusion": (C.c_int, [C.c_void_p, C.POINTER(Diffusion)]),
     "omodels_output_field_count": (C.c_int, []),
     "omodels_output_field_name": (C.c_char_p, [C.c_int32]),
     "omodels_output_fields": (C.c_int, [C.c_void_p, _dp]),
@@ -437,6 +444,21 @@ class Model:
         fr = np.zeros((nf, 3))
         check(lib().omodels_linearizer_flows(self.h, ptr(fl), ptr(fr)), "flows")
         return fl, fr
+
+    def set_diffusion(self, oil_in_oil=None, gas_in_oil=0.0, mw_oil=0.0, mw_gas=0.0, face_diffusivity=None):
+        """Molecular diffusion in the oil phase for the following linearizations
+        (coefficients in m^2/s; face_diffusivity [3][N] or None to derive it
+        from the porosity); oil_in_oil=None switches it off."""
+        if oil_in_oil is None:
+            check(lib().omodels_model_set_diffusion(self.h, None), "set_diffusion")
+            return
+        d = Diffusion(float(oil_in_oil), float(gas_in_oil), float(mw_oil), float(mw_gas), None)
+        if face_diffusivity is not None:
+            fd = np.ascontiguousarray(face_diffusivity, dtype=np.float64)
+            if fd.size != 3 * self.N:
+                raise ValueError("face_diffusivity must hold 3*N values")
+            d.face_diffusivity = ptr(fd)
+        check(lib().omodels_model_set_diffusion(self.h, C.byref(d)), "set_diffusion")
 
     def set_profiling(self, enable=True):
         check(lib().omodels_model_set_profiling(self.h, int(enable)), "set_profiling")

@@ -204,7 +204,7 @@ struct Grid {
     }
 };
 
-struct NbInfo { long J; int off; double trans, area, dZg; int dir; };
+struct NbInfo { long J; int off; double trans, area, dZg; int dir; double diffusivity; };
 
 struct Pattern {
     std::vector<int32_t> rowPtr, colIdx, diagOff, nbPtr;
@@ -227,7 +227,7 @@ void buildPattern(const Grid& G, Pattern& P)
             sp[I].insert(J);
             NbInfo n;
             n.J = J; n.off = -1; n.trans = G.trans(I, J, dir); n.area = G.area(dir);
-            n.dZg = (G.zc(I) - G.zc(J)) * G.g; n.dir = dir;
+            n.dZg = (G.zc(I) - G.zc(J)) * G.g; n.dir = dir; n.diffusivity = 0.0;
             P.nb.push_back(n);
         }
         P.nbPtr[I + 1] = (int32_t)P.nb.size();
@@ -269,6 +269,16 @@ void storageOf(const Pvt& pvt, const IQ<T>& q, T st[3])
     st[CO] = st[CO] * pvt.rho[O];
 }
 
+/* Constants of the diffusive face term.  toFrac is toFractionGasOil
+ * (blackoildiffusionmodule.hh:242-248): rho_o M_g / (rho_g M_o), with the
+ * molar masses replaced by 1 unless the mole-fraction form is asked for.
+ * effOil/effGas: the effective coefficients of the oil and the gas component
+ * in the oil phase, the arithmetic mean of two cells' constant coefficients
+ * (:568-590; tortuosity is disabled for the black-oil model, :384-394). */
+struct Diffusion {
+    double toFrac, effOil, effGas;
+};
+
 /* calculateFluxes_ (blackoillocalresidualtpfa.hh:325-504) with the
  * pressure difference of transfluxmodule.hh:254-303 and evalPhaseFluxes_
  * (:752-811), followed by the caller's multiplication with the face area
@@ -276,7 +286,8 @@ void storageOf(const Pvt& pvt, const IQ<T>& q, T st[3])
 /* darcyOut (optional): each phase's darcyFlux.value() * faceArea at its
  * component index, the FLORES values (blackoillocalresidualtpfa.hh:376-385) */
 void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, long J,
-              const NbInfo& nbi, double Vin, double Vex, E3 flux[3], double* darcyOut = nullptr)
+              const NbInfo& nbi, double Vin, double Vex, E3 flux[3], double* darcyOut = nullptr,
+              const Diffusion* dif = nullptr)
 {
     for (int e = 0; e < 3; ++e) flux[e] = E3(0.0);
     if (darcyOut) for (int e = 0; e < 3; ++e) darcyOut[e] = 0.0;
@@ -307,6 +318,26 @@ void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, lo
         }
         if (darcyOut) darcyOut[compOfPhase[a]] = darcy.v * nbi.area;
     }
+    /* BlackOilDiffusionModule::addDiffusiveFlux (blackoildiffusionmodule.hh:168-238),
+     * added after the phase loop (blackoillocalresidualtpfa.hh:469-479) with the
+     * face's diffusivity per area.  Only the oil phase carries a term here:
+     * water is skipped (no gas in water, :180-183) and the gas phase, without
+     * vaporized oil, has diffR = 0, so its terms are exact zeros (:205-209).
+     * The terms enter in surface-volume units, as the reference adds them. */
+    if (dif) {
+        E3 bSAvg = in.S[O] * in.invB[O];
+        bSAvg = bSAvg + ex.S[O] * ex.invB[O];
+        bSAvg = bSAvg / 2.0;
+        if (!(bSAvg.v < 1.0e-6)) {
+            const E3 rsAvg = (in.Rs + ex.Rs) / 2.0;
+            const E3 convFactor = E3(1.0) / (rsAvg + dif->toFrac);
+            const E3 diffR = in.Rs - ex.Rs;
+            const double tmpDiff = nbi.diffusivity / nbi.area;
+            /* solvent (oil in oil) and solute (gas in oil) */
+            flux[CO] = flux[CO] + (((-bSAvg * convFactor) * diffR) * tmpDiff) * dif->effOil;
+            flux[CG] = flux[CG] + (((bSAvg * diffR) * convFactor) * tmpDiff) * dif->effGas;
+        }
+    }
     for (int e = 0; e < 3; ++e) flux[e] = flux[e] * nbi.area;
 }
 
@@ -334,13 +365,46 @@ struct Problem {
     Pvt pvt;
     Pattern pat;
     std::vector<int> srcOfCell;
-    explicit Problem(const ob_problem* p) : P(p), G(p)
+    bool hasDiff = false;
+    Diffusion dif{};
+    explicit Problem(const ob_problem* p, const ob_diffusion* d = nullptr) : P(p), G(p)
     {
         pvt.init(p);
         buildPattern(G, pat);
         srcOfCell.assign(G.N, -1);
         for (int s = 0; s < p->n_src; ++s) srcOfCell[p->src_cell[s]] = s;
+        if (d) setDiffusion(*d);
     }
+    /* the problem's diffusivity(I, J) per face (tpfalinearizer.hh:462-464):
+     * given per face, or the harmonic mean of the half-face values
+     * A phi |n.d| / |d|^2 (the transmissibility form with porosity in place
+     * of permeability) */
+    void setDiffusion(const ob_diffusion& d)
+    {
+        if (!(d.oil_in_oil >= 0.0) || !(d.gas_in_oil >= 0.0))
+            throw std::invalid_argument("diffusion coefficients must be >= 0");
+        const bool mole = d.mw_oil > 0.0 && d.mw_gas > 0.0;
+        const double mO = mole ? d.mw_oil : 1.0, mG = mole ? d.mw_gas : 1.0;
+        dif.toFrac = pvt.rho[O] * mG / (pvt.rho[::G] * mO);
+        dif.effOil = 0.5 * (d.oil_in_oil + d.oil_in_oil);
+        dif.effGas = 0.5 * (d.gas_in_oil + d.gas_in_oil);
+        hasDiff = true;
+        for (long I = 0; I < G.N; ++I)
+            for (int k = pat.nbPtr[I]; k < pat.nbPtr[I + 1]; ++k) {
+                NbInfo& n = pat.nb[k];
+                const int ax = n.dir / 2;
+                const long lo = I < n.J ? I : n.J, hi = I < n.J ? n.J : I;
+                if (d.face_diffusivity) {
+                    n.diffusivity = d.face_diffusivity[(long)ax * G.N + lo];
+                } else {
+                    const double A = G.area(n.dir), ndot = 0.5 * G.h(n.dir), dist2 = ndot * ndot;
+                    const double D0 = A * P->poro[lo] * ndot / dist2;
+                    const double D1 = A * P->poro[hi] * ndot / dist2;
+                    n.diffusivity = D0 * D1 / (D0 + D1);
+                }
+            }
+    }
+    const Diffusion* diffusion() const { return hasDiff ? &dif : nullptr; }
 };
 
 void computeStorageAll(const Problem& pr, const double* pv, const uint8_t* mean, double* st)
@@ -379,7 +443,7 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
             IQ<double> qj;
             computeIQ<double>(pr.pvt, pv + 3 * nbi.J, mean[nbi.J], pr.P->poro[nbi.J], qj);
             E3 f[3];
-            faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f);
+            faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, nullptr, pr.diffusion());
             double* O_ = jac + 9L * nbi.off;
             for (int e = 0; e < 3; ++e) {
                 r[e] += f[e].v;
@@ -871,8 +935,14 @@ int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meani
  * (FLOWS) and the phases' reservoir-volume rates (FLORES), [entries][3]. */
 int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, double* flows, double* flores)
 {
+    return ob_flows_diff(p, nullptr, pv, meaning, flows, flores);
+}
+
+int ob_flows_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
+                  double* flows, double* flores)
+{
     try {
-        Problem pr(p);
+        Problem pr(p, dif);
         const double V = pr.G.volume();
         for (long I = 0; I < pr.G.N; ++I) {
             IQ<E3> qi;
@@ -883,7 +953,7 @@ int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, doub
                 computeIQ<double>(pr.pvt, pv + 3 * nbi.J, meaning[nbi.J], p->poro[nbi.J], qj);
                 E3 f[3];
                 double d[3];
-                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, d);
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, d, pr.diffusion());
                 for (int e = 0; e < 3; ++e) {
                     if (flows) flows[3L * k + e] = f[e].v;
                     if (flores) flores[3L * k + e] = d[e];
@@ -897,8 +967,14 @@ int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, doub
 int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  const double* storage1, double dt, double* residual, double* jac, int nthreads)
 {
+    return ob_linearize_diff(p, nullptr, pv, meaning, storage1, dt, residual, jac, nthreads);
+}
+
+int ob_linearize_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
+                      const double* storage1, double dt, double* residual, double* jac, int nthreads)
+{
     try {
-        Problem pr(p);
+        Problem pr(p, dif);
         linearize(pr, pv, meaning, storage1, dt, residual, jac, nthreads);
         return 0;
     } catch (...) { return -1; }
@@ -958,9 +1034,17 @@ int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps, 
                 const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
                 int log_cap, int32_t* log_len, double* timers, int nthreads)
 {
+    return ob_simulate_diff(p, nullptr, pv, meaning, n_steps, dt0, prm, pv_hist, dt_hist, log, log_cap, log_len,
+                            timers, nthreads);
+}
+
+int ob_simulate_diff(const ob_problem* p, const ob_diffusion* dif, double* pv, uint8_t* meaning, int n_steps,
+                     double dt0, const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
+                     int log_cap, int32_t* log_len, double* timers, int nthreads)
+{
     using clk = std::chrono::steady_clock;
     try {
-        Problem pr(p);
+        Problem pr(p, dif);
         const long N = pr.G.N;
         const int32_t* rp = pr.pat.rowPtr.data();
         const int32_t* ci = pr.pat.colIdx.data();

@@ -79,6 +79,17 @@ typedef struct ob_iter_log {
     double error, dt;
 } ob_iter_log;
 
+/* Molecular diffusion in the oil phase (BlackOilDiffusionModule,
+ * blackoildiffusionmodule.hh:168-238, face term of
+ * blackoillocalresidualtpfa.hh:469-479): the oil component and the dissolved
+ * gas diffuse along the face's Rs difference. */
+typedef struct ob_diffusion {
+    double oil_in_oil;              /* coefficient of the oil component in the oil phase [m^2/s] */
+    double gas_in_oil;              /* coefficient of the gas component in the oil phase [m^2/s] */
+    double mw_oil, mw_gas;          /* molar masses; both > 0: mole-fraction form, else mass fractions */
+    const double* face_diffusivity; /* [3][N], face I|I+e_axis at axis*N + I; NULL: from porosity */
+} ob_diffusion;
+
 /* ---- A1: pattern, neighbour info, block addresses ---- */
 int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2);
 int ob_build_pattern(const ob_problem* p, int32_t* row_ptr, int32_t* col_idx,
@@ -94,6 +105,11 @@ int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  double* jac /* [NNZB][3][3] */, int nthreads);
 /* face rates (FLOWS, FLORES) per neighbour entry of the pattern, [entries][3] */
 int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, double* flows, double* flores);
+/* the same three with the diffusive face term (dif may be NULL) */
+int ob_linearize_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
+                      const double* storage1, double dt, double* residual, double* jac, int nthreads);
+int ob_flows_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
+                  double* flows, double* flores);
 /* source terms as sparse blocks: cells [n_src], residual [n_src][3] (times -V),
  * derivatives [n_src][3][3]; returns n_src */
 int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cells,
@@ -126,6 +142,10 @@ int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps,
                 double* dt_hist /* [n_steps] */, ob_iter_log* log, int log_cap,
                 int32_t* log_len, double* timers /* [4] linearize, solve, update, total */,
                 int nthreads);
+
+int ob_simulate_diff(const ob_problem* p, const ob_diffusion* dif, double* pv, uint8_t* meaning, int n_steps,
+                     double dt0, const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
+                     int log_cap, int32_t* log_len, double* timers, int nthreads);
 
 void ob_default_params(ob_params* prm);
 

@@ -30,6 +30,22 @@ class ObProblem(C.Structure):
     ]
 
 
+class ObDiffusion(C.Structure):
+    """ob_diffusion (oracle.h), same layout as omodels_diffusion."""
+    _fields_ = [("oil_in_oil", C.c_double), ("gas_in_oil", C.c_double), ("mw_oil", C.c_double),
+                ("mw_gas", C.c_double), ("face_diffusivity", _dp)]
+
+
+def diffusion(oil_in_oil, gas_in_oil, mw_oil=0.0, mw_gas=0.0, face_diffusivity=None):
+    """ObDiffusion; keeps a reference to the face array it points at."""
+    d = ObDiffusion(oil_in_oil, gas_in_oil, mw_oil, mw_gas, None)
+    if face_diffusivity is not None:
+        fd = np.ascontiguousarray(face_diffusivity, dtype=np.float64)
+        d.face_diffusivity = fd.ctypes.data_as(_dp)
+        d._keep = fd
+    return d
+
+
 class ObParams(C.Structure):
     _fields_ = [
         ("newton_tol", C.c_double), ("newton_max_iter", C.c_int32), ("newton_target_iter", C.c_int32),
@@ -84,6 +100,12 @@ def lib():
                                         C.POINTER(ObParams), _ip]
         L.ob_simulate.argtypes = [C.POINTER(ObProblem), _dp, _u8p, C.c_int, C.c_double, C.POINTER(ObParams),
                                   _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp, C.c_int]
+        _dif = C.POINTER(ObDiffusion)
+        L.ob_linearize_diff.argtypes = [C.POINTER(ObProblem), _dif, _dp, _u8p, _dp, C.c_double, _dp, _dp, C.c_int]
+        L.ob_flows_diff.argtypes = [C.POINTER(ObProblem), _dif, _dp, _u8p, _dp, _dp]
+        L.ob_simulate_diff.argtypes = [C.POINTER(ObProblem), _dif, _dp, _u8p, C.c_int, C.c_double,
+                                       C.POINTER(ObParams), _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp,
+                                       C.c_int]
         L.ol_default_problem.argtypes = [C.POINTER(OlProblem)]
         L.ol_num_cells.argtypes = [C.POINTER(OlProblem), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.ol_pattern.argtypes = [C.POINTER(OlProblem), _ip, _ip, _ip]
@@ -143,14 +165,19 @@ def storage(prob, pv, mean):
     return st
 
 
-def linearize(prob, pv, mean, st1, dt, nthreads=1):
+def _dif(d):
+    return C.byref(d) if d is not None else None
+
+
+def linearize(prob, pv, mean, st1, dt, nthreads=1, diffusion=None):
     N = prob.nx * prob.ny * prob.nz
     nz = C.c_int64()
     f2 = C.c_int64()
     lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
     r = np.zeros(3 * N)
     J = np.zeros(9 * nz.value)
-    rc = lib().ob_linearize(C.byref(prob), P(pv), P(mean, _u8p), P(st1), dt, P(r), P(J), nthreads)
+    rc = lib().ob_linearize_diff(C.byref(prob), _dif(diffusion), P(pv), P(mean, _u8p), P(st1), dt, P(r), P(J),
+                                 nthreads)
     assert rc == 0
     return J, r
 
@@ -166,7 +193,7 @@ def source_blocks(prob, pv, mean):
     return cells[:n], r[:3 * n], d[:9 * n]
 
 
-def flows(prob, pv, mean):
+def flows(prob, pv, mean, diffusion=None):
     """(flows, flores), [neighbour entries][3] in the pattern's order."""
     nz = C.c_int64()
     f2 = C.c_int64()
@@ -175,7 +202,7 @@ def flows(prob, pv, mean):
     nf = nz.value - N
     fl = np.zeros((nf, 3))
     fr = np.zeros((nf, 3))
-    assert lib().ob_flows(C.byref(prob), P(pv), P(mean, _u8p), P(fl), P(fr)) == 0
+    assert lib().ob_flows_diff(C.byref(prob), _dif(diffusion), P(pv), P(mean, _u8p), P(fl), P(fr)) == 0
     return fl, fr
 
 
@@ -225,7 +252,7 @@ def update_primary(prob, cur, cur_mean, delta, was_switched, prm):
     return rc, nxt, nm, ws, ns.value
 
 
-def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep_history=True):
+def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep_history=True, diffusion=None):
     N = prob.nx * prob.ny * prob.nz
     pv = np.array(pv0, dtype=np.float64, copy=True)
     mean = np.array(mean0, dtype=np.uint8, copy=True)
@@ -234,8 +261,8 @@ def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep
     log = (ObIterLog * log_cap)()
     nlog = C.c_int32()
     timers = np.zeros(4)
-    rc = lib().ob_simulate(C.byref(prob), P(pv), P(mean, _u8p), n_steps, dt0, C.byref(prm), P(hist), P(dts), log,
-                           log_cap, C.byref(nlog), P(timers), nthreads)
+    rc = lib().ob_simulate_diff(C.byref(prob), _dif(diffusion), P(pv), P(mean, _u8p), n_steps, dt0, C.byref(prm),
+                                P(hist), P(dts), log, log_cap, C.byref(nlog), P(timers), nthreads)
     recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
             for l in log[:min(nlog.value, log_cap)]]
     return rc, pv, mean, hist, dts, recs, timers

@@ -9,7 +9,7 @@ import pytest
 
 import omodels_amd as om
 import oracle as orc
-from cases import case, mixed_state
+from cases import case, mixed_state, rel_err
 
 
 def _case_nowells(nx, ny, nz, seed=3, **kw):
@@ -543,3 +543,159 @@ def test_oracle_assembly_is_independent_of_thread_count():
     J1, r1 = orc.linearize(prob, pv, mean, st1, 1e4, nthreads=1)
     J4, r4 = orc.linearize(prob, pv, mean, st1, 1e4, nthreads=4)
     assert np.array_equal(J1, J4) and np.array_equal(r1, r4)
+
+
+# ----------------------------------------------------------------------------
+# Molecular diffusion in the oil phase (blackoildiffusionmodule.hh:168-238,
+# face term of blackoillocalresidualtpfa.hh:469-479)
+D_OIL, D_GAS = 3e-4, 2e-4           # large, so that the term is far above round-off
+
+
+def _diff(**kw):
+    return orc.diffusion(kw.pop("oil_in_oil", D_OIL), kw.pop("gas_in_oil", D_GAS), **kw)
+
+
+def _no_acc(prob, pv, mean, dif):
+    """residual and Jacobian of the face terms only (no accumulation, no wells)"""
+    return orc.linearize(prob, pv, mean, orc.storage(prob, pv, mean), 1e300, diffusion=dif)
+
+
+def _two_cells(rs_i, rs_j, sw=0.2, p=200e5):
+    c = _case_nowells(2, 1, 1, seed=4)
+    prob = orc.problem_from_case(c)
+    prob.gravity = 0.0                # equal pressures, no gravity: no advective flux
+    pv = np.array([sw, p, rs_i, sw, p, rs_j])
+    return c, prob, pv, np.array([1, 1], np.uint8)
+
+
+def test_diffusion_closed_form_between_two_cells():
+    """Two undersaturated cells at one pressure: the only flux is diffusion.
+    Gas leaves the cell with the larger Rs, oil enters it, in the ratio of the
+    coefficients; the amounts follow from the mean So/Bo, the Rs difference
+    over (rho_o/rho_g + mean Rs) and the harmonic mean of A phi / (h/2)."""
+    c, prob, pv, mean = _two_cells(140.0, 90.0)
+    _, r0 = _no_acc(prob, pv, mean, None)
+    assert np.all(r0 == 0.0)
+    _, r = _no_acc(prob, pv, mean, _diff())
+    qi = orc.intensive(prob, pv[:3], 1, cell=0)
+    qj = orc.intensive(prob, pv[3:], 1, cell=1)
+    so = 1.0 - 0.2
+    bs = 0.5 * (so * qi["invB_o"][0] + so * qj["invB_o"][0])
+    rho_w, rho_o, rho_g = om.default_fluid()[4][4:7]
+    conv = 1.0 / (rho_o / rho_g + 0.5 * (140.0 + 90.0))
+    A, h = c.spec.dy * c.spec.dz, c.spec.dx
+    d0, d1 = A * c.poro[0] / (h / 2), A * c.poro[1] / (h / 2)
+    dif = d0 * d1 / (d0 + d1)
+    gas = bs * (140.0 - 90.0) * conv * dif * D_GAS
+    assert gas > 0.0
+    assert r[2] == pytest.approx(gas, rel=1e-13)                      # gas component (index 2)
+    assert r[0] == pytest.approx(-gas * D_OIL / D_GAS, rel=1e-13)     # oil component (index 0)
+    assert r[1] == 0.0                                                # no water term
+    assert np.array_equal(r[3:], -r[:3])                              # the same face from J
+
+
+def test_diffusion_mole_fraction_form():
+    """With molar masses the conversion factor uses rho_o M_g / (rho_g M_o)."""
+    c, prob, pv, mean = _two_cells(140.0, 90.0)
+    _, rm = _no_acc(prob, pv, mean, _diff())
+    _, rx = _no_acc(prob, pv, mean, _diff(mw_oil=0.2, mw_gas=0.016))
+    rho_w, rho_o, rho_g = om.default_fluid()[4][4:7]
+    rs = 0.5 * (140.0 + 90.0)
+    ratio = (rho_o / rho_g + rs) / (rho_o * 0.016 / (rho_g * 0.2) + rs)
+    assert rx[2] == pytest.approx(rm[2] * ratio, rel=1e-13)
+    assert rx[0] == pytest.approx(rm[0] * ratio, rel=1e-13)
+
+
+def test_diffusion_needs_oil():
+    """Without oil in either cell (mean So/Bo below 1e-6) the term vanishes."""
+    c, prob, pv, mean = _two_cells(0.0, 0.0)
+    pv = np.array([0.3, 200e5, 0.7, 0.4, 200e5, 0.6])
+    mean = np.array([0, 0], np.uint8)
+    J0, r0 = _no_acc(prob, pv, mean, None)
+    J1, r1 = _no_acc(prob, pv, mean, _diff())
+    assert np.array_equal(r0.view(np.uint64), r1.view(np.uint64))
+    assert np.array_equal(J0.view(np.uint64), J1.view(np.uint64))
+
+
+def test_diffusion_face_diffusivity_given_or_derived():
+    """The derived face diffusivities equal an explicit array of the same
+    values bit for bit, and the term is linear in them."""
+    c = _case_nowells(4, 3, 2, seed=6)
+    pv, mean = mixed_state(c, seed=6)
+    prob = orc.problem_from_case(c)
+    N = c.N
+    nx, ny, nz = 4, 3, 2
+    h = (c.spec.dx, c.spec.dy, c.spec.dz)
+    area = (c.spec.dy * c.spec.dz, c.spec.dx * c.spec.dz, c.spec.dx * c.spec.dy)
+    fd = np.zeros(3 * N)
+    for I in range(N):
+        ijk = (I % nx, (I // nx) % ny, I // (nx * ny))
+        for ax, (n, stride) in enumerate(((nx, 1), (ny, nx), (nz, nx * ny))):
+            if ijk[ax] + 1 < n:
+                d0 = area[ax] * c.poro[I] * (0.5 * h[ax]) / (0.5 * h[ax]) ** 2
+                d1 = area[ax] * c.poro[I + stride] * (0.5 * h[ax]) / (0.5 * h[ax]) ** 2
+                fd[ax * N + I] = d0 * d1 / (d0 + d1)
+    _, r0 = _no_acc(prob, pv, mean, None)
+    Jd, rd = _no_acc(prob, pv, mean, _diff())
+    Jg, rg = _no_acc(prob, pv, mean, _diff(face_diffusivity=fd))
+    assert np.array_equal(rd.view(np.uint64), rg.view(np.uint64))
+    assert np.array_equal(Jd.view(np.uint64), Jg.view(np.uint64))
+    _, r2 = _no_acc(prob, pv, mean, _diff(face_diffusivity=2.0 * fd))
+    part = rd - r0
+    assert np.abs(part).max() > 1e-6 * np.abs(r0).max()
+    assert rel_err(r2 - r0, 2.0 * part) <= 1e-9
+
+
+def test_diffusion_jacobian_matches_central_differences():
+    """Derivatives of the diffusive part (with and without minus without)
+    against central differences of the residual, all cells and variables."""
+    c = _case_nowells(3, 2, 2, seed=8)
+    pv, mean = mixed_state(c, seed=8)
+    prob = orc.problem_from_case(c)
+    st1 = orc.storage(prob, pv, mean)
+    dif = _diff()
+
+    def part(x):
+        Jd, rd = orc.linearize(prob, x, mean, st1, 1e300, diffusion=dif)
+        J0, r0 = orc.linearize(prob, x, mean, st1, 1e300)
+        return Jd - J0, rd - r0
+
+    Jp, rp_ = part(pv)
+    pat = orc.pattern(prob)
+    N = c.N
+    steps = [1e-6, 1.0, 1e-6]
+    scale = np.abs(Jp).reshape(-1, 3, 3).max(axis=(0, 1))
+    assert scale.min() > 0.0
+    for col in range(N):
+        for var in range(3):
+            h = steps[var] if not (var == 2 and mean[col] == 1) else 1e-4
+            pp = pv.copy(); pp[3 * col + var] += h
+            pm = pv.copy(); pm[3 * col + var] -= h
+            fd = (part(pp)[1] - part(pm)[1]) / (2 * h)
+            for row in range(N):
+                ks = np.where(pat["col_idx"][pat["row_ptr"][row]:pat["row_ptr"][row + 1]] == col)[0]
+                blk = Jp[9 * (pat["row_ptr"][row] + ks[0]):][:9].reshape(3, 3) if len(ks) else np.zeros((3, 3))
+                assert np.max(np.abs(blk[:, var] - fd[3 * row:3 * row + 3])) <= 1e-5 * scale[var], (row, col, var)
+
+
+def test_diffusion_conserves_and_flows_sum_to_the_residual():
+    """With diffusion the interior face terms still cancel over the domain,
+    and the FLOWS (which include it) add up to the residual bit for bit."""
+    c = _case_nowells(5, 4, 3, seed=5)
+    pv, mean = mixed_state(c, seed=5)
+    prob = orc.problem_from_case(c)
+    dif = _diff()
+    J, r = _no_acc(prob, pv, mean, dif)
+    _, r0 = _no_acc(prob, pv, mean, None)
+    R, P = r.reshape(-1, 3), (r - r0).reshape(-1, 3)
+    assert np.abs(P).max() > 0.0
+    assert np.max(np.abs(R.sum(axis=0))) <= 1e-12 * np.abs(R).sum(axis=0).max()
+    fl, _ = orc.flows(prob, pv, mean, diffusion=dif)
+    nbp = orc.pattern(prob)["nb_ptr"]
+    acc = np.zeros(3 * c.N)
+    for I in range(c.N):
+        a = np.zeros(3)
+        for k in range(nbp[I], nbp[I + 1]):
+            a = a + fl[k]
+        acc[3 * I:3 * I + 3] = a
+    assert np.array_equal(acc.view(np.uint64), r.view(np.uint64))
