@@ -340,13 +340,26 @@ def main():
     for _ in range(8):
         om.check(om.lib().omodels_linearizer_linearize_domain(model.h, dt, 1), "linearize_domain")
     ms4, n4 = model.kernel_times()["assemble"]
-    model.set_diffusion(None)
     if n4 > n3:
         davg = (ms4 - ms3) / (n4 - n3)
         db = ab["assemble"] + F * 8          # plus one diffusivity per face
         line["assemble_diffusion"] = {"avg_us": round(davg * 1e3, 2), "bytes_per_launch": db,
                                       "achieved_GBs": round(db / (davg * 1e-3) / 1e9, 1),
                                       "mcells_per_s": round(N / (davg * 1e-3) / 1e6, 1)}
+    # and with dispersion as well (dispersion coefficient 10 m in every cell)
+    model.set_dispersion(cell_dispersion=np.full(N, 10.0))
+    for _ in range(8):
+        om.check(om.lib().omodels_linearizer_linearize_domain(model.h, dt, 1), "linearize_domain")
+    ms5, n5 = model.kernel_times()["assemble"]
+    model.set_dispersion(off=True)
+    model.set_diffusion(None)
+    if n5 > n4:
+        eavg = (ms5 - ms4) / (n5 - n4)
+        # plus a dispersivity per face, the cells' velocities read and written
+        eb = ab["assemble"] + 2 * F * 8 + 2 * N * 8
+        line["assemble_diffusion_dispersion"] = {"avg_us": round(eavg * 1e3, 2), "bytes_per_launch": eb,
+                                                 "achieved_GBs": round(eb / (eavg * 1e-3) / 1e9, 1),
+                                                 "mcells_per_s": round(N / (eavg * 1e-3) / 1e6, 1)}
     if not args.no_extra:
         line["beside"] = beside_runs(om, args.config, dt)
     if not args.no_cpu:

@@ -213,6 +213,27 @@ typedef struct omodels_diffusion {
  * deck.  The ensemble path has no diffusion. */
 int omodels_model_set_diffusion(omodels_model* m, const omodels_diffusion* d);
 
+/* Dispersion in the oil phase: the model with DISPERC, whose face term is
+ * BlackOilDispersionModule::addDispersiveFlux (blackoildispersionmodule.hh:176-254,
+ * added in blackoillocalresidualtpfa.hh:481-493): the oil component and the
+ * dissolved gas move along the Rs difference, weighted by the mean 1/Bo and
+ * the mean of the two cells' oil velocity.  A cell's velocity is the largest
+ * |oil Darcy flux per area| over its faces in the previous linearization
+ * (tpfalinearizer.hh:759-761 with blackoildispersionmodule.hh:355-383); it is
+ * zero before the first (createFlows_, tpfalinearizer.hh:571-619). */
+typedef struct omodels_dispersion {
+    const double* face_dispersivity; /* the problem's dispersivity(I, J) (tpfalinearizer.hh:465-467),
+                                      * [3][N] like face_diffusivity, or NULL */
+    const double* cell_dispersion;   /* [N] dispersion coefficients [m], used when the face array is
+                                      * NULL: harmonic mean of A E |n.d| / |d|^2 */
+} omodels_dispersion;
+/* Switch dispersion on (d != NULL: the cells' velocities start at zero) or off
+ * (d == NULL).  No reference counterpart as a call (DISPERC in the deck). */
+int omodels_model_set_dispersion(omodels_model* m, const omodels_dispersion* d);
+/* The cells' oil velocities [N] the next linearization will use (the
+ * normVelocityCell of BlackOilDispersionIntensiveQuantities); dispersion on. */
+int omodels_model_get_cell_velocity(omodels_model* m, double* velocity);
+
 /* ------------------------------------------------------------------------ */
 /* Output and restart files. */
 /* Output quantities of solution(0), [count][N] field after field, named as

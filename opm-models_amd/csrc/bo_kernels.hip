@@ -44,32 +44,49 @@ __device__ __forceinline__ void loadIqV(const BoFluid& f, const double* __restri
     iqValues(f, pv[3 * c], pv[3 * c + 1], pv[3 * c + 2], mean[c], q);
 }
 
-// The neighbour J's side of the diffusive term of one face: So * 1/Bo of J
-// (the value half of the own-cell saturation formula) and the face's
-// diffusivity, stored at the transmissibility's index.
-template <bool Diff>
-__device__ __forceinline__ DiffFace diffFace(const BoDiff& dif, const double* __restrict__ pv,
-                                             const uint8_t* __restrict__ mean, long J, const IqV& q, long face)
+// The neighbour J's side of the diffusive and dispersive terms of one face:
+// So * 1/Bo of J (the value half of the own-cell saturation formula), the
+// face's diffusivity and dispersivity (stored at the transmissibility's index)
+// and the mean of the two cells' oil velocities of the previous linearization.
+template <bool Diff, bool Disp>
+__device__ __forceinline__ DiffFace diffFace(const BoDiff& dif, const BoDisp& dsp, const double* __restrict__ pv,
+                                             const uint8_t* __restrict__ mean, long I, long J, const IqV& q,
+                                             long face)
 {
-    if constexpr (!Diff) {
-        return DiffFace{0.0, 0.0};
-    } else {
+    DiffFace df{0.0, 0.0, 0.0, 0.0};
+    if constexpr (Diff) {
         const double sg = (mean[J] == kMeanSg) ? pv[3 * J + 2] : 0.0;
         const double so = (1.0 - pv[3 * J]) - sg;
-        return DiffFace{so * q.invB[kO], dif.diff[face]};
+        df.sbEx = so * q.invB[kO];
+        df.diffusivity = dif.diff[face];
     }
+    if constexpr (Disp) {
+        df.dispersivity = dsp.disp[face];
+        df.nvAvg = 0.5 * (dsp.nvIn[I] + dsp.nvIn[J]);
+    }
+    return df;
+}
+
+// the cell's oil normVelocityCell: the largest |darcy[oil] / faceArea| over
+// its faces (tpfalinearizer.hh:759-761, blackoildispersionmodule.hh:374-380)
+__device__ __forceinline__ void maxVelocity(double& nv, const double darcy[3], double area)
+{
+    const double v = fabs(darcy[kCO] / area);
+    nv = nv < v ? v : nv;
 }
 
 // Accumulate one face of cell I into its residual / diagonal registers and
 // store the column block A[J][I] = 0 - dF/dx_I.
-template <bool Diff>
+template <bool Diff, bool Disp>
 __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const IqV& qj, double trans,
                                           double area, double dZg, bool upInTie, double r[3],
                                           double D[9], double* __restrict__ Jout, const BoDiff& dif,
-                                          DiffFace df)
+                                          DiffFace df, const BoDisp& dsp, double& nv)
 {
     E3 fl[3];
-    faceFluxT<false, Diff>(f, qi, qj, trans, area, dZg, upInTie, fl, nullptr, &dif, df);
+    double darcy[3];
+    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp);
+    if constexpr (Disp) maxVelocity(nv, darcy, area);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         r[e] += fl[e].v;
@@ -90,14 +107,14 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
 
 // One (x, y) tile of columns over the planes of one z chunk.  Shared by the
 // single-reservoir kernel and the ensemble kernel (per-instance pointers).
-// Diff: with the diffusive face term (the face diffusivities in dif.diff).
-template <bool Diff>
+// Diff / Disp: with the diffusive / dispersive face term.
+template <bool Diff, bool Disp>
 __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
     double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
-    int bx, int by, int bz, IqLds (*plane)[kTX + 2], const BoDiff& dif)
+    int bx, int by, int bz, IqLds (*plane)[kTX + 2], const BoDiff& dif, const BoDisp& dsp)
 {
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
     const int i = bx * kTX + tx;
@@ -147,6 +164,7 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
             const int mI = mean[I];
             iqEval(f, sw, po, x2, mI, qi);
             double r[3] = {0.0, 0.0, 0.0};
+            double nvo = 0.0;
             double D[9];
 #pragma unroll
             for (int t = 0; t < 9; ++t) D[t] = 0.0;
@@ -154,36 +172,36 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
             // intersection order: -x, +x, -y, +y, -z, +z
             if (i > 0) {
                 IqV q; fromLds(plane[ty + 1][tx], q);
-                applyFace<Diff>(f, qi, q, trans[I - 1], G.area[0], (zI - zI) * G.g, false, r, D,
-                                jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff>(dif, pv, mean, I - 1, q, I - 1));
+                applyFace<Diff, Disp>(f, qi, q, trans[I - 1], G.area[0], (zI - zI) * G.g, false, r, D,
+                                jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - 1, q, I - 1), dsp, nvo);
             }
             if (i + 1 < G.nx) {
                 IqV q; fromLds(plane[ty + 1][tx + 2], q);
-                applyFace<Diff>(f, qi, q, trans[I], G.area[0], (zI - zI) * G.g, true, r, D,
-                                jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff>(dif, pv, mean, I + 1, q, I));
+                applyFace<Diff, Disp>(f, qi, q, trans[I], G.area[0], (zI - zI) * G.g, true, r, D,
+                                jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + 1, q, I), dsp, nvo);
             }
             if (j > 0) {
                 IqV q; fromLds(plane[ty][tx + 1], q);
-                applyFace<Diff>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zI) * G.g, false, r, D,
+                applyFace<Diff, Disp>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zI) * G.g, false, r, D,
                                 jac + 9L * nbOff[2 * N + I], dif,
-                                diffFace<Diff>(dif, pv, mean, I - G.nx, q, N + I - G.nx));
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - G.nx, q, N + I - G.nx), dsp, nvo);
             }
             if (j + 1 < G.ny) {
                 IqV q; fromLds(plane[ty + 2][tx + 1], q);
-                applyFace<Diff>(f, qi, q, trans[N + I], G.area[1], (zI - zI) * G.g, true, r, D,
-                                jac + 9L * nbOff[3 * N + I], dif, diffFace<Diff>(dif, pv, mean, I + G.nx, q, N + I));
+                applyFace<Diff, Disp>(f, qi, q, trans[N + I], G.area[1], (zI - zI) * G.g, true, r, D,
+                                jac + 9L * nbOff[3 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo);
             }
             if (haveBelow) {
                 const double zJ = G.dz * ((double)(k - 1) + 0.5);
-                applyFace<Diff>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zJ) * G.g, false, r, D,
+                applyFace<Diff, Disp>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zJ) * G.g, false, r, D,
                                 jac + 9L * nbOff[4 * N + I], dif,
-                                diffFace<Diff>(dif, pv, mean, I - nxy, below, 2 * N + I - nxy));
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - nxy, below, 2 * N + I - nxy), dsp, nvo);
             }
             if (k + 1 < G.nz) {
                 const double zJ = G.dz * ((double)(k + 1) + 0.5);
-                applyFace<Diff>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zJ) * G.g, true, r, D,
+                applyFace<Diff, Disp>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zJ) * G.g, true, r, D,
                                 jac + 9L * nbOff[5 * N + I], dif,
-                                diffFace<Diff>(dif, pv, mean, I + nxy, above, 2 * N + I));
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + nxy, above, 2 * N + I), dsp, nvo);
             }
             // accumulation term (tpfalinearizer.hh:776-825)
             {
@@ -202,6 +220,7 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
             }
 #pragma unroll
             for (int e = 0; e < 3; ++e) res[3 * I + e] = r[e];
+            if constexpr (Disp) dsp.nvOut[I] = nvo;
             double* Dout = jac + 9L * diagOff[I];
 #pragma unroll
             for (int t = 0; t < 9; ++t) Dout[t] = D[t];
@@ -213,17 +232,17 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     }
 }
 
-template <bool Diff>
+template <bool Diff, bool Disp>
 __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
     double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
-    BoDiff dif)
+    BoDiff dif, BoDisp dsp)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
-    assembleTile<Diff>(f, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, blockIdx.x,
-                       blockIdx.y, blockIdx.z, plane, dif);
+    assembleTile<Diff, Disp>(f, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk,
+                             blockIdx.x, blockIdx.y, blockIdx.z, plane, dif, dsp);
 }
 
 // Cell-wise well sources (tpfalinearizer.hh:827-841), added after the flux
@@ -305,7 +324,7 @@ __global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const 
 // cell, the assembly's face evaluation.
 __global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, const uint8_t* __restrict__ mean,
                            const double* __restrict__ trans, const int* __restrict__ rowPtr,
-                           double* __restrict__ flows, double* __restrict__ flores, BoDiff dif)
+                           double* __restrict__ flows, double* __restrict__ flores, BoDiff dif, BoDisp dsp)
 {
     const long nxy = (long)G.nx * G.ny;
     const long N = nxy * G.nz;
@@ -321,11 +340,18 @@ __global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, c
         iqValues(f, pv[3 * J], pv[3 * J + 1], pv[3 * J + 2], mean[J], qj);
         E3 fl[3];
         double darcy[3];
-        if (dif.diff)
-            faceFluxT<true, true>(f, qi, qj, trans[fi], area, (zI - zJ) * G.g, upInTie, fl, darcy, &dif,
-                                  diffFace<true>(dif, pv, mean, J, qj, fi));
+        const double dZg = (zI - zJ) * G.g;
+        if (dif.diff && dsp.disp)
+            faceFluxT<true, true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
+                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp);
+        else if (dif.diff)
+            faceFluxT<true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
+                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi));
+        else if (dsp.disp)
+            faceFluxT<true, false, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
+                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp);
         else
-            faceFluxT<true>(f, qi, qj, trans[fi], area, (zI - zJ) * G.g, upInTie, fl, darcy);
+            faceFluxT<true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy);
         for (int c = 0; c < 3; ++c) {
             if (flows) flows[3 * e + c] = fl[c].v;
             if (flores) flores[3 * e + c] = darcy[c];
@@ -468,7 +494,7 @@ __global__ void k_bo_update(BoFluid f, long N, double* __restrict__ pv, uint8_t*
 void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
-                      const BoDiff* dif)
+                      const BoDiff* dif, const BoDisp* dsp)
 {
     const int gx = (G.nx + kTX - 1) / kTX, gy = (G.ny + kTY - 1) / kTY;
     // enough workgroups to fill the chip: split the z march into chunks
@@ -476,12 +502,21 @@ void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const
     gz = gz < 1 ? 1 : (gz > G.nz ? G.nz : gz);
     const int kChunk = (G.nz + gz - 1) / gz;
     gz = (G.nz + kChunk - 1) / kChunk;
-    if (dif && dif->diff)
-        hipLaunchKernelGGL(k_bo_assemble<true>, dim3(gx, gy, gz), dim3(kTX * kTY), 0, s, f, G, pv, mean, poro,
-                           trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, *dif);
+    const BoDiff d = (dif && dif->diff) ? *dif : BoDiff{};
+    const BoDisp e = (dsp && dsp->disp) ? *dsp : BoDisp{};
+    const dim3 grid(gx, gy, gz), block(kTX * kTY);
+    if (d.diff && e.disp)
+        hipLaunchKernelGGL((k_bo_assemble<true, true>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
+    else if (d.diff)
+        hipLaunchKernelGGL((k_bo_assemble<true, false>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
+    else if (e.disp)
+        hipLaunchKernelGGL((k_bo_assemble<false, true>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
     else
-        hipLaunchKernelGGL(k_bo_assemble<false>, dim3(gx, gy, gz), dim3(kTX * kTY), 0, s, f, G, pv, mean, poro,
-                           trans, st1, storefac, res, jac, diagOff, nbOff, kChunk, BoDiff{});
+        hipLaunchKernelGGL((k_bo_assemble<false, false>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
 }
 
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
@@ -503,12 +538,13 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
 }
 
 void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
-                   const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif)
+                   const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif,
+                   const BoDisp* dsp)
 {
     const long N = (long)G.nx * G.ny * G.nz;
     if (N <= 0) return;
     hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, f, G, pv, mean, trans, rowPtr,
-                       flows, flores, dif ? *dif : BoDiff{});
+                       flows, flores, dif ? *dif : BoDiff{}, dsp ? *dsp : BoDisp{});
 }
 
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
@@ -549,8 +585,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_e_assemble(BoFluid f, const EnsIn
     if (!active[t.x]) return;
     const EnsInst& I = insts[t.x];
     const int bx = t.y % I.gx, by = (t.y / I.gx) % I.gy, bz = t.y / (I.gx * I.gy);
-    assembleTile<false>(f, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac, I.diagOff,
-                        I.nbOff, I.kChunk, bx, by, bz, plane, BoDiff{});
+    assembleTile<false, false>(f, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac,
+                               I.diagOff, I.nbOff, I.kChunk, bx, by, bz, plane, BoDiff{}, BoDisp{});
 }
 
 __global__ void k_e_sources(BoFluid f, const EnsInst* __restrict__ insts, const int2* __restrict__ table, int n,

@@ -230,9 +230,21 @@ struct BoDiff {
     double toFrac, effOil, effGas;
     const double* diff;
 };
-// the neighbour's side of one face: So * 1/Bo and the face's diffusivity
+// Dispersion in the oil phase (BlackOilDispersionModule): toMassFrac is
+// rho_o / rho_g (toMassFractionGasOil, blackoildispersionmodule.hh:258-262);
+// disp the problem's dispersivity per face ([3][N] like the
+// transmissibilities); nvIn each cell's oil normVelocityCell of the previous
+// linearization, nvOut where this linearization writes its own.
+struct BoDisp {
+    double toMassFrac;
+    const double* disp;
+    double* nvIn;            // read only by the kernels
+    double* nvOut;
+};
+// the neighbour's side of one face: So * 1/Bo and the face's diffusivity;
+// the face's dispersivity and the mean of the two cells' oil velocities
 struct DiffFace {
-    double sbEx, diffusivity;
+    double sbEx, diffusivity, dispersivity, nvAvg;
 };
 
 // One face seen from its interior cell I (derivatives w.r.t. I only), times
@@ -241,11 +253,12 @@ struct DiffFace {
 // WithDarcy: also the phases' reservoir-volume rates across the face,
 // darcyFlux.value() * faceArea at the phase's component index (the FLORES
 // values, blackoillocalresidualtpfa.hh:376-385)
-// WithDiff: plus the diffusive term of the oil phase (see addDiffusion).
-template <bool WithDarcy, bool WithDiff = false>
+// WithDiff / WithDisp: plus the diffusive / dispersive term of the oil phase.
+template <bool WithDarcy, bool WithDiff = false, bool WithDisp = false>
 __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
                                           double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3],
-                                          const BoDiff* dif = nullptr, DiffFace df = DiffFace{0.0, 0.0})
+                                          const BoDiff* dif = nullptr, DiffFace df = DiffFace{0.0, 0.0, 0.0, 0.0},
+                                          const BoDisp* dsp = nullptr)
 {
     flux[0] = ec(0.0); flux[1] = ec(0.0); flux[2] = ec(0.0);
     if constexpr (WithDarcy) { darcyOut[0] = 0.0; darcyOut[1] = 0.0; darcyOut[2] = 0.0; }
@@ -295,6 +308,22 @@ __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const
             flux[kCO] = flux[kCO] + (((-bSAvg * convFactor) * diffR) * tmpDiff) * dif->effOil;
             flux[kCG] = flux[kCG] + (((bSAvg * diffR) * convFactor) * tmpDiff) * dif->effGas;
         }
+    }
+    // BlackOilDispersionModule::addDispersiveFlux (blackoildispersionmodule.hh:176-254)
+    // after the diffusive term (blackoillocalresidualtpfa.hh:481-493): the oil
+    // phase only (no gas in water; the gas phase is disabled, :193-197), with
+    // the mean 1/Bo, the mean of the two cells' normVelocityCell (:484-506) and
+    // the face's dispersivity per area.
+    if constexpr (WithDisp) {
+        E3 bAvg = in.invB[kO];
+        bAvg = bAvg + ex.invB[kO];
+        bAvg = bAvg * 0.5;
+        const E3 rsAvg = (in.Rs + ex.Rs) * 0.5;
+        const E3 convFactor = ec(1.0) / (rsAvg + dsp->toMassFrac);
+        const E3 diffR = in.Rs - ex.Rs;
+        const double tmpDisp = df.dispersivity / area;
+        flux[kCO] = flux[kCO] + ((((-bAvg * df.nvAvg) * convFactor) * tmpDisp) * diffR);
+        flux[kCG] = flux[kCG] + ((((bAvg * df.nvAvg) * convFactor) * tmpDisp) * diffR);
     }
     flux[0] = flux[0] * area; flux[1] = flux[1] * area; flux[2] = flux[2] * area;
 }

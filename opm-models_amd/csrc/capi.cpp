@@ -301,6 +301,25 @@ int omodels_model_set_diffusion(omodels_model* h, const omodels_diffusion* d)
     });
 }
 
+int omodels_model_set_dispersion(omodels_model* h, const omodels_dispersion* d)
+{
+    return guarded([&] {
+        h->m->setDispersion(d);
+        return 0;
+    });
+}
+
+int omodels_model_get_cell_velocity(omodels_model* h, double* velocity)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (!m.dispersion.disp) throw std::invalid_argument("dispersion is off");
+        OM_HIP(hipMemcpyAsync(velocity, m.dispersion.nvIn, m.N * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+        m.sync();
+        return 0;
+    });
+}
+
 // ---- output and restart files ----
 namespace {
 const char* const kOutNames[kOutFields] = {

@@ -96,8 +96,10 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     const double storefac = G.V / dt;
     timer.begin(kClsAssemble, stream);
     launchBoAssemble(fluid, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
-                     nbOff.p, stream, &diffusion);
+                     nbOff.p, stream, &diffusion, &dispersion);
     timer.end(stream);
+    // the velocities just written are the next linearization's input
+    if (dispersion.disp) std::swap(dispersion.nvIn, dispersion.nvOut);
     // cell-wise sources (:827-841), or the same terms as sparse blocks added
     // after the cell loop (:842-846); both add each cell's term last
     if (!separateSparseSources)
@@ -127,7 +129,7 @@ void Model::addSparseSources(int n, const int* cells, const double* r, const dou
 void Model::computeFlows(double* flows, double* flores)
 {
     timer.begin(kClsOther, stream);
-    launchBoFlows(fluid, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion);
+    launchBoFlows(fluid, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion, &dispersion);
     timer.end(stream);
     OM_HIP(hipGetLastError());
 }
@@ -168,6 +170,47 @@ void Model::setDiffusion(const omodels_diffusion* d)
     diffusion.effOil = 0.5 * (d->oil_in_oil + d->oil_in_oil);
     diffusion.effGas = 0.5 * (d->gas_in_oil + d->gas_in_oil);
     diffusion.diff = faceDiff.p;
+    sync();
+}
+
+void Model::setDispersion(const omodels_dispersion* d)
+{
+    if (!d) {
+        dispersion = BoDisp{};
+        faceDisp.release(); nvA.release(); nvB.release();
+        return;
+    }
+    if (!d->face_dispersivity && !d->cell_dispersion)
+        throw std::invalid_argument("dispersion needs face dispersivities or cell coefficients");
+    std::vector<double> fd(3 * (size_t)N, 0.0);
+    if (d->face_dispersivity) {
+        std::copy(d->face_dispersivity, d->face_dispersivity + 3 * N, fd.begin());
+    } else {
+        // harmonic mean of the half-face values A E |n.d| / |d|^2
+        const double* E = d->cell_dispersion;
+        const double h[3] = {G.dx, G.dy, G.dz};
+        const long stride[3] = {1, (long)G.nx, (long)G.nx * G.ny};
+        const int n[3] = {G.nx, G.ny, G.nz};
+        for (long I = 0; I < N; ++I) {
+            const long ijk[3] = {I % G.nx, (I / G.nx) % G.ny, I / ((long)G.nx * G.ny)};
+            for (int ax = 0; ax < 3; ++ax) {
+                if (ijk[ax] + 1 >= n[ax]) continue;
+                const long J = I + stride[ax];
+                const double A = G.area[ax], ndot = 0.5 * h[ax], dist2 = ndot * ndot;
+                const double E0 = A * E[I] * ndot / dist2;
+                const double E1 = A * E[J] * ndot / dist2;
+                fd[(size_t)ax * N + I] = E0 * E1 / (E0 + E1);
+            }
+        }
+    }
+    faceDisp.upload(fd.data(), fd.size(), stream);
+    nvA.alloc(N); nvB.alloc(N);
+    OM_HIP(hipMemsetAsync(nvA.p, 0, N * sizeof(double), stream));
+    OM_HIP(hipMemsetAsync(nvB.p, 0, N * sizeof(double), stream));
+    dispersion.toMassFrac = fluid.rho[kO] / fluid.rho[kG];
+    dispersion.disp = faceDisp.p;
+    dispersion.nvIn = nvA.p;
+    dispersion.nvOut = nvB.p;
     sync();
 }
 

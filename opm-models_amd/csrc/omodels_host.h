@@ -101,6 +101,12 @@ public:
     void setDiffusion(const omodels_diffusion* d);
     DevBuf<double> faceDiff;     // [3][N], at the transmissibility's index
     BoDiff diffusion{};          // diffusion.diff == nullptr: off
+    // BlackOilDispersionModule in the face term; d == nullptr switches it off.
+    // Switching it on zeroes the cells' oil velocities, which every
+    // linearization then replaces for the next one.
+    void setDispersion(const omodels_dispersion* d);
+    DevBuf<double> faceDisp, nvA, nvB;
+    BoDisp dispersion{};         // dispersion.disp == nullptr: off
     void ensureJacobian();
     void eraseMatrix() { jac.release(); }
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)

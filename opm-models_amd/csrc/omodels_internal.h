@@ -139,7 +139,7 @@ struct KernelTimer {
 void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
-                      const BoDiff* dif = nullptr);
+                      const BoDiff* dif = nullptr, const BoDisp* dsp = nullptr);
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                      int nsrc, const int* cell, const int* type, const double* a, const double* b,
                      double* res, double* jac, const int* diagOff, hipStream_t s);
@@ -149,7 +149,8 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
                           double* outRes, double* outDiag, hipStream_t s);
 void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
-                   const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif = nullptr);
+                   const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif = nullptr,
+                   const BoDisp* dsp = nullptr);
 constexpr int kOutFields = 18;
 void launchBoOutput(const BoFluid& f, long N, const double* pv, const uint8_t* mean, const double* poro,
                     double* out, hipStream_t s);

@@ -36,6 +36,11 @@ class Diffusion(C.Structure):
                 ("mw_gas", C.c_double), ("face_diffusivity", _dp)]
 
 
+class Dispersion(C.Structure):
+    """omodels_dispersion: dispersion in the oil phase."""
+    _fields_ = [("face_dispersivity", _dp), ("cell_dispersion", _dp)]
+
+
 class BlackOilSpec(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("field", C.c_int32),
@@ -112,6 +117,8 @@ SIGNATURES = {
     "omodels_linearizer_add_sparse_sources": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
     "omodels_linearizer_flows": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_model_set_diffusion": (C.c_int, [C.c_void_p, C.POINTER(Diffusion)]),
+    "omodels_model_set_dispersion": (C.c_int, [C.c_void_p, C.POINTER(Dispersion)]),
+    "omodels_model_get_cell_velocity": (C.c_int, [C.c_void_p, _dp]),
     "omodels_output_field_count": (C.c_int, []),
     "omodels_output_field_name": (C.c_char_p, [C.c_int32]),
     "omodels_output_fields": (C.c_int, [C.c_void_p, _dp]),
@@ -459,6 +466,30 @@ class Model:
                 raise ValueError("face_diffusivity must hold 3*N values")
             d.face_diffusivity = ptr(fd)
         check(lib().omodels_model_set_diffusion(self.h, C.byref(d)), "set_diffusion")
+
+    def set_dispersion(self, face_dispersivity=None, cell_dispersion=None, off=False):
+        """Dispersion in the oil phase for the following linearizations (face
+        dispersivities [3][N] or cell coefficients [N]); off=True switches it off."""
+        if off:
+            check(lib().omodels_model_set_dispersion(self.h, None), "set_dispersion")
+            return
+        d = Dispersion(None, None)
+        keep = []
+        for name, a, n in (("face_dispersivity", face_dispersivity, 3 * self.N),
+                           ("cell_dispersion", cell_dispersion, self.N)):
+            if a is not None:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                if a.size != n:
+                    raise ValueError("%s must hold %d values" % (name, n))
+                setattr(d, name, ptr(a))
+                keep.append(a)
+        check(lib().omodels_model_set_dispersion(self.h, C.byref(d)), "set_dispersion")
+
+    def cell_velocity(self):
+        """The cells' oil velocities the next linearization uses (dispersion on)."""
+        v = np.zeros(self.N)
+        check(lib().omodels_model_get_cell_velocity(self.h, ptr(v)), "get_cell_velocity")
+        return v
 
     def set_profiling(self, enable=True):
         check(lib().omodels_model_set_profiling(self.h, int(enable)), "set_profiling")

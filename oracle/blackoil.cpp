@@ -204,7 +204,7 @@ struct Grid {
     }
 };
 
-struct NbInfo { long J; int off; double trans, area, dZg; int dir; double diffusivity; };
+struct NbInfo { long J; int off; double trans, area, dZg; int dir; double diffusivity, dispersivity; };
 
 struct Pattern {
     std::vector<int32_t> rowPtr, colIdx, diagOff, nbPtr;
@@ -227,7 +227,7 @@ void buildPattern(const Grid& G, Pattern& P)
             sp[I].insert(J);
             NbInfo n;
             n.J = J; n.off = -1; n.trans = G.trans(I, J, dir); n.area = G.area(dir);
-            n.dZg = (G.zc(I) - G.zc(J)) * G.g; n.dir = dir; n.diffusivity = 0.0;
+            n.dZg = (G.zc(I) - G.zc(J)) * G.g; n.dir = dir; n.diffusivity = 0.0; n.dispersivity = 0.0;
             P.nb.push_back(n);
         }
         P.nbPtr[I + 1] = (int32_t)P.nb.size();
@@ -279,6 +279,13 @@ struct Diffusion {
     double toFrac, effOil, effGas;
 };
 
+/* Dispersion in the oil phase: toMassFrac = rho_o / rho_g
+ * (toMassFractionGasOil, blackoildispersionmodule.hh:258-262); nvI / nvJ the
+ * two cells' oil normVelocityCell of the previous linearization. */
+struct Dispersion {
+    double toMassFrac;
+};
+
 /* calculateFluxes_ (blackoillocalresidualtpfa.hh:325-504) with the
  * pressure difference of transfluxmodule.hh:254-303 and evalPhaseFluxes_
  * (:752-811), followed by the caller's multiplication with the face area
@@ -287,7 +294,8 @@ struct Diffusion {
  * component index, the FLORES values (blackoillocalresidualtpfa.hh:376-385) */
 void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, long J,
               const NbInfo& nbi, double Vin, double Vex, E3 flux[3], double* darcyOut = nullptr,
-              const Diffusion* dif = nullptr)
+              const Diffusion* dif = nullptr, const Dispersion* dsp = nullptr, double nvI = 0.0,
+              double nvJ = 0.0)
 {
     for (int e = 0; e < 3; ++e) flux[e] = E3(0.0);
     if (darcyOut) for (int e = 0; e < 3; ++e) darcyOut[e] = 0.0;
@@ -338,6 +346,23 @@ void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, lo
             flux[CG] = flux[CG] + (((bSAvg * diffR) * convFactor) * tmpDiff) * dif->effGas;
         }
     }
+    /* BlackOilDispersionModule::addDispersiveFlux (blackoildispersionmodule.hh:176-254)
+     * after the diffusive term (blackoillocalresidualtpfa.hh:481-493): the oil
+     * phase only (no gas in water; the gas phase is disabled, :193-197), with
+     * the mean 1/Bo, the mean of the two cells' normVelocityCell (:484-506)
+     * and the face's dispersivity per area. */
+    if (dsp) {
+        E3 bAvg = in.invB[O];
+        bAvg = bAvg + ex.invB[O];
+        bAvg = bAvg / 2.0;
+        const E3 rsAvg = (in.Rs + ex.Rs) / 2.0;
+        const E3 convFactor = E3(1.0) / (rsAvg + dsp->toMassFrac);
+        const E3 diffR = in.Rs - ex.Rs;
+        const double nvAvg = 0.5 * (nvI + nvJ);
+        const double tmpDisp = nbi.dispersivity / nbi.area;
+        flux[CO] = flux[CO] + ((((-bAvg * nvAvg) * convFactor) * tmpDisp) * diffR);
+        flux[CG] = flux[CG] + ((((bAvg * nvAvg) * convFactor) * tmpDisp) * diffR);
+    }
     for (int e = 0; e < 3; ++e) flux[e] = flux[e] * nbi.area;
 }
 
@@ -367,14 +392,43 @@ struct Problem {
     std::vector<int> srcOfCell;
     bool hasDiff = false;
     Diffusion dif{};
-    explicit Problem(const ob_problem* p, const ob_diffusion* d = nullptr) : P(p), G(p)
+    bool hasDisp = false;
+    Dispersion dsp{};
+    explicit Problem(const ob_problem* p, const ob_diffusion* d = nullptr, const ob_dispersion* e = nullptr)
+        : P(p), G(p)
     {
         pvt.init(p);
         buildPattern(G, pat);
         srcOfCell.assign(G.N, -1);
         for (int s = 0; s < p->n_src; ++s) srcOfCell[p->src_cell[s]] = s;
         if (d) setDiffusion(*d);
+        if (e) setDispersion(*e);
     }
+    /* the problem's dispersivity(I, J) per face (tpfalinearizer.hh:465-467):
+     * given per face, or the harmonic mean of A E |n.d| / |d|^2 of the two
+     * cells' dispersion coefficients E */
+    void setDispersion(const ob_dispersion& e)
+    {
+        if (!e.face_dispersivity && !e.cell_dispersion)
+            throw std::invalid_argument("dispersion needs face dispersivities or cell coefficients");
+        dsp.toMassFrac = pvt.rho[O] / pvt.rho[::G];
+        hasDisp = true;
+        for (long I = 0; I < G.N; ++I)
+            for (int k = pat.nbPtr[I]; k < pat.nbPtr[I + 1]; ++k) {
+                NbInfo& n = pat.nb[k];
+                const int ax = n.dir / 2;
+                const long lo = I < n.J ? I : n.J, hi = I < n.J ? n.J : I;
+                if (e.face_dispersivity) {
+                    n.dispersivity = e.face_dispersivity[(long)ax * G.N + lo];
+                } else {
+                    const double A = G.area(n.dir), ndot = 0.5 * G.h(n.dir), dist2 = ndot * ndot;
+                    const double E0 = A * e.cell_dispersion[lo] * ndot / dist2;
+                    const double E1 = A * e.cell_dispersion[hi] * ndot / dist2;
+                    n.dispersivity = E0 * E1 / (E0 + E1);
+                }
+            }
+    }
+    const Dispersion* dispersion() const { return hasDisp ? &dsp : nullptr; }
     /* the problem's diffusivity(I, J) per face (tpfalinearizer.hh:462-464):
      * given per face, or the harmonic mean of the half-face values
      * A phi |n.d| / |d|^2 (the transmissibility form with porosity in place
@@ -421,8 +475,14 @@ void computeStorageAll(const Problem& pr, const double* pv, const uint8_t* mean,
 /* TpfaLinearizer::linearize_ (tpfalinearizer.hh:711-866): resetSystem_ then
  * one pass over cells, each cell owning r[I], A[I][I] and the column blocks
  * A[J][I] of its neighbours. */
+/* With dispersion, nvIn holds each cell's oil normVelocityCell from the
+ * previous linearization (zeros before the first, createFlows_,
+ * tpfalinearizer.hh:571-619) and nvOut receives this linearization's: the
+ * largest |darcy[oil] / faceArea| over the cell's faces (:759-761 with
+ * blackoildispersionmodule.hh:355-383). */
 void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const double* st1,
-               double dt, double* res, double* jac, int nthreads)
+               double dt, double* res, double* jac, int nthreads, const double* nvIn = nullptr,
+               double* nvOut = nullptr)
 {
     const long N = pr.G.N;
     const long nnzb = pr.pat.rowPtr[N];
@@ -438,12 +498,21 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
         computeIQ<E3>(pr.pvt, pv + 3 * I, mean[I], pr.P->poro[I], qi);
         double* r = res + 3 * I;
         double* D = jac + 9L * pr.pat.diagOff[I];
+        if (pr.hasDisp) nvOut[I] = 0.0;
         for (int k = pr.pat.nbPtr[I]; k < pr.pat.nbPtr[I + 1]; ++k) {
             const NbInfo& nbi = pr.pat.nb[k];
             IQ<double> qj;
             computeIQ<double>(pr.pvt, pv + 3 * nbi.J, mean[nbi.J], pr.P->poro[nbi.J], qj);
             E3 f[3];
-            faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, nullptr, pr.diffusion());
+            if (pr.hasDisp) {
+                double darcy[3];
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, darcy, pr.diffusion(), &pr.dsp, nvIn[I],
+                         nvIn[nbi.J]);
+                const double v = std::fabs(darcy[CO] / nbi.area);
+                nvOut[I] = nvOut[I] < v ? v : nvOut[I];
+            } else {
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, nullptr, pr.diffusion());
+            }
             double* O_ = jac + 9L * nbi.off;
             for (int e = 0; e < 3; ++e) {
                 r[e] += f[e].v;
@@ -935,14 +1004,15 @@ int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meani
  * (FLOWS) and the phases' reservoir-volume rates (FLORES), [entries][3]. */
 int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, double* flows, double* flores)
 {
-    return ob_flows_diff(p, nullptr, pv, meaning, flows, flores);
+    return ob_flows_ext(p, nullptr, nullptr, nullptr, pv, meaning, flows, flores);
 }
 
-int ob_flows_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
-                  double* flows, double* flores)
+int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, const double* nv,
+                 const double* pv, const uint8_t* meaning, double* flows, double* flores)
 {
     try {
-        Problem pr(p, dif);
+        Problem pr(p, dif, dsp);
+        if (dsp && !nv) throw std::invalid_argument("dispersion needs the cells' velocities");
         const double V = pr.G.volume();
         for (long I = 0; I < pr.G.N; ++I) {
             IQ<E3> qi;
@@ -953,7 +1023,8 @@ int ob_flows_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv
                 computeIQ<double>(pr.pvt, pv + 3 * nbi.J, meaning[nbi.J], p->poro[nbi.J], qj);
                 E3 f[3];
                 double d[3];
-                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, d, pr.diffusion());
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, d, pr.diffusion(), pr.dispersion(),
+                         dsp ? nv[I] : 0.0, dsp ? nv[nbi.J] : 0.0);
                 for (int e = 0; e < 3; ++e) {
                     if (flows) flows[3L * k + e] = f[e].v;
                     if (flores) flores[3L * k + e] = d[e];
@@ -967,15 +1038,18 @@ int ob_flows_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv
 int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  const double* storage1, double dt, double* residual, double* jac, int nthreads)
 {
-    return ob_linearize_diff(p, nullptr, pv, meaning, storage1, dt, residual, jac, nthreads);
+    return ob_linearize_ext(p, nullptr, nullptr, nullptr, nullptr, pv, meaning, storage1, dt, residual, jac,
+                            nthreads);
 }
 
-int ob_linearize_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
-                      const double* storage1, double dt, double* residual, double* jac, int nthreads)
+int ob_linearize_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, const double* nv_in,
+                     double* nv_out, const double* pv, const uint8_t* meaning, const double* storage1, double dt,
+                     double* residual, double* jac, int nthreads)
 {
     try {
-        Problem pr(p, dif);
-        linearize(pr, pv, meaning, storage1, dt, residual, jac, nthreads);
+        Problem pr(p, dif, dsp);
+        if (dsp && (!nv_in || !nv_out)) throw std::invalid_argument("dispersion needs the cells' velocities");
+        linearize(pr, pv, meaning, storage1, dt, residual, jac, nthreads, nv_in, nv_out);
         return 0;
     } catch (...) { return -1; }
 }
@@ -1034,17 +1108,19 @@ int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps, 
                 const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
                 int log_cap, int32_t* log_len, double* timers, int nthreads)
 {
-    return ob_simulate_diff(p, nullptr, pv, meaning, n_steps, dt0, prm, pv_hist, dt_hist, log, log_cap, log_len,
-                            timers, nthreads);
+    return ob_simulate_ext(p, nullptr, nullptr, pv, meaning, n_steps, dt0, prm, pv_hist, dt_hist, log, log_cap,
+                           log_len, timers, nthreads);
 }
 
-int ob_simulate_diff(const ob_problem* p, const ob_diffusion* dif, double* pv, uint8_t* meaning, int n_steps,
-                     double dt0, const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
-                     int log_cap, int32_t* log_len, double* timers, int nthreads)
+int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, double* pv,
+                    uint8_t* meaning, int n_steps, double dt0, const ob_params* prm, double* pv_hist,
+                    double* dt_hist, ob_iter_log* log, int log_cap, int32_t* log_len, double* timers, int nthreads)
 {
     using clk = std::chrono::steady_clock;
     try {
-        Problem pr(p, dif);
+        Problem pr(p, dif, dsp);
+        /* the cells' oil velocities of the last linearization, zeros before the first */
+        std::vector<double> nvPrev(dsp ? pr.G.N : 0, 0.0), nvNext(dsp ? pr.G.N : 0, 0.0);
         const long N = pr.G.N;
         const int32_t* rp = pr.pat.rowPtr.data();
         const int32_t* ci = pr.pat.colIdx.data();
@@ -1076,7 +1152,9 @@ int ob_simulate_diff(const ob_problem* p, const ob_diffusion* dif, double* pv, u
                     std::copy(meaning, meaning + N, curMean.begin());
                     auto a = clk::now();
                     if (numIter == 0) computeStorageAll(pr, cur.data(), curMean.data(), st1.data());
-                    linearize(pr, cur.data(), curMean.data(), st1.data(), dt, res.data(), jac.data(), nthreads);
+                    linearize(pr, cur.data(), curMean.data(), st1.data(), dt, res.data(), jac.data(), nthreads,
+                              nvPrev.data(), nvNext.data());
+                    nvPrev.swap(nvNext);
                     auto b = clk::now();
                     tLin += std::chrono::duration<double>(b - a).count();
                     /* preSolve_ */

@@ -90,6 +90,15 @@ typedef struct ob_diffusion {
     const double* face_diffusivity; /* [3][N], face I|I+e_axis at axis*N + I; NULL: from porosity */
 } ob_diffusion;
 
+/* Dispersion in the oil phase (BlackOilDispersionModule,
+ * blackoildispersionmodule.hh:176-254, face term of
+ * blackoillocalresidualtpfa.hh:481-493), driven by each cell's oil velocity
+ * of the previous linearization. */
+typedef struct ob_dispersion {
+    const double* face_dispersivity; /* [3][N] like face_diffusivity, or NULL */
+    const double* cell_dispersion;   /* [N] dispersion coefficients [m], used when the face array is NULL */
+} ob_dispersion;
+
 /* ---- A1: pattern, neighbour info, block addresses ---- */
 int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2);
 int ob_build_pattern(const ob_problem* p, int32_t* row_ptr, int32_t* col_idx,
@@ -105,11 +114,14 @@ int ob_linearize(const ob_problem* p, const double* pv, const uint8_t* meaning,
                  double* jac /* [NNZB][3][3] */, int nthreads);
 /* face rates (FLOWS, FLORES) per neighbour entry of the pattern, [entries][3] */
 int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, double* flows, double* flores);
-/* the same three with the diffusive face term (dif may be NULL) */
-int ob_linearize_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
-                      const double* storage1, double dt, double* residual, double* jac, int nthreads);
-int ob_flows_diff(const ob_problem* p, const ob_diffusion* dif, const double* pv, const uint8_t* meaning,
-                  double* flows, double* flores);
+/* the same with the diffusive and the dispersive face terms (dif, dsp may be
+ * NULL).  With dsp: nv_in [N] the cells' oil velocities of the previous
+ * linearization, nv_out [N] receives this one's. */
+int ob_linearize_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, const double* nv_in,
+                     double* nv_out, const double* pv, const uint8_t* meaning, const double* storage1, double dt,
+                     double* residual, double* jac, int nthreads);
+int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, const double* nv,
+                 const double* pv, const uint8_t* meaning, double* flows, double* flores);
 /* source terms as sparse blocks: cells [n_src], residual [n_src][3] (times -V),
  * derivatives [n_src][3][3]; returns n_src */
 int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cells,
@@ -143,9 +155,9 @@ int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps,
                 int32_t* log_len, double* timers /* [4] linearize, solve, update, total */,
                 int nthreads);
 
-int ob_simulate_diff(const ob_problem* p, const ob_diffusion* dif, double* pv, uint8_t* meaning, int n_steps,
-                     double dt0, const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log,
-                     int log_cap, int32_t* log_len, double* timers, int nthreads);
+int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, double* pv,
+                    uint8_t* meaning, int n_steps, double dt0, const ob_params* prm, double* pv_hist,
+                    double* dt_hist, ob_iter_log* log, int log_cap, int32_t* log_len, double* timers, int nthreads);
 
 void ob_default_params(ob_params* prm);
 

@@ -46,6 +46,22 @@ def diffusion(oil_in_oil, gas_in_oil, mw_oil=0.0, mw_gas=0.0, face_diffusivity=N
     return d
 
 
+class ObDispersion(C.Structure):
+    """ob_dispersion (oracle.h), same layout as omodels_dispersion."""
+    _fields_ = [("face_dispersivity", _dp), ("cell_dispersion", _dp)]
+
+
+def dispersion(face_dispersivity=None, cell_dispersion=None):
+    d = ObDispersion(None, None)
+    d._keep = []
+    for name, a in (("face_dispersivity", face_dispersivity), ("cell_dispersion", cell_dispersion)):
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            setattr(d, name, a.ctypes.data_as(_dp))
+            d._keep.append(a)
+    return d
+
+
 class ObParams(C.Structure):
     _fields_ = [
         ("newton_tol", C.c_double), ("newton_max_iter", C.c_int32), ("newton_target_iter", C.c_int32),
@@ -101,11 +117,13 @@ def lib():
         L.ob_simulate.argtypes = [C.POINTER(ObProblem), _dp, _u8p, C.c_int, C.c_double, C.POINTER(ObParams),
                                   _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp, C.c_int]
         _dif = C.POINTER(ObDiffusion)
-        L.ob_linearize_diff.argtypes = [C.POINTER(ObProblem), _dif, _dp, _u8p, _dp, C.c_double, _dp, _dp, C.c_int]
-        L.ob_flows_diff.argtypes = [C.POINTER(ObProblem), _dif, _dp, _u8p, _dp, _dp]
-        L.ob_simulate_diff.argtypes = [C.POINTER(ObProblem), _dif, _dp, _u8p, C.c_int, C.c_double,
-                                       C.POINTER(ObParams), _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp,
-                                       C.c_int]
+        _dsp = C.POINTER(ObDispersion)
+        L.ob_linearize_ext.argtypes = [C.POINTER(ObProblem), _dif, _dsp, _dp, _dp, _dp, _u8p, _dp, C.c_double, _dp,
+                                       _dp, C.c_int]
+        L.ob_flows_ext.argtypes = [C.POINTER(ObProblem), _dif, _dsp, _dp, _dp, _u8p, _dp, _dp]
+        L.ob_simulate_ext.argtypes = [C.POINTER(ObProblem), _dif, _dsp, _dp, _u8p, C.c_int, C.c_double,
+                                      C.POINTER(ObParams), _dp, _dp, C.POINTER(ObIterLog), C.c_int, _ip, _dp,
+                                      C.c_int]
         L.ol_default_problem.argtypes = [C.POINTER(OlProblem)]
         L.ol_num_cells.argtypes = [C.POINTER(OlProblem), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.ol_pattern.argtypes = [C.POINTER(OlProblem), _ip, _ip, _ip]
@@ -169,17 +187,23 @@ def _dif(d):
     return C.byref(d) if d is not None else None
 
 
-def linearize(prob, pv, mean, st1, dt, nthreads=1, diffusion=None):
+def linearize(prob, pv, mean, st1, dt, nthreads=1, diffusion=None, dispersion=None, nv=None):
+    """(J, r); with dispersion: (J, r, nv_out), nv the cells' oil velocities of
+    the previous linearization (zeros if None)."""
     N = prob.nx * prob.ny * prob.nz
     nz = C.c_int64()
     f2 = C.c_int64()
     lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
     r = np.zeros(3 * N)
     J = np.zeros(9 * nz.value)
-    rc = lib().ob_linearize_diff(C.byref(prob), _dif(diffusion), P(pv), P(mean, _u8p), P(st1), dt, P(r), P(J),
-                                 nthreads)
+    nv_in = nv_out = None
+    if dispersion is not None:
+        nv_in = np.zeros(N) if nv is None else np.ascontiguousarray(nv, dtype=np.float64)
+        nv_out = np.zeros(N)
+    rc = lib().ob_linearize_ext(C.byref(prob), _dif(diffusion), _dif(dispersion), P(nv_in), P(nv_out), P(pv),
+                                P(mean, _u8p), P(st1), dt, P(r), P(J), nthreads)
     assert rc == 0
-    return J, r
+    return (J, r) if dispersion is None else (J, r, nv_out)
 
 
 def source_blocks(prob, pv, mean):
@@ -193,7 +217,7 @@ def source_blocks(prob, pv, mean):
     return cells[:n], r[:3 * n], d[:9 * n]
 
 
-def flows(prob, pv, mean, diffusion=None):
+def flows(prob, pv, mean, diffusion=None, dispersion=None, nv=None):
     """(flows, flores), [neighbour entries][3] in the pattern's order."""
     nz = C.c_int64()
     f2 = C.c_int64()
@@ -202,7 +226,9 @@ def flows(prob, pv, mean, diffusion=None):
     nf = nz.value - N
     fl = np.zeros((nf, 3))
     fr = np.zeros((nf, 3))
-    assert lib().ob_flows_diff(C.byref(prob), _dif(diffusion), P(pv), P(mean, _u8p), P(fl), P(fr)) == 0
+    nv = None if nv is None else np.ascontiguousarray(nv, dtype=np.float64)
+    assert lib().ob_flows_ext(C.byref(prob), _dif(diffusion), _dif(dispersion), P(nv), P(pv), P(mean, _u8p), P(fl),
+                              P(fr)) == 0
     return fl, fr
 
 
@@ -252,7 +278,8 @@ def update_primary(prob, cur, cur_mean, delta, was_switched, prm):
     return rc, nxt, nm, ws, ns.value
 
 
-def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep_history=True, diffusion=None):
+def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep_history=True, diffusion=None,
+             dispersion=None):
     N = prob.nx * prob.ny * prob.nz
     pv = np.array(pv0, dtype=np.float64, copy=True)
     mean = np.array(mean0, dtype=np.uint8, copy=True)
@@ -261,8 +288,8 @@ def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep
     log = (ObIterLog * log_cap)()
     nlog = C.c_int32()
     timers = np.zeros(4)
-    rc = lib().ob_simulate_diff(C.byref(prob), _dif(diffusion), P(pv), P(mean, _u8p), n_steps, dt0, C.byref(prm),
-                                P(hist), P(dts), log, log_cap, C.byref(nlog), P(timers), nthreads)
+    rc = lib().ob_simulate_ext(C.byref(prob), _dif(diffusion), _dif(dispersion), P(pv), P(mean, _u8p), n_steps, dt0,
+                               C.byref(prm), P(hist), P(dts), log, log_cap, C.byref(nlog), P(timers), nthreads)
     recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
             for l in log[:min(nlog.value, log_cap)]]
     return rc, pv, mean, hist, dts, recs, timers

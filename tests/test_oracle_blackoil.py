@@ -699,3 +699,124 @@ def test_diffusion_conserves_and_flows_sum_to_the_residual():
             a = a + fl[k]
         acc[3 * I:3 * I + 3] = a
     assert np.array_equal(acc.view(np.uint64), r.view(np.uint64))
+
+
+# ----------------------------------------------------------------------------
+# Dispersion in the oil phase (blackoildispersionmodule.hh:176-254), driven by
+# each cell's oil velocity of the previous linearization
+def _disp_lin(prob, pv, mean, nv, dsp, dif=None):
+    return orc.linearize(prob, pv, mean, orc.storage(prob, pv, mean), 1e300, diffusion=dif, dispersion=dsp, nv=nv)
+
+
+def test_dispersion_closed_form_between_two_cells():
+    """Two undersaturated cells at one pressure: with velocities v0, v1 of the
+    previous linearization the gas flux is mean(1/Bo) mean(v) (Rs_I - Rs_J)
+    / (rho_o/rho_g + mean Rs) times the face dispersivity, the oil flux its
+    negative; zero velocities (the first linearization) give no flux."""
+    c, prob, pv, mean = _two_cells(140.0, 90.0)
+    E = np.array([12.0, 30.0])
+    dsp = orc.dispersion(cell_dispersion=E)
+    J0, r0, nv0 = _disp_lin(prob, pv, mean, np.zeros(2), dsp)
+    assert np.all(r0 == 0.0) and np.all(nv0 == 0.0)        # no flow, no velocity
+    _, r, _ = _disp_lin(prob, pv, mean, np.array([2e-6, 5e-6]), dsp)
+    qi = orc.intensive(prob, pv[:3], 1, cell=0)
+    qj = orc.intensive(prob, pv[3:], 1, cell=1)
+    b = 0.5 * (qi["invB_o"][0] + qj["invB_o"][0])
+    rho_w, rho_o, rho_g = om.default_fluid()[4][4:7]
+    conv = 1.0 / (rho_o / rho_g + 0.5 * (140.0 + 90.0))
+    A, h = c.spec.dy * c.spec.dz, c.spec.dx
+    e0, e1 = A * E[0] / (h / 2), A * E[1] / (h / 2)
+    gas = b * 0.5 * (2e-6 + 5e-6) * conv * (e0 * e1 / (e0 + e1)) * (140.0 - 90.0)
+    assert gas > 0.0
+    assert r[2] == pytest.approx(gas, rel=1e-13)
+    assert r[0] == pytest.approx(-gas, rel=1e-13)
+    assert r[1] == 0.0
+    assert np.array_equal(r[3:], -r[:3])
+
+
+def test_dispersion_velocity_is_the_largest_oil_darcy_velocity():
+    """The velocity a linearization leaves for the next one is, per cell, the
+    largest |FLORES oil rate / face area| over its faces."""
+    c = _case_nowells(5, 4, 3, seed=11)
+    pv, mean = mixed_state(c, seed=11)
+    prob = orc.problem_from_case(c)
+    dsp = orc.dispersion(cell_dispersion=np.full(c.N, 10.0))
+    _, _, nv = _disp_lin(prob, pv, mean, np.zeros(c.N), dsp)
+    _, fr = orc.flows(prob, pv, mean)
+    pat = orc.pattern(prob)
+    for I in range(c.N):
+        ks = range(pat["nb_ptr"][I], pat["nb_ptr"][I + 1])
+        want = max([abs(fr[k, 0] / pat["nb_area"][k]) for k in ks], default=0.0)
+        assert nv[I] == want
+    assert nv.max() > 0.0
+
+
+def test_dispersion_given_or_derived_and_jacobian():
+    c = _case_nowells(3, 2, 2, seed=13)
+    pv, mean = mixed_state(c, seed=13)
+    prob = orc.problem_from_case(c)
+    N, nx, ny, nz = c.N, 3, 2, 2
+    E = np.random.default_rng(13).uniform(5.0, 50.0, N)
+    h = (c.spec.dx, c.spec.dy, c.spec.dz)
+    area = (c.spec.dy * c.spec.dz, c.spec.dx * c.spec.dz, c.spec.dx * c.spec.dy)
+    fd = np.zeros(3 * N)
+    for I in range(N):
+        ijk = (I % nx, (I // nx) % ny, I // (nx * ny))
+        for ax, (n, stride) in enumerate(((nx, 1), (ny, nx), (nz, nx * ny))):
+            if ijk[ax] + 1 < n:
+                e0 = area[ax] * E[I] * (0.5 * h[ax]) / (0.5 * h[ax]) ** 2
+                e1 = area[ax] * E[I + stride] * (0.5 * h[ax]) / (0.5 * h[ax]) ** 2
+                fd[ax * N + I] = e0 * e1 / (e0 + e1)
+    nv = np.random.default_rng(14).uniform(1e-7, 1e-5, N)
+    Jd, rd, _ = _disp_lin(prob, pv, mean, nv, orc.dispersion(cell_dispersion=E))
+    Jg, rg, _ = _disp_lin(prob, pv, mean, nv, orc.dispersion(face_dispersivity=fd))
+    assert np.array_equal(rd.view(np.uint64), rg.view(np.uint64))
+    assert np.array_equal(Jd.view(np.uint64), Jg.view(np.uint64))
+    # derivatives of the dispersive part against central differences
+    st1 = orc.storage(prob, pv, mean)
+    dsp = orc.dispersion(face_dispersivity=1e4 * fd)
+
+    def part(x):
+        J1, r1, _ = orc.linearize(prob, x, mean, st1, 1e300, dispersion=dsp, nv=nv)
+        J0, r0 = orc.linearize(prob, x, mean, st1, 1e300)
+        return J1 - J0, r1 - r0
+
+    Jp, _ = part(pv)
+    pat = orc.pattern(prob)
+    scale = np.abs(Jp).reshape(-1, 3, 3).max(axis=(0, 1))
+    assert scale[1] > 0.0 and scale[2] > 0.0
+    # round-off floor of the differences of the full residuals
+    rmax = np.abs(orc.linearize(prob, pv, mean, st1, 1e300)[1]).max()
+    steps = [1e-6, 1.0, 1e-6]
+    for col in range(N):
+        for var in range(3):
+            hh = steps[var] if not (var == 2 and mean[col] == 1) else 1e-4
+            pp = pv.copy(); pp[3 * col + var] += hh
+            pm = pv.copy(); pm[3 * col + var] -= hh
+            fdr = (part(pp)[1] - part(pm)[1]) / (2 * hh)
+            for row in range(N):
+                ks = np.where(pat["col_idx"][pat["row_ptr"][row]:pat["row_ptr"][row + 1]] == col)[0]
+                blk = Jp[9 * (pat["row_ptr"][row] + ks[0]):][:9].reshape(3, 3) if len(ks) else np.zeros((3, 3))
+                tol = 1e-5 * scale[var] + 1e-14 * rmax / hh
+                assert np.max(np.abs(blk[:, var] - fdr[3 * row:3 * row + 3])) <= tol, (row, col, var)
+
+
+def test_dispersion_conserves_and_flows_sum_to_the_residual():
+    c = _case_nowells(5, 4, 3, seed=5)
+    pv, mean = mixed_state(c, seed=5)
+    prob = orc.problem_from_case(c)
+    dsp = orc.dispersion(cell_dispersion=np.full(c.N, 1e3))
+    dif = _diff()
+    nv = np.random.default_rng(5).uniform(1e-7, 1e-5, c.N)
+    _, r, _ = _disp_lin(prob, pv, mean, nv, dsp, dif)
+    R = r.reshape(-1, 3)
+    assert np.max(np.abs(R.sum(axis=0))) <= 1e-12 * np.abs(R).sum(axis=0).max()
+    fl, _ = orc.flows(prob, pv, mean, diffusion=dif, dispersion=dsp, nv=nv)
+    nbp = orc.pattern(prob)["nb_ptr"]
+    acc = np.zeros(3 * c.N)
+    for I in range(c.N):
+        a = np.zeros(3)
+        for k in range(nbp[I], nbp[I + 1]):
+            a = a + fl[k]
+        acc[3 * I:3 * I + 3] = a
+    assert np.array_equal(acc.view(np.uint64), r.view(np.uint64))
