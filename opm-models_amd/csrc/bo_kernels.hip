@@ -232,8 +232,12 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     }
 }
 
+// The fluid tables come through a device pointer: passed by value they fill
+// the kernel arguments, and the compiler keeps them in scalar registers that
+// then spill into vector registers; read through the pointer they are scalar
+// loads where used, which leaves the kernel room for two waves per SIMD.
 template <bool Diff, bool Disp>
-__global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
+__global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(const BoFluid* __restrict__ fp, BoGrid G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(BoFluid f, BoGrid G,
     BoDiff dif, BoDisp dsp)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
-    assembleTile<Diff, Disp>(f, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk,
+    assembleTile<Diff, Disp>(*fp, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk,
                              blockIdx.x, blockIdx.y, blockIdx.z, plane, dif, dsp);
 }
 
@@ -491,7 +495,7 @@ __global__ void k_bo_update(BoFluid f, long N, double* __restrict__ pv, uint8_t*
 
 // ---------------------------------------------------------------------------
 // Host launchers.
-void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
                       const BoDiff* dif, const BoDisp* dsp)
@@ -506,16 +510,16 @@ void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const
     const BoDisp e = (dsp && dsp->disp) ? *dsp : BoDisp{};
     const dim3 grid(gx, gy, gz), block(kTX * kTY);
     if (d.diff && e.disp)
-        hipLaunchKernelGGL((k_bo_assemble<true, true>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+        hipLaunchKernelGGL((k_bo_assemble<true, true>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
                            storefac, res, jac, diagOff, nbOff, kChunk, d, e);
     else if (d.diff)
-        hipLaunchKernelGGL((k_bo_assemble<true, false>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+        hipLaunchKernelGGL((k_bo_assemble<true, false>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
                            storefac, res, jac, diagOff, nbOff, kChunk, d, e);
     else if (e.disp)
-        hipLaunchKernelGGL((k_bo_assemble<false, true>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+        hipLaunchKernelGGL((k_bo_assemble<false, true>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
                            storefac, res, jac, diagOff, nbOff, kChunk, d, e);
     else
-        hipLaunchKernelGGL((k_bo_assemble<false, false>), grid, block, 0, s, f, G, pv, mean, poro, trans, st1,
+        hipLaunchKernelGGL((k_bo_assemble<false, false>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
                            storefac, res, jac, diagOff, nbOff, kChunk, d, e);
 }
 
@@ -577,7 +581,8 @@ void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t
 
 namespace omodels {
 
-__global__ __launch_bounds__(kTX * kTY) void k_e_assemble(BoFluid f, const EnsInst* __restrict__ insts,
+// the fluid through a device pointer, as in k_bo_assemble
+__global__ __launch_bounds__(kTX * kTY) void k_e_assemble(const BoFluid* __restrict__ fp, const EnsInst* __restrict__ insts,
     const int2* __restrict__ table, const int* __restrict__ active)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
@@ -585,7 +590,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_e_assemble(BoFluid f, const EnsIn
     if (!active[t.x]) return;
     const EnsInst& I = insts[t.x];
     const int bx = t.y % I.gx, by = (t.y / I.gx) % I.gy, bz = t.y / (I.gx * I.gy);
-    assembleTile<false, false>(f, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac,
+    assembleTile<false, false>(*fp, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac,
                                I.diagOff, I.nbOff, I.kChunk, bx, by, bz, plane, BoDiff{}, BoDisp{});
 }
 
@@ -702,10 +707,11 @@ __global__ __launch_bounds__(kEnsChunk) void k_e_update(BoFluid f, const EnsInst
     if ((threadIdx.x % warpSize) == 0 && ball) atomicAdd(&I.counters[0], __popcll(ball));
 }
 
-void launchEnsAssemble(const BoFluid& f, const EnsInst* insts, const int2* table, int nBlocks, const int* active,
-                       const int2* srcTable, int nSrc, hipStream_t s)
+void launchEnsAssemble(const BoFluid& f, const BoFluid* fdev, const EnsInst* insts, const int2* table, int nBlocks,
+                       const int* active, const int2* srcTable, int nSrc, hipStream_t s)
 {
-    if (nBlocks > 0) hipLaunchKernelGGL(k_e_assemble, dim3(nBlocks), dim3(kTX * kTY), 0, s, f, insts, table, active);
+    if (nBlocks > 0)
+        hipLaunchKernelGGL(k_e_assemble, dim3(nBlocks), dim3(kTX * kTY), 0, s, fdev, insts, table, active);
     if (nSrc > 0)
         hipLaunchKernelGGL(k_e_sources, dim3((nSrc + 63) / 64), dim3(64), 0, s, f, insts, srcTable, nSrc, active);
 }

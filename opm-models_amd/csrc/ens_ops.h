@@ -6,8 +6,9 @@
 
 namespace omodels {
 
-void launchEnsAssemble(const BoFluid& f, const EnsInst* insts, const int2* table, int nBlocks, const int* active,
-                       const int2* srcTable, int nSrc, hipStream_t s);
+// fdev: the same fluid in device memory (read by the assembly)
+void launchEnsAssemble(const BoFluid& f, const BoFluid* fdev, const EnsInst* insts, const int2* table, int nBlocks,
+                       const int* active, const int2* srcTable, int nSrc, hipStream_t s);
 void launchEnsStorage(const BoFluid& f, const EnsInst* insts, const int2* table, int nChunks, const int* refresh,
                       hipStream_t s);
 void launchEnsUpdate(const BoFluid& f, const EnsInst* insts, const int2* table, int nChunks, const int* active,

@@ -77,7 +77,11 @@ int Ensemble::add(const omodels_blackoil_problem* p, const double* pv, const uin
     if ((int)members_.size() >= kMaxMembers) throw std::invalid_argument("ensemble is full");
     if (nSteps < 1 || !(dt0 > 0)) throw std::invalid_argument("invalid step count or dt");
     BoFluid f = makeFluid(p);
-    if (members_.empty() && !fluidSet_) { fluid_ = f; fluidSet_ = true; }
+    if (members_.empty() && !fluidSet_) {
+        fluid_ = f;
+        fluidSet_ = true;
+        fluidDev_.upload(&fluid_, 1, stream_);
+    }
     else if (std::memcmp(&f, &fluid_, sizeof(BoFluid)) != 0)
         throw std::invalid_argument("all members of an ensemble share one fluid description");
     auto m = std::make_unique<Member>();
@@ -460,7 +464,7 @@ void Ensemble::timeStep()
         setFlags(actRefresh_, refresh);
         launchEnsStorage(fluid_, desc_.p, chunkTable_.p, nChunks_, actRefresh_.p, stream_);
         timer.begin(kClsAssemble, stream_);
-        launchEnsAssemble(fluid_, desc_.p, asmTable_.p, nAsm_, actLin_.p, srcTable_.p, nSrc_, stream_);
+        launchEnsAssemble(fluid_, fluidDev_.p, desc_.p, asmTable_.p, nAsm_, actLin_.p, srcTable_.p, nSrc_, stream_);
         timer.end(stream_);
         ops.maxAbs<double>(actLin_.p, 0, 1, maxOut_.p);
         OM_HIP(hipMemcpyAsync(hostD_, maxOut_.p, B * sizeof(double), hipMemcpyDeviceToHost, stream_));

@@ -67,6 +67,7 @@ private:
     hipStream_t stream_ = nullptr;
     bool ownStream_ = false;
     BoFluid fluid_{};
+    DevBuf<BoFluid> fluidDev_;   // fluid_ in device memory, for the assembly
     bool fluidSet_ = false;
     std::vector<std::unique_ptr<Member>> members_;
     int nextId_ = 0;

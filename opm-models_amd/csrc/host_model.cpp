@@ -36,6 +36,7 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
     G.area[0] = p->dy * p->dz; G.area[1] = p->dx * p->dz; G.area[2] = p->dx * p->dy;
 
     trans.upload(topo.trans.data(), topo.trans.size(), stream);
+    fluidDev.upload(&fluid, 1, stream);
     poro.upload(p->poro, N, stream);
     rowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
     colIdx.upload(topo.colIdx.data(), topo.colIdx.size(), stream);
@@ -95,7 +96,7 @@ void Model::linearizeDomain(double dt, int iterationIndex)
         launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream);
     const double storefac = G.V / dt;
     timer.begin(kClsAssemble, stream);
-    launchBoAssemble(fluid, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
+    launchBoAssemble(fluidDev.p, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
                      nbOff.p, stream, &diffusion, &dispersion);
     timer.end(stream);
     // the velocities just written are the next linearization's input

@@ -71,6 +71,7 @@ public:
 
     // device state
     DevBuf<double> trans, poro, pv0, pv1, st1;
+    DevBuf<BoFluid> fluidDev;    // the fluid for kernels that read it through a pointer
     DevBuf<uint8_t> mean0, mean1, wasSw;
     DevBuf<int> nbOff;
     DevBuf<int> srcCell, srcType;

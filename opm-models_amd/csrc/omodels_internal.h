@@ -136,7 +136,8 @@ struct KernelTimer {
 };
 
 // ---- black-oil kernels (bo_kernels.hip) ----
-void launchBoAssemble(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
+// fdev: the fluid in device memory
+void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
                       const BoDiff* dif = nullptr, const BoDisp* dsp = nullptr);
