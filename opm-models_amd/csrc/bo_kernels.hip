@@ -326,10 +326,12 @@ __global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const 
 // residual takes them (FLOWS) and the phases' reservoir-volume rates (FLORES),
 // [entry][3] with entries in the pattern's neighbour order.  One thread per
 // cell, the assembly's face evaluation.
-__global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, const uint8_t* __restrict__ mean,
+__global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const double* __restrict__ pv,
+                           const uint8_t* __restrict__ mean,
                            const double* __restrict__ trans, const int* __restrict__ rowPtr,
                            double* __restrict__ flows, double* __restrict__ flores, BoDiff dif, BoDisp dsp)
 {
+    const BoFluid& f = *fp;
     const long nxy = (long)G.nx * G.ny;
     const long N = nxy * G.nz;
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -375,9 +377,11 @@ __global__ void k_bo_flows(BoFluid f, BoGrid G, const double* __restrict__ pv, c
 // density_ and mobility_<phase> and porosity (vtkmultiphasemodule.hh:306-323),
 // then VtkBlackOilModule's R_s, B_w, B_o, B_g and the gas meaning
 // (vtkblackoilmodule.hh:276-301).  Phases in the order water, oil, gas.
-__global__ void k_bo_output(BoFluid f, long N, const double* __restrict__ pv, const uint8_t* __restrict__ mean,
-                            const double* __restrict__ poro, double* __restrict__ out)
+__global__ void k_bo_output(const BoFluid* __restrict__ fp, long N, const double* __restrict__ pv,
+                            const uint8_t* __restrict__ mean, const double* __restrict__ poro,
+                            double* __restrict__ out)
 {
+    const BoFluid& f = *fp;
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
     IqE q;
@@ -395,11 +399,12 @@ __global__ void k_bo_output(BoFluid f, long N, const double* __restrict__ pv, co
     out[17 * N + I] = (double)mean[I];
 }
 
-void launchBoOutput(const BoFluid& f, long N, const double* pv, const uint8_t* mean, const double* poro,
+void launchBoOutput(const BoFluid* fdev, long N, const double* pv, const uint8_t* mean, const double* poro,
                     double* out, hipStream_t s)
 {
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_bo_output, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, f, N, pv, mean, poro, out);
+    hipLaunchKernelGGL(k_bo_output, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, fdev, N, pv, mean, poro,
+                       out);
 }
 
 // Storage of the current solution: the cache of time level 1 at the first
@@ -541,13 +546,13 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
                        cell, type, a, b, outRes, outDiag);
 }
 
-void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
+void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif,
                    const BoDisp* dsp)
 {
     const long N = (long)G.nx * G.ny * G.nz;
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, f, G, pv, mean, trans, rowPtr,
+    hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, fdev, G, pv, mean, trans, rowPtr,
                        flows, flores, dif ? *dif : BoDiff{}, dsp ? *dsp : BoDisp{});
 }
 

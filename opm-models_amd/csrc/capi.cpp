@@ -346,7 +346,7 @@ int omodels_output_fields(omodels_model* h, double* out)
         DevBuf<double> d;
         d.alloc((size_t)kOutFields * m.N);
         m.timer.begin(kClsOther, m.stream);
-        launchBoOutput(m.fluid, m.N, m.pv0.p, m.mean0.p, m.poro.p, d.p, m.stream);
+        launchBoOutput(m.fluidDev.p, m.N, m.pv0.p, m.mean0.p, m.poro.p, d.p, m.stream);
         m.timer.end(m.stream);
         OM_HIP(hipGetLastError());
         d.download(out, (size_t)kOutFields * m.N, m.stream);

@@ -130,7 +130,7 @@ void Model::addSparseSources(int n, const int* cells, const double* r, const dou
 void Model::computeFlows(double* flows, double* flores)
 {
     timer.begin(kClsOther, stream);
-    launchBoFlows(fluid, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion, &dispersion);
+    launchBoFlows(fluidDev.p, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion, &dispersion);
     timer.end(stream);
     OM_HIP(hipGetLastError());
 }

@@ -149,11 +149,11 @@ void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* 
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
                           double* outRes, double* outDiag, hipStream_t s);
-void launchBoFlows(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
+void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif = nullptr,
                    const BoDisp* dsp = nullptr);
 constexpr int kOutFields = 18;
-void launchBoOutput(const BoFluid& f, long N, const double* pv, const uint8_t* mean, const double* poro,
+void launchBoOutput(const BoFluid* fdev, long N, const double* pv, const uint8_t* mean, const double* poro,
                     double* out, hipStream_t s);
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
                             double* jac, const int* diagOff, hipStream_t s);
