@@ -57,7 +57,7 @@ CONFIGS = {
 HBM_TRAFFIC_500K = {
     "ilu_apply": 585807565,    # forward + backward sweep
     "spmv": 347363328,
-    "assemble": 482855219,
+    "assemble": 489952768,
     "ilu_factor": 634320998,
 }
 
