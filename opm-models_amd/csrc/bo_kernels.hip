@@ -236,8 +236,11 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
 // the kernel arguments, and the compiler keeps them in scalar registers that
 // then spill into vector registers; read through the pointer they are scalar
 // loads where used, which leaves the kernel room for two waves per SIMD.
+// The bound of two waves holds the variants with the diffusive and dispersive
+// terms there too, at the price of a few spilled registers (faster than one
+// wave per SIMD without spills).
 template <bool Diff, bool Disp>
-__global__ __launch_bounds__(kTX * kTY) void k_bo_assemble(const BoFluid* __restrict__ fp, BoGrid G,
+__global__ __launch_bounds__(kTX * kTY, 2) void k_bo_assemble(const BoFluid* __restrict__ fp, BoGrid G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
