@@ -589,8 +589,8 @@ void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t
 
 namespace omodels {
 
-// the fluid through a device pointer, as in k_bo_assemble
-__global__ __launch_bounds__(kTX * kTY) void k_e_assemble(const BoFluid* __restrict__ fp, const EnsInst* __restrict__ insts,
+// the fluid through a device pointer and two waves per SIMD, as in k_bo_assemble
+__global__ __launch_bounds__(kTX * kTY, 2) void k_e_assemble(const BoFluid* __restrict__ fp, const EnsInst* __restrict__ insts,
     const int2* __restrict__ table, const int* __restrict__ active)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
