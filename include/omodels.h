@@ -283,6 +283,12 @@ int omodels_solver_get_residual(omodels_solver* s, double* residual);
 /* setMatrix + solve(x) (:252-284); x host [N*bs] may be NULL (kept on device) */
 int omodels_solver_solve(omodels_solver* s, double* x);
 int omodels_solver_iterations(const omodels_solver* s);
+/* Scalar product of the backend's BiCGStab, OverlappingScalarProduct::dot
+ * (overlappingscalarproduct.hh:58-70) for one process: *out = sum x[i]*y[i]
+ * over host arrays of m < 2^28 entries.  The sum is exact and rounded once to
+ * the nearest double (ties to even), so it does not depend on the order of
+ * the additions; the reference's sequential loop approximates this value. */
+int omodels_scalar_product(int64_t m, const double* x, const double* y, double* out);
 /* Block ILU0 factor of the prepared matrix (diagonal blocks inverted), for
  * parity checks of PreconditionerWrapperILU (istlpreconditionerwrappers.hh:151-190). */
 int omodels_solver_get_ilu(omodels_solver* s, double* lu_values);

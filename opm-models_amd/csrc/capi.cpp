@@ -498,6 +498,31 @@ int omodels_solver_solve(omodels_solver* s, double* x)
 
 int omodels_solver_iterations(const omodels_solver* s) { return s->s->iterations(); }
 
+int omodels_scalar_product(int64_t m, const double* x, const double* y, double* out)
+{
+    return guarded([&] {
+        if (m < 0 || m >= (int64_t(1) << 28)) throw std::invalid_argument("scalar product: 0 <= m < 2^28");
+        if (!out || (m > 0 && (!x || !y))) throw std::invalid_argument("scalar product: missing array");
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("no HIP device available");
+        hipStream_t st = nullptr;
+        OM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } g{st};
+        DevBuf<double> dx, dy, dout;
+        DevBuf<ExactAcc> acc;
+        dx.upload(x, (size_t)m, st);
+        dy.upload(y, (size_t)m, st);
+        dout.alloc(1);
+        acc.alloc(1);
+        OM_HIP(hipMemsetAsync(acc.p, 0, sizeof(ExactAcc), st));
+        launchScalarProduct((long)m, dx.p, dy.p, acc.p, dout.p, st);
+        OM_HIP(hipGetLastError());
+        dout.download(out, 1, st);
+        OM_HIP(hipStreamSynchronize(st));
+        return 0;
+    });
+}
+
 int omodels_solver_get_ilu(omodels_solver* s, double* lu)
 {
     return guarded([&] { s->s->downloadIlu(lu); return 0; });

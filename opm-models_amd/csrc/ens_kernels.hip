@@ -79,32 +79,53 @@ __global__ __launch_bounds__(kT) void k_e_copy(const EnsInst* __restrict__ insts
     }
 }
 
-template <class S>
-__global__ __launch_bounds__(kT) void k_e_dot_partial(const EnsInst* __restrict__ insts,
-    const int2* __restrict__ table, const int* __restrict__ act, int ia, int ib, double* __restrict__ part)
+// Scalar products: each chunk adds its products exactly into its member's
+// accumulator (exact_sum.h), so a member's value does not depend on the batch
+// or on the chunking and equals the single-reservoir solver's.
+// NACC = 1: acc[0] += a.b;  NACC = 2: acc[0] += a.a, acc[1] += a.b
+template <class S, int NACC>
+__global__ __launch_bounds__(kT) void k_e_xdot_partial(const EnsInst* __restrict__ insts,
+    const int2* __restrict__ table, const int* __restrict__ act, int ia, int ib, ExactAcc* __restrict__ accAll)
 {
     const int2 t = table[blockIdx.x];
     if (!act[t.x]) return;
+    __shared__ unsigned long long sAcc[NACC * kAccLimbs];
+    __shared__ unsigned sFlags;
+    clearLimbs(sAcc, NACC, &sFlags);
+    __syncthreads();
     const EnsInst& I = insts[t.x];
     const long c = (long)t.y * kT + threadIdx.x;
-    double v = 0.0;
-    if (c < I.N) {
-        const S* a = vec<S>(I, ia);
-        const S* b = vec<S>(I, ib);
+    unsigned fl0 = 0u, fl1 = 0u;
+    ProductPiece p0[3], p1[3];
+    const S* a = vec<S>(I, ia);
+    const S* b = vec<S>(I, ib);
 #pragma unroll
-        for (int e = 0; e < 3; ++e) v += (double)a[3 * c + e] * (double)b[3 * c + e];
+    for (int e = 0; e < 3; ++e) {
+        double v0 = 0.0, v1 = 0.0;
+        if (c < I.N) {
+            const double ai = (double)a[3 * c + e];
+            if (NACC == 1) v0 = ai * (double)b[3 * c + e];
+            else { v0 = ai * ai; v1 = ai * (double)b[3 * c + e]; }
+        }
+        p0[e] = makePiece(v0, fl0);
+        if (NACC == 2) p1[e] = makePiece(v1, fl1);
     }
-    v = blockReduce<false>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
+    accumulatePieces<3>(p0, sAcc);
+    if (NACC == 2) accumulatePieces<3>(p1, sAcc + kAccLimbs);
+    if (fl0 | fl1) atomicOr(&sFlags, fl0 | (fl1 << 3));
+    __syncthreads();
+    flushLimbs(sAcc, NACC, &sFlags, accAll + 2 * t.x);
 }
 
-// op 0: rho_i and beta; op 1: denom and alpha (bicgstabsolver.hh:197-236)
-__global__ __launch_bounds__(kT) void k_e_dot_final(const int* __restrict__ first, const int* __restrict__ act,
-    const double* __restrict__ part, KrylovScalars* ksAll, int op, double eps, int tag)
+// op 0: rho_i and beta; op 1: denom and alpha (bicgstabsolver.hh:197-236).
+// One wave per member.
+__global__ __launch_bounds__(64) void k_e_xdot_final(const int* __restrict__ act, ExactAcc* accAll,
+                                                     KrylovScalars* ksAll, int op, double eps, int tag)
 {
     const int b = blockIdx.x;
     if (!act[b]) return;
-    const double s = instReduce<false>(part, first[b], first[b + 1]);
+    __shared__ long long sD[kAccLimbs];
+    const double s = finishExact(accAll + 2 * b, sD);
     if (threadIdx.x != 0) return;
     KrylovScalars* ks = ksAll + b;
     if (op == 0) {
@@ -196,38 +217,15 @@ __global__ __launch_bounds__(kT) void k_e_crit_final(const int* __restrict__ fir
     ksAll[b].stag_half = nz == 0.0 ? 1 : 0;
 }
 
-// <t,t>, <t,s> with t = y, s = r
-template <class S>
-__global__ __launch_bounds__(kT) void k_e_dot2_partial(const EnsInst* __restrict__ insts,
-    const int2* __restrict__ table, const int* __restrict__ act, double* __restrict__ part, int nChunks)
-{
-    const int2 t = table[blockIdx.x];
-    if (!act[t.x]) return;
-    const EnsInst& I = insts[t.x];
-    const long c = (long)t.y * kT + threadIdx.x;
-    double tt = 0.0, ts = 0.0;
-    if (c < I.N) {
-        const S* y = vec<S>(I, kVy);
-        const S* r = vec<S>(I, kVr);
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            const double ti = (double)y[3 * c + e];
-            tt += ti * ti;
-            ts += ti * (double)r[3 * c + e];
-        }
-    }
-    tt = blockReduce<false>(tt);
-    ts = blockReduce<false>(ts);
-    if (threadIdx.x == 0) { part[blockIdx.x] = tt; part[nChunks + blockIdx.x] = ts; }
-}
-
-__global__ __launch_bounds__(kT) void k_e_dot2_final(const int* __restrict__ first, const int* __restrict__ act,
-    const double* __restrict__ part, int nChunks, KrylovScalars* ksAll, double eps, int tag)
+// <t,t>, <t,s> with t = y, s = r: k_e_xdot_partial<S, 2>(.., kVy, kVr, ..)
+__global__ __launch_bounds__(64) void k_e_xdot2_final(const int* __restrict__ act, ExactAcc* accAll,
+                                                      KrylovScalars* ksAll, double eps, int tag)
 {
     const int b = blockIdx.x;
     if (!act[b]) return;
-    const double tt = instReduce<false>(part, first[b], first[b + 1]);
-    const double ts = instReduce<false>(part + nChunks, first[b], first[b + 1]);
+    __shared__ long long sD[kAccLimbs];
+    const double tt = finishExact(accAll + 2 * b, sD);
+    const double ts = finishExact(accAll + 2 * b + 1, sD);
     if (threadIdx.x != 0) return;
     KrylovScalars* ks = ksAll + b;
     if (fabs(tt) <= eps) markFail(ks, tag);
@@ -416,12 +414,14 @@ __global__ __launch_bounds__(kT) void k_e_residual64(const EnsInst* __restrict__
 
 // ---------------------------------------------------------------------------
 EnsOps::EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* first, int nInst, double* part,
-               KrylovScalars* ks, hipStream_t s)
-    : insts_(insts), table_(table), nChunks_(nChunks), first_(first), nInst_(nInst), part_(part), ks_(ks), s_(s)
+               ExactAcc* acc, KrylovScalars* ks, hipStream_t s)
+    : insts_(insts), table_(table), nChunks_(nChunks), first_(first), nInst_(nInst), part_(part), acc_(acc), ks_(ks),
+      s_(s)
 {}
 
 #define EGRID dim3(nChunks_), dim3(kT), 0, s_
 #define IGRID dim3(nInst_), dim3(kT), 0, s_
+#define WGRID dim3(nInst_), dim3(64), 0, s_
 
 template <class S> void EnsOps::copy(const int* act, int dst, int src)
 { if (nChunks_) hipLaunchKernelGGL((k_e_copy<S>), EGRID, insts_, table_, act, dst, src); }
@@ -429,9 +429,9 @@ template <class S> void EnsOps::copy(const int* act, int dst, int src)
 template <class S> void EnsOps::dot(const int* act, int ia, int ib, int op, int tag)
 {
     if (!nChunks_) return;
-    hipLaunchKernelGGL((k_e_dot_partial<S>), EGRID, insts_, table_, act, ia, ib, part_);
+    hipLaunchKernelGGL((k_e_xdot_partial<S, 1>), EGRID, insts_, table_, act, ia, ib, acc_);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_e_dot_final, IGRID, first_, act, part_, ks_, op, eps, tag);
+    hipLaunchKernelGGL(k_e_xdot_final, WGRID, act, acc_, ks_, op, eps, tag);
 }
 
 template <class S> void EnsOps::updateP(const int* act)
@@ -447,9 +447,9 @@ template <class S> void EnsOps::updateXR(const int* act)
 template <class S> void EnsOps::dot2(const int* act, int tag)
 {
     if (!nChunks_) return;
-    hipLaunchKernelGGL((k_e_dot2_partial<S>), EGRID, insts_, table_, act, part_, nChunks_);
+    hipLaunchKernelGGL((k_e_xdot_partial<S, 2>), EGRID, insts_, table_, act, kVy, kVr, acc_);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_e_dot2_final, IGRID, first_, act, part_, nChunks_, ks_, eps, tag);
+    hipLaunchKernelGGL(k_e_xdot2_final, WGRID, act, acc_, ks_, eps, tag);
 }
 
 template <class S> void EnsOps::updateXR2(const int* act, int tag)

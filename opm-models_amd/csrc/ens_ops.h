@@ -19,8 +19,9 @@ void assemblyGrid(const BoGrid& G, int& gx, int& gy, int& gz, int& kChunk, int t
 // single-reservoir solver, one launch per step for the whole ensemble)
 class EnsOps {
 public:
+    // acc: two exact accumulators per member (zero; every product leaves them zero)
     EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* first, int nInst, double* part,
-           KrylovScalars* ks, hipStream_t s);
+           ExactAcc* acc, KrylovScalars* ks, hipStream_t s);
     template <class S> void copy(const int* act, int dst, int src);   // src < 0: zero
     template <class S> void dot(const int* act, int ia, int ib, int op, int tag);
     template <class S> void updateP(const int* act);
@@ -42,6 +43,7 @@ private:
     const int* first_;
     int nInst_;
     double* part_;
+    ExactAcc* acc_;
     KrylovScalars* ks_;
     hipStream_t s_;
 };

@@ -57,6 +57,8 @@ Ensemble::Ensemble(const omodels_params& prm, hipStream_t s) : prm_(prm)
     OM_HIP(hipHostMalloc((void**)&hostD_, sizeof(double) * kMaxMembers, hipHostMallocDefault));
     OM_HIP(hipHostMalloc((void**)&hostI_, sizeof(int) * 2 * kMaxMembers, hipHostMallocDefault));
     ks_.alloc(kMaxMembers);
+    acc_.alloc(2 * kMaxMembers);
+    OM_HIP(hipMemsetAsync(acc_.p, 0, 2 * kMaxMembers * sizeof(ExactAcc), stream_));
     maxOut_.alloc(kMaxMembers);
     for (auto* a : {&actLin_, &actRefresh_, &actSolve_, &actInner_}) a->alloc(kMaxMembers);
     fail_.alloc(1);
@@ -261,7 +263,7 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
                         std::vector<int>& rc, std::vector<int>& its)
 {
     const int B = (int)members_.size();
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, ks_.p, stream_);
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
     const bool fl = sizeof(S) == 4;
     std::vector<int> act(set);
     rc.assign(B, 0);
@@ -290,6 +292,9 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
         k.fail_tag = -1; k.stag_full_tag = -1;
     }
     OM_HIP(hipMemcpyAsync(ks_.p, hostKs_, B * sizeof(KrylovScalars), hipMemcpyHostToDevice, stream_));
+    // every scalar product clears its accumulators again; this clears what a
+    // solve interrupted by an error may have left behind
+    OM_HIP(hipMemsetAsync(acc_.p, 0, 2 * (size_t)B * sizeof(ExactAcc), stream_));
     setFlags(actInner_, act);
     ops.copy<S>(a, kVv, -1);
     ops.copy<S>(a, kVp, -1);
@@ -341,7 +346,7 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
 void Ensemble::linearSolve(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its)
 {
     const int B = (int)members_.size();
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, ks_.p, stream_);
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
     setFlags(actSolve_, set);
     const int* a = actSolve_.p;
     // preconditioner: FP64 ILU(0) of the FP64 Jacobian
@@ -427,7 +432,7 @@ void Ensemble::timeStep()
     const int B = (int)members_.size();
     if (B == 0) return;
     using clk = std::chrono::steady_clock;
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, ks_.p, stream_);
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
     enum { kIdle = 0, kNewton = 1, kOk = 2, kFailedStep = 3 };
     std::vector<int> state(B, kIdle);
     for (int b = 0; b < B; ++b) {

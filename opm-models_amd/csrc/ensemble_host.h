@@ -80,6 +80,7 @@ private:
     int nAsm_ = 0, nChunks_ = 0, nSrc_ = 0;
     DevBuf<double> partial_, maxOut_;
     DevBuf<KrylovScalars> ks_;
+    DevBuf<ExactAcc> acc_;       // two exact accumulators per member slot
     KrylovScalars* hostKs_ = nullptr;
     double* hostD_ = nullptr;
     int* hostI_ = nullptr;

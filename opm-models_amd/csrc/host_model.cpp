@@ -70,6 +70,8 @@ void LinearSystem::initSystem(hipStream_t s)
     if (s) stream = s;
     else { OM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); ownStream = true; }
     partial.alloc(2 * kRedBlocks);
+    acc.alloc(2);
+    OM_HIP(hipMemsetAsync(acc.p, 0, 2 * sizeof(ExactAcc), stream));
     scal.alloc(8);
     OM_HIP(hipHostMalloc((void**)&hostScal, 8 * sizeof(double), hipHostMallocDefault));
 }
@@ -539,6 +541,9 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     OM_HIP(hipMemcpyAsync(ks_.p, &k0, sizeof(k0), hipMemcpyHostToDevice, s));
     OM_HIP(hipMemsetAsync(v, 0, m * sizeof(S), s));
     OM_HIP(hipMemsetAsync(p, 0, m * sizeof(S), s));
+    // the final kernel of every scalar product clears its accumulator; this
+    // clears what a solve interrupted by an error may have left behind
+    OM_HIP(hipMemsetAsync(m_.acc.p, 0, 2 * sizeof(ExactAcc), s));
 
     // first half: rho, p, y = M^-1 p, v = A y, alpha, x/r update, convergence test.
     // Every kernel of the solve returns at once after the test has ended it
@@ -546,7 +551,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     // seen the test of iteration it, and the device never waits for the host.
     bool fusedSpmv = false;     // the operator runs inside the preconditioner's backward sweep
     auto firstHalf = [&]() {
-        launchDotFinal<S>(m, b, r, m_.partial.p, ks_.p, 0, 0, s);
+        launchDotFinal<S>(m, b, r, m_.acc.p, ks_.p, 0, 0, s);
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
         fusedSpmv = precApply<S>(lu, dinv, p, y, run, A, v);
         if (!fusedSpmv) {
@@ -554,7 +559,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
             launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s, run);
             m_.timer.end(s);
         }
-        launchDotFinal<S>(m, b, v, m_.partial.p, ks_.p, 1, 0, s);
+        launchDotFinal<S>(m, b, v, m_.acc.p, ks_.p, 1, 0, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
     };
     // second half: z = M^-1 r, y = A z, omega, x/r update, next tag base
@@ -565,7 +570,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
             launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s, run);
             m_.timer.end(s);
         }
-        launchDot2Final<S>(m, y, r, m_.partial.p, ks_.p, 1, s);
+        launchDot2Final<S>(m, y, r, m_.acc.p, ks_.p, 1, s);
         launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, 2, s);
     };
 

@@ -12,6 +12,7 @@
 // are reproducible run to run; BiCGStab scalars stay in device memory.
 #include <hipcub/hipcub.hpp>
 
+#include "exact_sum.h"
 #include "omodels_internal.h"
 
 namespace omodels {
@@ -461,17 +462,55 @@ __device__ __forceinline__ double finalReduce(const double* __restrict__ partial
     return blockReduce<IsMax>(v);
 }
 
-template <class S>
-__global__ __launch_bounds__(kRedThreads) void k_dot_partial(long m, const S* __restrict__ x,
-                                                             const S* __restrict__ y, double* __restrict__ part,
-                                                             const KrylovScalars* __restrict__ ks)
+// Scalar products: the exact sum of the products, rounded once (exact_sum.h),
+// so the value is the same for every launch shape and equals the CPU
+// restatement's.  Each wave takes kDotQ * 64 consecutive entries per pass;
+// workgroups add their limbs to the device accumulator, whose final kernel
+// rounds, evaluates the BiCGStab scalars and clears it for the next use.
+constexpr int kDotQ = 8;
+constexpr int kDotThreads = 256;
+constexpr int kDotMaxBlocks = 1024;
+
+inline int dotBlocks(long m)
 {
-    if (!ks->run) return;
-    double v = 0.0;
-    for (long i = (long)blockIdx.x * kRedThreads + threadIdx.x; i < m; i += (long)kRedBlocks * kRedThreads)
-        v += (double)x[i] * (double)y[i];
-    v = blockReduce<false>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
+    const long per = (long)kDotQ * kDotThreads;
+    return (int)std::max(1L, std::min<long>(kDotMaxBlocks, (m + per - 1) / per));
+}
+
+// NACC = 1: acc[0] += a.b;  NACC = 2: acc[0] += a.b, acc[1] += a.c
+template <class S, int NACC>
+__global__ __launch_bounds__(kDotThreads) void k_xdot_partial(long m, const S* __restrict__ a,
+    const S* __restrict__ b, const S* __restrict__ c, ExactAcc* __restrict__ acc, const int* __restrict__ run)
+{
+    if (run && !*run) return;
+    __shared__ unsigned long long sAcc[NACC * kAccLimbs];
+    __shared__ unsigned sFlags;
+    clearLimbs(sAcc, NACC, &sFlags);
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63u);
+    const long wavesPerBlock = kDotThreads / 64, wavesTotal = (long)gridDim.x * wavesPerBlock;
+    unsigned fl0 = 0u, fl1 = 0u;
+    for (long w = (long)blockIdx.x * wavesPerBlock + (long)(threadIdx.x >> 6); w * (kDotQ * 64) < m; w += wavesTotal) {
+        const long base = w * (kDotQ * 64) + lane;
+        ProductPiece p0[kDotQ], p1[kDotQ];
+#pragma unroll
+        for (int q = 0; q < kDotQ; ++q) {
+            const long i = base + 64L * q;
+            double v0 = 0.0, v1 = 0.0;
+            if (i < m) {
+                const double ai = (double)a[i];
+                v0 = ai * (double)b[i];
+                if (NACC == 2) v1 = ai * (double)c[i];
+            }
+            p0[q] = makePiece(v0, fl0);
+            if (NACC == 2) p1[q] = makePiece(v1, fl1);
+        }
+        accumulatePieces<kDotQ>(p0, sAcc);
+        if (NACC == 2) accumulatePieces<kDotQ>(p1, sAcc + kAccLimbs);
+    }
+    if (fl0 | fl1) atomicOr(&sFlags, fl0 | (fl1 << 3));
+    __syncthreads();
+    flushLimbs(sAcc, NACC, &sFlags, acc);
 }
 
 // op 0: rho_i = <r0hat, r>, beta; op 1: denom = <r0hat, v>, alpha
@@ -480,11 +519,11 @@ __device__ __forceinline__ void markFail(KrylovScalars* ks, int tag)
     if (!ks->fail_code) { ks->fail_code = 2; ks->fail_tag = tag; }
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restrict__ part, KrylovScalars* ks,
-                                                           int op, double eps, int tagOff)
+__global__ __launch_bounds__(64) void k_xdot_final(ExactAcc* acc, KrylovScalars* ks, int op, double eps, int tagOff)
 {
     if (!ks->run) return;
-    const double s = finalReduce<false>(part);
+    __shared__ long long sD[kAccLimbs];
+    const double s = finishExact(acc, sD);
     if (threadIdx.x != 0) return;
     const int tag = ks->tag_base + tagOff;
     if (op == 0) {
@@ -501,17 +540,30 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_final(const double* __restr
 }
 
 template <class S>
-void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks, int op, int tagOff,
+void launchDotFinal(long m, const S* x, const S* y, ExactAcc* acc, KrylovScalars* ks, int op, int tagOff,
                     hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dot_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, x, y, partial, ks);
+    hipLaunchKernelGGL((k_xdot_partial<S, 1>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, x, y, y, acc,
+                       &ks->run);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, op, eps, tagOff);
+    hipLaunchKernelGGL(k_xdot_final, dim3(1), dim3(64), 0, st, acc, ks, op, eps, tagOff);
 }
 
-// next iteration: the half-iteration tags move on by two
-template void launchDotFinal<double>(long, const double*, const double*, double*, KrylovScalars*, int, int, hipStream_t);
-template void launchDotFinal<float>(long, const float*, const float*, double*, KrylovScalars*, int, int, hipStream_t);
+template void launchDotFinal<double>(long, const double*, const double*, ExactAcc*, KrylovScalars*, int, int, hipStream_t);
+
+__global__ __launch_bounds__(64) void k_xdot_value(ExactAcc* acc, double* out)
+{
+    __shared__ long long sD[kAccLimbs];
+    const double s = finishExact(acc, sD);
+    if (threadIdx.x == 0) *out = s;
+}
+void launchScalarProduct(long m, const double* x, const double* y, ExactAcc* acc, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_xdot_partial<double, 1>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, x, y, y, acc,
+                       nullptr);
+    hipLaunchKernelGGL(k_xdot_value, dim3(1), dim3(64), 0, st, acc, out);
+}
+template void launchDotFinal<float>(long, const float*, const float*, ExactAcc*, KrylovScalars*, int, int, hipStream_t);
 
 // p = r + (-beta) * (v*omega - p)   (bicgstabsolver.hh:211-222)
 template <class S>
@@ -600,28 +652,12 @@ template void launchUpdateXR<double>(long, const double*, const double*, double*
 template void launchUpdateXR<float>(long, const float*, const float*, float*, float*, double*, KrylovScalars*, hipStream_t);
 
 // <t,t> and <t,s> in one pass; omega = <t,s>/<t,t>  (:287-292)
-template <class S>
-__global__ __launch_bounds__(kRedThreads) void k_dot2_partial(long m, const S* __restrict__ t,
-                                                              const S* __restrict__ s, double* __restrict__ part,
-                                                              const KrylovScalars* __restrict__ ks)
+__global__ __launch_bounds__(64) void k_xdot2_final(ExactAcc* acc, KrylovScalars* ks, double eps, int tagOff)
 {
     if (!ks->run) return;
-    double tt = 0.0, ts = 0.0;
-    for (long i = (long)blockIdx.x * kRedThreads + threadIdx.x; i < m; i += (long)kRedBlocks * kRedThreads) {
-        const double ti = (double)t[i];
-        tt += ti * ti;
-        ts += ti * (double)s[i];
-    }
-    tt = blockReduce<false>(tt);
-    ts = blockReduce<false>(ts);
-    if (threadIdx.x == 0) { part[blockIdx.x] = tt; part[kRedBlocks + blockIdx.x] = ts; }
-}
-__global__ __launch_bounds__(kRedThreads) void k_dot2_final(const double* __restrict__ part, KrylovScalars* ks,
-                                                            double eps, int tagOff)
-{
-    if (!ks->run) return;
-    const double tt = finalReduce<false>(part);
-    const double ts = finalReduce<false>(part + kRedBlocks);
+    __shared__ long long sD[kAccLimbs];
+    const double tt = finishExact(acc, sD);
+    const double ts = finishExact(acc + 1, sD);
     if (threadIdx.x != 0) return;
     const int tag = ks->tag_base + tagOff;
     if (fabs(tt) <= eps) markFail(ks, tag);
@@ -629,14 +665,15 @@ __global__ __launch_bounds__(kRedThreads) void k_dot2_final(const double* __rest
     if (fabs(ks->omega) <= eps) markFail(ks, tag);
 }
 template <class S>
-void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks, int tagOff, hipStream_t st)
+void launchDot2Final(long m, const S* t, const S* s, ExactAcc* acc, KrylovScalars* ks, int tagOff, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dot2_partial<S>), dim3(kRedBlocks), dim3(kRedThreads), 0, st, m, t, s, partial, ks);
+    hipLaunchKernelGGL((k_xdot_partial<S, 2>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, t, t, s, acc,
+                       &ks->run);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedThreads), 0, st, partial, ks, eps, tagOff);
+    hipLaunchKernelGGL(k_xdot2_final, dim3(1), dim3(64), 0, st, acc, ks, eps, tagOff);
 }
-template void launchDot2Final<double>(long, const double*, const double*, double*, KrylovScalars*, int, hipStream_t);
-template void launchDot2Final<float>(long, const float*, const float*, double*, KrylovScalars*, int, hipStream_t);
+template void launchDot2Final<double>(long, const double*, const double*, ExactAcc*, KrylovScalars*, int, hipStream_t);
+template void launchDot2Final<float>(long, const float*, const float*, ExactAcc*, KrylovScalars*, int, hipStream_t);
 
 // x += omega*z (criterion change indicator z), r += -omega*t  (:296-325)
 template <class S>

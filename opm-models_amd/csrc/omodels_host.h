@@ -46,6 +46,7 @@ struct LinearSystem {
     DevBuf<int> rowPtr, colIdx, diagOff;
     LevelSchedule levels;
     DevBuf<double> partial;      // reduction partials
+    DevBuf<ExactAcc> acc;        // exact accumulators of the scalar products (2)
     DevBuf<double> scal;         // device scalars (norms)
     double* hostScal = nullptr;  // pinned mirror
     KernelTimer timer;

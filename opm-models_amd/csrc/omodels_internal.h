@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "bo_physics.h"
+#include "exact_sum.h"
 
 namespace omodels {
 
@@ -206,8 +207,12 @@ template <class S> void launchExtractInvDiag(int n, int bs, const int* diag, con
 
 // BiCGStab steps (see bicgstab in host_solver.cpp for the sequence)
 // tagOff: 0 for the first half of an iteration, 1 for the second (tag = ks->tag_base + tagOff)
-template <class S> void launchDotFinal(long m, const S* x, const S* y, double* partial, KrylovScalars* ks,
+// scalar products into an exact accumulator (exact_sum.h; the final kernel
+// clears it again): dot uses acc[0], dot2 acc[0] and acc[1]
+template <class S> void launchDotFinal(long m, const S* x, const S* y, ExactAcc* acc, KrylovScalars* ks,
                                        int op, int tagOff, hipStream_t st);
+// out = <x, y> on the device (acc: one zero accumulator, left zero)
+void launchScalarProduct(long m, const double* x, const double* y, ExactAcc* acc, double* out, hipStream_t st);
 template <class S> void launchUpdateP(long m, const S* r, const S* v, S* p, const KrylovScalars* ks,
                                       hipStream_t st);
 template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* r, double* partial,
@@ -215,7 +220,7 @@ template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* 
 // the second half ends the iteration: its final kernel advances tag_base by `advance`
 template <class S> void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial,
                                         KrylovScalars* ks, int tagOff, int advance, hipStream_t st);
-template <class S> void launchDot2Final(long m, const S* t, const S* s, double* partial, KrylovScalars* ks,
+template <class S> void launchDot2Final(long m, const S* t, const S* s, ExactAcc* acc, KrylovScalars* ks,
                                         int tagOff, hipStream_t st);
 template <class S> void launchMaxAbs(long m, const S* x, double* partial, double* out, hipStream_t st);
 void launchResidualUpdate(int n, int bs, const int* rp, const int* ci, const double* a, const double* x,

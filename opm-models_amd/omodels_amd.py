@@ -139,6 +139,7 @@ SIGNATURES = {
     "omodels_solver_solve": (C.c_int, [C.c_void_p, _dp]),
     "omodels_solver_iterations": (C.c_int, [C.c_void_p]),
     "omodels_solver_get_ilu": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_scalar_product": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
     "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
@@ -195,6 +196,16 @@ def ptr(a, t=_dp):
     if a is None:
         return None
     return a.ctypes.data_as(t)
+
+
+def scalar_product(x, y):
+    """The backend's scalar product on the device: the exact sum of x[i]*y[i]
+    rounded once (OverlappingScalarProduct::dot, overlappingscalarproduct.hh:58-70)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(1)
+    check(lib().omodels_scalar_product(x.size, ptr(x), ptr(y), ptr(out)), "scalar_product")
+    return float(out[0])
 
 
 def output_field_names():

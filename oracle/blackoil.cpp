@@ -675,12 +675,102 @@ void spmv(int n, int bs, const int32_t* rp, const int32_t* ci, const S* a, const
     }
 }
 
+/* Exact sum of doubles, rounded once to the nearest double (ties to even).
+ * The reference's scalar product is a sequential loop
+ * (overlappingscalarproduct.hh:58-70); its value depends on the order of the
+ * additions, so the restatement uses the exact sum, which that loop
+ * approximates and which any summation order reproduces.  Positive and
+ * negative products accumulate separately as unsigned magnitudes in 64-bit
+ * digits of 2^-1074 units (34 digits hold any finite double with headroom);
+ * the signed difference is rounded at the end.  A zero sum is +0; a NaN
+ * product, or +inf with -inf, gives NaN; otherwise an infinite product gives
+ * that infinity. */
+class ExactSum {
+public:
+    void add(double v)
+    {
+        if (std::isnan(v)) { nan_ = true; return; }
+        if (std::isinf(v)) { (v > 0 ? pinf_ : ninf_) = true; return; }
+        if (v == 0.0) return;
+        int ex = 0;
+        const double f = std::frexp(std::fabs(v), &ex);         /* |v| = f 2^ex, f in [0.5, 1) */
+        uint64_t m = (uint64_t)std::ldexp(f, 53);                /* integer: |v| = m 2^(ex - 53) */
+        int p = ex - 53 + 1074;                                  /* position of m's unit bit */
+        while (p < 0) { m >>= 1; ++p; }                          /* subnormals: exact shifts */
+        uint64_t* acc = v > 0 ? pos_ : neg_;
+        const int d = p / 64, s = p % 64;
+        const unsigned __int128 w = (unsigned __int128)m << s;
+        addAt(acc, d, (uint64_t)w);
+        addAt(acc, d + 1, (uint64_t)(w >> 64));
+    }
+    double value() const
+    {
+        if (nan_ || (pinf_ && ninf_)) return std::numeric_limits<double>::quiet_NaN();
+        if (pinf_) return std::numeric_limits<double>::infinity();
+        if (ninf_) return -std::numeric_limits<double>::infinity();
+        /* |pos - neg| and its sign */
+        int c = 0;
+        for (int k = kDigits - 1; k >= 0 && c == 0; --k)
+            c = pos_[k] > neg_[k] ? 1 : (pos_[k] < neg_[k] ? -1 : 0);
+        if (c == 0) return 0.0;
+        const uint64_t* a = c > 0 ? pos_ : neg_;
+        const uint64_t* b = c > 0 ? neg_ : pos_;
+        uint64_t dif[kDigits];
+        uint64_t borrow = 0;
+        for (int k = 0; k < kDigits; ++k) {
+            const unsigned __int128 t = (unsigned __int128)a[k] - b[k] - borrow;
+            dif[k] = (uint64_t)t;
+            borrow = (uint64_t)(t >> 127);
+        }
+        int top = kDigits - 1;
+        while (dif[top] == 0) --top;
+        const int topBit = 64 * top + 63 - __builtin_clzll(dif[top]);   /* in units of 2^-1074 */
+        double r;
+        if (topBit < 53) {
+            r = std::ldexp((double)dif[0], -1074);                      /* exact: below 2^53 units */
+        } else {
+            /* the 53 leading bits, the next bit, and whether anything is below */
+            const int low = topBit - 52;                                 /* unit bit of the mantissa */
+            auto bit = [&](int i) -> uint64_t { return (dif[i / 64] >> (i % 64)) & 1u; };
+            uint64_t mant = 0;
+            for (int i = topBit; i >= low; --i) mant = (mant << 1) | bit(i);
+            const bool half = bit(low - 1) != 0;
+            bool below = false;
+            for (int i = low - 2; i >= 0 && !below; --i) below = bit(i) != 0;
+            if (half && (below || (mant & 1u))) ++mant;
+            r = std::ldexp((double)mant, low - 1074);                   /* inf when too large */
+        }
+        return c > 0 ? r : -r;
+    }
+
+private:
+    static constexpr int kDigits = 34;
+    static void addAt(uint64_t* acc, int d, uint64_t v)
+    {
+        for (; v && d < kDigits; ++d) {
+            const uint64_t o = acc[d];
+            acc[d] = o + v;
+            v = acc[d] < o ? 1 : 0;
+        }
+    }
+    uint64_t pos_[kDigits] = {0}, neg_[kDigits] = {0};
+    bool nan_ = false, pinf_ = false, ninf_ = false;
+};
+
+/* Scalar product.  Double: the exact sum of the rounded products (ExactSum).
+ * Float (LinearSolverScalar = float): the reference's sequential loop in float. */
 template <class S>
 S dot(const std::vector<S>& x, const std::vector<S>& y)
 {
-    S s = 0;
-    for (size_t i = 0; i < x.size(); ++i) s += x[i] * y[i];
-    return s;
+    if constexpr (sizeof(S) == 8) {
+        ExactSum e;
+        for (size_t i = 0; i < x.size(); ++i) e.add(x[i] * y[i]);
+        return e.value();
+    } else {
+        S s = 0;
+        for (size_t i = 0; i < x.size(); ++i) s += x[i] * y[i];
+        return s;
+    }
 }
 
 /* CombinedCriterion (combinedcriterion.hh:54-236). */
@@ -1064,6 +1154,12 @@ int ob_ilu0_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
 {
     ilu0Apply<double>(n, bs, row_ptr, col_idx, lu, d, v, relax);
     return 0;
+}
+
+double ob_dot(int64_t m, const double* x, const double* y)
+{
+    std::vector<double> a(x, x + m), b(y, y + m);
+    return dot<double>(a, b);
 }
 
 int ob_spmv(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,

@@ -137,6 +137,9 @@ int ob_ilu0_factor(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx
                    double* vals);
 int ob_ilu0_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
                   const double* lu, const double* d, double* v, double relax);
+/* scalar product of the restated BiCGStab: the exact sum of the products
+ * x[i]*y[i], rounded once to the nearest double */
+double ob_dot(int64_t m, const double* x, const double* y);
 int ob_spmv(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
             const double* vals, const double* x, double* y);
 int ob_bicgstab(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,

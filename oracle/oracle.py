@@ -111,6 +111,8 @@ def lib():
         L.ob_ilu0_factor.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp]
         L.ob_ilu0_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
         L.ob_spmv.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.ob_dot.argtypes = [C.c_int64, _dp, _dp]
+        L.ob_dot.restype = C.c_double
         L.ob_bicgstab.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
         L.ob_update_primary.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp, _u8p, _u8p,
                                         C.POINTER(ObParams), _ip]
@@ -258,6 +260,13 @@ def spmv(n, bs, rp, ci, vals, x):
     y = np.zeros(n * bs)
     lib().ob_spmv(n, bs, P(rp, _ip), P(ci, _ip), P(vals), P(x), P(y))
     return y
+
+
+def dot(x, y):
+    """The restated BiCGStab's scalar product: exact sum of x[i]*y[i], rounded once."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().ob_dot(x.size, P(x), P(y))
 
 
 def bicgstab(n, bs, rp, ci, vals, b, prm):

@@ -30,6 +30,17 @@ def test_oracle_reproduces_golden(name):
                                                     orc.params(newton_tol=1e-10, max_dt=86400.0, min_dt=86400.0))
     assert rc == 0
     assert np.array_equal(_bits(hist), _bits(z["hist"]))
+    rc, pvf, mf, hist, dts, log, tim = orc.simulate(prob, g.pv0, g.mean0, 5, 86400.0, orc.params())
+    assert rc == 0
+    assert np.array_equal(_bits(hist), _bits(z["hist_ref"])) and np.array_equal(_bits(dts), _bits(z["hist_ref_dt"]))
+    assert [tuple(l[:6]) for l in log] == [tuple(l) for l in z["log_ref"]]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_golden_records_the_chosen_forms(name):
+    forms = list(GoldenCase(name).z["forms"])
+    assert any(f.startswith("scalar product: exact sum") for f in forms)
+    assert any(f.startswith("LiveOilPvt undersaturated") for f in forms)
 
 
 @pytest.mark.gpu
@@ -57,4 +68,14 @@ def test_gpu_reproduces_golden(name):
     p, po = hist[:, 1::3], z["hist"][:, 1::3]
     assert np.max(np.abs(p - po)) <= 1e-8 * np.max(np.abs(po))
     assert np.max(np.abs(hist[:, 0::3] - z["hist"][:, 0::3])) <= 1e-8
-    assert [tuple(l[:3]) for l in log] == [tuple(l[:3]) for l in z["log"]]
+    assert [tuple(l[:6]) for l in log] == [tuple(l) for l in z["log"]]
+    assert np.array_equal(_bits(hist), _bits(z["hist"]))
+    # the reference's own settings: step sizes, iteration counts and states bit for bit
+    m3 = om.Model(g)
+    m3.set_solution(g.pv0, g.mean0)
+    rc, hist, dts, log, st = m3.simulate(om.default_params(), 5, 86400.0)
+    assert rc == 0
+    assert np.array_equal(_bits(dts), _bits(z["hist_ref_dt"]))
+    assert [tuple(l[:6]) for l in log] == [tuple(l) for l in z["log_ref"]]
+    assert np.array_equal(_bits([l[6] for l in log]), _bits(z["log_ref_error"]))
+    assert np.array_equal(_bits(hist), _bits(z["hist_ref"]))
