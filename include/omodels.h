@@ -289,6 +289,16 @@ int omodels_solver_iterations(const omodels_solver* s);
  * the nearest double (ties to even), so it does not depend on the order of
  * the additions; the reference's sequential loop approximates this value. */
 int omodels_scalar_product(int64_t m, const double* x, const double* y, double* out);
+/* The same scalar product without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * x_dev, y_dev: device arrays of m < 2^28 doubles; out_dev: one device
+ * double; stream: the caller's hipStream_t (NULL: the null stream).  The work
+ * is only enqueued: the result is in out_dev once the stream has reached it.
+ * The exact accumulator lives in a static device array of 552 bytes per GPU,
+ * shared by all calls on that GPU: calls that overlap on one GPU race. */
+int omodels_scalar_product_noalloc(int64_t m, const double* x_dev, const double* y_dev, double* out_dev,
+                                   void* stream);
 /* Block ILU0 factor of the prepared matrix (diagonal blocks inverted), for
  * parity checks of PreconditionerWrapperILU (istlpreconditionerwrappers.hh:151-190). */
 int omodels_solver_get_ilu(omodels_solver* s, double* lu_values);

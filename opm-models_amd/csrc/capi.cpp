@@ -523,6 +523,15 @@ int omodels_scalar_product(int64_t m, const double* x, const double* y, double* 
     });
 }
 
+int omodels_scalar_product_noalloc(int64_t m, const double* x_dev, const double* y_dev, double* out_dev,
+                                   void* stream)
+{
+    return guarded([&] {
+        scalarProductNoAlloc((long)m, x_dev, y_dev, out_dev, static_cast<hipStream_t>(stream));
+        return 0;
+    });
+}
+
 int omodels_solver_get_ilu(omodels_solver* s, double* lu)
 {
     return guarded([&] { s->s->downloadIlu(lu); return 0; });

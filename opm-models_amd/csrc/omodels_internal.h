@@ -213,6 +213,8 @@ template <class S> void launchDotFinal(long m, const S* x, const S* y, ExactAcc*
                                        int op, int tagOff, hipStream_t st);
 // out = <x, y> on the device (acc: one zero accumulator, left zero)
 void launchScalarProduct(long m, const double* x, const double* y, ExactAcc* acc, double* out, hipStream_t st);
+// the same with the accumulator in a static device array (noalloc.hip): no allocation
+void scalarProductNoAlloc(long m, const double* x, const double* y, double* out, hipStream_t st);
 template <class S> void launchUpdateP(long m, const S* r, const S* v, S* p, const KrylovScalars* ks,
                                       hipStream_t st);
 template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* r, double* partial,

@@ -140,6 +140,7 @@ SIGNATURES = {
     "omodels_solver_iterations": (C.c_int, [C.c_void_p]),
     "omodels_solver_get_ilu": (C.c_int, [C.c_void_p, _dp]),
     "omodels_scalar_product": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
+    "omodels_scalar_product_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
     "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
