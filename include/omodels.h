@@ -127,6 +127,12 @@ typedef struct omodels_params {
                                    the backward sweep, tile by tile as the tiles it reads finish
                                    (same results; no faster at 500k, where the sweeps have no idle
                                    workgroups); 0 (default): as a kernel of its own */
+    int32_t ilu_order;          /* block ILU(0) ordering: 0 (default) the natural cell order of the
+                                   reference's SeqILU; 1 two-colour tiles of 4x2x8 cells (colour-0
+                                   tiles first, natural order inside a tile), whose sweeps run every
+                                   tile of a colour at once -- a different preconditioner (more
+                                   iterations, each far cheaper), for the FP32-inner and ensemble
+                                   forms; 3x3 blocks, FP64 or FP32-inner solves */
 } omodels_params;
 
 void omodels_default_params(omodels_params* prm);

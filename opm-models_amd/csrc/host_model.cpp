@@ -137,6 +137,18 @@ void Model::computeFlows(double* flows, double* flores)
     OM_HIP(hipGetLastError());
 }
 
+namespace {
+// harmonic combination of two half-face values; a face between two cells
+// without porosity (diffusion) or without a dispersion coefficient has none:
+// 0 rather than 0/0
+double harmonicFace(double a, double b) { return a + b == 0.0 ? 0.0 : a * b / (a + b); }
+void checkNonNegative(const double* v, long n, const char* what)
+{
+    for (long i = 0; i < n; ++i)
+        if (!(v[i] >= 0.0) || !std::isfinite(v[i])) throw std::invalid_argument(what);
+}
+}  // namespace
+
 void Model::setDiffusion(const omodels_diffusion* d)
 {
     if (!d) { diffusion = BoDiff{}; faceDiff.release(); return; }
@@ -145,6 +157,7 @@ void Model::setDiffusion(const omodels_diffusion* d)
         throw std::invalid_argument("diffusion coefficients must be finite and >= 0");
     std::vector<double> fd(3 * (size_t)N, 0.0);
     if (d->face_diffusivity) {
+        checkNonNegative(d->face_diffusivity, 3 * N, "face diffusivities must be finite and >= 0");
         std::copy(d->face_diffusivity, d->face_diffusivity + 3 * N, fd.begin());
     } else {
         // harmonic mean of the half-face values A phi |n.d| / |d|^2
@@ -162,7 +175,7 @@ void Model::setDiffusion(const omodels_diffusion* d)
                 const double A = G.area[ax], ndot = 0.5 * h[ax], dist2 = ndot * ndot;
                 const double D0 = A * phi[I] * ndot / dist2;
                 const double D1 = A * phi[J] * ndot / dist2;
-                fd[(size_t)ax * N + I] = D0 * D1 / (D0 + D1);
+                fd[(size_t)ax * N + I] = harmonicFace(D0, D1);
             }
         }
     }
@@ -187,8 +200,10 @@ void Model::setDispersion(const omodels_dispersion* d)
         throw std::invalid_argument("dispersion needs face dispersivities or cell coefficients");
     std::vector<double> fd(3 * (size_t)N, 0.0);
     if (d->face_dispersivity) {
+        checkNonNegative(d->face_dispersivity, 3 * N, "face dispersivities must be finite and >= 0");
         std::copy(d->face_dispersivity, d->face_dispersivity + 3 * N, fd.begin());
     } else {
+        checkNonNegative(d->cell_dispersion, N, "dispersion coefficients must be finite and >= 0");
         // harmonic mean of the half-face values A E |n.d| / |d|^2
         const double* E = d->cell_dispersion;
         const double h[3] = {G.dx, G.dy, G.dz};
@@ -202,7 +217,7 @@ void Model::setDispersion(const omodels_dispersion* d)
                 const double A = G.area[ax], ndot = 0.5 * h[ax], dist2 = ndot * ndot;
                 const double E0 = A * E[I] * ndot / dist2;
                 const double E1 = A * E[J] * ndot / dist2;
-                fd[(size_t)ax * N + I] = E0 * E1 / (E0 + E1);
+                fd[(size_t)ax * N + I] = harmonicFace(E0, E1);
             }
         }
     }
@@ -257,8 +272,13 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
     if (prm_.precision < 0 || prm_.precision > 2) throw std::invalid_argument("precision must be 0, 1 or 2");
     if (prm_.precision == 2 && prm_.precond != 0) throw std::invalid_argument("the FP32 solve uses block ILU(0)");
     if (prm_.ilu_schedule < 0 || prm_.ilu_schedule > 3) throw std::invalid_argument("ilu_schedule must be 0..3");
+    if (prm_.ilu_order < 0 || prm_.ilu_order > 1) throw std::invalid_argument("ilu_order must be 0 or 1");
     const bool tiles = prm_.ilu_schedule != 1;
-    if (prm_.precond == 0 && tiles && m.bs == 3 && prm_.precision != 2) {
+    if (prm_.precond == 0 && prm_.ilu_order == 1) {
+        if (m.bs != 3 || prm_.precision == 2)
+            throw std::invalid_argument("the two-colour ILU(0) is built for 3x3 blocks and FP64 pivots");
+        colored_ = std::make_unique<ColoredIlu>(m.gx, m.gy, m.gz, m.stream);
+    } else if (prm_.precond == 0 && tiles && m.bs == 3 && prm_.precision != 2) {
         TunedIlu t;
         if (prm_.autotune && prm_.ilu_schedule == 0 && lookupTunedIlu(m.gx, m.gy, m.gz, prm_.precision, t)) {
             tuned_ = true;
@@ -267,6 +287,10 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
             const int mode = prm_.ilu_schedule == 2 ? 1 : (prm_.ilu_schedule == 3 ? 2 : 0);
             tiled_ = std::make_unique<TiledIlu>(makeTileGeom(m.gx, m.gy, m.gz), m.stream, mode);
         }
+    }
+    if (prm_.precond != 0) {
+        if (prm_.prec_order < 0) throw std::invalid_argument("PreconditionerOrder must be >= 0");
+        jacTmp_.alloc(n * (prm_.precision == 0 ? sizeof(double) : sizeof(float)));
     }
     if (prm_.precision == 0) {
         for (auto& w : wd_) w.alloc(n);
@@ -417,6 +441,13 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
         launchIlu0Factor<float>(N, bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, luf_.p, m_.levels, fail_.p, s);
         launchExtractInvDiag<float>(N, bs, m_.diagOff.p, luf_.p, dinv64_.p, s);
         m_.timer.end(s);
+    } else if (colored_) {
+        // two-colour ILU(0): FP64 pivots only; the sweeps read the off-diagonal
+        // blocks of the solver's matrix (FP64, or its FP32 copy)
+        dinv64_.alloc(bb * (size_t)N);
+        m_.timer.begin(kClsIluFactor, s);
+        colored_->factor(m_.rowPtr.p, A64_, dinv64_.p, fail_.p, s);
+        m_.timer.end(s);
     } else if (prm_.precond == 0) {
         // ILU(0) factorization in FP64 (pivot blocks stay FP64 in both forms)
         lu64_.alloc(bb * nnzb);
@@ -456,7 +487,12 @@ bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
     hipStream_t s = m_.stream;
     const int N = (int)m_.N;
     bool fused = false;
-    if (prm_.precond == 0) {
+    if (colored_) {
+        m_.timer.begin(kClsIluApply, s);
+        const S* Am = sizeof(S) == 8 ? (const S*)A64_ : (const S*)Af_.p;
+        colored_->apply<S>(m_.rowPtr.p, Am, (const double*)dinv, d, v, prm_.prec_relax, s, run);
+        m_.timer.end(s);
+    } else if (prm_.precond == 0) {
         m_.timer.begin(kClsIluApply, s);
         if (tiled_ && A && prm_.spmv_in_sweep)
             fused = tiled_->apply<S, double>(m_.rowPtr.p, m_.diagOff.p, lu, (const double*)dinv, d, v, s, run,
@@ -469,8 +505,12 @@ bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
         m_.timer.end(s);
     } else {
         (void)lu;
-        // block Jacobi with FP64 inverted pivots: v = w D^-1 d
-        launchJacobiApplyMixed(N, (const double*)dinv, d, v, prm_.prec_relax, s);
+        // block Jacobi (Dune::SeqJac(A, PreconditionerOrder, w)): that many
+        // sweeps on v from its current content, FP64 inverted pivots
+        const S* Am = sizeof(S) == 8 ? (const S*)A64_ : (const S*)Af_.p;
+        S* tmp = (S*)jacTmp_.p;
+        launchJacobiSweeps<S>(N, prm_.prec_order, m_.rowPtr.p, m_.colIdx.p, Am, (const double*)dinv, d, v, tmp,
+                              prm_.prec_relax, run, s);
     }
     return fused;
 }
@@ -541,6 +581,10 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     OM_HIP(hipMemcpyAsync(ks_.p, &k0, sizeof(k0), hipMemcpyHostToDevice, s));
     OM_HIP(hipMemsetAsync(v, 0, m * sizeof(S), s));
     OM_HIP(hipMemsetAsync(p, 0, m * sizeof(S), s));
+    // y and z start as copies of x = 0 (bicgstabsolver.hh:187-191); block
+    // Jacobi sweeps start from their content
+    OM_HIP(hipMemsetAsync(y, 0, m * sizeof(S), s));
+    OM_HIP(hipMemsetAsync(z, 0, m * sizeof(S), s));
     // the final kernel of every scalar product clears its accumulator; this
     // clears what a solve interrupted by an error may have left behind
     OM_HIP(hipMemsetAsync(m_.acc.p, 0, 2 * sizeof(ExactAcc), s));
@@ -713,6 +757,7 @@ int BiCGStabBackend::solveImpl()
 
 void BiCGStabBackend::downloadIlu(double* hostValues)
 {
+    if (colored_) throw std::invalid_argument("the two-colour ILU(0) keeps only its pivots");
     lu64_.download(hostValues, (size_t)m_.bs * m_.bs * m_.nnzb, m_.stream);
     m_.sync();
 }
@@ -723,6 +768,7 @@ void BiCGStabBackend::applyPrecond(const double* hostD, double* hostV)
     DevBuf<double> d, v;
     d.upload(hostD, m, m_.stream);
     v.alloc(m);
+    OM_HIP(hipMemsetAsync(v.p, 0, m * sizeof(double), m_.stream));   // Jacobi sweeps start from zero
     precApply<double>(lu64_.p, dinv64_.p, d.p, v.p);
     v.download(hostV, m, m_.stream);
     m_.sync();

@@ -393,31 +393,54 @@ void launchJacobiInvert(int n, int bs, const int* diag, const S* a, S* dinv, int
 template void launchJacobiInvert<double>(int, int, const int*, const double*, double*, int*, hipStream_t);
 template void launchJacobiInvert<float>(int, int, const int*, const float*, float*, int*, hipStream_t);
 
-template <int BS, class S>
-__global__ void k_jacobi_apply(int n, const double* __restrict__ dinv, const S* __restrict__ d, S* __restrict__ v,
-                               double w)
+// One block-Jacobi sweep, Dune::SeqJac's step: xout = (1 - w) xin
+// + w D^{-1} (d - (L + U) xin), the products in the operation order of the
+// other kernels (row sums, then the subtraction); D^{-1} in FP64.
+template <class S>
+__global__ __launch_bounds__(256) void k_jacobi_sweep(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+    const S* __restrict__ a, const double* __restrict__ dinv, const S* __restrict__ d, const S* __restrict__ xin,
+    S* __restrict__ xout, double w, const int* __restrict__ run)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    constexpr int BB = BS * BS;
-    const double* D = dinv + (long)BB * i;
+    if (i >= n || (run && !*run)) return;
+    S rhs[3];
 #pragma unroll
-    for (int e = 0; e < BS; ++e) {
-        double s = D[e * BS] * (double)d[(long)BS * i];
+    for (int e = 0; e < 3; ++e) rhs[e] = d[3L * i + e];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        const int c = ci[k];
+        if (c == i) continue;
+        const S* blk = a + 9L * k;
 #pragma unroll
-        for (int q = 1; q < BS; ++q) s += D[e * BS + q] * (double)d[(long)BS * i + q];
-        const double o = 0.0 + 1.0 * s;
-        v[(long)BS * i + e] = (S)(w * o);
+        for (int e = 0; e < 3; ++e) {
+            S s = blk[3 * e] * xin[3L * c];
+            s += blk[3 * e + 1] * xin[3L * c + 1];
+            s += blk[3 * e + 2] * xin[3L * c + 2];
+            rhs[e] += S(-1) * s;
+        }
+    }
+    const double* D = dinv + 9L * i;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        double o = D[3 * e] * (double)rhs[0];
+        o += D[3 * e + 1] * (double)rhs[1];
+        o += D[3 * e + 2] * (double)rhs[2];
+        xout[3L * i + e] = (S)((1.0 - w) * (double)xin[3L * i + e] + w * (0.0 + 1.0 * o));
     }
 }
 template <class S>
-void launchJacobiApplyMixed(int n, const double* dinv, const S* d, S* v, double w, hipStream_t st)
+void launchJacobiSweeps(int n, int sweeps, const int* rp, const int* ci, const S* a, const double* dinv, const S* d,
+                        S* v, S* tmp, double w, const int* run, hipStream_t st)
 {
-    dim3 g((n + 255) / 256), b(256);
-    hipLaunchKernelGGL((k_jacobi_apply<3, S>), g, b, 0, st, n, dinv, d, v, w);
+    for (int k = 0; k < sweeps; ++k) {
+        OM_HIP(hipMemcpyAsync(tmp, v, 3L * n * sizeof(S), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL((k_jacobi_sweep<S>), dim3((n + 255) / 256), dim3(256), 0, st, n, rp, ci, a, dinv, d, tmp,
+                           v, w, run);
+    }
 }
-template void launchJacobiApplyMixed<double>(int, const double*, const double*, double*, double, hipStream_t);
-template void launchJacobiApplyMixed<float>(int, const double*, const float*, float*, double, hipStream_t);
+template void launchJacobiSweeps<double>(int, int, const int*, const int*, const double*, const double*,
+                                         const double*, double*, double*, double, const int*, hipStream_t);
+template void launchJacobiSweeps<float>(int, int, const int*, const int*, const float*, const double*, const float*,
+                                        float*, float*, double, const int*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Reductions: kRedBlocks fixed partials of 256 threads, then one block.

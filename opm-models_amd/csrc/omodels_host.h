@@ -12,6 +12,7 @@
 
 #include "../../include/omodels.h"
 #include "omodels_internal.h"
+#include "ilu_colored.h"
 #include "ilu_tiled.h"
 
 namespace omodels {
@@ -166,6 +167,7 @@ private:
     DevBuf<double> lu64_, dinv64_;
     DevBuf<float> Af_, luf_, bf_, xf_;
     // Krylov work vectors (either precision)
+    DevBuf<char> jacTmp_;        // block Jacobi: the previous sweep's vector
     DevBuf<double> wd_[5];
     DevBuf<float> wf_[5];
     DevBuf<KrylovScalars> ks_;
@@ -173,6 +175,7 @@ private:
     hipEvent_t ksEv_[2] = {nullptr, nullptr};
     DevBuf<int> fail_;
     std::unique_ptr<TiledIlu> tiled_;
+    std::unique_ptr<ColoredIlu> colored_;   // ilu_order 1
     bool prepared_ = false;
     bool tuned_ = false;
 };

@@ -201,8 +201,10 @@ template <class S, class P> void launchIlu0Apply(int n, int bs, const int* rp, c
                                                  const S* d, S* v, const LevelSchedule& L, hipStream_t st);
 template <class S> void launchJacobiInvert(int n, int bs, const int* diag, const S* a, S* dinv,
                                            int* dFail, hipStream_t st);
-template <class S> void launchJacobiApplyMixed(int n, const double* dinv, const S* d, S* v, double w,
-                                               hipStream_t st);
+// `sweeps` block-Jacobi sweeps on v (3x3 blocks) from its current content; tmp: scratch of v's size
+template <class S> void launchJacobiSweeps(int n, int sweeps, const int* rp, const int* ci, const S* a,
+                                           const double* dinv, const S* d, S* v, S* tmp, double w, const int* run,
+                                           hipStream_t st);
 template <class S> void launchExtractInvDiag(int n, int bs, const int* diag, const S* lu, double* dinv, hipStream_t st);
 
 // BiCGStab steps (see bicgstab in host_solver.cpp for the sequence)

@@ -64,6 +64,7 @@ class Params(C.Structure):
         ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
         ("max_dt", C.c_double), ("min_dt", C.c_double), ("precision", C.c_int32), ("verbose", C.c_int32),
         ("ilu_schedule", C.c_int32), ("autotune", C.c_int32), ("graph", C.c_int32), ("spmv_in_sweep", C.c_int32),
+        ("ilu_order", C.c_int32),
     ]
 
 

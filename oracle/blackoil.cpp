@@ -407,10 +407,21 @@ struct Problem {
     /* the problem's dispersivity(I, J) per face (tpfalinearizer.hh:465-467):
      * given per face, or the harmonic mean of A E |n.d| / |d|^2 of the two
      * cells' dispersion coefficients E */
+    /* harmonic combination of two half-face values; a face between two cells
+     * without porosity (diffusion) or without a dispersion coefficient has
+     * none: 0 rather than 0/0 */
+    static double harmonicFace(double a, double b) { return a + b == 0.0 ? 0.0 : a * b / (a + b); }
+    static void checkNonNegative(const double* v, long n, const char* what)
+    {
+        for (long i = 0; i < n; ++i)
+            if (!(v[i] >= 0.0) || !std::isfinite(v[i])) throw std::invalid_argument(what);
+    }
     void setDispersion(const ob_dispersion& e)
     {
         if (!e.face_dispersivity && !e.cell_dispersion)
             throw std::invalid_argument("dispersion needs face dispersivities or cell coefficients");
+        if (e.face_dispersivity) checkNonNegative(e.face_dispersivity, 3 * G.N, "face dispersivities must be finite and >= 0");
+        else checkNonNegative(e.cell_dispersion, G.N, "dispersion coefficients must be finite and >= 0");
         dsp.toMassFrac = pvt.rho[O] / pvt.rho[::G];
         hasDisp = true;
         for (long I = 0; I < G.N; ++I)
@@ -424,7 +435,7 @@ struct Problem {
                     const double A = G.area(n.dir), ndot = 0.5 * G.h(n.dir), dist2 = ndot * ndot;
                     const double E0 = A * e.cell_dispersion[lo] * ndot / dist2;
                     const double E1 = A * e.cell_dispersion[hi] * ndot / dist2;
-                    n.dispersivity = E0 * E1 / (E0 + E1);
+                    n.dispersivity = harmonicFace(E0, E1);
                 }
             }
     }
@@ -442,6 +453,7 @@ struct Problem {
         dif.toFrac = pvt.rho[O] * mG / (pvt.rho[::G] * mO);
         dif.effOil = 0.5 * (d.oil_in_oil + d.oil_in_oil);
         dif.effGas = 0.5 * (d.gas_in_oil + d.gas_in_oil);
+        if (d.face_diffusivity) checkNonNegative(d.face_diffusivity, 3 * G.N, "face diffusivities must be finite and >= 0");
         hasDiff = true;
         for (long I = 0; I < G.N; ++I)
             for (int k = pat.nbPtr[I]; k < pat.nbPtr[I + 1]; ++k) {
@@ -454,7 +466,7 @@ struct Problem {
                     const double A = G.area(n.dir), ndot = 0.5 * G.h(n.dir), dist2 = ndot * ndot;
                     const double D0 = A * P->poro[lo] * ndot / dist2;
                     const double D1 = A * P->poro[hi] * ndot / dist2;
-                    n.diffusivity = D0 * D1 / (D0 + D1);
+                    n.diffusivity = harmonicFace(D0, D1);
                 }
             }
     }
@@ -799,7 +811,164 @@ struct Criterion {
 struct Precond {
     int kind, bs, n; double relax; int order;
     const int32_t *rp, *ci;
+    int nx = 0, ny = 0, nz = 0;     /* grid of the two-colour tile ordering (kind 2) */
 };
+
+/* ------------------------------------------------------------------ */
+/* Block ILU(0) in a two-colour tile ordering: the preconditioner of the
+ * product's FP32-inner and ensemble forms (the reference has only the
+ * natural order, PreconditionerWrapperILU, istlpreconditionerwrappers.hh:
+ * 151-190; this restates the product's variant so that its kernels can be
+ * checked bit for bit).  Tiles of 4 x 2 x 8 cells, coloured (tx + ty + tz)
+ * mod 2; the order is: the cells of colour-0 tiles, then those of colour-1
+ * tiles, natural inside a tile.  On the 7-point pattern ILU(0) in any order
+ * only changes the diagonal blocks:
+ *     D_I = A_II - sum_{K lower} (A_IK D_K^-1) A_KI,
+ * M = (D + L_A) D^-1 (D + U_A).  Operation order per cell: the neighbours in
+ * other tiles in column order, then those of its own tile in column order. */
+struct TileColoring {
+    int nx, ny, nz, ntx, nty;
+    static constexpr int TX = 4, TY = 2, TZ = 8;
+    TileColoring(int x, int y, int z) : nx(x), ny(y), nz(z), ntx((x + TX - 1) / TX), nty((y + TY - 1) / TY) {}
+    void ijk(long I, int& i, int& j, int& k) const
+    {
+        i = (int)(I % nx); j = (int)((I / nx) % ny); k = (int)(I / ((long)nx * ny));
+    }
+    long tile(long I) const
+    {
+        int i, j, k; ijk(I, i, j, k);
+        return i / TX + (long)ntx * (j / TY + (long)nty * (k / TZ));
+    }
+    int color(long I) const
+    {
+        int i, j, k; ijk(I, i, j, k);
+        return (i / TX + j / TY + k / TZ) & 1;
+    }
+    /* J comes before I in the ordering */
+    bool before(long J, long I) const
+    {
+        if (tile(J) == tile(I)) return J < I;
+        return color(J) < color(I);
+    }
+    /* the cells in the ordering */
+    std::vector<long> order() const
+    {
+        std::vector<long> o;
+        const long N = (long)nx * ny * nz;
+        for (int c = 0; c < 2; ++c)
+            for (long t = 0, nt = (long)ntx * nty * ((nz + TZ - 1) / TZ); t < nt; ++t) {
+                const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), tz = (int)(t / ((long)ntx * nty));
+                if (((tx + ty + tz) & 1) != c) continue;
+                for (int k = tz * TZ; k < std::min(nz, tz * TZ + TZ); ++k)
+                    for (int j = ty * TY; j < std::min(ny, ty * TY + TY); ++j)
+                        for (int i = tx * TX; i < std::min(nx, tx * TX + TX); ++i)
+                            o.push_back(i + (long)nx * (j + (long)ny * k));
+            }
+        (void)N;
+        return o;
+    }
+    /* neighbours of I in column order, split into other-tile and own-tile ones */
+    void neighbours(long I, std::vector<long>& other, std::vector<long>& own) const
+    {
+        int i, j, k; ijk(I, i, j, k);
+        const long nxy = (long)nx * ny;
+        long cand[6]; int nc = 0;
+        if (k > 0) cand[nc++] = I - nxy;
+        if (j > 0) cand[nc++] = I - nx;
+        if (i > 0) cand[nc++] = I - 1;
+        if (i + 1 < nx) cand[nc++] = I + 1;
+        if (j + 1 < ny) cand[nc++] = I + nx;
+        if (k + 1 < nz) cand[nc++] = I + nxy;
+        other.clear(); own.clear();
+        for (int q = 0; q < nc; ++q) (tile(cand[q]) == tile(I) ? own : other).push_back(cand[q]);
+    }
+};
+
+long blockAt(const int32_t* rp, const int32_t* ci, long row, long col)
+{
+    for (long k = rp[row]; k < rp[row + 1]; ++k)
+        if (ci[k] == col) return k;
+    return -1;
+}
+
+/* pivots D_I^-1 [n][9]; 2 for a singular pivot */
+int coloredIluFactor(const TileColoring& tc, const int32_t* rp, const int32_t* ci, const double* A, double* dinv)
+{
+    std::vector<long> other, own;
+    for (long I : tc.order()) {
+        double D[9], P[9], T[9];
+        std::copy(A + 9 * blockAt(rp, ci, I, I), A + 9 * blockAt(rp, ci, I, I) + 9, D);
+        tc.neighbours(I, other, own);
+        auto eliminate = [&](long K) {
+            matmul(3, A + 9 * blockAt(rp, ci, I, K), dinv + 9 * K, P);
+            matmul(3, P, A + 9 * blockAt(rp, ci, K, I), T);
+            for (int q = 0; q < 9; ++q) D[q] -= T[q];
+        };
+        for (long K : other) if (tc.before(K, I)) eliminate(K);
+        for (long K : own) if (tc.before(K, I)) eliminate(K);
+        if (!invertBlock(3, D)) return 2;
+        std::copy(D, D + 9, dinv + 9 * I);
+    }
+    return 0;
+}
+
+/* v = M^-1 d, then v *= relax */
+void coloredIluApply(const TileColoring& tc, const int32_t* rp, const int32_t* ci, const double* A,
+                     const double* dinv, const double* d, double* v, double relax)
+{
+    std::vector<long> other, own;
+    const std::vector<long> ord = tc.order();
+    for (long I : ord) {
+        double acc[3] = {d[3 * I], d[3 * I + 1], d[3 * I + 2]};
+        tc.neighbours(I, other, own);
+        for (long K : other) if (tc.before(K, I)) matvecAcc(3, A + 9 * blockAt(rp, ci, I, K), v + 3 * K, acc, -1.0);
+        for (long K : own) if (tc.before(K, I)) matvecAcc(3, A + 9 * blockAt(rp, ci, I, K), v + 3 * K, acc, -1.0);
+        double w[3] = {0, 0, 0};
+        matvecAcc(3, dinv + 9 * I, acc, w, 1.0);
+        for (int e = 0; e < 3; ++e) v[3 * I + e] = w[e];
+    }
+    for (auto it = ord.rbegin(); it != ord.rend(); ++it) {
+        const long I = *it;
+        double acc[3] = {0, 0, 0};
+        tc.neighbours(I, other, own);
+        for (long J : other) if (tc.before(I, J)) matvecAcc(3, A + 9 * blockAt(rp, ci, I, J), v + 3 * J, acc, -1.0);
+        for (long J : own) if (tc.before(I, J)) matvecAcc(3, A + 9 * blockAt(rp, ci, I, J), v + 3 * J, acc, -1.0);
+        double o[3] = {0, 0, 0};
+        matvecAcc(3, dinv + 9 * I, acc, o, 1.0);
+        for (int e = 0; e < 3; ++e) v[3 * I + e] = v[3 * I + e] + o[e];
+    }
+    if (relax != 1.0)
+        for (long k = 0; k < 3L * tc.nx * tc.ny * tc.nz; ++k) v[k] *= relax;
+}
+
+/* Block Jacobi, Dune::SeqJac(A, order, w) as PreconditionerWrapperJacobi
+ * builds it (istlpreconditionerwrappers.hh:84-89): `order` sweeps
+ * x <- (1 - w) x + w D^{-1} (d - (L + U) x) starting from the output vector's
+ * current content (BiCGStab passes y, which holds the previous t = A z, and
+ * z; both start at zero).  Order 0 leaves the output as it is (the reference
+ * default, linalgparameters.hh:74).  M holds the inverted diagonal blocks.
+ * The relaxed update's form, (1 - w) x + w x_J, is dune's to decide ("parity
+ * unpinned" for w != 1; for w = 1 it is the plain Jacobi step). */
+template <class S>
+void jacobiSweeps(const Precond& pc, const S* A, const S* M, const std::vector<S>& in, std::vector<S>& out)
+{
+    const int bs = pc.bs, bb = bs * bs, n = pc.n;
+    const int32_t *rp = pc.rp, *ci = pc.ci;
+    std::vector<S> prev;
+    for (int sweep = 0; sweep < pc.order; ++sweep) {
+        prev = out;
+        for (int i = 0; i < n; ++i) {
+            S rhs[3];
+            for (int e = 0; e < bs; ++e) rhs[e] = in[bs * i + e];
+            for (int k = rp[i]; k < rp[i + 1]; ++k)
+                if (ci[k] != i) matvecAcc(bs, A + (long)bb * k, prev.data() + (long)bs * ci[k], rhs, S(-1));
+            S o[3] = {0, 0, 0};
+            matvecAcc(bs, M + (long)bb * i, rhs, o, S(1));
+            for (int e = 0; e < bs; ++e)
+                out[bs * i + e] = (S(1) - (S)pc.relax) * prev[bs * i + e] + (S)pc.relax * o[e];
+        }
+    }
+}
 
 /* BiCGStabSolver::apply (bicgstabsolver.hh:120-330) with the aliasing of the
  * reference (h == x, s == r, t == y).  Returns 0 converged, 1 not converged,
@@ -813,15 +982,9 @@ int bicgstab(int n, int bs, const int32_t* rp, const int32_t* ci, const S* A, co
     const S breakdownEps = std::numeric_limits<S>::min() * S(1e10);
     auto precApply = [&](std::vector<S>& out, const std::vector<S>& in) {
         if (pc.kind == 0) ilu0Apply<S>(n, bs, rp, ci, M, in.data(), out.data(), (S)pc.relax);
-        else {
-            /* block Jacobi: out = relax * D^{-1} in (M holds the inverted diagonals) */
-            const int bb = bs * bs;
-            for (int i = 0; i < n; ++i) {
-                S o[3] = {0, 0, 0};
-                matvecAcc(bs, M + (long)bb * i, in.data() + (long)bs * i, o, S(1));
-                for (int e = 0; e < bs; ++e) out[bs * i + e] = (S)pc.relax * o[e];
-            }
-        }
+        else if (pc.kind == 1) jacobiSweeps<S>(pc, A, M, in, out);
+        else if constexpr (sizeof(S) == 8)
+            coloredIluApply(TileColoring(pc.nx, pc.ny, pc.nz), rp, ci, A, M, in.data(), out.data(), pc.relax);
     };
     x.assign(m, S(0));
     std::vector<S> r = b;
@@ -887,13 +1050,22 @@ int bicgstab(int n, int bs, const int32_t* rp, const int32_t* ci, const S* A, co
  * prepare the preconditioner, run, copy back. */
 template <class S>
 int backendSolve(int n, int bs, const int32_t* rp, const int32_t* ci, const double* J,
-                 const double* b, double* xOut, const ob_params* prm, int* iters)
+                 const double* b, double* xOut, const ob_params* prm, int* iters, const int* dims = nullptr)
 {
     const int bb = bs * bs;
     const long nnzb = rp[n];
     std::vector<S> A(J, J + (long)bb * nnzb);
     std::vector<S> M;
-    if (prm->precond == 0) {
+    const bool colored = prm->precond == 0 && prm->ilu_order == 1;
+    if (colored) {
+        if constexpr (sizeof(S) == 8) {
+            if (!dims || bs != 3 || (long)dims[0] * dims[1] * dims[2] != n) return -1;
+            M.resize((long)bb * n);
+            if (coloredIluFactor(TileColoring(dims[0], dims[1], dims[2]), rp, ci, A.data(), M.data()) != 0) return 2;
+        } else {
+            return -1;
+        }
+    } else if (prm->precond == 0) {
         M = A;
         int rc = ilu0Factor<S>(n, bs, rp, ci, M.data());
         if (rc != 0) return 2;
@@ -907,7 +1079,8 @@ int backendSolve(int n, int bs, const int32_t* rp, const int32_t* ci, const doub
         }
     }
     std::vector<S> bv(b, b + (long)n * bs), x;
-    Precond pc{prm->precond, bs, n, prm->prec_relax, prm->prec_order, rp, ci};
+    Precond pc{colored ? 2 : prm->precond, bs, n, prm->prec_relax, prm->prec_order, rp, ci};
+    if (dims) { pc.nx = dims[0]; pc.ny = dims[1]; pc.nz = dims[2]; }
     double absTol = prm->lin_abs_tol < 0 ? prm->newton_tol / 100.0 : prm->lin_abs_tol;
     int rc = bicgstab<S>(n, bs, rp, ci, A.data(), M.data(), bv, x, pc, (S)prm->lin_tol, (S)absTol,
                          (S)prm->lin_max_error, prm->lin_max_iter, iters);
@@ -1156,6 +1329,46 @@ int ob_ilu0_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
     return 0;
 }
 
+int ob_jacobi_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                    const double* d, double* v, int order, double relax)
+{
+    /* the preconditioner of ob_bicgstab with precond = 1 applied to v in place */
+    const int bb = bs * bs;
+    std::vector<double> M((size_t)bb * n);
+    for (int i = 0; i < n; ++i) {
+        int k = row_ptr[i];
+        while (col_idx[k] != i) ++k;
+        std::copy(vals + (long)bb * k, vals + (long)bb * (k + 1), M.begin() + (long)bb * i);
+        if (!invertBlock(bs, M.data() + (long)bb * i)) return 2;
+    }
+    Precond pc{1, bs, n, relax, order, row_ptr, col_idx};
+    std::vector<double> in(d, d + (long)n * bs), out(v, v + (long)n * bs);
+    jacobiSweeps<double>(pc, vals, M.data(), in, out);
+    std::copy(out.begin(), out.end(), v);
+    return 0;
+}
+
+int ob_colored_ilu(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                   const double* d, double* v, double relax)
+{
+    const TileColoring tc(nx, ny, nz);
+    const long n = (long)nx * ny * nz;
+    std::vector<double> dinv(9 * n);
+    if (coloredIluFactor(tc, row_ptr, col_idx, vals, dinv.data()) != 0) return 2;
+    coloredIluApply(tc, row_ptr, col_idx, vals, dinv.data(), d, v, relax);
+    return 0;
+}
+
+int ob_bicgstab_grid(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                     const double* b, double* x, const ob_params* prm, int32_t* iterations)
+{
+    const int dims[3] = {nx, ny, nz};
+    int it = 0;
+    const int rc = backendSolve<double>(nx * ny * nz, 3, row_ptr, col_idx, vals, b, x, prm, &it, dims);
+    *iterations = it;
+    return rc;
+}
+
 double ob_dot(int64_t m, const double* x, const double* y)
 {
     std::vector<double> a(x, x + m), b(y, y + m);
@@ -1266,8 +1479,9 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
                     auto c = clk::now();
                     int it = 0;
                     int rc;
-                    if (prm->solver_float) rc = backendSolve<float>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it);
-                    else rc = backendSolve<double>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it);
+                    const int dims[3] = {p->nx, p->ny, p->nz};
+                    if (prm->solver_float) rc = backendSolve<float>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it, dims);
+                    else rc = backendSolve<double>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it, dims);
                     auto d = clk::now();
                     tSol += std::chrono::duration<double>(d - c).count();
                     if (rc != 0) {

@@ -69,6 +69,7 @@ typedef struct ob_params {
     double min_dt;              /* MinTimeStepSize (0) */
     int32_t solver_float;       /* 1: linear solve in float (LinearSolverScalar=float) */
     int32_t verbose;
+    int32_t ilu_order;          /* 0: natural (the reference); 1: two-colour 4x2x8 tiles */
 } ob_params;
 
 /* Per-Newton-iteration record, the tuple the reference prints at
@@ -137,6 +138,16 @@ int ob_ilu0_factor(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx
                    double* vals);
 int ob_ilu0_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx,
                   const double* lu, const double* d, double* v, double relax);
+/* block ILU(0) in the two-colour tile ordering (ilu_order = 1) of an
+ * nx x ny x nz 7-point pattern: factor, then v = M^-1 d (times relax) */
+int ob_colored_ilu(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                   const double* d, double* v, double relax);
+/* ob_bicgstab on the pattern of an nx x ny x nz grid (needed for ilu_order 1) */
+int ob_bicgstab_grid(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                     const double* b, double* x, const ob_params* prm, int32_t* iterations);
+/* block Jacobi of ob_bicgstab (precond = 1): `order` sweeps on v in place */
+int ob_jacobi_apply(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                    const double* d, double* v, int order, double relax);
 /* scalar product of the restated BiCGStab: the exact sum of the products
  * x[i]*y[i], rounded once to the nearest double */
 double ob_dot(int64_t m, const double* x, const double* y);

@@ -70,6 +70,7 @@ class ObParams(C.Structure):
         ("prec_relax", C.c_double), ("prec_order", C.c_int32), ("max_dt_divisions", C.c_int32),
         ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
         ("max_dt", C.c_double), ("min_dt", C.c_double), ("solver_float", C.c_int32), ("verbose", C.c_int32),
+        ("ilu_order", C.c_int32),
     ]
 
 
@@ -112,6 +113,9 @@ def lib():
         L.ob_ilu0_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
         L.ob_spmv.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp]
         L.ob_dot.argtypes = [C.c_int64, _dp, _dp]
+        L.ob_colored_ilu.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
+        L.ob_bicgstab_grid.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
+        L.ob_jacobi_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double]
         L.ob_dot.restype = C.c_double
         L.ob_bicgstab.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
         L.ob_update_primary.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp, _u8p, _u8p,
@@ -260,6 +264,32 @@ def spmv(n, bs, rp, ci, vals, x):
     y = np.zeros(n * bs)
     lib().ob_spmv(n, bs, P(rp, _ip), P(ci, _ip), P(vals), P(x), P(y))
     return y
+
+
+def colored_ilu_apply(dims, rp, ci, vals, d, relax=1.0):
+    """Block ILU(0) in the two-colour tile ordering: factor, then M^-1 d."""
+    nx, ny, nz = dims
+    v = np.zeros(3 * nx * ny * nz)
+    rc = lib().ob_colored_ilu(nx, ny, nz, P(rp, _ip), P(ci, _ip), P(np.ascontiguousarray(vals, dtype=np.float64)),
+                              P(np.ascontiguousarray(d, dtype=np.float64)), P(v), relax)
+    return rc, v
+
+
+def bicgstab_grid(dims, rp, ci, vals, b, prm):
+    nx, ny, nz = dims
+    x = np.zeros(3 * nx * ny * nz)
+    its = C.c_int32()
+    rc = lib().ob_bicgstab_grid(nx, ny, nz, P(rp, _ip), P(ci, _ip), P(np.ascontiguousarray(vals, dtype=np.float64)),
+                                P(np.ascontiguousarray(b, dtype=np.float64)), P(x), C.byref(prm), C.byref(its))
+    return rc, x, its.value
+
+
+def jacobi_apply(n, bs, rp, ci, vals, d, v0, order, relax=1.0):
+    """`order` block-Jacobi sweeps from v0 (the preconditioner of bicgstab with precond=1)."""
+    v = np.array(v0, dtype=np.float64, copy=True)
+    rc = lib().ob_jacobi_apply(n, bs, P(rp, _ip), P(ci, _ip), P(np.ascontiguousarray(vals, dtype=np.float64)),
+                               P(np.ascontiguousarray(d, dtype=np.float64)), P(v), order, relax)
+    return rc, v
 
 
 def dot(x, y):

@@ -111,7 +111,31 @@ def test_ilu0_factor_and_apply_parity(dims, schedule):
         assert np.array_equal(v.view(np.uint64), vo.view(np.uint64))
 
 
-@pytest.mark.parametrize("precond", [0, 1])
+@pytest.mark.parametrize("dims", [(6, 5, 1), (12, 10, 4)])
+@pytest.mark.parametrize("order,relax", [(0, 1.0), (1, 1.0), (2, 1.0), (3, 1.0), (3, 0.8)])
+def test_bicgstab_block_jacobi_parity(dims, order, relax):
+    """Block Jacobi with the reference's sweep count (PreconditionerOrder,
+    0 by default: a breakdown in the first iteration) and relaxation: return
+    code, iteration count and solution bit for bit, converged or not."""
+    c, J, r, pat = _system(dims, seed=4)
+    m = om.Model(c)
+    kw = dict(lin_tol=1e-8, precond=1, prec_order=order, prec_relax=relax, lin_max_iter=400)
+    s = om.Solver(m, om.default_params(**kw))
+    s.prepare(J, r)
+    rc, x, its = s.solve()
+    rco, xo, itso = orc.bicgstab(c.N, 3, pat["row_ptr"], pat["col_idx"], J, r, orc.params(**kw))
+    assert (rc, its) == (rco, itso)
+    if order == 0:
+        assert rc == 2 and its == 0
+    if rc != 2:
+        assert np.array_equal(x.view(np.uint64), xo.view(np.uint64))
+    d = np.random.default_rng(3).standard_normal(3 * c.N)
+    v = s.apply_precond(d)
+    rco2, vo = orc.jacobi_apply(c.N, 3, pat["row_ptr"], pat["col_idx"], J, d, np.zeros(3 * c.N), order, relax)
+    assert np.array_equal(v.view(np.uint64), vo.view(np.uint64))
+
+
+@pytest.mark.parametrize("precond", [0])
 @pytest.mark.parametrize("dims", [(6, 5, 1), (12, 10, 4)])
 def test_bicgstab_parity(dims, precond):
     c, J, r, pat = _system(dims, seed=4)
@@ -236,7 +260,9 @@ GRAPH_CONFIGS = [
     dict(precision=1),                        # FP32 inner BiCGStab in FP64 refinement
     dict(precision=2),                        # FP32 solve, level-scheduled ILU(0)
     dict(ilu_schedule=1),                     # FP64, one launch per ILU level
-    dict(precond=1, prec_relax=0.9),          # block Jacobi
+    dict(precond=1, prec_order=3, prec_relax=0.9),  # block Jacobi, three sweeps
+    dict(ilu_order=1),                        # two-colour tile ILU(0)
+    dict(ilu_order=1, precision=1),           # the same, FP32 inner
 ]
 
 
@@ -610,22 +636,28 @@ def test_diffusion_newton_run_matches_the_oracle():
 
 # ----------------------------------------------------------------------------
 # Dispersion in the oil phase (blackoildispersionmodule.hh:176-254)
+@pytest.mark.parametrize("with_diffusion", [True, False], ids=["with_diffusion", "dispersion_only"])
 @pytest.mark.parametrize("dims,seed", [((1, 1, 1), 1), ((4, 3, 2), 2), ((33, 9, 3), 3), ((40, 40, 10), 5)])
-def test_dispersion_two_linearizations_and_flows_parity(dims, seed):
-    """Dispersion (with diffusion) over two linearizations of one state: the
-    first sees zero velocities, the second the first's; residual, Jacobian,
-    the velocities handed on and the FLOWS bit-exact with the oracle."""
+def test_dispersion_two_linearizations_and_flows_parity(dims, seed, with_diffusion):
+    """Dispersion (with and without diffusion) over two linearizations of one
+    state: the first sees zero velocities, the second the first's; residual,
+    Jacobian, the velocities handed on and the FLOWS bit-exact with the
+    oracle.  Every tenth cell has no dispersion coefficient, so faces between
+    two such cells have none (0, not 0/0)."""
     c = case(*dims, seed=seed)
     pv, mean = mixed_state(c, seed=seed)
     prob = orc.problem_from_case(c)
     st1 = orc.storage(prob, *mixed_state(c, seed=seed + 100))
     E = np.random.default_rng(seed).uniform(5.0, 5e3, c.N)
+    E[::10] = 0.0
     m = om.Model(c)
     m.set_solution(pv, mean)
     m.set_storage1(st1)
-    m.set_diffusion(D_OIL, D_GAS)
+    if with_diffusion:
+        m.set_diffusion(D_OIL, D_GAS)
     m.set_dispersion(cell_dispersion=E)
-    dif, dsp = orc.diffusion(D_OIL, D_GAS), orc.dispersion(cell_dispersion=E)
+    dif = orc.diffusion(D_OIL, D_GAS) if with_diffusion else None
+    dsp = orc.dispersion(cell_dispersion=E)
     nv = np.zeros(c.N)
     for it in range(2):
         J, r = m.linearize(43200.0, iteration_index=1)
@@ -646,6 +678,10 @@ def test_dispersion_two_linearizations_and_flows_parity(dims, seed):
     assert np.array_equal(_bits(J0), _bits(J0o)) and np.array_equal(_bits(r0), _bits(r0o))
     with pytest.raises(RuntimeError):
         m.cell_velocity()
+    bad = np.full(c.N, 3.0)
+    bad[0] = -1.0
+    with pytest.raises(RuntimeError):
+        m.set_dispersion(cell_dispersion=bad)
 
 
 def test_dispersion_newton_run_matches_the_oracle():
@@ -665,11 +701,72 @@ def test_dispersion_newton_run_matches_the_oracle():
                                                         nthreads=4, diffusion=orc.diffusion(D_OIL, D_GAS),
                                                         dispersion=orc.dispersion(face_dispersivity=fd))
     assert rc == 0 and rco == 0
-    assert [l[:3] for l in log] == [l[:3] for l in logo]
+    assert log == logo
     ep, es, ex = _cmp_states(hist, histo)
     assert ep.max() <= 1e-8 and es.max() <= 1e-8 and ex.max() <= 1e-8, (ep, es, ex)
+    assert np.array_equal(_bits(hist), _bits(histo))
     rc2, _, _, hist2, *_ = orc.simulate(prob, c.pv0, c.mean0, 6, 86400.0, orc.params(**kw), nthreads=4,
                                        diffusion=orc.diffusion(D_OIL, D_GAS))
     assert rc2 == 0
     ep2, es2, ex2 = _cmp_states(hist2, histo)
     assert max(ep2.max(), ex2.max()) > 1e-7
+
+
+# ----------------------------------------------------------------------------
+# Failure handling of the Newton method and the time loop (numerical failures
+# forced through the parameters; every kernel runs to its normal end)
+def test_failed_attempts_halve_the_step_and_restore_the_state():
+    """One linear iteration cannot reach the linear tolerance, so every
+    attempt fails (FvBaseProblem::timeIntegration, fvbaseproblem.hh:512-557):
+    three attempts at 1, 1/2 and 1/4 day, return code 3, and the state handed
+    back is the start of the step -- as the oracle does."""
+    c = case(6, 5, 3, seed=4)
+    prob = orc.problem_from_case(c)
+    kw = dict(lin_max_iter=1, max_dt_divisions=3)
+    m = om.Model(c)
+    m.set_solution(c.pv0, c.mean0)
+    rc, hist, dts, log, st = m.simulate(om.default_params(**kw), 1, 86400.0)
+    rco, pvo, mo, histo, dtso, logo, _ = orc.simulate(prob, c.pv0, c.mean0, 1, 86400.0, orc.params(**kw))
+    assert rc == 3 and rco == 3
+    assert log == logo
+    attempts = sorted({(l[1], l[7]) for l in log})
+    assert attempts == [(0, 86400.0), (1, 43200.0), (2, 21600.0)]
+    pv, mean = m.get_solution()
+    assert np.array_equal(_bits(pv), _bits(c.pv0)) and np.array_equal(mean, c.mean0)
+    assert st.failed_attempts == 3
+
+
+def test_recovered_attempts_follow_the_oracle():
+    """Long steps with few Newton iterations allowed: attempts fail, the step
+    is halved and the state restored, later attempts converge and the run goes
+    on; attempts, step sizes, iteration counts and states bit for bit."""
+    c = case(12, 10, 4, seed=5)
+    prob = orc.problem_from_case(c)
+    kw = dict(newton_max_iter=4, max_dt=300 * 86400.0)
+    m = om.Model(c)
+    m.set_solution(c.pv0, c.mean0)
+    rc, hist, dts, log, st = m.simulate(om.default_params(**kw), 4, 300 * 86400.0)
+    rco, pvo, mo, histo, dtso, logo, _ = orc.simulate(prob, c.pv0, c.mean0, 4, 300 * 86400.0, orc.params(**kw))
+    assert rc == 0 and rco == 0
+    assert any(l[1] > 0 for l in log), "no attempt failed: the test does not exercise the restore"
+    assert log == logo
+    assert np.array_equal(_bits(dts), _bits(dtso))
+    assert np.array_equal(_bits(hist), _bits(histo))
+
+
+def test_non_finite_update_is_a_numerical_problem():
+    """A NaN in the Newton update makes the update fail with the
+    NumericalProblem code (2), as the reference's update does
+    (newtonmethod.hh:677-678); the oracle says the same."""
+    c = case(6, 5, 3, seed=4)
+    prob = orc.problem_from_case(c)
+    m = om.Model(c)
+    m.set_solution(c.pv0, c.mean0)
+    delta = np.zeros(3 * c.N)
+    delta[7] = np.nan
+    rc, nsw, sw = m.newton_update(delta, om.default_params())
+    rco, *_ = orc.update_primary(prob, c.pv0, c.mean0, delta, np.zeros(c.N, np.uint8), orc.params())
+    assert rc == 2 and rco == 2
+    delta[7] = np.inf
+    rc, nsw, sw = m.newton_update(delta, om.default_params())
+    assert rc == 2

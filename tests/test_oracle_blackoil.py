@@ -411,14 +411,19 @@ def test_ilu0_is_exact_lu_for_block_tridiagonal():
 
 @pytest.mark.parametrize("precond", [0, 1])
 def test_bicgstab_matches_dense_solve(precond):
+    """ILU(0), and block Jacobi with three sweeps (one or two sweeps do not
+    precondition this matrix well enough to converge)."""
     c, J, r, pat = _system((5, 4, 2), seed=3)
+    order = 3 if precond == 1 else 0
     rc, x, its = orc.bicgstab(c.N, 3, pat["row_ptr"], pat["col_idx"], J, r,
-                              orc.params(lin_tol=1e-13, precond=precond, lin_max_iter=5000, lin_abs_tol=0.0))
+                              orc.params(lin_tol=1e-13, precond=precond, prec_order=order, lin_max_iter=5000,
+                                         lin_abs_tol=0.0))
     assert rc == 0
     xd = np.linalg.solve(_dense(c, J, pat), r)
     assert np.max(np.abs(x - xd)) <= 1e-8 * np.abs(xd).max()
     # at the reference tolerance the max-norm residual is reduced by 1e-3
-    rc, x, its = orc.bicgstab(c.N, 3, pat["row_ptr"], pat["col_idx"], J, r, orc.params(precond=precond))
+    rc, x, its = orc.bicgstab(c.N, 3, pat["row_ptr"], pat["col_idx"], J, r,
+                              orc.params(precond=precond, prec_order=order))
     res = r - orc.spmv(c.N, 3, pat["row_ptr"], pat["col_idx"], J, x)
     assert rc == 0 and np.abs(res).max() <= 1.2e-3 * np.abs(r).max()
 
@@ -820,3 +825,33 @@ def test_dispersion_conserves_and_flows_sum_to_the_residual():
             a = a + fl[k]
         acc[3 * I:3 * I + 3] = a
     assert np.array_equal(acc.view(np.uint64), r.view(np.uint64))
+
+
+def test_face_coefficients_of_cells_without_porosity_or_dispersion_are_zero():
+    """A face between two cells without a dispersion coefficient has no
+    dispersivity (0, not the harmonic mean's 0/0); zero coefficients
+    everywhere give the residual of the model without dispersion, and
+    negative or non-finite coefficients are rejected."""
+    c = _case_nowells(4, 3, 2, seed=6)
+    pv, mean = mixed_state(c, seed=6)
+    prob = orc.problem_from_case(c)
+    J0, r0 = _no_acc(prob, pv, mean, None)
+    E = np.zeros(c.N)
+    J, r, nv = _disp_lin(prob, pv, mean, np.full(c.N, 1e-5), orc.dispersion(cell_dispersion=E))
+    Jn, rn = orc.linearize(prob, pv, mean, orc.storage(prob, pv, mean), 1e300)
+    assert np.all(np.isfinite(r)) and np.all(np.isfinite(J))
+    assert np.array_equal(r, rn) and np.array_equal(J, Jn)
+    # half the cells without a coefficient: every face touching two of them is 0
+    E[::2] = 0.0
+    E[1::2] = 7.0
+    J, r, nv = _disp_lin(prob, pv, mean, np.full(c.N, 1e-5), orc.dispersion(cell_dispersion=E))
+    assert np.all(np.isfinite(r)) and np.all(np.isfinite(J))
+    for bad in (-1.0, np.nan, np.inf):
+        Eb = np.full(c.N, 3.0)
+        Eb[5] = bad
+        with pytest.raises(AssertionError):
+            _disp_lin(prob, pv, mean, np.zeros(c.N), orc.dispersion(cell_dispersion=Eb))
+    fd = np.ones(3 * c.N)
+    fd[4] = -2.0
+    with pytest.raises(AssertionError):
+        _no_acc(prob, pv, mean, _diff(face_diffusivity=fd))
