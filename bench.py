@@ -92,7 +92,7 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=Fals
     dt = 86400.0
     # the reference's time-step control with steps of at most one day: a step
     # whose Newton solve fails is retried at half the size
-    prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1, verbose=1 if progress else 0)
+    prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1, ilu_order=1, verbose=1 if progress else 0)
     budget = 64 * 2.0 ** 30
     srv = om.Server(prm, budget, 64)
     for c, a in zip(cases, arrivals):
@@ -108,7 +108,8 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=Fals
         "requests": n_req, "steps_per_request": n_steps, "max_dt_days": 1.0,
         "cells_min": int(min(c.N for c in cases)), "cells_max": int(max(c.N for c in cases)),
         "cells_total": int(sum(c.N for c in cases)),
-        "arrival_rate_per_s": rate, "precision": "fp32-inner/fp64-outer", "newton_tol": 1e-7,
+        "arrival_rate_per_s": rate, "precision": "fp32-inner/fp64-outer",
+        "precond": "block-ILU0, two-colour 4x2x8 tiles", "newton_tol": 1e-7,
         "hbm_budget_GiB": 64, "max_batch": 64, "max_running": summ["max_running"],
         "batch_steps": summ["batch_steps"],
         "wall_s": round(wall, 3), "solves_per_s": round(n_req / wall, 4),
@@ -179,26 +180,53 @@ def copy_calibration(mib):
     hip.hipFree(b)
 
 
-def beside_runs(om, config, dt, steps=2):
+def beside_runs(om, np, config, dt, steps=2):
     """Newton iterations/s of workloads reported beside the headline: the
     432k-cell black-oil grid (120x120x30) in FP64, and the headline grid with
-    the FP32 inner solve inside FP64 refinement.  One untimed step each first."""
+    the FP32 inner solve inside FP64 refinement, with the natural-order and
+    with the two-colour tile ILU(0), and in FP64 with the two-colour ILU(0).
+    One untimed step each first, from the same initial state as the headline;
+    the error of each against the FP64 natural-order run (the headline's
+    arithmetic) is the largest relative pressure difference after the timed
+    steps.  ILU-apply kernel time from HIP events on every 4th launch."""
     out = {}
-    runs = [("c3_fp64", "c3", 0), ("%s_fp32_inner" % config, config, 1)]
-    for key, cfg_name, precision in runs:
+    runs = [("c3_fp64", "c3", 0, 0), ("%s_fp32_inner" % config, config, 1, 0),
+            ("%s_fp32_inner_colored" % config, config, 1, 1), ("%s_fp64_colored" % config, config, 0, 1),
+            ("c3_fp32_inner_colored", "c3", 1, 1)]
+    ref = {}
+    for key, cfg_name, precision, order in [("c3_fp64", "c3", 0, 0)] + runs + [("ref", config, 0, 0)]:
+        if key in out:
+            continue
         cfg = CONFIGS[cfg_name]
         case = om.BlackOilCase(om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024, **cfg.get("spec", {})))
-        prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, autotune=1)
+        prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, ilu_order=order, autotune=1)
         model = om.Model(case)
         model.set_solution(case.pv0, case.mean0)
         rc, _, _, _, _ = model.simulate(prm, 1, dt, keep_history=False)
+        model.set_profiling(4)
         t0 = time.time()
         rc2, _, _, _, st = model.simulate(prm, steps, dt, keep_history=False)
         wall = time.time() - t0
-        out[key] = {"workload": cfg["label"], "newton_iters_per_s": round(st.newton_iters / wall, 3),
-                    "newton_iters": st.newton_iters, "linear_iters_per_newton": round(st.linear_iters / max(st.newton_iters, 1), 2),
-                    "steps": steps, "ok": rc == 0 and rc2 == 0}
+        pv = model.get_solution()[0]
+        kt = model.kernel_times()
         model.close()
+        if key == "ref" or (precision == 0 and order == 0):
+            ref[cfg_name] = pv
+        if key == "ref":
+            continue
+        ms, cnt = kt["ilu_apply"]
+        out[key] = {"workload": cfg["label"], "precision": "fp64" if precision == 0 else "fp32-inner/fp64-outer",
+                    "ilu_order": "natural" if order == 0 else "two-colour 4x2x8 tiles",
+                    "newton_iters_per_s": round(st.newton_iters / wall, 3),
+                    "newton_iters": st.newton_iters,
+                    "linear_iters_per_newton": round(st.linear_iters / max(st.newton_iters, 1), 2),
+                    "ilu_apply_avg_us": round(ms / max(cnt, 1) * 1e3, 1),
+                    "steps": steps, "ok": rc == 0 and rc2 == 0, "_pv": pv, "_cfg": cfg_name}
+    for key, v in out.items():
+        pv = v.pop("_pv")
+        r = ref.get(v.pop("_cfg"))
+        if r is not None:
+            v["max_rel_pressure_diff_vs_fp64_natural"] = float(np.max(np.abs(pv[1::3] - r[1::3]) / np.abs(r[1::3])))
     return out
 
 
@@ -361,7 +389,7 @@ def main():
                                                  "achieved_GBs": round(eb / (eavg * 1e-3) / 1e9, 1),
                                                  "mcells_per_s": round(N / (eavg * 1e-3) / 1e6, 1)}
     if not args.no_extra:
-        line["beside"] = beside_runs(om, args.config, dt)
+        line["beside"] = beside_runs(om, np, args.config, dt)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc

@@ -359,6 +359,13 @@ int omodels_server_result(omodels_server* s, int id, double* pv, uint8_t* meanin
                           double* dt_hist, omodels_run_stats* stats);
 int omodels_server_log(omodels_server* s, int id, omodels_iter_log* log, int cap, int32_t* len);
 int omodels_server_summary(omodels_server* s, double* wall_s, int64_t* batch_steps, int32_t* max_running);
+/* Test hook for the failure handling of the tiled ILU's bounded dependency
+ * waits (ilu_order 0): sets the error flag that a wait which ran into its
+ * bound sets, so the waits of the next batched solve give up at once; every
+ * member of that solve then fails it as a NumericalProblem (its time step is
+ * halved and retried) and the flag and exchange slots are reset.  No
+ * counterpart in the reference (whose sweeps are sequential). */
+int omodels_server_inject_ilu_wait_error(omodels_server* s);
 int omodels_server_set_profiling(omodels_server* s, int enable);
 int omodels_server_kernel_times(omodels_server* s, double* ms, int64_t* counts, int n);
 

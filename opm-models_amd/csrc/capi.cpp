@@ -643,6 +643,14 @@ int omodels_server_summary(omodels_server* s, double* wall_s, int64_t* batch_ste
     });
 }
 
+int omodels_server_inject_ilu_wait_error(omodels_server* s)
+{
+    return guarded([&] {
+        s->s->ensemble().raiseIluError();
+        return 0;
+    });
+}
+
 int omodels_server_set_profiling(omodels_server* s, int enable)
 {
     return guarded([&] {

@@ -29,15 +29,18 @@
 namespace omodels {
 
 // ---------------------------------------------------------------------------
-size_t Ensemble::memoryEstimate(const omodels_blackoil_problem* p, int precision)
+size_t Ensemble::memoryEstimate(const omodels_blackoil_problem* p, const omodels_params& prm)
 {
     const size_t N = (size_t)p->nx * p->ny * p->nz;
     const size_t nnzb = 7 * N;
-    size_t b = nnzb * 9 * 8 * 2 + N * 9 * 8;                 // J, ILU factor, FP64 pivots
-    if (precision == 1) b += nnzb * 9 * 4 * 2;                // FP32 J and factor
+    const int precision = prm.precision;
+    const bool colored = prm.precond == 0 && prm.ilu_order == 1;
+    // J, the ILU factor (the two-colour ILU keeps only pivots), FP64 pivots
+    size_t b = nnzb * 9 * 8 * (colored ? 1 : 2) + N * 9 * 8;
+    if (precision == 1) b += nnzb * 9 * 4 * (colored ? 1 : 2);   // FP32 J and factor
     b += N * 3 * 8 * (kNumVec + 5) + N * 3 * 4 * kNumVec;      // vectors and state
     b += N * (4 * 8 + 6 * 4 + 3);                            // topology, transmissibilities, flags
-    b += N * 9 * 8 + (precision == 1 ? N * 9 * 4 : 0);       // exchange slots of the ILU sweeps
+    if (!colored) b += N * 9 * 8 + (precision == 1 ? N * 9 * 4 : 0);   // exchange slots of the tiled sweeps
     return b;
 }
 
@@ -53,6 +56,8 @@ Ensemble::Ensemble(const omodels_params& prm, hipStream_t s) : prm_(prm)
     // tile (serving stream of 32 requests: 1.57 solves/s batched, 1.48 with the
     // form chosen per launch, 1.42 streaming); ilu_schedule 2 forces streaming
     ilu_->setSweepMode(prm_.ilu_schedule == 2 ? 1 : 2);
+    if (prm_.ilu_order < 0 || prm_.ilu_order > 1) throw std::invalid_argument("ilu_order must be 0 or 1");
+    if (colored()) col_ = std::make_unique<ColoredIluSet>();
     OM_HIP(hipHostMalloc((void**)&hostKs_, sizeof(KrylovScalars) * kMaxMembers, hipHostMallocDefault));
     OM_HIP(hipHostMalloc((void**)&hostD_, sizeof(double) * kMaxMembers, hipHostMallocDefault));
     OM_HIP(hipHostMalloc((void**)&hostI_, sizeof(int) * 2 * kMaxMembers, hipHostMallocDefault));
@@ -127,12 +132,12 @@ int Ensemble::add(const omodels_blackoil_problem* p, const double* pv, const uin
     M.st1.alloc(3 * N);
     M.res.alloc(3 * N);
     M.jac.alloc(9 * (size_t)M.nnzb);
-    M.lu64.alloc(9 * (size_t)M.nnzb);
+    if (!colored()) M.lu64.alloc(9 * (size_t)M.nnzb);      // the two-colour ILU keeps only pivots
     M.dinv64.alloc(9 * (size_t)N);
     for (int v = 0; v < kNumVec; ++v) if (v != kVb) M.dv[v].alloc(3 * N);
     if (prm_.precision == 1) {
         M.Af.alloc(9 * (size_t)M.nnzb);
-        M.luf.alloc(9 * (size_t)M.nnzb);
+        if (!colored()) M.luf.alloc(9 * (size_t)M.nnzb);
         for (int v = 0; v < kNumVec; ++v) M.fv[v].alloc(3 * N);
     }
     M.counters.alloc(4);
@@ -208,7 +213,13 @@ void Ensemble::rebuild()
     nChunks_ = (int)chunkT.size();
     nSrc_ = (int)srcT.size();
     partial_.alloc(std::max<size_t>(2 * (size_t)nChunks_, 1));
-    ilu_->setInstances(geoms, stream_);
+    if (colored()) {
+        std::vector<ColGeom> cg(B);
+        for (int b = 0; b < B; ++b) cg[b] = colGeomOf(members_[b]->G.nx, members_[b]->G.ny, members_[b]->G.nz);
+        col_->setInstances(cg, stream_);
+    } else {
+        ilu_->setInstances(geoms, stream_);
+    }
     hostDesc_ = d;
     OM_HIP(hipStreamSynchronize(stream_));
     dirty_ = false;
@@ -257,6 +268,30 @@ std::vector<IluInst> Ensemble::iluInsts(const int* act, bool floatFactors, int i
     return v;
 }
 
+std::vector<ColInst> Ensemble::colInsts(const int* act, bool floatMatrix, int idD, int idV) const
+{
+    std::vector<ColInst> v(members_.size());
+    for (size_t b = 0; b < members_.size(); ++b) {
+        const Member& M = *members_[b];
+        ColInst& I = v[b];
+        std::memset(&I, 0, sizeof(I));
+        I.g = colGeomOf(M.G.nx, M.G.ny, M.G.nz);
+        I.rp = M.rowPtr.p;
+        I.dinv = M.dinv64.p;
+        I.active = act + b;
+        if (floatMatrix) {
+            I.A = M.Af.p;
+            I.d = M.fv[idD].p;
+            I.v = M.fv[idV].p;
+        } else {
+            I.A = M.jac.p;
+            I.d = idD == kVb ? (const void*)M.res.p : (const void*)M.dv[idD].p;
+            I.v = M.dv[idV].p;
+        }
+    }
+    return v;
+}
+
 // Batched BiCGStabSolver::apply with CombinedCriterion per member.
 template <class S>
 void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& tol, double absTol, int maxIter,
@@ -298,15 +333,20 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
     setFlags(actInner_, act);
     ops.copy<S>(a, kVv, -1);
     ops.copy<S>(a, kVp, -1);
-    const auto pInsts = iluInsts(a, fl, kVp, kVy);
-    const auto rInsts = iluInsts(a, fl, kVr, kVz);
+    const auto pInsts = colored() ? std::vector<IluInst>() : iluInsts(a, fl, kVp, kVy);
+    const auto rInsts = colored() ? std::vector<IluInst>() : iluInsts(a, fl, kVr, kVz);
+    const auto pCol = colored() ? colInsts(a, fl, kVp, kVy) : std::vector<ColInst>();
+    const auto rCol = colored() ? colInsts(a, fl, kVr, kVz) : std::vector<ColInst>();
     int it = 0;
     for (; it < maxIter; ++it) {
         const int tag = 2 * it;
         ops.dot<S>(a, kVb, kVr, 0, tag);
         ops.updateP<S>(a);
-        if (prm_.precond == 0) ilu_->apply<S>(pInsts, stream_);
-        else throw std::invalid_argument("the ensemble engine uses the block-ILU0 preconditioner");
+        if (prm_.precond != 0) throw std::invalid_argument("the ensemble engine uses the block-ILU0 preconditioner");
+        timer.begin(kClsIluApply, stream_);
+        if (colored()) col_->apply<S>(pCol, 0, stream_);
+        else ilu_->apply<S>(pInsts, stream_);
+        timer.end(stream_);
         ops.spmv<S>(a, kVy, kVv);
         ops.dot<S>(a, kVb, kVv, 1, tag);
         ops.updateXR<S>(a);
@@ -331,7 +371,10 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
         }
         if (changed) setFlags(actInner_, act);
         if (!any) return;
-        ilu_->apply<S>(rInsts, stream_);
+        timer.begin(kClsIluApply, stream_);
+        if (colored()) col_->apply<S>(rCol, 1, stream_);
+        else ilu_->apply<S>(rInsts, stream_);
+        timer.end(stream_);
         ops.spmv<S>(a, kVz, kVy);
         ops.dot2<S>(a, tag + 1);
         ops.updateXR2<S>(a, tag + 1);
@@ -345,21 +388,45 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
 // prepare + solve for the members in `set`; rc per member (0, 1, 2)
 void Ensemble::linearSolve(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its)
 {
+    linearSolveImpl(set, rc, its);
+    // A dependency wait of the tiled ILU that ran into its bound (in the
+    // factorization or a sweep) leaves results that cannot be trusted.  Its
+    // flag is one for all members, so every member of this solve fails it as
+    // a NumericalProblem (its step is halved and retried); the flag and the
+    // exchange slots are reset for the next solve.
+    if (!colored() && ilu_->takeError(stream_)) {
+        rc.assign(set.size(), 0);
+        for (size_t b = 0; b < set.size(); ++b) if (set[b]) rc[b] = 2;
+    }
+}
+
+void Ensemble::raiseIluError()
+{
+    if (colored()) throw std::invalid_argument("the two-colour ILU has no dependency waits");
+    ilu_->raiseError(stream_);
+}
+
+void Ensemble::linearSolveImpl(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its)
+{
     const int B = (int)members_.size();
     EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
     setFlags(actSolve_, set);
     const int* a = actSolve_.p;
     // preconditioner: FP64 ILU(0) of the FP64 Jacobian
     for (auto& m : members_) OM_HIP(hipMemsetAsync(m->counters.p, 0, 4 * sizeof(int), stream_));
-    ops.copyMatrix(a, 0);
     int* fail = fail_.p;
     OM_HIP(hipMemsetAsync(fail, 0, sizeof(int), stream_));
     timer.begin(kClsIluFactor, stream_);
-    ilu_->factor(iluInsts(a, false, kVr, kVz), fail, stream_);
+    if (colored()) {
+        col_->factor(colInsts(a, false, kVr, kVz), fail, stream_);
+    } else {
+        ops.copyMatrix(a, 0);
+        ilu_->factor(iluInsts(a, false, kVr, kVz), fail, stream_);
+    }
     timer.end(stream_);
     if (prm_.precision == 1) {
         ops.copyMatrix(a, 1);
-        ops.factorToFloat(a);
+        if (!colored()) ops.factorToFloat(a);
     }
     int f = 0;
     OM_HIP(hipMemcpyAsync(&f, fail, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -593,7 +660,7 @@ int Server::submit(const omodels_blackoil_problem* p, const double* pv, const ui
     r.nSteps = nSteps;
     r.dt0 = dt0;
     r.arrival = arrival;
-    r.bytes = Ensemble::memoryEstimate(p, prm_.precision);
+    r.bytes = Ensemble::memoryEstimate(p, prm_);
     if (r.bytes > budget_) throw std::invalid_argument("request does not fit the HBM budget");
     reqs_.push_back(std::move(r));
     return reqs_.back().id;
@@ -612,8 +679,10 @@ void Server::run()
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
-    std::vector<int> order(reqs_.size());
-    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    // the requests not run yet (a later call serves those submitted since)
+    std::vector<int> order;
+    for (size_t k = 0; k < reqs_.size(); ++k)
+        if (reqs_[k].admit < 0.0) order.push_back((int)k);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs_[a].arrival < reqs_[b].arrival; });
     size_t next = 0;
     double used = 0.0;

@@ -93,6 +93,12 @@ public:
     template <class S>
     bool apply(const std::vector<IluInst>& insts, hipStream_t s, const SpmvFuse* fuse = nullptr);
     void checkError(hipStream_t s);
+    // whether a dependency wait ran into its bound since the last call; the
+    // flag and the exchange slots are reset either way (no throw)
+    bool takeError(hipStream_t s);
+    // set the error flag from the host, as an exhausted wait does: the waits of
+    // the following launches give up at once (tests of the recovery)
+    void raiseError(hipStream_t s);
     // make the device descriptor of these instances resident (before a stream
     // capture, which must not record the upload); generation() changes
     // whenever resident descriptors are dropped

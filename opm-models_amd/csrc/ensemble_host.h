@@ -42,13 +42,16 @@ public:
     Ensemble& operator=(const Ensemble&) = delete;
 
     // device bytes one member of this problem's size needs (admission control)
-    static size_t memoryEstimate(const omodels_blackoil_problem* p, int precision);
+    static size_t memoryEstimate(const omodels_blackoil_problem* p, const omodels_params& prm);
     int add(const omodels_blackoil_problem* p, const double* pv, const uint8_t* mean, int nSteps, double dt0);
     void remove(int id);
     Member* find(int id);
     void download(int id, double* pv, uint8_t* mean);
     // one lockstep time step of every member with steps left
     void timeStep();
+    // set the tiled ILU's error flag as an exhausted dependency wait does
+    // (tests of the recovery; not with the two-colour ILU)
+    void raiseIluError();
     bool allDone() const;
     int size() const { return (int)members_.size(); }
     KernelTimer timer;
@@ -58,7 +61,10 @@ private:
     void refreshStorefac();
     void setFlags(DevBuf<int>& dst, const std::vector<int>& flags);
     std::vector<IluInst> iluInsts(const int* act, bool floatFactors, int idD, int idV) const;
+    std::vector<ColInst> colInsts(const int* act, bool floatMatrix, int idD, int idV) const;
+    bool colored() const { return prm_.precond == 0 && prm_.ilu_order == 1; }
     void linearSolve(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its);
+    void linearSolveImpl(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its);
     template <class S>
     void bicgstab(const std::vector<int>& set, const std::vector<double>& tol, double absTol, int maxIter,
                   std::vector<int>& rc, std::vector<int>& its);
@@ -72,7 +78,8 @@ private:
     std::vector<std::unique_ptr<Member>> members_;
     int nextId_ = 0;
     bool dirty_ = true;
-    std::unique_ptr<TiledIluSet> ilu_;
+    std::unique_ptr<TiledIluSet> ilu_;        // natural order (ilu_order 0)
+    std::unique_ptr<ColoredIluSet> col_;      // two-colour tiles (ilu_order 1)
     DevBuf<EnsInst> desc_;
     std::vector<EnsInst> hostDesc_;
     DevBuf<int2> asmTable_, chunkTable_, srcTable_;

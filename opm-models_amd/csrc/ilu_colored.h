@@ -16,6 +16,41 @@ namespace omodels {
 // red-black cells.
 constexpr int kColTX = 4, kColTY = 2, kColTZ = 8;
 
+struct ColGeom {
+    int nx, ny, nz, ntx, nty;     // grid and tiles along x and y
+};
+ColGeom colGeomOf(int nx, int ny, int nz);
+
+// one reservoir of a ColoredIluSet
+struct ColInst {
+    ColGeom g;
+    const int* rp;
+    const void* A;        // the matrix (FP64 for the factorization; S for the sweeps)
+    double* dinv;         // pivots
+    const void* d;        // S*, right-hand side of an application
+    void* v;              // S*, its result
+    const int* active;    // nullptr, or a device flag: the member is skipped while it is 0
+};
+
+// The same preconditioner for many reservoirs of different sizes in the same
+// launches (work items = (member, tile) per colour).
+class ColoredIluSet {
+public:
+    void setInstances(const std::vector<ColGeom>& geoms, hipStream_t s);
+    void factor(const std::vector<ColInst>& insts, int* fail, hipStream_t s);
+    // slot 0 or 1: two applications with different vectors keep their own descriptors
+    template <class S>
+    void apply(const std::vector<ColInst>& insts, int slot, hipStream_t s);
+
+private:
+    const ColInst* upload(const std::vector<ColInst>& insts, int slot, hipStream_t s);
+    int nInst_ = 0;
+    int nWork_[2] = {0, 0};
+    DevBuf<int2> work_[2];
+    DevBuf<ColInst> desc_[3];
+    std::vector<ColInst> host_[3];
+};
+
 class ColoredIlu {
 public:
     ColoredIlu(int nx, int ny, int nz, hipStream_t s);

@@ -24,15 +24,14 @@
 // a2 x2), subtracted from the accumulator.
 #include "ilu_colored.h"
 
+#include <cstring>
+#include <stdexcept>
+
 namespace omodels {
 
 namespace {
 
 constexpr int kWavesPerBlock = 4;
-
-struct ColGeom {
-    int nx, ny, nz, ntx, nty;
-};
 
 // cell of this lane in tile t; false when outside the grid (ragged edge tiles)
 __device__ __forceinline__ bool laneCell(const ColGeom& g, int t, int lane, int& gx, int& gy, int& gz, int& lx,
@@ -157,15 +156,14 @@ __device__ __forceinline__ bool inTileUpper(int q, int lx, int ly, int lz)
 constexpr int kLevels = (kColTX - 1) + (kColTY - 1) + (kColTZ - 1) + 1;
 
 // Factorization of the tiles of one colour: pivots D_I^-1 (FP64).
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_factor(ColGeom g, int color,
-    const int* __restrict__ tiles, int nTiles, const int* __restrict__ rp, const double* __restrict__ A,
-    double* __restrict__ dinv, int* __restrict__ fail)
+// (one wave: every lane of the wave calls it for the same tile)
+__device__ __forceinline__ void colFactorTile(const ColGeom& g, int color, int tile, const int* __restrict__ rp,
+                                              const double* __restrict__ A, double* __restrict__ dinv,
+                                              int* __restrict__ fail)
 {
-    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
-    if (w >= nTiles) return;                     // wave-uniform
     const int lane = (int)(threadIdx.x & 63u);
     int gx, gy, gz, lx, ly, lz;
-    const bool valid = laneCell(g, tiles[w], lane, gx, gy, gz, lx, ly, lz);
+    const bool valid = laneCell(g, tile, lane, gx, gy, gz, lx, ly, lz);
     const long I = ((long)gz * g.ny + gy) * g.nx + gx;
     double D[9];
     RowPos pos{};
@@ -234,16 +232,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_factor(ColGeom g, i
 
 // Forward sweep of one colour: w_I = D_I^-1 (d_I - sum_{K lower} A_IK w_K), w into v.
 template <class S>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_forward(ColGeom g, int color,
-    const int* __restrict__ tiles, int nTiles, const int* __restrict__ rp, const S* __restrict__ A,
-    const double* __restrict__ dinv, const S* __restrict__ d, S* __restrict__ v, const int* __restrict__ run)
+__device__ __forceinline__ void colForwardTile(const ColGeom& g, int color, int tile, const int* __restrict__ rp,
+                                               const S* __restrict__ A, const double* __restrict__ dinv,
+                                               const S* __restrict__ d, S* __restrict__ v)
 {
-    if (run && !*run) return;
-    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
-    if (w >= nTiles) return;
     const int lane = (int)(threadIdx.x & 63u);
     int gx, gy, gz, lx, ly, lz;
-    const bool valid = laneCell(g, tiles[w], lane, gx, gy, gz, lx, ly, lz);
+    const bool valid = laneCell(g, tile, lane, gx, gy, gz, lx, ly, lz);
     const long I = ((long)gz * g.ny + gy) * g.nx + gx;
     S acc[3] = {S(0), S(0), S(0)};
     S blk[3][9];
@@ -302,16 +297,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_forward(ColGeom g, 
 
 // Backward sweep of one colour, in place: x_I = w_I - D_I^-1 sum_{J upper} A_IJ x_J.
 template <class S>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_backward(ColGeom g, int color,
-    const int* __restrict__ tiles, int nTiles, const int* __restrict__ rp, const S* __restrict__ A,
-    const double* __restrict__ dinv, S* __restrict__ v, const int* __restrict__ run)
+__device__ __forceinline__ void colBackwardTile(const ColGeom& g, int color, int tile, const int* __restrict__ rp,
+                                                const S* __restrict__ A, const double* __restrict__ dinv,
+                                                S* __restrict__ v)
 {
-    if (run && !*run) return;
-    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
-    if (w >= nTiles) return;
     const int lane = (int)(threadIdx.x & 63u);
     int gx, gy, gz, lx, ly, lz;
-    const bool valid = laneCell(g, tiles[w], lane, gx, gy, gz, lx, ly, lz);
+    const bool valid = laneCell(g, tile, lane, gx, gy, gz, lx, ly, lz);
     const long I = ((long)gz * g.ny + gy) * g.nx + gx;
     S acc[3] = {S(0), S(0), S(0)};
     S wOwn[3] = {S(0), S(0), S(0)};
@@ -372,6 +364,71 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_backward(ColGeom g,
     }
 }
 
+// one reservoir: the tiles of one colour, one wave each
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_factor(ColGeom g, int color,
+    const int* __restrict__ tiles, int nTiles, const int* __restrict__ rp, const double* __restrict__ A,
+    double* __restrict__ dinv, int* __restrict__ fail)
+{
+    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    if (w >= nTiles) return;                     // wave-uniform
+    colFactorTile(g, color, tiles[w], rp, A, dinv, fail);
+}
+template <class S>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_forward(ColGeom g, int color,
+    const int* __restrict__ tiles, int nTiles, const int* __restrict__ rp, const S* __restrict__ A,
+    const double* __restrict__ dinv, const S* __restrict__ d, S* __restrict__ v, const int* __restrict__ run)
+{
+    if (run && !*run) return;
+    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    if (w >= nTiles) return;
+    colForwardTile<S>(g, color, tiles[w], rp, A, dinv, d, v);
+}
+template <class S>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_backward(ColGeom g, int color,
+    const int* __restrict__ tiles, int nTiles, const int* __restrict__ rp, const S* __restrict__ A,
+    const double* __restrict__ dinv, S* __restrict__ v, const int* __restrict__ run)
+{
+    if (run && !*run) return;
+    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    if (w >= nTiles) return;
+    colBackwardTile<S>(g, color, tiles[w], rp, A, dinv, v);
+}
+
+// several reservoirs: work items (member, tile) of one colour, one wave each;
+// members whose flag is 0 are skipped
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_factor_set(const ColInst* __restrict__ insts,
+    const int2* __restrict__ work, int nWork, int color, int* __restrict__ fail)
+{
+    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    if (w >= nWork) return;
+    const int2 it = work[w];
+    const ColInst& I = insts[it.x];
+    if (I.active && !*I.active) return;
+    colFactorTile(I.g, color, it.y, I.rp, (const double*)I.A, I.dinv, fail);
+}
+template <class S>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_forward_set(const ColInst* __restrict__ insts,
+    const int2* __restrict__ work, int nWork, int color)
+{
+    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    if (w >= nWork) return;
+    const int2 it = work[w];
+    const ColInst& I = insts[it.x];
+    if (I.active && !*I.active) return;
+    colForwardTile<S>(I.g, color, it.y, I.rp, (const S*)I.A, I.dinv, (const S*)I.d, (S*)I.v);
+}
+template <class S>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_backward_set(const ColInst* __restrict__ insts,
+    const int2* __restrict__ work, int nWork, int color)
+{
+    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
+    if (w >= nWork) return;
+    const int2 it = work[w];
+    const ColInst& I = insts[it.x];
+    if (I.active && !*I.active) return;
+    colBackwardTile<S>(I.g, color, it.y, I.rp, (const S*)I.A, I.dinv, (S*)I.v);
+}
+
 template <class S>
 __global__ void k_col_scale(long m, S* __restrict__ v, double w, const int* __restrict__ run)
 {
@@ -380,7 +437,9 @@ __global__ void k_col_scale(long m, S* __restrict__ v, double w, const int* __re
     v[i] = (S)(w * (double)v[i]);
 }
 
-ColGeom geomOf(int nx, int ny, int nz)
+}  // namespace
+
+ColGeom colGeomOf(int nx, int ny, int nz)
 {
     ColGeom g;
     g.nx = nx; g.ny = ny; g.nz = nz;
@@ -389,18 +448,21 @@ ColGeom geomOf(int nx, int ny, int nz)
     return g;
 }
 
-}  // namespace
-
-ColoredIlu::ColoredIlu(int nx, int ny, int nz, hipStream_t s) : nx_(nx), ny_(ny), nz_(nz)
+// tiles of each colour of a grid
+static void colourLists(const ColGeom& g, std::vector<int> (&lists)[2])
 {
-    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("colored ILU: empty grid");
-    const ColGeom g = geomOf(nx, ny, nz);
-    const int ntz = (nz + kColTZ - 1) / kColTZ;
-    std::vector<int> lists[2];
+    const int ntz = (g.nz + kColTZ - 1) / kColTZ;
     for (int tz = 0; tz < ntz; ++tz)
         for (int ty = 0; ty < g.nty; ++ty)
             for (int tx = 0; tx < g.ntx; ++tx)
                 lists[(tx + ty + tz) & 1].push_back(tx + g.ntx * (ty + g.nty * tz));
+}
+
+ColoredIlu::ColoredIlu(int nx, int ny, int nz, hipStream_t s) : nx_(nx), ny_(ny), nz_(nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("colored ILU: empty grid");
+    std::vector<int> lists[2];
+    colourLists(colGeomOf(nx, ny, nz), lists);
     for (int c = 0; c < 2; ++c) {
         nTiles_[c] = (int)lists[c].size();
         if (nTiles_[c]) tiles_[c].upload(lists[c].data(), lists[c].size(), s);
@@ -409,7 +471,7 @@ ColoredIlu::ColoredIlu(int nx, int ny, int nz, hipStream_t s) : nx_(nx), ny_(ny)
 
 void ColoredIlu::factor(const int* rp, const double* A, double* dinv, int* fail, hipStream_t s)
 {
-    const ColGeom g = geomOf(nx_, ny_, nz_);
+    const ColGeom g = colGeomOf(nx_, ny_, nz_);
     for (int c = 0; c < 2; ++c) {
         if (!nTiles_[c]) continue;
         const int blocks = (nTiles_[c] + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -423,7 +485,7 @@ template <class S>
 void ColoredIlu::apply(const int* rp, const S* A, const double* dinv, const S* d, S* v, double relax,
                        hipStream_t s, const int* run)
 {
-    const ColGeom g = geomOf(nx_, ny_, nz_);
+    const ColGeom g = colGeomOf(nx_, ny_, nz_);
     for (int c = 0; c < 2; ++c) {
         if (!nTiles_[c]) continue;
         const int blocks = (nTiles_[c] + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -447,5 +509,71 @@ template void ColoredIlu::apply<double>(const int*, const double*, const double*
                                         hipStream_t, const int*);
 template void ColoredIlu::apply<float>(const int*, const float*, const double*, const float*, float*, double,
                                        hipStream_t, const int*);
+
+}  // namespace omodels
+
+namespace omodels {
+
+// ---------------------------------------------------------------------------
+void ColoredIluSet::setInstances(const std::vector<ColGeom>& geoms, hipStream_t s)
+{
+    std::vector<int2> work[2];
+    for (size_t b = 0; b < geoms.size(); ++b) {
+        std::vector<int> lists[2];
+        colourLists(geoms[b], lists);
+        for (int c = 0; c < 2; ++c)
+            for (int t : lists[c]) work[c].push_back(make_int2((int)b, t));
+    }
+    for (int c = 0; c < 2; ++c) {
+        nWork_[c] = (int)work[c].size();
+        if (nWork_[c]) work_[c].upload(work[c].data(), work[c].size(), s);
+    }
+    nInst_ = (int)geoms.size();
+}
+
+const ColInst* ColoredIluSet::upload(const std::vector<ColInst>& insts, int slot, hipStream_t s)
+{
+    if ((int)insts.size() != nInst_) throw std::logic_error("colored ILU set: instance count changed");
+    DevBuf<ColInst>& d = desc_[slot];
+    if (host_[slot].size() != insts.size() ||
+        std::memcmp(host_[slot].data(), insts.data(), insts.size() * sizeof(ColInst)) != 0) {
+        host_[slot] = insts;
+        d.upload(insts.data(), insts.size(), s);
+    }
+    return d.p;
+}
+
+void ColoredIluSet::factor(const std::vector<ColInst>& insts, int* fail, hipStream_t s)
+{
+    const ColInst* di = upload(insts, 0, s);
+    for (int c = 0; c < 2; ++c) {
+        if (!nWork_[c]) continue;
+        const int blocks = (nWork_[c] + kWavesPerBlock - 1) / kWavesPerBlock;
+        hipLaunchKernelGGL(k_col_factor_set, dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, di, work_[c].p,
+                           nWork_[c], c, fail);
+    }
+    OM_HIP(hipGetLastError());
+}
+
+template <class S>
+void ColoredIluSet::apply(const std::vector<ColInst>& insts, int slot, hipStream_t s)
+{
+    const ColInst* di = upload(insts, 1 + slot, s);
+    for (int c = 0; c < 2; ++c) {
+        if (!nWork_[c]) continue;
+        const int blocks = (nWork_[c] + kWavesPerBlock - 1) / kWavesPerBlock;
+        hipLaunchKernelGGL((k_col_forward_set<S>), dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, di, work_[c].p,
+                           nWork_[c], c);
+    }
+    for (int c = 1; c >= 0; --c) {
+        if (!nWork_[c]) continue;
+        const int blocks = (nWork_[c] + kWavesPerBlock - 1) / kWavesPerBlock;
+        hipLaunchKernelGGL((k_col_backward_set<S>), dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, di, work_[c].p,
+                           nWork_[c], c);
+    }
+    OM_HIP(hipGetLastError());
+}
+template void ColoredIluSet::apply<double>(const std::vector<ColInst>&, int, hipStream_t);
+template void ColoredIluSet::apply<float>(const std::vector<ColInst>&, int, hipStream_t);
 
 }  // namespace omodels

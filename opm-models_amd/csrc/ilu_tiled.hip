@@ -1274,6 +1274,26 @@ const IluInst* TiledIluSet::upload(const std::vector<IluInst>& insts, hipStream_
     return dcache_.back().second->p;
 }
 
+bool TiledIluSet::takeError(hipStream_t s)
+{
+    int e = 0;
+    OM_HIP(hipMemcpyAsync(&e, err_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    OM_HIP(hipStreamSynchronize(s));
+    if (!e) return false;
+    // a launch that ran out of its waits leaves slots filled: empty them and the flag
+    resetExchange(s);
+    OM_HIP(hipMemsetAsync(err_.p, 0, sizeof(int), s));
+    OM_HIP(hipStreamSynchronize(s));
+    return true;
+}
+
+void TiledIluSet::raiseError(hipStream_t s)
+{
+    const int one = 1;
+    OM_HIP(hipMemcpyAsync(err_.p, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    OM_HIP(hipStreamSynchronize(s));
+}
+
 void TiledIluSet::checkError(hipStream_t s)
 {
     int e = 0;

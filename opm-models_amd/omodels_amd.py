@@ -169,6 +169,7 @@ SIGNATURES = {
     "omodels_server_summary": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int32)]),
     "omodels_server_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "omodels_server_inject_ilu_wait_error": (C.c_int, [C.c_void_p]),
     "omodels_server_kernel_times": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64), C.c_int]),
     "omodels_model_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "omodels_model_kernel_times": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64), C.c_int]),

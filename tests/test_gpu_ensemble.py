@@ -31,27 +31,35 @@ def _serve(members, params, arrivals=None, budget=64e9, max_batch=256, n_steps=3
     return srv, [srv.result(i) for i in ids]
 
 
-def test_ensemble_members_match_the_oracle():
+@pytest.mark.parametrize("ilu_order", [0, 1])
+def test_ensemble_members_match_the_oracle(ilu_order):
+    """At the reference's settings (NewtonTolerance 1e-6, adaptive steps from
+    one day) every member follows the oracle run of that member alone: step
+    sizes, iteration counts and final state bit for bit (scalar products are
+    exact sums), with the natural-order and the two-colour ILU(0)."""
     ms = _members()
-    srv, res = _serve(ms, om.default_params(newton_tol=1e-10, **FIXED))
+    srv, res = _serve(ms, om.default_params(ilu_order=ilu_order))
     for c, r in zip(ms, res):
         assert r["status"] == 0
         rco, pvo, mo, histo, dtso, logo, tim = orc.simulate(orc.problem_from_case(c), c.pv0, c.mean0, 3, 86400.0,
-                                                            orc.params(newton_tol=1e-10, **FIXED), nthreads=4)
+                                                            orc.params(ilu_order=ilu_order), nthreads=4)
         assert rco == 0
         p, po = r["pv"][1::3], pvo[1::3]
         assert np.max(np.abs(p - po)) <= 1e-8 * np.max(np.abs(po))
         assert np.max(np.abs(r["pv"][0::3] - pvo[0::3])) <= 1e-8
         assert np.array_equal(r["mean"], mo)
+        assert np.array_equal(r["dts"], dtso)
+        assert np.array_equal(r["pv"].view(np.uint64), pvo.view(np.uint64))
 
 
-@pytest.mark.parametrize("schedule", [0, 2, 3])
+@pytest.mark.parametrize("schedule,ilu_order", [(0, 0), (2, 0), (3, 0), (0, 1)])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule):
+def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule, ilu_order):
     """schedule 0 (batched sweeps for ensembles), 2 / 3 force the streaming /
-    batched sweeps: the arithmetic is the same in every case."""
+    batched sweeps, ilu_order 1 the two-colour ILU(0): the arithmetic is the
+    same in every case."""
     ms = _members()
-    prm = om.default_params(precision=precision, ilu_schedule=schedule)
+    prm = om.default_params(precision=precision, ilu_schedule=schedule, ilu_order=ilu_order)
     srv, mixed = _serve(ms, prm)
     for k, c in enumerate(ms):
         srv1, alone = _serve([c], prm)
@@ -61,10 +69,11 @@ def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule):
         assert np.array_equal(alone[0]["dts"], mixed[k]["dts"])
 
 
-def test_mixed_precision_ensemble_matches_fp64_oracle():
+@pytest.mark.parametrize("ilu_order", [0, 1])
+def test_mixed_precision_ensemble_matches_fp64_oracle(ilu_order):
     ms = _members()
     kw = dict(newton_tol=1e-7, **FIXED)
-    srv, res = _serve(ms, om.default_params(precision=1, **kw))
+    srv, res = _serve(ms, om.default_params(precision=1, ilu_order=ilu_order, **kw))
     for c, r in zip(ms, res):
         rco, pvo, mo, histo, dtso, logo, tim = orc.simulate(orc.problem_from_case(c), c.pv0, c.mean0, 3, 86400.0,
                                                             orc.params(**kw), nthreads=4)
@@ -106,3 +115,50 @@ def om_bytes(c):
     # same estimate as the library's admission control for FP64
     nnzb = 7 * N
     return nnzb * 9 * 8 * 2 + N * 9 * 8 + N * 3 * 8 * (8 + 5) + N * 3 * 4 * 8 + N * (4 * 8 + 6 * 4 + 3)
+
+
+def test_exhausted_ilu_wait_fails_the_solve_and_the_next_runs_clean():
+    """A dependency wait of the tiled ILU that ran into its bound (forced here
+    by setting the flag from the host, so every wait gives up and every kernel
+    ends normally) fails the batched solve of every member as a
+    NumericalProblem: each member halves its step and goes on.  The next
+    request on the same server runs clean: identical to a fresh server."""
+    ms = _members()[:2]
+    prm = om.default_params()
+    srv = om.Server(prm, 64e9, 64)
+    ids = [srv.submit(c, 2, 86400.0, 0.0) for c in ms]
+    om.check(om.lib().omodels_server_inject_ilu_wait_error(srv.h), "inject")
+    srv.run()
+    res = [srv.result(i) for i in ids]
+    for r in res:
+        assert r["status"] == 0
+        assert r["stats"].failed_attempts >= 1
+        assert r["dts"][0] == 43200.0
+    late = srv.submit(ms[0], 2, 86400.0, 0.0)
+    srv.run()
+    again = srv.result(late)
+    srv1, fresh = _serve([ms[0]], prm, n_steps=2)
+    assert again["status"] == 0 and again["stats"].failed_attempts == 0
+    assert np.array_equal(again["pv"].view(np.uint64), fresh[0]["pv"].view(np.uint64))
+
+
+def test_serving_members_of_8k_to_60k_cells():
+    """Members of the serving workload's sizes (8k to 60k cells, uneven
+    shapes, their own permeability fields and wells), FP32-inner with the
+    two-colour ILU(0), two one-day steps: each result is bit-identical alone
+    and in the mix, and within 1e-6 relative of the FP64 oracle in pressure."""
+    specs = [(20, 20, 20, 41, 0.5), (40, 25, 20, 42, 1.2), (50, 40, 20, 43, 2.0), (60, 50, 20, 44, 1.0)]
+    ms = [case(nx, ny, nz, seed=sd, wells=2, perm_sigma=sg) for nx, ny, nz, sd, sg in specs]
+    assert min(c.N for c in ms) >= 8000 and max(c.N for c in ms) >= 60000
+    kw = dict(newton_tol=1e-7, **FIXED)
+    prm = om.default_params(precision=1, ilu_order=1, **kw)
+    srv, mixed = _serve(ms, prm, n_steps=2)
+    for k, c in enumerate(ms):
+        srv1, alone = _serve([c], prm, n_steps=2)
+        assert alone[0]["status"] == 0 and mixed[k]["status"] == 0
+        assert np.array_equal(alone[0]["pv"].view(np.uint64), mixed[k]["pv"].view(np.uint64))
+        rco, pvo, mo, histo, dtso, logo, tim = orc.simulate(orc.problem_from_case(c), c.pv0, c.mean0, 2, 86400.0,
+                                                            orc.params(**kw), nthreads=8)
+        assert rco == 0
+        p, po = mixed[k]["pv"][1::3], pvo[1::3]
+        assert np.max(np.abs(p - po)) <= 1e-6 * np.max(np.abs(po)), k
