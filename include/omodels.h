@@ -301,8 +301,8 @@ int omodels_scalar_product(int64_t m, const double* x, const double* y, double* 
  * x_dev, y_dev: device arrays of m < 2^28 doubles; out_dev: one device
  * double; stream: the caller's hipStream_t (NULL: the null stream).  The work
  * is only enqueued: the result is in out_dev once the stream has reached it.
- * The exact accumulator lives in a static device array of 552 bytes per GPU,
- * shared by all calls on that GPU: calls that overlap on one GPU race. */
+ * The partial sums live in a static device array of 1.3 KiB per GPU, shared
+ * by all calls on that GPU: calls that overlap on one GPU race. */
 int omodels_scalar_product_noalloc(int64_t m, const double* x_dev, const double* y_dev, double* out_dev,
                                    void* stream);
 /* Block ILU0 factor of the prepared matrix (diagonal blocks inverted), for

@@ -509,13 +509,13 @@ int omodels_scalar_product(int64_t m, const double* x, const double* y, double* 
         OM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } } g{st};
         DevBuf<double> dx, dy, dout;
-        DevBuf<ExactAcc> acc;
+        DevBuf<DotScratch> sc;
         dx.upload(x, (size_t)m, st);
         dy.upload(y, (size_t)m, st);
         dout.alloc(1);
-        acc.alloc(1);
-        OM_HIP(hipMemsetAsync(acc.p, 0, sizeof(ExactAcc), st));
-        launchScalarProduct((long)m, dx.p, dy.p, acc.p, dout.p, st);
+        sc.alloc(1);
+        OM_HIP(hipMemsetAsync(sc.p, 0, sizeof(DotScratch), st));
+        launchScalarProduct((long)m, dx.p, dy.p, sc.p, dout.p, st);
         OM_HIP(hipGetLastError());
         dout.download(out, 1, st);
         OM_HIP(hipStreamSynchronize(st));

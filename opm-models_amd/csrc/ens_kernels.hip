@@ -79,65 +79,73 @@ __global__ __launch_bounds__(kT) void k_e_copy(const EnsInst* __restrict__ insts
     }
 }
 
-// Scalar products: each chunk adds its products exactly into its member's
-// accumulator (exact_sum.h), so a member's value does not depend on the batch
-// or on the chunking and equals the single-reservoir solver's.
-// NACC = 1: acc[0] += a.b;  NACC = 2: acc[0] += a.a, acc[1] += a.b
+// Scalar products: the exact sum of the products (exact_sum.h), so a member's
+// value does not depend on the batch or on the launch shape and equals the
+// single-reservoir solver's.  A workgroup takes kEnsDotCells cells of one
+// member (item = member, index among the member's workgroups, their count),
+// adds its limbs to the member's device accumulators, and the member's last
+// workgroup to finish rounds and evaluates the member's scalars.
+// NACC = 1: a.b;  NACC = 2: a.a and a.b.  mode: 0 rho/beta, 1 denom/alpha,
+// 2 omega (bicgstabsolver.hh:197-236, :287-292).
 template <class S, int NACC>
-__global__ __launch_bounds__(kT) void k_e_xdot_partial(const EnsInst* __restrict__ insts,
-    const int2* __restrict__ table, const int* __restrict__ act, int ia, int ib, ExactAcc* __restrict__ accAll)
+__global__ __launch_bounds__(kT) void k_e_xdot(const EnsInst* __restrict__ insts, const int4* __restrict__ items,
+    const int* __restrict__ act, int ia, int ib, ExactAcc* __restrict__ accAll,
+    unsigned* __restrict__ tickets, KrylovScalars* __restrict__ ksAll, int mode, double eps, int tag)
 {
-    const int2 t = table[blockIdx.x];
-    if (!act[t.x]) return;
+    const int4 it = items[blockIdx.x];
+    if (!act[it.x]) return;
+    constexpr int nw = kT / 64;
     __shared__ unsigned long long sAcc[NACC * kAccLimbs];
-    __shared__ unsigned sFlags;
-    clearLimbs(sAcc, NACC, &sFlags);
+    __shared__ unsigned sMeta[5];
+    clearLimbs(sAcc, NACC, sMeta);
     __syncthreads();
-    const EnsInst& I = insts[t.x];
-    const long c = (long)t.y * kT + threadIdx.x;
+    const EnsInst& I = insts[it.x];
     unsigned fl0 = 0u, fl1 = 0u;
-    ProductPiece p0[3], p1[3];
     const S* a = vec<S>(I, ia);
     const S* b = vec<S>(I, ib);
+    constexpr int kPass = kEnsDotCells / kT;     // cells per lane
+    double v0[3 * kPass], v1[3 * kPass];
 #pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        double v0 = 0.0, v1 = 0.0;
-        if (c < I.N) {
-            const double ai = (double)a[3 * c + e];
-            if (NACC == 1) v0 = ai * (double)b[3 * c + e];
-            else { v0 = ai * ai; v1 = ai * (double)b[3 * c + e]; }
+    for (int pass = 0; pass < kPass; ++pass) {
+        const long c = (long)it.y * kEnsDotCells + (long)pass * kT + threadIdx.x;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            double x0 = 0.0, x1 = 0.0;
+            if (c < I.N) {
+                const double ai = (double)a[3 * c + e];
+                if (NACC == 1) x0 = ai * (double)b[3 * c + e];
+                else { x0 = ai * ai; x1 = ai * (double)b[3 * c + e]; }
+            }
+            v0[3 * pass + e] = x0;
+            v1[3 * pass + e] = x1;
         }
-        p0[e] = makePiece(v0, fl0);
-        if (NACC == 2) p1[e] = makePiece(v1, fl1);
     }
-    accumulatePieces<3>(p0, sAcc);
-    if (NACC == 2) accumulatePieces<3>(p1, sAcc + kAccLimbs);
-    if (fl0 | fl1) atomicOr(&sFlags, fl0 | (fl1 << 3));
+    accumulateBands<3 * kPass>(v0, sAcc, fl0);
+    if (NACC == 2) accumulateBands<3 * kPass>(v1, sAcc + kAccLimbs, fl1);
+    if (fl0 | fl1) atomicOr(&sMeta[0], fl0 | (fl1 << 3));
     __syncthreads();
-    flushLimbs(sAcc, NACC, &sFlags, accAll + 2 * t.x);
-}
-
-// op 0: rho_i and beta; op 1: denom and alpha (bicgstabsolver.hh:197-236).
-// One wave per member.
-__global__ __launch_bounds__(64) void k_e_xdot_final(const int* __restrict__ act, ExactAcc* accAll,
-                                                     KrylovScalars* ksAll, int op, double eps, int tag)
-{
-    const int b = blockIdx.x;
-    if (!act[b]) return;
+    ExactAcc* acc = accAll + 2 * it.x;
+    flushLimbs(sAcc, NACC, sMeta, acc);
+    if (!lastToFinish(tickets + it.x, it.z, sMeta)) return;
     __shared__ long long sD[kAccLimbs];
-    const double s = finishExact(accAll + 2 * b, sD);
+    const double s0 = finishExact(&acc[0], sD, sMeta);
+    const double s1 = NACC == 2 ? finishExact(&acc[1], sD, sMeta) : 0.0;
     if (threadIdx.x != 0) return;
-    KrylovScalars* ks = ksAll + b;
-    if (op == 0) {
-        ks->rho_i = s;
+    KrylovScalars* ks = ksAll + it.x;
+    if (mode == 0) {
+        ks->rho_i = s0;
         if (fabs(ks->rho) <= eps || fabs(ks->omega) <= eps) markFail(ks, tag);
-        ks->beta = (s / ks->rho) * (ks->alpha / ks->omega);
-        ks->rho = s;
-    } else {
-        ks->denom = s;
-        if (fabs(s) <= eps) markFail(ks, tag);
-        ks->alpha = ks->rho / s;
+        ks->beta = (s0 / ks->rho) * (ks->alpha / ks->omega);
+        ks->rho = s0;
+    } else if (mode == 1) {
+        ks->denom = s0;
+        if (fabs(s0) <= eps) markFail(ks, tag);
+        ks->alpha = ks->rho / s0;
         if (fabs(ks->alpha) <= eps) markFail(ks, tag);
+    } else {
+        if (fabs(s0) <= eps) markFail(ks, tag);
+        ks->omega = s1 / s0;
+        if (fabs(ks->omega) <= eps) markFail(ks, tag);
     }
 }
 
@@ -215,22 +223,6 @@ __global__ __launch_bounds__(kT) void k_e_crit_final(const int* __restrict__ fir
     if (threadIdx.x != 0) return;
     ksAll[b].resnorm = mx;
     ksAll[b].stag_half = nz == 0.0 ? 1 : 0;
-}
-
-// <t,t>, <t,s> with t = y, s = r: k_e_xdot_partial<S, 2>(.., kVy, kVr, ..)
-__global__ __launch_bounds__(64) void k_e_xdot2_final(const int* __restrict__ act, ExactAcc* accAll,
-                                                      KrylovScalars* ksAll, double eps, int tag)
-{
-    const int b = blockIdx.x;
-    if (!act[b]) return;
-    __shared__ long long sD[kAccLimbs];
-    const double tt = finishExact(accAll + 2 * b, sD);
-    const double ts = finishExact(accAll + 2 * b + 1, sD);
-    if (threadIdx.x != 0) return;
-    KrylovScalars* ks = ksAll + b;
-    if (fabs(tt) <= eps) markFail(ks, tag);
-    ks->omega = ts / tt;
-    if (fabs(ks->omega) <= eps) markFail(ks, tag);
 }
 
 // x += omega z; r += -omega y; partial of (z != 0)
@@ -414,14 +406,14 @@ __global__ __launch_bounds__(kT) void k_e_residual64(const EnsInst* __restrict__
 
 // ---------------------------------------------------------------------------
 EnsOps::EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* first, int nInst, double* part,
-               ExactAcc* acc, KrylovScalars* ks, hipStream_t s)
-    : insts_(insts), table_(table), nChunks_(nChunks), first_(first), nInst_(nInst), part_(part), acc_(acc), ks_(ks),
+               const EnsDots& dots, KrylovScalars* ks, hipStream_t s)
+    : insts_(insts), table_(table), nChunks_(nChunks), first_(first), nInst_(nInst), part_(part), dots_(dots), ks_(ks),
       s_(s)
 {}
 
 #define EGRID dim3(nChunks_), dim3(kT), 0, s_
 #define IGRID dim3(nInst_), dim3(kT), 0, s_
-#define WGRID dim3(nInst_), dim3(64), 0, s_
+#define DGRID dim3(dots_.nItems), dim3(kT), 0, s_
 
 template <class S> void EnsOps::copy(const int* act, int dst, int src)
 { if (nChunks_) hipLaunchKernelGGL((k_e_copy<S>), EGRID, insts_, table_, act, dst, src); }
@@ -429,9 +421,9 @@ template <class S> void EnsOps::copy(const int* act, int dst, int src)
 template <class S> void EnsOps::dot(const int* act, int ia, int ib, int op, int tag)
 {
     if (!nChunks_) return;
-    hipLaunchKernelGGL((k_e_xdot_partial<S, 1>), EGRID, insts_, table_, act, ia, ib, acc_);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_e_xdot_final, WGRID, act, acc_, ks_, op, eps, tag);
+    hipLaunchKernelGGL((k_e_xdot<S, 1>), DGRID, insts_, dots_.items, act, ia, ib, dots_.acc, dots_.tickets, ks_, op,
+                       eps, tag);
 }
 
 template <class S> void EnsOps::updateP(const int* act)
@@ -447,9 +439,9 @@ template <class S> void EnsOps::updateXR(const int* act)
 template <class S> void EnsOps::dot2(const int* act, int tag)
 {
     if (!nChunks_) return;
-    hipLaunchKernelGGL((k_e_xdot_partial<S, 2>), EGRID, insts_, table_, act, kVy, kVr, acc_);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_e_xdot2_final, WGRID, act, acc_, ks_, eps, tag);
+    hipLaunchKernelGGL((k_e_xdot<S, 2>), DGRID, insts_, dots_.items, act, kVy, kVr, dots_.acc, dots_.tickets, ks_, 2,
+                       eps, tag);
 }
 
 template <class S> void EnsOps::updateXR2(const int* act, int tag)

@@ -15,13 +15,22 @@ void launchEnsUpdate(const BoFluid& f, const EnsInst* insts, const int2* table, 
                      const NewtonUpdateParams& prm, hipStream_t s);
 void assemblyGrid(const BoGrid& G, int& gx, int& gy, int& gz, int& kChunk, int targetBlocks);
 
+// Work of the exact scalar products: items (member, workgroup index within the
+// member, the member's workgroup count, unused), two device accumulators and
+// a ticket per member (zero between products).
+struct EnsDots {
+    const int4* items;
+    int nItems;
+    ExactAcc* acc;
+    unsigned* tickets;
+};
+
 // BiCGStab steps over all active instances (the same sequence as the
 // single-reservoir solver, one launch per step for the whole ensemble)
 class EnsOps {
 public:
-    // acc: two exact accumulators per member (zero; every product leaves them zero)
     EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* first, int nInst, double* part,
-           ExactAcc* acc, KrylovScalars* ks, hipStream_t s);
+           const EnsDots& dots, KrylovScalars* ks, hipStream_t s);
     template <class S> void copy(const int* act, int dst, int src);   // src < 0: zero
     template <class S> void dot(const int* act, int ia, int ib, int op, int tag);
     template <class S> void updateP(const int* act);
@@ -43,7 +52,7 @@ private:
     const int* first_;
     int nInst_;
     double* part_;
-    ExactAcc* acc_;
+    EnsDots dots_;
     KrylovScalars* ks_;
     hipStream_t s_;
 };

@@ -62,8 +62,10 @@ Ensemble::Ensemble(const omodels_params& prm, hipStream_t s) : prm_(prm)
     OM_HIP(hipHostMalloc((void**)&hostD_, sizeof(double) * kMaxMembers, hipHostMallocDefault));
     OM_HIP(hipHostMalloc((void**)&hostI_, sizeof(int) * 2 * kMaxMembers, hipHostMallocDefault));
     ks_.alloc(kMaxMembers);
-    acc_.alloc(2 * kMaxMembers);
-    OM_HIP(hipMemsetAsync(acc_.p, 0, 2 * kMaxMembers * sizeof(ExactAcc), stream_));
+    tickets_.alloc(kMaxMembers);
+    OM_HIP(hipMemsetAsync(tickets_.p, 0, kMaxMembers * sizeof(unsigned int), stream_));
+    dotAcc_.alloc(2 * kMaxMembers);
+    OM_HIP(hipMemsetAsync(dotAcc_.p, 0, 2 * kMaxMembers * sizeof(ExactAcc), stream_));
     maxOut_.alloc(kMaxMembers);
     for (auto* a : {&actLin_, &actRefresh_, &actSolve_, &actInner_}) a->alloc(kMaxMembers);
     fail_.alloc(1);
@@ -213,6 +215,14 @@ void Ensemble::rebuild()
     nChunks_ = (int)chunkT.size();
     nSrc_ = (int)srcT.size();
     partial_.alloc(std::max<size_t>(2 * (size_t)nChunks_, 1));
+    std::vector<int4> items;
+    for (int b = 0; b < B; ++b) {
+        const int nb = (int)((members_[b]->N + kEnsDotCells - 1) / kEnsDotCells);
+        for (int k = 0; k < nb; ++k) items.push_back(make_int4(b, k, nb, 0));
+    }
+    nDotItems_ = (int)items.size();
+    dotItems_.upload(items.data(), items.size(), stream_);
+
     if (colored()) {
         std::vector<ColGeom> cg(B);
         for (int b = 0; b < B; ++b) cg[b] = colGeomOf(members_[b]->G.nx, members_[b]->G.ny, members_[b]->G.nz);
@@ -298,7 +308,7 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
                         std::vector<int>& rc, std::vector<int>& its)
 {
     const int B = (int)members_.size();
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, dots(), ks_.p, stream_);
     const bool fl = sizeof(S) == 4;
     std::vector<int> act(set);
     rc.assign(B, 0);
@@ -327,9 +337,11 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
         k.fail_tag = -1; k.stag_full_tag = -1;
     }
     OM_HIP(hipMemcpyAsync(ks_.p, hostKs_, B * sizeof(KrylovScalars), hipMemcpyHostToDevice, stream_));
-    // every scalar product clears its accumulators again; this clears what a
-    // solve interrupted by an error may have left behind
-    OM_HIP(hipMemsetAsync(acc_.p, 0, 2 * (size_t)B * sizeof(ExactAcc), stream_));
+    // the last workgroup of every scalar product clears its member's ticket
+    // and accumulators; this clears what a solve interrupted by an error may
+    // have left behind
+    OM_HIP(hipMemsetAsync(tickets_.p, 0, (size_t)B * sizeof(unsigned int), stream_));
+    OM_HIP(hipMemsetAsync(dotAcc_.p, 0, 2 * (size_t)B * sizeof(ExactAcc), stream_));
     setFlags(actInner_, act);
     ops.copy<S>(a, kVv, -1);
     ops.copy<S>(a, kVp, -1);
@@ -409,7 +421,7 @@ void Ensemble::raiseIluError()
 void Ensemble::linearSolveImpl(const std::vector<int>& set, std::vector<int>& rc, std::vector<int>& its)
 {
     const int B = (int)members_.size();
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, dots(), ks_.p, stream_);
     setFlags(actSolve_, set);
     const int* a = actSolve_.p;
     // preconditioner: FP64 ILU(0) of the FP64 Jacobian
@@ -499,7 +511,7 @@ void Ensemble::timeStep()
     const int B = (int)members_.size();
     if (B == 0) return;
     using clk = std::chrono::steady_clock;
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, acc_.p, ks_.p, stream_);
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, dots(), ks_.p, stream_);
     enum { kIdle = 0, kNewton = 1, kOk = 2, kFailedStep = 3 };
     std::vector<int> state(B, kIdle);
     for (int b = 0; b < B; ++b) {

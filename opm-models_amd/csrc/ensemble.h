@@ -12,6 +12,7 @@
 namespace omodels {
 
 constexpr int kEnsChunk = 256;      // cells per block of the cell-parallel ensemble kernels
+constexpr int kEnsDotCells = 1024;  // cells per workgroup of the ensemble's scalar products
 
 // vectors of the Krylov solver (per instance, FP64 and FP32 sets)
 enum EnsVec : int { kVb = 0, kVx, kVr, kVp, kVv, kVy, kVz, kVr64, kNumVec };

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/omodels.h"
+#include "ens_ops.h"
 #include "ensemble.h"
 #include "omodels_host.h"
 
@@ -87,7 +88,11 @@ private:
     int nAsm_ = 0, nChunks_ = 0, nSrc_ = 0;
     DevBuf<double> partial_, maxOut_;
     DevBuf<KrylovScalars> ks_;
-    DevBuf<ExactAcc> acc_;       // two exact accumulators per member slot
+    DevBuf<int4> dotItems_;          // work items of the scalar products
+    DevBuf<ExactAcc> dotAcc_;        // two per member (zero between products)
+    DevBuf<unsigned int> tickets_;   // per member (zero between products)
+    int nDotItems_ = 0;
+    EnsDots dots() const { return EnsDots{dotItems_.p, nDotItems_, dotAcc_.p, tickets_.p}; }
     KrylovScalars* hostKs_ = nullptr;
     double* hostD_ = nullptr;
     int* hostI_ = nullptr;

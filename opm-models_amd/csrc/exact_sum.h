@@ -11,11 +11,24 @@
 // restatement computes the same value.
 //
 // Representation.  A finite double is m * 2^(pos - 1074) with an integer
-// m < 2^53 and 0 <= pos <= 2045.  The accumulator holds kAccLimbs signed
-// 64-bit limbs, limb j weighing 2^(32 j - 1074); a product is cut at the
-// limb boundary below it, so it adds a value below 2^32 to each of at most
-// four limbs.  Carries are deferred: with fewer than 2^28 products per
-// accumulator no limb can overflow.
+// m < 2^53 and 0 <= pos <= 2045.  An accumulator holds kAccLimbs signed
+// 64-bit limbs, limb j weighing 2^(32 j - 1074).  A product is cut at the
+// limb boundary below it (k = pos / 32): the integer m * 2^(pos mod 32) <
+// 2^85 goes as three chunks below 2^32 into limbs k, k+1, k+2.  Carries are
+// deferred: with fewer than 2^28 products no limb can overflow.
+//
+// Reduction.  A wave splits each product error-free into four bands fixed
+// by the wave's largest product (q = (sigma + r) - sigma, r -= q, for
+// sigma = 2^(e_max + 10), 2^(e_max - 33), ...): every band's parts are
+// multiples of its unit and their sum over the wave stays below 2^53 units,
+// so the bands add exactly in double; a product with bits below the last
+// band (2^-119 and less of the largest) adds its remainder to LDS directly.
+// The wave's four band sums go into the workgroup's limbs in LDS (waves whose
+// largest product is outside [2^-700, 2^1000] use a register window of
+// integer limbs instead).  Each workgroup then adds its nonzero limbs to the
+// device accumulator with device-scope atomics; the last workgroup to finish
+// (a ticket counter) reads the accumulator, rounds, and clears it and the
+// ticket for the next use.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,10 +38,11 @@ namespace omodels {
 
 constexpr int kAccLimbs = 68;      // 32-bit digits from 2^-1074; the largest product ends in digit 66
 
+// The device accumulator of one sum: limbs with deferred carries and the
+// non-finite products seen (1 NaN, 2 +inf, 4 -inf); zero between products.
 struct ExactAcc {
-    unsigned long long limb[kAccLimbs];   // two's-complement limbs, carries deferred
-    unsigned int flags;                    // non-finite products seen: 1 NaN, 2 +inf, 4 -inf
-    unsigned int pad_;
+    unsigned long long limb[kAccLimbs];
+    unsigned int flags, pad_;
 };
 
 // One product placed in the accumulator: |p| = (hi * 2^64 + lo) * 2^(32 k - 1074)
@@ -120,69 +134,126 @@ __host__ __device__ inline double exactRound(long long* d, int lo, int hi, unsig
 }
 
 #ifdef __HIPCC__
-// Adds the pieces of a wave (Q per lane) into the workgroup's limbs in LDS.
-// Lanes whose pieces share a limb index are summed exactly in 128 bits across
-// the wave, then one lane adds the sum to four limbs.  Every lane of the wave
-// must call this (absent entries are pieces with k < 0).
-template <int Q>
-__device__ __forceinline__ void accumulatePieces(const ProductPiece (&pc)[Q], unsigned long long* sAcc)
+constexpr int kWindow = 8;      // limbs of a wave's register window (256 bits)
+
+// Adds U products per lane of a wave to the workgroup's limbs sAcc (LDS,
+// kAccLimbs).  Every lane of the wave must call it (absent products: 0.0).
+template <int U>
+__device__ __forceinline__ void accumulateWave(const double (&p)[U], unsigned long long* sAcc, unsigned& flags)
 {
-    unsigned pend = 0u;
+    int k[U];
+    unsigned long long c0[U], c1[U], c2[U];
+    int kmax = -1;
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if (pc[q].k >= 0) pend |= 1u << q;
+    for (int u = 0; u < U; ++u) {
+        const ProductPiece q = makePiece(p[u], flags);
+        k[u] = q.k;
+        c0[u] = q.lo & 0xffffffffull; c1[u] = q.lo >> 32; c2[u] = q.hi;
+        if (q.neg) { c0[u] = 0ull - c0[u]; c1[u] = 0ull - c1[u]; c2[u] = 0ull - c2[u]; }
+        kmax = max(kmax, q.k);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, 64));
+    if (kmax < 0) return;                        // wave-uniform: no nonzero finite product
+    const int kb = kmax + 3 - kWindow;           // absolute limb of window limb 0
+    unsigned long long acc[kWindow];
+#pragma unroll
+    for (int w = 0; w < kWindow; ++w) acc[w] = 0ull;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (k[u] < 0) continue;
+        const int t = k[u] - kb;
+        if (t >= 0) {
+#pragma unroll
+            for (int w = 0; w < kWindow; ++w)
+                acc[w] += w == t ? c0[u] : (w == t + 1 ? c1[u] : (w == t + 2 ? c2[u] : 0ull));
+        } else {                                 // far below the wave's largest product
+            atomicAdd(&sAcc[k[u]], c0[u]);
+            atomicAdd(&sAcc[k[u] + 1], c1[u]);
+            atomicAdd(&sAcc[k[u] + 2], c2[u]);
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < kWindow; ++w)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[w] += __shfl_xor(acc[w], o, 64);
     const int lane = (int)(threadIdx.x & 63u);
-    for (;;) {
-        const unsigned long long any = __ballot(pend != 0u);
-        if (any == 0ull) break;
-        const int leader = __ffsll((unsigned long long)any) - 1;
-        int myk = 0;
+    unsigned long long v = 0ull;
 #pragma unroll
-        for (int q = Q - 1; q >= 0; --q)
-            if ((pend >> q) & 1u) myk = pc[q].k;
-        const int kk = __shfl(myk, leader, 64);
-        unsigned long long lo = 0ull;
-        long long hi = 0;
+    for (int w = 0; w < kWindow; ++w) v = lane == w ? acc[w] : v;
+    if (lane < kWindow && v && kb + lane >= 0) atomicAdd(&sAcc[kb + lane], v);
+}
+
+// one value's chunks straight into LDS limbs
+__device__ __forceinline__ void addToLimbs(double v, unsigned long long* sAcc, unsigned& flags)
+{
+    const ProductPiece q = makePiece(v, flags);
+    if (q.k < 0) return;
+    unsigned long long c0 = q.lo & 0xffffffffull, c1 = q.lo >> 32, c2 = q.hi;
+    if (q.neg) { c0 = 0ull - c0; c1 = 0ull - c1; c2 = 0ull - c2; }
+    atomicAdd(&sAcc[q.k], c0);
+    atomicAdd(&sAcc[q.k + 1], c1);
+    if (c2) atomicAdd(&sAcc[q.k + 2], c2);
+}
+
+// Adds U products per lane of a wave to the workgroup's limbs sAcc (LDS) by
+// the error-free band split.  Every lane of the wave must call it (absent
+// products: 0.0); U <= 64.
+template <int U>
+__device__ __forceinline__ void accumulateBands(const double (&p)[U], unsigned long long* sAcc, unsigned& flags)
+{
+    int emax = -2000;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            if (((pend >> q) & 1u) && pc[q].k == kk) {
-                pend &= ~(1u << q);
-                if (!pc[q].neg) {
-                    const unsigned long long n = lo + pc[q].lo;
-                    hi += (long long)pc[q].hi + (long long)(n < lo);
-                    lo = n;
-                } else {
-                    const unsigned long long n = lo - pc[q].lo;
-                    hi -= (long long)pc[q].hi + (long long)(lo < pc[q].lo);
-                    lo = n;
-                }
-            }
-        }
+    for (int u = 0; u < U; ++u)
+        if (p[u] != 0.0 && isfinite(p[u])) emax = max(emax, ilogb(p[u]));
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long olo = __shfl_xor(lo, o, 64);
-            const long long ohi = __shfl_xor(hi, o, 64);
-            const unsigned long long n = lo + olo;
-            hi += ohi + (long long)(n < lo);
-            lo = n;
+    for (int o = 32; o > 0; o >>= 1) emax = max(emax, __shfl_xor(emax, o, 64));
+    if (emax < -700 || emax > 1000) {            // no finite nonzero product, or extreme ones
+        accumulateWave<U>(p, sAcc, flags);
+        return;
+    }
+    constexpr int kBands = 4;
+    double S[kBands];
+#pragma unroll
+    for (int i = 0; i < kBands; ++i) S[i] = 0.0;
+    const double sigma0 = ldexp(1.0, emax + 10);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!isfinite(p[u])) { addToLimbs(p[u], sAcc, flags); continue; }
+        double r = p[u], sigma = sigma0;
+#pragma unroll
+        for (int i = 0; i < kBands; ++i) {
+            const double q = (sigma + r) - sigma;
+            S[i] += q;
+            r -= q;
+            sigma *= 0x1p-43;
         }
-        if (lane == 0) {
-            atomicAdd(&sAcc[kk], lo & 0xffffffffull);
-            atomicAdd(&sAcc[kk + 1], lo >> 32);
-            atomicAdd(&sAcc[kk + 2], (unsigned long long)hi & 0xffffffffull);
-            atomicAdd(&sAcc[kk + 3], (unsigned long long)(hi >> 32));
-        }
+        if (r != 0.0) addToLimbs(r, sAcc, flags);
+    }
+#pragma unroll
+    for (int i = 0; i < kBands; ++i)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) S[i] += __shfl_xor(S[i], o, 64);
+    const int lane = (int)(threadIdx.x & 63u);
+    if (lane < kBands) {
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < kBands; ++i) v = lane == i ? S[i] : v;
+        addToLimbs(v, sAcc, flags);
     }
 }
 
-// Workgroup-level accumulation: clear the LDS limbs, ... accumulate ..., flush
-// the nonzero limbs into the device accumulator with device-scope atomics.
-__device__ __forceinline__ void clearLimbs(unsigned long long* sAcc, int nAcc, unsigned* sFlags)
+// The workgroup's limbs of nAcc sums (LDS, nAcc * kAccLimbs) and their flags
+// (3 bits each) in sMeta[0]; sMeta[1..4] scratch.
+__device__ __forceinline__ void clearLimbs(unsigned long long* sAcc, int nAcc, unsigned* sMeta)
 {
     for (int t = (int)threadIdx.x; t < nAcc * kAccLimbs; t += (int)blockDim.x) sAcc[t] = 0ull;
-    if (threadIdx.x == 0) *sFlags = 0u;
+    if (threadIdx.x == 0) sMeta[0] = 0u;
 }
-__device__ __forceinline__ void flushLimbs(const unsigned long long* sAcc, int nAcc, const unsigned* sFlags,
+
+// Adds the workgroup's limbs of nAcc sums to the device accumulators g[0..nAcc)
+// (after the barrier that ends the accumulation).
+__device__ __forceinline__ void flushLimbs(const unsigned long long* sAcc, int nAcc, const unsigned* sMeta,
                                            ExactAcc* g)
 {
     for (int t = (int)threadIdx.x; t < nAcc * kAccLimbs; t += (int)blockDim.x) {
@@ -190,37 +261,51 @@ __device__ __forceinline__ void flushLimbs(const unsigned long long* sAcc, int n
         if (v) __hip_atomic_fetch_add(&g[t / kAccLimbs].limb[t % kAccLimbs], v, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (threadIdx.x == 0 && *sFlags)
-        for (int a = 0; a < nAcc; ++a)
-            __hip_atomic_fetch_or(&g[a].flags, (*sFlags >> (3 * a)) & 7u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0 && sMeta[0])
+        for (int a = 0; a < nAcc; ++a) {
+            const unsigned f = (sMeta[0] >> (3 * a)) & 7u;
+            if (f) __hip_atomic_fetch_or(&g[a].flags, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
 }
 
-// Reads and clears the device accumulator g and returns its rounded value in
-// every thread.  The block is exactly one wave of 64 threads; sD: LDS scratch
-// of kAccLimbs long longs.
-__device__ __forceinline__ double finishExact(ExactAcc* g, long long* sD)
+// After every thread's atomics: true in every thread of the workgroup that
+// finishes last among nBlocks; that workgroup resets the ticket.
+__device__ __forceinline__ bool lastToFinish(unsigned* ticket, int nBlocks, unsigned* sMeta)
 {
-    const int lane = (int)threadIdx.x;
-    const unsigned long long a0 = __hip_atomic_load(&g->limb[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long a1 = 0ull;
-    if (lane + 64 < kAccLimbs)
-        a1 = __hip_atomic_load(&g->limb[lane + 64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned flags = __hip_atomic_load(&g->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a0) __hip_atomic_store(&g->limb[lane], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a1) __hip_atomic_store(&g->limb[lane + 64], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lane == 0 && flags) __hip_atomic_store(&g->flags, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long nz0 = __ballot(a0 != 0ull), nz1 = __ballot(a1 != 0ull);
-    sD[lane] = (long long)a0;
-    if (lane + 64 < kAccLimbs) sD[lane + 64] = (long long)a1;
-    int lo = kAccLimbs, hi = -1;
-    if (nz0) { lo = __ffsll(nz0) - 1; hi = 63 - __clzll((long long)nz0); }
-    if (nz1) {
-        if (lo == kAccLimbs) lo = 64 + __ffsll(nz1) - 1;
-        hi = 64 + 63 - __clzll((long long)nz1);
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // this wave's atomics acknowledged
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool last = t == (unsigned)nBlocks - 1u;
+        if (last) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sMeta[1] = last ? 1u : 0u;
+    }
+    __syncthreads();
+    return sMeta[1] != 0u;
+}
+
+// In the last workgroup: the rounded value of the device accumulator g, which
+// is cleared for the next use.  sD: LDS of kAccLimbs long longs.
+__device__ __forceinline__ double finishExact(ExactAcc* g, long long* sD, unsigned* sMeta)
+{
+    if (threadIdx.x < kAccLimbs) {
+        const unsigned long long v =
+            __hip_atomic_load(&g->limb[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sD[threadIdx.x] = (long long)v;
+        if (v) __hip_atomic_store(&g->limb[threadIdx.x], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) {
+        sMeta[2] = __hip_atomic_load(&g->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sMeta[2]) __hip_atomic_store(&g->flags, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     __shared__ double sR;
-    if (lane == 0) sR = exactRound(sD, lo, hi, flags);
+    if (threadIdx.x == 0) {
+        int lo = kAccLimbs, hi = -1;
+        for (int t = 0; t < kAccLimbs; ++t)
+            if (sD[t]) { if (lo == kAccLimbs) lo = t; hi = t; }
+        sR = exactRound(sD, lo, hi, sMeta[2]);
+    }
     __syncthreads();
     return sR;
 }

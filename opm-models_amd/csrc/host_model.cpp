@@ -70,8 +70,8 @@ void LinearSystem::initSystem(hipStream_t s)
     if (s) stream = s;
     else { OM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); ownStream = true; }
     partial.alloc(2 * kRedBlocks);
-    acc.alloc(2);
-    OM_HIP(hipMemsetAsync(acc.p, 0, 2 * sizeof(ExactAcc), stream));
+    dots.alloc(1);
+    OM_HIP(hipMemsetAsync(dots.p, 0, sizeof(DotScratch), stream));
     scal.alloc(8);
     OM_HIP(hipHostMalloc((void**)&hostScal, 8 * sizeof(double), hipHostMallocDefault));
 }
@@ -585,9 +585,10 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     // Jacobi sweeps start from their content
     OM_HIP(hipMemsetAsync(y, 0, m * sizeof(S), s));
     OM_HIP(hipMemsetAsync(z, 0, m * sizeof(S), s));
-    // the final kernel of every scalar product clears its accumulator; this
-    // clears what a solve interrupted by an error may have left behind
-    OM_HIP(hipMemsetAsync(m_.acc.p, 0, 2 * sizeof(ExactAcc), s));
+    // the last workgroup of every scalar product clears the ticket and the
+    // accumulators; this clears what a solve interrupted by an error may have
+    // left behind
+    OM_HIP(hipMemsetAsync(m_.dots.p, 0, sizeof(DotScratch), s));
 
     // first half: rho, p, y = M^-1 p, v = A y, alpha, x/r update, convergence test.
     // Every kernel of the solve returns at once after the test has ended it
@@ -595,7 +596,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     // seen the test of iteration it, and the device never waits for the host.
     bool fusedSpmv = false;     // the operator runs inside the preconditioner's backward sweep
     auto firstHalf = [&]() {
-        launchDotFinal<S>(m, b, r, m_.acc.p, ks_.p, 0, 0, s);
+        launchDotFinal<S>(m, b, r, m_.dots.p, ks_.p, 0, 0, s);
         launchUpdateP<S>(m, r, v, p, ks_.p, s);
         fusedSpmv = precApply<S>(lu, dinv, p, y, run, A, v);
         if (!fusedSpmv) {
@@ -603,7 +604,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
             launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s, run);
             m_.timer.end(s);
         }
-        launchDotFinal<S>(m, b, v, m_.acc.p, ks_.p, 1, 0, s);
+        launchDotFinal<S>(m, b, v, m_.dots.p, ks_.p, 1, 0, s);
         launchUpdateXR<S>(m, y, v, x, r, m_.partial.p, ks_.p, s);
     };
     // second half: z = M^-1 r, y = A z, omega, x/r update, next tag base
@@ -614,7 +615,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
             launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s, run);
             m_.timer.end(s);
         }
-        launchDot2Final<S>(m, y, r, m_.acc.p, ks_.p, 1, s);
+        launchDot2Final<S>(m, y, r, m_.dots.p, ks_.p, 1, s);
         launchUpdateXR2<S>(m, z, y, x, r, m_.partial.p, ks_.p, 1, 2, s);
     };
 

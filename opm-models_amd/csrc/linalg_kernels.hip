@@ -487,106 +487,103 @@ __device__ __forceinline__ double finalReduce(const double* __restrict__ partial
 
 // Scalar products: the exact sum of the products, rounded once (exact_sum.h),
 // so the value is the same for every launch shape and equals the CPU
-// restatement's.  Each wave takes kDotQ * 64 consecutive entries per pass;
-// workgroups add their limbs to the device accumulator, whose final kernel
-// rounds, evaluates the BiCGStab scalars and clears it for the next use.
-constexpr int kDotQ = 8;
-constexpr int kDotThreads = 256;
-constexpr int kDotMaxBlocks = 1024;
+// restatement's.  Every workgroup adds its products into limbs in LDS and
+// those into the device accumulator; the last workgroup to finish rounds and
+// evaluates the BiCGStab scalars of the product (one launch per product).
+constexpr int kDotUnroll = 8;
 
 inline int dotBlocks(long m)
 {
-    const long per = (long)kDotQ * kDotThreads;
+    const long per = (long)kDotUnroll * kDotThreads;
     return (int)std::max(1L, std::min<long>(kDotMaxBlocks, (m + per - 1) / per));
 }
 
-// NACC = 1: acc[0] += a.b;  NACC = 2: acc[0] += a.b, acc[1] += a.c
-template <class S, int NACC>
-__global__ __launch_bounds__(kDotThreads) void k_xdot_partial(long m, const S* __restrict__ a,
-    const S* __restrict__ b, const S* __restrict__ c, ExactAcc* __restrict__ acc, const int* __restrict__ run)
-{
-    if (run && !*run) return;
-    __shared__ unsigned long long sAcc[NACC * kAccLimbs];
-    __shared__ unsigned sFlags;
-    clearLimbs(sAcc, NACC, &sFlags);
-    __syncthreads();
-    const int lane = (int)(threadIdx.x & 63u);
-    const long wavesPerBlock = kDotThreads / 64, wavesTotal = (long)gridDim.x * wavesPerBlock;
-    unsigned fl0 = 0u, fl1 = 0u;
-    for (long w = (long)blockIdx.x * wavesPerBlock + (long)(threadIdx.x >> 6); w * (kDotQ * 64) < m; w += wavesTotal) {
-        const long base = w * (kDotQ * 64) + lane;
-        ProductPiece p0[kDotQ], p1[kDotQ];
-#pragma unroll
-        for (int q = 0; q < kDotQ; ++q) {
-            const long i = base + 64L * q;
-            double v0 = 0.0, v1 = 0.0;
-            if (i < m) {
-                const double ai = (double)a[i];
-                v0 = ai * (double)b[i];
-                if (NACC == 2) v1 = ai * (double)c[i];
-            }
-            p0[q] = makePiece(v0, fl0);
-            if (NACC == 2) p1[q] = makePiece(v1, fl1);
-        }
-        accumulatePieces<kDotQ>(p0, sAcc);
-        if (NACC == 2) accumulatePieces<kDotQ>(p1, sAcc + kAccLimbs);
-    }
-    if (fl0 | fl1) atomicOr(&sFlags, fl0 | (fl1 << 3));
-    __syncthreads();
-    flushLimbs(sAcc, NACC, &sFlags, acc);
-}
-
-// op 0: rho_i = <r0hat, r>, beta; op 1: denom = <r0hat, v>, alpha
 __device__ __forceinline__ void markFail(KrylovScalars* ks, int tag)
 {
     if (!ks->fail_code) { ks->fail_code = 2; ks->fail_tag = tag; }
 }
 
-__global__ __launch_bounds__(64) void k_xdot_final(ExactAcc* acc, KrylovScalars* ks, int op, double eps, int tagOff)
+// what the last workgroup does with the sums (s0, s1)
+__device__ __forceinline__ void dotEpilogue(int mode, double s0, double s1, KrylovScalars* ks, double eps, int tagOff,
+                                            double* out)
 {
-    if (!ks->run) return;
-    __shared__ long long sD[kAccLimbs];
-    const double s = finishExact(acc, sD);
-    if (threadIdx.x != 0) return;
+    if (mode == kDotValue) { *out = s0; return; }
     const int tag = ks->tag_base + tagOff;
-    if (op == 0) {
-        ks->rho_i = s;
+    if (mode == kDotRho) {             // rho_i = <r0hat, r>, beta (bicgstabsolver.hh:197-206)
+        ks->rho_i = s0;
         if (fabs(ks->rho) <= eps || fabs(ks->omega) <= eps) markFail(ks, tag);
-        ks->beta = (s / ks->rho) * (ks->alpha / ks->omega);
-        ks->rho = s;
-    } else {
-        ks->denom = s;
-        if (fabs(s) <= eps) markFail(ks, tag);
-        ks->alpha = ks->rho / s;
+        ks->beta = (s0 / ks->rho) * (ks->alpha / ks->omega);
+        ks->rho = s0;
+    } else if (mode == kDotAlpha) {    // denom = <r0hat, v>, alpha (:230-236)
+        ks->denom = s0;
+        if (fabs(s0) <= eps) markFail(ks, tag);
+        ks->alpha = ks->rho / s0;
         if (fabs(ks->alpha) <= eps) markFail(ks, tag);
+    } else {                           // omega = <t, s> / <t, t> (:287-292)
+        if (fabs(s0) <= eps) markFail(ks, tag);
+        ks->omega = s1 / s0;
+        if (fabs(ks->omega) <= eps) markFail(ks, tag);
     }
 }
 
+// NACC = 1: sum a.b;  NACC = 2: sums a.b and a.c
+template <class S, int NACC>
+__global__ __launch_bounds__(kDotThreads) void k_xdot(long m, const S* __restrict__ a, const S* __restrict__ b,
+    const S* __restrict__ c, DotScratch* __restrict__ sc, KrylovScalars* ks, int mode, double eps, int tagOff,
+    double* out)
+{
+    if (ks && !ks->run) return;
+    constexpr int nw = kDotThreads / 64;
+    __shared__ unsigned long long sAcc[NACC * kAccLimbs];
+    __shared__ unsigned sMeta[5];
+    clearLimbs(sAcc, NACC, sMeta);
+    __syncthreads();
+    const int lane = (int)(threadIdx.x & 63u);
+    const long wavesTotal = (long)gridDim.x * nw;
+    unsigned fl0 = 0u, fl1 = 0u;
+    // each wave takes kDotUnroll * 64 consecutive entries per pass (a uniform loop)
+    for (long w = (long)blockIdx.x * nw + (long)(threadIdx.x >> 6); w * (kDotUnroll * 64) < m; w += wavesTotal) {
+        const long base = w * (kDotUnroll * 64) + lane;
+        double v0[kDotUnroll], v1[kDotUnroll];
+#pragma unroll
+        for (int u = 0; u < kDotUnroll; ++u) {
+            const long i = base + 64L * u;
+            v0[u] = 0.0; v1[u] = 0.0;
+            if (i < m) {
+                const double ai = (double)a[i];
+                v0[u] = ai * (double)b[i];
+                if (NACC == 2) v1[u] = ai * (double)c[i];
+            }
+        }
+        accumulateBands<kDotUnroll>(v0, sAcc, fl0);
+        if (NACC == 2) accumulateBands<kDotUnroll>(v1, sAcc + kAccLimbs, fl1);
+    }
+    if (fl0 | fl1) atomicOr(&sMeta[0], fl0 | (fl1 << 3));
+    __syncthreads();
+    flushLimbs(sAcc, NACC, sMeta, sc->acc);
+    if (!lastToFinish(&sc->ticket, (int)gridDim.x, sMeta)) return;
+    __shared__ long long sD[kAccLimbs];
+    const double s0 = finishExact(&sc->acc[0], sD, sMeta);
+    const double s1 = NACC == 2 ? finishExact(&sc->acc[1], sD, sMeta) : 0.0;
+    if (threadIdx.x == 0) dotEpilogue(mode, s0, s1, ks, eps, tagOff, out);
+}
+
 template <class S>
-void launchDotFinal(long m, const S* x, const S* y, ExactAcc* acc, KrylovScalars* ks, int op, int tagOff,
+void launchDotFinal(long m, const S* x, const S* y, DotScratch* sc, KrylovScalars* ks, int op, int tagOff,
                     hipStream_t st)
 {
-    hipLaunchKernelGGL((k_xdot_partial<S, 1>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, x, y, y, acc,
-                       &ks->run);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_xdot_final, dim3(1), dim3(64), 0, st, acc, ks, op, eps, tagOff);
+    hipLaunchKernelGGL((k_xdot<S, 1>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, x, y, y, sc, ks,
+                       op == 0 ? (int)kDotRho : (int)kDotAlpha, eps, tagOff, (double*)nullptr);
 }
+template void launchDotFinal<double>(long, const double*, const double*, DotScratch*, KrylovScalars*, int, int, hipStream_t);
+template void launchDotFinal<float>(long, const float*, const float*, DotScratch*, KrylovScalars*, int, int, hipStream_t);
 
-template void launchDotFinal<double>(long, const double*, const double*, ExactAcc*, KrylovScalars*, int, int, hipStream_t);
-
-__global__ __launch_bounds__(64) void k_xdot_value(ExactAcc* acc, double* out)
+void launchScalarProduct(long m, const double* x, const double* y, DotScratch* sc, double* out, hipStream_t st)
 {
-    __shared__ long long sD[kAccLimbs];
-    const double s = finishExact(acc, sD);
-    if (threadIdx.x == 0) *out = s;
+    hipLaunchKernelGGL((k_xdot<double, 1>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, x, y, y, sc,
+                       (KrylovScalars*)nullptr, (int)kDotValue, 0.0, 0, out);
 }
-void launchScalarProduct(long m, const double* x, const double* y, ExactAcc* acc, double* out, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_xdot_partial<double, 1>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, x, y, y, acc,
-                       nullptr);
-    hipLaunchKernelGGL(k_xdot_value, dim3(1), dim3(64), 0, st, acc, out);
-}
-template void launchDotFinal<float>(long, const float*, const float*, ExactAcc*, KrylovScalars*, int, int, hipStream_t);
 
 // p = r + (-beta) * (v*omega - p)   (bicgstabsolver.hh:211-222)
 template <class S>
@@ -675,28 +672,15 @@ template void launchUpdateXR<double>(long, const double*, const double*, double*
 template void launchUpdateXR<float>(long, const float*, const float*, float*, float*, double*, KrylovScalars*, hipStream_t);
 
 // <t,t> and <t,s> in one pass; omega = <t,s>/<t,t>  (:287-292)
-__global__ __launch_bounds__(64) void k_xdot2_final(ExactAcc* acc, KrylovScalars* ks, double eps, int tagOff)
-{
-    if (!ks->run) return;
-    __shared__ long long sD[kAccLimbs];
-    const double tt = finishExact(acc, sD);
-    const double ts = finishExact(acc + 1, sD);
-    if (threadIdx.x != 0) return;
-    const int tag = ks->tag_base + tagOff;
-    if (fabs(tt) <= eps) markFail(ks, tag);
-    ks->omega = ts / tt;
-    if (fabs(ks->omega) <= eps) markFail(ks, tag);
-}
 template <class S>
-void launchDot2Final(long m, const S* t, const S* s, ExactAcc* acc, KrylovScalars* ks, int tagOff, hipStream_t st)
+void launchDot2Final(long m, const S* t, const S* s, DotScratch* sc, KrylovScalars* ks, int tagOff, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_xdot_partial<S, 2>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, t, t, s, acc,
-                       &ks->run);
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
-    hipLaunchKernelGGL(k_xdot2_final, dim3(1), dim3(64), 0, st, acc, ks, eps, tagOff);
+    hipLaunchKernelGGL((k_xdot<S, 2>), dim3(dotBlocks(m)), dim3(kDotThreads), 0, st, m, t, t, s, sc, ks,
+                       (int)kDotOmega, eps, tagOff, (double*)nullptr);
 }
-template void launchDot2Final<double>(long, const double*, const double*, ExactAcc*, KrylovScalars*, int, hipStream_t);
-template void launchDot2Final<float>(long, const float*, const float*, ExactAcc*, KrylovScalars*, int, hipStream_t);
+template void launchDot2Final<double>(long, const double*, const double*, DotScratch*, KrylovScalars*, int, hipStream_t);
+template void launchDot2Final<float>(long, const float*, const float*, DotScratch*, KrylovScalars*, int, hipStream_t);
 
 // x += omega*z (criterion change indicator z), r += -omega*t  (:296-325)
 template <class S>

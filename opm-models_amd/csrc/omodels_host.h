@@ -47,7 +47,7 @@ struct LinearSystem {
     DevBuf<int> rowPtr, colIdx, diagOff;
     LevelSchedule levels;
     DevBuf<double> partial;      // reduction partials
-    DevBuf<ExactAcc> acc;        // exact accumulators of the scalar products (2)
+    DevBuf<DotScratch> dots;     // scratch of the exact scalar products
     DevBuf<double> scal;         // device scalars (norms)
     double* hostScal = nullptr;  // pinned mirror
     KernelTimer timer;

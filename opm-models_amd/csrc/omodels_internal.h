@@ -209,12 +209,22 @@ template <class S> void launchExtractInvDiag(int n, int bs, const int* diag, con
 
 // BiCGStab steps (see bicgstab in host_solver.cpp for the sequence)
 // tagOff: 0 for the first half of an iteration, 1 for the second (tag = ks->tag_base + tagOff)
-// scalar products into an exact accumulator (exact_sum.h; the final kernel
-// clears it again): dot uses acc[0], dot2 acc[0] and acc[1]
-template <class S> void launchDotFinal(long m, const S* x, const S* y, ExactAcc* acc, KrylovScalars* ks,
+// Scratch of the exact scalar products (exact_sum.h): the device
+// accumulators of up to two sums and the ticket of the workgroup that
+// finishes last (all zero between products: that workgroup clears them).
+constexpr int kDotMaxBlocks = 256;
+constexpr int kDotThreads = 512;
+struct DotScratch {
+    ExactAcc acc[2];
+    unsigned int ticket;
+    unsigned int pad_[63];
+};
+enum DotMode : int { kDotRho = 0, kDotAlpha = 1, kDotOmega = 2, kDotValue = 3 };
+// one launch per product: op 0 rho_i/beta, op 1 denom/alpha
+template <class S> void launchDotFinal(long m, const S* x, const S* y, DotScratch* sc, KrylovScalars* ks,
                                        int op, int tagOff, hipStream_t st);
-// out = <x, y> on the device (acc: one zero accumulator, left zero)
-void launchScalarProduct(long m, const double* x, const double* y, ExactAcc* acc, double* out, hipStream_t st);
+// out = <x, y> on the device
+void launchScalarProduct(long m, const double* x, const double* y, DotScratch* sc, double* out, hipStream_t st);
 // the same with the accumulator in a static device array (noalloc.hip): no allocation
 void scalarProductNoAlloc(long m, const double* x, const double* y, double* out, hipStream_t st);
 template <class S> void launchUpdateP(long m, const S* r, const S* v, S* p, const KrylovScalars* ks,
@@ -224,7 +234,7 @@ template <class S> void launchUpdateXR(long m, const S* y, const S* v, S* x, S* 
 // the second half ends the iteration: its final kernel advances tag_base by `advance`
 template <class S> void launchUpdateXR2(long m, const S* z, const S* t, S* x, S* r, double* partial,
                                         KrylovScalars* ks, int tagOff, int advance, hipStream_t st);
-template <class S> void launchDot2Final(long m, const S* t, const S* s, ExactAcc* acc, KrylovScalars* ks,
+template <class S> void launchDot2Final(long m, const S* t, const S* s, DotScratch* sc, KrylovScalars* ks,
                                         int tagOff, hipStream_t st);
 template <class S> void launchMaxAbs(long m, const S* x, double* partial, double* out, hipStream_t st);
 void launchResidualUpdate(int n, int bs, const int* rp, const int* ci, const double* a, const double* x,
