@@ -163,6 +163,26 @@ int omodels_model_get_storage1(omodels_model* m, double* storage1);
 int omodels_linearizer_pattern(omodels_model* m, int32_t* row_ptr, int32_t* col_idx,
                                int32_t* diag_off, int32_t* nb_ptr, int32_t* nb_cell,
                                int32_t* nb_off, double* nb_trans, double* nb_dzg);
+/* updateStoredTransmissibilities() (tpfalinearizer.hh:868-883) and the other
+ * per-neighbour inputs createMatrix_ takes from the problem and the model
+ * (:443-452), given by the caller in place of the structured grid's own:
+ *   trans   [3][N] problem.transmissibility(I, J) of face I|I+e_axis at
+ *           axis*N + I (entries of faces on the boundary are ignored);
+ *   depth   [N] problem.dofCenterDepth(I), the cell centre's coordinate along
+ *           gravity's axis: dZg = (depth_I - depth_J) * gravity; the grid's
+ *           own is dz * (k + 0.5);
+ *   volume  [N] model.dofTotalVolume(I) > 0: storefac = volume / dt (:776-778),
+ *           the sources' volume (:834) and the Vin / Vex of the upwind
+ *           decision when the pressure difference is zero;
+ *   thpres  [3][N] problem.thresholdPressure(I, J) >= 0 at the face's index:
+ *           a face whose |pressure difference| is below it carries no flow,
+ *           above it the difference is reduced by it.
+ * Each may be NULL for the grid's own (for thpres: none); a call replaces all
+ * four.  The new values hold for every later linearization, FLOWS output and
+ * simulate call of the model, and nb_trans / nb_dzg of
+ * omodels_linearizer_pattern report them. */
+int omodels_linearizer_update_discretization(omodels_model* m, const double* trans, const double* depth,
+                                             const double* volume, const double* thpres);
 /* linearizeDomain() (tpfalinearizer.hh:189-245, linearize_ :711-866) at the
  * current solution(0); iteration_index 0 refreshes the storage cache of
  * time level 1 as recycleFirstIterationStorage() does (:786-812). */
@@ -177,9 +197,12 @@ int omodels_linearizer_erase_matrix(omodels_model* m);
  * (computeSourceDense instead of computeSource, :829-833) and adds them after
  * it as sparse blocks, as wellModel().addReservoirSourceTerms does (:842-846);
  * the problem's source list plays the well model.  Either way each cell's
- * source term is the last addition to its row, so both settings give the same
- * bits.  Further terms (an external well model's) go in through
- * add_sparse_sources after linearize_domain. */
+ * source term is the last addition to its row.  The cell loop adds a cell's
+ * one rate (problem.source: the sum of the cell's sources in the list's
+ * order, times -V); the sparse path adds each source's block in turn.  Where
+ * no cell has two sources both settings give the same bits.  Further terms
+ * (an external well model's) go in through add_sparse_sources after
+ * linearize_domain. */
 int omodels_linearizer_set_separate_sparse_sources(omodels_model* m, int enable);
 /* The problem's source terms at the current solution(0) as sparse blocks:
  * cells [n_src], residual contributions [n_src][3] (already times -V, the

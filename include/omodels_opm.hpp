@@ -1,0 +1,183 @@
+/* omodels_opm.hpp -- header-only C++ adapters that put the C ABI of omodels.h
+ * behind the interfaces opm-models' TypeTag properties expect of a linearizer
+ * (TpfaLinearizer, tpfalinearizer.hh:116-379 and :868-883), a linear-solver
+ * backend (ParallelBaseBackend, parallelbasebackend.hh:137-290) and the
+ * black-oil Newton update (BlackOilNewtonMethod::updatePrimaryVariables_,
+ * blackoilnewtonmethod.hh:226-427).
+ *
+ * The simulator side is reached through one customization point, Glue<Sim>,
+ * which the integrating code specializes for its Simulator type:
+ *
+ *   template <> struct omodels_opm::Glue<MySimulator> {
+ *     // the problem description (grid, permeability, porosity, PVT tables,
+ *     // wells as source terms); the arrays must outlive the call
+ *     static omodels_blackoil_problem describe(const MySimulator&);
+ *     // solution(0) as [N][3] primary variables and [N] gas meanings
+ *     static void packSolution(const MySimulator&, std::vector<double>& pv, std::vector<uint8_t>& meaning);
+ *     static void unpackSolution(MySimulator&, const std::vector<double>& pv, const std::vector<uint8_t>& meaning);
+ *     static double timeStepSize(const MySimulator&);          // simulator.timeStepSize()
+ *     static int newtonIteration(const MySimulator&);          // newtonMethod().numIterations()
+ *     // optional per-neighbour inputs (tpfalinearizer.hh:443-452); leave a
+ *     // vector empty for the structured grid's own values
+ *     static void discretization(const MySimulator&, std::vector<double>& trans, std::vector<double>& depth,
+ *                                std::vector<double>& volume, std::vector<double>& thpres);
+ *   };
+ *
+ * Errors: a negative return code of the C ABI throws std::runtime_error with
+ * omodels_last_error(); a numerical failure (return code 2) throws
+ * omodels_opm::NumericalProblem, the reference's Opm::NumericalProblem role.
+ */
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "omodels.h"
+
+namespace omodels_opm {
+
+struct NumericalProblem : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline int check(int rc, const char* what)
+{
+    if (rc < 0) throw std::runtime_error(std::string(what) + ": " + omodels_last_error());
+    if (rc == 2) throw NumericalProblem(std::string(what) + ": " + omodels_last_error());
+    return rc;
+}
+
+template <class Simulator>
+struct Glue;    // specialized by the integrating code
+
+/* Linearizer: createMatrix_ (init), linearizeDomain, jacobian() / residual()
+ * as host arrays in the Dune layouts (row-major 3x3 blocks in the sorted
+ * BCRS pattern, residual [N][3]), eraseMatrix, updateStoredTransmissibilities. */
+template <class Simulator, class G = Glue<Simulator>>
+class GpuTpfaLinearizer {
+public:
+    GpuTpfaLinearizer() = default;
+    GpuTpfaLinearizer(const GpuTpfaLinearizer&) = delete;
+    GpuTpfaLinearizer& operator=(const GpuTpfaLinearizer&) = delete;
+    ~GpuTpfaLinearizer()
+    {
+        if (model_) omodels_model_destroy(model_);
+    }
+
+    void init(Simulator& simulator)
+    {
+        simulator_ = &simulator;
+        const omodels_blackoil_problem p = G::describe(simulator);
+        model_ = omodels_model_create(&p);
+        if (!model_) throw std::runtime_error(std::string("model_create: ") + omodels_last_error());
+        int64_t n = 0, nnzb = 0;
+        check(omodels_model_num_cells(model_, &n, &nnzb), "num_cells");
+        n_ = n;
+        nnzb_ = nnzb;
+        rowPtr_.resize(n + 1);
+        colIdx_.resize(nnzb);
+        check(omodels_linearizer_pattern(model_, rowPtr_.data(), colIdx_.data(), nullptr, nullptr, nullptr,
+                                         nullptr, nullptr, nullptr),
+              "pattern");
+        jacobian_.assign(9 * (size_t)nnzb, 0.0);
+        residual_.assign(3 * (size_t)n, 0.0);
+        updateStoredTransmissibilities();
+    }
+
+    /* tpfalinearizer.hh:189-245 at the simulator's current solution */
+    void linearizeDomain()
+    {
+        std::vector<double> pv;
+        std::vector<uint8_t> meaning;
+        G::packSolution(*simulator_, pv, meaning);
+        if ((int64_t)pv.size() != 3 * n_ || (int64_t)meaning.size() != n_)
+            throw std::invalid_argument("packSolution: wrong sizes");
+        check(omodels_model_set_solution(model_, pv.data(), meaning.data(), 0), "set_solution");
+        check(omodels_linearizer_linearize_domain(model_, G::timeStepSize(*simulator_),
+                                                  G::newtonIteration(*simulator_)),
+              "linearize_domain");
+        check(omodels_linearizer_get_jacobian(model_, jacobian_.data()), "get_jacobian");
+        check(omodels_linearizer_get_residual(model_, residual_.data()), "get_residual");
+    }
+
+    /* updateStoredTransmissibilities (tpfalinearizer.hh:868-883), with the
+     * depths, volumes and threshold pressures createMatrix_ reads (:443-452) */
+    void updateStoredTransmissibilities()
+    {
+        std::vector<double> t, z, v, th;
+        G::discretization(*simulator_, t, z, v, th);
+        auto arr = [](const std::vector<double>& a, int64_t n) -> const double* {
+            if (a.empty()) return nullptr;
+            if ((int64_t)a.size() != n) throw std::invalid_argument("discretization: wrong sizes");
+            return a.data();
+        };
+        check(omodels_linearizer_update_discretization(model_, arr(t, 3 * n_), arr(z, n_), arr(v, n_),
+                                                       arr(th, 3 * n_)),
+              "update_discretization");
+    }
+
+    void eraseMatrix() { check(omodels_linearizer_erase_matrix(model_), "erase_matrix"); }
+
+    const std::vector<int32_t>& rowPtr() const { return rowPtr_; }
+    const std::vector<int32_t>& colIdx() const { return colIdx_; }
+    const std::vector<double>& jacobian() const { return jacobian_; }
+    const std::vector<double>& residual() const { return residual_; }
+    int64_t numCells() const { return n_; }
+    omodels_model* handle() const { return model_; }
+
+private:
+    Simulator* simulator_ = nullptr;
+    omodels_model* model_ = nullptr;
+    int64_t n_ = 0, nnzb_ = 0;
+    std::vector<int32_t> rowPtr_, colIdx_;
+    std::vector<double> jacobian_, residual_;
+};
+
+/* Linear-solver backend: prepare(M, b), setResidual, getResidual, solve(x),
+ * iterations (parallelbasebackend.hh:137-290) with ISTL BiCGStab and the
+ * block ILU(0) / Jacobi preconditioner the parameters select. */
+class GpuBiCGStabBackend {
+public:
+    GpuBiCGStabBackend(omodels_model* model, const omodels_params& prm)
+        : solver_(omodels_solver_create(model, &prm))
+    {
+        if (!solver_) throw std::runtime_error(std::string("solver_create: ") + omodels_last_error());
+    }
+    GpuBiCGStabBackend(const GpuBiCGStabBackend&) = delete;
+    GpuBiCGStabBackend& operator=(const GpuBiCGStabBackend&) = delete;
+    ~GpuBiCGStabBackend() { omodels_solver_destroy(solver_); }
+
+    void prepare(const std::vector<double>& values, const std::vector<double>& b)
+    {
+        check(omodels_solver_prepare(solver_, values.data(), b.data()), "prepare");
+    }
+    void setResidual(const std::vector<double>& b) { check(omodels_solver_prepare(solver_, nullptr, b.data()), "set_residual"); }
+    void getResidual(std::vector<double>& b) const { check(omodels_solver_get_residual(solver_, b.data()), "get_residual"); }
+    /* false when the solver did not converge, as the reference's solve() */
+    bool solve(std::vector<double>& x) { return check(omodels_solver_solve(solver_, x.data()), "solve") == 0; }
+    int iterations() const { return omodels_solver_iterations(solver_); }
+
+private:
+    omodels_solver* solver_;
+};
+
+/* The black-oil Newton update of the linearizer's model state
+ * (blackoilnewtonmethod.hh:226-427): solution(0) -= delta with the chopping
+ * and the switching of the gas meaning; returns the number of switched cells.
+ * The caller copies the new state back with Glue::unpackSolution. */
+template <class Simulator, class G = Glue<Simulator>>
+int newtonUpdate(GpuTpfaLinearizer<Simulator, G>& lin, Simulator& sim, const omodels_params& prm,
+                 const std::vector<double>& delta)
+{
+    int32_t switched = 0;
+    check(omodels_newton_update(lin.handle(), &prm, delta.data(), &switched), "newton_update");
+    std::vector<double> pv(3 * (size_t)lin.numCells());
+    std::vector<uint8_t> meaning((size_t)lin.numCells());
+    check(omodels_model_get_solution(lin.handle(), pv.data(), meaning.data()), "get_solution");
+    G::unpackSolution(sim, pv, meaning);
+    return switched;
+}
+
+}  // namespace omodels_opm

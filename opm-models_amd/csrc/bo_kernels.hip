@@ -81,11 +81,11 @@ template <bool Diff, bool Disp>
 __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const IqV& qj, double trans,
                                           double area, double dZg, bool upInTie, double r[3],
                                           double D[9], double* __restrict__ Jout, const BoDiff& dif,
-                                          DiffFace df, const BoDisp& dsp, double& nv)
+                                          DiffFace df, const BoDisp& dsp, double& nv, double thp)
 {
     E3 fl[3];
     double darcy[3];
-    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp);
+    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp, thp);
     if constexpr (Disp) maxVelocity(nv, darcy, area);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -107,14 +107,17 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
 
 // One (x, y) tile of columns over the planes of one z chunk.  Shared by the
 // single-reservoir kernel and the ensemble kernel (per-instance pointers).
-// Diff / Disp: with the diffusive / dispersive face term.
-template <bool Diff, bool Disp>
+// Diff / Disp: with the diffusive / dispersive face term.  Geo: depths,
+// volumes and threshold pressures from the caller (geo) instead of the
+// structured grid's own; the volume ties of the upwind decision are then
+// decided by the volumes, as the reference's Vin / Vex do.
+template <bool Diff, bool Disp, bool Geo>
 __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
     double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
-    int bx, int by, int bz, IqLds (*plane)[kTX + 2], const BoDiff& dif, const BoDisp& dsp)
+    int bx, int by, int bz, IqLds (*plane)[kTX + 2], const BoDiff& dif, const BoDisp& dsp, const BoGeo& geo)
 {
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
     const int i = bx * kTX + tx;
@@ -168,54 +171,77 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
             double D[9];
 #pragma unroll
             for (int t = 0; t < 9; ++t) D[t] = 0.0;
-            const double zI = G.dz * ((double)k + 0.5);
+            const double zI = Geo ? geo.depth[I] : G.dz * ((double)k + 0.5);
+            // the neighbour's depth, the volume tie of the upwind decision and
+            // the face's threshold pressure (plain: the grid's own)
+            auto zOf = [&](long J, int dk) -> double {
+                if constexpr (Geo) return geo.depth[J];
+                else return dk == 0 ? zI : G.dz * ((double)(k + dk) + 0.5);
+            };
+            auto tieOf = [&](long J) -> bool {
+                if constexpr (Geo) {
+                    const double vi = geo.vol[I], vj = geo.vol[J];
+                    return vi > vj || (vi == vj && I < J);
+                } else {
+                    return I < J;
+                }
+            };
+            auto thpOf = [&](long fi) -> double {
+                if constexpr (Geo) return geo.thp[fi];
+                else return 0.0;
+            };
             // intersection order: -x, +x, -y, +y, -z, +z
             if (i > 0) {
                 IqV q; fromLds(plane[ty + 1][tx], q);
-                applyFace<Diff, Disp>(f, qi, q, trans[I - 1], G.area[0], (zI - zI) * G.g, false, r, D,
-                                jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - 1, q, I - 1), dsp, nvo);
+                applyFace<Diff, Disp>(f, qi, q, trans[I - 1], G.area[0], (zI - zOf(I - 1, 0)) * G.g, tieOf(I - 1), r, D,
+                                jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - 1, q, I - 1), dsp,
+                                nvo, thpOf(I - 1));
             }
             if (i + 1 < G.nx) {
                 IqV q; fromLds(plane[ty + 1][tx + 2], q);
-                applyFace<Diff, Disp>(f, qi, q, trans[I], G.area[0], (zI - zI) * G.g, true, r, D,
-                                jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + 1, q, I), dsp, nvo);
+                applyFace<Diff, Disp>(f, qi, q, trans[I], G.area[0], (zI - zOf(I + 1, 0)) * G.g, tieOf(I + 1), r, D,
+                                jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + 1, q, I), dsp,
+                                nvo, thpOf(I));
             }
             if (j > 0) {
                 IqV q; fromLds(plane[ty][tx + 1], q);
-                applyFace<Diff, Disp>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zI) * G.g, false, r, D,
-                                jac + 9L * nbOff[2 * N + I], dif,
-                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - G.nx, q, N + I - G.nx), dsp, nvo);
+                applyFace<Diff, Disp>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zOf(I - G.nx, 0)) * G.g,
+                                tieOf(I - G.nx), r, D, jac + 9L * nbOff[2 * N + I], dif,
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - G.nx, q, N + I - G.nx), dsp, nvo,
+                                thpOf(N + I - G.nx));
             }
             if (j + 1 < G.ny) {
                 IqV q; fromLds(plane[ty + 2][tx + 1], q);
-                applyFace<Diff, Disp>(f, qi, q, trans[N + I], G.area[1], (zI - zI) * G.g, true, r, D,
-                                jac + 9L * nbOff[3 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo);
+                applyFace<Diff, Disp>(f, qi, q, trans[N + I], G.area[1], (zI - zOf(I + G.nx, 0)) * G.g, tieOf(I + G.nx),
+                                r, D, jac + 9L * nbOff[3 * N + I], dif,
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo, thpOf(N + I));
             }
             if (haveBelow) {
-                const double zJ = G.dz * ((double)(k - 1) + 0.5);
-                applyFace<Diff, Disp>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zJ) * G.g, false, r, D,
-                                jac + 9L * nbOff[4 * N + I], dif,
-                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - nxy, below, 2 * N + I - nxy), dsp, nvo);
+                applyFace<Diff, Disp>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zOf(I - nxy, -1)) * G.g,
+                                tieOf(I - nxy), r, D, jac + 9L * nbOff[4 * N + I], dif,
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - nxy, below, 2 * N + I - nxy), dsp, nvo,
+                                thpOf(2 * N + I - nxy));
             }
             if (k + 1 < G.nz) {
-                const double zJ = G.dz * ((double)(k + 1) + 0.5);
-                applyFace<Diff, Disp>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zJ) * G.g, true, r, D,
-                                jac + 9L * nbOff[5 * N + I], dif,
-                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + nxy, above, 2 * N + I), dsp, nvo);
+                applyFace<Diff, Disp>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zOf(I + nxy, 1)) * G.g,
+                                tieOf(I + nxy), r, D, jac + 9L * nbOff[5 * N + I], dif,
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + nxy, above, 2 * N + I), dsp, nvo,
+                                thpOf(2 * N + I));
             }
             // accumulation term (tpfalinearizer.hh:776-825)
             {
                 E3 st[3];
                 storageEval(f, qi, poro[I], st);
+                const double sf = Geo ? geo.vol[I] / geo.dt : storefac;
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
                     double rv = st[e].v;
                     rv -= st1[3 * I + e];
-                    rv *= storefac;
+                    rv *= sf;
                     r[e] += rv;
-                    D[3 * e + 0] += st[e].d0 * storefac;
-                    D[3 * e + 1] += st[e].d1 * storefac;
-                    D[3 * e + 2] += st[e].d2 * storefac;
+                    D[3 * e + 0] += st[e].d0 * sf;
+                    D[3 * e + 1] += st[e].d1 * sf;
+                    D[3 * e + 2] += st[e].d2 * sf;
                 }
             }
 #pragma unroll
@@ -239,42 +265,67 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
 // The bound of two waves holds the variants with the diffusive and dispersive
 // terms there too, at the price of a few spilled registers (faster than one
 // wave per SIMD without spills).
-template <bool Diff, bool Disp>
+template <bool Diff, bool Disp, bool Geo>
 __global__ __launch_bounds__(kTX * kTY, 2) void k_bo_assemble(const BoFluid* __restrict__ fp, BoGrid G,
     const double* __restrict__ pv, const uint8_t* __restrict__ mean,
     const double* __restrict__ poro, const double* __restrict__ trans,
     const double* __restrict__ st1, double storefac, double* __restrict__ res,
     double* __restrict__ jac, const int* __restrict__ diagOff, const int* __restrict__ nbOff, int kChunk,
-    BoDiff dif, BoDisp dsp)
+    BoDiff dif, BoDisp dsp, BoGeo geo)
 {
     __shared__ IqLds plane[kTY + 2][kTX + 2];
-    assembleTile<Diff, Disp>(*fp, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk,
-                             blockIdx.x, blockIdx.y, blockIdx.z, plane, dif, dsp);
+    assembleTile<Diff, Disp, Geo>(*fp, G, pv, mean, poro, trans, st1, storefac, res, jac, diagOff, nbOff, kChunk,
+                                  blockIdx.x, blockIdx.y, blockIdx.z, plane, dif, dsp, geo);
+}
+
+// The source term of one cell: problem.source(globI) is one rate per cell,
+// the sum of the cell's sources in the list's order, then times -V
+// (tpfalinearizer.hh:827-835).  grp = {first, count} into order.
+__device__ __forceinline__ void cellSourceTerm(const BoFluid& f, const IqE& qi, int2 grp, const int* __restrict__ order,
+                                               const int* __restrict__ type, const double* __restrict__ a,
+                                               const double* __restrict__ b, double V, E3 t[3])
+{
+    const int s0 = order[grp.x];
+    E3 q[3];
+    sourceEval(f, qi, type[s0], a[s0], b[s0], V, q);
+    for (int k = 1; k < grp.y; ++k) {
+        const int s = order[grp.x + k];
+        E3 q2[3];
+        sourceEval(f, qi, type[s], a[s], b[s], V, q2);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) q[e] = q[e] + q2[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) t[e] = q[e] * (-V);
 }
 
 // Cell-wise well sources (tpfalinearizer.hh:827-841), added after the flux
-// and accumulation terms exactly as the reference's per-cell order does.
+// and accumulation terms exactly as the reference's per-cell order does; one
+// thread per cell with sources.
 __global__ void k_bo_sources(BoFluid f, BoGrid G, const double* __restrict__ pv,
-                             const uint8_t* __restrict__ mean, int nsrc, const int* __restrict__ cell,
+                             const uint8_t* __restrict__ mean, int ngroups, const int2* __restrict__ groups,
+                             const int* __restrict__ order, const int* __restrict__ cell,
                              const int* __restrict__ type, const double* __restrict__ a,
                              const double* __restrict__ b, double* __restrict__ res,
-                             double* __restrict__ jac, const int* __restrict__ diagOff)
+                             double* __restrict__ jac, const int* __restrict__ diagOff,
+                             const double* __restrict__ vol)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsrc) return;
-    const long I = cell[s];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const int2 grp = groups[g];
+    const long I = cell[order[grp.x]];
+    const double V = vol ? vol[I] : G.V;
     IqE qi;
     iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
-    E3 q[3];
-    sourceEval(f, qi, type[s], a[s], b[s], G.V, q);
+    E3 t[3];
+    cellSourceTerm(f, qi, grp, order, type, a, b, V, t);
     double* D = jac + 9L * diagOff[I];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        const E3 t = q[e] * (-G.V);
-        res[3 * I + e] += t.v;
-        D[3 * e + 0] += t.d0;
-        D[3 * e + 1] += t.d1;
-        D[3 * e + 2] += t.d2;
+        res[3 * I + e] += t[e].v;
+        D[3 * e + 0] += t[e].d0;
+        D[3 * e + 1] += t[e].d1;
+        D[3 * e + 2] += t[e].d2;
     }
 }
 
@@ -287,18 +338,19 @@ __global__ void k_bo_source_blocks(BoFluid f, BoGrid G, const double* __restrict
                                    const uint8_t* __restrict__ mean, int nsrc, const int* __restrict__ cell,
                                    const int* __restrict__ type, const double* __restrict__ a,
                                    const double* __restrict__ b, double* __restrict__ outRes,
-                                   double* __restrict__ outDiag)
+                                   double* __restrict__ outDiag, const double* __restrict__ vol)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsrc) return;
     const long I = cell[s];
+    const double V = vol ? vol[I] : G.V;
     IqE qi;
     iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
     E3 q[3];
-    sourceEval(f, qi, type[s], a[s], b[s], G.V, q);
+    sourceEval(f, qi, type[s], a[s], b[s], V, q);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        const E3 t = q[e] * (-G.V);
+        const E3 t = q[e] * (-V);
         outRes[3 * s + e] = t.v;
         outDiag[9 * s + 3 * e + 0] = t.d0;
         outDiag[9 * s + 3 * e + 1] = t.d1;
@@ -332,7 +384,8 @@ __global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const 
 __global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const double* __restrict__ pv,
                            const uint8_t* __restrict__ mean,
                            const double* __restrict__ trans, const int* __restrict__ rowPtr,
-                           double* __restrict__ flows, double* __restrict__ flores, BoDiff dif, BoDisp dsp)
+                           double* __restrict__ flows, double* __restrict__ flores, BoDiff dif, BoDisp dsp,
+                           BoGeo geo)
 {
     const BoFluid& f = *fp;
     const long nxy = (long)G.nx * G.ny;
@@ -342,25 +395,33 @@ __global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const doubl
     const int i = (int)(I % G.nx), j = (int)((I / G.nx) % G.ny), k = (int)(I / nxy);
     IqE qi;
     iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
-    const double zI = G.dz * ((double)k + 0.5);
+    const double zI = geo.depth ? geo.depth[I] : G.dz * ((double)k + 0.5);
     long e = rowPtr[I] - I;     // first neighbour entry of I
     auto face = [&](long J, long fi, double area, double zJ, bool upInTie) {
         IqV qj;
         iqValues(f, pv[3 * J], pv[3 * J + 1], pv[3 * J + 2], mean[J], qj);
         E3 fl[3];
         double darcy[3];
+        double thp = 0.0;
+        if (geo.depth) {
+            zJ = geo.depth[J];
+            const double vi = geo.vol[I], vj = geo.vol[J];
+            upInTie = vi > vj || (vi == vj && I < J);
+            thp = geo.thp[fi];
+        }
         const double dZg = (zI - zJ) * G.g;
         if (dif.diff && dsp.disp)
             faceFluxT<true, true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp);
+                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp);
         else if (dif.diff)
             faceFluxT<true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi));
+                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi), nullptr, thp);
         else if (dsp.disp)
             faceFluxT<true, false, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp);
+                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp);
         else
-            faceFluxT<true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy);
+            faceFluxT<true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, nullptr,
+                            DiffFace{0.0, 0.0, 0.0, 0.0}, nullptr, thp);
         for (int c = 0; c < 3; ++c) {
             if (flows) flows[3 * e + c] = fl[c].v;
             if (flores) flores[3 * e + c] = darcy[c];
@@ -506,7 +567,7 @@ __global__ void k_bo_update(BoFluid f, long N, double* __restrict__ pv, uint8_t*
 void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
-                      const BoDiff* dif, const BoDisp* dsp)
+                      const BoDiff* dif, const BoDisp* dsp, const BoGeo* geo)
 {
     const int gx = (G.nx + kTX - 1) / kTX, gy = (G.ny + kTY - 1) / kTY;
     // enough workgroups to fill the chip: split the z march into chunks
@@ -516,47 +577,53 @@ void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, co
     gz = (G.nz + kChunk - 1) / kChunk;
     const BoDiff d = (dif && dif->diff) ? *dif : BoDiff{};
     const BoDisp e = (dsp && dsp->disp) ? *dsp : BoDisp{};
+    const BoGeo g = (geo && geo->depth) ? *geo : BoGeo{};
     const dim3 grid(gx, gy, gz), block(kTX * kTY);
-    if (d.diff && e.disp)
-        hipLaunchKernelGGL((k_bo_assemble<true, true>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
-                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
-    else if (d.diff)
-        hipLaunchKernelGGL((k_bo_assemble<true, false>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
-                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
-    else if (e.disp)
-        hipLaunchKernelGGL((k_bo_assemble<false, true>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
-                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
-    else
-        hipLaunchKernelGGL((k_bo_assemble<false, false>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1,
-                           storefac, res, jac, diagOff, nbOff, kChunk, d, e);
+#define OM_ASSEMBLE(DIFF, DISP, GEO)                                                                          \
+    hipLaunchKernelGGL((k_bo_assemble<DIFF, DISP, GEO>), grid, block, 0, s, fdev, G, pv, mean, poro, trans, st1, \
+                       storefac, res, jac, diagOff, nbOff, kChunk, d, e, g)
+    if (g.depth) {
+        if (d.diff && e.disp) OM_ASSEMBLE(true, true, true);
+        else if (d.diff) OM_ASSEMBLE(true, false, true);
+        else if (e.disp) OM_ASSEMBLE(false, true, true);
+        else OM_ASSEMBLE(false, false, true);
+    } else {
+        if (d.diff && e.disp) OM_ASSEMBLE(true, true, false);
+        else if (d.diff) OM_ASSEMBLE(true, false, false);
+        else if (e.disp) OM_ASSEMBLE(false, true, false);
+        else OM_ASSEMBLE(false, false, false);
+    }
+#undef OM_ASSEMBLE
 }
 
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
-                     int nsrc, const int* cell, const int* type, const double* a, const double* b,
-                     double* res, double* jac, const int* diagOff, hipStream_t s)
+                     int ngroups, const int2* groups, const int* order, const int* cell, const int* type,
+                     const double* a, const double* b, double* res, double* jac, const int* diagOff, hipStream_t s,
+                     const double* vol)
 {
-    if (nsrc <= 0) return;
-    hipLaunchKernelGGL(k_bo_sources, dim3((nsrc + 63) / 64), dim3(64), 0, s, f, G, pv, mean, nsrc,
-                       cell, type, a, b, res, jac, diagOff);
+    if (ngroups <= 0) return;
+    hipLaunchKernelGGL(k_bo_sources, dim3((ngroups + 63) / 64), dim3(64), 0, s, f, G, pv, mean, ngroups, groups,
+                       order, cell, type, a, b, res, jac, diagOff, vol);
 }
 
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
-                          double* outRes, double* outDiag, hipStream_t s)
+                          double* outRes, double* outDiag, hipStream_t s, const double* vol)
 {
     if (nsrc <= 0) return;
     hipLaunchKernelGGL(k_bo_source_blocks, dim3((nsrc + 63) / 64), dim3(64), 0, s, f, G, pv, mean, nsrc,
-                       cell, type, a, b, outRes, outDiag);
+                       cell, type, a, b, outRes, outDiag, vol);
 }
 
 void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif,
-                   const BoDisp* dsp)
+                   const BoDisp* dsp, const BoGeo* geo)
 {
     const long N = (long)G.nx * G.ny * G.nz;
     if (N <= 0) return;
     hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, fdev, G, pv, mean, trans, rowPtr,
-                       flows, flores, dif ? *dif : BoDiff{}, dsp ? *dsp : BoDisp{});
+                       flows, flores, dif ? *dif : BoDiff{}, dsp ? *dsp : BoDisp{},
+                       (geo && geo->depth) ? *geo : BoGeo{});
 }
 
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
@@ -598,10 +665,12 @@ __global__ __launch_bounds__(kTX * kTY, 2) void k_e_assemble(const BoFluid* __re
     if (!active[t.x]) return;
     const EnsInst& I = insts[t.x];
     const int bx = t.y % I.gx, by = (t.y / I.gx) % I.gy, bz = t.y / (I.gx * I.gy);
-    assembleTile<false, false>(*fp, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac,
-                               I.diagOff, I.nbOff, I.kChunk, bx, by, bz, plane, BoDiff{}, BoDisp{});
+    assembleTile<false, false, false>(*fp, I.G, I.pv0, I.mean0, I.poro, I.trans, I.st1, I.storefac, I.res, I.jac,
+                                      I.diagOff, I.nbOff, I.kChunk, bx, by, bz, plane, BoDiff{}, BoDisp{}, BoGeo{});
 }
 
+// the cell-wise source pass of every instance: table entries (instance,
+// group), one thread per cell with sources
 __global__ void k_e_sources(BoFluid f, const EnsInst* __restrict__ insts, const int2* __restrict__ table, int n,
                             const int* __restrict__ active)
 {
@@ -610,23 +679,22 @@ __global__ void k_e_sources(BoFluid f, const EnsInst* __restrict__ insts, const 
     const int2 t = table[q];
     if (!active[t.x]) return;
     const EnsInst& I = insts[t.x];
-    const long c = I.srcCell[t.y];
+    const int2 grp = I.srcGroups[t.y];
+    const long c = I.srcCell[I.srcOrder[grp.x]];
     IqE qi;
     iqEval(f, I.pv0[3 * c], I.pv0[3 * c + 1], I.pv0[3 * c + 2], I.mean0[c], qi);
-    E3 s[3];
-    sourceEval(f, qi, I.srcType[t.y], I.srcA[t.y], I.srcB[t.y], I.G.V, s);
+    E3 v[3];
+    cellSourceTerm(f, qi, grp, I.srcOrder, I.srcType, I.srcA, I.srcB, I.G.V, v);
     double* D = I.jac + 9L * I.diagOff[c];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        const E3 v = s[e] * (-I.G.V);
-        I.res[3 * c + e] += v.v;
-        D[3 * e + 0] += v.d0;
-        D[3 * e + 1] += v.d1;
-        D[3 * e + 2] += v.d2;
+        I.res[3 * c + e] += v[e].v;
+        D[3 * e + 0] += v[e].d0;
+        D[3 * e + 1] += v[e].d1;
+        D[3 * e + 2] += v[e].d2;
     }
 }
 
-// storage cache of time level 1 for the instances starting a Newton solve
 __global__ __launch_bounds__(kEnsChunk) void k_e_storage(BoFluid f, const EnsInst* __restrict__ insts,
     const int2* __restrict__ table, const int* __restrict__ refresh)
 {

@@ -258,7 +258,7 @@ template <bool WithDarcy, bool WithDiff = false, bool WithDisp = false>
 __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
                                           double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3],
                                           const BoDiff* dif = nullptr, DiffFace df = DiffFace{0.0, 0.0, 0.0, 0.0},
-                                          const BoDisp* dsp = nullptr)
+                                          const BoDisp* dsp = nullptr, double thp = 0.0)
 {
     flux[0] = ec(0.0); flux[1] = ec(0.0); flux[2] = ec(0.0);
     if constexpr (WithDarcy) { darcyOut[0] = 0.0; darcyOut[1] = 0.0; darcyOut[2] = 0.0; }
@@ -268,7 +268,14 @@ __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const
         if (in.lam[a].v <= 0.0 && ex.lam[a] <= 0.0) continue;
         const E3 rhoAvg = (in.rho[a] + ex.rho[a]) * 0.5;
         const E3 pEx = rhoAvg * dZg + ex.p;
-        const E3 dp = pEx - in.p;
+        E3 dp = pEx - in.p;
+        // threshold pressure of the face, in the form of the reference's
+        // solvent flux (blackoilsolventmodules.hh:1484-1492): below it the
+        // face carries no flow
+        if (thp > 0.0) {
+            if (fabs(dp.v) > thp) dp = dp.v < 0.0 ? dp + thp : dp - thp;
+            else dp = ec(0.0);
+        }
         const bool upIn = dp.v > 0.0 ? false : (dp.v < 0.0 ? true : upInTie);
         E3 svf;
         E3 gas;

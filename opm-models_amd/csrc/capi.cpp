@@ -211,13 +211,24 @@ int omodels_linearizer_pattern(omodels_model* h, int32_t* row_ptr, int32_t* col_
                 if (nb_trans) nb_trans[k] = T.trans[(long)ax * N + face];
                 if (nb_dzg) {
                     const long kj = J / nxy;
-                    const double zI = m.G.dz * ((double)kk + 0.5), zJ = m.G.dz * ((double)kj + 0.5);
+                    const bool own = m.depthHost.empty();
+                    const double zI = own ? m.G.dz * ((double)kk + 0.5) : m.depthHost[I];
+                    const double zJ = own ? m.G.dz * ((double)kj + 0.5) : m.depthHost[J];
                     nb_dzg[k] = (zI - zJ) * m.G.g;
                 }
                 ++k;
             }
             if (nb_ptr) nb_ptr[I + 1] = (int32_t)k;
         }
+        return 0;
+    });
+}
+
+int omodels_linearizer_update_discretization(omodels_model* h, const double* trans, const double* depth,
+                                             const double* volume, const double* thpres)
+{
+    return guarded([&] {
+        h->m->updateDiscretization(trans, depth, volume, thpres);
         return 0;
     });
 }

@@ -124,6 +124,13 @@ int Ensemble::add(const omodels_blackoil_problem* p, const double* pv, const uin
         M.srcType.upload(p->src_type, M.nsrc, s);
         M.srcA.upload(p->src_a, M.nsrc, s);
         M.srcB.upload(p->src_b, M.nsrc, s);
+        std::vector<int2> groups;
+        std::vector<int> order;
+        sourceGroups(p->src_cell, M.nsrc, groups, order);
+        M.nSrcGroups = (int)groups.size();
+        M.srcGroups.upload(groups.data(), groups.size(), s);
+        M.srcOrder.upload(order.data(), order.size(), s);
+        OM_HIP(hipStreamSynchronize(s));
     }
     M.pv0.upload(pv, 3 * N, s);
     M.mean0.upload(mean, N, s);
@@ -195,6 +202,7 @@ void Ensemble::rebuild()
         I.pv0 = M.pv0.p; I.mean0 = M.mean0.p; I.wasSw = M.wasSw.p; I.st1 = M.st1.p; I.res = M.res.p;
         I.jac = M.jac.p; I.rowPtr = M.rowPtr.p; I.colIdx = M.colIdx.p; I.diagOff = M.diagOff.p; I.nbOff = M.nbOff.p;
         I.srcCell = M.srcCell.p; I.srcType = M.srcType.p; I.srcA = M.srcA.p; I.srcB = M.srcB.p;
+        I.srcGroups = M.srcGroups.p; I.srcOrder = M.srcOrder.p;
         I.lu64 = M.lu64.p; I.dinv64 = M.dinv64.p; I.Af = M.Af.p; I.luf = M.luf.p;
         for (int v = 0; v < kNumVec; ++v) { I.dv[v] = M.dv[v].p; I.fv[v] = M.fv[v].p; }
         I.dv[kVb] = M.res.p;            // the right-hand side is the residual itself
@@ -203,7 +211,7 @@ void Ensemble::rebuild()
         const int nc = (int)((M.N + kEnsChunk - 1) / kEnsChunk);
         for (int k = 0; k < nc; ++k) chunkT.push_back(make_int2(b, k));
         first[b + 1] = first[b] + nc;
-        for (int k = 0; k < M.nsrc; ++k) srcT.push_back(make_int2(b, k));
+        for (int k = 0; k < M.nSrcGroups; ++k) srcT.push_back(make_int2(b, k));
         geoms[b] = M.tg;
     }
     desc_.upload(d.data(), d.size(), stream_);

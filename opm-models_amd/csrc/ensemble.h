@@ -38,6 +38,8 @@ struct EnsInst {
     const int* srcType;
     const double* srcA;
     const double* srcB;
+    const int2* srcGroups;           // sources grouped by cell: {first, count} into srcOrder
+    const int* srcOrder;
     double* lu64;
     double* dinv64;
     float* Af;

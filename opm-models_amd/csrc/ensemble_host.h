@@ -25,7 +25,9 @@ public:
         int gx = 0, gy = 0, gz = 0, kChunk = 1, nsrc = 0;
         DevBuf<double> trans, poro, pv0, pv1, st1, res, jac, lu64, dinv64, srcA, srcB;
         DevBuf<uint8_t> mean0, mean1, wasSw;
-        DevBuf<int> rowPtr, colIdx, diagOff, nbOff, srcCell, srcType, counters;
+        DevBuf<int> rowPtr, colIdx, diagOff, nbOff, srcCell, srcType, counters, srcOrder;
+        DevBuf<int2> srcGroups;
+        int nSrcGroups = 0;
         DevBuf<float> Af, luf;
         DevBuf<double> dv[kNumVec];
         DevBuf<float> fv[kNumVec];

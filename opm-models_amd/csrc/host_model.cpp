@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 
 #include "omodels_host.h"
 
@@ -16,6 +17,29 @@ void hipCheck(hipError_t e, const char* what)
 {
     if (e != hipSuccess)
         throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+void sourceGroups(const int32_t* cell, int n, std::vector<int2>& groups, std::vector<int>& order)
+{
+    groups.clear();
+    order.clear();
+    std::vector<int> next(n, -1), firstOf;
+    std::map<int32_t, std::pair<int, int>> cells;    // cell -> (first, last) source
+    for (int k = 0; k < n; ++k) {
+        auto it = cells.find(cell[k]);
+        if (it == cells.end()) {
+            cells.emplace(cell[k], std::make_pair(k, k));
+            firstOf.push_back(k);
+        } else {
+            next[it->second.second] = k;
+            it->second.second = k;
+        }
+    }
+    for (int f : firstOf) {
+        const int start = (int)order.size();
+        for (int k = f; k >= 0; k = next[k]) order.push_back(k);
+        groups.push_back(make_int2(start, (int)order.size() - start));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -36,6 +60,7 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
     G.area[0] = p->dy * p->dz; G.area[1] = p->dx * p->dz; G.area[2] = p->dx * p->dy;
 
     trans.upload(topo.trans.data(), topo.trans.size(), stream);
+    transGrid = topo.trans;
     fluidDev.upload(&fluid, 1, stream);
     poro.upload(p->poro, N, stream);
     rowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
@@ -53,6 +78,13 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
         srcType.upload(p->src_type, nsrc, stream);
         srcA.upload(p->src_a, nsrc, stream);
         srcB.upload(p->src_b, nsrc, stream);
+        std::vector<int2> groups;
+        std::vector<int> order;
+        sourceGroups(p->src_cell, nsrc, groups, order);
+        nSrcGroups = (int)groups.size();
+        srcGroups.upload(groups.data(), groups.size(), stream);
+        srcOrder.upload(order.data(), order.size(), stream);
+        sync();
     }
     pv0.alloc(3 * N); pv1.alloc(3 * N); st1.alloc(3 * N); res.alloc(3 * N);
     mean0.alloc(N); mean1.alloc(N); wasSw.alloc(N);
@@ -97,17 +129,18 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     if (iterationIndex == 0)
         launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream);
     const double storefac = G.V / dt;
+    geo.dt = dt;
     timer.begin(kClsAssemble, stream);
     launchBoAssemble(fluidDev.p, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, storefac, res.p, jac.p, diagOff.p,
-                     nbOff.p, stream, &diffusion, &dispersion);
+                     nbOff.p, stream, &diffusion, &dispersion, &geo);
     timer.end(stream);
     // the velocities just written are the next linearization's input
     if (dispersion.disp) std::swap(dispersion.nvIn, dispersion.nvOut);
     // cell-wise sources (:827-841), or the same terms as sparse blocks added
     // after the cell loop (:842-846); both add each cell's term last
     if (!separateSparseSources)
-        launchBoSources(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, res.p, jac.p,
-                        diagOff.p, stream);
+        launchBoSources(fluid, G, pv0.p, mean0.p, nSrcGroups, srcGroups.p, srcOrder.p, srcCell.p, srcType.p, srcA.p,
+                        srcB.p, res.p, jac.p, diagOff.p, stream, geo.vol);
     else
         addBuiltinSparseSources();
     OM_HIP(hipGetLastError());
@@ -118,7 +151,7 @@ void Model::computeSourceBlocks()
     srcRes.alloc(3 * (size_t)std::max(nsrc, 1));
     srcDiag.alloc(9 * (size_t)std::max(nsrc, 1));
     launchBoSourceBlocks(fluid, G, pv0.p, mean0.p, nsrc, srcCell.p, srcType.p, srcA.p, srcB.p, srcRes.p,
-                         srcDiag.p, stream);
+                         srcDiag.p, stream, geo.vol);
     OM_HIP(hipGetLastError());
 }
 
@@ -132,7 +165,8 @@ void Model::addSparseSources(int n, const int* cells, const double* r, const dou
 void Model::computeFlows(double* flows, double* flores)
 {
     timer.begin(kClsOther, stream);
-    launchBoFlows(fluidDev.p, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion, &dispersion);
+    launchBoFlows(fluidDev.p, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion, &dispersion,
+                  &geo);
     timer.end(stream);
     OM_HIP(hipGetLastError());
 }
@@ -148,6 +182,52 @@ void checkNonNegative(const double* v, long n, const char* what)
         if (!(v[i] >= 0.0) || !std::isfinite(v[i])) throw std::invalid_argument(what);
 }
 }  // namespace
+
+void Model::updateDiscretization(const double* faceTrans, const double* depth, const double* volume,
+                                 const double* thpres)
+{
+    if (faceTrans) checkNonNegative(faceTrans, 3 * N, "transmissibilities must be finite and >= 0");
+    if (thpres) checkNonNegative(thpres, 3 * N, "threshold pressures must be finite and >= 0");
+    if (depth)
+        for (long I = 0; I < N; ++I)
+            if (!std::isfinite(depth[I])) throw std::invalid_argument("depths must be finite");
+    if (volume)
+        for (long I = 0; I < N; ++I)
+            if (!(volume[I] > 0.0) || !std::isfinite(volume[I]))
+                throw std::invalid_argument("cell volumes must be finite and > 0");
+    // faces without a neighbour keep 0, as the grid's own table has them
+    const long nxy = (long)G.nx * G.ny;
+    std::vector<double> t = transGrid;
+    if (faceTrans)
+        for (long I = 0; I < N; ++I) {
+            const long ijk[3] = {I % G.nx, (I / G.nx) % G.ny, I / nxy};
+            const int n[3] = {G.nx, G.ny, G.nz};
+            for (int ax = 0; ax < 3; ++ax)
+                if (ijk[ax] + 1 < n[ax]) t[(size_t)ax * N + I] = faceTrans[(size_t)ax * N + I];
+        }
+    sync();
+    trans.upload(t.data(), t.size(), stream);
+    topo.trans = t;
+    if (!depth && !volume && !thpres) {
+        geo = BoGeo{};
+        depthHost.clear();
+        geoDepth.release(); geoVol.release(); geoThp.release();
+        sync();
+        return;
+    }
+    std::vector<double> z(N), v(N), th(3 * (size_t)N, 0.0);
+    for (long I = 0; I < N; ++I) {
+        z[I] = depth ? depth[I] : G.dz * ((double)(I / nxy) + 0.5);
+        v[I] = volume ? volume[I] : G.V;
+    }
+    if (thpres) std::copy(thpres, thpres + 3 * N, th.begin());
+    geoDepth.upload(z.data(), z.size(), stream);
+    geoVol.upload(v.data(), v.size(), stream);
+    geoThp.upload(th.data(), th.size(), stream);
+    depthHost = z;
+    geo.depth = geoDepth.p; geo.vol = geoVol.p; geo.thp = geoThp.p;
+    sync();
+}
 
 void Model::setDiffusion(const omodels_diffusion* d)
 {

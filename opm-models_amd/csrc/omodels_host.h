@@ -31,6 +31,10 @@ void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::v
               std::vector<double>& pv, std::vector<uint8_t>& mean, std::vector<int32_t>& srcCell,
               std::vector<int32_t>& srcType, std::vector<double>& srcA, std::vector<double>& srcB);
 void buildTopology(const omodels_blackoil_problem* p, HostTopology& T);
+// The problem's sources grouped by cell, for the cell-wise source pass:
+// groups[g] = {first, count} into order (source indices in the list's order),
+// one group per cell with sources, in the order of each cell's first source.
+void sourceGroups(const int32_t* cell, int n, std::vector<int2>& groups, std::vector<int>& order);
 
 // ---------------------------------------------------------------------------
 // The linearized system as the linear-solver backend sees it (the
@@ -79,6 +83,9 @@ public:
     DevBuf<int> srcCell, srcType;
     DevBuf<double> srcA, srcB;
     int nsrc = 0;
+    DevBuf<int2> srcGroups;      // sources grouped by cell (sourceGroups)
+    DevBuf<int> srcOrder;
+    int nSrcGroups = 0;
     DevBuf<int> counters;        // [0] switched, [1] non-finite update
 
     // TpfaLinearizer
@@ -110,6 +117,18 @@ public:
     void setDispersion(const omodels_dispersion* d);
     DevBuf<double> faceDisp, nvA, nvB;
     BoDisp dispersion{};         // dispersion.disp == nullptr: off
+    // updateStoredTransmissibilities (tpfalinearizer.hh:868-883) and the other
+    // per-neighbour inputs of createMatrix_ (:443-452) from the caller:
+    // transmissibilities [3][N] (face I|I+e_axis at axis*N + I), depths [N],
+    // volumes [N], threshold pressures [3][N]; each nullptr for the grid's own.
+    // With depths, volumes or threshold pressures the assembly reads them
+    // (BoGeo); without, it is the plain form of the structured grid.
+    void updateDiscretization(const double* faceTrans, const double* depth, const double* volume,
+                              const double* thpres);
+    std::vector<double> transGrid;       // the grid's own transmissibilities
+    std::vector<double> depthHost;       // the caller's depths (empty: the grid's own)
+    DevBuf<double> geoDepth, geoVol, geoThp;
+    BoGeo geo{};                         // geo.depth == nullptr: the grid's own
     void ensureJacobian();
     void eraseMatrix() { jac.release(); }
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)

@@ -28,6 +28,19 @@ struct BoGrid {
     double area[3];       // face areas of x, y and z faces
 };
 
+// The linearizer's per-neighbour inputs given by the caller instead of the
+// structured grid's own (tpfalinearizer.hh:443-452): each cell's centre depth
+// (dofCenterDepth) and total volume (dofTotalVolume), the threshold pressure
+// per face ([3][N] at the transmissibility's index, zeros where none), and the
+// time step of the linearization (storefac = volume / dt, :776-778).
+// depth == nullptr: the grid's own (the kernels' plain form).
+struct BoGeo {
+    const double* depth;
+    const double* vol;
+    const double* thp;
+    double dt;
+};
+
 struct NewtonUpdateParams {
     double dpMaxRel, dsMax, oscThresh;
 };
@@ -141,18 +154,21 @@ struct KernelTimer {
 void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean,
                       const double* poro, const double* trans, const double* st1, double storefac,
                       double* res, double* jac, const int* diagOff, const int* nbOff, hipStream_t s,
-                      const BoDiff* dif = nullptr, const BoDisp* dsp = nullptr);
+                      const BoDiff* dif = nullptr, const BoDisp* dsp = nullptr, const BoGeo* geo = nullptr);
+// the cell-wise source pass over the sources grouped by cell (groups[g] =
+// {first, count} into order); vol: nullptr (every cell G.V) or the cells' volumes
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
-                     int nsrc, const int* cell, const int* type, const double* a, const double* b,
-                     double* res, double* jac, const int* diagOff, hipStream_t s);
+                     int ngroups, const int2* groups, const int* order, const int* cell, const int* type,
+                     const double* a, const double* b, double* res, double* jac, const int* diagOff, hipStream_t s,
+                     const double* vol = nullptr);
 void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
                      const double* poro, double* st, hipStream_t s);
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
-                          double* outRes, double* outDiag, hipStream_t s);
+                          double* outRes, double* outDiag, hipStream_t s, const double* vol = nullptr);
 void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif = nullptr,
-                   const BoDisp* dsp = nullptr);
+                   const BoDisp* dsp = nullptr, const BoGeo* geo = nullptr);
 constexpr int kOutFields = 18;
 void launchBoOutput(const BoFluid* fdev, long N, const double* pv, const uint8_t* mean, const double* poro,
                     double* out, hipStream_t s);

@@ -110,6 +110,7 @@ SIGNATURES = {
     "omodels_model_get_storage1": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_pattern": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]),
     "omodels_linearizer_linearize_domain": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "omodels_linearizer_update_discretization": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     "omodels_linearizer_get_jacobian": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_get_residual": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_erase_matrix": (C.c_int, [C.c_void_p]),
@@ -401,6 +402,24 @@ class Model:
                                                ptr(nbc, _ip), ptr(nbo, _ip), ptr(nbt), ptr(nbz)), "pattern")
         return dict(row_ptr=rp, col_idx=ci, diag_off=dg, nb_ptr=nbp, nb_cell=nbc, nb_off=nbo, nb_trans=nbt,
                     nb_dzg=nbz)
+
+    def update_discretization(self, trans=None, depth=None, volume=None, thpres=None):
+        """updateStoredTransmissibilities and the other per-neighbour inputs:
+        trans / thpres [3][N] at the face's index (axis*N + I for face
+        I|I+e_axis), depth / volume [N]; None for the grid's own."""
+        N = self.N
+        arrs = []
+        for a, n in ((trans, 3 * N), (depth, N), (volume, N), (thpres, 3 * N)):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            if a.size != n:
+                raise ValueError("update_discretization: array of %d values, expected %d" % (a.size, n))
+            arrs.append(a)
+        self._geo = arrs        # the library copies them; kept for the caller's inspection
+        check(lib().omodels_linearizer_update_discretization(
+            self.h, *[ptr(a) if a is not None else None for a in arrs]), "update_discretization")
 
     def linearize(self, dt, iteration_index=1):
         check(lib().omodels_linearizer_linearize_domain(self.h, dt, iteration_index), "linearize_domain")

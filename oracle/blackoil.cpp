@@ -171,8 +171,19 @@ struct Grid {
     explicit Grid(const ob_problem* p)
         : nx(p->nx), ny(p->ny), nz(p->nz), N((long)p->nx * p->ny * p->nz),
           dx(p->dx), dy(p->dy), dz(p->dz), g(p->gravity), P(p) {}
-    double volume() const { return dx * dy * dz; }
-    double zc(long I) const { long k = I / ((long)nx * ny); return dz * (k + 0.5); }
+    /* dofTotalVolume and dofCenterDepth: the caller's, or the structured grid's */
+    double volume(long I) const { return P->volume ? P->volume[I] : dx * dy * dz; }
+    double zc(long I) const
+    {
+        if (P->depth) return P->depth[I];
+        long k = I / ((long)nx * ny);
+        return dz * (k + 0.5);
+    }
+    /* thresholdPressure(I, J), 0 without the caller's */
+    double thpres(long I, long J, int dir) const
+    {
+        return P->thpres ? P->thpres[(long)(dir / 2) * N + (I < J ? I : J)] : 0.0;
+    }
     /* neighbour of I across intersection `dir` (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z), -1 if boundary */
     long nb(long I, int dir) const
     {
@@ -196,6 +207,7 @@ struct Grid {
     double trans(long I, long J, int dir) const
     {
         const int ax = dir / 2;
+        if (P->face_trans) return P->face_trans[(long)ax * N + (I < J ? I : J)];
         const double A = area(dir), ndot = 0.5 * h(dir), dist2 = ndot * ndot;
         const double K0 = P->perm[(long)ax * N + I], K1 = P->perm[(long)ax * N + J];
         const double T0 = A * K0 * ndot / dist2;
@@ -204,7 +216,7 @@ struct Grid {
     }
 };
 
-struct NbInfo { long J; int off; double trans, area, dZg; int dir; double diffusivity, dispersivity; };
+struct NbInfo { long J; int off; double trans, area, dZg; int dir; double diffusivity, dispersivity, thpres; };
 
 struct Pattern {
     std::vector<int32_t> rowPtr, colIdx, diagOff, nbPtr;
@@ -228,6 +240,7 @@ void buildPattern(const Grid& G, Pattern& P)
             NbInfo n;
             n.J = J; n.off = -1; n.trans = G.trans(I, J, dir); n.area = G.area(dir);
             n.dZg = (G.zc(I) - G.zc(J)) * G.g; n.dir = dir; n.diffusivity = 0.0; n.dispersivity = 0.0;
+            n.thpres = G.thpres(I, J, dir);
             P.nb.push_back(n);
         }
         P.nbPtr[I + 1] = (int32_t)P.nb.size();
@@ -304,6 +317,13 @@ void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, lo
         E3 rhoAvg = (in.rho[a] + ex.rho[a]) * 0.5;
         E3 pEx = rhoAvg * nbi.dZg + ex.p[a];
         E3 dp = pEx - in.p[a];
+        /* threshold pressure: the form of the reference's solvent flux
+         * (blackoilsolventmodules.hh:1484-1492), applied only where a face
+         * has one; below it the face carries no flow */
+        if (nbi.thpres > 0.0) {
+            if (std::fabs(dp.v) > nbi.thpres) dp = dp.v < 0.0 ? dp + nbi.thpres : dp - nbi.thpres;
+            else dp = E3(0.0);
+        }
         bool upIn;
         if (dp.v > 0.0) upIn = false;
         else if (dp.v < 0.0) upIn = true;
@@ -389,7 +409,9 @@ struct Problem {
     Grid G;
     Pvt pvt;
     Pattern pat;
-    std::vector<int> srcOfCell;
+    /* the problem's source terms per cell: the first source of the cell and
+     * the next source of the same cell (-1: none), in the list's order */
+    std::vector<int> srcOfCell, srcNext;
     bool hasDiff = false;
     Diffusion dif{};
     bool hasDisp = false;
@@ -397,10 +419,27 @@ struct Problem {
     explicit Problem(const ob_problem* p, const ob_diffusion* d = nullptr, const ob_dispersion* e = nullptr)
         : P(p), G(p)
     {
+        if (p->face_trans) checkNonNegative(p->face_trans, 3 * G.N, "transmissibilities must be finite and >= 0");
+        if (p->thpres) checkNonNegative(p->thpres, 3 * G.N, "threshold pressures must be finite and >= 0");
+        if (p->depth)
+            for (long I = 0; I < G.N; ++I)
+                if (!std::isfinite(p->depth[I])) throw std::invalid_argument("depths must be finite");
+        if (p->volume)
+            for (long I = 0; I < G.N; ++I)
+                if (!(p->volume[I] > 0.0) || !std::isfinite(p->volume[I]))
+                    throw std::invalid_argument("cell volumes must be finite and > 0");
         pvt.init(p);
         buildPattern(G, pat);
         srcOfCell.assign(G.N, -1);
-        for (int s = 0; s < p->n_src; ++s) srcOfCell[p->src_cell[s]] = s;
+        srcNext.assign(p->n_src, -1);
+        std::vector<int> last(G.N, -1);
+        for (int s = 0; s < p->n_src; ++s) {
+            const long c = p->src_cell[s];
+            if (c < 0 || c >= G.N) throw std::invalid_argument("source cell out of range");
+            if (last[c] < 0) srcOfCell[c] = s;
+            else srcNext[last[c]] = s;
+            last[c] = s;
+        }
         if (d) setDiffusion(*d);
         if (e) setDispersion(*e);
     }
@@ -500,7 +539,6 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
     const long nnzb = pr.pat.rowPtr[N];
     std::fill(res, res + 3 * N, 0.0);
     std::fill(jac, jac + 9 * nnzb, 0.0);
-    const double V = pr.G.volume();
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for num_threads(nthreads) schedule(static)
@@ -510,6 +548,7 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
         computeIQ<E3>(pr.pvt, pv + 3 * I, mean[I], pr.P->poro[I], qi);
         double* r = res + 3 * I;
         double* D = jac + 9L * pr.pat.diagOff[I];
+        const double V = pr.G.volume(I);
         if (pr.hasDisp) nvOut[I] = 0.0;
         for (int k = pr.pat.nbPtr[I]; k < pr.pat.nbPtr[I + 1]; ++k) {
             const NbInfo& nbi = pr.pat.nb[k];
@@ -518,12 +557,12 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
             E3 f[3];
             if (pr.hasDisp) {
                 double darcy[3];
-                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, darcy, pr.diffusion(), &pr.dsp, nvIn[I],
-                         nvIn[nbi.J]);
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, darcy, pr.diffusion(), &pr.dsp,
+                         nvIn[I], nvIn[nbi.J]);
                 const double v = std::fabs(darcy[CO] / nbi.area);
                 nvOut[I] = nvOut[I] < v ? v : nvOut[I];
             } else {
-                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, nullptr, pr.diffusion());
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, nullptr, pr.diffusion());
             }
             double* O_ = jac + 9L * nbi.off;
             for (int e = 0; e < 3; ++e) {
@@ -548,10 +587,17 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
             }
         }
         /* cell-wise sources (:827-841) */
+        /* problem.source(globI) is the cell's one rate: the sum of its
+         * sources in the list's order, then times -V (:829-835) */
         const int s = pr.srcOfCell[I];
         if (s >= 0) {
             E3 q[3];
             sourceOf(pr.pvt, qi, pr.P->src_type[s], pr.P->src_a[s], pr.P->src_b[s], V, q);
+            for (int s2 = pr.srcNext[s]; s2 >= 0; s2 = pr.srcNext[s2]) {
+                E3 q2[3];
+                sourceOf(pr.pvt, qi, pr.P->src_type[s2], pr.P->src_a[s2], pr.P->src_b[s2], V, q2);
+                for (int e = 0; e < 3; ++e) q[e] = q[e] + q2[e];
+            }
             for (int e = 0; e < 3; ++e) {
                 E3 t = q[e] * (-V);
                 r[e] += t.v;
@@ -1244,9 +1290,9 @@ int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meani
 {
     try {
         Problem pr(p);
-        const double V = pr.G.volume();
         for (int s = 0; s < p->n_src; ++s) {
             const long I = p->src_cell[s];
+            const double V = pr.G.volume(I);
             IQ<E3> qi;
             computeIQ<E3>(pr.pvt, pv + 3 * I, meaning[I], p->poro[I], qi);
             E3 q[3];
@@ -1276,7 +1322,6 @@ int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersi
     try {
         Problem pr(p, dif, dsp);
         if (dsp && !nv) throw std::invalid_argument("dispersion needs the cells' velocities");
-        const double V = pr.G.volume();
         for (long I = 0; I < pr.G.N; ++I) {
             IQ<E3> qi;
             computeIQ<E3>(pr.pvt, pv + 3 * I, meaning[I], p->poro[I], qi);
@@ -1286,7 +1331,8 @@ int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersi
                 computeIQ<double>(pr.pvt, pv + 3 * nbi.J, meaning[nbi.J], p->poro[nbi.J], qj);
                 E3 f[3];
                 double d[3];
-                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, V, f, d, pr.diffusion(), pr.dispersion(),
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, pr.G.volume(I), pr.G.volume(nbi.J), f, d, pr.diffusion(),
+                         pr.dispersion(),
                          dsp ? nv[I] : 0.0, dsp ? nv[nbi.J] : 0.0);
                 for (int e = 0; e < 3; ++e) {
                     if (flows) flows[3L * k + e] = f[e].v;

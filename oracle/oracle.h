@@ -44,6 +44,16 @@ typedef struct ob_problem {
     const int32_t* src_type; /* 0: mass-rate water injector, 1: BHP producer */
     const double* src_a;     /* injector: water mass rate [kg/s]; producer: WI [m^3] */
     const double* src_b;     /* producer: bottom-hole pressure [Pa] */
+    /* The linearizer's per-neighbour inputs given by the caller
+     * (tpfalinearizer.hh:443-452), each NULL for the structured grid's own:
+     * face_trans [3][N], face I|I+e_axis at axis*N + I (transmissibility);
+     * depth [N] (dofCenterDepth; dZg = (depth_I - depth_J) * gravity, the
+     * grid's own is dz * (k + 0.5)); volume [N] (dofTotalVolume, > 0);
+     * thpres [3][N] (thresholdPressure, >= 0, at the face's index). */
+    const double* face_trans;
+    const double* depth;
+    const double* volume;
+    const double* thpres;
 } ob_problem;
 
 /* Newton / linear-solver parameters; defaults are the reference's

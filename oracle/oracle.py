@@ -17,7 +17,8 @@ _u8p = C.POINTER(C.c_uint8)
 
 
 class ObProblem(C.Structure):
-    """Same layout as ob_problem (oracle.h) and omodels_blackoil_problem."""
+    """ob_problem (oracle.h): omodels_blackoil_problem's layout plus the
+    caller-given discretization (NULL: the structured grid's own)."""
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_src", C.c_int32),
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("gravity", C.c_double),
@@ -27,6 +28,7 @@ class ObProblem(C.Structure):
         ("oil_comp", C.c_double), ("water_pref", C.c_double), ("water_comp", C.c_double),
         ("water_mu", C.c_double), ("rho_ref", C.c_double * 3),
         ("src_cell", _ip), ("src_type", _ip), ("src_a", _dp), ("src_b", _dp),
+        ("face_trans", _dp), ("depth", _dp), ("volume", _dp), ("thpres", _dp),
     ]
 
 
@@ -158,6 +160,25 @@ def params(**kw):
 def problem_from_case(case):
     """ob_problem view of a BlackOilCase built by the product's generator."""
     return case.descriptor(ObProblem)
+
+
+def with_discretization(prob, trans=None, depth=None, volume=None, thpres=None):
+    """prob with the caller-given transmissibilities / thresholds ([3][N] at
+    the face's index) and depths / volumes ([N]); None for the grid's own.
+    The arrays are kept alive by the returned problem."""
+    N = prob.nx * prob.ny * prob.nz
+    keep = []
+    for name, a, n in (("face_trans", trans, 3 * N), ("depth", depth, N), ("volume", volume, N),
+                       ("thpres", thpres, 3 * N)):
+        if a is None:
+            setattr(prob, name, None)
+            continue
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        assert a.size == n, (name, a.size, n)
+        keep.append(a)
+        setattr(prob, name, P(a))
+    prob._geo_keep = keep
+    return prob
 
 
 def pattern(prob):

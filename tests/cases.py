@@ -40,3 +40,38 @@ def block_rel_err(Ja, Jb):
     B = Jb.reshape(-1, 9)
     s = np.maximum(np.max(np.abs(B), axis=1), 1e-300)
     return float(np.max(np.max(np.abs(A - B), axis=1) / s))
+
+
+def own_trans(prob):
+    """[3][N] transmissibilities of the structured grid at the face's index
+    (axis*N + I for face I|I+e_axis), from the oracle's pattern"""
+    pat = orc.pattern(prob)
+    N = prob.nx * prob.ny * prob.nz
+    t = np.zeros(3 * N)
+    for I in range(N):
+        for k in range(pat["nb_ptr"][I], pat["nb_ptr"][I + 1]):
+            if pat["nb_cell"][k] > I:
+                t[(pat["nb_dir"][k] // 2) * N + I] = pat["nb_trans"][k]
+    return t
+
+
+def own_depth(c):
+    s = c.spec
+    return s.dz * (np.arange(c.N) // (s.nx * s.ny) + 0.5)
+
+
+def random_discretization(c, seed, thp_fraction=0.3, thp_max=4e5):
+    """Transmissibilities, depths, volumes and threshold pressures of a
+    distorted grid: the grid's own values perturbed, some volumes equal (ties
+    of the upwind decision), some faces closed, thresholds on a fraction of
+    the faces."""
+    rng = np.random.default_rng(seed)
+    N = c.N
+    V = c.spec.dx * c.spec.dy * c.spec.dz
+    t = own_trans(orc.problem_from_case(c)) * (0.5 + rng.random(3 * N))
+    t[rng.random(3 * N) < 0.05] = 0.0
+    z = own_depth(c) + 0.3 * c.spec.dz * rng.standard_normal(N)
+    vol = V * (0.5 + rng.random(N))
+    vol[::7] = V
+    thp = np.where(rng.random(3 * N) < thp_fraction, thp_max * rng.random(3 * N), 0.0)
+    return t, z, vol, thp

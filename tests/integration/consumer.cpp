@@ -1,0 +1,163 @@
+// A C++ consumer of the adapters in omodels_opm.hpp, built against
+// include/ and libomodels_amd.so the way reference-side code would be: a
+// stand-in Simulator with its Glue specialization, one Newton iteration
+// through the linearizer, the BiCGStab backend and the black-oil update, and
+// a second linearization.  Reads a case from a directory of raw arrays and
+// writes what it computed there.
+//
+// usage: omodels_consumer <dir>
+//   <dir>/dims.txt: nx ny nz n_src n_oil n_gas dx dy dz gravity dt
+//   <dir>/<name>.bin: perm poro oil_tab gas_tab consts src_cell src_type src_a
+//                     src_b pv0 mean0 [trans depth volume thpres]
+// writes jac0 res0 x pv1 mean1 jac1 res1 (.out) and prints one summary line.
+// Exit codes: 0 done, 3 an error of the library (message on stderr).
+#include <cstdio>
+#include <cstdint>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "omodels_opm.hpp"
+
+namespace {
+
+template <class T>
+bool readBin(const std::string& path, std::vector<T>& v, bool optional = false)
+{
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) {
+        if (optional) { v.clear(); return false; }
+        throw std::runtime_error("missing " + path);
+    }
+    const std::streamsize n = f.tellg();
+    f.seekg(0);
+    v.resize((size_t)n / sizeof(T));
+    f.read(reinterpret_cast<char*>(v.data()), n);
+    return true;
+}
+
+template <class T>
+void writeBin(const std::string& path, const std::vector<T>& v)
+{
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
+}
+
+// the reference's Simulator, reduced to what the glue reads
+struct StandInSimulator {
+    int nx = 0, ny = 0, nz = 0, nSrc = 0, nOil = 0, nGas = 0;
+    double dx = 0, dy = 0, dz = 0, gravity = 0, dt = 0;
+    std::vector<double> perm, poro, oilTab, gasTab, consts, srcA, srcB;
+    std::vector<int32_t> srcCell, srcType;
+    std::vector<double> pv;            // solution(0)
+    std::vector<uint8_t> meaning;
+    int newtonIteration = 0;
+    std::vector<double> trans, depth, volume, thpres;
+};
+
+}  // namespace
+
+template <>
+struct omodels_opm::Glue<StandInSimulator> {
+    static omodels_blackoil_problem describe(const StandInSimulator& s)
+    {
+        omodels_blackoil_problem p{};
+        p.nx = s.nx; p.ny = s.ny; p.nz = s.nz; p.n_src = s.nSrc;
+        p.dx = s.dx; p.dy = s.dy; p.dz = s.dz; p.gravity = s.gravity;
+        p.perm = s.perm.data(); p.poro = s.poro.data();
+        p.n_oil = s.nOil; p.n_gas = s.nGas;
+        p.oil_tab = s.oilTab.data(); p.gas_tab = s.gasTab.data();
+        p.oil_comp = s.consts[0]; p.water_pref = s.consts[1]; p.water_comp = s.consts[2]; p.water_mu = s.consts[3];
+        p.rho_ref[0] = s.consts[4]; p.rho_ref[1] = s.consts[5]; p.rho_ref[2] = s.consts[6];
+        p.src_cell = s.srcCell.data(); p.src_type = s.srcType.data();
+        p.src_a = s.srcA.data(); p.src_b = s.srcB.data();
+        return p;
+    }
+    static void packSolution(const StandInSimulator& s, std::vector<double>& pv, std::vector<uint8_t>& meaning)
+    {
+        pv = s.pv;
+        meaning = s.meaning;
+    }
+    static void unpackSolution(StandInSimulator& s, const std::vector<double>& pv, const std::vector<uint8_t>& meaning)
+    {
+        s.pv = pv;
+        s.meaning = meaning;
+    }
+    static double timeStepSize(const StandInSimulator& s) { return s.dt; }
+    static int newtonIteration(const StandInSimulator& s) { return s.newtonIteration; }
+    static void discretization(const StandInSimulator& s, std::vector<double>& trans, std::vector<double>& depth,
+                               std::vector<double>& volume, std::vector<double>& thpres)
+    {
+        trans = s.trans;
+        depth = s.depth;
+        volume = s.volume;
+        thpres = s.thpres;
+    }
+};
+
+int main(int argc, char** argv)
+{
+    if (argc != 2) {
+        std::cerr << "usage: omodels_consumer <dir>\n";
+        return 1;
+    }
+    const std::string dir = argv[1];
+    StandInSimulator sim;
+    try {
+        std::ifstream d(dir + "/dims.txt");
+        d >> sim.nx >> sim.ny >> sim.nz >> sim.nSrc >> sim.nOil >> sim.nGas >> sim.dx >> sim.dy >> sim.dz
+            >> sim.gravity >> sim.dt;
+        if (!d) throw std::runtime_error("bad dims.txt");
+        readBin(dir + "/perm.bin", sim.perm);
+        readBin(dir + "/poro.bin", sim.poro);
+        readBin(dir + "/oil_tab.bin", sim.oilTab);
+        readBin(dir + "/gas_tab.bin", sim.gasTab);
+        readBin(dir + "/consts.bin", sim.consts);
+        readBin(dir + "/src_cell.bin", sim.srcCell);
+        readBin(dir + "/src_type.bin", sim.srcType);
+        readBin(dir + "/src_a.bin", sim.srcA);
+        readBin(dir + "/src_b.bin", sim.srcB);
+        readBin(dir + "/pv0.bin", sim.pv);
+        readBin(dir + "/mean0.bin", sim.meaning);
+        readBin(dir + "/trans.bin", sim.trans, true);
+        readBin(dir + "/depth.bin", sim.depth, true);
+        readBin(dir + "/volume.bin", sim.volume, true);
+        readBin(dir + "/thpres.bin", sim.thpres, true);
+    } catch (const std::exception& e) {
+        std::cerr << "input: " << e.what() << "\n";
+        return 1;
+    }
+    try {
+        omodels_opm::GpuTpfaLinearizer<StandInSimulator> lin;
+        lin.init(sim);
+        omodels_params prm;
+        omodels_default_params(&prm);
+        omodels_opm::GpuBiCGStabBackend backend(lin.handle(), prm);
+
+        sim.newtonIteration = 0;
+        lin.linearizeDomain();
+        writeBin(dir + "/jac0.out", lin.jacobian());
+        writeBin(dir + "/res0.out", lin.residual());
+        backend.prepare(lin.jacobian(), lin.residual());
+        std::vector<double> x(lin.residual().size(), 0.0);
+        const bool converged = backend.solve(x);
+        writeBin(dir + "/x.out", x);
+        const int switched = omodels_opm::newtonUpdate(lin, sim, prm, x);
+        writeBin(dir + "/pv1.out", sim.pv);
+        writeBin(dir + "/mean1.out", sim.meaning);
+        sim.newtonIteration = 1;
+        lin.linearizeDomain();
+        writeBin(dir + "/jac1.out", lin.jacobian());
+        writeBin(dir + "/res1.out", lin.residual());
+        std::printf("{\"converged\": %d, \"iterations\": %d, \"switched\": %d, \"cells\": %lld}\n", converged ? 1 : 0,
+                    backend.iterations(), switched, (long long)lin.numCells());
+        return 0;
+    } catch (const omodels_opm::NumericalProblem& e) {
+        std::cerr << "numerical problem: " << e.what() << "\n";
+        return 3;
+    } catch (const std::exception& e) {
+        std::cerr << "error: " << e.what() << "\n";
+        return 3;
+    }
+}
