@@ -44,9 +44,11 @@ CONFIGS = {
     # step control (at most one day, halved on failure) sets the steps
     "c4": dict(nx=200, ny=200, nz=50, adaptive=True, label="blackoil 200x200x50 (2M cells), 3x3 BCRS"),
     # channelised field (1000 mD channels in a 1 mD background, sigma 2 within
-    # each facies), five-spot, the reference's adaptive step control
-    "c4ch": dict(nx=200, ny=200, nz=50, spec=dict(field=1, perm_sigma=2.0, wells=1), adaptive=True,
-                 label="blackoil 200x200x50 (2M cells) channelised five-spot, 3x3 BCRS"),
+    # each facies), five-spot with the injector in a channel (250 kg/s split
+    # over its layers by kh), oil 10% undersaturated at the start, the
+    # reference's adaptive step control
+    "c4ch": dict(nx=200, ny=200, nz=50, spec=dict(field=1, perm_sigma=2.0, wells=1, inj_rate=250.0, rs_frac=0.9),
+                 adaptive=True, label="blackoil 200x200x50 (2M cells) channelised five-spot, 3x3 BCRS"),
 }
 
 

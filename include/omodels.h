@@ -72,9 +72,13 @@ typedef struct omodels_blackoil_spec {
     double sw_init;
     int32_t wells;           /* 0: corner injector/producer, 1: five-spot, 2: random pair */
     int32_t pad;
-    double inj_rate;         /* total water injection [kg/s] */
+    double inj_rate;         /* total water injection [kg/s]; field 1 splits it over the
+                                injector's layers by kh and places the five-spot injector
+                                in the column of the largest kh near the centre */
     double prod_bhp;         /* producer bottom-hole pressure [Pa] */
     uint64_t seed;
+    double rs_frac;          /* initial Rs as a fraction of the saturated value (0 or 1:
+                                saturated oil, bubble point at the initial pressure) */
 } omodels_blackoil_spec;
 
 /* Fill the PVT tables of tests/problems/reservoirproblem.hh:223-327

@@ -172,17 +172,29 @@ void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::v
         const long k = I / (nx * ny);
         pv[3 * I] = sp.sw_init;
         pv[3 * I + 1] = pk[k];
-        pv[3 * I + 2] = tabHost(rsSat, pk[k]);
+        const double f = (sp.rs_frac > 0.0 && sp.rs_frac < 1.0) ? sp.rs_frac : 1.0;
+        pv[3 * I + 2] = f * tabHost(rsSat, pk[k]);
         mean[I] = kMeanRs;
     }
     // wells as source terms on full columns
     srcCell.clear(); srcType.clear(); srcA.clear(); srcB.clear();
+    auto columnKh = [&](long i, long j) {
+        double s = 0.0;
+        for (long k = 0; k < nz; ++k) s += perm[i + nx * (j + ny * k)];
+        return s;
+    };
     auto column = [&](long i, long j, int type, double a) {
+        // the channelised field's injector gives each layer the share of the
+        // rate its kh has of the column's (the completions' allocation); the
+        // log-normal fields split it evenly
+        const double khSum = columnKh(i, j);
         for (long k = 0; k < nz; ++k) {
             const long I = i + nx * (j + ny * k);
             srcCell.push_back((int32_t)I); srcType.push_back(type);
-            if (type == 0) { srcA.push_back(a / nz); srcB.push_back(0.0); }
-            else {
+            if (type == 0) {
+                srcA.push_back(sp.field == 1 ? a * (perm[I] / khSum) : a / nz);
+                srcB.push_back(0.0);
+            } else {
                 const double kh = std::sqrt(perm[I] * perm[N + I]);
                 const double re = 0.2 * sp.dx, rw = 0.1;
                 srcA.push_back(a * 6.283185307179586 * kh * sp.dz / std::log(re / rw));
@@ -194,7 +206,19 @@ void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::v
         column(0, 0, 0, sp.inj_rate);
         column(nx - 1, ny - 1, 1, 1.0);
     } else if (sp.wells == 1) {
-        column(nx / 2, ny / 2, 0, sp.inj_rate);
+        // the channelised field's injector sits in the column of the largest
+        // kh within a few cells of the centre (a well placed in a channel)
+        long ci = nx / 2, cj = ny / 2;
+        if (sp.field == 1) {
+            const long r = std::max<long>(1, std::min(nx, ny) / 20);
+            double best = -1.0;
+            for (long j = std::max<long>(0, ny / 2 - r); j <= std::min(ny - 1, ny / 2 + r); ++j)
+                for (long i = std::max<long>(0, nx / 2 - r); i <= std::min(nx - 1, nx / 2 + r); ++i) {
+                    const double v = columnKh(i, j);
+                    if (v > best) { best = v; ci = i; cj = j; }
+                }
+        }
+        column(ci, cj, 0, sp.inj_rate);
         column(0, 0, 1, 1.0); column(nx - 1, 0, 1, 1.0);
         column(0, ny - 1, 1, 1.0); column(nx - 1, ny - 1, 1, 1.0);
     } else {

@@ -51,7 +51,7 @@ class BlackOilSpec(C.Structure):
         ("poro_min", C.c_double), ("poro_max", C.c_double),
         ("p_top", C.c_double), ("sw_init", C.c_double),
         ("wells", C.c_int32), ("pad", C.c_int32),
-        ("inj_rate", C.c_double), ("prod_bhp", C.c_double), ("seed", C.c_uint64),
+        ("inj_rate", C.c_double), ("prod_bhp", C.c_double), ("seed", C.c_uint64), ("rs_frac", C.c_double),
     ]
 
 
@@ -285,7 +285,7 @@ def default_fluid():
 def make_spec(nx, ny, nz, dx=50.0, dy=50.0, dz=5.0, field=0, perm_mean_md=100.0, perm_sigma=1.0,
               kz_ratio=0.1, chan_perm_md=1000.0, back_perm_md=1.0, chan_fraction=0.3, corr_len=0.0,
               poro_min=0.15, poro_max=0.30, p_top=200e5, sw_init=0.2, wells=0, inj_rate=None,
-              prod_bhp=150e5, seed=1234):
+              prod_bhp=150e5, seed=1234, rs_frac=1.0):
     s = BlackOilSpec()
     s.nx, s.ny, s.nz, s.field = nx, ny, nz, field
     s.dx, s.dy, s.dz = dx, dy, dz
@@ -297,6 +297,7 @@ def make_spec(nx, ny, nz, dx=50.0, dy=50.0, dz=5.0, field=0, perm_mean_md=100.0,
         # 0.01% of the pore volume of water per day
         inj_rate = 1e-4 * (nx * dx * ny * dy * nz * dz) * 0.225 * 1037.0 / 86400.0
     s.inj_rate, s.prod_bhp, s.seed = inj_rate, prod_bhp, seed
+    s.rs_frac = rs_frac
     return s
 
 

@@ -128,6 +128,20 @@ int main(int argc, char** argv)
         std::cerr << "input: " << e.what() << "\n";
         return 1;
     }
+    {
+        // the host-only entry points: defaults and the reference problem's tables
+        omodels_params prm;
+        omodels_default_params(&prm);
+        std::vector<double> oil(4 * 16), gas(3 * 16), consts(7);
+        int32_t nOil = 0, nGas = 0;
+        if (omodels_blackoil_default_fluid(oil.data(), &nOil, gas.data(), &nGas, consts.data()) != 0) {
+            std::cerr << "error: default_fluid: " << omodels_last_error() << "\n";
+            return 3;
+        }
+        std::printf("{\"host\": {\"lin_tol\": %g, \"newton_max_iter\": %d, \"n_oil\": %d, \"n_gas\": %d}}\n",
+                    prm.lin_tol, prm.newton_max_iter, nOil, nGas);
+        std::fflush(stdout);
+    }
     try {
         omodels_opm::GpuTpfaLinearizer<StandInSimulator> lin;
         lin.init(sim);

@@ -49,6 +49,8 @@ def test_consumer_compiles_and_reports_library_errors(tmp_path):
     # without a GPU the model cannot be created: the adapter throws with the
     # library's message; with one the iteration runs
     assert r.returncode in (0, 3), r.stderr
+    # the host-only entry points answered before any device work
+    assert '"lin_tol": 0.001' in r.stdout and '"n_oil": 9' in r.stdout and '"n_gas": 10' in r.stdout, r.stdout
     if r.returncode == 3:
         assert "model_create" in r.stderr and omodels_error_text(r.stderr)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
