@@ -187,6 +187,22 @@ int omodels_linearizer_pattern(omodels_model* m, int32_t* row_ptr, int32_t* col_
  * omodels_linearizer_pattern report them. */
 int omodels_linearizer_update_discretization(omodels_model* m, const double* trans, const double* depth,
                                              const double* volume, const double* thpres);
+/* Rock compressibility and compaction of the problem
+ * (blackoilintensivequantities.hh:447-480, blackoilproblem.hh:131-181):
+ *   compressibility [N] problem.rockCompressibility(I) >= 0 [1/Pa]: where > 0
+ *           the porosity is multiplied by 1 + x + x^2/2, x = c (p_o - p_ref);
+ *   ref_pressure [N] problem.rockReferencePressure(I) [Pa] (NULL: 1e5 Pa);
+ *   table [3][n_table] (n_table in [2, 12], or 0 for none): oil pressure
+ *           (increasing), rockCompPoroMultiplier and rockCompTransMultiplier
+ *           (> 0), piecewise linear in p_o (linear beyond the ends); the
+ *           porosity is multiplied by the first, and each face's
+ *           transmissibility by the mean of its two cells' second
+ *           (blackoillocalresidualtpfa.hh:372-381).
+ * NULL compressibility and n_table 0 switch the rock terms off (the
+ * reference's defaults: 0, 1e5 Pa, multipliers 1).  The storage cache,
+ * linearizations, FLOWS, output fields and simulate calls include them. */
+int omodels_model_set_rock(omodels_model* m, const double* compressibility, const double* ref_pressure,
+                           int32_t n_table, const double* table);
 /* linearizeDomain() (tpfalinearizer.hh:189-245, linearize_ :711-866) at the
  * current solution(0); iteration_index 0 refreshes the storage cache of
  * time level 1 as recycleFirstIterationStorage() does (:786-812). */

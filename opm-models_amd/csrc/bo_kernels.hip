@@ -81,11 +81,12 @@ template <bool Diff, bool Disp>
 __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const IqV& qj, double trans,
                                           double area, double dZg, bool upInTie, double r[3],
                                           double D[9], double* __restrict__ Jout, const BoDiff& dif,
-                                          DiffFace df, const BoDisp& dsp, double& nv, double thp)
+                                          DiffFace df, const BoDisp& dsp, double& nv, double thp,
+                                          const FaceTM* tm)
 {
     E3 fl[3];
     double darcy[3];
-    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp, thp);
+    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp, thp, tm);
     if constexpr (Disp) maxVelocity(nv, darcy, area);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -190,48 +191,60 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
                 if constexpr (Geo) return geo.thp[fi];
                 else return 0.0;
             };
+            // the face's transmissibility multipliers (given geometry only)
+            FaceTM tmv{ec(1.0), 1.0};
+            if constexpr (Geo) tmv.in = rockTransMult(geo.rock, qi.p);
+            auto tmOf = [&](const IqV& q) -> const FaceTM* {
+                if constexpr (Geo) {
+                    tmv.ex = rockTransMultV(geo.rock, q.p);
+                    return &tmv;
+                } else {
+                    return nullptr;
+                }
+            };
             // intersection order: -x, +x, -y, +y, -z, +z
             if (i > 0) {
                 IqV q; fromLds(plane[ty + 1][tx], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[I - 1], G.area[0], (zI - zOf(I - 1, 0)) * G.g, tieOf(I - 1), r, D,
                                 jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - 1, q, I - 1), dsp,
-                                nvo, thpOf(I - 1));
+                                nvo, thpOf(I - 1), tmOf(q));
             }
             if (i + 1 < G.nx) {
                 IqV q; fromLds(plane[ty + 1][tx + 2], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[I], G.area[0], (zI - zOf(I + 1, 0)) * G.g, tieOf(I + 1), r, D,
                                 jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + 1, q, I), dsp,
-                                nvo, thpOf(I));
+                                nvo, thpOf(I), tmOf(q));
             }
             if (j > 0) {
                 IqV q; fromLds(plane[ty][tx + 1], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zOf(I - G.nx, 0)) * G.g,
                                 tieOf(I - G.nx), r, D, jac + 9L * nbOff[2 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - G.nx, q, N + I - G.nx), dsp, nvo,
-                                thpOf(N + I - G.nx));
+                                thpOf(N + I - G.nx), tmOf(q));
             }
             if (j + 1 < G.ny) {
                 IqV q; fromLds(plane[ty + 2][tx + 1], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[N + I], G.area[1], (zI - zOf(I + G.nx, 0)) * G.g, tieOf(I + G.nx),
                                 r, D, jac + 9L * nbOff[3 * N + I], dif,
-                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo, thpOf(N + I));
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo, thpOf(N + I), tmOf(q));
             }
             if (haveBelow) {
                 applyFace<Diff, Disp>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zOf(I - nxy, -1)) * G.g,
                                 tieOf(I - nxy), r, D, jac + 9L * nbOff[4 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - nxy, below, 2 * N + I - nxy), dsp, nvo,
-                                thpOf(2 * N + I - nxy));
+                                thpOf(2 * N + I - nxy), tmOf(below));
             }
             if (k + 1 < G.nz) {
                 applyFace<Diff, Disp>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zOf(I + nxy, 1)) * G.g,
                                 tieOf(I + nxy), r, D, jac + 9L * nbOff[5 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + nxy, above, 2 * N + I), dsp, nvo,
-                                thpOf(2 * N + I));
+                                thpOf(2 * N + I), tmOf(above));
             }
             // accumulation term (tpfalinearizer.hh:776-825)
             {
                 E3 st[3];
-                storageEval(f, qi, poro[I], st);
+                if constexpr (Geo) storageEval(f, qi, rockPorosity(geo.rock, I, poro[I], qi.p), st);
+                else storageEval(f, qi, poro[I], st);
                 const double sf = Geo ? geo.vol[I] / geo.dt : storefac;
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
@@ -403,25 +416,30 @@ __global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const doubl
         E3 fl[3];
         double darcy[3];
         double thp = 0.0;
+        FaceTM tmv{ec(1.0), 1.0};
+        const FaceTM* tm = nullptr;
         if (geo.depth) {
             zJ = geo.depth[J];
             const double vi = geo.vol[I], vj = geo.vol[J];
             upInTie = vi > vj || (vi == vj && I < J);
             thp = geo.thp[fi];
+            tmv.in = rockTransMult(geo.rock, qi.p);
+            tmv.ex = rockTransMultV(geo.rock, qj.p);
+            tm = &tmv;
         }
         const double dZg = (zI - zJ) * G.g;
         if (dif.diff && dsp.disp)
             faceFluxT<true, true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp);
+                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp, tm);
         else if (dif.diff)
             faceFluxT<true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi), nullptr, thp);
+                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi), nullptr, thp, tm);
         else if (dsp.disp)
             faceFluxT<true, false, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp);
+                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp, tm);
         else
             faceFluxT<true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, nullptr,
-                            DiffFace{0.0, 0.0, 0.0, 0.0}, nullptr, thp);
+                            DiffFace{0.0, 0.0, 0.0, 0.0}, nullptr, thp, tm);
         for (int c = 0; c < 3; ++c) {
             if (flows) flows[3 * e + c] = fl[c].v;
             if (flores) flores[3 * e + c] = darcy[c];
@@ -443,7 +461,7 @@ __global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const doubl
 // (vtkblackoilmodule.hh:276-301).  Phases in the order water, oil, gas.
 __global__ void k_bo_output(const BoFluid* __restrict__ fp, long N, const double* __restrict__ pv,
                             const uint8_t* __restrict__ mean, const double* __restrict__ poro,
-                            double* __restrict__ out)
+                            double* __restrict__ out, BoRock rock)
 {
     const BoFluid& f = *fp;
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -457,30 +475,31 @@ __global__ void k_bo_output(const BoFluid* __restrict__ fp, long N, const double
         out[(6 + a) * N + I] = q.rho[ph[a]].v;
         out[(9 + a) * N + I] = q.lam[ph[a]].v;
     }
-    out[12 * N + I] = poro[I];
+    out[12 * N + I] = rockPorosityV(rock, I, poro[I], pv[3 * I + 1]);
     out[13 * N + I] = q.Rs.v;
     for (int a = 0; a < 3; ++a) out[(14 + a) * N + I] = 1.0 / q.invB[ph[a]].v;
     out[17 * N + I] = (double)mean[I];
 }
 
 void launchBoOutput(const BoFluid* fdev, long N, const double* pv, const uint8_t* mean, const double* poro,
-                    double* out, hipStream_t s)
+                    double* out, hipStream_t s, const BoRock* rock)
 {
     if (N <= 0) return;
     hipLaunchKernelGGL(k_bo_output, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, fdev, N, pv, mean, poro,
-                       out);
+                       out, rock ? *rock : BoRock{});
 }
 
 // Storage of the current solution: the cache of time level 1 at the first
 // Newton iteration (recycleFirstIterationStorage, tpfalinearizer.hh:786-812).
 __global__ void k_bo_storage(BoFluid f, long N, const double* __restrict__ pv,
                              const uint8_t* __restrict__ mean, const double* __restrict__ poro,
-                             double* __restrict__ st)
+                             double* __restrict__ st, BoRock rock)
 {
     const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= N) return;
     double s[3];
-    storageValues(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], poro[I], s);
+    storageValues(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], rockPorosityV(rock, I, poro[I], pv[3 * I + 1]),
+                  s);
     st[3 * I] = s[0]; st[3 * I + 1] = s[1]; st[3 * I + 2] = s[2];
 }
 
@@ -634,9 +653,10 @@ void launchAddSparseSources(int n, const int* cell, const double* r, const doubl
 }
 
 void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
-                     const double* poro, double* st, hipStream_t s)
+                     const double* poro, double* st, hipStream_t s, const BoRock* rock)
 {
-    hipLaunchKernelGGL(k_bo_storage, dim3((N + 255) / 256), dim3(256), 0, s, f, N, pv, mean, poro, st);
+    hipLaunchKernelGGL(k_bo_storage, dim3((N + 255) / 256), dim3(256), 0, s, f, N, pv, mean, poro, st,
+                       rock ? *rock : BoRock{});
 }
 
 void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,

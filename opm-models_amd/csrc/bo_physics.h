@@ -198,6 +198,22 @@ __device__ __forceinline__ void storageEval(const BoFluid& f, const IqE& q, doub
     st[kCO] = st[kCO] * f.rho[kO];
 }
 
+// the same with a porosity that depends on the cell's pressure (rock compressibility)
+__device__ __forceinline__ void storageEval(const BoFluid& f, const IqE& q, E3 phi, E3 st[3])
+{
+    st[0] = ec(0.0); st[1] = ec(0.0); st[2] = ec(0.0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const E3 sv = (q.S[a] * q.invB[a]) * phi;
+        const int c = compOfPhase(a);
+        st[c] = st[c] + sv;
+        if (a == kO) st[kCG] = st[kCG] + q.Rs * sv;
+    }
+    st[kCW] = st[kCW] * f.rho[kW];
+    st[kCG] = st[kCG] * f.rho[kG];
+    st[kCO] = st[kCO] * f.rho[kO];
+}
+
 __device__ __forceinline__ void storageValues(const BoFluid& f, double sw, double po, double x2, int mean,
                                               double phi, double st[3])
 {
@@ -226,6 +242,52 @@ __device__ __forceinline__ void storageValues(const BoFluid& f, double sw, doubl
 // in the oil phase: the arithmetic mean of the two cells' constant
 // coefficients (:568-590), tortuosity being disabled for black-oil (:384-394).
 // diff: the problem's diffusivity per face, [3][N] like the transmissibilities.
+// the two cells' transmissibility multipliers of one face
+struct FaceTM {
+    E3 in;
+    double ex;
+};
+
+// Rock compressibility and compaction (blackoilintensivequantities.hh:447-480):
+// porosity *= 1 + x + x^2/2 with x = c (p_o - p_ref) where c > 0, then times
+// the problem's pore-volume multiplier of p_o (rockCompPoroMultiplier); the
+// face term takes the transmissibility multiplier of p_o
+// (rockCompTransMultiplier).  comp / pref per cell (nullptr: 0 / 1e5 Pa);
+// poroMult / transMult: piecewise-linear tables of p_o, or nullptr (1).
+struct BoRock {
+    const double* comp;
+    const double* pref;
+    const Tab* poroMult;
+    const Tab* transMult;
+};
+
+__device__ __forceinline__ E3 rockPorosity(const BoRock& r, long I, double poro, E3 p)
+{
+    E3 phi = ec(poro);
+    const double c = r.comp ? r.comp[I] : 0.0;
+    if (c > 0.0) {
+        const double pref = r.pref ? r.pref[I] : 1e5;
+        const E3 x = (p - pref) * c;
+        phi = phi * ((x + 1.0) + (x * 0.5) * x);
+    }
+    if (r.poroMult) phi = phi * tabEval(*r.poroMult, p);
+    return phi;
+}
+__device__ __forceinline__ double rockPorosityV(const BoRock& r, long I, double poro, double p)
+{
+    double phi = poro;
+    const double c = r.comp ? r.comp[I] : 0.0;
+    if (c > 0.0) {
+        const double pref = r.pref ? r.pref[I] : 1e5;
+        const double x = (p - pref) * c;
+        phi = phi * ((x + 1.0) + (x * 0.5) * x);
+    }
+    if (r.poroMult) phi = phi * tabEval(*r.poroMult, p);
+    return phi;
+}
+__device__ __forceinline__ E3 rockTransMult(const BoRock& r, E3 p) { return r.transMult ? tabEval(*r.transMult, p) : ec(1.0); }
+__device__ __forceinline__ double rockTransMultV(const BoRock& r, double p) { return r.transMult ? tabEval(*r.transMult, p) : 1.0; }
+
 struct BoDiff {
     double toFrac, effOil, effGas;
     const double* diff;
@@ -254,11 +316,14 @@ struct DiffFace {
 // darcyFlux.value() * faceArea at the phase's component index (the FLORES
 // values, blackoillocalresidualtpfa.hh:376-385)
 // WithDiff / WithDisp: plus the diffusive / dispersive term of the oil phase.
+// tm: nullptr, or the two cells' rockCompTransMultiplier (the interior's with
+// derivatives, the exterior's value): the face takes their arithmetic mean
+// (blackoillocalresidualtpfa.hh:372-381).
 template <bool WithDarcy, bool WithDiff = false, bool WithDisp = false>
 __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
                                           double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3],
                                           const BoDiff* dif = nullptr, DiffFace df = DiffFace{0.0, 0.0, 0.0, 0.0},
-                                          const BoDisp* dsp = nullptr, double thp = 0.0)
+                                          const BoDisp* dsp = nullptr, double thp = 0.0, const FaceTM* tm = nullptr)
 {
     flux[0] = ec(0.0); flux[1] = ec(0.0); flux[2] = ec(0.0);
     if constexpr (WithDarcy) { darcyOut[0] = 0.0; darcyOut[1] = 0.0; darcyOut[2] = 0.0; }
@@ -281,12 +346,12 @@ __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const
         E3 gas;
         double darcyV;
         if (upIn) {
-            const E3 darcy = (dp * in.lam[a]) * tf;
+            const E3 darcy = tm ? ((dp * in.lam[a]) * ((tm->in + tm->ex) * 0.5)) * tf : (dp * in.lam[a]) * tf;
             svf = in.invB[a] * darcy;
             if (a == kO) gas = in.Rs * svf;
             darcyV = darcy.v;
         } else {
-            const E3 darcy = dp * (ex.lam[a] * tf);
+            const E3 darcy = tm ? dp * ((((tm->in + tm->ex) * 0.5) * ex.lam[a]) * tf) : dp * (ex.lam[a] * tf);
             svf = darcy * ex.invB[a];
             if (a == kO) gas = svf * ex.Rs;
             darcyV = darcy.v;

@@ -166,7 +166,7 @@ int omodels_model_set_storage1(omodels_model* h, const double* st)
     return guarded([&] {
         Model& m = *h->m;
         if (st) m.st1.upload(st, 3 * m.N, m.stream);
-        else launchBoStorage(m.fluid, m.N, m.pv1.p, m.mean1.p, m.poro.p, m.st1.p, m.stream);
+        else launchBoStorage(m.fluid, m.N, m.pv1.p, m.mean1.p, m.poro.p, m.st1.p, m.stream, &m.geo.rock);
         m.sync();
         return 0;
     });
@@ -229,6 +229,15 @@ int omodels_linearizer_update_discretization(omodels_model* h, const double* tra
 {
     return guarded([&] {
         h->m->updateDiscretization(trans, depth, volume, thpres);
+        return 0;
+    });
+}
+
+int omodels_model_set_rock(omodels_model* h, const double* compressibility, const double* ref_pressure,
+                           int32_t n_table, const double* table)
+{
+    return guarded([&] {
+        h->m->setRock(compressibility, ref_pressure, n_table, table);
         return 0;
     });
 }
@@ -357,7 +366,7 @@ int omodels_output_fields(omodels_model* h, double* out)
         DevBuf<double> d;
         d.alloc((size_t)kOutFields * m.N);
         m.timer.begin(kClsOther, m.stream);
-        launchBoOutput(m.fluidDev.p, m.N, m.pv0.p, m.mean0.p, m.poro.p, d.p, m.stream);
+        launchBoOutput(m.fluidDev.p, m.N, m.pv0.p, m.mean0.p, m.poro.p, d.p, m.stream, &m.geo.rock);
         m.timer.end(m.stream);
         OM_HIP(hipGetLastError());
         d.download(out, (size_t)kOutFields * m.N, m.stream);

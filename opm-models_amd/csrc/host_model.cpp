@@ -127,7 +127,7 @@ void Model::linearizeDomain(double dt, int iterationIndex)
 {
     ensureJacobian();
     if (iterationIndex == 0)
-        launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream);
+        launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream, &geo.rock);
     const double storefac = G.V / dt;
     geo.dt = dt;
     timer.begin(kClsAssemble, stream);
@@ -208,25 +208,69 @@ void Model::updateDiscretization(const double* faceTrans, const double* depth, c
     sync();
     trans.upload(t.data(), t.size(), stream);
     topo.trans = t;
-    if (!depth && !volume && !thpres) {
-        geo = BoGeo{};
-        depthHost.clear();
-        geoDepth.release(); geoVol.release(); geoThp.release();
+    geoGiven = depth || volume || thpres;
+    depthHost.clear();
+    geoDepth.release(); geoVol.release(); geoThp.release();
+    if (geoGiven) {
+        std::vector<double> z(N), v(N), th(3 * (size_t)N, 0.0);
+        for (long I = 0; I < N; ++I) {
+            z[I] = depth ? depth[I] : G.dz * ((double)(I / nxy) + 0.5);
+            v[I] = volume ? volume[I] : G.V;
+        }
+        if (thpres) std::copy(thpres, thpres + 3 * N, th.begin());
+        geoDepth.upload(z.data(), z.size(), stream);
+        geoVol.upload(v.data(), v.size(), stream);
+        geoThp.upload(th.data(), th.size(), stream);
+        depthHost = z;
         sync();
+    }
+    refreshGeo();
+}
+
+void Model::refreshGeo()
+{
+    const bool rockOn = rockComp.p || rockTabs.p;
+    if (!geoGiven && !rockOn) {
+        geoDepth.release(); geoVol.release(); geoThp.release();
+        geo = BoGeo{};
         return;
     }
-    std::vector<double> z(N), v(N), th(3 * (size_t)N, 0.0);
-    for (long I = 0; I < N; ++I) {
-        z[I] = depth ? depth[I] : G.dz * ((double)(I / nxy) + 0.5);
-        v[I] = volume ? volume[I] : G.V;
+    if (!geoGiven && !geoDepth.p) {
+        // the grid's own geometry for the kernels' extended form
+        const long nxy = (long)G.nx * G.ny;
+        std::vector<double> z(N), v(N, G.V), th(3 * (size_t)N, 0.0);
+        for (long I = 0; I < N; ++I) z[I] = G.dz * ((double)(I / nxy) + 0.5);
+        geoDepth.upload(z.data(), z.size(), stream);
+        geoVol.upload(v.data(), v.size(), stream);
+        geoThp.upload(th.data(), th.size(), stream);
+        sync();
     }
-    if (thpres) std::copy(thpres, thpres + 3 * N, th.begin());
-    geoDepth.upload(z.data(), z.size(), stream);
-    geoVol.upload(v.data(), v.size(), stream);
-    geoThp.upload(th.data(), th.size(), stream);
-    depthHost = z;
     geo.depth = geoDepth.p; geo.vol = geoVol.p; geo.thp = geoThp.p;
+    geo.rock = BoRock{rockComp.p, rockPref.p, rockTabs.p, rockTabs.p ? rockTabs.p + 1 : nullptr};
+}
+
+void Model::setRock(const double* comp, const double* pref, int nTab, const double* table)
+{
+    if (nTab < 0 || (nTab > 0 && !table)) throw std::invalid_argument("rock table: n >= 0 and a table when n > 0");
+    if (comp) checkNonNegative(comp, N, "rock compressibilities must be finite and >= 0");
+    if (pref)
+        for (long I = 0; I < N; ++I)
+            if (!std::isfinite(pref[I])) throw std::invalid_argument("rock reference pressures must be finite");
+    Tab tabs[2];
+    if (nTab > 0) {
+        for (int k = 0; k < 2 * nTab; ++k)
+            if (!(table[nTab + k] > 0.0) || !std::isfinite(table[nTab + k]))
+                throw std::invalid_argument("rock multipliers must be finite and > 0");
+        fillTab(tabs[0], table, table + nTab, nTab);
+        fillTab(tabs[1], table, table + 2 * nTab, nTab);
+    }
     sync();
+    rockComp.release(); rockPref.release(); rockTabs.release();
+    if (comp) rockComp.upload(comp, N, stream);
+    if (comp && pref) rockPref.upload(pref, N, stream);
+    if (nTab > 0) rockTabs.upload(tabs, 2, stream);
+    sync();
+    refreshGeo();
 }
 
 void Model::setDiffusion(const omodels_diffusion* d)

@@ -27,6 +27,8 @@ struct HostTopology {
 
 void defaultFluid(std::vector<double>& oil, int& nOil, std::vector<double>& gas, int& nGas, double c[7]);
 BoFluid makeFluid(const omodels_blackoil_problem* p);
+// a piecewise-linear table of n in [2, kMaxTab] nodes with increasing x
+void fillTab(Tab& t, const double* x, const double* y, int n);
 void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::vector<double>& poro,
               std::vector<double>& pv, std::vector<uint8_t>& mean, std::vector<int32_t>& srcCell,
               std::vector<int32_t>& srcType, std::vector<double>& srcA, std::vector<double>& srcB);
@@ -128,7 +130,17 @@ public:
     std::vector<double> transGrid;       // the grid's own transmissibilities
     std::vector<double> depthHost;       // the caller's depths (empty: the grid's own)
     DevBuf<double> geoDepth, geoVol, geoThp;
-    BoGeo geo{};                         // geo.depth == nullptr: the grid's own
+    bool geoGiven = false;               // depths, volumes or thresholds from the caller
+    BoGeo geo{};                         // geo.depth == nullptr: the grid's own, no rock terms
+    // rockCompressibility / rockReferencePressure per cell (comp, pref) and the
+    // rockCompPoroMultiplier / rockCompTransMultiplier tables of the oil
+    // pressure (table [3][nTab]: p, pore-volume and transmissibility
+    // multipliers), blackoilintensivequantities.hh:447-480; all null / 0: none
+    void setRock(const double* comp, const double* pref, int nTab, const double* table);
+    DevBuf<double> rockComp, rockPref;
+    DevBuf<Tab> rockTabs;
+    // the kernels' BoGeo from the given geometry and the rock terms
+    void refreshGeo();
     void ensureJacobian();
     void eraseMatrix() { jac.release(); }
     // state management (FvBaseDiscretization::advanceTimeLevel / updateFailed)

@@ -31,14 +31,16 @@ struct BoGrid {
 // The linearizer's per-neighbour inputs given by the caller instead of the
 // structured grid's own (tpfalinearizer.hh:443-452): each cell's centre depth
 // (dofCenterDepth) and total volume (dofTotalVolume), the threshold pressure
-// per face ([3][N] at the transmissibility's index, zeros where none), and the
-// time step of the linearization (storefac = volume / dt, :776-778).
-// depth == nullptr: the grid's own (the kernels' plain form).
+// per face ([3][N] at the transmissibility's index, zeros where none), the
+// time step of the linearization (storefac = volume / dt, :776-778) and the
+// rock's compressibility and multipliers (BoRock).  depth == nullptr: the
+// grid's own geometry and no rock terms (the kernels' plain form).
 struct BoGeo {
     const double* depth;
     const double* vol;
     const double* thp;
     double dt;
+    BoRock rock;
 };
 
 struct NewtonUpdateParams {
@@ -162,7 +164,7 @@ void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const 
                      const double* a, const double* b, double* res, double* jac, const int* diagOff, hipStream_t s,
                      const double* vol = nullptr);
 void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
-                     const double* poro, double* st, hipStream_t s);
+                     const double* poro, double* st, hipStream_t s, const BoRock* rock = nullptr);
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
                           double* outRes, double* outDiag, hipStream_t s, const double* vol = nullptr);
@@ -171,7 +173,7 @@ void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const
                    const BoDisp* dsp = nullptr, const BoGeo* geo = nullptr);
 constexpr int kOutFields = 18;
 void launchBoOutput(const BoFluid* fdev, long N, const double* pv, const uint8_t* mean, const double* poro,
-                    double* out, hipStream_t s);
+                    double* out, hipStream_t s, const BoRock* rock = nullptr);
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,
                             double* jac, const int* diagOff, hipStream_t s);
 void launchBoUpdate(const BoFluid& f, long N, double* pv, uint8_t* mean, uint8_t* wasSw,

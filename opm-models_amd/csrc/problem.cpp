@@ -46,15 +46,15 @@ void defaultFluid(std::vector<double>& oil, int& nOil, std::vector<double>& gas,
     c[4] = 1037.0; c[5] = 786.0; c[6] = 0.97;   // rho_ref water, oil, gas
 }
 
-static void fillTab(Tab& t, const double* x, const double* y, int n)
+void fillTab(Tab& t, const double* x, const double* y, int n)
 {
-    if (n < 2 || n > kMaxTab) throw std::invalid_argument("PVT table size must be in [2, 12]");
+    if (n < 2 || n > kMaxTab) throw std::invalid_argument("table size must be in [2, 12]");
     t.n = n;
     for (int i = 0; i < kMaxTab; ++i) { t.x[i] = 0.0; t.y[i] = 0.0; t.s[i] = 0.0; }
     for (int i = 0; i < n; ++i) { t.x[i] = x[i]; t.y[i] = y[i]; }
     for (int i = 0; i + 1 < n; ++i) t.s[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
     for (int i = 1; i < n; ++i)
-        if (!(x[i] > x[i - 1])) throw std::invalid_argument("PVT table abscissae must increase");
+        if (!(x[i] > x[i - 1])) throw std::invalid_argument("table abscissae must increase");
 }
 
 BoFluid makeFluid(const omodels_blackoil_problem* p)

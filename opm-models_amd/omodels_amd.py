@@ -111,6 +111,7 @@ SIGNATURES = {
     "omodels_linearizer_pattern": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp]),
     "omodels_linearizer_linearize_domain": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "omodels_linearizer_update_discretization": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    "omodels_model_set_rock": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int32, _dp]),
     "omodels_linearizer_get_jacobian": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_get_residual": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_erase_matrix": (C.c_int, [C.c_void_p]),
@@ -421,6 +422,22 @@ class Model:
         self._geo = arrs        # the library copies them; kept for the caller's inspection
         check(lib().omodels_linearizer_update_discretization(
             self.h, *[ptr(a) if a is not None else None for a in arrs]), "update_discretization")
+
+    def set_rock(self, comp=None, pref=None, table=None):
+        """rock compressibility [N] (1/Pa), reference pressures [N] (Pa) and
+        the multiplier table [3][n] (p_o, pore-volume multiplier,
+        transmissibility multiplier); None for the problem's defaults."""
+        N = self.N
+        c = None if comp is None else np.ascontiguousarray(comp, dtype=np.float64).ravel()
+        p = None if pref is None else np.ascontiguousarray(pref, dtype=np.float64).ravel()
+        t = None if table is None else np.ascontiguousarray(table, dtype=np.float64).ravel()
+        for a in (c, p):
+            if a is not None and a.size != N:
+                raise ValueError("set_rock: per-cell arrays of %d values" % N)
+        if t is not None and t.size % 3:
+            raise ValueError("set_rock: the table holds three rows")
+        n = 0 if t is None else t.size // 3
+        check(lib().omodels_model_set_rock(self.h, ptr(c), ptr(p), n, ptr(t)), "set_rock")
 
     def linearize(self, dt, iteration_index=1):
         check(lib().omodels_linearizer_linearize_domain(self.h, dt, iteration_index), "linearize_domain")

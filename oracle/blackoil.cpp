@@ -99,7 +99,7 @@ struct Pvt {
  * blackoilintensivequantities.hh:179-516 for the Sw/Po/{Sg,Rs} meanings. */
 template <class T>
 struct IQ {
-    T S[3], p[3], kr[3], invB[3], mu[3], lam[3], rho[3], Rs, phi;
+    T S[3], p[3], kr[3], invB[3], mu[3], lam[3], rho[3], Rs, phi, tm;
 };
 
 inline double valueOf(double x) { return x; }
@@ -158,8 +158,65 @@ void computeIQ(const Pvt& pvt, const double* pv, int meaning, double poro, IQ<T>
     q.rho[W] = q.invB[W] * pvt.rho[W];
     q.rho[G] = q.invB[G] * pvt.rho[G];
     q.rho[O] = q.invB[O] * pvt.rho[O] + (q.invB[O] * q.Rs) * pvt.rho[G];
-    /* porosity: no rock compressibility, multiplier 1 (blackoilproblem.hh) */
+    /* porosity; without rock terms the problem's defaults (compressibility
+     * 0, multipliers 1, blackoilproblem.hh:131-181) */
     q.phi = T(poro);
+    q.tm = T(1.0);
+}
+
+/* Rock compressibility and compaction (blackoilintensivequantities.hh:447-
+ * 480): porosity *= 1 + x + 0.5 x^2 with x = c (p_o - p_ref) where c > 0,
+ * then *= rockCompPoroMultiplier(p_o); rockCompTransMultiplier(p_o) for the
+ * face term. */
+struct Rock {
+    const double* comp = nullptr;
+    const double* pref = nullptr;
+    bool tab = false;
+    Table poroMult, transMult;
+};
+
+inline bool rockOn(const ob_problem* p) { return p->rock_comp || p->n_rock > 0; }
+inline Rock makeRock(const ob_problem* p)
+{
+    Rock r;
+    const long N = (long)p->nx * p->ny * p->nz;
+    if (p->rock_comp)
+        for (long I = 0; I < N; ++I)
+            if (!(p->rock_comp[I] >= 0.0) || !std::isfinite(p->rock_comp[I]))
+                throw std::invalid_argument("rock compressibilities must be finite and >= 0");
+    if (p->rock_pref)
+        for (long I = 0; I < N; ++I)
+            if (!std::isfinite(p->rock_pref[I])) throw std::invalid_argument("rock reference pressures must be finite");
+    r.comp = p->rock_comp;
+    r.pref = p->rock_pref;
+    if (p->n_rock > 0) {
+        const int n = p->n_rock;
+        if (n < 2 || !p->rock_tab) throw std::invalid_argument("rock table needs at least two nodes");
+        for (int k = 0; k < 2 * n; ++k)
+            if (!(p->rock_tab[n + k] > 0.0) || !std::isfinite(p->rock_tab[n + k]))
+                throw std::invalid_argument("rock multipliers must be finite and > 0");
+        for (int k = 1; k < n; ++k)
+            if (!(p->rock_tab[k] > p->rock_tab[k - 1])) throw std::invalid_argument("rock table pressures must increase");
+        r.tab = true;
+        r.poroMult.set(p->rock_tab, p->rock_tab + n, n);
+        r.transMult.set(p->rock_tab, p->rock_tab + 2 * n, n);
+    }
+    return r;
+}
+
+template <class T>
+void applyRock(const Rock& r, long cell, IQ<T>& q)
+{
+    const double c = r.comp ? r.comp[cell] : 0.0;
+    if (c > 0.0) {
+        const double pref = r.pref ? r.pref[cell] : 1e5;
+        const T x = (q.p[O] - pref) * c;
+        q.phi = q.phi * ((x + 1.0) + (x * 0.5) * x);
+    }
+    if (r.tab) {
+        q.phi = q.phi * r.poroMult.eval(q.p[O]);
+        q.tm = r.transMult.eval(q.p[O]);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -308,8 +365,11 @@ struct Dispersion {
 void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, long J,
               const NbInfo& nbi, double Vin, double Vex, E3 flux[3], double* darcyOut = nullptr,
               const Diffusion* dif = nullptr, const Dispersion* dsp = nullptr, double nvI = 0.0,
-              double nvJ = 0.0)
+              double nvJ = 0.0, bool tmOn = false)
 {
+    /* the mean of the two cells' rockCompTransMultiplier
+     * (blackoillocalresidualtpfa.hh:372-381) */
+    const E3 transMult = (in.tm + ex.tm) / 2.0;
     for (int e = 0; e < 3; ++e) flux[e] = E3(0.0);
     if (darcyOut) for (int e = 0; e < 3; ++e) darcyOut[e] = 0.0;
     for (int a = 0; a < 3; ++a) {
@@ -333,10 +393,10 @@ void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, lo
         const double tf = (-nbi.trans) / nbi.area;
         E3 darcy, svf;
         if (upIn) {
-            darcy = (dp * in.lam[a]) * tf;
+            darcy = tmOn ? ((dp * in.lam[a]) * transMult) * tf : (dp * in.lam[a]) * tf;
             svf = in.invB[a] * darcy;
         } else {
-            darcy = dp * (ex.lam[a] * tf);
+            darcy = tmOn ? dp * ((transMult * ex.lam[a]) * tf) : dp * (ex.lam[a] * tf);
             svf = darcy * ex.invB[a];
         }
         flux[compOfPhase[a]] = flux[compOfPhase[a]] + svf * pvt.rho[a];
@@ -412,6 +472,15 @@ struct Problem {
     /* the problem's source terms per cell: the first source of the cell and
      * the next source of the same cell (-1: none), in the list's order */
     std::vector<int> srcOfCell, srcNext;
+    bool hasRock = false;
+    Rock rock;
+    /* the intensive quantities of cell I with the rock terms */
+    template <class T>
+    void iq(const double* pv, const uint8_t* mean, long I, IQ<T>& q) const
+    {
+        computeIQ<T>(pvt, pv + 3 * I, mean[I], P->poro[I], q);
+        if (hasRock) applyRock(rock, I, q);
+    }
     bool hasDiff = false;
     Diffusion dif{};
     bool hasDisp = false;
@@ -429,6 +498,8 @@ struct Problem {
                 if (!(p->volume[I] > 0.0) || !std::isfinite(p->volume[I]))
                     throw std::invalid_argument("cell volumes must be finite and > 0");
         pvt.init(p);
+        hasRock = rockOn(p);
+        if (hasRock) rock = makeRock(p);
         buildPattern(G, pat);
         srcOfCell.assign(G.N, -1);
         srcNext.assign(p->n_src, -1);
@@ -516,7 +587,7 @@ void computeStorageAll(const Problem& pr, const double* pv, const uint8_t* mean,
 {
     for (long I = 0; I < pr.G.N; ++I) {
         IQ<double> q;
-        computeIQ<double>(pr.pvt, pv + 3 * I, mean[I], pr.P->poro[I], q);
+        pr.iq<double>(pv, mean, I, q);
         double s[3];
         storageOf<double>(pr.pvt, q, s);
         for (int e = 0; e < 3; ++e) st[3 * I + e] = s[e];
@@ -545,7 +616,7 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
 #endif
     for (long I = 0; I < N; ++I) {
         IQ<E3> qi;
-        computeIQ<E3>(pr.pvt, pv + 3 * I, mean[I], pr.P->poro[I], qi);
+        pr.iq<E3>(pv, mean, I, qi);
         double* r = res + 3 * I;
         double* D = jac + 9L * pr.pat.diagOff[I];
         const double V = pr.G.volume(I);
@@ -553,16 +624,17 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
         for (int k = pr.pat.nbPtr[I]; k < pr.pat.nbPtr[I + 1]; ++k) {
             const NbInfo& nbi = pr.pat.nb[k];
             IQ<double> qj;
-            computeIQ<double>(pr.pvt, pv + 3 * nbi.J, mean[nbi.J], pr.P->poro[nbi.J], qj);
+            pr.iq<double>(pv, mean, nbi.J, qj);
             E3 f[3];
             if (pr.hasDisp) {
                 double darcy[3];
                 faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, darcy, pr.diffusion(), &pr.dsp,
-                         nvIn[I], nvIn[nbi.J]);
+                         nvIn[I], nvIn[nbi.J], pr.rock.tab);
                 const double v = std::fabs(darcy[CO] / nbi.area);
                 nvOut[I] = nvOut[I] < v ? v : nvOut[I];
             } else {
-                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, nullptr, pr.diffusion());
+                faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, nullptr, pr.diffusion(), nullptr,
+                         0.0, 0.0, pr.rock.tab);
             }
             double* O_ = jac + 9L * nbi.off;
             for (int e = 0; e < 3; ++e) {
@@ -1257,14 +1329,19 @@ int ob_build_pattern(const ob_problem* p, int32_t* row_ptr, int32_t* col_idx, in
 
 int ob_storage(const ob_problem* p, const double* pv, const uint8_t* meaning, double* storage)
 {
-    Grid G(p);
-    Pvt pvt; pvt.init(p);
-    for (long I = 0; I < G.N; ++I) {
-        IQ<double> q;
-        computeIQ<double>(pvt, pv + 3 * I, meaning[I], p->poro[I], q);
-        storageOf<double>(pvt, q, storage + 3 * I);
-    }
-    return 0;
+    try {
+        Grid G(p);
+        Pvt pvt; pvt.init(p);
+        const bool on = rockOn(p);
+        const Rock rock = on ? makeRock(p) : Rock{};
+        for (long I = 0; I < G.N; ++I) {
+            IQ<double> q;
+            computeIQ<double>(pvt, pv + 3 * I, meaning[I], p->poro[I], q);
+            if (on) applyRock(rock, I, q);
+            storageOf<double>(pvt, q, storage + 3 * I);
+        }
+        return 0;
+    } catch (...) { return -1; }
 }
 
 int ob_intensive(const ob_problem* p, const double* pv3, int meaning, int cell, double* out)
@@ -1272,6 +1349,7 @@ int ob_intensive(const ob_problem* p, const double* pv3, int meaning, int cell, 
     Pvt pvt; pvt.init(p);
     IQ<E3> q;
     computeIQ<E3>(pvt, pv3, meaning, p->poro[cell], q);
+    if (rockOn(p)) applyRock(makeRock(p), cell, q);
     const E3* f[13] = {&q.p[0], &q.rho[W], &q.rho[O], &q.rho[G], &q.lam[W], &q.lam[O], &q.lam[G],
                        &q.invB[W], &q.invB[O], &q.invB[G], &q.Rs, &q.phi, &q.S[O]};
     for (int k = 0; k < 13; ++k) {
@@ -1294,7 +1372,7 @@ int ob_source_blocks(const ob_problem* p, const double* pv, const uint8_t* meani
             const long I = p->src_cell[s];
             const double V = pr.G.volume(I);
             IQ<E3> qi;
-            computeIQ<E3>(pr.pvt, pv + 3 * I, meaning[I], p->poro[I], qi);
+            pr.iq<E3>(pv, meaning, I, qi);
             E3 q[3];
             sourceOf(pr.pvt, qi, p->src_type[s], p->src_a[s], p->src_b[s], V, q);
             cells[s] = (int32_t)I;
@@ -1324,16 +1402,15 @@ int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersi
         if (dsp && !nv) throw std::invalid_argument("dispersion needs the cells' velocities");
         for (long I = 0; I < pr.G.N; ++I) {
             IQ<E3> qi;
-            computeIQ<E3>(pr.pvt, pv + 3 * I, meaning[I], p->poro[I], qi);
+            pr.iq<E3>(pv, meaning, I, qi);
             for (int k = pr.pat.nbPtr[I]; k < pr.pat.nbPtr[I + 1]; ++k) {
                 const NbInfo& nbi = pr.pat.nb[k];
                 IQ<double> qj;
-                computeIQ<double>(pr.pvt, pv + 3 * nbi.J, meaning[nbi.J], p->poro[nbi.J], qj);
+                pr.iq<double>(pv, meaning, nbi.J, qj);
                 E3 f[3];
                 double d[3];
                 faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, pr.G.volume(I), pr.G.volume(nbi.J), f, d, pr.diffusion(),
-                         pr.dispersion(),
-                         dsp ? nv[I] : 0.0, dsp ? nv[nbi.J] : 0.0);
+                         pr.dispersion(), dsp ? nv[I] : 0.0, dsp ? nv[nbi.J] : 0.0, pr.rock.tab);
                 for (int e = 0; e < 3; ++e) {
                     if (flows) flows[3L * k + e] = f[e].v;
                     if (flores) flores[3L * k + e] = d[e];

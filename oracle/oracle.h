@@ -54,6 +54,14 @@ typedef struct ob_problem {
     const double* depth;
     const double* volume;
     const double* thpres;
+    /* rock compressibility and compaction (blackoilintensivequantities.hh:
+     * 447-480): rock_comp [N] (NULL: 0), rock_pref [N] (NULL: 1e5 Pa),
+     * rock_tab [3][n_rock]: p_o, pore-volume and transmissibility multipliers
+     * (n_rock 0: none) */
+    const double* rock_comp;
+    const double* rock_pref;
+    int32_t n_rock;
+    const double* rock_tab;
 } ob_problem;
 
 /* Newton / linear-solver parameters; defaults are the reference's

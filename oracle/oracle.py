@@ -29,6 +29,7 @@ class ObProblem(C.Structure):
         ("water_mu", C.c_double), ("rho_ref", C.c_double * 3),
         ("src_cell", _ip), ("src_type", _ip), ("src_a", _dp), ("src_b", _dp),
         ("face_trans", _dp), ("depth", _dp), ("volume", _dp), ("thpres", _dp),
+        ("rock_comp", _dp), ("rock_pref", _dp), ("n_rock", C.c_int32), ("rock_tab", _dp),
     ]
 
 
@@ -178,6 +179,23 @@ def with_discretization(prob, trans=None, depth=None, volume=None, thpres=None):
         keep.append(a)
         setattr(prob, name, P(a))
     prob._geo_keep = keep
+    return prob
+
+
+def with_rock(prob, comp=None, pref=None, table=None):
+    """prob with rock compressibilities / reference pressures ([N]) and the
+    multiplier table ([3][n]: p, pore-volume, transmissibility multipliers);
+    None for none.  The arrays are kept alive by the returned problem."""
+    keep = []
+    for name, a in (("rock_comp", comp), ("rock_pref", pref), ("rock_tab", table)):
+        if a is None:
+            setattr(prob, name, None)
+            continue
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        keep.append(a)
+        setattr(prob, name, P(a))
+    prob.n_rock = 0 if table is None else len(np.ravel(table)) // 3
+    prob._rock_keep = keep
     return prob
 
 
