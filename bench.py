@@ -80,7 +80,9 @@ def algorithmic_bytes(N, F, nnzb, n=3, s=8):
 
 def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=False):
     """Poisson stream of uneven black-oil realizations through the server
-    (progress: the server prints a line per finished request)."""
+    with each batched BiCGStab iteration replayed from a hipGraph, then the
+    same stream without the graphs and, request by request, through the plain
+    FP64 path (progress: the server prints a line per finished request)."""
     rng = np.random.default_rng(seed)
     cases = []
     for k in range(n_req):
@@ -94,18 +96,27 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=Fals
     dt = 86400.0
     # the reference's time-step control with steps of at most one day: a step
     # whose Newton solve fails is retried at half the size
-    prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1, ilu_order=1, verbose=1 if progress else 0)
     budget = 64 * 2.0 ** 30
-    srv = om.Server(prm, budget, 64)
-    for c, a in zip(cases, arrivals):
-        srv.submit(c, n_steps, dt, float(a))
-    t0 = time.time()
-    srv.run()
-    wall = time.time() - t0
-    res = [srv.result(i) for i in range(n_req)]
+
+    def served(graph):
+        prm = om.default_params(max_dt=dt, newton_tol=1e-7, precision=1, ilu_order=1, graph=graph,
+                                verbose=1 if progress else 0)
+        srv = om.Server(prm, budget, 64)
+        for c, a in zip(cases, arrivals):
+            srv.submit(c, n_steps, dt, float(a))
+        t0 = time.time()
+        srv.run()
+        wall = time.time() - t0
+        res = [srv.result(i) for i in range(n_req)]
+        summ = srv.summary()
+        srv.close()
+        return wall, res, summ
+
+    wall0, res0, _ = served(0)
+    wall, res, summ = served(1)
     lat = np.array([r["finish"] - r["arrival"] for r in res])
-    summ = srv.summary()
-    srv.close()
+    lat0 = np.array([r["finish"] - r["arrival"] for r in res0])
+    same = all(np.array_equal(a["pv"].view(np.uint64), b["pv"].view(np.uint64)) for a, b in zip(res, res0))
     out = {
         "requests": n_req, "steps_per_request": n_steps, "max_dt_days": 1.0,
         "cells_min": int(min(c.N for c in cases)), "cells_max": int(max(c.N for c in cases)),
@@ -113,10 +124,15 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=Fals
         "arrival_rate_per_s": rate, "precision": "fp32-inner/fp64-outer",
         "precond": "block-ILU0, two-colour 4x2x8 tiles", "newton_tol": 1e-7,
         "hbm_budget_GiB": 64, "max_batch": 64, "max_running": summ["max_running"],
-        "batch_steps": summ["batch_steps"],
+        "batch_newton_rounds": summ["batch_rounds"], "stepping": "per member (rounds of one Newton iteration)",
+        "graph": "each batched BiCGStab half-iteration replayed from a hipGraph",
         "wall_s": round(wall, 3), "solves_per_s": round(n_req / wall, 4),
         "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+        "without_graph": {"wall_s": round(wall0, 3), "solves_per_s": round(n_req / wall0, 4),
+                          "p50_latency_s": round(float(np.percentile(lat0, 50)), 3),
+                          "p99_latency_s": round(float(np.percentile(lat0, 99)), 3),
+                          "results_bit_identical": bool(same)},
         "failed": int(sum(1 for r in res if r["status"] != 0)),
         "newton_iters": int(sum(r["stats"].newton_iters for r in res)),
         "linear_iters": int(sum(r["stats"].linear_iters for r in res)),
@@ -249,7 +265,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-steps", type=int, default=10)
-    ap.add_argument("--serve-plain", type=int, default=4)
+    ap.add_argument("--serve-plain", type=int, default=32)
     ap.add_argument("--serve-progress", action="store_true", help="a line per finished request (long streams)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -404,12 +420,35 @@ def main():
         cw = time.time() - t0
         # every record but the converged check that ends each time step is a Newton update
         cn = len(logo) - args.cpu_steps
+        # the assembly alone, median of 5 linearizations on all threads used
+        # and on one (the linear solve is sequential in both)
+        st1 = orc.storage(prob, case.pv0, case.mean0)
+
+        def asm_rate(nt):
+            ts = []
+            for _ in range(5):
+                ta = time.time()
+                orc.linearize(prob, case.pv0, case.mean0, st1, dt, nthreads=nt)
+                ts.append(time.time() - ta)
+            return round(N / float(np.median(ts)) / 1e6, 2)
+
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                for ln in f:
+                    if ln.startswith("model name"):
+                        cpu_model = ln.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
         line["cpu_baseline"] = {
             "value": round(cn / cw, 4), "unit": "newton_iters/s", "cores": threads, "kind": "port",
             "sample": "%d time step(s) of the same 500k workload from the initial state, oracle (C++ restatement): "
                       "OpenMP assembly on %d threads, sequential ILU0/BiCGStab as the reference" % (args.cpu_steps, threads),
             "wall_s": round(cw, 2), "newton_iters": cn,
-            "assembly_mcells_per_s": round(N * (cn + args.cpu_steps) / max(tim[0], 1e-9) / 1e6, 2),
+            "assembly_mcells_per_s": asm_rate(threads),
+            "assembly_mcells_per_s_1_thread": asm_rate(1),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model,
         }
     if not args.no_serve:
         line["ensemble"] = serve_workload(om, np, args.serve_requests, args.serve_rate, args.serve_steps,

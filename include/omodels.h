@@ -385,8 +385,11 @@ int omodels_simulate(omodels_model* m, const omodels_params* prm, int n_steps, d
  * Newton/time-step control (newtonmethod.hh:228-465, fvbaseproblem.hh:512-606)
  * while the kernels of all members run in the same launches.  Requests are
  * admitted in arrival order when their device memory fits hbm_budget_bytes
- * and at most max_batch run at once; a request that arrives while others run
- * joins at the next time-step boundary.  All members share one fluid. */
+ * and at most max_batch run at once.  The batch advances in rounds of one
+ * Newton iteration per running member; each member starts and ends its time
+ * steps on its own (no member waits for another's step), a finished request
+ * leaves and an arrived one joins between rounds.  All members share one
+ * fluid. */
 typedef struct omodels_server omodels_server;
 omodels_server* omodels_server_create(const omodels_params* prm, double hbm_budget_bytes, int max_batch);
 void omodels_server_destroy(omodels_server* s);
@@ -401,7 +404,8 @@ int omodels_server_run(omodels_server* s);
 int omodels_server_result(omodels_server* s, int id, double* pv, uint8_t* meaning, double* times,
                           double* dt_hist, omodels_run_stats* stats);
 int omodels_server_log(omodels_server* s, int id, omodels_iter_log* log, int cap, int32_t* len);
-int omodels_server_summary(omodels_server* s, double* wall_s, int64_t* batch_steps, int32_t* max_running);
+/* wall time of the last run, batched Newton rounds, most members at once */
+int omodels_server_summary(omodels_server* s, double* wall_s, int64_t* batch_rounds, int32_t* max_running);
 /* Test hook for the failure handling of the tiled ILU's bounded dependency
  * waits (ilu_order 0): sets the error flag that a wait which ran into its
  * bound sets, so the waits of the next batched solve give up at once; every

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kT) void k_e_copy(const EnsInst* __restrict__ insts
 template <class S, int NACC>
 __global__ __launch_bounds__(kT) void k_e_xdot(const EnsInst* __restrict__ insts, const int4* __restrict__ items,
     const int* __restrict__ act, int ia, int ib, ExactAcc* __restrict__ accAll,
-    unsigned* __restrict__ tickets, KrylovScalars* __restrict__ ksAll, int mode, double eps, int tag)
+    unsigned* __restrict__ tickets, KrylovScalars* __restrict__ ksAll, int mode, double eps, int tagOff)
 {
     const int4 it = items[blockIdx.x];
     if (!act[it.x]) return;
@@ -132,6 +132,8 @@ __global__ __launch_bounds__(kT) void k_e_xdot(const EnsInst* __restrict__ insts
     const double s1 = NACC == 2 ? finishExact(&acc[1], sD, sMeta) : 0.0;
     if (threadIdx.x != 0) return;
     KrylovScalars* ks = ksAll + it.x;
+    // failures carry the iteration's tag (tag_base = 2 it, advanced on the device)
+    const int tag = ks->tag_base + tagOff;
     if (mode == 0) {
         ks->rho_i = s0;
         if (fabs(ks->rho) <= eps || fabs(ks->omega) <= eps) markFail(ks, tag);
@@ -255,14 +257,18 @@ __global__ __launch_bounds__(kT) void k_e_update_xr2(const EnsInst* __restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = nz;
 }
 
+// the last kernel of an iteration: the stagnation verdict of the second half,
+// then the next iteration's tag base (so a replayed graph needs no new
+// arguments per iteration)
 __global__ __launch_bounds__(kT) void k_e_stag_final(const int* __restrict__ first, const int* __restrict__ act,
-    const double* __restrict__ part, KrylovScalars* ksAll, int tag)
+    const double* __restrict__ part, KrylovScalars* ksAll, int tagOff)
 {
     const int b = blockIdx.x;
     if (!act[b]) return;
     const double nz = instReduce<true>(part, first[b], first[b + 1]);
     if (threadIdx.x != 0) return;
-    if (nz == 0.0 && ksAll[b].stag_full_tag < 0) ksAll[b].stag_full_tag = tag;
+    if (nz == 0.0 && ksAll[b].stag_full_tag < 0) ksAll[b].stag_full_tag = ksAll[b].tag_base + tagOff;
+    ksAll[b].tag_base += 2;
 }
 
 // y = A x, A = FP64 Jacobian (useFloat = 0) or its FP32 copy
@@ -418,12 +424,12 @@ EnsOps::EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* 
 template <class S> void EnsOps::copy(const int* act, int dst, int src)
 { if (nChunks_) hipLaunchKernelGGL((k_e_copy<S>), EGRID, insts_, table_, act, dst, src); }
 
-template <class S> void EnsOps::dot(const int* act, int ia, int ib, int op, int tag)
+template <class S> void EnsOps::dot(const int* act, int ia, int ib, int op, int tagOff)
 {
     if (!nChunks_) return;
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
     hipLaunchKernelGGL((k_e_xdot<S, 1>), DGRID, insts_, dots_.items, act, ia, ib, dots_.acc, dots_.tickets, ks_, op,
-                       eps, tag);
+                       eps, tagOff);
 }
 
 template <class S> void EnsOps::updateP(const int* act)
@@ -436,19 +442,19 @@ template <class S> void EnsOps::updateXR(const int* act)
     hipLaunchKernelGGL(k_e_crit_final, IGRID, first_, act, part_, nChunks_, ks_);
 }
 
-template <class S> void EnsOps::dot2(const int* act, int tag)
+template <class S> void EnsOps::dot2(const int* act, int tagOff)
 {
     if (!nChunks_) return;
     const double eps = (double)std::numeric_limits<S>::min() * 1e10;
     hipLaunchKernelGGL((k_e_xdot<S, 2>), DGRID, insts_, dots_.items, act, kVy, kVr, dots_.acc, dots_.tickets, ks_, 2,
-                       eps, tag);
+                       eps, tagOff);
 }
 
-template <class S> void EnsOps::updateXR2(const int* act, int tag)
+template <class S> void EnsOps::updateXR2(const int* act, int tagOff)
 {
     if (!nChunks_) return;
     hipLaunchKernelGGL((k_e_update_xr2<S>), EGRID, insts_, table_, act, ks_, part_);
-    hipLaunchKernelGGL(k_e_stag_final, IGRID, first_, act, part_, ks_, tag);
+    hipLaunchKernelGGL(k_e_stag_final, IGRID, first_, act, part_, ks_, tagOff);
 }
 
 template <class S> void EnsOps::spmv(const int* act, int ix, int iy)

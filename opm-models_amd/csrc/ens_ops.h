@@ -32,11 +32,11 @@ public:
     EnsOps(const EnsInst* insts, const int2* table, int nChunks, const int* first, int nInst, double* part,
            const EnsDots& dots, KrylovScalars* ks, hipStream_t s);
     template <class S> void copy(const int* act, int dst, int src);   // src < 0: zero
-    template <class S> void dot(const int* act, int ia, int ib, int op, int tag);
+    template <class S> void dot(const int* act, int ia, int ib, int op, int tagOff);
     template <class S> void updateP(const int* act);
     template <class S> void updateXR(const int* act);
-    template <class S> void dot2(const int* act, int tag);
-    template <class S> void updateXR2(const int* act, int tag);
+    template <class S> void dot2(const int* act, int tagOff);
+    template <class S> void updateXR2(const int* act, int tagOff);
     template <class S> void spmv(const int* act, int ix, int iy);
     template <class S> void maxAbs(const int* act, int id, int useRes, double* out);
     void copyMatrix(const int* act, int toFloat);

@@ -357,19 +357,63 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
     const auto rInsts = colored() ? std::vector<IluInst>() : iluInsts(a, fl, kVr, kVz);
     const auto pCol = colored() ? colInsts(a, fl, kVp, kVy) : std::vector<ColInst>();
     const auto rCol = colored() ? colInsts(a, fl, kVr, kVz) : std::vector<ColInst>();
-    int it = 0;
-    for (; it < maxIter; ++it) {
-        const int tag = 2 * it;
-        ops.dot<S>(a, kVb, kVr, 0, tag);
+    // first half: rho, p, y = M^-1 p, v = A y, alpha, x/r update; second
+    // half: z = M^-1 r, y = A z, omega, x/r update and the next iteration's
+    // tag base (device scalars: no argument changes from one iteration to
+    // the next, so either half can be replayed from a graph)
+    auto firstHalf = [&]() {
+        ops.dot<S>(a, kVb, kVr, 0, 0);
         ops.updateP<S>(a);
-        if (prm_.precond != 0) throw std::invalid_argument("the ensemble engine uses the block-ILU0 preconditioner");
         timer.begin(kClsIluApply, stream_);
         if (colored()) col_->apply<S>(pCol, 0, stream_);
         else ilu_->apply<S>(pInsts, stream_);
         timer.end(stream_);
         ops.spmv<S>(a, kVy, kVv);
-        ops.dot<S>(a, kVb, kVv, 1, tag);
+        ops.dot<S>(a, kVb, kVv, 1, 0);
         ops.updateXR<S>(a);
+    };
+    auto secondHalf = [&]() {
+        timer.begin(kClsIluApply, stream_);
+        if (colored()) col_->apply<S>(rCol, 1, stream_);
+        else ilu_->apply<S>(rInsts, stream_);
+        timer.end(stream_);
+        ops.spmv<S>(a, kVz, kVy);
+        ops.dot2<S>(a, 1);
+        ops.updateXR2<S>(a, 1);
+    };
+    // graph replay (params.graph) with the two-colour ILU, whose kernels and
+    // descriptors are all a graph can hold; the halves of iteration 0 run
+    // eagerly (they upload the ILU descriptors), later ones are replayed
+    const bool useGraph = prm_.graph && colored();
+    hipGraphExec_t g1 = nullptr, g2 = nullptr;
+    auto freeGraphs = [&]() {
+        if (g1) (void)hipGraphExecDestroy(g1);
+        if (g2) (void)hipGraphExecDestroy(g2);
+        g1 = g2 = nullptr;
+    };
+    auto runHalf = [&](int half, int it) {
+        if (!useGraph || it == 0) { half == 0 ? firstHalf() : secondHalf(); return; }
+        if (!g1) {
+            const bool timed = timer.enabled;
+            timer.enabled = false;      // event records stay out of the graphs
+            try {
+                g1 = captureGraph(stream_, firstHalf);
+                g2 = captureGraph(stream_, secondHalf);
+            } catch (...) {
+                timer.enabled = timed;
+                freeGraphs();
+                throw;
+            }
+            timer.enabled = timed;
+        }
+        OM_HIP(hipGraphLaunch(half == 0 ? g1 : g2, stream_));
+    };
+    int it = 0;
+    try {
+    for (; it < maxIter; ++it) {
+        const int tag = 2 * it;
+        if (prm_.precond != 0) throw std::invalid_argument("the ensemble engine uses the block-ILU0 preconditioner");
+        runHalf(0, it);
         OM_HIP(hipMemcpyAsync(hostKs_, ks_.p, B * sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream_));
         OM_HIP(hipStreamSynchronize(stream_));
         bool changed = false;
@@ -390,15 +434,14 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
             else any = true;
         }
         if (changed) setFlags(actInner_, act);
-        if (!any) return;
-        timer.begin(kClsIluApply, stream_);
-        if (colored()) col_->apply<S>(rCol, 1, stream_);
-        else ilu_->apply<S>(rInsts, stream_);
-        timer.end(stream_);
-        ops.spmv<S>(a, kVz, kVy);
-        ops.dot2<S>(a, tag + 1);
-        ops.updateXR2<S>(a, tag + 1);
+        if (!any) { freeGraphs(); return; }
+        runHalf(1, it);
     }
+    } catch (...) {
+        freeGraphs();
+        throw;
+    }
+    freeGraphs();
     OM_HIP(hipMemcpyAsync(hostKs_, ks_.p, B * sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream_));
     OM_HIP(hipStreamSynchronize(stream_));
     for (int b = 0; b < B; ++b)
@@ -513,19 +556,21 @@ void Ensemble::linearSolveImpl(const std::vector<int>& set, std::vector<int>& rc
 }
 
 // One lockstep time step of every member that still has steps to do.
-void Ensemble::timeStep()
+void Ensemble::round()
 {
     if (dirty_) rebuild();
     const int B = (int)members_.size();
     if (B == 0) return;
     using clk = std::chrono::steady_clock;
-    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, dots(), ks_.p, stream_);
     enum { kIdle = 0, kNewton = 1, kOk = 2, kFailedStep = 3 };
     std::vector<int> state(B, kIdle);
     for (int b = 0; b < B; ++b) {
         Member& M = *members_[b];
-        if (M.stepsDone < M.stepsTotal && !M.error) { state[b] = kNewton; M.attempt = 0; M.numIter = 0;
-            M.errorN = 1e100; M.lastErrorN = 1e100; }
+        if (M.stepsDone >= M.stepsTotal || M.error) { M.phase = 0; continue; }
+        if (M.phase == 0) {     // the next time step begins
+            M.phase = 1; M.attempt = 0; M.numIter = 0; M.errorN = 1e100; M.lastErrorN = 1e100;
+        }
+        state[b] = kNewton;
     }
     auto restartOrFail = [&](int b) {
         Member& M = *members_[b];
@@ -542,10 +587,11 @@ void Ensemble::timeStep()
         }
         M.dt = ndt < prm_.min_dt ? prm_.min_dt : ndt;
         M.numIter = 0; M.errorN = 1e100; M.lastErrorN = 1e100;
-        state[b] = kNewton;
+        // the retried attempt linearizes in the next round
     };
     const int tgt = prm_.newton_target_iter;
-    for (int round = 0; round < 100000; ++round) {
+    EnsOps ops(desc_.p, chunkTable_.p, nChunks_, first_.p, B, partial_.p, dots(), ks_.p, stream_);
+    do {
         std::vector<int> lin(B, 0), refresh(B, 0);
         bool any = false;
         for (int b = 0; b < B; ++b) if (state[b] == kNewton) { lin[b] = 1; refresh[b] = members_[b]->numIter == 0; any = true; }
@@ -588,7 +634,7 @@ void Ensemble::timeStep()
             solve[b] = 1;
             anySolve = true;
         }
-        if (!anySolve) continue;
+        if (!anySolve) break;
         std::vector<int> rc, its;
         linearSolve(solve, rc, its);
         auto t2 = clk::now();
@@ -608,7 +654,7 @@ void Ensemble::timeStep()
             upd[b] = 1;
             anyUpd = true;
         }
-        if (!anyUpd) continue;
+        if (!anyUpd) break;
         setFlags(actSolve_, upd);
         for (auto& m : members_) OM_HIP(hipMemsetAsync(m->counters.p, 0, 4 * sizeof(int), stream_));
         NewtonUpdateParams up{prm_.dp_max_rel, prm_.ds_max, prm_.privar_osc_thresh};
@@ -627,15 +673,18 @@ void Ensemble::timeStep()
             if (hostI_[2 * b + 1]) { M.numIter = 2 * tgt; restartOrFail(b); continue; }
             M.numIter++;
         }
-    }
-    // advanceTimeLevel and the next step size (suggestTimeStepSize)
+    } while (false);
+    // a step that ended: advanceTimeLevel and the next step size
+    // (suggestTimeStepSize); a member that failed for good stops
     for (int b = 0; b < B; ++b) {
-        if (state[b] != kOk) continue;
         Member& M = *members_[b];
+        if (state[b] == kFailedStep) { M.phase = 0; continue; }
+        if (state[b] != kOk) continue;
         OM_HIP(hipMemcpyAsync(M.pv1.p, M.pv0.p, 3 * M.N * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         OM_HIP(hipMemcpyAsync(M.mean1.p, M.mean0.p, M.N, hipMemcpyDeviceToDevice, stream_));
         M.dtHist.push_back(M.dt);
         M.stepsDone++;
+        M.phase = 0;
         double nd;
         if (M.numIter > tgt) nd = std::max(prm_.min_dt, M.dt / (1.0 + double(M.numIter - tgt) / tgt));
         else nd = std::max(prm_.min_dt, M.dt * (1.0 + double(tgt - M.numIter) / tgt / 1.2));
@@ -728,7 +777,7 @@ void Server::run()
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
             continue;
         }
-        ens_.timeStep();
+        ens_.round();
         steps_++;
         // retire finished members
         for (size_t k = 0; k < running.size();) {

@@ -33,6 +33,7 @@ public:
         DevBuf<float> fv[kNumVec];
         // time stepping and Newton state
         int stepsTotal = 0, stepsDone = 0, attempt = 0, numIter = 0, error = 0;
+        int phase = 0;                   // 0: between time steps, 1: in a step's Newton iterations
         double dt = 0.0, errorN = 1e100, lastErrorN = 1e100;
         omodels_run_stats stats{};
         std::vector<omodels_iter_log> log;
@@ -50,8 +51,11 @@ public:
     void remove(int id);
     Member* find(int id);
     void download(int id, double* pv, uint8_t* mean);
-    // one lockstep time step of every member with steps left
-    void timeStep();
+    // one batched Newton iteration of every member with steps left: a member
+    // between time steps starts its next step here, a member whose step ends
+    // (converged, or failed for good) advances on its own -- no member waits
+    // for another's step to end
+    void round();
     // set the tiled ILU's error flag as an exhausted dependency wait does
     // (tests of the recovery; not with the two-colour ILU)
     void raiseIluError();

@@ -647,29 +647,6 @@ void BiCGStabBackend::IterGraphs::reset()
     generation = -1;
 }
 
-namespace {
-// record the work `body` puts on stream s into an executable graph
-template <class F>
-hipGraphExec_t captureGraph(hipStream_t s, F&& body)
-{
-    OM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipGraph_t g = nullptr;
-    try {
-        body();
-    } catch (...) {
-        (void)hipStreamEndCapture(s, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-    }
-    OM_HIP(hipStreamEndCapture(s, &g));
-    hipGraphExec_t e = nullptr;
-    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    OM_HIP(rc);
-    return e;
-}
-}  // namespace
-
 // BiCGStabSolver::apply (bicgstabsolver.hh:120-330) with CombinedCriterion;
 // one host synchronisation per iteration (the convergence test).  The device
 // scalars carry the iteration's tag base (2 it): the reductions of the first

@@ -33,6 +33,27 @@ void generate(const omodels_blackoil_spec& sp, std::vector<double>& perm, std::v
               std::vector<double>& pv, std::vector<uint8_t>& mean, std::vector<int32_t>& srcCell,
               std::vector<int32_t>& srcType, std::vector<double>& srcA, std::vector<double>& srcB);
 void buildTopology(const omodels_blackoil_problem* p, HostTopology& T);
+
+// record the work `body` puts on stream s into an executable graph
+template <class F>
+hipGraphExec_t captureGraph(hipStream_t s, F&& body)
+{
+    OM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        body();
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    OM_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    OM_HIP(rc);
+    return e;
+}
 // The problem's sources grouped by cell, for the cell-wise source pass:
 // groups[g] = {first, count} into order (source indices in the list's order),
 // one group per cell with sources, in the order of each cell's first source.

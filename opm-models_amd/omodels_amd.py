@@ -673,7 +673,7 @@ class Server:
         bs = C.c_int64()
         mr = C.c_int32()
         check(lib().omodels_server_summary(self.h, C.byref(w), C.byref(bs), C.byref(mr)), "server_summary")
-        return dict(wall=w.value, batch_steps=bs.value, max_running=mr.value)
+        return dict(wall=w.value, batch_rounds=bs.value, max_running=mr.value)
 
     def set_profiling(self, enable=True):
         check(lib().omodels_server_set_profiling(self.h, int(enable)), "server_set_profiling")

@@ -31,14 +31,15 @@ def _serve(members, params, arrivals=None, budget=64e9, max_batch=256, n_steps=3
     return srv, [srv.result(i) for i in ids]
 
 
-@pytest.mark.parametrize("ilu_order", [0, 1])
-def test_ensemble_members_match_the_oracle(ilu_order):
+@pytest.mark.parametrize("ilu_order,graph", [(0, 0), (1, 0), (1, 1)])
+def test_ensemble_members_match_the_oracle(ilu_order, graph):
     """At the reference's settings (NewtonTolerance 1e-6, adaptive steps from
     one day) every member follows the oracle run of that member alone: step
     sizes, iteration counts and final state bit for bit (scalar products are
-    exact sums), with the natural-order and the two-colour ILU(0)."""
+    exact sums), with the natural-order and the two-colour ILU(0), and with
+    the batched BiCGStab iterations replayed from hipGraphs."""
     ms = _members()
-    srv, res = _serve(ms, om.default_params(ilu_order=ilu_order))
+    srv, res = _serve(ms, om.default_params(ilu_order=ilu_order, graph=graph))
     for c, r in zip(ms, res):
         assert r["status"] == 0
         rco, pvo, mo, histo, dtso, logo, tim = orc.simulate(orc.problem_from_case(c), c.pv0, c.mean0, 3, 86400.0,
@@ -52,14 +53,15 @@ def test_ensemble_members_match_the_oracle(ilu_order):
         assert np.array_equal(r["pv"].view(np.uint64), pvo.view(np.uint64))
 
 
-@pytest.mark.parametrize("schedule,ilu_order", [(0, 0), (2, 0), (3, 0), (0, 1)])
+@pytest.mark.parametrize("schedule,ilu_order,graph", [(0, 0, 0), (2, 0, 0), (3, 0, 0), (0, 1, 0), (0, 1, 1)])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule, ilu_order):
+def test_member_result_is_identical_alone_and_in_the_mix(precision, schedule, ilu_order, graph):
     """schedule 0 (batched sweeps for ensembles), 2 / 3 force the streaming /
     batched sweeps, ilu_order 1 the two-colour ILU(0): the arithmetic is the
-    same in every case."""
+    same in every case, also with the BiCGStab iterations replayed from
+    hipGraphs."""
     ms = _members()
-    prm = om.default_params(precision=precision, ilu_schedule=schedule, ilu_order=ilu_order)
+    prm = om.default_params(precision=precision, ilu_schedule=schedule, ilu_order=ilu_order, graph=graph)
     srv, mixed = _serve(ms, prm)
     for k, c in enumerate(ms):
         srv1, alone = _serve([c], prm)
@@ -97,6 +99,23 @@ def test_continuous_batching_admits_arrivals_while_others_run():
     for k, c in enumerate(ms):
         srv1, alone = _serve([c], prm, n_steps=6)
         assert np.array_equal(alone[0]["pv"].view(np.uint64), res[k]["pv"].view(np.uint64))
+
+
+def test_members_step_on_their_own():
+    """The batch runs rounds of one Newton iteration per running member: a
+    member needing fewer iterations for its steps finishes (and leaves) before
+    the slowest one instead of waiting at every step boundary, and the number
+    of rounds is the slowest member's number of linearizations."""
+    ms = _members()
+    srv, res = _serve(ms, om.default_params(), n_steps=4)
+    assert all(r["status"] == 0 for r in res)
+    lins = [r["stats"].linearizations for r in res]
+    assert min(lins) < max(lins)
+    assert srv.summary()["batch_rounds"] == max(lins)
+    slow = int(np.argmax(lins))
+    for k, r in enumerate(res):
+        if lins[k] < lins[slow]:
+            assert r["finish"] < res[slow]["finish"], (k, lins)
 
 
 def test_hbm_budget_limits_the_batch():
