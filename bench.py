@@ -64,6 +64,11 @@ HBM_TRAFFIC_500K = {
 }
 
 
+def note(msg):
+    """a progress line on stderr (the JSON line alone goes to stdout)"""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(N, F, nnzb, n=3, s=8):
     """Per-launch algorithmic HBM bytes of each kernel class."""
     return {
@@ -113,7 +118,9 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=Fals
         return wall, res, summ
 
     wall0, res0, _ = served(0)
+    note("served %d requests without graphs in %.1f s" % (n_req, wall0))
     wall, res, summ = served(1)
+    note("served %d requests with graphs in %.1f s" % (n_req, wall))
     lat = np.array([r["finish"] - r["arrival"] for r in res])
     lat0 = np.array([r["finish"] - r["arrival"] for r in res0])
     same = all(np.array_equal(a["pv"].view(np.uint64), b["pv"].view(np.uint64)) for a, b in zip(res, res0))
@@ -147,6 +154,8 @@ def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=Fals
             m = om.Model(cases[k])
             m.set_solution(cases[k].pv0, cases[k].mean0)
             rc, _, dts, _, _ = m.simulate(pprm, n_steps, dt, keep_history=False)
+            if k % 8 == 7:
+                note("plain FP64: %d of %d requests" % (k + 1, n_plain))
             if rc != 0:
                 raise SystemExit("plain FP64 run of request %d failed rc=%d" % (k, rc))
             finals.append((m.get_solution()[0], dts))
@@ -215,6 +224,7 @@ def beside_runs(om, np, config, dt, steps=2):
     for key, cfg_name, precision, order in [("c3_fp64", "c3", 0, 0)] + runs + [("ref", config, 0, 0)]:
         if key in out:
             continue
+        note("beside: " + key)
         cfg = CONFIGS[cfg_name]
         case = om.BlackOilCase(om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024, **cfg.get("spec", {})))
         prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, ilu_order=order, autotune=1)
@@ -407,6 +417,7 @@ def main():
                                                  "achieved_GBs": round(eb / (eavg * 1e-3) / 1e9, 1),
                                                  "mcells_per_s": round(N / (eavg * 1e-3) / 1e6, 1)}
     if not args.no_extra:
+        note("headline done: %.3f newton_iters/s" % value)
         line["beside"] = beside_runs(om, np, args.config, dt)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -451,6 +462,7 @@ def main():
             "nproc": os.cpu_count(), "cpu_model": cpu_model,
         }
     if not args.no_serve:
+        note("cpu baseline done")
         line["ensemble"] = serve_workload(om, np, args.serve_requests, args.serve_rate, args.serve_steps,
                                           min(args.serve_plain, args.serve_requests),
                                           progress=args.serve_progress)

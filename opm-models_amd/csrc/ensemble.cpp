@@ -71,9 +71,19 @@ Ensemble::Ensemble(const omodels_params& prm, hipStream_t s) : prm_(prm)
     fail_.alloc(1);
 }
 
+void Ensemble::freeGraphs()
+{
+    for (auto& g : graphs_)
+        for (auto& h : g) {
+            if (h) (void)hipGraphExecDestroy(h);
+            h = nullptr;
+        }
+}
+
 Ensemble::~Ensemble()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    freeGraphs();
     members_.clear();
     if (hostKs_) (void)hipHostFree(hostKs_);
     if (hostD_) (void)hipHostFree(hostD_);
@@ -186,6 +196,7 @@ void Ensemble::download(int id, double* pv, uint8_t* mean)
 void Ensemble::rebuild()
 {
     OM_HIP(hipStreamSynchronize(stream_));
+    freeGraphs();       // they hold the old set's pointers
     const int B = (int)members_.size();
     std::vector<EnsInst> d(B);
     std::vector<int2> asmT, chunkT, srcT;
@@ -382,23 +393,25 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
         ops.updateXR2<S>(a, 1);
     };
     // graph replay (params.graph) with the two-colour ILU, whose kernels and
-    // descriptors are all a graph can hold; the halves of iteration 0 run
-    // eagerly (they upload the ILU descriptors), later ones are replayed
+    // descriptors are all a graph needs: captured in the first solve of a
+    // member set that iterates twice, replayed by every later iteration until
+    // the set changes
     const bool useGraph = prm_.graph && colored();
-    hipGraphExec_t g1 = nullptr, g2 = nullptr;
-    auto freeGraphs = [&]() {
-        if (g1) (void)hipGraphExecDestroy(g1);
-        if (g2) (void)hipGraphExecDestroy(g2);
-        g1 = g2 = nullptr;
-    };
+    hipGraphExec_t* gg = graphs_[sizeof(S) == 4 ? 1 : 0];
+    if (useGraph) {
+        col_->prime(pCol, 0, stream_);
+        col_->prime(rCol, 1, stream_);
+    }
+    // before the first capture, iteration 0 runs eagerly: its kernels set up
+    // what they allocate on first use, which a capture must not do
     auto runHalf = [&](int half, int it) {
-        if (!useGraph || it == 0) { half == 0 ? firstHalf() : secondHalf(); return; }
-        if (!g1) {
+        if (!useGraph || (!gg[0] && it == 0)) { half == 0 ? firstHalf() : secondHalf(); return; }
+        if (!gg[0]) {
             const bool timed = timer.enabled;
             timer.enabled = false;      // event records stay out of the graphs
             try {
-                g1 = captureGraph(stream_, firstHalf);
-                g2 = captureGraph(stream_, secondHalf);
+                gg[0] = captureGraph(stream_, firstHalf);
+                gg[1] = captureGraph(stream_, secondHalf);
             } catch (...) {
                 timer.enabled = timed;
                 freeGraphs();
@@ -406,13 +419,11 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
             }
             timer.enabled = timed;
         }
-        OM_HIP(hipGraphLaunch(half == 0 ? g1 : g2, stream_));
+        OM_HIP(hipGraphLaunch(gg[half], stream_));
     };
-    int it = 0;
-    try {
-    for (; it < maxIter; ++it) {
+    if (prm_.precond != 0) throw std::invalid_argument("the ensemble engine uses the block-ILU0 preconditioner");
+    for (int it = 0; it < maxIter; ++it) {
         const int tag = 2 * it;
-        if (prm_.precond != 0) throw std::invalid_argument("the ensemble engine uses the block-ILU0 preconditioner");
         runHalf(0, it);
         OM_HIP(hipMemcpyAsync(hostKs_, ks_.p, B * sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream_));
         OM_HIP(hipStreamSynchronize(stream_));
@@ -434,14 +445,9 @@ void Ensemble::bicgstab(const std::vector<int>& set, const std::vector<double>& 
             else any = true;
         }
         if (changed) setFlags(actInner_, act);
-        if (!any) { freeGraphs(); return; }
+        if (!any) return;
         runHalf(1, it);
     }
-    } catch (...) {
-        freeGraphs();
-        throw;
-    }
-    freeGraphs();
     OM_HIP(hipMemcpyAsync(hostKs_, ks_.p, B * sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream_));
     OM_HIP(hipStreamSynchronize(stream_));
     for (int b = 0; b < B; ++b)

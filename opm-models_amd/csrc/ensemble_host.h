@@ -85,6 +85,10 @@ private:
     std::vector<std::unique_ptr<Member>> members_;
     int nextId_ = 0;
     bool dirty_ = true;
+    // the batched BiCGStab half-iterations as executable graphs, [FP64, FP32][half];
+    // valid until the member set changes
+    hipGraphExec_t graphs_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    void freeGraphs();
     std::unique_ptr<TiledIluSet> ilu_;        // natural order (ilu_order 0)
     std::unique_ptr<ColoredIluSet> col_;      // two-colour tiles (ilu_order 1)
     DevBuf<EnsInst> desc_;

@@ -38,9 +38,13 @@ class ColoredIluSet {
 public:
     void setInstances(const std::vector<ColGeom>& geoms, hipStream_t s);
     void factor(const std::vector<ColInst>& insts, int* fail, hipStream_t s);
-    // slot 0 or 1: two applications with different vectors keep their own descriptors
+    // slot 0 or 1: two applications with different vectors keep their own
+    // descriptors (descriptor slots 1 and 2; slot 0 is the factorization's)
     template <class S>
     void apply(const std::vector<ColInst>& insts, int slot, hipStream_t s);
+    // make the slot's device descriptors those of insts (an application
+    // replayed from a graph uses them without the host's check)
+    void prime(const std::vector<ColInst>& insts, int slot, hipStream_t s) { (void)upload(insts, 1 + slot, s); }
 
 private:
     const ColInst* upload(const std::vector<ColInst>& insts, int slot, hipStream_t s);
