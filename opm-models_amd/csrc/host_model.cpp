@@ -566,11 +566,21 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
         launchExtractInvDiag<float>(N, bs, m_.diagOff.p, luf_.p, dinv64_.p, s);
         m_.timer.end(s);
     } else if (colored_) {
-        // two-colour ILU(0): FP64 pivots only; the sweeps read the off-diagonal
-        // blocks of the solver's matrix (FP64, or its FP32 copy)
+        // two-colour ILU(0): FP64 pivots only; the sweeps and the inner
+        // operator read the solver's matrix in the structured layout (FP64, or
+        // its FP32 copy)
+        const long nd = 7L * bb * N;
+        if (prm_.precision == 1) {
+            diaf_.alloc(nd);
+            launchToDia<float>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, diaf_.p, s);
+        } else {
+            dia64_.alloc(nd);
+            launchToDia<double>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, dia64_.p, s);
+        }
         dinv64_.alloc(bb * (size_t)N);
         m_.timer.begin(kClsIluFactor, s);
-        colored_->factor(m_.rowPtr.p, A64_, dinv64_.p, fail_.p, s);
+        if (prm_.precision == 1) colored_->factor(m_.rowPtr.p, A64_, dinv64_.p, fail_.p, s);
+        else colored_->factor(nullptr, dia64_.p, dinv64_.p, fail_.p, s);
         m_.timer.end(s);
     } else if (prm_.precond == 0) {
         // ILU(0) factorization in FP64 (pivot blocks stay FP64 in both forms)
@@ -613,8 +623,7 @@ bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
     bool fused = false;
     if (colored_) {
         m_.timer.begin(kClsIluApply, s);
-        const S* Am = sizeof(S) == 8 ? (const S*)A64_ : (const S*)Af_.p;
-        colored_->apply<S>(m_.rowPtr.p, Am, (const double*)dinv, d, v, prm_.prec_relax, s, run);
+        colored_->apply<S>(nullptr, diaOf<S>(), (const double*)dinv, d, v, prm_.prec_relax, s, run);
         m_.timer.end(s);
     } else if (prm_.precond == 0) {
         m_.timer.begin(kClsIluApply, s);
@@ -702,7 +711,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         fusedSpmv = precApply<S>(lu, dinv, p, y, run, A, v);
         if (!fusedSpmv) {
             m_.timer.begin(kClsSpmv, s);
-            launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, y, v, s, run);
+            spmvInner<S>(A, y, v, run);
             m_.timer.end(s);
         }
         launchDotFinal<S>(m, b, v, m_.dots.p, ks_.p, 1, 0, s);
@@ -713,7 +722,7 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
         fusedSpmv = precApply<S>(lu, dinv, r, z, run, A, y);
         if (!fusedSpmv) {
             m_.timer.begin(kClsSpmv, s);
-            launchSpmv<S>(N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, z, y, s, run);
+            spmvInner<S>(A, z, y, run);
             m_.timer.end(s);
         }
         launchDot2Final<S>(m, y, r, m_.dots.p, ks_.p, 1, s);
@@ -788,6 +797,13 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
     *its = maxIter;
     if (ksHost_->fail_code) return 2;
     return 1;
+}
+
+template <class S>
+void BiCGStabBackend::spmvInner(const S* A, const S* x, S* y, const int* run)
+{
+    if (colored_) launchSpmvDia<S>(m_.gx, m_.gy, m_.gz, diaOf<S>(), x, y, m_.stream, run);
+    else launchSpmv<S>((int)m_.N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, x, y, m_.stream, run);
 }
 
 int BiCGStabBackend::solve()

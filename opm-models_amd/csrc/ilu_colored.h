@@ -62,7 +62,9 @@ public:
     ColoredIlu& operator=(const ColoredIlu&) = delete;
     // pivots of A (FP64): dinv[N][9] = inverses of D, the diagonal of the factor
     // (D_I = A_II - sum_{K lower} A_IK D_K^-1 A_KI); fail: set to 2 for a
-    // singular pivot
+    // singular pivot.  rp: the sorted BCRS rows of A, or nullptr when A is in
+    // the structured layout [7][N][9] (slots -z, -y, -x, I, +x, +y, +z) --
+    // likewise for apply.
     void factor(const int* rp, const double* A, double* dinv, int* fail, hipStream_t s);
     // v = M^-1 d with M = (D + L_A) D^-1 (D + U_A), the off-diagonal blocks of
     // A (FP64, or its FP32 copy) and the FP64 pivots; v *= relax at the end

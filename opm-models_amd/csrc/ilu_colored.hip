@@ -63,6 +63,30 @@ __device__ __forceinline__ RowPos rowPositions(const ColGeom& g, long k0, int gx
     return p;
 }
 
+// The same positions in the structured layout [7][N][9] (slots -z, -y, -x,
+// I, +x, +y, +z; blocks of absent neighbours are zero and never read): the
+// blocks of one slot of consecutive cells are consecutive in memory.
+__device__ __forceinline__ RowPos diaPositions(const ColGeom& g, long I, int gx, int gy, int gz)
+{
+    const long N = (long)g.nx * g.ny * g.nz;
+    RowPos p;
+    p.blk[4] = gz > 0 ? I : -1;
+    p.blk[2] = gy > 0 ? N + I : -1;
+    p.blk[0] = gx > 0 ? 2 * N + I : -1;
+    p.diag = 3 * N + I;
+    p.blk[1] = gx + 1 < g.nx ? 4 * N + I : -1;
+    p.blk[3] = gy + 1 < g.ny ? 5 * N + I : -1;
+    p.blk[5] = gz + 1 < g.nz ? 6 * N + I : -1;
+    return p;
+}
+
+// rp != nullptr: the sorted BCRS rows; rp == nullptr: the structured layout
+__device__ __forceinline__ RowPos positionsOf(const ColGeom& g, const int* __restrict__ rp, long I, int gx, int gy,
+                                             int gz)
+{
+    return rp ? rowPositions(g, rp[I], gx, gy, gz) : diaPositions(g, I, gx, gy, gz);
+}
+
 // whether the neighbour in direction dir lies in another tile
 __device__ __forceinline__ bool crossesTile(int dir, int lx, int ly, int lz)
 {
@@ -168,7 +192,7 @@ __device__ __forceinline__ void colFactorTile(const ColGeom& g, int color, int t
     double D[9];
     RowPos pos{};
     if (valid) {
-        pos = rowPositions(g, rp[I], gx, gy, gz);
+        pos = positionsOf(g, rp, I, gx, gy, gz);
 #pragma unroll
         for (int q = 0; q < 9; ++q) D[q] = A[9 * pos.diag + q];
         // neighbours in tiles of the earlier colour (colour 1 only), column order
@@ -180,7 +204,7 @@ __device__ __forceinline__ void colFactorTile(const ColGeom& g, int color, int t
                 const long K = neighbourCell(g, I, dir);
                 // A_KI: the block of row K in column I, at K's opposite-direction position
                 int kx = (int)(K % g.nx), ky = (int)((K / g.nx) % g.ny), kz = (int)(K / ((long)g.nx * g.ny));
-                const RowPos pk = rowPositions(g, rp[K], kx, ky, kz);
+                const RowPos pk = positionsOf(g, rp, K, kx, ky, kz);
                 const long pKI = pk.blk[dir ^ 1];
                 double P[9], T[9];
                 matmul3(A + 9 * pos.blk[dir], dinv + 9 * K, P);
@@ -212,7 +236,7 @@ __device__ __forceinline__ void colFactorTile(const ColGeom& g, int color, int t
                 const int dir = kInLowerDir[q];
                 const long K = neighbourCell(g, I, dir);
                 int kx = (int)(K % g.nx), ky = (int)((K / g.nx) % g.ny), kz = (int)(K / ((long)g.nx * g.ny));
-                const RowPos pk = rowPositions(g, rp[K], kx, ky, kz);
+                const RowPos pk = positionsOf(g, rp, K, kx, ky, kz);
                 double P[9], T[9];
                 matmul3(A + 9 * pos.blk[dir], nb[q], P);
                 matmul3(P, A + 9 * pk.blk[dir ^ 1], T);
@@ -244,7 +268,7 @@ __device__ __forceinline__ void colForwardTile(const ColGeom& g, int color, int 
     S blk[3][9];
     double Di[9];
     if (valid) {
-        const RowPos pos = rowPositions(g, rp[I], gx, gy, gz);
+        const RowPos pos = positionsOf(g, rp, I, gx, gy, gz);
 #pragma unroll
         for (int e = 0; e < 3; ++e) acc[e] = d[3 * I + e];
         if (color == 1) {
@@ -311,7 +335,7 @@ __device__ __forceinline__ void colBackwardTile(const ColGeom& g, int color, int
     double Di[9];
     bool hasUp[3] = {false, false, false};
     if (valid) {
-        const RowPos pos = rowPositions(g, rp[I], gx, gy, gz);
+        const RowPos pos = positionsOf(g, rp, I, gx, gy, gz);
 #pragma unroll
         for (int e = 0; e < 3; ++e) wOwn[e] = v[3 * I + e];
         // neighbours in tiles of the later colour (colour 0 only), column order

@@ -165,6 +165,122 @@ void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a, const S
         hipLaunchKernelGGL((k_spmv<2, S>), g, b, 0, st, n, rp, ci, a, x, y, run);
     }
 }
+// ---------------------------------------------------------------------------
+// The structured layout of a 7-point 3x3 BCRS matrix on an nx x ny x nz grid:
+// D[slot][I][9] with slots -z, -y, -x, I, +x, +y, +z (the sorted row order),
+// zero blocks where a neighbour is absent.  The blocks of one slot of
+// consecutive cells are consecutive, so the sweeps and the operator read
+// whole cache lines.
+__device__ __forceinline__ bool diaPresent(int slot, int i, int j, int k, int nx, int ny, int nz)
+{
+    switch (slot) {
+    case 0: return k > 0;
+    case 1: return j > 0;
+    case 2: return i > 0;
+    case 3: return true;
+    case 4: return i + 1 < nx;
+    case 5: return j + 1 < ny;
+    default: return k + 1 < nz;
+    }
+}
+__device__ __forceinline__ long diaNeighbour(int slot, long I, int nx, long nxy)
+{
+    switch (slot) {
+    case 0: return I - nxy;
+    case 1: return I - nx;
+    case 2: return I - 1;
+    case 3: return I;
+    case 4: return I + 1;
+    case 5: return I + nx;
+    default: return I + nxy;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_to_dia(int nx, int ny, int nz, const int* __restrict__ rp,
+                                                const double* __restrict__ A, S* __restrict__ D)
+{
+    const long N = (long)nx * ny * nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 7 * N) return;
+    const int slot = (int)(t / N);
+    const long I = t - slot * N;
+    const int i = (int)(I % nx), j = (int)((I / nx) % ny), k = (int)(I / ((long)nx * ny));
+    S* out = D + 9 * t;
+    if (!diaPresent(slot, i, j, k, nx, ny, nz)) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) out[c] = S(0);
+        return;
+    }
+    long pos = rp[I];
+    for (int q = 0; q < slot; ++q) pos += diaPresent(q, i, j, k, nx, ny, nz) ? 1 : 0;
+    const double* in = A + 9 * pos;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) out[c] = (S)in[c];
+}
+
+// y = A x in the structured layout: the arithmetic of k_spmv3 (row sums of
+// the present blocks in column order), so the same bits
+template <class S>
+__global__ __launch_bounds__(256) void k_spmv_dia(int nx, int ny, int nz, const S* __restrict__ D,
+                                                  const S* __restrict__ x, S* __restrict__ y,
+                                                  const int* __restrict__ run)
+{
+    if (run && !*run) return;
+    const long N = (long)nx * ny * nz;
+    const long nxy = (long)nx * ny;
+    const long g = (long)xcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long I = g / 3;
+    const int e = (int)(g - 3 * I);
+    if (I >= N) return;
+    const int i = (int)(I % nx), j = (int)((I / nx) % ny), k = (int)(I / nxy);
+    S v[7][3], xv[7][3];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        if (diaPresent(q, i, j, k, nx, ny, nz)) {
+            const long c = diaNeighbour(q, I, nx, nxy);
+            const S* row = D + 9 * (q * N + I) + 3 * e;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                v[q][r] = row[r];
+                xv[q][r] = x[3 * c + r];
+            }
+        }
+    }
+    S acc = S(0);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        if (diaPresent(q, i, j, k, nx, ny, nz)) {
+            S s = v[q][0] * xv[q][0];
+            s += v[q][1] * xv[q][1];
+            s += v[q][2] * xv[q][2];
+            acc += s;
+        }
+    }
+    y[3 * I + e] = acc;
+}
+
+template <class S>
+void launchToDia(int nx, int ny, int nz, const int* rp, const double* A, S* D, hipStream_t st)
+{
+    const long n = 7L * nx * ny * nz;
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_to_dia<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nx, ny, nz, rp, A, D);
+}
+template void launchToDia<double>(int, int, int, const int*, const double*, double*, hipStream_t);
+template void launchToDia<float>(int, int, int, const int*, const double*, float*, hipStream_t);
+
+template <class S>
+void launchSpmvDia(int nx, int ny, int nz, const S* D, const S* x, S* y, hipStream_t st, const int* run)
+{
+    const long lanes = 3L * nx * ny * nz;
+    if (lanes <= 0) return;
+    hipLaunchKernelGGL((k_spmv_dia<S>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, nx, ny, nz, D, x, y,
+                       run);
+}
+template void launchSpmvDia<double>(int, int, int, const double*, const double*, double*, hipStream_t, const int*);
+template void launchSpmvDia<float>(int, int, int, const float*, const float*, float*, hipStream_t, const int*);
+
 template void launchSpmv<double>(int, int, const int*, const int*, const double*, const double*, double*, hipStream_t,
                                  const int*);
 template void launchSpmv<float>(int, int, const int*, const int*, const float*, const float*, float*, hipStream_t,

@@ -228,6 +228,19 @@ private:
     DevBuf<int> fail_;
     std::unique_ptr<TiledIlu> tiled_;
     std::unique_ptr<ColoredIlu> colored_;   // ilu_order 1
+    // with the two-colour ILU the solver's matrix in the structured layout
+    // (the sweeps and the operator read it; FP64 for the FP64 solve, FP32 for
+    // the FP32 inner solve)
+    DevBuf<double> dia64_;
+    DevBuf<float> diaf_;
+    template <class S> const S* diaOf() const
+    {
+        if constexpr (sizeof(S) == 8) return dia64_.p;
+        else return diaf_.p;
+    }
+    // the inner solve's operator: the structured layout with the two-colour
+    // ILU, the BCRS matrix otherwise
+    template <class S> void spmvInner(const S* A, const S* x, S* y, const int* run);
     bool prepared_ = false;
     bool tuned_ = false;
 };

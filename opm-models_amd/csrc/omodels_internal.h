@@ -205,6 +205,13 @@ constexpr int kRedBlocks = 1024;     // partial sums per reduction (fixed => det
 template <class S> void launchSpmv(int n, int bs, const int* rp, const int* ci, const S* a,
                                    const S* x, S* y, hipStream_t st, const int* run = nullptr);
 template <class S> void launchCopyConvertD(long m, const double* x, S* y, hipStream_t st);
+// the structured layout D[7][N][9] of a 7-point 3x3 BCRS matrix on an
+// nx x ny x nz grid (slots -z, -y, -x, I, +x, +y, +z; zero blocks where a
+// neighbour is absent), converted to S, and y = A x in it (same bits as launchSpmv)
+template <class S>
+void launchToDia(int nx, int ny, int nz, const int* rp, const double* A, S* D, hipStream_t st);
+template <class S>
+void launchSpmvDia(int nx, int ny, int nz, const S* D, const S* x, S* y, hipStream_t st, const int* run = nullptr);
 
 // ILU0 with level scheduling; lvlPtr/lvlRows on the device, lvlPtrHost on the host
 struct LevelSchedule {
