@@ -286,6 +286,30 @@ __global__ __launch_bounds__(kT) void k_e_spmv(const EnsInst* __restrict__ insts
     const S* x = vec<S>(I, ix);
     S* y = vec<S>(I, iy);
     S acc[3] = {S(0), S(0), S(0)};
+    if (sizeof(S) == 4 && I.afDia) {
+        // the structured layout: the present blocks in the same (column) order
+        const int nx = I.G.nx, ny = I.G.ny, nz = I.G.nz;
+        const long nxy = (long)nx * ny;
+        const int ci = (int)(i % nx), cj = (int)((i / nx) % ny), ck = (int)(i / nxy);
+        const long off[7] = {-nxy, -(long)nx, -1, 0, 1, (long)nx, nxy};
+        const bool has[7] = {ck > 0, cj > 0, ci > 0, true, ci + 1 < nx, cj + 1 < ny, ck + 1 < nz};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            if (!has[q]) continue;
+            const long c = i + off[q];
+            const S* blk = a + 9L * ((long)q * I.N + i);
+            const S x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                S s = blk[e * 3] * x0;
+                s += blk[e * 3 + 1] * x1;
+                s += blk[e * 3 + 2] * x2;
+                acc[e] += s;
+            }
+        }
+        y[3 * i] = acc[0]; y[3 * i + 1] = acc[1]; y[3 * i + 2] = acc[2];
+        return;
+    }
     const int k1 = I.rowPtr[i + 1];
     for (int k = I.rowPtr[i]; k < k1; ++k) {
         const long c = I.colIdx[k];
@@ -347,7 +371,27 @@ __global__ __launch_bounds__(kT) void k_e_copy_matrix(const EnsInst* __restrict_
     const EnsInst& I = insts[t.x];
     long lo, hi;
     chunkValueRange(I, t.y, lo, hi);
-    if (toFloat) {
+    if (toFloat && I.afDia) {
+        // the chunk's cells in the structured layout, slot by slot
+        const long r0 = (long)t.y * kT;
+        const int nc = (int)((r0 + kT < I.N ? r0 + kT : I.N) - r0);
+        const int nx = I.G.nx, ny = I.G.ny, nz = I.G.nz;
+        const long nxy = (long)nx * ny;
+        for (int idx = threadIdx.x; idx < 7 * nc; idx += kT) {
+            const int q = idx / nc;
+            const long c = r0 + (idx - q * nc);
+            const int ci = (int)(c % nx), cj = (int)((c / nx) % ny), ck = (int)(c / nxy);
+            const bool has[7] = {ck > 0, cj > 0, ci > 0, true, ci + 1 < nx, cj + 1 < ny, ck + 1 < nz};
+            float* out = I.Af + 9L * ((long)q * I.N + c);
+            if (!has[q]) {
+                for (int v = 0; v < 9; ++v) out[v] = 0.0f;
+                continue;
+            }
+            long pos = I.rowPtr[c];
+            for (int r = 0; r < q; ++r) pos += has[r] ? 1 : 0;
+            for (int v = 0; v < 9; ++v) out[v] = (float)I.jac[9 * pos + v];
+        }
+    } else if (toFloat) {
         for (long k = lo + threadIdx.x; k < hi; k += kT) I.Af[k] = (float)I.jac[k];
     } else {
         for (long k = lo + threadIdx.x; k < hi; k += kT) I.lu64[k] = I.jac[k];

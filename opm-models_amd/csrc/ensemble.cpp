@@ -155,7 +155,8 @@ int Ensemble::add(const omodels_blackoil_problem* p, const double* pv, const uin
     M.dinv64.alloc(9 * (size_t)N);
     for (int v = 0; v < kNumVec; ++v) if (v != kVb) M.dv[v].alloc(3 * N);
     if (prm_.precision == 1) {
-        M.Af.alloc(9 * (size_t)M.nnzb);
+        // the two-colour ILU reads the FP32 matrix in the structured layout
+        M.Af.alloc(9 * (colored() ? 7 * (size_t)M.N : (size_t)M.nnzb));
         if (!colored()) M.luf.alloc(9 * (size_t)M.nnzb);
         for (int v = 0; v < kNumVec; ++v) M.fv[v].alloc(3 * N);
     }
@@ -207,6 +208,7 @@ void Ensemble::rebuild()
         EnsInst& I = d[b];
         std::memset(&I, 0, sizeof(I));
         I.G = M.G; I.N = (int)M.N; I.nnzb = (int)M.nnzb; I.nsrc = M.nsrc; I.kChunk = M.kChunk;
+        I.afDia = colored() ? 1 : 0;
         I.gx = M.gx; I.gy = M.gy; I.gz = M.gz; I.nAsm = M.gx * M.gy * M.gz;
         I.storefac = M.G.V / M.dt;
         I.trans = M.trans.p; I.poro = M.poro.p;
@@ -309,6 +311,7 @@ std::vector<ColInst> Ensemble::colInsts(const int* act, bool floatMatrix, int id
         I.dinv = M.dinv64.p;
         I.active = act + b;
         if (floatMatrix) {
+            I.rp = nullptr;             // Af in the structured layout
             I.A = M.Af.p;
             I.d = M.fv[idD].p;
             I.v = M.fv[idV].p;

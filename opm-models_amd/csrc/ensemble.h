@@ -20,6 +20,7 @@ enum EnsVec : int { kVb = 0, kVx, kVr, kVp, kVv, kVy, kVz, kVr64, kNumVec };
 struct EnsInst {
     BoGrid G;
     int N, nnzb, nsrc, kChunk;
+    int afDia;                       // 1: Af holds the structured layout [7][N][9] (two-colour ILU)
     int gx, gy, gz, nAsm;            // assembly grid of this instance
     double storefac;                 // cell volume / dt of the current solve
     const double* trans;
