@@ -552,6 +552,19 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     if (prm_.autotune && prm_.ilu_schedule == 0 && tiled_ && !tuned_) tuneIlu();
     OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
     const int N = (int)m_.N;
+    // the inner solve's operator (and the two-colour sweeps) read the matrix in
+    // the structured 7-slot layout of a 3x3 7-point grid system: whole cache
+    // lines, no column indices, the same bits as the BCRS product
+    if (structured() && prm_.precision != 2) {
+        const long nd = 7L * bb * N;
+        if (prm_.precision == 1) {
+            diaf_.alloc(nd);
+            launchToDia<float>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, diaf_.p, s);
+        } else {
+            dia64_.alloc(nd);
+            launchToDia<double>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, dia64_.p, s);
+        }
+    }
     if (prm_.precision == 2) {
         // the reference's LinearSolverScalar = float: matrix, factorization and
         // Krylov solve all in FP32 (the inverted FP32 pivots are kept as FP64
@@ -566,17 +579,8 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
         launchExtractInvDiag<float>(N, bs, m_.diagOff.p, luf_.p, dinv64_.p, s);
         m_.timer.end(s);
     } else if (colored_) {
-        // two-colour ILU(0): FP64 pivots only; the sweeps and the inner
-        // operator read the solver's matrix in the structured layout (FP64, or
-        // its FP32 copy)
-        const long nd = 7L * bb * N;
-        if (prm_.precision == 1) {
-            diaf_.alloc(nd);
-            launchToDia<float>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, diaf_.p, s);
-        } else {
-            dia64_.alloc(nd);
-            launchToDia<double>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, dia64_.p, s);
-        }
+        // two-colour ILU(0): FP64 pivots only; the sweeps read the solver's
+        // matrix in the structured layout (FP64, or its FP32 copy)
         dinv64_.alloc(bb * (size_t)N);
         m_.timer.begin(kClsIluFactor, s);
         if (prm_.precision == 1) colored_->factor(m_.rowPtr.p, A64_, dinv64_.p, fail_.p, s);
@@ -802,8 +806,13 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
 template <class S>
 void BiCGStabBackend::spmvInner(const S* A, const S* x, S* y, const int* run)
 {
-    if (colored_) launchSpmvDia<S>(m_.gx, m_.gy, m_.gz, diaOf<S>(), x, y, m_.stream, run);
+    if (diaOf<S>()) launchSpmvDia<S>(m_.gx, m_.gy, m_.gz, diaOf<S>(), x, y, m_.stream, run);
     else launchSpmv<S>((int)m_.N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, x, y, m_.stream, run);
+}
+
+bool BiCGStabBackend::structured() const
+{
+    return m_.bs == 3 && m_.gx > 0 && (long)m_.gx * m_.gy * m_.gz == m_.N && m_.nnzb <= 7L * m_.N;
 }
 
 int BiCGStabBackend::solve()

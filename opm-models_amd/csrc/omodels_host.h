@@ -241,6 +241,8 @@ private:
     // the inner solve's operator: the structured layout with the two-colour
     // ILU, the BCRS matrix otherwise
     template <class S> void spmvInner(const S* A, const S* x, S* y, const int* run);
+    // a 3x3 system on the model's nx x ny x nz grid (7-point pattern)
+    bool structured() const;
     bool prepared_ = false;
     bool tuned_ = false;
 };
