@@ -558,6 +558,7 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     if (structured() && prm_.precision != 2) {
         const long nd = 7L * bb * N;
         if (prm_.precision == 1) {
+            dia64_.release();       // an FP64 copy made for a test application would be stale
             diaf_.alloc(nd);
             launchToDia<float>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, diaf_.p, s);
         } else {
@@ -626,6 +627,7 @@ bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
     const int N = (int)m_.N;
     bool fused = false;
     if (colored_) {
+        if (!diaOf<S>()) throw std::logic_error("two-colour ILU: no structured matrix of this precision");
         m_.timer.begin(kClsIluApply, s);
         colored_->apply<S>(nullptr, diaOf<S>(), (const double*)dinv, d, v, prm_.prec_relax, s, run);
         m_.timer.end(s);
@@ -896,6 +898,12 @@ void BiCGStabBackend::applyPrecond(const double* hostD, double* hostV)
     d.upload(hostD, m, m_.stream);
     v.alloc(m);
     OM_HIP(hipMemsetAsync(v.p, 0, m * sizeof(double), m_.stream));   // Jacobi sweeps start from zero
+    if (colored_ && !dia64_.p) {
+        // the FP32-inner form keeps only the FP32 structured matrix: the FP64
+        // application here reads an FP64 one
+        dia64_.alloc(7L * m_.bs * m_.bs * m_.N);
+        launchToDia<double>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, dia64_.p, m_.stream);
+    }
     precApply<double>(lu64_.p, dinv64_.p, d.p, v.p);
     v.download(hostV, m, m_.stream);
     m_.sync();
