@@ -439,11 +439,7 @@ def main():
         st1 = orc.storage(prob, case.pv0, case.mean0)
 
         def asm_rate(nt):
-            ts = []
-            for _ in range(5):
-                ta = time.time()
-                orc.linearize(prob, case.pv0, case.mean0, st1, dt, nthreads=nt)
-                ts.append(time.time() - ta)
+            ts = orc.linearize_times(prob, case.pv0, case.mean0, st1, dt, nt, reps=5)
             return round(N / float(np.median(ts)) / 1e6, 2)
 
         cpu_model = ""

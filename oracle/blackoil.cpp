@@ -1440,6 +1440,25 @@ int ob_linearize_ext(const ob_problem* p, const ob_diffusion* dif, const ob_disp
     } catch (...) { return -1; }
 }
 
+/* The CPU baseline's assembly figure: the problem and its pattern built once,
+ * then reps linearizations on nthreads threads; seconds[k] the wall time of
+ * the k-th (the loop of linearize alone). */
+int ob_linearize_timed(const ob_problem* p, const double* pv, const uint8_t* meaning, const double* storage1,
+                       double dt, int nthreads, int reps, double* seconds)
+{
+    try {
+        Problem pr(p);
+        const long N = pr.G.N, nnzb = pr.pat.rowPtr[N];
+        std::vector<double> r(3 * N), J(9 * nnzb);
+        for (int k = 0; k < reps; ++k) {
+            const auto t0 = std::chrono::steady_clock::now();
+            linearize(pr, pv, meaning, storage1, dt, r.data(), J.data(), nthreads);
+            seconds[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return 0;
+    } catch (...) { return -1; }
+}
+
 int ob_ilu0_factor(int n, int bs, const int32_t* row_ptr, const int32_t* col_idx, double* vals)
 {
     return ilu0Factor<double>(n, bs, row_ptr, col_idx, vals);

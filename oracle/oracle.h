@@ -136,6 +136,9 @@ int ob_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, doub
 /* the same with the diffusive and the dispersive face terms (dif, dsp may be
  * NULL).  With dsp: nv_in [N] the cells' oil velocities of the previous
  * linearization, nv_out [N] receives this one's. */
+/* reps linearizations of one problem on nthreads threads, the wall time of each */
+int ob_linearize_timed(const ob_problem* p, const double* pv, const uint8_t* meaning, const double* storage1,
+                       double dt, int nthreads, int reps, double* seconds);
 int ob_linearize_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, const double* nv_in,
                      double* nv_out, const double* pv, const uint8_t* meaning, const double* storage1, double dt,
                      double* residual, double* jac, int nthreads);

@@ -109,6 +109,7 @@ def lib():
         L.ob_build_pattern.argtypes = [C.POINTER(ObProblem), _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip]
         L.ob_storage.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp]
         L.ob_linearize.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, _dp, _dp, C.c_int]
+        L.ob_linearize_timed.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, C.c_int, C.c_int, _dp]
         L.ob_source_blocks.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _ip, _dp, _dp]
         L.ob_flows.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp]
         L.ob_intensive.argtypes = [C.POINTER(ObProblem), _dp, C.c_int, C.c_int, _dp]
@@ -249,6 +250,13 @@ def linearize(prob, pv, mean, st1, dt, nthreads=1, diffusion=None, dispersion=No
                                 P(mean, _u8p), P(st1), dt, P(r), P(J), nthreads)
     assert rc == 0
     return (J, r) if dispersion is None else (J, r, nv_out)
+
+
+def linearize_times(prob, pv, mean, st1, dt, nthreads, reps=5):
+    """wall times of reps linearizations (the problem built once)"""
+    t = np.zeros(reps)
+    assert lib().ob_linearize_timed(C.byref(prob), P(pv), P(mean, _u8p), P(st1), dt, nthreads, reps, P(t)) == 0
+    return t
 
 
 def source_blocks(prob, pv, mean):
