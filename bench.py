@@ -62,6 +62,13 @@ HBM_TRAFFIC_500K = {
     "assemble": 489551360,
     "ilu_factor": 634073804,
 }
+# the same at 500k with the FP32 inner solve and the two-colour ILU
+# (profiles/blackoil_500k_fp32_two_colour.md)
+HBM_TRAFFIC_500K_FP32_COLORED = {
+    "ilu_apply": 387115213,    # both colours' forward and backward sweeps
+    "spmv": 138695680,
+    "ilu_factor": 296877773,
+}
 
 
 def note(msg):
@@ -69,18 +76,28 @@ def note(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(N, F, nnzb, n=3, s=8):
-    """Per-launch algorithmic HBM bytes of each kernel class."""
-    return {
+def algorithmic_bytes(N, F, nnzb, n=3, s=8, colored=False):
+    """Per-launch algorithmic HBM bytes of each kernel class; s = bytes of the
+    inner solve's scalar (8, or 4 with the FP32 inner solve), colored = the
+    two-colour ILU (FP64 pivots, off-diagonal blocks of the solver's matrix)."""
+    ab = {
         # primaries, old storage, porosity, depth, meaning flags; one T per face;
-        # residual and Jacobian writes
-        "assemble": N * (2 * n * 8 + 8 + 8 + 4) + F * 8 + N * n * 8 + nnzb * n * n * s,
+        # residual and Jacobian (FP64) writes
+        "assemble": N * (2 * n * 8 + 8 + 8 + 4) + F * 8 + N * n * 8 + nnzb * n * n * 8,
         "spmv": nnzb * (n * n * s + 4) + (N + 1) * 4 + 2 * N * n * s,
         "ilu_apply": nnzb * (n * n * s + 4) + (N + 1) * 4 + 3 * N * n * s,
+        # FP64 factor in both precisions: read and write every block once
         "ilu_factor": 2 * nnzb * n * n * 8 + nnzb * 4 + (N + 1) * 4,
         # preconditioner application with the operator application in its backward sweep
         "ilu_apply_spmv": 2 * (nnzb * (n * n * s + 4) + (N + 1) * 4) + 5 * N * n * s,
     }
+    if colored:
+        # both colours' forward and backward sweeps: every off-diagonal block
+        # once, the FP64 pivots twice, d read and v written in each direction
+        ab["ilu_apply"] = (nnzb - N) * n * n * s + 2 * 8 * n * n * N + 4 * n * s * N
+        # the blocks of A once (FP64), the pivots written
+        ab["ilu_factor"] = nnzb * n * n * 8 + N * n * n * 8
+    return ab
 
 
 def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=False):
@@ -237,6 +254,8 @@ def beside_runs(om, np, config, dt, steps=2):
         wall = time.time() - t0
         pv = model.get_solution()[0]
         kt = model.kernel_times()
+        ab = algorithmic_bytes(model.N, (model.nnzb - model.N) // 2, model.nnzb, s=8 if precision == 0 else 4,
+                               colored=order == 1)
         model.close()
         if key == "ref" or (precision == 0 and order == 0):
             ref[cfg_name] = pv
@@ -249,6 +268,7 @@ def beside_runs(om, np, config, dt, steps=2):
                     "newton_iters": st.newton_iters,
                     "linear_iters_per_newton": round(st.linear_iters / max(st.newton_iters, 1), 2),
                     "ilu_apply_avg_us": round(ms / max(cnt, 1) * 1e3, 1),
+                    "ilu_apply_achieved_GBs": round(ab["ilu_apply"] / max(ms / max(cnt, 1) * 1e-3, 1e-12) / 1e9, 1),
                     "steps": steps, "ok": rc == 0 and rc2 == 0, "_pv": pv, "_cfg": cfg_name}
     for key, v in out.items():
         pv = v.pop("_pv")
@@ -327,7 +347,7 @@ def main():
 
     newton = st.newton_iters
     value = newton / wall
-    ab = algorithmic_bytes(N, F, nnzb)
+    ab = algorithmic_bytes(N, F, nnzb, s=8 if args.precision == 0 else 4, colored=args.ilu_order == 1)
     kernels = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0 or k not in ab:
@@ -340,7 +360,11 @@ def main():
     asm = kernels.get("assemble")
     roof = {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(kernels[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": HBM_TRAFFIC_500K.get(dom) if args.config == "500k" and args.precision == 0 else None}
+            "traffic": None}
+    if args.config == "500k" and args.precision == 0 and args.ilu_order == 0:
+        roof["traffic"] = HBM_TRAFFIC_500K.get(dom)
+    elif args.config == "500k" and args.precision == 1 and args.ilu_order == 1:
+        roof["traffic"] = HBM_TRAFFIC_500K_FP32_COLORED.get(dom)
 
     line = {
         "metric": "Newton steps/s & assembly Mcells/s at 500k blackoil; ensemble p50/p99 latency",

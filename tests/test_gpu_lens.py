@@ -122,3 +122,31 @@ def test_graph_run_is_identical_to_eager(precision):
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))
+
+
+@pytest.mark.parametrize("precision,solver_float,tol", [(0, 0, 1e-10), (2, 1, 1e-8)])
+def test_reference_run_to_30000s(precision, solver_float, tol):
+    """The reference binary's run: 30 000 s from dt0 = 250 s (FP64 solve, and
+    the binary's FP32 solve).  End time reached by both; with the same step
+    sizes the states agree within 1e-8 (FP64) / 1e-6 (FP32); the FP64 run
+    must take the oracle's step sizes, the FP32 one may not (device and host
+    pow differ in the last bit, which the FP32 solve can turn into another
+    step sequence), its step count within two of the oracle's."""
+    pg, po = problems()
+    m = om.LensModel(pg)
+    rc, th, dth, log, st = m.simulate(om.default_params(newton_tol=tol, precision=precision), 30e3, 250.0)
+    rco, pvo, tho, dtho, logo = orc.lens_simulate(po, orc.lens_initial(po), 30e3, 250.0,
+                                                  orc.params(newton_tol=tol, solver_float=solver_float))
+    assert rc == 0 and rco == 0
+    print("steps", len(th), len(tho), "newton", st.newton_iters, flush=True)
+    assert th[-1] == pytest.approx(30e3, rel=1e-12) and tho[-1] == pytest.approx(30e3, rel=1e-12)
+    assert abs(len(th) - len(tho)) <= 2
+    pv = m.get_solution()
+    if np.array_equal(dth, dtho):
+        bound = 1e-8 if precision == 0 else 1e-6
+        assert np.abs(pv[0::2] - pvo[0::2]).max() / np.abs(pvo[0::2]).max() <= bound
+        assert np.abs(pv[1::2] - pvo[1::2]).max() <= bound
+    elif precision == 0:
+        raise AssertionError("FP64 run took other step sizes than the oracle")
+    s = pv[1::2]
+    assert s.max() <= 1.0 + 1e-9 and s.min() > 0.0
