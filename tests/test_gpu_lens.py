@@ -126,27 +126,39 @@ def test_graph_run_is_identical_to_eager(precision):
 
 @pytest.mark.parametrize("precision,solver_float,tol", [(0, 0, 1e-10), (2, 1, 1e-8)])
 def test_reference_run_to_30000s(precision, solver_float, tol):
-    """The reference binary's run: 30 000 s from dt0 = 250 s (FP64 solve, and
-    the binary's FP32 solve).  End time reached by both; with the same step
-    sizes the states agree within 1e-8 (FP64) / 1e-6 (FP32); the FP64 run
-    must take the oracle's step sizes, the FP32 one may not (device and host
-    pow differ in the last bit, which the FP32 solve can turn into another
-    step sequence), its step count within two of the oracle's."""
+    """The reference binary's run: 30 000 s from dt0 = 250 s (FP64 solve at
+    Newton tolerance 1e-10, and the binary's FP32 solve at its 1e-8).  Both
+    reach the end time with step counts within two of each other.  The step
+    sizes follow the Newton iteration counts, and over 24 steps a Newton error
+    within rounding of the tolerance (device and host pow differ in the last
+    bit) can take one more iteration on one side, so the sequences may part;
+    the oracle is then replayed with the GPU's step sizes, one step per call,
+    and the states at 30 000 s agree within 1e-8 (FP64) / 1e-6 (FP32)."""
     pg, po = problems()
+    prm_o = orc.params(newton_tol=tol, solver_float=solver_float)
     m = om.LensModel(pg)
     rc, th, dth, log, st = m.simulate(om.default_params(newton_tol=tol, precision=precision), 30e3, 250.0)
-    rco, pvo, tho, dtho, logo = orc.lens_simulate(po, orc.lens_initial(po), 30e3, 250.0,
-                                                  orc.params(newton_tol=tol, solver_float=solver_float))
+    rco, pvo, tho, dtho, logo = orc.lens_simulate(po, orc.lens_initial(po), 30e3, 250.0, prm_o)
     assert rc == 0 and rco == 0
     print("steps", len(th), len(tho), "newton", st.newton_iters, flush=True)
     assert th[-1] == pytest.approx(30e3, rel=1e-12) and tho[-1] == pytest.approx(30e3, rel=1e-12)
     assert abs(len(th) - len(tho)) <= 2
+    n = min(len(dth), len(dtho))
+    parted = np.nonzero(dth[:n] != dtho[:n])[0]
+    if len(parted):
+        k = int(parted[0])
+        print("step sizes part at step", k, "(GPU / oracle:", dth[k], dtho[k], ")")
+        for name, lg in (("GPU", log), ("oracle", logo)):
+            print(name, [(r[0], r[2], r[6]) for r in lg if k - 1 <= r[0] <= k])
+    pr = orc.lens_initial(po)
+    for k, dt in enumerate(dth):
+        rck, pr, _, dtk, _ = orc.lens_simulate(po, pr, dt, dt, prm_o)
+        assert rck == 0 and len(dtk) == 1 and dtk[0] == dt, k
     pv = m.get_solution()
-    if np.array_equal(dth, dtho):
-        bound = 1e-8 if precision == 0 else 1e-6
-        assert np.abs(pv[0::2] - pvo[0::2]).max() / np.abs(pvo[0::2]).max() <= bound
-        assert np.abs(pv[1::2] - pvo[1::2]).max() <= bound
-    elif precision == 0:
-        raise AssertionError("FP64 run took other step sizes than the oracle")
+    bound = 1e-8 if precision == 0 else 1e-6
+    err_p = np.abs(pv[0::2] - pr[0::2]).max() / np.abs(pr[0::2]).max()
+    err_s = np.abs(pv[1::2] - pr[1::2]).max()
+    print("replayed oracle vs GPU at 30000 s: pressure", err_p, "saturation", err_s)
+    assert err_p <= bound and err_s <= bound
     s = pv[1::2]
     assert s.max() <= 1.0 + 1e-9 and s.min() > 0.0
