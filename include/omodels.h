@@ -103,7 +103,9 @@ typedef struct omodels_params {
     double lin_abs_tol;         /* LinearSolverAbsTolerance (<0: newton_tol/100) */
     double lin_max_error;       /* LinearSolverMaxError */
     int32_t lin_max_iter;       /* LinearSolverMaxIterations */
-    int32_t precond;            /* 0: block ILU0 (PreconditionerWrapperILU), 1: block Jacobi */
+    int32_t precond;            /* 0: block ILU0 (PreconditionerWrapperILU), 1: block Jacobi,
+                                   2: aggregation AMG (in place of ParallelAmgBackend,
+                                   parallelamgbackend.hh:85-316; 3x3 grid systems, FP64) */
     double prec_relax;          /* PreconditionerRelaxation */
     int32_t prec_order;         /* PreconditionerOrder */
     int32_t max_dt_divisions;   /* MaxTimeStepDivisions (fvbaseparameters.hh:112) */
@@ -137,6 +139,20 @@ typedef struct omodels_params {
                                    tile of a colour at once -- a different preconditioner (more
                                    iterations, each far cheaper), for the FP32-inner and ensemble
                                    forms; 3x3 blocks, FP64 or FP32-inner solves */
+    /* precond = 2: box aggregates of amg_agg cells per direction (the
+     * reference's aggregateSizePerDim = 3, parallelamgbackend.hh:294-295),
+     * Galerkin coarse operators, levels until one has at most
+     * amg_coarsen_target cells (AmgCoarsenTarget, :72; here at most 64, the
+     * coarsest level being solved by a dense inverse, where the reference
+     * stops at 5000 with a sparse direct solver), a V-cycle with
+     * amg_pre_smooth forward and amg_post_smooth backward block Gauss-Seidel
+     * sweeps (SOR with relaxation 1, :272-276; two-colour tile order) around
+     * the coarse correction, prolongated with the factor amg_prolong_damp */
+    int32_t amg_coarsen_target; /* 1..64 (default 32) */
+    int32_t amg_pre_smooth;     /* >= 1 (default 2) */
+    int32_t amg_post_smooth;    /* >= 0 (default 2) */
+    double amg_prolong_damp;    /* default 1.6 */
+    int32_t amg_agg[3];         /* >= 1 each (default 3, 3, 3) */
 } omodels_params;
 
 void omodels_default_params(omodels_params* prm);
@@ -353,6 +369,12 @@ int omodels_scalar_product_noalloc(int64_t m, const double* x_dev, const double*
 int omodels_solver_get_ilu(omodels_solver* s, double* lu_values);
 /* Preconditioner apply v = M^-1 d on host arrays. */
 int omodels_solver_apply_precond(omodels_solver* s, const double* d, double* v);
+/* The aggregation AMG (precond = 2) of the prepared matrix, for parity
+ * checks: the number of levels (dims[3 * l + q], q = x, y, z, for the first
+ * `cap` levels), and level l's matrix in the structured layout [7][N_l][9]
+ * (slots -z, -y, -x, I, +x, +y, +z; absent neighbours' blocks zero). */
+int omodels_solver_amg_levels(omodels_solver* s, int32_t* dims, int cap);
+int omodels_solver_get_amg_level(omodels_solver* s, int level, double* values);
 
 /* ------------------------------------------------------------------------ */
 /* Newton method + time loop -- replaces BlackOilNewtonMethod::apply and

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libomodels_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["bo_kernels.hip", "linalg_kernels.hip", "ilu_tiled.hip", "ilu_colored.hip", "ens_kernels.hip", "lens_kernels.hip", "noalloc.hip", "lens_host.cpp", "problem.cpp", "host_model.cpp", "io.cpp",
+SOURCES = ["bo_kernels.hip", "linalg_kernels.hip", "ilu_tiled.hip", "ilu_colored.hip", "amg.hip", "ens_kernels.hip", "lens_kernels.hip", "noalloc.hip", "lens_host.cpp", "problem.cpp", "host_model.cpp", "io.cpp",
            "ensemble.cpp", "capi.cpp"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",

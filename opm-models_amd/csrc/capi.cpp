@@ -79,6 +79,11 @@ void omodels_default_params(omodels_params* p)
     p->min_dt = 0.0;
     p->precision = 0;
     p->verbose = 0;
+    p->amg_coarsen_target = 32;
+    p->amg_pre_smooth = 2;
+    p->amg_post_smooth = 2;
+    p->amg_prolong_damp = 1.6;
+    p->amg_agg[0] = p->amg_agg[1] = p->amg_agg[2] = 3;
 }
 
 int omodels_blackoil_default_fluid(double* oil_tab, int32_t* n_oil, double* gas_tab, int32_t* n_gas,
@@ -560,6 +565,24 @@ int omodels_solver_get_ilu(omodels_solver* s, double* lu)
 int omodels_solver_apply_precond(omodels_solver* s, const double* d, double* v)
 {
     return guarded([&] { s->s->applyPrecond(d, v); return 0; });
+}
+
+int omodels_solver_amg_levels(omodels_solver* s, int32_t* dims, int cap)
+{
+    return guarded([&] {
+        const AmgHierarchy& a = s->s->amg();
+        for (int l = 0; l < a.levels() && l < cap; ++l)
+            for (int q = 0; q < 3; ++q) dims[3 * l + q] = a.dims(l)[q];
+        return a.levels();
+    });
+}
+
+int omodels_solver_get_amg_level(omodels_solver* s, int level, double* values)
+{
+    return guarded([&] {
+        s->s->downloadAmgLevel(level, values);
+        return 0;
+    });
 }
 
 int omodels_newton_update(omodels_model* h, const omodels_params* prm, const double* delta, int32_t* num_switched)

@@ -387,7 +387,8 @@ void Model::restoreTimeLevel()
 BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m_(m), prm_(prm)
 {
     const long n = (long)m.bs * m.N;
-    if (prm_.precond != 0 && m.bs != 3) throw std::invalid_argument("block Jacobi is built for 3x3 blocks");
+    if (prm_.precond < 0 || prm_.precond > 2) throw std::invalid_argument("precond must be 0, 1 or 2");
+    if (prm_.precond == 1 && m.bs != 3) throw std::invalid_argument("block Jacobi is built for 3x3 blocks");
     b64_.alloc(n); x64_.alloc(n);
     ks_.alloc(1);
     fail_.alloc(1);
@@ -412,9 +413,20 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
             tiled_ = std::make_unique<TiledIlu>(makeTileGeom(m.gx, m.gy, m.gz), m.stream, mode);
         }
     }
-    if (prm_.precond != 0) {
+    if (prm_.precond == 1) {
         if (prm_.prec_order < 0) throw std::invalid_argument("PreconditionerOrder must be >= 0");
         jacTmp_.alloc(n * (prm_.precision == 0 ? sizeof(double) : sizeof(float)));
+    }
+    if (prm_.precond == 2) {
+        if (!structured()) throw std::invalid_argument("the AMG is built for 3x3 systems on a 7-point grid");
+        if (prm_.precision != 0) throw std::invalid_argument("the AMG runs the FP64 solve (precision 0)");
+        AmgParams ap;
+        ap.coarsenTarget = prm_.amg_coarsen_target;
+        ap.pre = prm_.amg_pre_smooth;
+        ap.post = prm_.amg_post_smooth;
+        ap.damp = prm_.amg_prolong_damp;
+        for (int q = 0; q < 3; ++q) ap.agg[q] = prm_.amg_agg[q];
+        amg_ = std::make_unique<AmgHierarchy>(m.gx, m.gy, m.gz, ap, m.stream);
     }
     if (prm_.precision == 0) {
         for (auto& w : wd_) w.alloc(n);
@@ -579,6 +591,11 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
         launchIlu0Factor<float>(N, bs, m_.rowPtr.p, m_.colIdx.p, m_.diagOff.p, luf_.p, m_.levels, fail_.p, s);
         launchExtractInvDiag<float>(N, bs, m_.diagOff.p, luf_.p, dinv64_.p, s);
         m_.timer.end(s);
+    } else if (amg_) {
+        // the AMG hierarchy of the structured FP64 matrix
+        m_.timer.begin(kClsIluFactor, s);
+        amg_->setup(dia64_.p, fail_.p, s);
+        m_.timer.end(s);
     } else if (colored_) {
         // two-colour ILU(0): FP64 pivots only; the sweeps read the solver's
         // matrix in the structured layout (FP64, or its FP32 copy)
@@ -626,7 +643,15 @@ bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
     hipStream_t s = m_.stream;
     const int N = (int)m_.N;
     bool fused = false;
-    if (colored_) {
+    if (amg_) {
+        if constexpr (sizeof(S) == 8) {
+            m_.timer.begin(kClsIluApply, s);
+            amg_->apply(d, v, run, s);
+            m_.timer.end(s);
+        } else {
+            throw std::logic_error("the AMG runs the FP64 solve");
+        }
+    } else if (colored_) {
         if (!diaOf<S>()) throw std::logic_error("two-colour ILU: no structured matrix of this precision");
         m_.timer.begin(kClsIluApply, s);
         colored_->apply<S>(nullptr, diaOf<S>(), (const double*)dinv, d, v, prm_.prec_relax, s, run);
@@ -888,6 +913,23 @@ void BiCGStabBackend::downloadIlu(double* hostValues)
 {
     if (colored_) throw std::invalid_argument("the two-colour ILU(0) keeps only its pivots");
     lu64_.download(hostValues, (size_t)m_.bs * m_.bs * m_.nnzb, m_.stream);
+    m_.sync();
+}
+
+const AmgHierarchy& BiCGStabBackend::amg() const
+{
+    if (!amg_) throw std::invalid_argument("the solver has no AMG (precond != 2)");
+    return *amg_;
+}
+
+void BiCGStabBackend::downloadAmgLevel(int level, double* hostValues)
+{
+    const AmgHierarchy& a = amg();
+    if (!prepared_) throw std::logic_error("AMG levels before a successful prepare");
+    if (level < 0 || level >= a.levels()) throw std::invalid_argument("AMG level out of range");
+    const auto d = a.dims(level);
+    const size_t n = 63UL * d[0] * d[1] * d[2];
+    OM_HIP(hipMemcpyAsync(hostValues, a.matrix(level), n * sizeof(double), hipMemcpyDeviceToHost, m_.stream));
     m_.sync();
 }
 

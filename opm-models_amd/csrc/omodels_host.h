@@ -12,6 +12,7 @@
 
 #include "../../include/omodels.h"
 #include "omodels_internal.h"
+#include "amg.h"
 #include "ilu_colored.h"
 #include "ilu_tiled.h"
 
@@ -185,6 +186,9 @@ public:
     const double* solution() const { return x64_.p; }
     void downloadIlu(double* hostValues);
     void applyPrecond(const double* hostD, double* hostV);
+    // the aggregation AMG (precond = 2) and its level matrices
+    const AmgHierarchy& amg() const;
+    void downloadAmgLevel(int level, double* hostValues);
 
 private:
     template <class S> int bicgstab(const S* A, const S* lu, const void* dinv, const S* b, S* x,
@@ -228,6 +232,7 @@ private:
     DevBuf<int> fail_;
     std::unique_ptr<TiledIlu> tiled_;
     std::unique_ptr<ColoredIlu> colored_;   // ilu_order 1
+    std::unique_ptr<AmgHierarchy> amg_;     // precond 2
     // with the two-colour ILU the solver's matrix in the structured layout
     // (the sweeps and the operator read it; FP64 for the FP64 solve, FP32 for
     // the FP32 inner solve)

@@ -64,7 +64,8 @@ class Params(C.Structure):
         ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
         ("max_dt", C.c_double), ("min_dt", C.c_double), ("precision", C.c_int32), ("verbose", C.c_int32),
         ("ilu_schedule", C.c_int32), ("autotune", C.c_int32), ("graph", C.c_int32), ("spmv_in_sweep", C.c_int32),
-        ("ilu_order", C.c_int32),
+        ("ilu_order", C.c_int32), ("amg_coarsen_target", C.c_int32), ("amg_pre_smooth", C.c_int32),
+        ("amg_post_smooth", C.c_int32), ("amg_prolong_damp", C.c_double), ("amg_agg", C.c_int32 * 3),
     ]
 
 
@@ -145,6 +146,8 @@ SIGNATURES = {
     "omodels_scalar_product": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
     "omodels_scalar_product_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_solver_amg_levels": (C.c_int, [C.c_void_p, _ip, C.c_int]),
+    "omodels_solver_get_amg_level": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
     "omodels_model_get_switched": (C.c_int, [C.c_void_p, _u8p]),
     "omodels_lens_default_problem": (None, [C.POINTER(LensProblem)]),
@@ -268,7 +271,11 @@ def default_params(**kw):
     p = Params()
     lib().omodels_default_params(C.byref(p))
     for k, v in kw.items():
-        setattr(p, k, v)
+        if k == "amg_agg":
+            for q in range(3):
+                p.amg_agg[q] = int(v[q])
+        else:
+            setattr(p, k, v)
     return p
 
 
@@ -615,6 +622,19 @@ class Solver:
         v = np.zeros_like(d)
         check(lib().omodels_solver_apply_precond(self.h, ptr(d), ptr(v)), "apply_precond")
         return v
+
+    def amg_dims(self):
+        """level dimensions of the AMG hierarchy (precond = 2), [levels][3]"""
+        dims = np.zeros((16, 3), dtype=np.int32)
+        n = check(lib().omodels_solver_amg_levels(self.h, ptr(dims, _ip), 16), "amg_levels")
+        return dims[:n]
+
+    def amg_level(self, level):
+        """level `level`'s matrix in the structured layout [7][N_l][9]"""
+        d = self.amg_dims()[level]
+        a = np.zeros(63 * int(np.prod(d)))
+        check(lib().omodels_solver_get_amg_level(self.h, level, ptr(a)), "get_amg_level")
+        return a
 
 
 class Server:

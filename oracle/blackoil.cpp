@@ -32,6 +32,7 @@
 #include "ad.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -926,10 +927,12 @@ struct Criterion {
     bool failed() const { return !converged() && (stag || res > maxErr); }
 };
 
+struct Amg;
 struct Precond {
     int kind, bs, n; double relax; int order;
     const int32_t *rp, *ci;
     int nx = 0, ny = 0, nz = 0;     /* grid of the two-colour tile ordering (kind 2) */
+    const Amg* amg = nullptr;       /* kind 3 */
 };
 
 /* ------------------------------------------------------------------ */
@@ -1059,6 +1062,267 @@ void coloredIluApply(const TileColoring& tc, const int32_t* rp, const int32_t* c
         for (long k = 0; k < 3L * tc.nx * tc.ny * tc.nz; ++k) v[k] *= relax;
 }
 
+/* ------------------------------------------------------------------ */
+/* Aggregation AMG of a 3x3 system on the 7-point grid: the product's
+ * preconditioner in place of the reference's ParallelAmgBackend
+ * (parallelamgbackend.hh:85-316: BiCGStab preconditioned by dune-istl's
+ * aggregation AMG, SOR smoother with one iteration and relaxation 1 (:272-276),
+ * aggregates of about 3 cells per direction (:294-295), coarsening until
+ * AmgCoarsenTarget (:72, :290)).  dune-istl is not part of the reference, so
+ * its aggregation (strength of connection by the blocks' Frobenius norm), its
+ * cycle and its coarse solver are not restated; this is the product's own
+ * algorithm, which the product's kernels follow bit for bit:
+ *   - box aggregates of ax x ay x az cells (cell (i,j,k) -> (i/ax, j/ay,
+ *     k/az)), levels until one has at most `target` cells;
+ *   - Galerkin coarse operators P^T A P with piecewise-constant P: a coarse
+ *     block is the sum of the fine blocks between the two aggregates (again a
+ *     7-point pattern), summed over the fine cells of the aggregate in natural
+ *     order and, per cell, over its slots in order -z, -y, -x, I, +x, +y, +z;
+ *   - the V-cycle on level l for rhs b: x = 0; `pre` forward block
+ *     Gauss-Seidel sweeps (the first from x = 0); r = b - A x; the coarse rhs
+ *     is the sum of r over each aggregate; the coarse solution xc; x += damp *
+ *     P xc; `post` backward sweeps;
+ *   - Gauss-Seidel in the two-colour tile order of the two-colour ILU:
+ *     x_I = A_II^-1 (b_I - sum_{K != I} A_IK x_K) with the current x, the
+ *     neighbours in other tiles in column order, then those of the own tile;
+ *   - the coarsest level: x = M^-1 b with M^-1 by in-place Gauss-Jordan with
+ *     partial pivoting (the first largest |entry| of the column), rows
+ *     summed in column order. */
+struct AmgLevel {
+    int nx, ny, nz;
+    long N;
+    int ax = 1, ay = 1, az = 1;        /* aggregation to the next level */
+    std::vector<double> A;             /* [7][N][9], slots -z, -y, -x, I, +x, +y, +z */
+    std::vector<double> dinv;          /* [N][9] inverted diagonal blocks */
+};
+struct Amg {
+    std::vector<AmgLevel> lv;
+    std::vector<double> dense;         /* the coarsest level's inverse, [n][n] row-major */
+    int pre = 2, post = 2;
+    double damp = 1.6;
+};
+
+/* neighbour of I through slot s (0..6), or -1 */
+long slotNeighbour(int nx, int ny, int nz, long I, int s)
+{
+    const int i = (int)(I % nx), j = (int)((I / nx) % ny), k = (int)(I / ((long)nx * ny));
+    const long nxy = (long)nx * ny;
+    switch (s) {
+    case 0: return k > 0 ? I - nxy : -1;
+    case 1: return j > 0 ? I - nx : -1;
+    case 2: return i > 0 ? I - 1 : -1;
+    case 3: return I;
+    case 4: return i + 1 < nx ? I + 1 : -1;
+    case 5: return j + 1 < ny ? I + nx : -1;
+    default: return k + 1 < nz ? I + nxy : -1;
+    }
+}
+
+/* the sorted 7-point BCRS rows in the structured layout (absent blocks zero) */
+std::vector<double> diaFromBcrs(int nx, int ny, int nz, const int32_t* rp, const int32_t* ci, const double* J)
+{
+    const long N = (long)nx * ny * nz;
+    std::vector<double> A(63 * N, 0.0);
+    for (long I = 0; I < N; ++I)
+        for (int s = 0; s < 7; ++s) {
+            const long K = slotNeighbour(nx, ny, nz, I, s);
+            if (K < 0) continue;
+            const long b = blockAt(rp, ci, I, K);
+            if (b >= 0) std::copy(J + 9 * b, J + 9 * b + 9, A.begin() + 9 * (s * N + I));
+        }
+    return A;
+}
+
+/* level dims: aggregates of agg[] cells until a level has at most target
+ * cells (or no direction can be coarsened, or 16 levels) */
+std::vector<std::array<int, 3>> amgDims(int nx, int ny, int nz, const int agg[3], int target)
+{
+    std::vector<std::array<int, 3>> d{{nx, ny, nz}};
+    while ((long)d.back()[0] * d.back()[1] * d.back()[2] > target && d.size() < 16) {
+        const auto& f = d.back();
+        std::array<int, 3> c{(f[0] + agg[0] - 1) / agg[0], (f[1] + agg[1] - 1) / agg[1], (f[2] + agg[2] - 1) / agg[2]};
+        if ((long)c[0] * c[1] * c[2] == (long)f[0] * f[1] * f[2]) break;
+        d.push_back(c);
+    }
+    return d;
+}
+
+/* coarse A from the fine level f (aggregation f.ax x f.ay x f.az) */
+void amgGalerkin(const AmgLevel& f, AmgLevel& c)
+{
+    const long nxy = (long)f.nx * f.ny;
+    for (long a = 0; a < c.N; ++a) {
+        const int ci = (int)(a % c.nx), cj = (int)((a / c.nx) % c.ny), ck = (int)(a / ((long)c.nx * c.ny));
+        for (int e = 0; e < 9; ++e) {
+            double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int k = ck * f.az; k < std::min(f.nz, ck * f.az + f.az); ++k)
+                for (int j = cj * f.ay; j < std::min(f.ny, cj * f.ay + f.ay); ++j)
+                    for (int i = ci * f.ax; i < std::min(f.nx, ci * f.ax + f.ax); ++i) {
+                        const long I = i + (long)f.nx * j + nxy * k;
+                        for (int s = 0; s < 7; ++s) {
+                            const long J = slotNeighbour(f.nx, f.ny, f.nz, I, s);
+                            if (J < 0) continue;
+                            const int ji = (int)(J % f.nx), jj = (int)((J / f.nx) % f.ny), jk = (int)(J / nxy);
+                            const bool same = ji / f.ax == ci && jj / f.ay == cj && jk / f.az == ck;
+                            acc[same ? 3 : s] += f.A[9 * (s * f.N + I) + e];
+                        }
+                    }
+            for (int t = 0; t < 7; ++t) c.A[9 * (t * c.N + a) + e] = acc[t];
+        }
+    }
+}
+
+/* one Gauss-Seidel sweep, in place (zero: x = 0 before the sweep) */
+void amgGsSweep(const AmgLevel& L, const double* b, double* x, bool forward, bool zero)
+{
+    const TileColoring tc(L.nx, L.ny, L.nz);
+    std::vector<long> ord = tc.order();
+    if (!forward) std::reverse(ord.begin(), ord.end());
+    std::vector<char> done(L.N, 0);
+    std::vector<long> other, own;
+    auto blockOf = [&](long I, long K) {
+        for (int s = 0; s < 7; ++s)
+            if (s != 3 && slotNeighbour(L.nx, L.ny, L.nz, I, s) == K) return L.A.data() + 9 * (s * L.N + I);
+        return (const double*)nullptr;
+    };
+    for (long I : ord) {
+        double acc[3] = {b[3 * I], b[3 * I + 1], b[3 * I + 2]};
+        tc.neighbours(I, other, own);
+        for (long K : other) if (done[K] || !zero) matvecAcc(3, blockOf(I, K), x + 3 * K, acc, -1.0);
+        for (long K : own) if (done[K] || !zero) matvecAcc(3, blockOf(I, K), x + 3 * K, acc, -1.0);
+        double w[3] = {0, 0, 0};
+        matvecAcc(3, L.dinv.data() + 9 * I, acc, w, 1.0);
+        for (int e = 0; e < 3; ++e) x[3 * I + e] = w[e];
+        done[I] = 1;
+    }
+}
+
+/* r = b - A x, per row: the slots in order */
+void amgResidual(const AmgLevel& L, const double* b, const double* x, double* r)
+{
+    for (long I = 0; I < L.N; ++I) {
+        double acc[3] = {b[3 * I], b[3 * I + 1], b[3 * I + 2]};
+        for (int s = 0; s < 7; ++s) {
+            const long J = slotNeighbour(L.nx, L.ny, L.nz, I, s);
+            if (J >= 0) matvecAcc(3, L.A.data() + 9 * (s * L.N + I), x + 3 * J, acc, -1.0);
+        }
+        for (int e = 0; e < 3; ++e) r[3 * I + e] = acc[e];
+    }
+}
+
+/* coarse rhs: the sum of r over each aggregate, its cells in natural order */
+void amgRestrict(const AmgLevel& f, const AmgLevel& c, const double* r, double* bc)
+{
+    for (long a = 0; a < c.N; ++a) {
+        const int ci = (int)(a % c.nx), cj = (int)((a / c.nx) % c.ny), ck = (int)(a / ((long)c.nx * c.ny));
+        for (int e = 0; e < 3; ++e) {
+            double acc = 0.0;
+            for (int k = ck * f.az; k < std::min(f.nz, ck * f.az + f.az); ++k)
+                for (int j = cj * f.ay; j < std::min(f.ny, cj * f.ay + f.ay); ++j)
+                    for (int i = ci * f.ax; i < std::min(f.nx, ci * f.ax + f.ax); ++i)
+                        acc += r[3 * (i + (long)f.nx * (j + (long)f.ny * k)) + e];
+            bc[3 * a + e] = acc;
+        }
+    }
+}
+
+/* x += damp * P xc */
+void amgProlong(const AmgLevel& f, const AmgLevel& c, const double* xc, double* x, double damp)
+{
+    for (long I = 0; I < f.N; ++I) {
+        const int i = (int)(I % f.nx), j = (int)((I / f.nx) % f.ny), k = (int)(I / ((long)f.nx * f.ny));
+        const long a = i / f.ax + (long)c.nx * (j / f.ay + (long)c.ny * (k / f.az));
+        for (int e = 0; e < 3; ++e) x[3 * I + e] = x[3 * I + e] + damp * xc[3 * a + e];
+    }
+}
+
+/* in-place Gauss-Jordan inverse with partial pivoting; 2 when singular or
+ * not finite */
+int denseInverse(int n, double* M)
+{
+    for (long q = 0; q < (long)n * n; ++q)
+        if (!std::isfinite(M[q])) return 2;
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = std::abs(M[(long)k * n + k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(M[(long)i * n + k]) > best) { best = std::abs(M[(long)i * n + k]); p = i; }
+        if (!(best != 0.0)) return 2;
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(M[(long)k * n + j], M[(long)p * n + j]);
+        const double r = 1.0 / M[(long)k * n + k];
+        M[(long)k * n + k] = 1.0;
+        for (int j = 0; j < n; ++j) M[(long)k * n + j] *= r;
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const double f = M[(long)i * n + k];
+            M[(long)i * n + k] = 0.0;
+            for (int j = 0; j < n; ++j) M[(long)i * n + j] -= M[(long)k * n + j] * f;
+        }
+    }
+    for (int k = n - 1; k >= 0; --k)
+        if (piv[k] != k)
+            for (int i = 0; i < n; ++i) std::swap(M[(long)i * n + k], M[(long)i * n + piv[k]]);
+    return 0;
+}
+
+/* hierarchy from the fine matrix in the structured layout; 2 for a singular
+ * diagonal block or coarsest matrix */
+int amgSetup(Amg& amg, int nx, int ny, int nz, std::vector<double> A0, const int agg[3], int target)
+{
+    const auto dims = amgDims(nx, ny, nz, agg, target);
+    amg.lv.clear();
+    for (size_t l = 0; l < dims.size(); ++l) {
+        AmgLevel L;
+        L.nx = dims[l][0]; L.ny = dims[l][1]; L.nz = dims[l][2];
+        L.N = (long)L.nx * L.ny * L.nz;
+        if (l + 1 < dims.size()) { L.ax = agg[0]; L.ay = agg[1]; L.az = agg[2]; }
+        if (l == 0) L.A = std::move(A0);
+        else { L.A.assign(63 * L.N, 0.0); amgGalerkin(amg.lv.back(), L); }
+        L.dinv.resize(9 * L.N);
+        for (long I = 0; I < L.N; ++I) {
+            std::copy(L.A.begin() + 9 * (3 * L.N + I), L.A.begin() + 9 * (3 * L.N + I) + 9, L.dinv.begin() + 9 * I);
+            if (l + 1 < dims.size() && !invertBlock(3, L.dinv.data() + 9 * I)) return 2;
+        }
+        amg.lv.push_back(std::move(L));
+    }
+    const AmgLevel& C = amg.lv.back();
+    const int n = (int)(3 * C.N);
+    amg.dense.assign((long)n * n, 0.0);
+    for (long I = 0; I < C.N; ++I)
+        for (int s = 0; s < 7; ++s) {
+            const long J = slotNeighbour(C.nx, C.ny, C.nz, I, s);
+            if (J < 0) continue;
+            for (int e = 0; e < 3; ++e)
+                for (int f = 0; f < 3; ++f) amg.dense[(3 * I + e) * (long)n + 3 * J + f] = C.A[9 * (s * C.N + I) + 3 * e + f];
+        }
+    return denseInverse(n, amg.dense.data());
+}
+
+void amgCycle(const Amg& amg, size_t l, const double* b, double* x)
+{
+    const AmgLevel& L = amg.lv[l];
+    if (l + 1 == amg.lv.size()) {
+        const long n = 3 * L.N;
+        for (long i = 0; i < n; ++i) {
+            double s = amg.dense[i * n] * b[0];
+            for (long j = 1; j < n; ++j) s += amg.dense[i * n + j] * b[j];
+            x[i] = s;
+        }
+        return;
+    }
+    const AmgLevel& C = amg.lv[l + 1];
+    for (int k = 0; k < amg.pre; ++k) amgGsSweep(L, b, x, true, k == 0);
+    std::vector<double> r(3 * L.N), bc(3 * C.N), xc(3 * C.N);
+    amgResidual(L, b, x, r.data());
+    amgRestrict(L, C, r.data(), bc.data());
+    amgCycle(amg, l + 1, bc.data(), xc.data());
+    amgProlong(L, C, xc.data(), x, amg.damp);
+    for (int k = 0; k < amg.post; ++k) amgGsSweep(L, b, x, false, false);
+}
+
 /* Block Jacobi, Dune::SeqJac(A, order, w) as PreconditionerWrapperJacobi
  * builds it (istlpreconditionerwrappers.hh:84-89): `order` sweeps
  * x <- (1 - w) x + w D^{-1} (d - (L + U) x) starting from the output vector's
@@ -1101,8 +1365,12 @@ int bicgstab(int n, int bs, const int32_t* rp, const int32_t* ci, const S* A, co
     auto precApply = [&](std::vector<S>& out, const std::vector<S>& in) {
         if (pc.kind == 0) ilu0Apply<S>(n, bs, rp, ci, M, in.data(), out.data(), (S)pc.relax);
         else if (pc.kind == 1) jacobiSweeps<S>(pc, A, M, in, out);
-        else if constexpr (sizeof(S) == 8)
-            coloredIluApply(TileColoring(pc.nx, pc.ny, pc.nz), rp, ci, A, M, in.data(), out.data(), pc.relax);
+        else if constexpr (sizeof(S) == 8) {
+            if (pc.kind == 2)
+                coloredIluApply(TileColoring(pc.nx, pc.ny, pc.nz), rp, ci, A, M, in.data(), out.data(), pc.relax);
+            else
+                amgCycle(*pc.amg, 0, in.data(), out.data());
+        }
     };
     x.assign(m, S(0));
     std::vector<S> r = b;
@@ -1162,6 +1430,13 @@ int bicgstab(int n, int bs, const int32_t* rp, const int32_t* ci, const S* A, co
     return 1;
 }
 
+bool amgParamsValid(const ob_params* p)
+{
+    return p->amg_coarsen_target >= 1 && p->amg_coarsen_target <= 64 && p->amg_pre_smooth >= 1 &&
+           p->amg_post_smooth >= 0 && p->amg_agg[0] >= 1 && p->amg_agg[1] >= 1 && p->amg_agg[2] >= 1 &&
+           std::isfinite(p->amg_prolong_damp);
+}
+
 /* ParallelBaseBackend::solve (parallelbasebackend.hh:252-284) +
  * ParallelBiCGStabSolverBackend::prepareSolver_ (parallelbicgstabbackend.hh:131-160):
  * copy (and convert to the solver scalar, overlappingbcrsmatrix.hh:220-262),
@@ -1175,7 +1450,18 @@ int backendSolve(int n, int bs, const int32_t* rp, const int32_t* ci, const doub
     std::vector<S> A(J, J + (long)bb * nnzb);
     std::vector<S> M;
     const bool colored = prm->precond == 0 && prm->ilu_order == 1;
-    if (colored) {
+    Amg amg;
+    if (prm->precond == 2) {
+        if constexpr (sizeof(S) == 8) {
+            if (!dims || bs != 3 || (long)dims[0] * dims[1] * dims[2] != n || !amgParamsValid(prm)) return -1;
+            amg.pre = prm->amg_pre_smooth; amg.post = prm->amg_post_smooth; amg.damp = prm->amg_prolong_damp;
+            if (amgSetup(amg, dims[0], dims[1], dims[2], diaFromBcrs(dims[0], dims[1], dims[2], rp, ci, J), prm->amg_agg,
+                         prm->amg_coarsen_target) != 0)
+                return 2;
+        } else {
+            return -1;
+        }
+    } else if (colored) {
         if constexpr (sizeof(S) == 8) {
             if (!dims || bs != 3 || (long)dims[0] * dims[1] * dims[2] != n) return -1;
             M.resize((long)bb * n);
@@ -1197,8 +1483,9 @@ int backendSolve(int n, int bs, const int32_t* rp, const int32_t* ci, const doub
         }
     }
     std::vector<S> bv(b, b + (long)n * bs), x;
-    Precond pc{colored ? 2 : prm->precond, bs, n, prm->prec_relax, prm->prec_order, rp, ci};
+    Precond pc{colored ? 2 : (prm->precond == 2 ? 3 : prm->precond), bs, n, prm->prec_relax, prm->prec_order, rp, ci};
     if (dims) { pc.nx = dims[0]; pc.ny = dims[1]; pc.nz = dims[2]; }
+    pc.amg = &amg;
     double absTol = prm->lin_abs_tol < 0 ? prm->newton_tol / 100.0 : prm->lin_abs_tol;
     int rc = bicgstab<S>(n, bs, rp, ci, A.data(), M.data(), bv, x, pc, (S)prm->lin_tol, (S)absTol,
                          (S)prm->lin_max_error, prm->lin_max_iter, iters);
@@ -1291,7 +1578,9 @@ void ob_default_params(ob_params* p)
     p->lin_max_error = 1e7; p->lin_max_iter = 1000; p->precond = 0; p->prec_relax = 1.0;
     p->prec_order = 0; p->max_dt_divisions = 10; p->dp_max_rel = 0.3; p->ds_max = 0.2;
     p->privar_osc_thresh = 1e-5; p->max_dt = std::numeric_limits<double>::infinity();
-    p->min_dt = 0.0; p->solver_float = 0; p->verbose = 0;
+    p->min_dt = 0.0; p->solver_float = 0; p->verbose = 0; p->ilu_order = 0;
+    p->amg_coarsen_target = 32; p->amg_pre_smooth = 2; p->amg_post_smooth = 2; p->amg_prolong_damp = 1.6;
+    p->amg_agg[0] = p->amg_agg[1] = p->amg_agg[2] = 3;
 }
 
 int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2)
@@ -1509,6 +1798,65 @@ int ob_bicgstab_grid(int nx, int ny, int nz, const int32_t* row_ptr, const int32
     const int rc = backendSolve<double>(nx * ny * nz, 3, row_ptr, col_idx, vals, b, x, prm, &it, dims);
     *iterations = it;
     return rc;
+}
+
+/* the AMG hierarchy's level dimensions ([levels][3]); returns the number of
+ * levels (at most cap), -1 for invalid parameters */
+int ob_amg_dims(int nx, int ny, int nz, const ob_params* prm, int32_t* dims, int cap)
+{
+    if (!amgParamsValid(prm)) return -1;
+    const auto d = amgDims(nx, ny, nz, prm->amg_agg, prm->amg_coarsen_target);
+    for (size_t l = 0; l < d.size() && (int)l < cap; ++l)
+        for (int q = 0; q < 3; ++q) dims[3 * l + q] = d[l][q];
+    return (int)d.size();
+}
+
+/* level `level`'s matrix in the structured layout [7][N_l][9] */
+int ob_amg_level_matrix(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                        const ob_params* prm, int level, double* A7)
+{
+    if (!amgParamsValid(prm)) return -1;
+    Amg amg;
+    const int rc = amgSetup(amg, nx, ny, nz, diaFromBcrs(nx, ny, nz, row_ptr, col_idx, vals), prm->amg_agg,
+                            prm->amg_coarsen_target);
+    if (level < 0 || level >= (int)amg.lv.size()) return -1;
+    std::copy(amg.lv[level].A.begin(), amg.lv[level].A.end(), A7);
+    return rc;
+}
+
+/* v = one V-cycle applied to d */
+int ob_amg_apply(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                 const ob_params* prm, const double* d, double* v)
+{
+    if (!amgParamsValid(prm)) return -1;
+    Amg amg;
+    amg.pre = prm->amg_pre_smooth; amg.post = prm->amg_post_smooth; amg.damp = prm->amg_prolong_damp;
+    if (amgSetup(amg, nx, ny, nz, diaFromBcrs(nx, ny, nz, row_ptr, col_idx, vals), prm->amg_agg,
+                 prm->amg_coarsen_target) != 0)
+        return 2;
+    amgCycle(amg, 0, d, v);
+    return 0;
+}
+
+/* one Gauss-Seidel sweep of the structured-layout matrix A7 in place */
+int ob_gs_sweep(int nx, int ny, int nz, const double* A7, const double* b, double* x, int forward, int zero)
+{
+    AmgLevel L;
+    L.nx = nx; L.ny = ny; L.nz = nz; L.N = (long)nx * ny * nz;
+    L.A.assign(A7, A7 + 63 * L.N);
+    L.dinv.resize(9 * L.N);
+    for (long I = 0; I < L.N; ++I) {
+        std::copy(A7 + 9 * (3 * L.N + I), A7 + 9 * (3 * L.N + I) + 9, L.dinv.begin() + 9 * I);
+        if (!invertBlock(3, L.dinv.data() + 9 * I)) return 2;
+    }
+    amgGsSweep(L, b, x, forward != 0, zero != 0);
+    return 0;
+}
+
+/* in-place inverse of a dense n x n matrix (row-major) */
+int ob_dense_inverse(int n, double* M)
+{
+    return denseInverse(n, M);
 }
 
 double ob_dot(int64_t m, const double* x, const double* y)

@@ -73,7 +73,8 @@ class ObParams(C.Structure):
         ("prec_relax", C.c_double), ("prec_order", C.c_int32), ("max_dt_divisions", C.c_int32),
         ("dp_max_rel", C.c_double), ("ds_max", C.c_double), ("privar_osc_thresh", C.c_double),
         ("max_dt", C.c_double), ("min_dt", C.c_double), ("solver_float", C.c_int32), ("verbose", C.c_int32),
-        ("ilu_order", C.c_int32),
+        ("ilu_order", C.c_int32), ("amg_coarsen_target", C.c_int32), ("amg_pre_smooth", C.c_int32),
+        ("amg_post_smooth", C.c_int32), ("amg_prolong_damp", C.c_double), ("amg_agg", C.c_int32 * 3),
     ]
 
 
@@ -120,6 +121,12 @@ def lib():
         L.ob_colored_ilu.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double]
         L.ob_bicgstab_grid.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
         L.ob_jacobi_apply.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double]
+        L.ob_amg_dims.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(ObParams), _ip, C.c_int]
+        L.ob_amg_level_matrix.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, C.POINTER(ObParams), C.c_int,
+                                          _dp]
+        L.ob_amg_apply.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, C.POINTER(ObParams), _dp, _dp]
+        L.ob_gs_sweep.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int]
+        L.ob_dense_inverse.argtypes = [C.c_int, _dp]
         L.ob_dot.restype = C.c_double
         L.ob_bicgstab.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
         L.ob_update_primary.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp, _u8p, _u8p,
@@ -155,7 +162,11 @@ def params(**kw):
     p = ObParams()
     lib().ob_default_params(C.byref(p))
     for k, v in kw.items():
-        setattr(p, k, v)
+        if k == "amg_agg":
+            for q in range(3):
+                p.amg_agg[q] = int(v[q])
+        else:
+            setattr(p, k, v)
     return p
 
 
@@ -329,6 +340,65 @@ def bicgstab_grid(dims, rp, ci, vals, b, prm):
     rc = lib().ob_bicgstab_grid(nx, ny, nz, P(rp, _ip), P(ci, _ip), P(np.ascontiguousarray(vals, dtype=np.float64)),
                                 P(np.ascontiguousarray(b, dtype=np.float64)), P(x), C.byref(prm), C.byref(its))
     return rc, x, its.value
+
+
+def amg_params(**kw):
+    """ob_params of the aggregation AMG (precond = 2); keywords as ob_params,
+    amg_agg as a 3-tuple"""
+    return params(precond=2, **kw)
+
+
+def amg_dims(dims, prm):
+    out = np.zeros((16, 3), dtype=np.int32)
+    n = lib().ob_amg_dims(*dims, C.byref(prm), P(out, _ip), 16)
+    return out[:max(n, 0)]
+
+
+def amg_level_matrix(dims, rp, ci, vals, prm, level):
+    """level `level`'s matrix of the AMG hierarchy, structured layout [7][N_l][9]"""
+    d = amg_dims(dims, prm)
+    A7 = np.zeros(63 * int(np.prod(d[level])))
+    rc = lib().ob_amg_level_matrix(*dims, P(rp, _ip), P(ci, _ip), P(np.ascontiguousarray(vals, dtype=np.float64)),
+                                   C.byref(prm), level, P(A7))
+    return rc, A7
+
+
+def amg_apply(dims, rp, ci, vals, prm, d):
+    """one V-cycle of the AMG built from the matrix: v = M^-1 d"""
+    v = np.zeros(3 * int(np.prod(dims)))
+    rc = lib().ob_amg_apply(*dims, P(rp, _ip), P(ci, _ip), P(np.ascontiguousarray(vals, dtype=np.float64)),
+                            C.byref(prm), P(np.ascontiguousarray(d, dtype=np.float64)), P(v))
+    return rc, v
+
+
+def gs_sweep(dims, A7, b, x0, forward=True, zero=False):
+    """one block Gauss-Seidel sweep (two-colour tile order) of a structured-layout matrix"""
+    x = np.array(x0, dtype=np.float64, copy=True)
+    rc = lib().ob_gs_sweep(*dims, P(np.ascontiguousarray(A7, dtype=np.float64)),
+                           P(np.ascontiguousarray(b, dtype=np.float64)), P(x), int(forward), int(zero))
+    return rc, x
+
+
+def dense_inverse(M):
+    A = np.array(M, dtype=np.float64, copy=True, order="C")
+    rc = lib().ob_dense_inverse(A.shape[0], P(A))
+    return rc, A
+
+
+def dia_from_bcrs(dims, rp, ci, vals):
+    """the 7-point BCRS matrix in the structured layout [7][N][9] (absent blocks zero)"""
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    A7 = np.zeros((7, N, 9))
+    vals = np.asarray(vals).reshape(-1, 9)
+    off = (-nx * ny, -nx, -1, 0, 1, nx, nx * ny)
+    for I in range(N):
+        i, j, k = I % nx, (I // nx) % ny, I // (nx * ny)
+        row = {int(ci[q]): q for q in range(rp[I], rp[I + 1])}
+        for s, present in enumerate((k > 0, j > 0, i > 0, True, i + 1 < nx, j + 1 < ny, k + 1 < nz)):
+            if present and I + off[s] in row:
+                A7[s, I] = vals[row[I + off[s]]]
+    return A7.reshape(-1)
 
 
 def jacobi_apply(n, bs, rp, ci, vals, d, v0, order, relax=1.0):
