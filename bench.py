@@ -21,7 +21,7 @@ first requests of the same stream for comparison (solves/s, and the pressure
 difference of the served results against it).
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--config 500k|c2|c3]
-                  [--precision 0|1] [--no-cpu] [--no-serve]
+                  [--precision 0|1] [--ilu-order 0|1] [--precond 0|2] [--no-cpu] [--no-serve]
                   [--serve-requests R] [--serve-rate REQ_PER_S] [--serve-plain P]
 """
 import argparse
@@ -98,6 +98,52 @@ def algorithmic_bytes(N, F, nnzb, n=3, s=8, colored=False):
         # the blocks of A once (FP64), the pivots written
         ab["ilu_factor"] = nnzb * n * n * 8 + N * n * n * 8
     return ab
+
+
+def amg_levels(nx, ny, nz, agg=(3, 3, 3), target=32):
+    """level dimensions of the aggregation AMG (precond 2): boxes of agg cells
+    until a level has at most target cells"""
+    d = [(nx, ny, nz)]
+    while d[-1][0] * d[-1][1] * d[-1][2] > target and len(d) < 16:
+        c = tuple(-(-d[-1][q] // agg[q]) for q in range(3))
+        if c == d[-1]:
+            break
+        d.append(c)
+    return d
+
+
+def amg_bytes(levels, pre=2, post=2, n=3, s=8):
+    """algorithmic HBM bytes of one V-cycle and of one setup.  Per level of N
+    cells and NNZB blocks: a sweep reads the off-diagonal blocks, the inverted
+    diagonal blocks, b and x and writes x (the first, from x = 0, reads only
+    the blocks of neighbours swept before a cell: half of the off-diagonal
+    ones, and no x); the residual reads the blocks, b and x and writes r; the
+    restriction reads r; the prolongation reads and writes x.  The coarsest
+    level reads its dense inverse.  Setup: every level's matrix read once
+    (structured layout, 7 slots), coarse matrices and inverted diagonal
+    blocks written."""
+    bb = n * n * s
+    cyc = 0
+    setup = 0
+    for l, (nx, ny, nz) in enumerate(levels):
+        N = nx * ny * nz
+        nnzb = N + 2 * ((nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1))
+        vec = n * s * N
+        if l + 1 == len(levels):
+            cyc += (n * N) ** 2 * 8 + 2 * vec
+            setup += 7 * bb * N + (n * N) ** 2 * 8
+            continue
+        Nc = 1
+        for q in levels[l + 1]:
+            Nc *= q
+        zero_sweep = ((nnzb - N) // 2 + N) * bb + 2 * vec
+        sweep = nnzb * bb + 3 * vec
+        cyc += zero_sweep + (pre - 1 + post) * sweep
+        cyc += nnzb * bb + 3 * vec                 # residual
+        cyc += vec + n * s * Nc                    # restriction
+        cyc += 2 * vec + n * s * Nc                # prolongation
+        setup += 7 * bb * N + bb * N + (7 * bb * Nc)
+    return cyc, setup
 
 
 def serve_workload(om, np, n_req, rate, n_steps, n_plain, seed=77, progress=False):
@@ -228,23 +274,26 @@ def beside_runs(om, np, config, dt, steps=2):
     """Newton iterations/s of workloads reported beside the headline: the
     432k-cell black-oil grid (120x120x30) in FP64, and the headline grid with
     the FP32 inner solve inside FP64 refinement, with the natural-order and
-    with the two-colour tile ILU(0), and in FP64 with the two-colour ILU(0).
+    with the two-colour tile ILU(0), in FP64 with the two-colour ILU(0), and
+    both grids in FP64 with the aggregation AMG (the ParallelAmgBackend form).
     One untimed step each first, from the same initial state as the headline;
     the error of each against the FP64 natural-order run (the headline's
     arithmetic) is the largest relative pressure difference after the timed
     steps.  ILU-apply kernel time from HIP events on every 4th launch."""
     out = {}
-    runs = [("c3_fp64", "c3", 0, 0), ("%s_fp32_inner" % config, config, 1, 0),
-            ("%s_fp32_inner_colored" % config, config, 1, 1), ("%s_fp64_colored" % config, config, 0, 1),
-            ("c3_fp32_inner_colored", "c3", 1, 1)]
+    runs = [("c3_fp64", "c3", 0, 0, 0), ("%s_fp32_inner" % config, config, 1, 0, 0),
+            ("%s_fp32_inner_colored" % config, config, 1, 1, 0), ("%s_fp64_colored" % config, config, 0, 1, 0),
+            ("c3_fp32_inner_colored", "c3", 1, 1, 0), ("%s_fp64_amg" % config, config, 0, 0, 2),
+            ("c3_fp64_amg", "c3", 0, 0, 2)]
     ref = {}
-    for key, cfg_name, precision, order in [("c3_fp64", "c3", 0, 0)] + runs + [("ref", config, 0, 0)]:
+    for key, cfg_name, precision, order, precond in [("c3_fp64", "c3", 0, 0, 0)] + runs + [("ref", config, 0, 0, 0)]:
         if key in out:
             continue
         note("beside: " + key)
         cfg = CONFIGS[cfg_name]
         case = om.BlackOilCase(om.make_spec(cfg["nx"], cfg["ny"], cfg["nz"], seed=2024, **cfg.get("spec", {})))
-        prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, ilu_order=order, autotune=1)
+        prm = om.default_params(max_dt=dt, min_dt=dt, precision=precision, ilu_order=order, autotune=1,
+                                precond=precond)
         model = om.Model(case)
         model.set_solution(case.pv0, case.mean0)
         rc, _, _, _, _ = model.simulate(prm, 1, dt, keep_history=False)
@@ -256,19 +305,23 @@ def beside_runs(om, np, config, dt, steps=2):
         kt = model.kernel_times()
         ab = algorithmic_bytes(model.N, (model.nnzb - model.N) // 2, model.nnzb, s=8 if precision == 0 else 4,
                                colored=order == 1)
+        if precond == 2:
+            ab["ilu_apply"] = amg_bytes(amg_levels(cfg["nx"], cfg["ny"], cfg["nz"]))[0]
         model.close()
-        if key == "ref" or (precision == 0 and order == 0):
+        if key == "ref" or (precision == 0 and order == 0 and precond == 0):
             ref[cfg_name] = pv
         if key == "ref":
             continue
         ms, cnt = kt["ilu_apply"]
         out[key] = {"workload": cfg["label"], "precision": "fp64" if precision == 0 else "fp32-inner/fp64-outer",
-                    "ilu_order": "natural" if order == 0 else "two-colour 4x2x8 tiles",
+                    "preconditioner": ("aggregation AMG" if precond == 2 else
+                                       "block-ILU0, " + ("natural order" if order == 0 else "two-colour 4x2x8 tiles")),
                     "newton_iters_per_s": round(st.newton_iters / wall, 3),
                     "newton_iters": st.newton_iters,
                     "linear_iters_per_newton": round(st.linear_iters / max(st.newton_iters, 1), 2),
-                    "ilu_apply_avg_us": round(ms / max(cnt, 1) * 1e3, 1),
-                    "ilu_apply_achieved_GBs": round(ab["ilu_apply"] / max(ms / max(cnt, 1) * 1e-3, 1e-12) / 1e9, 1),
+                    "precond_apply_avg_us": round(ms / max(cnt, 1) * 1e3, 1),
+                    "precond_apply_achieved_GBs": round(ab["ilu_apply"] / max(ms / max(cnt, 1) * 1e-3, 1e-12) / 1e9,
+                                                        1),
                     "steps": steps, "ok": rc == 0 and rc2 == 0, "_pv": pv, "_cfg": cfg_name}
     for key, v in out.items():
         pv = v.pop("_pv")
@@ -285,6 +338,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="500k", choices=sorted(CONFIGS))
     ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--precond", type=int, default=0, choices=[0, 2],
+                    help="0: block ILU(0) (the reference's default backend); 2: the aggregation AMG (FP64)")
     ap.add_argument("--ilu-order", type=int, default=0, choices=[0, 1],
                     help="block ILU(0) order of the timed run: 0 natural (the reference's), 1 two-colour tiles")
     ap.add_argument("--no-cpu", action="store_true")
@@ -316,7 +371,7 @@ def main():
     case = om.BlackOilCase(spec)
     dt = 86400.0
     prm = om.default_params(max_dt=dt, min_dt=0.0 if cfg.get("adaptive") else dt, precision=args.precision,
-                            autotune=1, ilu_order=args.ilu_order)
+                            autotune=1, ilu_order=args.ilu_order, precond=args.precond)
 
     t_setup = time.time()
     model = om.Model(case)
@@ -348,20 +403,24 @@ def main():
     newton = st.newton_iters
     value = newton / wall
     ab = algorithmic_bytes(N, F, nnzb, s=8 if args.precision == 0 else 4, colored=args.ilu_order == 1)
+    if args.precond == 2:
+        # the preconditioner classes hold the V-cycle and the hierarchy's setup
+        ab["ilu_apply"], ab["ilu_factor"] = amg_bytes(amg_levels(cfg["nx"], cfg["ny"], cfg["nz"]))
     kernels = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0 or k not in ab:
             continue
         avg = ms / cnt
         gbs = ab[k] / (avg * 1e-3) / 1e9
-        kernels[k] = dict(total_ms=round(ms, 3), launches=cnt, avg_us=round(avg * 1e3, 2),
-                          bytes_per_launch=ab[k], achieved_GBs=round(gbs, 1))
+        name = {"ilu_apply": "amg_vcycle", "ilu_factor": "amg_setup"}.get(k, k) if args.precond == 2 else k
+        kernels[name] = dict(total_ms=round(ms, 3), launches=cnt, avg_us=round(avg * 1e3, 2),
+                             bytes_per_launch=ab[k], achieved_GBs=round(gbs, 1))
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     asm = kernels.get("assemble")
     roof = {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(kernels[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
             "traffic": None}
-    if args.config == "500k" and args.precision == 0 and args.ilu_order == 0:
+    if args.config == "500k" and args.precision == 0 and args.ilu_order == 0 and args.precond == 0:
         roof["traffic"] = HBM_TRAFFIC_500K.get(dom)
     elif args.config == "500k" and args.precision == 1 and args.ilu_order == 1:
         roof["traffic"] = HBM_TRAFFIC_500K_FP32_COLORED.get(dom)
@@ -381,7 +440,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": cfg["label"], "cells": N, "nnzb": nnzb,
                    "dt_days": "adaptive, at most 1.0" if cfg.get("adaptive") else 1.0,
-                   "precond": "block-ILU0" + (", two-colour 4x2x8 tiles" if args.ilu_order else ""),
+                   "precond": ("aggregation AMG (3x3x3 boxes, V-cycle, 2+2 Gauss-Seidel sweeps)"
+                               if args.precond == 2 else
+                               "block-ILU0" + (", two-colour 4x2x8 tiles" if args.ilu_order else "")),
                    "precision": "fp64" if args.precision == 0 else "fp32-inner/fp64-outer",
                    "parallelism": "single GPU"},
         "assembly_mcells_per_s": round(N * asm["launches"] / (asm["total_ms"] * 1e-3) / 1e6, 1) if asm else None,

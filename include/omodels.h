@@ -142,13 +142,13 @@ typedef struct omodels_params {
     /* precond = 2: box aggregates of amg_agg cells per direction (the
      * reference's aggregateSizePerDim = 3, parallelamgbackend.hh:294-295),
      * Galerkin coarse operators, levels until one has at most
-     * amg_coarsen_target cells (AmgCoarsenTarget, :72; here at most 64, the
+     * amg_coarsen_target cells (AmgCoarsenTarget, :72; here at most 32, the
      * coarsest level being solved by a dense inverse, where the reference
      * stops at 5000 with a sparse direct solver), a V-cycle with
      * amg_pre_smooth forward and amg_post_smooth backward block Gauss-Seidel
-     * sweeps (SOR with relaxation 1, :272-276; two-colour tile order) around
+     * sweeps (SOR with relaxation 1, :272-276; red-black order) around
      * the coarse correction, prolongated with the factor amg_prolong_damp */
-    int32_t amg_coarsen_target; /* 1..64 (default 32) */
+    int32_t amg_coarsen_target; /* 1..32 (default 32) */
     int32_t amg_pre_smooth;     /* >= 1 (default 2) */
     int32_t amg_post_smooth;    /* >= 0 (default 2) */
     double amg_prolong_damp;    /* default 1.6 */

@@ -7,13 +7,12 @@
 #include <memory>
 #include <vector>
 
-#include "ilu_colored.h"
 #include "omodels_internal.h"
 
 namespace omodels {
 
 // the coarsest level is solved by a dense inverse of at most this many cells
-constexpr int kAmgMaxCoarse = 64;
+constexpr int kAmgMaxCoarse = 32;
 
 struct AmgParams {
     int coarsenTarget = 32;     // levels until one has at most this many cells (1..kAmgMaxCoarse)
@@ -31,31 +30,33 @@ public:
     AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, hipStream_t s);
     AmgHierarchy(const AmgHierarchy&) = delete;
     AmgHierarchy& operator=(const AmgHierarchy&) = delete;
-    // the levels from the fine matrix A0 (structured layout [7][N][9], FP64,
-    // read again by every application): Galerkin coarse operators, inverted
-    // diagonal blocks, the coarsest level's dense inverse; fail |= 2 for a
-    // singular diagonal block or coarsest matrix
+    // the levels from the fine matrix A0 (structured layout [7][N][9], FP64):
+    // its red-black copy, Galerkin coarse operators, inverted diagonal
+    // blocks, the coarsest level's dense inverse; fail |= 2 for a singular
+    // diagonal block or coarsest matrix
     void setup(const double* A0, int* fail, hipStream_t s);
     // v = one V-cycle applied to d (device vectors of 3N); skipped while *run == 0
     void apply(const double* d, double* v, const int* run, hipStream_t s);
     int levels() const { return (int)lv_.size(); }
     std::array<int, 3> dims(int l) const { return {lv_[l]->nx, lv_[l]->ny, lv_[l]->nz}; }
-    // level l's matrix (structured layout), level 0 the one given to setup
-    const double* matrix(int l) const { return l == 0 ? A0_ : lv_[l]->A.p; }
+    // level l's matrix in the natural structured layout [7][N_l][9] (host)
+    void downloadLevel(int l, double* host, hipStream_t s) const;
 
 private:
     struct Level {
         int nx, ny, nz;
-        long N;
-        int ax = 1, ay = 1, az = 1;            // aggregation to the next level
-        DevBuf<double> A, dinv, b, x, r;
-        std::unique_ptr<ColoredIlu> tiles;      // the two-colour tile order of the sweeps
+        long N, N0;                             // cells, even cells
+        int ax = 1, ay = 1, az = 1;             // aggregation to the next level
+        DevBuf<double> A;                       // red-black layout: even cells' rows, then odd ones'
+        DevBuf<double> dinv, b, x, r;
+        DevBuf<int> rbOf, rbInv;                // natural cell -> red-black row, and back
+        std::vector<int> hostRbOf;
     };
     void cycle(size_t l, const double* b, double* x, const int* run, hipStream_t s);
     std::vector<std::unique_ptr<Level>> lv_;
-    const double* A0_ = nullptr;
     DevBuf<double> dense_;
     AmgParams p_;
+    bool ready_ = false;
 };
 
 }  // namespace omodels

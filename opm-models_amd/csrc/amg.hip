@@ -7,25 +7,34 @@
 // (:294-295), coarsening until AmgCoarsenTarget cells (:72, :290), a direct
 // solver on the coarsest level.  Here the grid itself gives the aggregates:
 //   - box aggregates of ax x ay x az cells; every level is again a 7-point
-//     grid, its matrix in the structured layout [7][N][9] (slots -z, -y, -x,
-//     I, +x, +y, +z), so the fine level's kernels serve every level;
+//     grid, so one set of kernels serves every level;
 //   - Galerkin coarse operators P^T A P with piecewise-constant P: a coarse
 //     block sums the fine blocks between the two aggregates, over the
 //     aggregate's cells in natural order and each cell's slots in order;
 //   - a multiplicative V-cycle: `pre` forward block Gauss-Seidel sweeps from
 //     x = 0, the residual summed over each aggregate as the coarse right-hand
 //     side, the coarse solution prolongated with damping, `post` backward
-//     sweeps; Gauss-Seidel in the two-colour tile order of ilu_colored.hip
-//     (every tile of a colour at once, the sweep a pass over the matrix);
+//     sweeps;
+//   - Gauss-Seidel in red-black order: the cells with i + j + k even, then
+//     the odd ones (backward: odd first); a cell's six neighbours have the
+//     other colour, so a colour is one launch with one independent lane per
+//     cell.  Each level keeps its matrix in a red-black layout [7][N][9]
+//     whose rows are the even cells in natural order, then the odd ones:
+//     the lanes of a colour read consecutive blocks of every slot (the
+//     natural layout would give them every other block).  Vectors stay in
+//     natural order;
 //   - the coarsest level (at most kAmgMaxCoarse cells) solved by its dense
 //     inverse, computed in place by Gauss-Jordan with partial pivoting in one
-//     workgroup.
+//     workgroup's LDS.
 // Every kernel fixes its operation order, which the CPU restatement follows.
 #include "amg.h"
 
 #include <cmath>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
+
+#include "ilu_colored.h"
 
 namespace omodels {
 
@@ -53,28 +62,88 @@ __device__ __forceinline__ long slotNeighbour(const AmgGrid& g, long I, int i, i
     }
 }
 
-// r = b - A x; one lane per row (cell t / 3, row t % 3), slots in order
-__global__ void k_amg_residual(AmgGrid g, const double* __restrict__ A, const double* __restrict__ b,
-                               const double* __restrict__ x, double* __restrict__ r, const int* __restrict__ run)
+// r = b - A x; one lane per row (red-black row L = t / 3 of cell I, row
+// e = t % 3), slots in order
+__global__ void k_amg_residual(AmgGrid g, const int* __restrict__ rbInv, const double* __restrict__ A,
+                               const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
+                               const int* __restrict__ run)
 {
     if (run && !*run) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 3 * g.N) return;
-    const long I = t / 3;
-    const int e = (int)(t - 3 * I);
+    const long L = t / 3;
+    const int e = (int)(t - 3 * L);
+    const long I = rbInv[L];
     const int i = (int)(I % g.nx), j = (int)((I / g.nx) % g.ny), k = (int)(I / ((long)g.nx * g.ny));
-    double acc = b[t];
+    double acc = b[3 * I + e];
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
         const long J = slotNeighbour(g, I, i, j, k, s);
         if (J < 0) continue;
-        const double* a = A + 9 * (s * g.N + I) + 3 * e;
+        const double* a = A + 9 * (s * g.N + L) + 3 * e;
         double v = a[0] * x[3 * J];
         v += a[1] * x[3 * J + 1];
         v += a[2] * x[3 * J + 2];
         acc += -1.0 * v;
     }
-    r[t] = acc;
+    r[3 * I + e] = acc;
+}
+
+// One colour of a block Gauss-Seidel sweep, in place: the rows [base,
+// base + count) of the red-black layout, x_I = A_II^-1 (b_I - sum_{K != I}
+// A_IK x_K), the neighbours in column order; skip: the neighbours' values
+// are zero (the first colour of a sweep from x = 0) and are not read.
+__global__ void k_amg_rb_gs(AmgGrid g, long base, long count, const int* __restrict__ rbInv,
+                            const double* __restrict__ A, const double* __restrict__ dinv,
+                            const double* __restrict__ b, double* __restrict__ x, int skip,
+                            const int* __restrict__ run)
+{
+    if (run && !*run) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const long L = base + t;
+    const long I = rbInv[L];
+    const int i = (int)(I % g.nx), j = (int)((I / g.nx) % g.ny), k = (int)(I / ((long)g.nx * g.ny));
+    double acc[3] = {b[3 * I], b[3 * I + 1], b[3 * I + 2]};
+    if (!skip) {
+        constexpr int kOrder[6] = {0, 1, 2, 4, 5, 6};
+#pragma unroll
+        for (int o = 0; o < 6; ++o) {
+            const int s = kOrder[o];
+            const long K = slotNeighbour(g, I, i, j, k, s);
+            if (K < 0) continue;
+            const double* a = A + 9 * (s * g.N + L);
+            const double x0 = x[3 * K], x1 = x[3 * K + 1], x2 = x[3 * K + 2];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                double v = a[3 * e] * x0;
+                v += a[3 * e + 1] * x1;
+                v += a[3 * e + 2] * x2;
+                acc[e] += -1.0 * v;
+            }
+        }
+    }
+    const double* Di = dinv + 9 * L;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        double o = Di[3 * e] * acc[0];
+        o += Di[3 * e + 1] * acc[1];
+        o += Di[3 * e + 2] * acc[2];
+        x[3 * I + e] = 0.0 + o;
+    }
+}
+
+// the natural structured layout in the red-black one: one lane per (row, entry)
+__global__ void k_amg_to_rb(AmgGrid g, const int* __restrict__ rbInv, const double* __restrict__ A,
+                            double* __restrict__ Arb)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 63 * g.N) return;
+    const long s = t / (9 * g.N);
+    const long rem = t - s * 9 * g.N;
+    const long L = rem / 9;
+    const int q = (int)(rem - 9 * L);
+    Arb[t] = A[9 * (s * g.N + rbInv[L]) + q];
 }
 
 // coarse right-hand side: the sum of r over each aggregate, natural order
@@ -109,22 +178,47 @@ __global__ void k_amg_prolong(AmgGrid f, AmgGrid c, int ax, int ay, int az, doub
     x[t] = x[t] + damp * xc[3 * a + e];
 }
 
-// coarse operator: one lane per (coarse cell, block entry); the fine blocks
-// of the aggregate's cells (natural order) and slots (in order) summed into
-// the coarse slot of the neighbour's aggregate
-__global__ void k_amg_galerkin(AmgGrid f, AmgGrid c, int ax, int ay, int az, const double* __restrict__ Af,
+// coarse operator: one lane per (coarse slot t, coarse cell, block entry).
+// The fine blocks of the aggregate's cells (natural order) and slots (in
+// order) are summed into the coarse slot of the neighbour's aggregate; each
+// lane forms one of those sums, so it visits only the cells that feed it:
+// for a face slot the aggregate's layer on that face, for the diagonal all
+// cells and their slots whose neighbour stays inside.  rbOfF: nullptr for a
+// fine matrix in the natural layout, else its red-black rows.
+__global__ void k_amg_galerkin(AmgGrid f, AmgGrid c, int ax, int ay, int az, const int* __restrict__ rbOfF,
+                               const int* __restrict__ rbOfC, const double* __restrict__ Af,
                                double* __restrict__ Ac)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 9 * c.N) return;
-    const long a = t / 9;
-    const int q = (int)(t - 9 * a);
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 63 * c.N) return;
+    const int t = (int)(g / (9 * c.N));
+    const long rem = g - (long)t * 9 * c.N;
+    const long a = rem / 9;
+    const int q = (int)(rem - 9 * a);
     const int ci = (int)(a % c.nx), cj = (int)((a / c.nx) % c.ny), ck = (int)(a / ((long)c.nx * c.ny));
-    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = ck * az; k < min(f.nz, ck * az + az); ++k)
-        for (int j = cj * ay; j < min(f.ny, cj * ay + ay); ++j)
-            for (int i = ci * ax; i < min(f.nx, ci * ax + ax); ++i) {
+    int k0 = ck * az, k1 = min(f.nz, ck * az + az), j0 = cj * ay, j1 = min(f.ny, cj * ay + ay);
+    int i0 = ci * ax, i1 = min(f.nx, ci * ax + ax);
+    // a face slot: only the aggregate's layer on that face (absent when the
+    // aggregate is clipped by the grid's edge on that side)
+    switch (t) {
+    case 0: k1 = k0 + 1; break;
+    case 1: j1 = j0 + 1; break;
+    case 2: i1 = i0 + 1; break;
+    case 4: i0 = ci * ax + ax - 1; break;
+    case 5: j0 = cj * ay + ay - 1; break;
+    case 6: k0 = ck * az + az - 1; break;
+    default: break;
+    }
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) {
                 const long I = i + (long)f.nx * (j + (long)f.ny * k);
+                const long row = rbOfF ? rbOfF[I] : I;
+                if (t != 3) {
+                    if (slotNeighbour(f, I, i, j, k, t) >= 0) acc += Af[9 * (t * f.N + row) + q];
+                    continue;
+                }
 #pragma unroll
                 for (int s = 0; s < 7; ++s) {
                     if (slotNeighbour(f, I, i, j, k, s) < 0) continue;
@@ -140,15 +234,15 @@ __global__ void k_amg_galerkin(AmgGrid f, AmgGrid c, int ax, int ay, int az, con
                     case 5: same = j + 1 < cj * ay + ay; break;
                     default: same = k + 1 < ck * az + az; break;
                     }
-                    acc[same ? 3 : s] += Af[9 * (s * f.N + I) + q];
+                    if (same) acc += Af[9 * (s * f.N + row) + q];
                 }
             }
-#pragma unroll
-    for (int s = 0; s < 7; ++s) Ac[9 * (s * c.N + a) + q] = acc[s];
+    Ac[9 * (t * c.N + rbOfC[a]) + q] = acc;
 }
 
 // the coarsest level as a dense row-major n x n matrix, n = 3N
-__global__ void k_amg_dense_build(AmgGrid g, const double* __restrict__ A, double* __restrict__ M)
+__global__ void k_amg_dense_build(AmgGrid g, const int* __restrict__ rbOf, const double* __restrict__ A,
+                                  double* __restrict__ M)
 {
     const long n = 3 * g.N;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -159,52 +253,65 @@ __global__ void k_amg_dense_build(AmgGrid g, const double* __restrict__ A, doubl
     const int i = (int)(I % g.nx), j = (int)((I / g.nx) % g.ny), k = (int)(I / ((long)g.nx * g.ny));
     double v = 0.0;
     for (int s = 0; s < 7; ++s)
-        if (slotNeighbour(g, I, i, j, k, s) == J) { v = A[9 * (s * g.N + I) + 3 * e + f]; break; }
+        if (slotNeighbour(g, I, i, j, k, s) == J) { v = A[9 * (s * g.N + rbOf[I]) + 3 * e + f]; break; }
     M[t] = v;
 }
 
-constexpr int kGjThreads = 256;
+constexpr int kGjThreads = 1024;
+constexpr int kDenseMax = 3 * kAmgMaxCoarse;
+constexpr int kDenseLd = kDenseMax + 1;      // LDS row stride: column reads hit distinct banks
 
-// in-place Gauss-Jordan inverse of the dense n x n matrix with partial
-// pivoting (the first row of the largest |entry| of the column), one
-// workgroup; fail |= 2 when an entry is not finite or a pivot is zero
+// Gauss-Jordan inverse of the dense n x n matrix M with partial pivoting (the
+// first row of the largest |entry| of the column), in one workgroup's LDS;
+// the inverse is written back transposed (M[j n + i] = M^-1_ij, so that the
+// application reads it by columns); fail |= 2 when an entry is not finite or
+// a pivot is zero.  Step k on the matrix S after steps 0..k-1: rows k and p
+// exchanged, row k scaled by r = 1 / S_pk,
+//     S'_kj = (j == k ? 1 : S_pj) r,
+//     S'_ij = (j == k ? 0 : S_sigma(i)j) - S'_kj S_sigma(i)k   (i != k),
+// sigma exchanging k and p; a step reads one LDS copy and writes the other.
+// The column exchanges of the pivots are applied when the result is written.
 __global__ __launch_bounds__(kGjThreads) void k_amg_dense_invert(int n, double* __restrict__ M, int* __restrict__ fail)
 {
-    __shared__ double fcol[3 * kAmgMaxCoarse];
-    __shared__ int piv[3 * kAmgMaxCoarse];
-    __shared__ double wv[kGjThreads / 64];
-    __shared__ int wi[kGjThreads / 64];
+    __shared__ double S[2][kDenseMax * kDenseLd];
+    __shared__ int piv[kDenseMax];
+    __shared__ int col[kDenseMax];
     __shared__ int bad;
     const int tid = (int)threadIdx.x;
-    const long nn = (long)n * n;
+    const int nn = n * n;
     if (tid == 0) bad = 0;
     __syncthreads();
-    for (long q = tid; q < nn; q += kGjThreads)
+    for (int q = tid; q < nn; q += kGjThreads) {
+        const int i = q / n, j = q - i * n;
+        S[0][i * kDenseLd + j] = M[q];
         if (!isfinite(M[q])) bad = 1;
+    }
     __syncthreads();
     if (bad) {
         if (tid == 0) atomicOr(fail, 2);
         return;
     }
+    int cur = 0;
     for (int k = 0; k < n; ++k) {
-        double best = -1.0;
-        int bi = n;
-        for (int i = k + tid; i < n; i += kGjThreads) {
-            const double v = fabs(M[(long)i * n + k]);
-            if (v > best || (v == best && i < bi)) { best = v; bi = i; }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_down(best, off, 64);
-            const int oi = __shfl_down(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        if ((tid & 63) == 0) { wv[tid >> 6] = best; wi[tid >> 6] = bi; }
-        __syncthreads();
-        if (tid == 0) {
-            for (int w = 1; w < kGjThreads / 64; ++w)
-                if (wv[w] > best || (wv[w] == best && wi[w] < bi)) { best = wv[w]; bi = wi[w]; }
-            piv[k] = bi;
-            if (!(best > 0.0)) bad = 1;
+        const double* A = S[cur];
+        double* B = S[cur ^ 1];
+        if (tid < 64) {
+            // pivot search by one wave: lane l looks at rows k + l, k + l + 64
+            double best = -1.0;
+            int bi = n;
+            for (int i = k + tid; i < n; i += 64) {
+                const double v = fabs(A[i * kDenseLd + k]);
+                if (v > best || (v == best && i < bi)) { best = v; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_down(best, off, 64);
+                const int oi = __shfl_down(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if (tid == 0) {
+                piv[k] = bi;
+                if (!(best > 0.0)) bad = 1;
+            }
         }
         __syncthreads();
         if (bad) {
@@ -212,48 +319,53 @@ __global__ __launch_bounds__(kGjThreads) void k_amg_dense_invert(int n, double* 
             return;
         }
         const int p = piv[k];
-        if (p != k)
-            for (int j = tid; j < n; j += kGjThreads) {
-                const double tmp = M[(long)k * n + j];
-                M[(long)k * n + j] = M[(long)p * n + j];
-                M[(long)p * n + j] = tmp;
+        const double r = 1.0 / A[p * kDenseLd + k];
+        for (int q = tid; q < nn; q += kGjThreads) {
+            const int i = q / n, j = q - i * n;
+            const double rowk = (j == k ? 1.0 : A[p * kDenseLd + j]) * r;
+            if (i == k) {
+                B[i * kDenseLd + j] = rowk;
+            } else {
+                const int si = i == p ? k : i;
+                B[i * kDenseLd + j] = (j == k ? 0.0 : A[si * kDenseLd + j]) - rowk * A[si * kDenseLd + k];
             }
-        __syncthreads();
-        const double r = 1.0 / M[(long)k * n + k];
-        for (int i = tid; i < n; i += kGjThreads) fcol[i] = M[(long)i * n + k];
-        __syncthreads();
-        for (int j = tid; j < n; j += kGjThreads) M[(long)k * n + j] = (j == k ? 1.0 : M[(long)k * n + j]) * r;
-        __syncthreads();
-        for (long q = tid; q < nn; q += kGjThreads) {
-            const int i = (int)(q / n), j = (int)(q - (long)i * n);
-            if (i == k) continue;
-            M[q] = (j == k ? 0.0 : M[q]) - M[(long)k * n + j] * fcol[i];
         }
+        cur ^= 1;
         __syncthreads();
     }
-    for (int k = n - 1; k >= 0; --k) {
-        const int p = piv[k];
-        if (p != k)
-            for (int i = tid; i < n; i += kGjThreads) {
-                const double tmp = M[(long)i * n + k];
-                M[(long)i * n + k] = M[(long)i * n + p];
-                M[(long)i * n + p] = tmp;
-            }
-        __syncthreads();
+    // the column exchanges k <-> piv[k], k descending: result column c holds column col[c]
+    if (tid == 0) {
+        for (int c = 0; c < n; ++c) col[c] = c;
+        for (int k = n - 1; k >= 0; --k) {
+            const int t = col[k];
+            col[k] = col[piv[k]];
+            col[piv[k]] = t;
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < nn; q += kGjThreads) {
+        const int i = q / n, c = q - i * n;
+        M[c * n + i] = S[cur][i * kDenseLd + col[c]];
     }
 }
 
-// x = M^-1 b on the coarsest level: one lane per row, columns in order
-__global__ void k_amg_dense_apply(int n, const double* __restrict__ Minv, const double* __restrict__ b,
-                                  double* __restrict__ x, const int* __restrict__ run)
+// x = M^-1 b on the coarsest level (the inverse stored transposed): the
+// inverse and b staged in LDS, then one lane per row, columns in order
+__global__ __launch_bounds__(kGjThreads) void k_amg_dense_apply(int n, const double* __restrict__ MinvT,
+                                                                const double* __restrict__ b,
+                                                                double* __restrict__ x, const int* __restrict__ run)
 {
     if (run && !*run) return;
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    const double* row = Minv + (long)i * n;
-    double s = row[0] * b[0];
-    for (int j = 1; j < n; ++j) s += row[j] * b[j];
-    x[i] = s;
+    __shared__ double S[kDenseMax * kDenseMax];
+    __shared__ double bs[kDenseMax];
+    const int tid = (int)threadIdx.x;
+    for (int q = tid; q < n * n; q += kGjThreads) S[q] = MinvT[q];
+    if (tid < n) bs[tid] = b[tid];
+    __syncthreads();
+    if (tid >= n) return;
+    double s = S[tid] * bs[0];
+    for (int j = 1; j < n; ++j) s += S[j * n + tid] * bs[j];
+    x[tid] = s;
 }
 
 unsigned blocksFor(long threads, int per) { return (unsigned)((threads + per - 1) / per); }
@@ -295,44 +407,67 @@ AmgHierarchy::AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, hipStream
         Level& L = *lv_[l];
         L.nx = dims[l][0]; L.ny = dims[l][1]; L.nz = dims[l][2];
         L.N = (long)L.nx * L.ny * L.nz;
+        if (L.N > INT32_MAX) throw std::invalid_argument("AMG: grid too large");
         const bool last = l + 1 == dims.size();
+        // red-black rows: the even cells (i + j + k even) in natural order, then the odd ones
+        std::vector<int> inv(L.N);
+        L.hostRbOf.resize(L.N);
+        long e = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (long I = 0; I < L.N; ++I) {
+                const int i = (int)(I % L.nx), j = (int)((I / L.nx) % L.ny), k = (int)(I / ((long)L.nx * L.ny));
+                if (((i + j + k) & 1) != pass) continue;
+                inv[e] = (int)I;
+                L.hostRbOf[I] = (int)e;
+                ++e;
+            }
+            if (pass == 0) L.N0 = e;
+        }
+        L.rbInv.upload(inv.data(), inv.size(), s);
+        L.rbOf.upload(L.hostRbOf.data(), L.hostRbOf.size(), s);
+        L.A.alloc(63 * L.N);
         if (!last) {
             L.ax = p.agg[0]; L.ay = p.agg[1]; L.az = p.agg[2];
             L.dinv.alloc(9 * L.N);
             L.r.alloc(3 * L.N);
-            L.tiles = std::make_unique<ColoredIlu>(L.nx, L.ny, L.nz, s);
         }
         if (l > 0) {
-            L.A.alloc(63 * L.N);
             L.b.alloc(3 * L.N);
             L.x.alloc(3 * L.N);
         }
     }
     const long n = 3 * lv_.back()->N;
     dense_.alloc(n * n);
+    OM_HIP(hipStreamSynchronize(s));     // the host copies of the maps are uploaded from
 }
 
 void AmgHierarchy::setup(const double* A0, int* fail, hipStream_t s)
 {
-    A0_ = A0;
+    {
+        Level& F = *lv_[0];
+        const AmgGrid f = gridOf(F.nx, F.ny, F.nz);
+        hipLaunchKernelGGL(k_amg_to_rb, dim3(blocksFor(63 * f.N, 256)), dim3(256), 0, s, f, F.rbInv.p, A0, F.A.p);
+    }
     for (size_t l = 0; l < lv_.size(); ++l) {
         Level& L = *lv_[l];
-        const double* A = matrix((int)l);
         if (l > 0) {
             const Level& F = *lv_[l - 1];
             const AmgGrid f = gridOf(F.nx, F.ny, F.nz), c = gridOf(L.nx, L.ny, L.nz);
-            hipLaunchKernelGGL(k_amg_galerkin, dim3(blocksFor(9 * c.N, 256)), dim3(256), 0, s, f, c, F.ax, F.ay,
-                               F.az, matrix((int)l - 1), L.A.p);
+            // the first coarse level from the caller's natural layout (whole
+            // cache lines), the others from the red-black copies
+            hipLaunchKernelGGL(k_amg_galerkin, dim3(blocksFor(63 * c.N, 256)), dim3(256), 0, s, f, c, F.ax, F.ay,
+                               F.az, l == 1 ? nullptr : F.rbOf.p, L.rbOf.p, l == 1 ? A0 : F.A.p, L.A.p);
         }
-        if (l + 1 < lv_.size()) launchDiaBlockInverse(L.N, A, L.dinv.p, fail, s);
+        if (l + 1 < lv_.size()) launchDiaBlockInverse(L.N, L.A.p, L.dinv.p, fail, s);
     }
     const Level& C = *lv_.back();
     const AmgGrid c = gridOf(C.nx, C.ny, C.nz);
     const long n = 3 * C.N;
-    hipLaunchKernelGGL(k_amg_dense_build, dim3(blocksFor(n * n, 256)), dim3(256), 0, s, c, matrix((int)lv_.size() - 1),
+    hipLaunchKernelGGL(k_amg_dense_build, dim3(blocksFor(n * n, 256)), dim3(256), 0, s, c, C.rbOf.p, C.A.p,
                        dense_.p);
     hipLaunchKernelGGL(k_amg_dense_invert, dim3(1), dim3(kGjThreads), 0, s, (int)n, dense_.p, fail);
     OM_HIP(hipGetLastError());
+    ready_ = true;
 }
 
 void AmgHierarchy::cycle(size_t l, const double* b, double* x, const int* run, hipStream_t s)
@@ -340,27 +475,51 @@ void AmgHierarchy::cycle(size_t l, const double* b, double* x, const int* run, h
     Level& L = *lv_[l];
     if (l + 1 == lv_.size()) {
         const int n = (int)(3 * L.N);
-        hipLaunchKernelGGL(k_amg_dense_apply, dim3(blocksFor(n, 256)), dim3(256), 0, s, n, dense_.p, b, x, run);
+        hipLaunchKernelGGL(k_amg_dense_apply, dim3(1), dim3(kGjThreads), 0, s, n, dense_.p, b, x, run);
         return;
     }
     Level& C = *lv_[l + 1];
-    const double* A = matrix((int)l);
     const AmgGrid f = gridOf(L.nx, L.ny, L.nz), c = gridOf(C.nx, C.ny, C.nz);
-    for (int k = 0; k < p_.pre; ++k) L.tiles->gsSweep<double>(A, L.dinv.p, b, x, true, k == 0, s, run);
-    hipLaunchKernelGGL(k_amg_residual, dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, A, b, x, L.r.p, run);
+    // one colour: rows [base, base + count) of the red-black layout
+    auto colour = [&](int col, bool skip) {
+        const long base = col == 0 ? 0 : L.N0, count = col == 0 ? L.N0 : L.N - L.N0;
+        if (count == 0) return;
+        hipLaunchKernelGGL(k_amg_rb_gs, dim3(blocksFor(count, 256)), dim3(256), 0, s, f, base, count, L.rbInv.p,
+                           L.A.p, L.dinv.p, b, x, skip ? 1 : 0, run);
+    };
+    for (int k = 0; k < p_.pre; ++k) {
+        colour(0, k == 0);          // from x = 0 the even cells' neighbours are still zero
+        colour(1, false);
+    }
+    hipLaunchKernelGGL(k_amg_residual, dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, L.rbInv.p, L.A.p, b, x,
+                       L.r.p, run);
     hipLaunchKernelGGL(k_amg_restrict, dim3(blocksFor(3 * c.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az,
                        L.r.p, C.b.p, run);
     cycle(l + 1, C.b.p, C.x.p, run, s);
     hipLaunchKernelGGL(k_amg_prolong, dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az, p_.damp,
                        C.x.p, x, run);
-    for (int k = 0; k < p_.post; ++k) L.tiles->gsSweep<double>(A, L.dinv.p, b, x, false, false, s, run);
+    for (int k = 0; k < p_.post; ++k) {
+        colour(1, false);
+        colour(0, false);
+    }
 }
 
 void AmgHierarchy::apply(const double* d, double* v, const int* run, hipStream_t s)
 {
-    if (!A0_) throw std::logic_error("AMG: apply before setup");
+    if (!ready_) throw std::logic_error("AMG: apply before setup");
     cycle(0, d, v, run, s);
     OM_HIP(hipGetLastError());
+}
+
+void AmgHierarchy::downloadLevel(int l, double* host, hipStream_t s) const
+{
+    const Level& L = *lv_[l];
+    std::vector<double> rb(63 * L.N);
+    OM_HIP(hipMemcpyAsync(rb.data(), L.A.p, rb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OM_HIP(hipStreamSynchronize(s));
+    for (int sl = 0; sl < 7; ++sl)
+        for (long I = 0; I < L.N; ++I)
+            for (int q = 0; q < 9; ++q) host[9 * (sl * L.N + I) + q] = rb[9 * (sl * L.N + L.hostRbOf[I]) + q];
 }
 
 }  // namespace omodels

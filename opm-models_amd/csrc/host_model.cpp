@@ -927,10 +927,7 @@ void BiCGStabBackend::downloadAmgLevel(int level, double* hostValues)
     const AmgHierarchy& a = amg();
     if (!prepared_) throw std::logic_error("AMG levels before a successful prepare");
     if (level < 0 || level >= a.levels()) throw std::invalid_argument("AMG level out of range");
-    const auto d = a.dims(level);
-    const size_t n = 63UL * d[0] * d[1] * d[2];
-    OM_HIP(hipMemcpyAsync(hostValues, a.matrix(level), n * sizeof(double), hipMemcpyDeviceToHost, m_.stream));
-    m_.sync();
+    a.downloadLevel(level, hostValues, m_.stream);
 }
 
 void BiCGStabBackend::applyPrecond(const double* hostD, double* hostV)

@@ -55,8 +55,8 @@ private:
     std::vector<ColInst> host_[3];
 };
 
-// dinv[N][9] = inverses of the diagonal blocks of A ([7][N][9] structured
-// layout); fail |= 2 for a singular block
+// dinv[N][9] = inverses of the diagonal blocks of A (slot 3 of a [7][N][9]
+// structured layout, rows in any order); fail |= 2 for a singular block
 void launchDiaBlockInverse(long N, const double* A, double* dinv, int* fail, hipStream_t s);
 
 class ColoredIlu {
@@ -77,14 +77,6 @@ public:
     template <class S>
     void apply(const int* rp, const S* A, const double* dinv, const S* d, S* v, double relax, hipStream_t s,
                const int* run = nullptr);
-    // one block Gauss-Seidel sweep (SOR, relaxation 1) in the same ordering,
-    // in place on x: x_I = A_II^-1 (b_I - sum_{K != I} A_IK x_K) with the
-    // current values; forward (colour 0 first, cells ascending) or backward;
-    // zero: x is zero before the sweep and is not read.  A in the structured
-    // layout, dinv the inverted diagonal blocks (launchDiaBlockInverse).
-    template <class S>
-    void gsSweep(const S* A, const double* dinv, const S* b, S* x, bool forward, bool zero, hipStream_t s,
-                 const int* run = nullptr);
     int nx() const { return nx_; }
     int ny() const { return ny_; }
     int nz() const { return nz_; }

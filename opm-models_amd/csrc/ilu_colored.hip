@@ -388,116 +388,6 @@ __device__ __forceinline__ void colBackwardTile(const ColGeom& g, int color, int
     }
 }
 
-// Block Gauss-Seidel (SOR with relaxation 1) over the tiles of one colour,
-// in place, A in the structured layout:
-//     x_I = A_II^-1 (b_I - sum_{K != I} A_IK x_K)
-// with the current x: the new values of the cells the sweep has passed, the
-// old ones of the others.  Forward: colour 0 then colour 1, a tile's cells
-// ascending; backward: colour 1 then colour 0, descending.  zero: x is zero
-// before the sweep (old values are not read).  dinv: the inverted diagonal
-// blocks A_II^-1.  Operation order per cell: the neighbours in other tiles in
-// column order, then those of its own tile in column order (-z, -y, -x below
-// it, +x, +y, +z above).
-template <class S, bool Fwd>
-__device__ __forceinline__ void colGsTile(const ColGeom& g, int color, int tile, const S* __restrict__ A,
-                                          const double* __restrict__ dinv, const S* __restrict__ b,
-                                          S* __restrict__ x, bool zero)
-{
-    const int lane = (int)(threadIdx.x & 63u);
-    int gx, gy, gz, lx, ly, lz;
-    const bool valid = laneCell(g, tile, lane, gx, gy, gz, lx, ly, lz);
-    const long I = ((long)gz * g.ny + gy) * g.nx + gx;
-    // the tiles of the other colour were swept before this colour's
-    const bool otherNew = Fwd ? color == 1 : color == 0;
-    S acc[3] = {S(0), S(0), S(0)};
-    S lo[3][9], up[3][9];            // in-tile neighbour blocks below (-z, -y, -x) and above (+x, +y, +z)
-    S old[3][3];                     // old values of the in-tile neighbours swept after this cell
-    bool hasLo[3] = {false, false, false}, hasUp[3] = {false, false, false};
-    double Di[9];
-    if (valid) {
-        const RowPos pos = diaPositions(g, I, gx, gy, gz);
-#pragma unroll
-        for (int e = 0; e < 3; ++e) acc[e] = b[3 * I + e];
-        if (otherNew || !zero) {
-#pragma unroll
-            for (int o = 0; o < 6; ++o) {
-                const int dir = kColOrder[o];
-                if (pos.blk[dir] < 0 || !crossesTile(dir, lx, ly, lz)) continue;
-                const long K = neighbourCell(g, I, dir);
-                subBlockVec(A + 9 * pos.blk[dir], x[3 * K], x[3 * K + 1], x[3 * K + 2], acc);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            hasLo[q] = inTileLower(q, lx, ly, lz);
-            hasUp[q] = inTileUpper(q, lx, ly, lz) && pos.blk[kInUpperDir[q]] >= 0;
-            const long atL = hasLo[q] ? pos.blk[kInLowerDir[q]] : pos.diag;   // a valid address either way
-            const long atU = hasUp[q] ? pos.blk[kInUpperDir[q]] : pos.diag;
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                lo[q][c] = hasLo[q] ? A[9 * atL + c] : S(0);
-                up[q][c] = hasUp[q] ? A[9 * atU + c] : S(0);
-            }
-            const bool later = Fwd ? hasUp[q] : hasLo[q];
-            const long K = later ? neighbourCell(g, I, Fwd ? kInUpperDir[q] : kInLowerDir[q]) : I;
-#pragma unroll
-            for (int e = 0; e < 3; ++e) old[q][e] = (later && !zero) ? x[3 * K + e] : S(0);
-        }
-#pragma unroll
-        for (int c = 0; c < 9; ++c) Di[c] = dinv[9 * I + c];
-    }
-    S xI[3] = {S(0), S(0), S(0)};
-    const int lev = lx + ly + lz;
-    for (int step = 0; step < kLevels; ++step) {
-        const int l = Fwd ? step : kLevels - 1 - step;
-        // new values of the in-tile neighbours swept before this cell
-        S nb[3][3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int src = Fwd ? (lane - kInLowerLane[q] < 0 ? lane : lane - kInLowerLane[q])
-                                : (lane + kInUpperLane[q] > 63 ? lane : lane + kInUpperLane[q]);
-#pragma unroll
-            for (int e = 0; e < 3; ++e) nb[q][e] = __shfl(xI[e], src, 64);
-        }
-        if (valid && lev == l) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (!hasLo[q]) continue;
-                if (Fwd) subBlockVec(lo[q], nb[q][0], nb[q][1], nb[q][2], acc);
-                else if (!zero) subBlockVec(lo[q], old[q][0], old[q][1], old[q][2], acc);
-            }
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (!hasUp[q]) continue;
-                if (!Fwd) subBlockVec(up[q], nb[q][0], nb[q][1], nb[q][2], acc);
-                else if (!zero) subBlockVec(up[q], old[q][0], old[q][1], old[q][2], acc);
-            }
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                double o = Di[3 * e] * (double)acc[0];
-                o += Di[3 * e + 1] * (double)acc[1];
-                o += Di[3 * e + 2] * (double)acc[2];
-                xI[e] = (S)(0.0 + o);
-            }
-        }
-    }
-    if (valid) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) x[3 * I + e] = xI[e];
-    }
-}
-
-template <class S, bool Fwd>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_col_gs(ColGeom g, int color, const int* __restrict__ tiles,
-    int nTiles, const S* __restrict__ A, const double* __restrict__ dinv, const S* __restrict__ b,
-    S* __restrict__ x, int zero, const int* __restrict__ run)
-{
-    if (run && !*run) return;
-    const int w = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6);
-    if (w >= nTiles) return;
-    colGsTile<S, Fwd>(g, color, tiles[w], A, dinv, b, x, zero != 0);
-}
-
 // inverted diagonal blocks of a structured-layout matrix; fail |= 2 for a
 // singular block
 __global__ void k_dia_block_inverse(long N, const double* __restrict__ A, double* __restrict__ dinv,
@@ -653,27 +543,6 @@ void ColoredIlu::apply(const int* rp, const S* A, const double* dinv, const S* d
     }
     OM_HIP(hipGetLastError());
 }
-
-template <class S>
-void ColoredIlu::gsSweep(const S* A, const double* dinv, const S* b, S* x, bool forward, bool zero, hipStream_t s,
-                         const int* run)
-{
-    const ColGeom g = colGeomOf(nx_, ny_, nz_);
-    for (int k = 0; k < 2; ++k) {
-        const int c = forward ? k : 1 - k;
-        if (!nTiles_[c]) continue;
-        const int blocks = (nTiles_[c] + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (forward)
-            hipLaunchKernelGGL((k_col_gs<S, true>), dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, g, c,
-                               tiles_[c].p, nTiles_[c], A, dinv, b, x, zero ? 1 : 0, run);
-        else
-            hipLaunchKernelGGL((k_col_gs<S, false>), dim3(blocks), dim3(64 * kWavesPerBlock), 0, s, g, c,
-                               tiles_[c].p, nTiles_[c], A, dinv, b, x, zero ? 1 : 0, run);
-    }
-    OM_HIP(hipGetLastError());
-}
-template void ColoredIlu::gsSweep<double>(const double*, const double*, const double*, double*, bool, bool,
-                                          hipStream_t, const int*);
 
 void launchDiaBlockInverse(long N, const double* A, double* dinv, int* fail, hipStream_t s)
 {

@@ -948,9 +948,11 @@ struct Precond {
  * M = (D + L_A) D^-1 (D + U_A).  Operation order per cell: the neighbours in
  * other tiles in column order, then those of its own tile in column order. */
 struct TileColoring {
-    int nx, ny, nz, ntx, nty;
-    static constexpr int TX = 4, TY = 2, TZ = 8;
-    TileColoring(int x, int y, int z) : nx(x), ny(y), nz(z), ntx((x + TX - 1) / TX), nty((y + TY - 1) / TY) {}
+    int nx, ny, nz;
+    int TX = 4, TY = 2, TZ = 8;       /* tile shape (1 x 1 x 1: red-black cells) */
+    int ntx, nty;
+    TileColoring(int x, int y, int z, int tx = 4, int ty = 2, int tz = 8)
+        : nx(x), ny(y), nz(z), TX(tx), TY(ty), TZ(tz), ntx((x + tx - 1) / tx), nty((y + ty - 1) / ty) {}
     void ijk(long I, int& i, int& j, int& k) const
     {
         i = (int)(I % nx); j = (int)((I / nx) % ny); k = (int)(I / ((long)nx * ny));
@@ -1082,9 +1084,10 @@ void coloredIluApply(const TileColoring& tc, const int32_t* rp, const int32_t* c
  *     Gauss-Seidel sweeps (the first from x = 0); r = b - A x; the coarse rhs
  *     is the sum of r over each aggregate; the coarse solution xc; x += damp *
  *     P xc; `post` backward sweeps;
- *   - Gauss-Seidel in the two-colour tile order of the two-colour ILU:
+ *   - Gauss-Seidel in red-black order (the cells with (i + j + k) even
+ *     first, each colour in natural order; backward: the reverse):
  *     x_I = A_II^-1 (b_I - sum_{K != I} A_IK x_K) with the current x, the
- *     neighbours in other tiles in column order, then those of the own tile;
+ *     neighbours in column order (all of them of the other colour);
  *   - the coarsest level: x = M^-1 b with M^-1 by in-place Gauss-Jordan with
  *     partial pivoting (the first largest |entry| of the column), rows
  *     summed in column order. */
@@ -1175,7 +1178,7 @@ void amgGalerkin(const AmgLevel& f, AmgLevel& c)
 /* one Gauss-Seidel sweep, in place (zero: x = 0 before the sweep) */
 void amgGsSweep(const AmgLevel& L, const double* b, double* x, bool forward, bool zero)
 {
-    const TileColoring tc(L.nx, L.ny, L.nz);
+    const TileColoring tc(L.nx, L.ny, L.nz, 1, 1, 1);
     std::vector<long> ord = tc.order();
     if (!forward) std::reverse(ord.begin(), ord.end());
     std::vector<char> done(L.N, 0);
@@ -1432,7 +1435,7 @@ int bicgstab(int n, int bs, const int32_t* rp, const int32_t* ci, const S* A, co
 
 bool amgParamsValid(const ob_params* p)
 {
-    return p->amg_coarsen_target >= 1 && p->amg_coarsen_target <= 64 && p->amg_pre_smooth >= 1 &&
+    return p->amg_coarsen_target >= 1 && p->amg_coarsen_target <= 32 && p->amg_pre_smooth >= 1 &&
            p->amg_post_smooth >= 0 && p->amg_agg[0] >= 1 && p->amg_agg[1] >= 1 && p->amg_agg[2] >= 1 &&
            std::isfinite(p->amg_prolong_damp);
 }
@@ -1838,7 +1841,7 @@ int ob_amg_apply(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* 
     return 0;
 }
 
-/* one Gauss-Seidel sweep of the structured-layout matrix A7 in place */
+/* one red-black Gauss-Seidel sweep of the structured-layout matrix A7 in place */
 int ob_gs_sweep(int nx, int ny, int nz, const double* A7, const double* b, double* x, int forward, int zero)
 {
     AmgLevel L;

@@ -89,7 +89,7 @@ typedef struct ob_params {
     int32_t verbose;
     int32_t ilu_order;          /* 0: natural (the reference); 1: two-colour 4x2x8 tiles */
     /* precond = 2: aggregation AMG (in place of ParallelAmgBackend) */
-    int32_t amg_coarsen_target; /* coarsest level: at most this many cells (1..64; 32) */
+    int32_t amg_coarsen_target; /* coarsest level: at most this many cells (1..32; 32) */
     int32_t amg_pre_smooth;     /* forward Gauss-Seidel sweeps before the coarse correction (>= 1; 2) */
     int32_t amg_post_smooth;    /* backward sweeps after it (2) */
     double amg_prolong_damp;    /* x += damp * P xc (1.6) */
@@ -176,8 +176,8 @@ int ob_bicgstab_grid(int nx, int ny, int nz, const int32_t* row_ptr, const int32
  * dimensions ([levels][3], returns the level count), a level's matrix in the
  * structured layout [7][N_l][9] (slots -z, -y, -x, I, +x, +y, +z), one
  * V-cycle v = M^-1 d; one block Gauss-Seidel sweep of a structured-layout
- * matrix in the two-colour tile order (forward / backward, zero: x = 0
- * before); the in-place dense inverse of the coarsest level */
+ * matrix in red-black order (forward / backward, zero: x = 0 before); the
+ * in-place dense inverse of the coarsest level */
 int ob_amg_dims(int nx, int ny, int nz, const ob_params* prm, int32_t* dims, int cap);
 int ob_amg_level_matrix(int nx, int ny, int nz, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
                         const ob_params* prm, int level, double* A7);

@@ -1,9 +1,10 @@
 """The oracle's aggregation AMG (the product's precond = 2, in place of the
 reference's ParallelAmgBackend, parallelamgbackend.hh:85-316), pinned by
 dense linear algebra written here independently of the oracle's sweeps:
-- a Gauss-Seidel sweep in the two-colour tile order is the block triangular
-  solve of the permuted matrix, (D + L) x_new = b - U x_old (forward) and
-  (D + U) x_new = b - L x_old (backward), with x_old = 0 for the zero start;
+- a Gauss-Seidel sweep in red-black order (cells with i + j + k even first,
+  natural order within a colour) is the block triangular solve of the
+  permuted matrix, (D + L) x_new = b - U x_old (forward) and (D + U) x_new =
+  b - L x_old (backward), with x_old = 0 for the zero start;
 - the coarse operators are P^T A P for the piecewise-constant P of the box
   aggregates;
 - the coarsest solve inverts its matrix (numpy's inverse, singular input
@@ -18,7 +19,13 @@ import pytest
 
 import oracle as orc
 from cases import case, mixed_state
-from test_oracle_colored import coloured_order
+
+
+def red_black_order(nx, ny, nz):
+    """cells with i + j + k even, then the odd ones, natural order within each"""
+    I = np.arange(nx * ny * nz)
+    par = (I % nx + (I // nx) % ny + I // (nx * ny)) % 2
+    return np.concatenate([I[par == 0], I[par == 1]])
 
 OFF = lambda nx, ny: (-nx * ny, -nx, -1, 0, 1, nx, nx * ny)
 
@@ -74,7 +81,7 @@ def _diag_dominant_dia(dims, seed):
 
 @pytest.mark.parametrize("dims", [(4, 2, 8), (5, 3, 9), (9, 4, 17), (1, 1, 7), (6, 1, 3)])
 @pytest.mark.parametrize("forward,zero", [(True, True), (True, False), (False, False)])
-def test_gs_sweep_is_the_block_triangular_solve_in_tile_order(dims, forward, zero):
+def test_gs_sweep_is_the_block_triangular_solve_in_red_black_order(dims, forward, zero):
     N = int(np.prod(dims))
     A7 = _diag_dominant_dia(dims, seed=N)
     rng = np.random.default_rng(7)
@@ -82,7 +89,7 @@ def test_gs_sweep_is_the_block_triangular_solve_in_tile_order(dims, forward, zer
     x0 = np.zeros(3 * N) if zero else rng.standard_normal(3 * N)
     rc, x = orc.gs_sweep(dims, A7, b, x0, forward=forward, zero=zero)
     assert rc == 0
-    order = coloured_order(*dims)
+    order = red_black_order(*dims)
     if not forward:
         order = order[::-1]
     want = _dense_gs(_dense_from_dia(dims, A7), order, b, x0)
@@ -199,7 +206,7 @@ def test_two_level_cycle_is_smoothing_around_the_coarse_correction(pre, post, da
     assert rc == 0
     M = _dense_from_dia(dims, orc.dia_from_bcrs(dims, pat["row_ptr"], pat["col_idx"], J))
     P, cdims = _prolongator(dims, agg)
-    order = coloured_order(*dims)
+    order = red_black_order(*dims)
     x = np.zeros(3 * N)
     for k in range(pre):
         x = _dense_gs(M, order, d, x)
@@ -216,7 +223,7 @@ def test_hierarchy_dimensions():
     assert [tuple(x) for x in d] == [(100, 100, 50), (34, 34, 17), (12, 12, 6), (4, 4, 2)]
     d = orc.amg_dims((1, 1, 5), prm)
     assert [tuple(x) for x in d] == [(1, 1, 5)]
-    assert orc.amg_dims((10, 10, 10), orc.amg_params(amg_coarsen_target=65)).size == 0   # beyond the dense limit
+    assert orc.amg_dims((10, 10, 10), orc.amg_params(amg_coarsen_target=33)).size == 0   # beyond the dense limit
     assert orc.amg_dims((10, 10, 10), orc.amg_params(amg_pre_smooth=0)).size == 0
 
 
