@@ -113,15 +113,16 @@ def amg_levels(nx, ny, nz, agg=(3, 3, 3), target=32):
 
 
 def amg_bytes(levels, pre=2, post=2, n=3, s=8):
-    """algorithmic HBM bytes of one V-cycle and of one setup.  Per level of N
-    cells and NNZB blocks: a sweep reads the off-diagonal blocks, the inverted
-    diagonal blocks, b and x and writes x (the first, from x = 0, reads only
-    the blocks of neighbours swept before a cell: half of the off-diagonal
-    ones, and no x); the residual reads the blocks, b and x and writes r; the
-    restriction reads r; the prolongation reads and writes x.  The coarsest
-    level reads its dense inverse.  Setup: every level's matrix read once
-    (structured layout, 7 slots), coarse matrices and inverted diagonal
-    blocks written."""
+    """algorithmic HBM bytes of one V-cycle and of one setup (s: bytes of the
+    level matrices' and vectors' scalar; inverted diagonal blocks and the
+    coarsest inverse are FP64).  Per level of N cells and NNZB blocks: a
+    sweep reads the off-diagonal blocks, the inverted diagonal blocks, b and x
+    and writes x (the first, from x = 0, reads only the blocks of neighbours
+    swept before a cell: half of the off-diagonal ones, and no x); the
+    residual reads the blocks, b and x and writes r; the restriction reads r;
+    the prolongation reads and writes x.  The coarsest level reads its dense
+    inverse.  Setup (FP64): every level's matrix read once (structured
+    layout, 7 slots), coarse matrices and inverted diagonal blocks written."""
     bb = n * n * s
     cyc = 0
     setup = 0
@@ -129,20 +130,22 @@ def amg_bytes(levels, pre=2, post=2, n=3, s=8):
         N = nx * ny * nz
         nnzb = N + 2 * ((nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1))
         vec = n * s * N
+        piv = n * n * 8 * N
+        bb64 = n * n * 8
         if l + 1 == len(levels):
             cyc += (n * N) ** 2 * 8 + 2 * vec
-            setup += 7 * bb * N + (n * N) ** 2 * 8
+            setup += 7 * bb64 * N + (n * N) ** 2 * 8
             continue
         Nc = 1
         for q in levels[l + 1]:
             Nc *= q
-        zero_sweep = ((nnzb - N) // 2 + N) * bb + 2 * vec
-        sweep = nnzb * bb + 3 * vec
+        zero_sweep = ((nnzb - N) // 2) * bb + piv + 2 * vec
+        sweep = (nnzb - N) * bb + piv + 3 * vec
         cyc += zero_sweep + (pre - 1 + post) * sweep
         cyc += nnzb * bb + 3 * vec                 # residual
         cyc += vec + n * s * Nc                    # restriction
         cyc += 2 * vec + n * s * Nc                # prolongation
-        setup += 7 * bb * N + bb * N + (7 * bb * Nc)
+        setup += 7 * bb64 * N + piv + 7 * bb64 * Nc
     return cyc, setup
 
 
@@ -284,7 +287,8 @@ def beside_runs(om, np, config, dt, steps=2):
     runs = [("c3_fp64", "c3", 0, 0, 0), ("%s_fp32_inner" % config, config, 1, 0, 0),
             ("%s_fp32_inner_colored" % config, config, 1, 1, 0), ("%s_fp64_colored" % config, config, 0, 1, 0),
             ("c3_fp32_inner_colored", "c3", 1, 1, 0), ("%s_fp64_amg" % config, config, 0, 0, 2),
-            ("c3_fp64_amg", "c3", 0, 0, 2)]
+            ("c3_fp64_amg", "c3", 0, 0, 2), ("%s_fp32_inner_amg" % config, config, 1, 0, 2),
+            ("c3_fp32_inner_amg", "c3", 1, 0, 2)]
     ref = {}
     for key, cfg_name, precision, order, precond in [("c3_fp64", "c3", 0, 0, 0)] + runs + [("ref", config, 0, 0, 0)]:
         if key in out:
@@ -306,7 +310,7 @@ def beside_runs(om, np, config, dt, steps=2):
         ab = algorithmic_bytes(model.N, (model.nnzb - model.N) // 2, model.nnzb, s=8 if precision == 0 else 4,
                                colored=order == 1)
         if precond == 2:
-            ab["ilu_apply"] = amg_bytes(amg_levels(cfg["nx"], cfg["ny"], cfg["nz"]))[0]
+            ab["ilu_apply"] = amg_bytes(amg_levels(cfg["nx"], cfg["ny"], cfg["nz"]), s=8 if precision == 0 else 4)[0]
         model.close()
         if key == "ref" or (precision == 0 and order == 0 and precond == 0):
             ref[cfg_name] = pv
@@ -405,7 +409,8 @@ def main():
     ab = algorithmic_bytes(N, F, nnzb, s=8 if args.precision == 0 else 4, colored=args.ilu_order == 1)
     if args.precond == 2:
         # the preconditioner classes hold the V-cycle and the hierarchy's setup
-        ab["ilu_apply"], ab["ilu_factor"] = amg_bytes(amg_levels(cfg["nx"], cfg["ny"], cfg["nz"]))
+        ab["ilu_apply"], ab["ilu_factor"] = amg_bytes(amg_levels(cfg["nx"], cfg["ny"], cfg["nz"]),
+                                                      s=8 if args.precision == 0 else 4)
     kernels = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0 or k not in ab:

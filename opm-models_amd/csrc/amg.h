@@ -27,7 +27,9 @@ std::vector<std::array<int, 3>> amgLevelDims(int nx, int ny, int nz, const AmgPa
 
 class AmgHierarchy {
 public:
-    AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, hipStream_t s);
+    // fp32: also keep FP32 copies of the level matrices for applications in
+    // FP32 (the inner solve of the precision form; pivots stay FP64)
+    AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, bool fp32, hipStream_t s);
     AmgHierarchy(const AmgHierarchy&) = delete;
     AmgHierarchy& operator=(const AmgHierarchy&) = delete;
     // the levels from the fine matrix A0 (structured layout [7][N][9], FP64):
@@ -35,8 +37,10 @@ public:
     // blocks, the coarsest level's dense inverse; fail |= 2 for a singular
     // diagonal block or coarsest matrix
     void setup(const double* A0, int* fail, hipStream_t s);
-    // v = one V-cycle applied to d (device vectors of 3N); skipped while *run == 0
-    void apply(const double* d, double* v, const int* run, hipStream_t s);
+    // v = one V-cycle applied to d (device vectors of 3N, FP64, or FP32 with
+    // the FP32 copies); skipped while *run == 0
+    template <class S>
+    void apply(const S* d, S* v, const int* run, hipStream_t s);
     int levels() const { return (int)lv_.size(); }
     std::array<int, 3> dims(int l) const { return {lv_[l]->nx, lv_[l]->ny, lv_[l]->nz}; }
     // level l's matrix in the natural structured layout [7][N_l][9] (host)
@@ -49,13 +53,31 @@ private:
         int ax = 1, ay = 1, az = 1;             // aggregation to the next level
         DevBuf<double> A;                       // red-black layout: even cells' rows, then odd ones'
         DevBuf<double> dinv, b, x, r;
+        DevBuf<float> Af, bf, xf, rf;           // FP32 copies (fp32)
         DevBuf<int> rbOf, rbInv;                // natural cell -> red-black row, and back
         std::vector<int> hostRbOf;
+        template <class S> const S* mat() const
+        {
+            if constexpr (sizeof(S) == 8) return A.p; else return Af.p;
+        }
+        template <class S> S* res()
+        {
+            if constexpr (sizeof(S) == 8) return r.p; else return rf.p;
+        }
+        template <class S> S* rhs()
+        {
+            if constexpr (sizeof(S) == 8) return b.p; else return bf.p;
+        }
+        template <class S> S* sol()
+        {
+            if constexpr (sizeof(S) == 8) return x.p; else return xf.p;
+        }
     };
-    void cycle(size_t l, const double* b, double* x, const int* run, hipStream_t s);
+    template <class S> void cycle(size_t l, const S* b, S* x, const int* run, hipStream_t s);
     std::vector<std::unique_ptr<Level>> lv_;
     DevBuf<double> dense_;
     AmgParams p_;
+    bool fp32_ = false;
     bool ready_ = false;
 };
 

@@ -64,8 +64,9 @@ __device__ __forceinline__ long slotNeighbour(const AmgGrid& g, long I, int i, i
 
 // r = b - A x; one lane per row (red-black row L = t / 3 of cell I, row
 // e = t % 3), slots in order
-__global__ void k_amg_residual(AmgGrid g, const int* __restrict__ rbInv, const double* __restrict__ A,
-                               const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
+template <class S>
+__global__ void k_amg_residual(AmgGrid g, const int* __restrict__ rbInv, const S* __restrict__ A,
+                               const S* __restrict__ b, const S* __restrict__ x, S* __restrict__ r,
                                const int* __restrict__ run)
 {
     if (run && !*run) return;
@@ -75,16 +76,16 @@ __global__ void k_amg_residual(AmgGrid g, const int* __restrict__ rbInv, const d
     const int e = (int)(t - 3 * L);
     const long I = rbInv[L];
     const int i = (int)(I % g.nx), j = (int)((I / g.nx) % g.ny), k = (int)(I / ((long)g.nx * g.ny));
-    double acc = b[3 * I + e];
+    S acc = b[3 * I + e];
 #pragma unroll
     for (int s = 0; s < 7; ++s) {
         const long J = slotNeighbour(g, I, i, j, k, s);
         if (J < 0) continue;
-        const double* a = A + 9 * (s * g.N + L) + 3 * e;
-        double v = a[0] * x[3 * J];
+        const S* a = A + 9 * (s * g.N + L) + 3 * e;
+        S v = a[0] * x[3 * J];
         v += a[1] * x[3 * J + 1];
         v += a[2] * x[3 * J + 2];
-        acc += -1.0 * v;
+        acc += S(-1) * v;
     }
     r[3 * I + e] = acc;
 }
@@ -92,10 +93,12 @@ __global__ void k_amg_residual(AmgGrid g, const int* __restrict__ rbInv, const d
 // One colour of a block Gauss-Seidel sweep, in place: the rows [base,
 // base + count) of the red-black layout, x_I = A_II^-1 (b_I - sum_{K != I}
 // A_IK x_K), the neighbours in column order; skip: the neighbours' values
-// are zero (the first colour of a sweep from x = 0) and are not read.
+// are zero (the first colour of a sweep from x = 0) and are not read.  The
+// inverted diagonal blocks are FP64 in both precisions.
+template <class S>
 __global__ void k_amg_rb_gs(AmgGrid g, long base, long count, const int* __restrict__ rbInv,
-                            const double* __restrict__ A, const double* __restrict__ dinv,
-                            const double* __restrict__ b, double* __restrict__ x, int skip,
+                            const S* __restrict__ A, const double* __restrict__ dinv,
+                            const S* __restrict__ b, S* __restrict__ x, int skip,
                             const int* __restrict__ run)
 {
     if (run && !*run) return;
@@ -104,7 +107,7 @@ __global__ void k_amg_rb_gs(AmgGrid g, long base, long count, const int* __restr
     const long L = base + t;
     const long I = rbInv[L];
     const int i = (int)(I % g.nx), j = (int)((I / g.nx) % g.ny), k = (int)(I / ((long)g.nx * g.ny));
-    double acc[3] = {b[3 * I], b[3 * I + 1], b[3 * I + 2]};
+    S acc[3] = {b[3 * I], b[3 * I + 1], b[3 * I + 2]};
     if (!skip) {
         constexpr int kOrder[6] = {0, 1, 2, 4, 5, 6};
 #pragma unroll
@@ -112,24 +115,24 @@ __global__ void k_amg_rb_gs(AmgGrid g, long base, long count, const int* __restr
             const int s = kOrder[o];
             const long K = slotNeighbour(g, I, i, j, k, s);
             if (K < 0) continue;
-            const double* a = A + 9 * (s * g.N + L);
-            const double x0 = x[3 * K], x1 = x[3 * K + 1], x2 = x[3 * K + 2];
+            const S* a = A + 9 * (s * g.N + L);
+            const S x0 = x[3 * K], x1 = x[3 * K + 1], x2 = x[3 * K + 2];
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
-                double v = a[3 * e] * x0;
+                S v = a[3 * e] * x0;
                 v += a[3 * e + 1] * x1;
                 v += a[3 * e + 2] * x2;
-                acc[e] += -1.0 * v;
+                acc[e] += S(-1) * v;
             }
         }
     }
     const double* Di = dinv + 9 * L;
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        double o = Di[3 * e] * acc[0];
-        o += Di[3 * e + 1] * acc[1];
-        o += Di[3 * e + 2] * acc[2];
-        x[3 * I + e] = 0.0 + o;
+        double o = Di[3 * e] * (double)acc[0];
+        o += Di[3 * e + 1] * (double)acc[1];
+        o += Di[3 * e + 2] * (double)acc[2];
+        x[3 * I + e] = (S)(0.0 + o);
     }
 }
 
@@ -147,8 +150,9 @@ __global__ void k_amg_to_rb(AmgGrid g, const int* __restrict__ rbInv, const doub
 }
 
 // coarse right-hand side: the sum of r over each aggregate, natural order
-__global__ void k_amg_restrict(AmgGrid f, AmgGrid c, int ax, int ay, int az, const double* __restrict__ r,
-                               double* __restrict__ bc, const int* __restrict__ run)
+template <class S>
+__global__ void k_amg_restrict(AmgGrid f, AmgGrid c, int ax, int ay, int az, const S* __restrict__ r,
+                               S* __restrict__ bc, const int* __restrict__ run)
 {
     if (run && !*run) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,7 +160,7 @@ __global__ void k_amg_restrict(AmgGrid f, AmgGrid c, int ax, int ay, int az, con
     const long a = t / 3;
     const int e = (int)(t - 3 * a);
     const int ci = (int)(a % c.nx), cj = (int)((a / c.nx) % c.ny), ck = (int)(a / ((long)c.nx * c.ny));
-    double acc = 0.0;
+    S acc = S(0);
     for (int k = ck * az; k < min(f.nz, ck * az + az); ++k)
         for (int j = cj * ay; j < min(f.ny, cj * ay + ay); ++j)
             for (int i = ci * ax; i < min(f.nx, ci * ax + ax); ++i)
@@ -165,8 +169,9 @@ __global__ void k_amg_restrict(AmgGrid f, AmgGrid c, int ax, int ay, int az, con
 }
 
 // x += damp * P xc
+template <class S>
 __global__ void k_amg_prolong(AmgGrid f, AmgGrid c, int ax, int ay, int az, double damp,
-                              const double* __restrict__ xc, double* __restrict__ x, const int* __restrict__ run)
+                              const S* __restrict__ xc, S* __restrict__ x, const int* __restrict__ run)
 {
     if (run && !*run) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,7 +180,7 @@ __global__ void k_amg_prolong(AmgGrid f, AmgGrid c, int ax, int ay, int az, doub
     const int e = (int)(t - 3 * I);
     const int i = (int)(I % f.nx), j = (int)((I / f.nx) % f.ny), k = (int)(I / ((long)f.nx * f.ny));
     const long a = i / ax + (long)c.nx * (j / ay + (long)c.ny * (k / az));
-    x[t] = x[t] + damp * xc[3 * a + e];
+    x[t] = x[t] + (S)damp * xc[3 * a + e];
 }
 
 // coarse operator: one lane per (coarse slot t, coarse cell, block entry).
@@ -351,21 +356,29 @@ __global__ __launch_bounds__(kGjThreads) void k_amg_dense_invert(int n, double* 
 
 // x = M^-1 b on the coarsest level (the inverse stored transposed): the
 // inverse and b staged in LDS, then one lane per row, columns in order
+template <class S>
 __global__ __launch_bounds__(kGjThreads) void k_amg_dense_apply(int n, const double* __restrict__ MinvT,
-                                                                const double* __restrict__ b,
-                                                                double* __restrict__ x, const int* __restrict__ run)
+                                                                const S* __restrict__ b, S* __restrict__ x,
+                                                                const int* __restrict__ run)
 {
     if (run && !*run) return;
-    __shared__ double S[kDenseMax * kDenseMax];
+    __shared__ double M[kDenseMax * kDenseMax];
     __shared__ double bs[kDenseMax];
     const int tid = (int)threadIdx.x;
-    for (int q = tid; q < n * n; q += kGjThreads) S[q] = MinvT[q];
-    if (tid < n) bs[tid] = b[tid];
+    for (int q = tid; q < n * n; q += kGjThreads) M[q] = MinvT[q];
+    if (tid < n) bs[tid] = (double)b[tid];
     __syncthreads();
     if (tid >= n) return;
-    double s = S[tid] * bs[0];
-    for (int j = 1; j < n; ++j) s += S[j * n + tid] * bs[j];
-    x[tid] = s;
+    double s = M[tid] * bs[0];
+    for (int j = 1; j < n; ++j) s += M[j * n + tid] * bs[j];
+    x[tid] = (S)s;
+}
+
+// FP32 copy of a level matrix
+__global__ void k_amg_to_float(long m, const double* __restrict__ a, float* __restrict__ f)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) f[t] = (float)a[t];
 }
 
 unsigned blocksFor(long threads, int per) { return (unsigned)((threads + per - 1) / per); }
@@ -395,7 +408,8 @@ std::vector<std::array<int, 3>> amgLevelDims(int nx, int ny, int nz, const AmgPa
     return d;
 }
 
-AmgHierarchy::AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, hipStream_t s) : p_(p)
+AmgHierarchy::AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, bool fp32, hipStream_t s)
+    : p_(p), fp32_(fp32)
 {
     checkAmgParams(p);
     const auto dims = amgLevelDims(nx, ny, nz, p);
@@ -435,6 +449,11 @@ AmgHierarchy::AmgHierarchy(int nx, int ny, int nz, const AmgParams& p, hipStream
             L.b.alloc(3 * L.N);
             L.x.alloc(3 * L.N);
         }
+        if (fp32) {
+            L.Af.alloc(63 * L.N);
+            if (!last) L.rf.alloc(3 * L.N);
+            if (l > 0) { L.bf.alloc(3 * L.N); L.xf.alloc(3 * L.N); }
+        }
     }
     const long n = 3 * lv_.back()->N;
     dense_.alloc(n * n);
@@ -466,50 +485,63 @@ void AmgHierarchy::setup(const double* A0, int* fail, hipStream_t s)
     hipLaunchKernelGGL(k_amg_dense_build, dim3(blocksFor(n * n, 256)), dim3(256), 0, s, c, C.rbOf.p, C.A.p,
                        dense_.p);
     hipLaunchKernelGGL(k_amg_dense_invert, dim3(1), dim3(kGjThreads), 0, s, (int)n, dense_.p, fail);
+    if (fp32_)
+        for (auto& L : lv_)
+            hipLaunchKernelGGL(k_amg_to_float, dim3(blocksFor(63 * L->N, 256)), dim3(256), 0, s, 63 * L->N, L->A.p,
+                               L->Af.p);
     OM_HIP(hipGetLastError());
     ready_ = true;
 }
 
-void AmgHierarchy::cycle(size_t l, const double* b, double* x, const int* run, hipStream_t s)
+template <class S>
+void AmgHierarchy::cycle(size_t l, const S* b, S* x, const int* run, hipStream_t s)
 {
     Level& L = *lv_[l];
     if (l + 1 == lv_.size()) {
         const int n = (int)(3 * L.N);
-        hipLaunchKernelGGL(k_amg_dense_apply, dim3(1), dim3(kGjThreads), 0, s, n, dense_.p, b, x, run);
+        hipLaunchKernelGGL((k_amg_dense_apply<S>), dim3(1), dim3(kGjThreads), 0, s, n, dense_.p, b, x, run);
         return;
     }
     Level& C = *lv_[l + 1];
     const AmgGrid f = gridOf(L.nx, L.ny, L.nz), c = gridOf(C.nx, C.ny, C.nz);
+    const S* A = L.mat<S>();
+    S* r = L.res<S>();
+    S* cb = C.rhs<S>();
+    S* cx = C.sol<S>();
     // one colour: rows [base, base + count) of the red-black layout
     auto colour = [&](int col, bool skip) {
         const long base = col == 0 ? 0 : L.N0, count = col == 0 ? L.N0 : L.N - L.N0;
         if (count == 0) return;
-        hipLaunchKernelGGL(k_amg_rb_gs, dim3(blocksFor(count, 256)), dim3(256), 0, s, f, base, count, L.rbInv.p,
-                           L.A.p, L.dinv.p, b, x, skip ? 1 : 0, run);
+        hipLaunchKernelGGL((k_amg_rb_gs<S>), dim3(blocksFor(count, 256)), dim3(256), 0, s, f, base, count,
+                           L.rbInv.p, A, L.dinv.p, b, x, skip ? 1 : 0, run);
     };
     for (int k = 0; k < p_.pre; ++k) {
         colour(0, k == 0);          // from x = 0 the even cells' neighbours are still zero
         colour(1, false);
     }
-    hipLaunchKernelGGL(k_amg_residual, dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, L.rbInv.p, L.A.p, b, x,
-                       L.r.p, run);
-    hipLaunchKernelGGL(k_amg_restrict, dim3(blocksFor(3 * c.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az,
-                       L.r.p, C.b.p, run);
-    cycle(l + 1, C.b.p, C.x.p, run, s);
-    hipLaunchKernelGGL(k_amg_prolong, dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az, p_.damp,
-                       C.x.p, x, run);
+    hipLaunchKernelGGL((k_amg_residual<S>), dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, L.rbInv.p, A, b, x,
+                       r, run);
+    hipLaunchKernelGGL((k_amg_restrict<S>), dim3(blocksFor(3 * c.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az,
+                       (const S*)r, cb, run);
+    cycle<S>(l + 1, cb, cx, run, s);
+    hipLaunchKernelGGL((k_amg_prolong<S>), dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az,
+                       p_.damp, (const S*)cx, x, run);
     for (int k = 0; k < p_.post; ++k) {
         colour(1, false);
         colour(0, false);
     }
 }
 
-void AmgHierarchy::apply(const double* d, double* v, const int* run, hipStream_t s)
+template <class S>
+void AmgHierarchy::apply(const S* d, S* v, const int* run, hipStream_t s)
 {
     if (!ready_) throw std::logic_error("AMG: apply before setup");
-    cycle(0, d, v, run, s);
+    if (sizeof(S) == 4 && !fp32_) throw std::logic_error("AMG: no FP32 copy of the hierarchy");
+    cycle<S>(0, d, v, run, s);
     OM_HIP(hipGetLastError());
 }
+template void AmgHierarchy::apply<double>(const double*, double*, const int*, hipStream_t);
+template void AmgHierarchy::apply<float>(const float*, float*, const int*, hipStream_t);
 
 void AmgHierarchy::downloadLevel(int l, double* host, hipStream_t s) const
 {

@@ -419,14 +419,14 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
     }
     if (prm_.precond == 2) {
         if (!structured()) throw std::invalid_argument("the AMG is built for 3x3 systems on a 7-point grid");
-        if (prm_.precision != 0) throw std::invalid_argument("the AMG runs the FP64 solve (precision 0)");
+        if (prm_.precision == 2) throw std::invalid_argument("the AMG runs the FP64 solve or the FP32 inner solve");
         AmgParams ap;
         ap.coarsenTarget = prm_.amg_coarsen_target;
         ap.pre = prm_.amg_pre_smooth;
         ap.post = prm_.amg_post_smooth;
         ap.damp = prm_.amg_prolong_damp;
         for (int q = 0; q < 3; ++q) ap.agg[q] = prm_.amg_agg[q];
-        amg_ = std::make_unique<AmgHierarchy>(m.gx, m.gy, m.gz, ap, m.stream);
+        amg_ = std::make_unique<AmgHierarchy>(m.gx, m.gy, m.gz, ap, prm_.precision == 1, m.stream);
     }
     if (prm_.precision == 0) {
         for (auto& w : wd_) w.alloc(n);
@@ -570,7 +570,13 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     if (structured() && prm_.precision != 2) {
         const long nd = 7L * bb * N;
         if (prm_.precision == 1) {
-            dia64_.release();       // an FP64 copy made for a test application would be stale
+            if (amg_) {
+                // the AMG hierarchy is formed in FP64
+                dia64_.alloc(nd);
+                launchToDia<double>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, dia64_.p, s);
+            } else {
+                dia64_.release();   // an FP64 copy made for a test application would be stale
+            }
             diaf_.alloc(nd);
             launchToDia<float>(m_.gx, m_.gy, m_.gz, m_.rowPtr.p, A64_, diaf_.p, s);
         } else {
@@ -644,13 +650,9 @@ bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v,
     const int N = (int)m_.N;
     bool fused = false;
     if (amg_) {
-        if constexpr (sizeof(S) == 8) {
-            m_.timer.begin(kClsIluApply, s);
-            amg_->apply(d, v, run, s);
-            m_.timer.end(s);
-        } else {
-            throw std::logic_error("the AMG runs the FP64 solve");
-        }
+        m_.timer.begin(kClsIluApply, s);
+        amg_->apply<S>(d, v, run, s);
+        m_.timer.end(s);
     } else if (colored_) {
         if (!diaOf<S>()) throw std::logic_error("two-colour ILU: no structured matrix of this precision");
         m_.timer.begin(kClsIluApply, s);

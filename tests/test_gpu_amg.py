@@ -1,10 +1,13 @@
 """GPU parity of the aggregation AMG (precond = 2, in place of the
 reference's ParallelAmgBackend) with the oracle's restatement of it, bit for
 bit: the hierarchy's level matrices (Galerkin products), one V-cycle (block
-Gauss-Seidel sweeps in the two-colour tile order, restriction, damped
+Gauss-Seidel sweeps in red-black order, restriction, damped
 prolongation, the coarsest level's dense inverse), BiCGStab with it, and
 whole Newton runs, eager and replayed from graphs.  At 500k cells one
-V-cycle is compared with the oracle and the solve is checked by its residual."""
+V-cycle is compared with the oracle and the solve is checked by its residual.
+The FP32-inner form with the AMG (FP32 level matrices and sweeps, FP64
+hierarchy and pivots, FP64 outer refinement) is held to the precision form's
+bar: pressures within 1e-6 relative of the FP64 oracle per step."""
 import numpy as np
 import pytest
 
@@ -129,4 +132,44 @@ def test_cycle_and_solve_at_500k():
     res = r - orc.spmv(c.N, 3, pat["row_ptr"], pat["col_idx"], J, x)
     print("500k: BiCGStab iterations with the AMG", its, "residual reduction",
           np.max(np.abs(res)) / np.max(np.abs(r)), flush=True)
+    assert np.max(np.abs(res)) <= 1e-3 * np.max(np.abs(r))
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_fp32_inner_amg_run_matches_the_fp64_oracle(graph):
+    """20 one-day steps of the 40x40x10 black-oil workload in the FP32-inner
+    form with the AMG: per-step pressures within 1e-6 relative of the FP64
+    natural-order oracle, saturations within 1e-6."""
+    c = case(40, 40, 10, seed=1234)
+    kw = dict(newton_tol=1e-7, max_dt=86400.0, min_dt=86400.0)
+    m = om.Model(c)
+    m.set_solution(c.pv0, c.mean0)
+    rc, hist, dts, log, st = m.simulate(om.default_params(precision=1, precond=2, graph=graph, **kw), 20, 86400.0)
+    rco, pvo, mo, histo, dtso, logo, _ = orc.simulate(orc.problem_from_case(c), c.pv0, c.mean0, 20, 86400.0,
+                                                      orc.params(**kw), nthreads=4)
+    assert rc == 0 and rco == 0
+    p, po = hist[:, 1::3], histo[:, 1::3]
+    err = np.max(np.abs(p - po), axis=1) / np.max(np.abs(po), axis=1)
+    print("FP32-inner AMG: per-step pressure error", err.max(), "linear iterations", st.linear_iters, flush=True)
+    assert err.max() <= 1e-6, err
+    assert np.max(np.abs(hist[:, 0::3] - histo[:, 0::3])) <= 1e-6
+
+
+def test_fp32_inner_amg_at_500k_solves_the_system():
+    """The 100x100x50 system with the FP32-inner solve and the AMG: the true
+    FP64 residual of the refined solution within the linear tolerance."""
+    dims = (100, 100, 50)
+    c = om.BlackOilCase(om.make_spec(*dims, seed=2024))
+    prob = orc.problem_from_case(c)
+    st1 = orc.storage(prob, c.pv0, c.mean0)
+    J, r = orc.linearize(prob, c.pv0, c.mean0, st1, 86400.0, nthreads=16)
+    pat = orc.pattern(prob)
+    m = om.Model(c)
+    s = om.Solver(m, om.default_params(precond=2, precision=1))
+    s.prepare(J, r)
+    rc, x, its = s.solve()
+    assert rc == 0
+    res = r - orc.spmv(c.N, 3, pat["row_ptr"], pat["col_idx"], J, x)
+    print("500k FP32-inner AMG: iterations", its, "residual reduction", np.max(np.abs(res)) / np.max(np.abs(r)),
+          flush=True)
     assert np.max(np.abs(res)) <= 1e-3 * np.max(np.abs(r))
