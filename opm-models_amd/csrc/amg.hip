@@ -14,7 +14,9 @@
 //   - a multiplicative V-cycle: `pre` forward block Gauss-Seidel sweeps from
 //     x = 0, the residual summed over each aggregate as the coarse right-hand
 //     side, the coarse solution prolongated with damping, `post` backward
-//     sweeps;
+//     sweeps.  The residual is formed and restricted on the even cells only:
+//     the odd ones, swept last, solve their rows with their neighbours' final
+//     values, so theirs is zero but for rounding;
 //   - Gauss-Seidel in red-black order: the cells with i + j + k even, then
 //     the odd ones (backward: odd first); a cell's six neighbours have the
 //     other colour, so a colour is one launch with one independent lane per
@@ -62,16 +64,17 @@ __device__ __forceinline__ long slotNeighbour(const AmgGrid& g, long I, int i, i
     }
 }
 
-// r = b - A x; one lane per row (red-black row L = t / 3 of cell I, row
-// e = t % 3), slots in order
+// r = b - A x on the rows [0, N0) of the red-black layout (the even cells);
+// one lane per row (red-black row L = t / 3 of cell I, row e = t % 3), slots
+// in order
 template <class S>
-__global__ void k_amg_residual(AmgGrid g, const int* __restrict__ rbInv, const S* __restrict__ A,
+__global__ void k_amg_residual(AmgGrid g, long nRows, const int* __restrict__ rbInv, const S* __restrict__ A,
                                const S* __restrict__ b, const S* __restrict__ x, S* __restrict__ r,
                                const int* __restrict__ run)
 {
     if (run && !*run) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * g.N) return;
+    if (t >= 3 * nRows) return;
     const long L = t / 3;
     const int e = (int)(t - 3 * L);
     const long I = rbInv[L];
@@ -164,7 +167,7 @@ __global__ void k_amg_restrict(AmgGrid f, AmgGrid c, int ax, int ay, int az, con
     for (int k = ck * az; k < min(f.nz, ck * az + az); ++k)
         for (int j = cj * ay; j < min(f.ny, cj * ay + ay); ++j)
             for (int i = ci * ax; i < min(f.nx, ci * ax + ax); ++i)
-                acc += r[3 * (i + (long)f.nx * (j + (long)f.ny * k)) + e];
+                if (((i + j + k) & 1) == 0) acc += r[3 * (i + (long)f.nx * (j + (long)f.ny * k)) + e];
     bc[t] = acc;
 }
 
@@ -519,8 +522,8 @@ void AmgHierarchy::cycle(size_t l, const S* b, S* x, const int* run, hipStream_t
         colour(0, k == 0);          // from x = 0 the even cells' neighbours are still zero
         colour(1, false);
     }
-    hipLaunchKernelGGL((k_amg_residual<S>), dim3(blocksFor(3 * f.N, 256)), dim3(256), 0, s, f, L.rbInv.p, A, b, x,
-                       r, run);
+    hipLaunchKernelGGL((k_amg_residual<S>), dim3(blocksFor(3 * L.N0, 256)), dim3(256), 0, s, f, L.N0, L.rbInv.p, A,
+                       b, x, r, run);
     hipLaunchKernelGGL((k_amg_restrict<S>), dim3(blocksFor(3 * c.N, 256)), dim3(256), 0, s, f, c, L.ax, L.ay, L.az,
                        (const S*)r, cb, run);
     cycle<S>(l + 1, cb, cx, run, s);

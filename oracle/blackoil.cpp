@@ -1081,9 +1081,10 @@ void coloredIluApply(const TileColoring& tc, const int32_t* rp, const int32_t* c
  *     7-point pattern), summed over the fine cells of the aggregate in natural
  *     order and, per cell, over its slots in order -z, -y, -x, I, +x, +y, +z;
  *   - the V-cycle on level l for rhs b: x = 0; `pre` forward block
- *     Gauss-Seidel sweeps (the first from x = 0); r = b - A x; the coarse rhs
- *     is the sum of r over each aggregate; the coarse solution xc; x += damp *
- *     P xc; `post` backward sweeps;
+ *     Gauss-Seidel sweeps (the first from x = 0); r = b - A x on the even
+ *     cells (the odd ones, swept last, have r = 0 but for rounding); the
+ *     coarse rhs is the sum of r over each aggregate's even cells; the
+ *     coarse solution xc; x += damp * P xc; `post` backward sweeps;
  *   - Gauss-Seidel in red-black order (the cells with (i + j + k) even
  *     first, each colour in natural order; backward: the reverse):
  *     x_I = A_II^-1 (b_I - sum_{K != I} A_IK x_K) with the current x, the
@@ -1201,9 +1202,18 @@ void amgGsSweep(const AmgLevel& L, const double* b, double* x, bool forward, boo
 }
 
 /* r = b - A x, per row: the slots in order */
+/* r = b - A x of the even cells (i + j + k even); the odd cells, swept last,
+ * solve their rows with their neighbours' final values, so their residual
+ * is zero but for rounding and is neither formed nor restricted */
+bool evenCell(const AmgLevel& L, long I)
+{
+    return ((I % L.nx + (I / L.nx) % L.ny + I / ((long)L.nx * L.ny)) & 1) == 0;
+}
+
 void amgResidual(const AmgLevel& L, const double* b, const double* x, double* r)
 {
     for (long I = 0; I < L.N; ++I) {
+        if (!evenCell(L, I)) continue;
         double acc[3] = {b[3 * I], b[3 * I + 1], b[3 * I + 2]};
         for (int s = 0; s < 7; ++s) {
             const long J = slotNeighbour(L.nx, L.ny, L.nz, I, s);
@@ -1213,7 +1223,7 @@ void amgResidual(const AmgLevel& L, const double* b, const double* x, double* r)
     }
 }
 
-/* coarse rhs: the sum of r over each aggregate, its cells in natural order */
+/* coarse rhs: the sum of r over each aggregate's even cells, in natural order */
 void amgRestrict(const AmgLevel& f, const AmgLevel& c, const double* r, double* bc)
 {
     for (long a = 0; a < c.N; ++a) {
@@ -1223,7 +1233,7 @@ void amgRestrict(const AmgLevel& f, const AmgLevel& c, const double* r, double* 
             for (int k = ck * f.az; k < std::min(f.nz, ck * f.az + f.az); ++k)
                 for (int j = cj * f.ay; j < std::min(f.ny, cj * f.ay + f.ay); ++j)
                     for (int i = ci * f.ax; i < std::min(f.nx, ci * f.ax + f.ax); ++i)
-                        acc += r[3 * (i + (long)f.nx * (j + (long)f.ny * k)) + e];
+                        if (((i + j + k) & 1) == 0) acc += r[3 * (i + (long)f.nx * (j + (long)f.ny * k)) + e];
             bc[3 * a + e] = acc;
         }
     }
