@@ -219,6 +219,16 @@ int omodels_linearizer_update_discretization(omodels_model* m, const double* tra
  * linearizations, FLOWS, output fields and simulate calls include them. */
 int omodels_model_set_rock(omodels_model* m, const double* compressibility, const double* ref_pressure,
                            int32_t n_table, const double* table);
+/* Directional mobilities (BlackOilIntensiveQuantities::mobility(phaseIdx,
+ * facedir), blackoilintensivequantities.hh:530-550, used by the face flux at
+ * blackoillocalresidualtpfa.hh:341, :375-381).  The reference leaves the
+ * directional relative permeabilities to the problem (updateRelperms,
+ * multiphasebaseproblem.hh:257-261); here they are kr_dir = m_dir kr with the
+ * caller's multipliers m [3][3][N] (axis x, y, z; phase water, oil, gas; cell),
+ * finite and >= 0: the upwind cell's mobility along an axis is its mobility
+ * times its multiplier.  NULL switches them off.  The assembly, FLOWS/FLORES
+ * and the Newton runs of the model use them; the ensemble does not. */
+int omodels_model_set_directional_mobility(omodels_model* m, const double* multipliers);
 /* linearizeDomain() (tpfalinearizer.hh:189-245, linearize_ :711-866) at the
  * current solution(0); iteration_index 0 refreshes the storage cache of
  * time level 1 as recycleFirstIterationStorage() does (:786-812). */

@@ -82,11 +82,11 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
                                           double area, double dZg, bool upInTie, double r[3],
                                           double D[9], double* __restrict__ Jout, const BoDiff& dif,
                                           DiffFace df, const BoDisp& dsp, double& nv, double thp,
-                                          const FaceTM* tm)
+                                          const FaceTM* tm, const FaceDM* dm)
 {
     E3 fl[3];
     double darcy[3];
-    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp, thp, tm);
+    faceFluxT<Disp, Diff, Disp>(f, qi, qj, trans, area, dZg, upInTie, fl, darcy, &dif, df, &dsp, thp, tm, dm);
     if constexpr (Disp) maxVelocity(nv, darcy, area);
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -202,43 +202,60 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
                     return nullptr;
                 }
             };
+            // the directional mobility multipliers of the face's axis (given
+            // geometry only)
+            FaceDM dmv;
+            auto dmOf = [&](long J, int ax) -> const FaceDM* {
+                if constexpr (Geo) {
+                    if (!geo.dirMob) return nullptr;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        dmv.in[a] = geo.dirMob[(3 * ax + a) * N + I];
+                        dmv.ex[a] = geo.dirMob[(3 * ax + a) * N + J];
+                    }
+                    return &dmv;
+                } else {
+                    return nullptr;
+                }
+            };
             // intersection order: -x, +x, -y, +y, -z, +z
             if (i > 0) {
                 IqV q; fromLds(plane[ty + 1][tx], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[I - 1], G.area[0], (zI - zOf(I - 1, 0)) * G.g, tieOf(I - 1), r, D,
                                 jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - 1, q, I - 1), dsp,
-                                nvo, thpOf(I - 1), tmOf(q));
+                                nvo, thpOf(I - 1), tmOf(q), dmOf(I - 1, 0));
             }
             if (i + 1 < G.nx) {
                 IqV q; fromLds(plane[ty + 1][tx + 2], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[I], G.area[0], (zI - zOf(I + 1, 0)) * G.g, tieOf(I + 1), r, D,
                                 jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + 1, q, I), dsp,
-                                nvo, thpOf(I), tmOf(q));
+                                nvo, thpOf(I), tmOf(q), dmOf(I + 1, 0));
             }
             if (j > 0) {
                 IqV q; fromLds(plane[ty][tx + 1], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zOf(I - G.nx, 0)) * G.g,
                                 tieOf(I - G.nx), r, D, jac + 9L * nbOff[2 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - G.nx, q, N + I - G.nx), dsp, nvo,
-                                thpOf(N + I - G.nx), tmOf(q));
+                                thpOf(N + I - G.nx), tmOf(q), dmOf(I - G.nx, 1));
             }
             if (j + 1 < G.ny) {
                 IqV q; fromLds(plane[ty + 2][tx + 1], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[N + I], G.area[1], (zI - zOf(I + G.nx, 0)) * G.g, tieOf(I + G.nx),
                                 r, D, jac + 9L * nbOff[3 * N + I], dif,
-                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo, thpOf(N + I), tmOf(q));
+                                diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo, thpOf(N + I), tmOf(q),
+                                dmOf(I + G.nx, 1));
             }
             if (haveBelow) {
                 applyFace<Diff, Disp>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zOf(I - nxy, -1)) * G.g,
                                 tieOf(I - nxy), r, D, jac + 9L * nbOff[4 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - nxy, below, 2 * N + I - nxy), dsp, nvo,
-                                thpOf(2 * N + I - nxy), tmOf(below));
+                                thpOf(2 * N + I - nxy), tmOf(below), dmOf(I - nxy, 2));
             }
             if (k + 1 < G.nz) {
                 applyFace<Diff, Disp>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zOf(I + nxy, 1)) * G.g,
                                 tieOf(I + nxy), r, D, jac + 9L * nbOff[5 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + nxy, above, 2 * N + I), dsp, nvo,
-                                thpOf(2 * N + I), tmOf(above));
+                                thpOf(2 * N + I), tmOf(above), dmOf(I + nxy, 2));
             }
             // accumulation term (tpfalinearizer.hh:776-825)
             {
@@ -418,6 +435,8 @@ __global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const doubl
         double thp = 0.0;
         FaceTM tmv{ec(1.0), 1.0};
         const FaceTM* tm = nullptr;
+        FaceDM dmv;
+        const FaceDM* dm = nullptr;
         if (geo.depth) {
             zJ = geo.depth[J];
             const double vi = geo.vol[I], vj = geo.vol[J];
@@ -426,20 +445,30 @@ __global__ void k_bo_flows(const BoFluid* __restrict__ fp, BoGrid G, const doubl
             tmv.in = rockTransMult(geo.rock, qi.p);
             tmv.ex = rockTransMultV(geo.rock, qj.p);
             tm = &tmv;
+            if (geo.dirMob) {
+                const long ax = fi / N;     // the face's index: axis * N + lower cell
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    dmv.in[a] = geo.dirMob[(3 * ax + a) * N + I];
+                    dmv.ex[a] = geo.dirMob[(3 * ax + a) * N + J];
+                }
+                dm = &dmv;
+            }
         }
         const double dZg = (zI - zJ) * G.g;
         if (dif.diff && dsp.disp)
             faceFluxT<true, true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp, tm);
+                                        diffFace<true, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp, tm, dm);
         else if (dif.diff)
             faceFluxT<true, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi), nullptr, thp, tm);
+                                  diffFace<true, false>(dif, dsp, pv, mean, I, J, qj, fi), nullptr, thp, tm, dm);
         else if (dsp.disp)
             faceFluxT<true, false, true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, &dif,
-                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp, tm);
+                                         diffFace<false, true>(dif, dsp, pv, mean, I, J, qj, fi), &dsp, thp, tm,
+                                         dm);
         else
             faceFluxT<true>(f, qi, qj, trans[fi], area, dZg, upInTie, fl, darcy, nullptr,
-                            DiffFace{0.0, 0.0, 0.0, 0.0}, nullptr, thp, tm);
+                            DiffFace{0.0, 0.0, 0.0, 0.0}, nullptr, thp, tm, dm);
         for (int c = 0; c < 3; ++c) {
             if (flows) flows[3 * e + c] = fl[c].v;
             if (flores) flores[3 * e + c] = darcy[c];

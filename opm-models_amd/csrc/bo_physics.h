@@ -248,6 +248,16 @@ struct FaceTM {
     double ex;
 };
 
+// Directional relative permeability multipliers of a face's two cells for
+// the face's axis, per phase: the upwind cell's mobility along the face is
+// its mobility times its multiplier (BlackOilIntensiveQuantities::mobility
+// (phaseIdx, facedir), blackoilintensivequantities.hh:530-550, with the
+// problem's directional relative permeabilities from updateRelperms,
+// multiphasebaseproblem.hh:257-261, taken as kr_dir = m_dir kr).
+struct FaceDM {
+    double in[3], ex[3];
+};
+
 // Rock compressibility and compaction (blackoilintensivequantities.hh:447-480):
 // porosity *= 1 + x + x^2/2 with x = c (p_o - p_ref) where c > 0, then times
 // the problem's pore-volume multiplier of p_o (rockCompPoroMultiplier); the
@@ -323,7 +333,8 @@ template <bool WithDarcy, bool WithDiff = false, bool WithDisp = false>
 __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const IqV& ex, double trans,
                                           double area, double dZg, bool upInTie, E3 flux[3], double darcyOut[3],
                                           const BoDiff* dif = nullptr, DiffFace df = DiffFace{0.0, 0.0, 0.0, 0.0},
-                                          const BoDisp* dsp = nullptr, double thp = 0.0, const FaceTM* tm = nullptr)
+                                          const BoDisp* dsp = nullptr, double thp = 0.0, const FaceTM* tm = nullptr,
+                                          const FaceDM* dm = nullptr)
 {
     flux[0] = ec(0.0); flux[1] = ec(0.0); flux[2] = ec(0.0);
     if constexpr (WithDarcy) { darcyOut[0] = 0.0; darcyOut[1] = 0.0; darcyOut[2] = 0.0; }
@@ -346,12 +357,14 @@ __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const
         E3 gas;
         double darcyV;
         if (upIn) {
-            const E3 darcy = tm ? ((dp * in.lam[a]) * ((tm->in + tm->ex) * 0.5)) * tf : (dp * in.lam[a]) * tf;
+            const E3 lam = dm ? in.lam[a] * dm->in[a] : in.lam[a];
+            const E3 darcy = tm ? ((dp * lam) * ((tm->in + tm->ex) * 0.5)) * tf : (dp * lam) * tf;
             svf = in.invB[a] * darcy;
             if (a == kO) gas = in.Rs * svf;
             darcyV = darcy.v;
         } else {
-            const E3 darcy = tm ? dp * ((((tm->in + tm->ex) * 0.5) * ex.lam[a]) * tf) : dp * (ex.lam[a] * tf);
+            const double lam = dm ? ex.lam[a] * dm->ex[a] : ex.lam[a];
+            const E3 darcy = tm ? dp * ((((tm->in + tm->ex) * 0.5) * lam) * tf) : dp * (lam * tf);
             svf = darcy * ex.invB[a];
             if (a == kO) gas = svf * ex.Rs;
             darcyV = darcy.v;

@@ -247,6 +247,14 @@ int omodels_model_set_rock(omodels_model* h, const double* compressibility, cons
     });
 }
 
+int omodels_model_set_directional_mobility(omodels_model* h, const double* multipliers)
+{
+    return guarded([&] {
+        h->m->setDirectionalMobility(multipliers);
+        return 0;
+    });
+}
+
 int omodels_linearizer_linearize_domain(omodels_model* h, double dt, int iteration_index)
 {
     return guarded([&] {

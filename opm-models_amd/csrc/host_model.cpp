@@ -229,7 +229,7 @@ void Model::updateDiscretization(const double* faceTrans, const double* depth, c
 
 void Model::refreshGeo()
 {
-    const bool rockOn = rockComp.p || rockTabs.p;
+    const bool rockOn = rockComp.p || rockTabs.p || dirMob.p;
     if (!geoGiven && !rockOn) {
         geoDepth.release(); geoVol.release(); geoThp.release();
         geo = BoGeo{};
@@ -247,6 +247,17 @@ void Model::refreshGeo()
     }
     geo.depth = geoDepth.p; geo.vol = geoVol.p; geo.thp = geoThp.p;
     geo.rock = BoRock{rockComp.p, rockPref.p, rockTabs.p, rockTabs.p ? rockTabs.p + 1 : nullptr};
+    geo.dirMob = dirMob.p;
+}
+
+void Model::setDirectionalMobility(const double* mult)
+{
+    if (mult) checkNonNegative(mult, 9 * N, "directional mobility multipliers must be finite and >= 0");
+    sync();
+    dirMob.release();
+    if (mult) dirMob.upload(mult, 9 * (size_t)N, stream);
+    sync();
+    refreshGeo();
 }
 
 void Model::setRock(const double* comp, const double* pref, int nTab, const double* table)

@@ -161,6 +161,9 @@ public:
     void setRock(const double* comp, const double* pref, int nTab, const double* table);
     DevBuf<double> rockComp, rockPref;
     DevBuf<Tab> rockTabs;
+    // directional relative permeability multipliers [3 axes][3 phases][N]
+    void setDirectionalMobility(const double* mult);
+    DevBuf<double> dirMob;
     // the kernels' BoGeo from the given geometry and the rock terms
     void refreshGeo();
     void ensureJacobian();

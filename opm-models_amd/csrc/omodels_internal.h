@@ -41,6 +41,7 @@ struct BoGeo {
     const double* thp;
     double dt;
     BoRock rock;
+    const double* dirMob;     // [3 axes][3 phases][N] directional mobility multipliers, or nullptr
 };
 
 struct NewtonUpdateParams {

@@ -146,6 +146,7 @@ SIGNATURES = {
     "omodels_scalar_product": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
     "omodels_scalar_product_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_model_set_directional_mobility": (C.c_int, [C.c_void_p, _dp]),
     "omodels_solver_amg_levels": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "omodels_solver_get_amg_level": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "omodels_newton_update": (C.c_int, [C.c_void_p, C.POINTER(Params), _dp, _ip]),
@@ -445,6 +446,15 @@ class Model:
             raise ValueError("set_rock: the table holds three rows")
         n = 0 if t is None else t.size // 3
         check(lib().omodels_model_set_rock(self.h, ptr(c), ptr(p), n, ptr(t)), "set_rock")
+
+    def set_directional_mobility(self, mult=None):
+        """directional relative permeability multipliers [3][3][N] (axis x, y,
+        z; phase water, oil, gas; cell); None switches them off"""
+        m = None if mult is None else np.ascontiguousarray(mult, dtype=np.float64).ravel()
+        if m is not None and m.size != 9 * self.N:
+            raise ValueError("set_directional_mobility: [3][3][N] multipliers")
+        self._dirmob = m
+        check(lib().omodels_model_set_directional_mobility(self.h, ptr(m)), "set_directional_mobility")
 
     def linearize(self, dt, iteration_index=1):
         check(lib().omodels_linearizer_linearize_domain(self.h, dt, iteration_index), "linearize_domain")

@@ -366,7 +366,7 @@ struct Dispersion {
 void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, long J,
               const NbInfo& nbi, double Vin, double Vex, E3 flux[3], double* darcyOut = nullptr,
               const Diffusion* dif = nullptr, const Dispersion* dsp = nullptr, double nvI = 0.0,
-              double nvJ = 0.0, bool tmOn = false)
+              double nvJ = 0.0, bool tmOn = false, const double* dirMob = nullptr, long N = 0)
 {
     /* the mean of the two cells' rockCompTransMultiplier
      * (blackoillocalresidualtpfa.hh:372-381) */
@@ -393,11 +393,17 @@ void faceFlux(const Pvt& pvt, const IQ<E3>& in, const IQ<double>& ex, long I, lo
         else upIn = (I < J);
         const double tf = (-nbi.trans) / nbi.area;
         E3 darcy, svf;
+        /* the upwind cell's mobility along the face's axis
+         * (BlackOilIntensiveQuantities::mobility(phaseIdx, facedir),
+         * blackoilintensivequantities.hh:530-550): kr_dir = m_dir kr */
+        const long dmAt = (3L * (nbi.dir / 2) + a) * N;
         if (upIn) {
-            darcy = tmOn ? ((dp * in.lam[a]) * transMult) * tf : (dp * in.lam[a]) * tf;
+            const E3 lam = dirMob ? in.lam[a] * dirMob[dmAt + I] : in.lam[a];
+            darcy = tmOn ? ((dp * lam) * transMult) * tf : (dp * lam) * tf;
             svf = in.invB[a] * darcy;
         } else {
-            darcy = tmOn ? dp * ((transMult * ex.lam[a]) * tf) : dp * (ex.lam[a] * tf);
+            const double lam = dirMob ? ex.lam[a] * dirMob[dmAt + J] : ex.lam[a];
+            darcy = tmOn ? dp * ((transMult * lam) * tf) : dp * (lam * tf);
             svf = darcy * ex.invB[a];
         }
         flux[compOfPhase[a]] = flux[compOfPhase[a]] + svf * pvt.rho[a];
@@ -490,6 +496,7 @@ struct Problem {
         : P(p), G(p)
     {
         if (p->face_trans) checkNonNegative(p->face_trans, 3 * G.N, "transmissibilities must be finite and >= 0");
+        if (p->dir_mob) checkNonNegative(p->dir_mob, 9 * G.N, "directional mobility multipliers must be finite and >= 0");
         if (p->thpres) checkNonNegative(p->thpres, 3 * G.N, "threshold pressures must be finite and >= 0");
         if (p->depth)
             for (long I = 0; I < G.N; ++I)
@@ -630,12 +637,12 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
             if (pr.hasDisp) {
                 double darcy[3];
                 faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, darcy, pr.diffusion(), &pr.dsp,
-                         nvIn[I], nvIn[nbi.J], pr.rock.tab);
+                         nvIn[I], nvIn[nbi.J], pr.rock.tab, pr.P->dir_mob, N);
                 const double v = std::fabs(darcy[CO] / nbi.area);
                 nvOut[I] = nvOut[I] < v ? v : nvOut[I];
             } else {
                 faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, V, pr.G.volume(nbi.J), f, nullptr, pr.diffusion(), nullptr,
-                         0.0, 0.0, pr.rock.tab);
+                         0.0, 0.0, pr.rock.tab, pr.P->dir_mob, N);
             }
             double* O_ = jac + 9L * nbi.off;
             for (int e = 0; e < 3; ++e) {
@@ -1712,7 +1719,8 @@ int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersi
                 E3 f[3];
                 double d[3];
                 faceFlux(pr.pvt, qi, qj, I, nbi.J, nbi, pr.G.volume(I), pr.G.volume(nbi.J), f, d, pr.diffusion(),
-                         pr.dispersion(), dsp ? nv[I] : 0.0, dsp ? nv[nbi.J] : 0.0, pr.rock.tab);
+                         pr.dispersion(), dsp ? nv[I] : 0.0, dsp ? nv[nbi.J] : 0.0, pr.rock.tab, pr.P->dir_mob,
+                         pr.G.N);
                 for (int e = 0; e < 3; ++e) {
                     if (flows) flows[3L * k + e] = f[e].v;
                     if (flores) flores[3L * k + e] = d[e];

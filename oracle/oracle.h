@@ -62,6 +62,10 @@ typedef struct ob_problem {
     const double* rock_pref;
     int32_t n_rock;
     const double* rock_tab;
+    /* directional mobilities (blackoilintensivequantities.hh:530-550): the
+     * relative permeabilities along axis d are m_d kr with dir_mob [3][3][N]
+     * (axis, phase water/oil/gas, cell; NULL: none) */
+    const double* dir_mob;
 } ob_problem;
 
 /* Newton / linear-solver parameters; defaults are the reference's

@@ -29,7 +29,7 @@ class ObProblem(C.Structure):
         ("water_mu", C.c_double), ("rho_ref", C.c_double * 3),
         ("src_cell", _ip), ("src_type", _ip), ("src_a", _dp), ("src_b", _dp),
         ("face_trans", _dp), ("depth", _dp), ("volume", _dp), ("thpres", _dp),
-        ("rock_comp", _dp), ("rock_pref", _dp), ("n_rock", C.c_int32), ("rock_tab", _dp),
+        ("rock_comp", _dp), ("rock_pref", _dp), ("n_rock", C.c_int32), ("rock_tab", _dp), ("dir_mob", _dp),
     ]
 
 
@@ -208,6 +208,19 @@ def with_rock(prob, comp=None, pref=None, table=None):
         setattr(prob, name, P(a))
     prob.n_rock = 0 if table is None else len(np.ravel(table)) // 3
     prob._rock_keep = keep
+    return prob
+
+
+def with_directional_mobility(prob, mult):
+    """prob with directional relative permeability multipliers [3][3][N]
+    (axis, phase water/oil/gas, cell); None for none"""
+    if mult is None:
+        prob.dir_mob = None
+        prob._dm_keep = None
+        return prob
+    a = np.ascontiguousarray(mult, dtype=np.float64).ravel()
+    prob.dir_mob = P(a)
+    prob._dm_keep = a
     return prob
 
 
