@@ -219,6 +219,27 @@ int omodels_linearizer_update_discretization(omodels_model* m, const double* tra
  * linearizations, FLOWS, output fields and simulate calls include them. */
 int omodels_model_set_rock(omodels_model* m, const double* compressibility, const double* ref_pressure,
                            int32_t n_table, const double* table);
+/* Boundary conditions: the reference's boundary faces with a condition
+ * (TpfaLinearizer::createMatrix_ collects them when the problem has
+ * nonTrivialBoundaryConditions, tpfalinearizer.hh:476-497; linearize_ adds
+ * their terms after the cell loop, :849-865; the term is
+ * BlackOilLocalResidualTPFA::computeBoundaryFlux,
+ * blackoillocalresidualtpfa.hh:507-620).  n faces; face[b] (0..5: -x, +x,
+ * -y, +y, -z, +z) must be a face of cell[b] without a neighbour; type[b]:
+ * 0 none (BCType::NONE), 1 rate (RATE: mass_rate[b][3], kg/(s m^2) in
+ * equation order oil, water, gas, positive out of the cell), 2 free and 3
+ * Dirichlet (FREE / DIRICHLET: the boundary's fluid state from the primary
+ * variables state[b][3] with gas meaning meaning[b]).  trans[b]: the face's
+ * transmissibility (NULL: the cell's half transmissibility A K d/|d|^2 of
+ * transfluxmodule.hh:457-479); face_depth[b]: the face's depth (NULL: the
+ * cell's depth for lateral faces, minus / plus half a cell for -z / +z);
+ * the defaults are evaluated when the conditions are set.  The terms are
+ * added at the end of linearize_domain, each cell's faces in face order;
+ * blocks a caller adds with add_sparse_sources afterwards come after them.
+ * n = 0 removes all.  The ensemble does not take boundary conditions. */
+int omodels_model_set_boundary_conditions(omodels_model* m, int32_t n, const int32_t* cell, const int32_t* face,
+                                          const int32_t* type, const double* mass_rate, const double* state,
+                                          const uint8_t* meaning, const double* trans, const double* face_depth);
 /* Directional mobilities (BlackOilIntensiveQuantities::mobility(phaseIdx,
  * facedir), blackoilintensivequantities.hh:530-550, used by the face flux at
  * blackoillocalresidualtpfa.hh:341, :375-381).  The reference leaves the

@@ -663,6 +663,65 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
                        cell, type, a, b, outRes, outDiag, vol);
 }
 
+// Boundary terms after the cell loop and the sources (tpfalinearizer.hh:
+// 849-865): one thread per cell with conditions, its faces in face order,
+// each face's term added to the residual and the diagonal block.
+__global__ void k_bo_boundary(const BoFluid* __restrict__ fp, BoGrid G, const double* __restrict__ pv,
+                              const uint8_t* __restrict__ mean, BoBc bc, const double* __restrict__ depth,
+                              double* __restrict__ res, double* __restrict__ jac, const int* __restrict__ diagOff)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= bc.nGroups) return;
+    const BoFluid& f = *fp;
+    const int2 grp = bc.groups[t];
+    const long I = bc.cell[grp.x];
+    const long nxy = (long)G.nx * G.ny;
+    const double zI = depth ? depth[I] : G.dz * ((double)(I / nxy) + 0.5);
+    IqE qi;
+    iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
+    double r[3], D[9];
+    double* Dout = jac + 9L * diagOff[I];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) r[e] = res[3 * I + e];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) D[q] = Dout[q];
+    for (int q = 0; q < grp.y; ++q) {
+        const int b = grp.x + q;
+        const int type = bc.type[b];
+        if (type == 0) continue;
+        const int ax = bc.face[b] / 2;
+        const double area = G.area[ax];
+        E3 out[3];
+        if (type == 1) {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) out[e] = ec(bc.rate[3 * b + e] * area);
+        } else {
+            IqV qb;
+            iqValues(f, bc.state[3 * b], bc.state[3 * b + 1], bc.state[3 * b + 2], bc.meaning[b], qb);
+            boundaryFreeFlux(f, qi, qb, bc.trans[b], area, (zI - bc.zFace[b]) * G.g, out);
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            r[e] += out[e].v;
+            D[3 * e + 0] += out[e].d0;
+            D[3 * e + 1] += out[e].d1;
+            D[3 * e + 2] += out[e].d2;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) res[3 * I + e] = r[e];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Dout[q] = D[q];
+}
+
+void launchBoBoundary(const BoFluid* fp, const BoGrid& G, const double* pv, const uint8_t* mean, const BoBc& bc,
+                      const double* depth, double* res, double* jac, const int* diagOff, hipStream_t s)
+{
+    if (bc.nGroups <= 0) return;
+    hipLaunchKernelGGL(k_bo_boundary, dim3((unsigned)((bc.nGroups + 127) / 128)), dim3(128), 0, s, fp, G, pv, mean, bc,
+                       depth, res, jac, diagOff);
+}
+
 void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif,
                    const BoDisp* dsp, const BoGeo* geo)

@@ -413,6 +413,49 @@ __device__ __forceinline__ void faceFluxT(const BoFluid& f, const IqE& in, const
     flux[0] = flux[0] * area; flux[1] = flux[1] * area; flux[2] = flux[2] * area;
 }
 
+// BlackOilLocalResidualTPFA::computeBoundaryFlux (blackoillocalresidualtpfa.hh:
+// 507-620) of a free (or Dirichlet) boundary face, times the face area
+// (tpfalinearizer.hh:849-865).  qb: the boundary's fluid state; per phase the
+// trans flux module's boundary gradient (transfluxmodule.hh:323-409) per
+// area: the boundary pressure plus the mean density times the depth
+// difference to the face, the upstream side by that difference (ties: the
+// cell), the upstream mobility; the flux leaves with the cell's 1/B and Rs
+// when the boundary pressure is below the cell's, enters with the
+// boundary's when it is above; evalPhaseFluxes_ (:753-811) with the
+// reference densities, then adaptMassConservationQuantities_ (:825-851)
+// multiplies by them once more, as the reference does.
+__device__ __forceinline__ void boundaryFreeFlux(const BoFluid& f, const IqE& qi, const IqV& qb, double trans,
+                                                 double area, double dZg, E3 out[3])
+{
+    const double tf = (-trans) / area;
+    E3 bdy[3] = {ec(0.0), ec(0.0), ec(0.0)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        E3 tmp[3] = {ec(0.0), ec(0.0), ec(0.0)};
+        const E3 rhoAvg = (qi.rho[a] + qb.rho[a]) * 0.5;
+        const E3 pEx = rhoAvg * dZg + qb.p;
+        const E3 dp = pEx - qi.p;
+        const E3 darcy = dp.v > 0.0 ? (dp * qb.lam[a]) * tf : (dp * qi.lam[a]) * tf;
+        const int c = compOfPhase(a);
+        if (qb.p < qi.p.v) {
+            const E3 svf = qi.invB[a] * darcy;
+            tmp[c] = tmp[c] + svf * f.rho[a];
+            if (a == kO) tmp[kCG] = tmp[kCG] + (qi.Rs * svf) * f.rho[kG];
+        } else if (qb.p > qi.p.v) {
+            const E3 svf = darcy * qb.invB[a];
+            tmp[c] = tmp[c] + svf * f.rho[a];
+            if (a == kO) tmp[kCG] = tmp[kCG] + (svf * qb.Rs) * f.rho[kG];
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) bdy[e] = bdy[e] + tmp[e];
+    }
+    bdy[kCW] = bdy[kCW] * f.rho[kW];
+    bdy[kCG] = bdy[kCG] * f.rho[kG];
+    bdy[kCO] = bdy[kCO] * f.rho[kO];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) out[e] = bdy[e] * area;
+}
+
 // Well source terms per unit volume (ReservoirProblem-style problem.source()).
 __device__ __forceinline__ void sourceEval(const BoFluid& f, const IqE& q, int type, double a, double b,
                                            double V, E3 src[3])

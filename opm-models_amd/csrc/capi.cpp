@@ -247,6 +247,16 @@ int omodels_model_set_rock(omodels_model* h, const double* compressibility, cons
     });
 }
 
+int omodels_model_set_boundary_conditions(omodels_model* h, int32_t n, const int32_t* cell, const int32_t* face,
+                                          const int32_t* type, const double* mass_rate, const double* state,
+                                          const uint8_t* meaning, const double* trans, const double* face_depth)
+{
+    return guarded([&] {
+        h->m->setBoundary(n, cell, face, type, mass_rate, state, meaning, trans, face_depth);
+        return 0;
+    });
+}
+
 int omodels_model_set_directional_mobility(omodels_model* h, const double* multipliers)
 {
     return guarded([&] {

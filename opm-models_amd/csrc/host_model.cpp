@@ -61,6 +61,7 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
 
     trans.upload(topo.trans.data(), topo.trans.size(), stream);
     transGrid = topo.trans;
+    permHost.assign(p->perm, p->perm + 3 * (size_t)N);
     fluidDev.upload(&fluid, 1, stream);
     poro.upload(p->poro, N, stream);
     rowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
@@ -143,7 +144,103 @@ void Model::linearizeDomain(double dt, int iterationIndex)
                         srcB.p, res.p, jac.p, diagOff.p, stream, geo.vol);
     else
         addBuiltinSparseSources();
+    // boundary terms after the cell loop (:849-865)
+    launchBoBoundary(fluidDev.p, G, pv0.p, mean0.p, bc, geo.depth, res.p, jac.p, diagOff.p, stream);
     OM_HIP(hipGetLastError());
+}
+
+void Model::setBoundary(int n, const int* cell, const int* face, const int* type, const double* rate,
+                        const double* state, const uint8_t* meaning, const double* btrans, const double* zface)
+{
+    struct Face {
+        int cell, face, type;
+        double rate[3], state[3], trans, zf;
+        uint8_t meaning;
+    };
+    std::vector<Face> fs;
+    if (n < 0) throw std::invalid_argument("boundary: n >= 0");
+    if (n > 0 && (!cell || !face || !type)) throw std::invalid_argument("boundary: cells, faces and types");
+    const long nxy = (long)G.nx * G.ny;
+    const double h[3] = {G.dx, G.dy, G.dz};
+    for (int b = 0; b < n; ++b) {
+        Face f{};
+        f.cell = cell[b]; f.face = face[b]; f.type = type[b];
+        if (f.cell < 0 || f.cell >= N || f.face < 0 || f.face > 5)
+            throw std::invalid_argument("boundary: cell or face out of range");
+        const long I = f.cell;
+        const long ijk[3] = {I % G.nx, (I / G.nx) % G.ny, I / nxy};
+        const int ext[3] = {G.nx, G.ny, G.nz};
+        const int ax = f.face / 2;
+        const bool lower = (f.face & 1) == 0;
+        if (lower ? ijk[ax] > 0 : ijk[ax] + 1 < ext[ax]) throw std::invalid_argument("boundary: the face has a neighbour");
+        if (f.type < 0 || f.type > 3) throw std::invalid_argument("boundary: type must be 0..3");
+        if (f.type == 1) {
+            if (!rate) throw std::invalid_argument("boundary: rate faces need rates");
+            for (int e = 0; e < 3; ++e) {
+                f.rate[e] = rate[3 * b + e];
+                if (!std::isfinite(f.rate[e])) throw std::invalid_argument("boundary: rates must be finite");
+            }
+        }
+        if (f.type >= 2) {
+            if (!state || !meaning) throw std::invalid_argument("boundary: free faces need a state");
+            for (int e = 0; e < 3; ++e) {
+                f.state[e] = state[3 * b + e];
+                if (!std::isfinite(f.state[e])) throw std::invalid_argument("boundary: states must be finite");
+            }
+            f.meaning = meaning[b];
+            if (f.meaning != kMeanSg && f.meaning != kMeanRs) throw std::invalid_argument("boundary: meaning");
+        }
+        if (btrans) {
+            f.trans = btrans[b];
+            if (!(f.trans >= 0.0) || !std::isfinite(f.trans))
+                throw std::invalid_argument("boundary: transmissibilities must be finite and >= 0");
+        } else {
+            // the cell's half of a face's transmissibility (transfluxmodule.hh:457-479)
+            const double A = G.area[ax], ndot = 0.5 * h[ax], dist2 = ndot * ndot;
+            f.trans = A * permHost[(size_t)ax * N + I] * ndot / dist2;
+        }
+        if (zface) {
+            f.zf = zface[b];
+            if (!std::isfinite(f.zf)) throw std::invalid_argument("boundary: face depths must be finite");
+        } else {
+            const double zc = depthHost.empty() ? G.dz * ((double)ijk[2] + 0.5) : depthHost[I];
+            f.zf = ax < 2 ? zc : (f.face == 4 ? zc - 0.5 * G.dz : zc + 0.5 * G.dz);
+        }
+        fs.push_back(f);
+    }
+    std::stable_sort(fs.begin(), fs.end(),
+                     [](const Face& a, const Face& b) { return a.cell != b.cell ? a.cell < b.cell : a.face < b.face; });
+    for (size_t k = 1; k < fs.size(); ++k)
+        if (fs[k].cell == fs[k - 1].cell && fs[k].face == fs[k - 1].face)
+            throw std::invalid_argument("boundary: a face given twice");
+    std::vector<int2> groups;
+    std::vector<int> c(fs.size()), fa(fs.size()), ty(fs.size());
+    std::vector<double> ra(3 * fs.size()), st(3 * fs.size()), tr(fs.size()), zf(fs.size());
+    std::vector<uint8_t> me(fs.size());
+    for (size_t k = 0; k < fs.size(); ++k) {
+        if (k == 0 || fs[k].cell != fs[k - 1].cell) groups.push_back(make_int2((int)k, 0));
+        groups.back().y++;
+        c[k] = fs[k].cell; fa[k] = fs[k].face; ty[k] = fs[k].type;
+        for (int e = 0; e < 3; ++e) { ra[3 * k + e] = fs[k].rate[e]; st[3 * k + e] = fs[k].state[e]; }
+        tr[k] = fs[k].trans; zf[k] = fs[k].zf; me[k] = fs[k].meaning;
+    }
+    sync();
+    bc = BoBc{};
+    bcGroups.release(); bcCell.release(); bcFace.release(); bcType.release(); bcRate.release(); bcState.release();
+    bcMean.release(); bcTrans.release(); bcZ.release();
+    if (fs.empty()) return;
+    bcGroups.upload(groups.data(), groups.size(), stream);
+    bcCell.upload(c.data(), c.size(), stream);
+    bcFace.upload(fa.data(), fa.size(), stream);
+    bcType.upload(ty.data(), ty.size(), stream);
+    bcRate.upload(ra.data(), ra.size(), stream);
+    bcState.upload(st.data(), st.size(), stream);
+    bcMean.upload(me.data(), me.size(), stream);
+    bcTrans.upload(tr.data(), tr.size(), stream);
+    bcZ.upload(zf.data(), zf.size(), stream);
+    sync();
+    bc = BoBc{(int)groups.size(), bcGroups.p, bcCell.p, bcFace.p, bcType.p, bcRate.p, bcState.p, bcMean.p,
+              bcTrans.p, bcZ.p};
 }
 
 void Model::computeSourceBlocks()

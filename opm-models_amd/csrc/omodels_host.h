@@ -164,6 +164,15 @@ public:
     // directional relative permeability multipliers [3 axes][3 phases][N]
     void setDirectionalMobility(const double* mult);
     DevBuf<double> dirMob;
+    // boundary conditions (see omodels_model_set_boundary_conditions)
+    void setBoundary(int n, const int* cell, const int* face, const int* type, const double* rate,
+                     const double* state, const uint8_t* meaning, const double* trans, const double* zface);
+    BoBc bc{};
+    DevBuf<int2> bcGroups;
+    DevBuf<int> bcCell, bcFace, bcType;
+    DevBuf<double> bcRate, bcState, bcTrans, bcZ;
+    DevBuf<uint8_t> bcMean;
+    std::vector<double> permHost;
     // the kernels' BoGeo from the given geometry and the rock terms
     void refreshGeo();
     void ensureJacobian();

@@ -44,6 +44,21 @@ struct BoGeo {
     const double* dirMob;     // [3 axes][3 phases][N] directional mobility multipliers, or nullptr
 };
 
+// Boundary faces with a condition, sorted by cell and face, grouped by cell
+// ({first, count} per cell that has any)
+struct BoBc {
+    int nGroups;
+    const int2* groups;
+    const int* cell;
+    const int* face;
+    const int* type;            // 0 none, 1 rate, 2 / 3 free (Dirichlet)
+    const double* rate;         // [n][3] mass rate per area, equation order
+    const double* state;        // [n][3] the boundary's primary variables
+    const uint8_t* meaning;     // [n]
+    const double* trans;        // [n]
+    const double* zFace;        // [n]
+};
+
 struct NewtonUpdateParams {
     double dpMaxRel, dsMax, oscThresh;
 };
@@ -169,6 +184,8 @@ void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* 
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
                           double* outRes, double* outDiag, hipStream_t s, const double* vol = nullptr);
+void launchBoBoundary(const BoFluid* fp, const BoGrid& G, const double* pv, const uint8_t* mean, const BoBc& bc,
+                      const double* depth, double* res, double* jac, const int* diagOff, hipStream_t s);
 void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif = nullptr,
                    const BoDisp* dsp = nullptr, const BoGeo* geo = nullptr);

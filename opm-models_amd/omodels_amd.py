@@ -146,6 +146,7 @@ SIGNATURES = {
     "omodels_scalar_product": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
     "omodels_scalar_product_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_model_set_boundary_conditions": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip, _dp, _dp, _u8p, _dp, _dp]),
     "omodels_model_set_directional_mobility": (C.c_int, [C.c_void_p, _dp]),
     "omodels_solver_amg_levels": (C.c_int, [C.c_void_p, _ip, C.c_int]),
     "omodels_solver_get_amg_level": (C.c_int, [C.c_void_p, C.c_int, _dp]),
@@ -446,6 +447,27 @@ class Model:
             raise ValueError("set_rock: the table holds three rows")
         n = 0 if t is None else t.size // 3
         check(lib().omodels_model_set_rock(self.h, ptr(c), ptr(p), n, ptr(t)), "set_rock")
+
+    def set_boundary(self, cell=None, face=None, type=None, rate=None, state=None, meaning=None, trans=None,
+                     zface=None):
+        """boundary conditions on boundary faces (see
+        omodels_model_set_boundary_conditions); cell None removes them"""
+        if cell is None:
+            check(lib().omodels_model_set_boundary_conditions(self.h, 0, None, None, None, None, None, None, None,
+                                                              None), "set_boundary")
+            self._bc = None
+            return
+        keep = dict(cell=np.ascontiguousarray(cell, dtype=np.int32).ravel(),
+                    face=np.ascontiguousarray(face, dtype=np.int32).ravel(),
+                    type=np.ascontiguousarray(type, dtype=np.int32).ravel())
+        for k, a, dt in (("rate", rate, np.float64), ("state", state, np.float64), ("meaning", meaning, np.uint8),
+                         ("trans", trans, np.float64), ("zface", zface, np.float64)):
+            keep[k] = None if a is None else np.ascontiguousarray(a, dtype=dt).ravel()
+        n = keep["cell"].size
+        check(lib().omodels_model_set_boundary_conditions(
+            self.h, n, ptr(keep["cell"], _ip), ptr(keep["face"], _ip), ptr(keep["type"], _ip), ptr(keep["rate"]),
+            ptr(keep["state"]), ptr(keep["meaning"], _u8p), ptr(keep["trans"]), ptr(keep["zface"])), "set_boundary")
+        self._bc = keep
 
     def set_directional_mobility(self, mult=None):
         """directional relative permeability multipliers [3][3][N] (axis x, y,

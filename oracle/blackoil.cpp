@@ -521,6 +521,73 @@ struct Problem {
         }
         if (d) setDiffusion(*d);
         if (e) setDispersion(*e);
+        buildBoundary();
+    }
+    /* the boundary faces with a condition, in cell and then face order (the
+     * order TpfaLinearizer::createMatrix_ collects them, tpfalinearizer.hh:
+     * 476-497) */
+    struct Bc {
+        long cell;
+        int face, type;
+        double rate[3], state[3], trans, zf;
+        uint8_t meaning;
+    };
+    std::vector<Bc> bcs;
+    void buildBoundary()
+    {
+        const ob_problem* p = P;
+        bcs.clear();
+        if (p->n_bc <= 0) return;
+        if (!p->bc_cell || !p->bc_face || !p->bc_type) throw std::invalid_argument("boundary: cells, faces and types");
+        for (int b = 0; b < p->n_bc; ++b) {
+            Bc c{};
+            c.cell = p->bc_cell[b];
+            c.face = p->bc_face[b];
+            c.type = p->bc_type[b];
+            if (c.cell < 0 || c.cell >= G.N || c.face < 0 || c.face > 5)
+                throw std::invalid_argument("boundary: cell or face out of range");
+            if (G.nb(c.cell, c.face) >= 0) throw std::invalid_argument("boundary: the face has a neighbour");
+            if (c.type < 0 || c.type > 3) throw std::invalid_argument("boundary: type must be 0..3");
+            if (c.type == 1) {
+                if (!p->bc_rate) throw std::invalid_argument("boundary: rate faces need rates");
+                for (int e = 0; e < 3; ++e) {
+                    c.rate[e] = p->bc_rate[3 * b + e];
+                    if (!std::isfinite(c.rate[e])) throw std::invalid_argument("boundary: rates must be finite");
+                }
+            }
+            if (c.type >= 2) {
+                if (!p->bc_state || !p->bc_meaning) throw std::invalid_argument("boundary: free faces need a state");
+                for (int e = 0; e < 3; ++e) {
+                    c.state[e] = p->bc_state[3 * b + e];
+                    if (!std::isfinite(c.state[e])) throw std::invalid_argument("boundary: states must be finite");
+                }
+                c.meaning = p->bc_meaning[b];
+                if (c.meaning != MEAN_SG && c.meaning != MEAN_RS) throw std::invalid_argument("boundary: meaning");
+            }
+            const int ax = c.face / 2;
+            if (p->bc_trans) {
+                c.trans = p->bc_trans[b];
+                if (!(c.trans >= 0.0) || !std::isfinite(c.trans))
+                    throw std::invalid_argument("boundary: transmissibilities must be finite and >= 0");
+            } else {
+                /* the cell's half of a face's transmissibility (transfluxmodule.hh:457-479) */
+                const double A = G.area(c.face), ndot = 0.5 * G.h(c.face), dist2 = ndot * ndot;
+                c.trans = A * p->perm[(long)ax * G.N + c.cell] * ndot / dist2;
+            }
+            if (p->bc_zface) {
+                c.zf = p->bc_zface[b];
+                if (!std::isfinite(c.zf)) throw std::invalid_argument("boundary: face depths must be finite");
+            } else {
+                c.zf = ax < 2 ? G.zc(c.cell) : (c.face == 4 ? G.zc(c.cell) - 0.5 * G.dz : G.zc(c.cell) + 0.5 * G.dz);
+            }
+            bcs.push_back(c);
+        }
+        std::stable_sort(bcs.begin(), bcs.end(), [](const Bc& a, const Bc& b) {
+            return a.cell != b.cell ? a.cell < b.cell : a.face < b.face;
+        });
+        for (size_t k = 1; k < bcs.size(); ++k)
+            if (bcs[k].cell == bcs[k - 1].cell && bcs[k].face == bcs[k - 1].face)
+                throw std::invalid_argument("boundary: a face given twice");
     }
     /* the problem's dispersivity(I, J) per face (tpfalinearizer.hh:465-467):
      * given per face, or the harmonic mean of A E |n.d| / |d|^2 of the two
@@ -605,6 +672,57 @@ void computeStorageAll(const Problem& pr, const double* pv, const uint8_t* mean,
 /* TpfaLinearizer::linearize_ (tpfalinearizer.hh:711-866): resetSystem_ then
  * one pass over cells, each cell owning r[I], A[I][I] and the column blocks
  * A[J][I] of its neighbours. */
+/* BlackOilLocalResidualTPFA::computeBoundaryFlux (blackoillocalresidualtpfa.hh:
+ * 507-620) times the face area (tpfalinearizer.hh:849-865).  Rate faces:
+ * the mass rates per area (setMassRate without the surface-volume
+ * conversion, blackoilratevector.hh:92-117).  Free faces: per phase the
+ * gradient of the trans flux module's boundary faces (transfluxmodule.hh:
+ * 323-409) per area: the boundary pressure plus the mean density times the
+ * depth difference to the face, the upstream side by that difference (ties:
+ * the cell), the upstream mobility (the boundary's from its own saturations
+ * and viscosities); the flux leaves with the cell's 1/B and Rs when the
+ * boundary pressure is below the cell's, enters with the boundary's when it
+ * is above, nothing when equal; evalPhaseFluxes_ (:753-811) with the
+ * reference densities, then adaptMassConservationQuantities_ (:825-851)
+ * multiplies by them once more, as the reference does. */
+void boundaryFlux(const Pvt& pvt, const IQ<E3>& qi, const Problem::Bc& bc, double area, double zIn, double g,
+                  E3 out[3])
+{
+    for (int e = 0; e < 3; ++e) out[e] = E3(0.0);
+    if (bc.type == 1) {
+        for (int e = 0; e < 3; ++e) out[e] = E3(bc.rate[e] * area);
+        return;
+    }
+    IQ<double> qb;
+    computeIQ<double>(pvt, bc.state, bc.meaning, 0.0, qb);
+    const double dZg = (zIn - bc.zf) * g;
+    const double tf = (-bc.trans) / area;
+    E3 bdy[3] = {E3(0.0), E3(0.0), E3(0.0)};
+    for (int a = 0; a < 3; ++a) {
+        E3 tmp[3] = {E3(0.0), E3(0.0), E3(0.0)};
+        const E3 rhoAvg = (qi.rho[a] + qb.rho[a]) * 0.5;
+        const E3 pEx = rhoAvg * dZg + qb.p[a];
+        const E3 dp = pEx - qi.p[a];
+        const E3 darcy = dp.v > 0.0 ? (dp * qb.lam[a]) * tf : (dp * qi.lam[a]) * tf;
+        const double pB = qb.p[a], pIn = qi.p[a].v;
+        const int c = compOfPhase[a];
+        if (pB < pIn) {
+            const E3 svf = qi.invB[a] * darcy;
+            tmp[c] = tmp[c] + svf * pvt.rho[a];
+            if (a == O) tmp[CG] = tmp[CG] + (qi.Rs * svf) * pvt.rho[G];
+        } else if (pB > pIn) {
+            const E3 svf = darcy * qb.invB[a];
+            tmp[c] = tmp[c] + svf * pvt.rho[a];
+            if (a == O) tmp[CG] = tmp[CG] + (svf * qb.Rs) * pvt.rho[G];
+        }
+        for (int e = 0; e < 3; ++e) bdy[e] = bdy[e] + tmp[e];
+    }
+    bdy[CW] = bdy[CW] * pvt.rho[W];
+    bdy[CG] = bdy[CG] * pvt.rho[G];
+    bdy[CO] = bdy[CO] * pvt.rho[O];
+    for (int e = 0; e < 3; ++e) out[e] = bdy[e] * area;
+}
+
 /* With dispersion, nvIn holds each cell's oil normVelocityCell from the
  * previous linearization (zeros before the first, createFlows_,
  * tpfalinearizer.hh:571-619) and nvOut receives this linearization's: the
@@ -683,6 +801,21 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
                 r[e] += t.v;
                 for (int pvi = 0; pvi < 3; ++pvi) D[3 * e + pvi] += t.d[pvi];
             }
+        }
+    }
+    /* boundary terms after the cell loop, the faces with a condition in cell
+     * and face order (tpfalinearizer.hh:849-865) */
+    for (const Problem::Bc& bc : pr.bcs) {
+        if (bc.type == 0) continue;
+        IQ<E3> qi;
+        pr.iq<E3>(pv, mean, bc.cell, qi);
+        E3 f[3];
+        boundaryFlux(pr.pvt, qi, bc, pr.G.area(bc.face), pr.G.zc(bc.cell), pr.G.g, f);
+        double* r = res + 3 * bc.cell;
+        double* D = jac + 9L * pr.pat.diagOff[bc.cell];
+        for (int e = 0; e < 3; ++e) {
+            r[e] += f[e].v;
+            for (int pvi = 0; pvi < 3; ++pvi) D[3 * e + pvi] += f[e].d[pvi];
         }
     }
 }

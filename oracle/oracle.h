@@ -66,6 +66,24 @@ typedef struct ob_problem {
      * relative permeabilities along axis d are m_d kr with dir_mob [3][3][N]
      * (axis, phase water/oil/gas, cell; NULL: none) */
     const double* dir_mob;
+    /* boundary conditions (TpfaLinearizer::boundaryInfo_, tpfalinearizer.hh:
+     * 476-497; BlackOilLocalResidualTPFA::computeBoundaryFlux,
+     * blackoillocalresidualtpfa.hh:507-620): n_bc faces, each a boundary
+     * face bc_face (0..5: -x, +x, -y, +y, -z, +z) of cell bc_cell; bc_type 0
+     * none, 1 rate (bc_rate [n][3] mass rate per area in equation order oil,
+     * water, gas), 2 free (bc_state [n][3] primary variables with
+     * bc_meaning [n] give the boundary's fluid state); bc_trans [n] (NULL:
+     * the cell's half transmissibility) and bc_zface [n] (NULL: the grid's
+     * face depth) */
+    int32_t n_bc;
+    const int32_t* bc_cell;
+    const int32_t* bc_face;
+    const int32_t* bc_type;
+    const double* bc_rate;
+    const double* bc_state;
+    const uint8_t* bc_meaning;
+    const double* bc_trans;
+    const double* bc_zface;
 } ob_problem;
 
 /* Newton / linear-solver parameters; defaults are the reference's

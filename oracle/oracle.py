@@ -30,6 +30,8 @@ class ObProblem(C.Structure):
         ("src_cell", _ip), ("src_type", _ip), ("src_a", _dp), ("src_b", _dp),
         ("face_trans", _dp), ("depth", _dp), ("volume", _dp), ("thpres", _dp),
         ("rock_comp", _dp), ("rock_pref", _dp), ("n_rock", C.c_int32), ("rock_tab", _dp), ("dir_mob", _dp),
+        ("n_bc", C.c_int32), ("bc_cell", _ip), ("bc_face", _ip), ("bc_type", _ip), ("bc_rate", _dp),
+        ("bc_state", _dp), ("bc_meaning", _u8p), ("bc_trans", _dp), ("bc_zface", _dp),
     ]
 
 
@@ -221,6 +223,30 @@ def with_directional_mobility(prob, mult):
     a = np.ascontiguousarray(mult, dtype=np.float64).ravel()
     prob.dir_mob = P(a)
     prob._dm_keep = a
+    return prob
+
+
+def with_boundary(prob, cell=None, face=None, type=None, rate=None, state=None, meaning=None, trans=None,
+                  zface=None):
+    """prob with boundary conditions (see ob_problem); cell None for none"""
+    if cell is None:
+        prob.n_bc = 0
+        prob._bc_keep = None
+        return prob
+    keep = {}
+    n = len(cell)
+    for name, a, dt, pt in (("bc_cell", cell, np.int32, _ip), ("bc_face", face, np.int32, _ip),
+                            ("bc_type", type, np.int32, _ip), ("bc_rate", rate, np.float64, _dp),
+                            ("bc_state", state, np.float64, _dp), ("bc_meaning", meaning, np.uint8, _u8p),
+                            ("bc_trans", trans, np.float64, _dp), ("bc_zface", zface, np.float64, _dp)):
+        if a is None:
+            setattr(prob, name, None)
+            continue
+        arr = np.ascontiguousarray(a, dtype=dt).ravel()
+        keep[name] = arr
+        setattr(prob, name, P(arr, pt))
+    prob.n_bc = n
+    prob._bc_keep = keep
     return prob
 
 
