@@ -240,6 +240,45 @@ int omodels_model_set_rock(omodels_model* m, const double* compressibility, cons
 int omodels_model_set_boundary_conditions(omodels_model* m, int32_t n, const int32_t* cell, const int32_t* face,
                                           const int32_t* type, const double* mass_rate, const double* state,
                                           const uint8_t* meaning, const double* trans, const double* face_depth);
+/* Auxiliary equations (BaseAuxiliaryModule, baseauxiliarymodule.hh:55-121):
+ * n_dofs degrees of freedom after the N cells (rows N .. N + n_dofs - 1 of
+ * the system, each with the model's three equations, setDofOffset) and the
+ * pattern entries their module adds (addNeighbors, added by createMatrix_ at
+ * tpfalinearizer.hh:503-507): block (row[k], col[k]) for k < n_links, rows
+ * and columns in 0 .. N + n_dofs - 1; every auxiliary row needs its diagonal
+ * block among them.  The exported pattern, Jacobian, residual, solver
+ * vectors and Newton update then have N + n_dofs rows; the grid rows' blocks
+ * keep their values, the auxiliary entries start from zero in each
+ * linearization (resetSystem_); the auxiliary solution starts at zero
+ * (applyInitial: set it with omodels_model_set_auxiliary_solution).  The
+ * Newton error leaves the auxiliary rows out (newtonmethod.hh:583-585) and
+ * their update is current - delta (:704-708).  The solver uses the
+ * level-scheduled block ILU(0) or block Jacobi of the whole system (no AMG,
+ * no two-colour ILU).  n_dofs = 0 returns to the grid's pattern.  FLOWS and
+ * the ensemble do not see auxiliary DOFs. */
+int omodels_model_set_auxiliary(omodels_model* m, int32_t n_dofs, int32_t n_links, const int32_t* row,
+                                const int32_t* col);
+/* solution(0) of the auxiliary DOFs, [n_dofs][3] (also solution(1) when
+ * both_levels != 0) */
+int omodels_model_set_auxiliary_solution(omodels_model* m, const double* x, int both_levels);
+int omodels_model_get_auxiliary_solution(omodels_model* m, double* x);
+/* The module's linearize(matrix, residual) (linearizeAuxiliaryEquations,
+ * tpfalinearizer.hh:254-281): after linearize_domain, block k (row-major 3x3)
+ * is added to the Jacobian's block (block_row[k], block_col[k]), which must
+ * be in the pattern, and res[k][3] to the residual's row res_row[k], in the
+ * given order.  Nothing is added when any entry is rejected. */
+int omodels_linearizer_add_auxiliary(omodels_model* m, int32_t n_blocks, const int32_t* block_row,
+                                     const int32_t* block_col, const double* blocks, int32_t n_res,
+                                     const int32_t* res_row, const double* res);
+/* The module inside omodels_simulate: fn(user, m, newton_iteration) is called
+ * after every linearize_domain of the Newton loop (NewtonMethod::apply calls
+ * linearizeAuxiliaryEquations_ there, newtonmethod.hh:287) and adds its
+ * terms with omodels_linearizer_add_auxiliary; a nonzero return fails the
+ * attempt like the reference's NumericalProblem (the step is then retried
+ * with half the time step).  The auxiliary solution follows the time levels
+ * with the cells'.  omodels_simulate with auxiliary DOFs needs it. */
+typedef int (*omodels_aux_linearize_fn)(void* user, omodels_model* m, int newton_iteration);
+int omodels_model_set_auxiliary_module(omodels_model* m, omodels_aux_linearize_fn fn, void* user);
 /* Directional mobilities (BlackOilIntensiveQuantities::mobility(phaseIdx,
  * facedir), blackoilintensivequantities.hh:530-550, used by the face flux at
  * blackoillocalresidualtpfa.hh:341, :375-381).  The reference leaves the

@@ -405,6 +405,36 @@ __global__ void k_add_sparse_sources(int n, const int* __restrict__ cell, const 
     }
 }
 
+// The auxiliary module's additions (linearizeAuxiliaryEquations,
+// tpfalinearizer.hh:254-281): one wave; lane q < 9 owns entry q of every
+// block and lane 9 + e entry e of every residual row, each walking the list in
+// order, so entries that meet in one block add in the given order.
+__global__ void k_add_auxiliary(int nb, const int* __restrict__ boff, const double* __restrict__ blocks, int nr,
+                                const int* __restrict__ rrow, const double* __restrict__ rvals,
+                                double* __restrict__ jac, double* __restrict__ res)
+{
+    const int c = threadIdx.x;
+    if (c < 9) {
+        for (int k = 0; k < nb; ++k) jac[9L * boff[k] + c] += blocks[9L * k + c];
+    } else if (c < 12) {
+        const int e = c - 9;
+        for (int k = 0; k < nr; ++k) res[3L * rrow[k] + e] += rvals[3L * k + e];
+    }
+}
+
+// The auxiliary DOFs' Newton update (newtonmethod.hh:704-708):
+// next = current - update; a non-finite update is counted in counters[1]
+// (the reference rejects any non-finite update, :677-678).
+__global__ void k_aux_update(long n, double* __restrict__ x, const double* __restrict__ d,
+                             int* __restrict__ counters)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double u = d[i];
+    if (!isfinite(u)) atomicAdd(&counters[1], 1);
+    x[i] = x[i] - u;
+}
+
 // Face rates of the current solution (TpfaLinearizer::updateFlowsInfo,
 // tpfalinearizer.hh:647-705): for every cell I and every neighbour J in
 // intersection order, the values of the face's component fluxes as the
@@ -731,6 +761,20 @@ void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const
     hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, fdev, G, pv, mean, trans, rowPtr,
                        flows, flores, dif ? *dif : BoDiff{}, dsp ? *dsp : BoDisp{},
                        (geo && geo->depth) ? *geo : BoGeo{});
+}
+
+void launchAddAuxiliary(int nb, const int* blockOff, const double* blocks, int nr, const int* resRow,
+                        const double* resVals, double* jac, double* res, hipStream_t s)
+{
+    if (nb <= 0 && nr <= 0) return;
+    hipLaunchKernelGGL(k_add_auxiliary, dim3(1), dim3(64), 0, s, nb, blockOff, blocks, nr, resRow, resVals, jac,
+                       res);
+}
+
+void launchAuxUpdate(long n, double* x, const double* delta, int* counters, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_aux_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, delta, counters);
 }
 
 void launchAddSparseSources(int n, const int* cell, const double* r, const double* d, double* res,

@@ -195,9 +195,14 @@ int omodels_linearizer_pattern(omodels_model* h, int32_t* row_ptr, int32_t* col_
         const Model& m = *h->m;
         const HostTopology& T = m.topo;
         const long N = T.N;
-        if (row_ptr) std::copy(T.rowPtr.begin(), T.rowPtr.end(), row_ptr);
-        if (col_idx) std::copy(T.colIdx.begin(), T.colIdx.end(), col_idx);
-        if (diag_off) std::copy(T.diagOff.begin(), T.diagOff.end(), diag_off);
+        const bool aux = m.nAux > 0;   // the pattern with the auxiliary module's rows and entries
+        const std::vector<int>& rp = aux ? m.xRowPtr : T.rowPtr;
+        const std::vector<int>& ci = aux ? m.xColIdx : T.colIdx;
+        const std::vector<int>& dg = aux ? m.xDiagOff : T.diagOff;
+        const std::vector<int>& nbo = aux ? m.xNbOff : T.nbOff;
+        if (row_ptr) std::copy(rp.begin(), rp.end(), row_ptr);
+        if (col_idx) std::copy(ci.begin(), ci.end(), col_idx);
+        if (diag_off) std::copy(dg.begin(), dg.end(), diag_off);
         // neighbour info in intersection order (-x,+x,-y,+y,-z,+z)
         long k = 0;
         if (nb_ptr) nb_ptr[0] = 0;
@@ -205,7 +210,7 @@ int omodels_linearizer_pattern(omodels_model* h, int32_t* row_ptr, int32_t* col_
         for (long I = 0; I < N; ++I) {
             const long kk = I / nxy;
             for (int d = 0; d < 6; ++d) {
-                const int off = T.nbOff[(long)d * N + I];
+                const int off = nbo[(long)d * N + I];
                 if (off < 0) continue;
                 const long J = d == 0 ? I - 1 : d == 1 ? I + 1 : d == 2 ? I - T.nx : d == 3 ? I + T.nx
                              : d == 4 ? I - nxy : I + nxy;
@@ -257,6 +262,82 @@ int omodels_model_set_boundary_conditions(omodels_model* h, int32_t n, const int
     });
 }
 
+int omodels_model_set_auxiliary(omodels_model* h, int32_t n_dofs, int32_t n_links, const int32_t* row,
+                                const int32_t* col)
+{
+    return guarded([&] {
+        h->m->setAuxiliary(n_dofs, n_links, row, col);
+        return 0;
+    });
+}
+
+int omodels_model_set_auxiliary_solution(omodels_model* h, const double* x, int both_levels)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (m.nAux == 0) throw std::invalid_argument("no auxiliary DOFs");
+        if (!x) throw std::invalid_argument("auxiliary solution must not be NULL");
+        m.aux0.upload(x, 3 * (size_t)m.nAux, m.stream);
+        if (both_levels) m.aux1.upload(x, 3 * (size_t)m.nAux, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_model_get_auxiliary_solution(omodels_model* h, double* x)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (m.nAux == 0) throw std::invalid_argument("no auxiliary DOFs");
+        m.aux0.download(x, 3 * (size_t)m.nAux, m.stream);
+        m.sync();
+        return 0;
+    });
+}
+
+int omodels_model_set_auxiliary_module(omodels_model* h, omodels_aux_linearize_fn fn, void* user)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        m.auxFn = reinterpret_cast<int (*)(void*, void*, int)>(fn);
+        m.auxUser = user;
+        m.auxHandle = h;
+        return 0;
+    });
+}
+
+int omodels_linearizer_add_auxiliary(omodels_model* h, int32_t n_blocks, const int32_t* block_row,
+                                     const int32_t* block_col, const double* blocks, int32_t n_res,
+                                     const int32_t* res_row, const double* res)
+{
+    return guarded([&] {
+        Model& m = *h->m;
+        if (n_blocks < 0 || n_res < 0) throw std::invalid_argument("auxiliary: counts must be >= 0");
+        if ((n_blocks > 0 && (!block_row || !block_col || !blocks)) || (n_res > 0 && (!res_row || !res)))
+            throw std::invalid_argument("auxiliary: arrays must not be NULL");
+        std::vector<int> off(n_blocks), rr(n_res);
+        for (int k = 0; k < n_blocks; ++k) {
+            const long o = m.blockOffset(block_row[k], block_col[k]);
+            if (o < 0) throw std::invalid_argument("auxiliary: a block outside the pattern");
+            off[k] = (int)o;
+        }
+        for (int k = 0; k < n_res; ++k) {
+            if (res_row[k] < 0 || res_row[k] >= m.rows()) throw std::invalid_argument("auxiliary: row out of range");
+            rr[k] = res_row[k];
+        }
+        if (n_blocks == 0 && n_res == 0) return 0;
+        DevBuf<int> doff, drr;
+        DevBuf<double> db, dr;
+        doff.upload(off.data(), off.size(), m.stream);
+        db.upload(blocks, 9 * (size_t)n_blocks, m.stream);
+        drr.upload(rr.data(), rr.size(), m.stream);
+        dr.upload(res, 3 * (size_t)n_res, m.stream);
+        m.addAuxiliary(n_blocks, doff.p, db.p, n_res, drr.p, dr.p);
+        m.sync();
+        return 0;
+    });
+}
+
 int omodels_model_set_directional_mobility(omodels_model* h, const double* multipliers)
 {
     return guarded([&] {
@@ -289,7 +370,7 @@ int omodels_linearizer_get_residual(omodels_model* h, double* r)
 {
     return guarded([&] {
         Model& m = *h->m;
-        m.res.download(r, 3 * (size_t)m.N, m.stream);
+        m.res.download(r, 3 * (size_t)m.rows(), m.stream);
         m.sync();
         return 0;
     });
@@ -323,7 +404,7 @@ int omodels_linearizer_flows(omodels_model* h, double* flows, double* flores)
 {
     return guarded([&] {
         Model& m = *h->m;
-        const size_t nf = (size_t)(m.nnzb - m.N);
+        const size_t nf = (size_t)(m.topo.rowPtr[m.N] - m.N);
         if (nf == 0 || (!flows && !flores)) return 0;
         DevBuf<double> df, dr;
         if (flows) df.alloc(3 * nf);
@@ -520,7 +601,7 @@ int omodels_solver_get_residual(omodels_solver* s, double* residual)
 {
     return guarded([&] {
         Model& m = *s->owner->m;
-        OM_HIP(hipMemcpyAsync(residual, m.res.p, 3 * m.N * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+        OM_HIP(hipMemcpyAsync(residual, m.res.p, 3 * m.rows() * sizeof(double), hipMemcpyDeviceToHost, m.stream));
         m.sync();
         return 0;
     });
@@ -532,7 +613,8 @@ int omodels_solver_solve(omodels_solver* s, double* x)
         const int rc = s->s->solve();
         Model& m = *s->owner->m;
         if (x) {
-            OM_HIP(hipMemcpyAsync(x, s->s->solution(), 3 * m.N * sizeof(double), hipMemcpyDeviceToHost, m.stream));
+            OM_HIP(hipMemcpyAsync(x, s->s->solution(), 3 * m.rows() * sizeof(double), hipMemcpyDeviceToHost,
+                                  m.stream));
             m.sync();
         }
         return rc;
@@ -608,7 +690,7 @@ int omodels_newton_update(omodels_model* h, const omodels_params* prm, const dou
     return guarded([&] {
         Model& m = *h->m;
         DevBuf<double> d;
-        d.upload(delta, 3 * m.N, m.stream);
+        d.upload(delta, 3 * m.rows(), m.stream);
         BlackOilNewtonMethod nm(m, *prm);
         int nsw = 0;
         const int rc = nm.update(d.p, &nsw);

@@ -65,6 +65,7 @@ Model::Model(const omodels_blackoil_problem* p, hipStream_t s)
     fluidDev.upload(&fluid, 1, stream);
     poro.upload(p->poro, N, stream);
     rowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
+    gridRowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
     colIdx.upload(topo.colIdx.data(), topo.colIdx.size(), stream);
     diagOff.upload(topo.diagOff.data(), topo.diagOff.size(), stream);
     nbOff.upload(topo.nbOff.data(), topo.nbOff.size(), stream);
@@ -127,6 +128,12 @@ void Model::ensureJacobian()
 void Model::linearizeDomain(double dt, int iterationIndex)
 {
     ensureJacobian();
+    // resetSystem_: with auxiliary rows the blocks the cell loop does not
+    // write (the auxiliary entries) start from zero
+    if (nAux > 0) {
+        OM_HIP(hipMemsetAsync(jac.p, 0, 9 * (size_t)nnzb * sizeof(double), stream));
+        OM_HIP(hipMemsetAsync(res.p, 0, 3 * (size_t)rows() * sizeof(double), stream));
+    }
     if (iterationIndex == 0)
         launchBoStorage(fluid, N, pv0.p, mean0.p, poro.p, st1.p, stream, &geo.rock);
     const double storefac = G.V / dt;
@@ -262,7 +269,7 @@ void Model::addSparseSources(int n, const int* cells, const double* r, const dou
 void Model::computeFlows(double* flows, double* flores)
 {
     timer.begin(kClsOther, stream);
-    launchBoFlows(fluidDev.p, G, pv0.p, mean0.p, trans.p, rowPtr.p, flows, flores, stream, &diffusion, &dispersion,
+    launchBoFlows(fluidDev.p, G, pv0.p, mean0.p, trans.p, gridRowPtr.p, flows, flores, stream, &diffusion, &dispersion,
                   &geo);
     timer.end(stream);
     OM_HIP(hipGetLastError());
@@ -483,18 +490,144 @@ void Model::advanceTimeLevel()
 {
     OM_HIP(hipMemcpyAsync(pv1.p, pv0.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, stream));
     OM_HIP(hipMemcpyAsync(mean1.p, mean0.p, N, hipMemcpyDeviceToDevice, stream));
+    if (nAux > 0)
+        OM_HIP(hipMemcpyAsync(aux1.p, aux0.p, 3 * nAux * sizeof(double), hipMemcpyDeviceToDevice, stream));
 }
 
 void Model::restoreTimeLevel()
 {
     OM_HIP(hipMemcpyAsync(pv0.p, pv1.p, 3 * N * sizeof(double), hipMemcpyDeviceToDevice, stream));
     OM_HIP(hipMemcpyAsync(mean0.p, mean1.p, N, hipMemcpyDeviceToDevice, stream));
+    if (nAux > 0)
+        OM_HIP(hipMemcpyAsync(aux0.p, aux1.p, 3 * nAux * sizeof(double), hipMemcpyDeviceToDevice, stream));
+}
+
+void buildLevelSchedule(long R, const std::vector<int>& rp, const std::vector<int>& ci, LevelSchedule& L,
+                        hipStream_t s)
+{
+    // lower-numbered rows coupled through an entry above the diagonal
+    std::vector<std::vector<int>> upFrom(R);
+    for (long r = 0; r < R; ++r)
+        for (int k = rp[r]; k < rp[r + 1]; ++k)
+            if (ci[k] > r) upFrom[ci[k]].push_back((int)r);
+    std::vector<int> lvl(R, 0);
+    int nL = 0;
+    for (long i = 0; i < R; ++i) {
+        int l = 0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] < i) l = std::max(l, lvl[ci[k]] + 1);
+        for (int j : upFrom[i]) l = std::max(l, lvl[j] + 1);
+        lvl[i] = l;
+        nL = std::max(nL, l + 1);
+    }
+    L.nLevels = nL;
+    L.ptrHost.assign(nL + 1, 0);
+    for (long i = 0; i < R; ++i) L.ptrHost[lvl[i] + 1]++;
+    for (int l = 0; l < nL; ++l) L.ptrHost[l + 1] += L.ptrHost[l];
+    std::vector<int> order(R), fill(L.ptrHost.begin(), L.ptrHost.end() - 1);
+    for (long i = 0; i < R; ++i) order[fill[lvl[i]]++] = (int)i;
+    L.rows.upload(order.data(), order.size(), s);
+}
+
+long Model::blockOffset(long row, long col) const
+{
+    const std::vector<int>& rp = nAux > 0 ? xRowPtr : topo.rowPtr;
+    const std::vector<int>& ci = nAux > 0 ? xColIdx : topo.colIdx;
+    if (row < 0 || row >= rows() || col < 0 || col >= rows()) return -1;
+    const int* b = ci.data() + rp[row];
+    const int* e = ci.data() + rp[row + 1];
+    const int* it = std::lower_bound(b, e, (int)col);
+    return (it == e || *it != col) ? -1 : (long)(it - ci.data());
+}
+
+// createMatrix_ with the auxiliary module's addNeighbors (tpfalinearizer.hh:
+// 503-507): the module's DOFs follow the cells; every row's columns are the
+// sorted union of its grid neighbours and the module's entries.
+void Model::setAuxiliary(int n, int nLinks, const int* row, const int* col)
+{
+    if (n < 0 || nLinks < 0) throw std::invalid_argument("auxiliary: counts must be >= 0");
+    if (n > 0 && (nLinks == 0 || !row || !col)) throw std::invalid_argument("auxiliary: the module's pattern entries");
+    const long R = N + n;
+    std::vector<int> rp(R + 1, 0), ci, dg(R), nb(6 * (size_t)N, -1);
+    if (n > 0) {
+        std::vector<std::vector<int>> extra(R);
+        for (int k = 0; k < nLinks; ++k) {
+            if (row[k] < 0 || row[k] >= R || col[k] < 0 || col[k] >= R)
+                throw std::invalid_argument("auxiliary: entry out of range");
+            extra[row[k]].push_back(col[k]);
+        }
+        std::vector<int> cols;
+        for (long r = 0; r < R; ++r) {
+            cols.clear();
+            if (r < N) cols.assign(topo.colIdx.begin() + topo.rowPtr[r], topo.colIdx.begin() + topo.rowPtr[r + 1]);
+            cols.insert(cols.end(), extra[r].begin(), extra[r].end());
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            if (r >= N && !std::binary_search(cols.begin(), cols.end(), (int)r))
+                throw std::invalid_argument("auxiliary: a row without its diagonal block");
+            ci.insert(ci.end(), cols.begin(), cols.end());
+            rp[r + 1] = (int)ci.size();
+        }
+        if (ci.size() >= (size_t)INT32_MAX / 9) throw std::invalid_argument("auxiliary: pattern too large");
+        auto at = [&](long r, long c) {
+            const int* it = std::lower_bound(ci.data() + rp[r], ci.data() + rp[r + 1], (int)c);
+            return (int)(it - ci.data());
+        };
+        for (long r = 0; r < R; ++r) dg[r] = at(r, r);
+        const long nxy = (long)G.nx * G.ny;
+        for (int d = 0; d < 6; ++d)
+            for (long I = 0; I < N; ++I) {
+                if (topo.nbOff[(long)d * N + I] < 0) continue;
+                const long J = d == 0 ? I - 1 : d == 1 ? I + 1 : d == 2 ? I - G.nx : d == 3 ? I + G.nx
+                             : d == 4 ? I - nxy : I + nxy;
+                nb[(long)d * N + I] = at(J, I);    // the block A[J][I]
+            }
+    }
+    sync();
+    nAux = n;
+    if (n > 0) {
+        xRowPtr = std::move(rp); xColIdx = std::move(ci); xDiagOff = std::move(dg); xNbOff = std::move(nb);
+        rowPtr.upload(xRowPtr.data(), xRowPtr.size(), stream);
+        colIdx.upload(xColIdx.data(), xColIdx.size(), stream);
+        diagOff.upload(xDiagOff.data(), xDiagOff.size(), stream);
+        nbOff.upload(xNbOff.data(), xNbOff.size(), stream);
+        nnzb = xRowPtr[R];
+        buildLevelSchedule(R, xRowPtr, xColIdx, levels, stream);
+        aux0.alloc(3 * (size_t)n); aux1.alloc(3 * (size_t)n);
+        OM_HIP(hipMemsetAsync(aux0.p, 0, 3 * (size_t)n * sizeof(double), stream));
+        OM_HIP(hipMemsetAsync(aux1.p, 0, 3 * (size_t)n * sizeof(double), stream));
+    } else {
+        xRowPtr.clear(); xColIdx.clear(); xDiagOff.clear(); xNbOff.clear();
+        rowPtr.upload(topo.rowPtr.data(), topo.rowPtr.size(), stream);
+        colIdx.upload(topo.colIdx.data(), topo.colIdx.size(), stream);
+        diagOff.upload(topo.diagOff.data(), topo.diagOff.size(), stream);
+        nbOff.upload(topo.nbOff.data(), topo.nbOff.size(), stream);
+        nnzb = topo.rowPtr[N];
+        levels.nLevels = (int)topo.levelPtr.size() - 1;
+        levels.ptrHost.assign(topo.levelPtr.begin(), topo.levelPtr.end());
+        levels.rows.upload(topo.levelRows.data(), topo.levelRows.size(), stream);
+        aux0.release(); aux1.release();
+    }
+    jac.release();
+    ensureJacobian();
+    res.alloc(3 * (size_t)rows());
+    OM_HIP(hipMemsetAsync(res.p, 0, 3 * (size_t)rows() * sizeof(double), stream));
+    OM_HIP(hipMemsetAsync(jac.p, 0, 9 * (size_t)nnzb * sizeof(double), stream));
+    sync();
+}
+
+void Model::addAuxiliary(int nb, const int* blockOff, const double* blocks, int nr, const int* resRow,
+                         const double* resVals)
+{
+    ensureJacobian();
+    launchAddAuxiliary(nb, blockOff, blocks, nr, resRow, resVals, jac.p, res.p, stream);
+    OM_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
 BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m_(m), prm_(prm)
 {
-    const long n = (long)m.bs * m.N;
+    const long n = (long)m.bs * m.rows();
     if (prm_.precond < 0 || prm_.precond > 2) throw std::invalid_argument("precond must be 0, 1 or 2");
     if (prm_.precond == 1 && m.bs != 3) throw std::invalid_argument("block Jacobi is built for 3x3 blocks");
     b64_.alloc(n); x64_.alloc(n);
@@ -506,9 +639,11 @@ BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m
     if (prm_.precision == 2 && prm_.precond != 0) throw std::invalid_argument("the FP32 solve uses block ILU(0)");
     if (prm_.ilu_schedule < 0 || prm_.ilu_schedule > 3) throw std::invalid_argument("ilu_schedule must be 0..3");
     if (prm_.ilu_order < 0 || prm_.ilu_order > 1) throw std::invalid_argument("ilu_order must be 0 or 1");
-    const bool tiles = prm_.ilu_schedule != 1;
+    // the tile and two-colour sweeps are built for grid systems; auxiliary
+    // rows take the level-scheduled sweeps
+    const bool tiles = prm_.ilu_schedule != 1 && m.nAux == 0;
     if (prm_.precond == 0 && prm_.ilu_order == 1) {
-        if (m.bs != 3 || prm_.precision == 2)
+        if (m.bs != 3 || prm_.precision == 2 || m.nAux > 0)
             throw std::invalid_argument("the two-colour ILU(0) is built for 3x3 blocks and FP64 pivots");
         colored_ = std::make_unique<ColoredIlu>(m.gx, m.gy, m.gz, m.stream);
     } else if (prm_.precond == 0 && tiles && m.bs == 3 && prm_.precision != 2) {
@@ -568,7 +703,7 @@ void BiCGStabBackend::tuneIlu()
 {
     tuned_ = true;
     hipStream_t s = m_.stream;
-    const int N = (int)m_.N;
+    const int N = (int)m_.rows();
     const long nnzb = m_.nnzb, n = 3L * N;
     DevBuf<double> lu, dinv, v64;
     DevBuf<float> luf, df, vf;
@@ -664,14 +799,14 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
 {
     hipStream_t s = m_.stream;
     const int bs = m_.bs, bb = bs * bs;
-    const long n = (long)bs * m_.N, nnzb = m_.nnzb;
+    const long n = (long)bs * m_.rows(), nnzb = m_.nnzb;
     if (hostValues) { Acopy_.upload(hostValues, bb * nnzb, s); A64_ = Acopy_.p; }
     else A64_ = m_.jac.p;
     if (hostResidual) b64_.upload(hostResidual, n, s);
     else OM_HIP(hipMemcpyAsync(b64_.p, m_.res.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (prm_.autotune && prm_.ilu_schedule == 0 && tiled_ && !tuned_) tuneIlu();
     OM_HIP(hipMemsetAsync(fail_.p, 0, sizeof(int), s));
-    const int N = (int)m_.N;
+    const int N = (int)m_.rows();
     // the inner solve's operator (and the two-colour sweeps) read the matrix in
     // the structured 7-slot layout of a 3x3 7-point grid system: whole cache
     // lines, no column indices, the same bits as the BCRS product
@@ -755,7 +890,7 @@ template <class S>
 bool BiCGStabBackend::precApply(const S* lu, const void* dinv, const S* d, S* v, const int* run, const S* A, S* Av)
 {
     hipStream_t s = m_.stream;
-    const int N = (int)m_.N;
+    const int N = (int)m_.rows();
     bool fused = false;
     if (amg_) {
         m_.timer.begin(kClsIluApply, s);
@@ -806,8 +941,8 @@ int BiCGStabBackend::bicgstab(const S* A, const S* lu, const void* dinv, const S
                               double absTol, int maxIter, int* its)
 {
     hipStream_t s = m_.stream;
-    const long m = (long)m_.bs * m_.N;
-    const int N = (int)m_.N;
+    const long m = (long)m_.bs * m_.rows();
+    const int N = (int)m_.rows();
     S* r; S* p; S* v; S* y; S* z;
     if constexpr (sizeof(S) == 8) { r = (S*)wd_[0].p; p = (S*)wd_[1].p; v = (S*)wd_[2].p; y = (S*)wd_[3].p; z = (S*)wd_[4].p; }
     else { r = (S*)wf_[0].p; p = (S*)wf_[1].p; v = (S*)wf_[2].p; y = (S*)wf_[3].p; z = (S*)wf_[4].p; }
@@ -944,12 +1079,12 @@ template <class S>
 void BiCGStabBackend::spmvInner(const S* A, const S* x, S* y, const int* run)
 {
     if (diaOf<S>()) launchSpmvDia<S>(m_.gx, m_.gy, m_.gz, diaOf<S>(), x, y, m_.stream, run);
-    else launchSpmv<S>((int)m_.N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A, x, y, m_.stream, run);
+    else launchSpmv<S>((int)m_.rows(), m_.bs, m_.rowPtr.p, m_.colIdx.p, A, x, y, m_.stream, run);
 }
 
 bool BiCGStabBackend::structured() const
 {
-    return m_.bs == 3 && m_.gx > 0 && (long)m_.gx * m_.gy * m_.gz == m_.N && m_.nnzb <= 7L * m_.N;
+    return m_.bs == 3 && m_.nAux == 0 && m_.gx > 0 && (long)m_.gx * m_.gy * m_.gz == m_.N && m_.nnzb <= 7L * m_.N;
 }
 
 int BiCGStabBackend::solve()
@@ -963,7 +1098,7 @@ int BiCGStabBackend::solveImpl()
 {
     if (!prepared_) return 2;   // preconditioner could not be created (NumericalProblem)
     hipStream_t s = m_.stream;
-    const long m = (long)m_.bs * m_.N;
+    const long m = (long)m_.bs * m_.rows();
     const double absTol = prm_.lin_abs_tol < 0 ? prm_.newton_tol / 100.0 : prm_.lin_abs_tol;
     iters_ = 0;
     if (prm_.precision == 0) {
@@ -1006,7 +1141,7 @@ int BiCGStabBackend::solveImpl()
         total += its;
         if (rc == 2) { iters_ = total; return 2; }
         launchAddFromFloat(m, xf_.p, x64_.p, s);
-        launchResidualUpdate((int)m_.N, m_.bs, m_.rowPtr.p, m_.colIdx.p, A64_, x64_.p, b64_.p, r64_.p, s);
+        launchResidualUpdate((int)m_.rows(), m_.bs, m_.rowPtr.p, m_.colIdx.p, A64_, x64_.p, b64_.p, r64_.p, s);
         launchMaxAbs<double>(m, r64_.p, m_.partial.p, m_.scal.p, s);
         OM_HIP(hipMemcpyAsync(m_.hostScal, m_.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
         m_.sync();
@@ -1042,7 +1177,7 @@ void BiCGStabBackend::downloadAmgLevel(int level, double* hostValues)
 
 void BiCGStabBackend::applyPrecond(const double* hostD, double* hostV)
 {
-    const long m = (long)m_.bs * m_.N;
+    const long m = (long)m_.bs * m_.rows();
     DevBuf<double> d, v;
     d.upload(hostD, m, m_.stream);
     v.alloc(m);
@@ -1065,6 +1200,8 @@ int BlackOilNewtonMethod::update(const double* delta, int* numSwitched)
     NewtonUpdateParams up{prm_.dp_max_rel, prm_.ds_max, prm_.privar_osc_thresh};
     OM_HIP(hipMemsetAsync(m_.counters.p, 0, 4 * sizeof(int), s));
     launchBoUpdate(m_.fluid, m_.N, m_.pv0.p, m_.mean0.p, m_.wasSw.p, delta, up, m_.counters.p, s);
+    // the auxiliary DOFs (newtonmethod.hh:704-708)
+    launchAuxUpdate(3 * m_.nAux, m_.aux0.p, delta + 3 * m_.N, m_.counters.p, s);
     int c[4];
     OM_HIP(hipMemcpyAsync(c, m_.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     m_.sync();
@@ -1100,6 +1237,15 @@ bool BlackOilNewtonMethod::apply(double dt, int step, int attempt, std::vector<o
     while (proceed()) {
         auto t0 = clk::now();
         m_.linearizeDomain(dt, numIter_);
+        // linearizeAuxiliaryEquations_ (newtonmethod.hh:287, :566-570): a
+        // module that fails makes the attempt fail (NumericalProblem)
+        if (m_.nAux > 0) {
+            if (!m_.auxFn) throw std::invalid_argument("auxiliary DOFs without a module to linearize them");
+            if (m_.auxFn(m_.auxUser, m_.auxHandle, numIter_) != 0) {
+                numIter_ = 2 * prm_.newton_target_iter;
+                return false;
+            }
+        }
         lastError_ = error_;
         error_ = m_.maxAbsResidual();           // preSolve_ (:572-607)
         auto t1 = clk::now();

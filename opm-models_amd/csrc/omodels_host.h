@@ -67,6 +67,8 @@ void sourceGroups(const int32_t* cell, int n, std::vector<int2>& groups, std::ve
 // schedule of the pattern and the stream everything runs on.
 struct LinearSystem {
     long N = 0, nnzb = 0;
+    long nAux = 0;               // auxiliary degrees of freedom after the N grid rows
+    long rows() const { return N + nAux; }
     int bs = 3;                  // block size (3 black-oil, 2 two-phase)
     int gx = 0, gy = 0, gz = 0;  // structured grid sizes (tile-wavefront ILU, bs 3)
     hipStream_t stream = nullptr;
@@ -173,6 +175,24 @@ public:
     DevBuf<double> bcRate, bcState, bcTrans, bcZ;
     DevBuf<uint8_t> bcMean;
     std::vector<double> permHost;
+    // auxiliary equations (BaseAuxiliaryModule): n degrees of freedom after
+    // the cells and the pattern entries (row[k], col[k]) their module adds
+    // (addNeighbors); n = 0 returns to the grid's pattern
+    void setAuxiliary(int n, int nLinks, const int* row, const int* col);
+    // the module's linearize(matrix, residual): blocks (device offsets into
+    // the Jacobian, [n][9]) and residual rows ([n][3]) added in order
+    void addAuxiliary(int nb, const int* blockOff, const double* blocks, int nr, const int* resRow,
+                      const double* resVals);
+    // block offset of (row, col) in the current pattern, -1 if absent
+    long blockOffset(long row, long col) const;
+    std::vector<int> xRowPtr, xColIdx, xDiagOff, xNbOff;   // the pattern with the auxiliary entries
+    DevBuf<int> gridRowPtr;      // the grid's own row pointers (FLOWS' neighbour order)
+    DevBuf<double> aux0, aux1;   // the auxiliary DOFs' solution(0) and solution(1), [nAux][3]
+    // linearizeAuxiliaryEquations of the Newton loop: called after each
+    // linearizeDomain (nonzero return: the linearization failed)
+    int (*auxFn)(void* user, void* handle, int iteration) = nullptr;
+    void* auxUser = nullptr;
+    void* auxHandle = nullptr;
     // the kernels' BoGeo from the given geometry and the rock terms
     void refreshGeo();
     void ensureJacobian();

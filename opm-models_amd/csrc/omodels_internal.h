@@ -237,6 +237,16 @@ struct LevelSchedule {
     std::vector<int> ptrHost;     // [nLevels+1]
     DevBuf<int> rows;             // rows ordered by level
 };
+// the levels of a block pattern's rows for the triangular sweeps: a row's
+// level is one more than the largest level of the lower-numbered rows it is
+// coupled with in either direction, so the forward sweep (levels ascending)
+// and the backward sweep (descending) find every row they read done also
+// when the pattern is not symmetric
+void buildLevelSchedule(long rows, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
+                        LevelSchedule& L, hipStream_t s);
+void launchAddAuxiliary(int nb, const int* blockOff, const double* blocks, int nr, const int* resRow,
+                        const double* resVals, double* jac, double* res, hipStream_t s);
+void launchAuxUpdate(long n, double* x, const double* delta, int* counters, hipStream_t s);
 template <class S> int launchIlu0Factor(int n, int bs, const int* rp, const int* ci, const int* diag,
                                         S* lu, const LevelSchedule& L, int* dFail, hipStream_t st);
 template <class S, class P> void launchIlu0Apply(int n, int bs, const int* rp, const int* ci,

@@ -95,6 +95,8 @@ class LensProblem(C.Structure):
     ]
 
 
+AUX_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int)
+
 # every symbol declared in include/omodels.h, with its ctypes signature
 SIGNATURES = {
     "omodels_blackoil_default_fluid": (C.c_int, [_dp, _ip, _dp, _ip, _dp]),
@@ -146,6 +148,11 @@ SIGNATURES = {
     "omodels_scalar_product": (C.c_int, [C.c_int64, _dp, _dp, _dp]),
     "omodels_scalar_product_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "omodels_solver_apply_precond": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "omodels_model_set_auxiliary": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _ip]),
+    "omodels_model_set_auxiliary_solution": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "omodels_model_get_auxiliary_solution": (C.c_int, [C.c_void_p, _dp]),
+    "omodels_linearizer_add_auxiliary": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _dp]),
+    "omodels_model_set_auxiliary_module": (C.c_int, [C.c_void_p, AUX_FN, C.c_void_p]),
     "omodels_model_set_boundary_conditions": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip, _dp, _dp, _u8p, _dp, _dp]),
     "omodels_model_set_directional_mobility": (C.c_int, [C.c_void_p, _dp]),
     "omodels_solver_amg_levels": (C.c_int, [C.c_void_p, _ip, C.c_int]),
@@ -367,6 +374,14 @@ class Model:
         z = C.c_int64()
         lib().omodels_model_num_cells(self.h, C.byref(n), C.byref(z))
         self.N, self.nnzb = n.value, z.value
+        self.grid_nnzb = self.nnzb
+        self.n_aux = 0
+        self._aux_fn = None
+
+    @property
+    def rows(self):
+        """rows of the system: the cells and the auxiliary DOFs"""
+        return self.N + self.n_aux
 
     def close(self):
         if self.h:
@@ -399,11 +414,11 @@ class Model:
         return st
 
     def pattern(self):
-        N, z = self.N, self.nnzb
-        nf = z - N
-        rp = np.zeros(N + 1, np.int32)
+        N, z, R = self.N, self.nnzb, self.rows
+        nf = self.grid_nnzb - N
+        rp = np.zeros(R + 1, np.int32)
         ci = np.zeros(z, np.int32)
-        dg = np.zeros(N, np.int32)
+        dg = np.zeros(R, np.int32)
         nbp = np.zeros(N + 1, np.int32)
         nbc = np.zeros(nf, np.int32)
         nbo = np.zeros(nf, np.int32)
@@ -469,6 +484,52 @@ class Model:
             ptr(keep["state"]), ptr(keep["meaning"], _u8p), ptr(keep["trans"]), ptr(keep["zface"])), "set_boundary")
         self._bc = keep
 
+    def set_auxiliary(self, n_dofs=0, rows=None, cols=None):
+        """auxiliary DOFs after the cells and the pattern entries (rows[k],
+        cols[k]) their module adds; 0 for none"""
+        r = np.ascontiguousarray([] if rows is None else rows, dtype=np.int32).ravel()
+        c = np.ascontiguousarray([] if cols is None else cols, dtype=np.int32).ravel()
+        check(lib().omodels_model_set_auxiliary(self.h, int(n_dofs), r.size, ptr(r, _ip), ptr(c, _ip)), "set_auxiliary")
+        n = C.c_int64()
+        z = C.c_int64()
+        lib().omodels_model_num_cells(self.h, C.byref(n), C.byref(z))
+        self.nnzb = z.value
+        self.n_aux = int(n_dofs)
+
+    def set_auxiliary_solution(self, x, both_levels=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(lib().omodels_model_set_auxiliary_solution(self.h, ptr(x), int(both_levels)), "set_auxiliary_solution")
+
+    def get_auxiliary_solution(self):
+        x = np.zeros(3 * self.n_aux)
+        check(lib().omodels_model_get_auxiliary_solution(self.h, ptr(x)), "get_auxiliary_solution")
+        return x
+
+    def add_auxiliary(self, brow=(), bcol=(), blocks=None, rrow=(), rvals=None):
+        br = np.ascontiguousarray(brow, dtype=np.int32).ravel()
+        bc = np.ascontiguousarray(bcol, dtype=np.int32).ravel()
+        bl = np.ascontiguousarray(np.zeros(0) if blocks is None else blocks, dtype=np.float64).ravel()
+        rr = np.ascontiguousarray(rrow, dtype=np.int32).ravel()
+        rv = np.ascontiguousarray(np.zeros(0) if rvals is None else rvals, dtype=np.float64).ravel()
+        check(lib().omodels_linearizer_add_auxiliary(self.h, br.size, ptr(br, _ip), ptr(bc, _ip), ptr(bl), rr.size,
+                                                     ptr(rr, _ip), ptr(rv)), "add_auxiliary")
+
+    def set_auxiliary_module(self, fn=None):
+        """fn(model, newton_iteration) -> 0 adds the module's terms after each
+        linearization of simulate (nonzero: the linearization failed)"""
+        if fn is None:
+            self._aux_fn = None
+            check(lib().omodels_model_set_auxiliary_module(self.h, None, None), "set_auxiliary_module")
+            return
+
+        def tramp(user, handle, it):
+            try:
+                return int(fn(self, it))
+            except Exception:
+                return 1
+        self._aux_fn = AUX_FN(tramp)
+        check(lib().omodels_model_set_auxiliary_module(self.h, self._aux_fn, None), "set_auxiliary_module")
+
     def set_directional_mobility(self, mult=None):
         """directional relative permeability multipliers [3][3][N] (axis x, y,
         z; phase water, oil, gas; cell); None switches them off"""
@@ -481,7 +542,7 @@ class Model:
     def linearize(self, dt, iteration_index=1):
         check(lib().omodels_linearizer_linearize_domain(self.h, dt, iteration_index), "linearize_domain")
         J = np.zeros(9 * self.nnzb)
-        r = np.zeros(3 * self.N)
+        r = np.zeros(3 * self.rows)
         check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
         check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
         return J, r
@@ -489,7 +550,7 @@ class Model:
     def system(self):
         """The linearizer's current Jacobian values and residual (no relinearization)."""
         J = np.zeros(9 * self.nnzb)
-        r = np.zeros(3 * self.N)
+        r = np.zeros(3 * self.rows)
         check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
         check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
         return J, r
@@ -536,7 +597,7 @@ class Model:
 
     def flows(self):
         """(flows, flores), each [nnzb - N][3] in the pattern's neighbour order, at solution(0)."""
-        nf = self.nnzb - self.N
+        nf = self.grid_nnzb - self.N
         fl = np.zeros((nf, 3))
         fr = np.zeros((nf, 3))
         check(lib().omodels_linearizer_flows(self.h, ptr(fl), ptr(fr)), "flows")
@@ -640,7 +701,7 @@ class Solver:
         check(lib().omodels_solver_prepare(self.h, ptr(values), ptr(residual)), "solver_prepare")
 
     def solve(self):
-        x = np.zeros(3 * self.model.N)
+        x = np.zeros(3 * self.model.rows)
         rc = check(lib().omodels_solver_solve(self.h, ptr(x)), "solver_solve")
         return rc, x, lib().omodels_solver_iterations(self.h)
 

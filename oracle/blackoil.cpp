@@ -286,7 +286,15 @@ struct Pattern {
 void buildPattern(const Grid& G, Pattern& P)
 {
     const long N = G.N;
-    std::vector<std::set<long>> sp(N);
+    /* the auxiliary module's degrees of freedom follow the cells
+     * (numTotalDof = numGridDof + the modules' numDofs, setDofOffset) */
+    const ob_problem* pp = G.P;
+    const long nAux = pp->n_aux;
+    if (nAux < 0 || pp->n_aux_links < 0) throw std::invalid_argument("auxiliary: counts must be >= 0");
+    if (nAux > 0 && (pp->n_aux_links == 0 || !pp->aux_row || !pp->aux_col))
+        throw std::invalid_argument("auxiliary: the module's pattern entries");
+    const long R = N + nAux;
+    std::vector<std::set<long>> sp(R);
     P.nbPtr.assign(N + 1, 0);
     P.nb.clear();
     for (long I = 0; I < N; ++I) {
@@ -303,10 +311,18 @@ void buildPattern(const Grid& G, Pattern& P)
         }
         P.nbPtr[I + 1] = (int32_t)P.nb.size();
     }
-    P.rowPtr.assign(N + 1, 0);
-    for (long I = 0; I < N; ++I) P.rowPtr[I + 1] = P.rowPtr[I] + (int32_t)sp[I].size();
-    P.colIdx.resize(P.rowPtr[N]);
-    for (long I = 0; I < N; ++I) {
+    /* addNeighbors of the auxiliary module (tpfalinearizer.hh:503-507) */
+    for (long k = 0; nAux > 0 && k < pp->n_aux_links; ++k) {
+        const long r = pp->aux_row[k], c = pp->aux_col[k];
+        if (r < 0 || r >= R || c < 0 || c >= R) throw std::invalid_argument("auxiliary: entry out of range");
+        sp[r].insert(c);
+    }
+    for (long r = N; r < R; ++r)
+        if (!sp[r].count(r)) throw std::invalid_argument("auxiliary: a row without its diagonal block");
+    P.rowPtr.assign(R + 1, 0);
+    for (long I = 0; I < R; ++I) P.rowPtr[I + 1] = P.rowPtr[I] + (int32_t)sp[I].size();
+    P.colIdx.resize(P.rowPtr[R]);
+    for (long I = 0; I < R; ++I) {
         int k = P.rowPtr[I];
         for (long c : sp[I]) P.colIdx[k++] = (int32_t)c;
     }
@@ -317,11 +333,12 @@ void buildPattern(const Grid& G, Pattern& P)
         if (it == e || *it != col) throw std::logic_error("missing block");
         return (int32_t)(it - P.colIdx.data());
     };
-    P.diagOff.resize(N);
+    P.diagOff.resize(R);
     for (long I = 0; I < N; ++I) {
         P.diagOff[I] = blockAddress(I, I);
         for (int k = P.nbPtr[I]; k < P.nbPtr[I + 1]; ++k) P.nb[k].off = blockAddress(P.nb[k].J, I);
     }
+    for (long r = N; r < R; ++r) P.diagOff[r] = blockAddress(r, r);
 }
 
 /* BlackOilLocalResidualTPFA::computeStorage (blackoillocalresidualtpfa.hh:151-227)
@@ -733,8 +750,10 @@ void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const d
                double* nvOut = nullptr)
 {
     const long N = pr.G.N;
-    const long nnzb = pr.pat.rowPtr[N];
-    std::fill(res, res + 3 * N, 0.0);
+    /* resetSystem_ zeroes every row, the auxiliary ones included */
+    const long R = (long)pr.pat.rowPtr.size() - 1;
+    const long nnzb = pr.pat.rowPtr[R];
+    std::fill(res, res + 3 * R, 0.0);
     std::fill(jac, jac + 9 * nnzb, 0.0);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -1744,6 +1763,39 @@ int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2)
         for (long I = 0; I < G.N; ++I)
             for (int d = 0; d < 6; ++d) if (G.nb(I, d) >= 0) ++f2;
         *nfaces2 = f2; *nnzb = G.N + f2;
+        if (p->n_aux > 0) {
+            Pattern P;
+            buildPattern(G, P);
+            *nnzb = P.rowPtr.back();
+        }
+        return 0;
+    } catch (...) { return -1; }
+}
+
+int ob_add_auxiliary(const ob_problem* p, int32_t nb, const int32_t* brow, const int32_t* bcol,
+                     const double* blocks, int32_t nr, const int32_t* rrow, const double* rvals, double* jac,
+                     double* residual)
+{
+    try {
+        Grid G(p);
+        Pattern P;
+        buildPattern(G, P);
+        const long R = (long)P.rowPtr.size() - 1;
+        for (int32_t k = 0; k < nb; ++k) {
+            const long r = brow[k], c = bcol[k];
+            if (r < 0 || r >= R) return -1;
+            const int32_t* b = P.colIdx.data() + P.rowPtr[r];
+            const int32_t* e = P.colIdx.data() + P.rowPtr[r + 1];
+            const int32_t* it = std::lower_bound(b, e, (int32_t)c);
+            if (it == e || *it != c) return -1;
+            double* B = jac + 9L * (it - P.colIdx.data());
+            for (int q = 0; q < 9; ++q) B[q] += blocks[9L * k + q];
+        }
+        for (int32_t k = 0; k < nr; ++k) {
+            const long r = rrow[k];
+            if (r < 0 || r >= R) return -1;
+            for (int e = 0; e < 3; ++e) residual[3 * r + e] += rvals[3L * k + e];
+        }
         return 0;
     } catch (...) { return -1; }
 }
@@ -1891,8 +1943,8 @@ int ob_linearize_timed(const ob_problem* p, const double* pv, const uint8_t* mea
 {
     try {
         Problem pr(p);
-        const long N = pr.G.N, nnzb = pr.pat.rowPtr[N];
-        std::vector<double> r(3 * N), J(9 * nnzb);
+        const long R = (long)pr.pat.rowPtr.size() - 1, nnzb = pr.pat.rowPtr[R];
+        std::vector<double> r(3 * R), J(9 * nnzb);
         for (int k = 0; k < reps; ++k) {
             const auto t0 = std::chrono::steady_clock::now();
             linearize(pr, pv, meaning, storage1, dt, r.data(), J.data(), nthreads);
@@ -2065,21 +2117,28 @@ int ob_simulate(const ob_problem* p, double* pv, uint8_t* meaning, int n_steps, 
                            log_len, timers, nthreads);
 }
 
-int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, double* pv,
-                    uint8_t* meaning, int n_steps, double dt0, const ob_params* prm, double* pv_hist,
-                    double* dt_hist, ob_iter_log* log, int log_cap, int32_t* log_len, double* timers, int nthreads)
+/* NewtonMethod::apply inside FvBaseProblem::timeIntegration; with auxiliary
+ * DOFs (aux: their solution(0), [n_aux][3]) the module's terms come from fn
+ * after every linearization (linearizeAuxiliaryEquations_, newtonmethod.hh:
+ * 287) */
+static int simulateImpl(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, double* pv,
+                 uint8_t* meaning, double* aux, ob_aux_fn fn, void* user, int n_steps, double dt0,
+                 const ob_params* prm, double* pv_hist, double* dt_hist, ob_iter_log* log, int log_cap,
+                 int32_t* log_len, double* timers, int nthreads)
 {
     using clk = std::chrono::steady_clock;
     try {
+        if (p->n_aux > 0 && (!fn || !aux)) throw std::invalid_argument("auxiliary equations need their module");
         Problem pr(p, dif, dsp);
         /* the cells' oil velocities of the last linearization, zeros before the first */
         std::vector<double> nvPrev(dsp ? pr.G.N : 0, 0.0), nvNext(dsp ? pr.G.N : 0, 0.0);
         const long N = pr.G.N;
+        const long nAux = p->n_aux > 0 ? p->n_aux : 0, R = N + nAux;
         const int32_t* rp = pr.pat.rowPtr.data();
         const int32_t* ci = pr.pat.colIdx.data();
-        const long nnzb = rp[N];
-        std::vector<double> sol1(pv, pv + 3 * N), st1(3 * N), res(3 * N), jac(9 * nnzb), upd(3 * N),
-            cur(3 * N);
+        const long nnzb = rp[R];
+        std::vector<double> sol1(pv, pv + 3 * N), st1(3 * N), res(3 * R), jac(9 * nnzb), upd(3 * R),
+            cur(3 * N), aux1(aux ? aux : pv, aux ? aux + 3 * nAux : pv);
         std::vector<uint8_t> mean1(meaning, meaning + N), curMean(N), wasSw(N, 0);
         double tLin = 0, tSol = 0, tUpd = 0;
         auto t0 = clk::now();
@@ -2108,6 +2167,10 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
                     linearize(pr, cur.data(), curMean.data(), st1.data(), dt, res.data(), jac.data(), nthreads,
                               nvPrev.data(), nvNext.data());
                     nvPrev.swap(nvNext);
+                    if (nAux > 0 && fn(user, cur.data(), curMean.data(), aux, jac.data(), res.data(), numIter) != 0) {
+                        failed = true;
+                        break;
+                    }
                     auto b = clk::now();
                     tLin += std::chrono::duration<double>(b - a).count();
                     /* preSolve_ */
@@ -2124,8 +2187,8 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
                     int it = 0;
                     int rc;
                     const int dims[3] = {p->nx, p->ny, p->nz};
-                    if (prm->solver_float) rc = backendSolve<float>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it, dims);
-                    else rc = backendSolve<double>((int)N, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it, dims);
+                    if (prm->solver_float) rc = backendSolve<float>((int)R, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it, dims);
+                    else rc = backendSolve<double>((int)R, 3, rp, ci, jac.data(), res.data(), upd.data(), prm, &it, dims);
                     auto d = clk::now();
                     tSol += std::chrono::duration<double>(d - c).count();
                     if (rc != 0) {
@@ -2135,7 +2198,7 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
                     }
                     /* update_ */
                     bool finite = true;
-                    for (long k = 0; k < 3 * N; ++k) if (!std::isfinite(upd[k])) { finite = false; break; }
+                    for (long k = 0; k < 3 * R; ++k) if (!std::isfinite(upd[k])) { finite = false; break; }
                     if (!finite) { failed = true; break; }
                     int nsw = 0;
                     for (long I = 0; I < N; ++I) {
@@ -2143,6 +2206,8 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
                                       pv + 3 * I, meaning[I], wasSw[I], prm);
                         nsw += wasSw[I];
                     }
+                    /* the auxiliary DOFs: next = current - update (newtonmethod.hh:704-708) */
+                    for (long k = 0; k < 3 * nAux; ++k) aux[k] = aux[k] - upd[3 * N + k];
                     tUpd += std::chrono::duration<double>(clk::now() - d).count();
                     if (nlog < log_cap && log) log[nlog++] = {step, attempt, numIter, it, nsw, 1, error, dt};
                     if (prm->verbose)
@@ -2155,6 +2220,7 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
                 numIter = 2 * prm->newton_target_iter;
                 std::copy(sol1.begin(), sol1.end(), pv);
                 std::copy(mean1.begin(), mean1.end(), meaning);
+                if (nAux > 0) std::copy(aux1.begin(), aux1.end(), aux);
                 double ndt = dt / 2.0;
                 if (dt < prm->min_dt * (1 + 1e-9)) break;
                 if (ndt < prm->min_dt) ndt = prm->min_dt;
@@ -2165,6 +2231,7 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
             /* advanceTimeLevel */
             std::copy(pv, pv + 3 * N, sol1.begin());
             std::copy(meaning, meaning + N, mean1.begin());
+            if (nAux > 0) std::copy(aux, aux + 3 * nAux, aux1.begin());
             if (pv_hist) std::copy(pv, pv + 3 * N, pv_hist + (long)step * 3 * N);
             /* suggestTimeStepSize */
             const int tgt = prm->newton_target_iter;
@@ -2188,6 +2255,22 @@ int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispe
         fprintf(stderr, "ob_simulate: %s\n", e.what());
         return -1;
     }
+}
+
+int ob_simulate_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersion* dsp, double* pv,
+                    uint8_t* meaning, int n_steps, double dt0, const ob_params* prm, double* pv_hist,
+                    double* dt_hist, ob_iter_log* log, int log_cap, int32_t* log_len, double* timers, int nthreads)
+{
+    return simulateImpl(p, dif, dsp, pv, meaning, nullptr, nullptr, nullptr, n_steps, dt0, prm, pv_hist, dt_hist,
+                        log, log_cap, log_len, timers, nthreads);
+}
+
+int ob_simulate_aux(const ob_problem* p, double* pv, uint8_t* meaning, double* aux, ob_aux_fn fn, void* user,
+                    int n_steps, double dt0, const ob_params* prm, double* pv_hist, double* dt_hist,
+                    ob_iter_log* log, int log_cap, int32_t* log_len, int nthreads)
+{
+    return simulateImpl(p, nullptr, nullptr, pv, meaning, aux, fn, user, n_steps, dt0, prm, pv_hist, dt_hist, log,
+                        log_cap, log_len, nullptr, nthreads);
 }
 
 } // extern "C"

@@ -84,6 +84,16 @@ typedef struct ob_problem {
     const uint8_t* bc_meaning;
     const double* bc_trans;
     const double* bc_zface;
+    /* auxiliary equations (BaseAuxiliaryModule, baseauxiliarymodule.hh:
+     * 55-121): n_aux degrees of freedom after the N cells (rows N ..
+     * N + n_aux - 1 of the system) and the pattern entries their module adds
+     * (addNeighbors, tpfalinearizer.hh:503-507): block (aux_row[k],
+     * aux_col[k]) for k < n_aux_links; every auxiliary row needs its
+     * diagonal among them */
+    int32_t n_aux;
+    int32_t n_aux_links;
+    const int32_t* aux_row;
+    const int32_t* aux_col;
 } ob_problem;
 
 /* Newton / linear-solver parameters; defaults are the reference's
@@ -148,6 +158,23 @@ typedef struct ob_dispersion {
 
 /* ---- A1: pattern, neighbour info, block addresses ---- */
 int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2);
+/* linearizeAuxiliaryEquations (tpfalinearizer.hh:254-281): what the
+ * auxiliary module's linearize(matrix, residual) adds, in order: block k
+ * (row brow[k], column bcol[k], row-major 3x3) into J, entry k of the
+ * residual (row rrow[k], 3 values) into r; rc -1 when a block is not in the
+ * pattern or a row out of range. */
+int ob_add_auxiliary(const ob_problem* p, int32_t nb, const int32_t* brow, const int32_t* bcol,
+                     const double* blocks, int32_t nr, const int32_t* rrow, const double* rvals, double* jac,
+                     double* residual);
+/* ob_simulate with auxiliary DOFs: aux [n_aux][3] their solution(0) (in and
+ * out); after every linearization fn(user, pv, meaning, aux, jac, residual,
+ * newton_iteration) adds the module's terms (a nonzero return fails the
+ * attempt) */
+typedef int (*ob_aux_fn)(void* user, const double* pv, const uint8_t* meaning, const double* aux, double* jac,
+                         double* residual, int iteration);
+int ob_simulate_aux(const ob_problem* p, double* pv, uint8_t* meaning, double* aux, ob_aux_fn fn, void* user,
+                    int n_steps, double dt0, const ob_params* prm, double* pv_hist, double* dt_hist,
+                    ob_iter_log* log, int log_cap, int32_t* log_len, int nthreads);
 int ob_build_pattern(const ob_problem* p, int32_t* row_ptr, int32_t* col_idx,
                      int32_t* diag_off, int32_t* nb_ptr, int32_t* nb_cell,
                      int32_t* nb_off, double* nb_trans, double* nb_area,

@@ -32,6 +32,7 @@ class ObProblem(C.Structure):
         ("rock_comp", _dp), ("rock_pref", _dp), ("n_rock", C.c_int32), ("rock_tab", _dp), ("dir_mob", _dp),
         ("n_bc", C.c_int32), ("bc_cell", _ip), ("bc_face", _ip), ("bc_type", _ip), ("bc_rate", _dp),
         ("bc_state", _dp), ("bc_meaning", _u8p), ("bc_trans", _dp), ("bc_zface", _dp),
+        ("n_aux", C.c_int32), ("n_aux_links", C.c_int32), ("aux_row", _ip), ("aux_col", _ip),
     ]
 
 
@@ -130,6 +131,11 @@ def lib():
         L.ob_gs_sweep.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int]
         L.ob_dense_inverse.argtypes = [C.c_int, _dp]
         L.ob_dot.restype = C.c_double
+        L.ob_simulate_aux.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, OB_AUX_FN, C.c_void_p, C.c_int,
+                                      C.c_double, C.POINTER(ObParams), _dp, _dp, C.POINTER(ObIterLog), C.c_int,
+                                      _ip, C.c_int]
+        L.ob_add_auxiliary.argtypes = [C.POINTER(ObProblem), C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _dp, _dp,
+                                       _dp]
         L.ob_bicgstab.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
         L.ob_update_primary.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, _dp, _u8p, _u8p,
                                         C.POINTER(ObParams), _ip]
@@ -250,15 +256,53 @@ def with_boundary(prob, cell=None, face=None, type=None, rate=None, state=None, 
     return prob
 
 
+def with_auxiliary(prob, n_aux=0, rows=None, cols=None):
+    """prob with n_aux auxiliary degrees of freedom after the cells and the
+    pattern entries (rows[k], cols[k]) their module adds; 0 for none"""
+    if n_aux == 0:
+        prob.n_aux = prob.n_aux_links = 0
+        prob.aux_row = prob.aux_col = None
+        prob._aux_keep = None
+        return prob
+    r = np.ascontiguousarray(rows, dtype=np.int32).ravel()
+    c = np.ascontiguousarray(cols, dtype=np.int32).ravel()
+    prob.n_aux, prob.n_aux_links = int(n_aux), r.size
+    prob.aux_row, prob.aux_col = P(r, _ip), P(c, _ip)
+    prob._aux_keep = (r, c)
+    return prob
+
+
+def rows(prob):
+    """rows of the system: the cells and the auxiliary degrees of freedom"""
+    return prob.nx * prob.ny * prob.nz + prob.n_aux
+
+
+def add_auxiliary(prob, J, r, brow=(), bcol=(), blocks=None, rrow=(), rvals=None):
+    """(J, r) with the auxiliary module's blocks and residual entries added in
+    order (ob_add_auxiliary)"""
+    J = np.array(J, dtype=np.float64)
+    r = np.array(r, dtype=np.float64)
+    br = np.ascontiguousarray(brow, dtype=np.int32).ravel()
+    bc = np.ascontiguousarray(bcol, dtype=np.int32).ravel()
+    bl = np.ascontiguousarray(np.zeros(0) if blocks is None else blocks, dtype=np.float64).ravel()
+    rr = np.ascontiguousarray(rrow, dtype=np.int32).ravel()
+    rv = np.ascontiguousarray(np.zeros(0) if rvals is None else rvals, dtype=np.float64).ravel()
+    assert bl.size == 9 * br.size and rv.size == 3 * rr.size
+    rc = lib().ob_add_auxiliary(C.byref(prob), br.size, P(br, _ip), P(bc, _ip), P(bl), rr.size, P(rr, _ip), P(rv),
+                                P(J), P(r))
+    assert rc == 0
+    return J, r
+
+
 def pattern(prob):
     nz = C.c_int64()
     f2 = C.c_int64()
-    lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
-    N = prob.nx * prob.ny * prob.nz
+    assert lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2)) == 0
+    N = rows(prob)
     rp = np.zeros(N + 1, np.int32)
     ci = np.zeros(nz.value, np.int32)
     dg = np.zeros(N, np.int32)
-    nbp = np.zeros(N + 1, np.int32)
+    nbp = np.zeros(prob.nx * prob.ny * prob.nz + 1, np.int32)
     nbc = np.zeros(f2.value, np.int32)
     nbo = np.zeros(f2.value, np.int32)
     nbt = np.zeros(f2.value)
@@ -289,8 +333,8 @@ def linearize(prob, pv, mean, st1, dt, nthreads=1, diffusion=None, dispersion=No
     N = prob.nx * prob.ny * prob.nz
     nz = C.c_int64()
     f2 = C.c_int64()
-    lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2))
-    r = np.zeros(3 * N)
+    assert lib().ob_pattern_size(C.byref(prob), C.byref(nz), C.byref(f2)) == 0
+    r = np.zeros(3 * rows(prob))
     J = np.zeros(9 * nz.value)
     nv_in = nv_out = None
     if dispersion is not None:
@@ -488,6 +532,46 @@ def simulate(prob, pv0, mean0, n_steps, dt0, prm, nthreads=1, log_cap=4096, keep
     recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
             for l in log[:min(nlog.value, log_cap)]]
     return rc, pv, mean, hist, dts, recs, timers
+
+
+OB_AUX_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _u8p, _dp, _dp, _dp, C.c_int)
+
+
+def simulate_aux(prob, pv0, mean0, aux0, n_steps, dt0, prm, module, nthreads=1, log_cap=4096):
+    """simulate with auxiliary DOFs: module(pv, mean, aux, newton_iteration)
+    returns the terms its linearize adds, (brow, bcol, blocks, rrow, rvals),
+    or None when the linearization fails.  Returns rc, pv, mean, aux, hist,
+    dts, log."""
+    N = prob.nx * prob.ny * prob.nz
+    na = 3 * prob.n_aux
+    pv = np.array(pv0, dtype=np.float64, copy=True)
+    mean = np.array(mean0, dtype=np.uint8, copy=True)
+    aux = np.array(aux0, dtype=np.float64, copy=True)
+    hist = np.zeros((n_steps, 3 * N))
+    dts = np.zeros(n_steps)
+    log = (ObIterLog * log_cap)()
+    nlog = C.c_int32()
+
+    def fn(user, pvp, mp, ap, jp, rp, it):
+        terms = module(np.ctypeslib.as_array(pvp, (3 * N,)).copy(), np.ctypeslib.as_array(mp, (N,)).copy(),
+                       np.ctypeslib.as_array(ap, (na,)).copy(), it)
+        if terms is None:
+            return 1
+        brow, bcol, blocks, rrow, rvals = (np.ascontiguousarray(a) for a in terms)
+        br = brow.astype(np.int32).ravel()
+        bc = bcol.astype(np.int32).ravel()
+        bl = blocks.astype(np.float64).ravel()
+        rr = rrow.astype(np.int32).ravel()
+        rv = rvals.astype(np.float64).ravel()
+        return 0 if lib().ob_add_auxiliary(C.byref(prob), br.size, P(br, _ip), P(bc, _ip), P(bl), rr.size,
+                                           P(rr, _ip), P(rv), jp, rp) == 0 else 1
+
+    cb = OB_AUX_FN(fn)
+    rc = lib().ob_simulate_aux(C.byref(prob), P(pv), P(mean, _u8p), P(aux), cb, None, n_steps, dt0, C.byref(prm),
+                               P(hist), P(dts), log, log_cap, C.byref(nlog), nthreads)
+    recs = [(l.step, l.attempt, l.iter, l.lin_iters, l.num_switched, l.lin_converged, l.error, l.dt)
+            for l in log[:min(nlog.value, log_cap)]]
+    return rc, pv, mean, aux, hist, dts, recs
 
 
 # ---------------------------------------------------------------------------

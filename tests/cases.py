@@ -75,3 +75,64 @@ def random_discretization(c, seed, thp_fraction=0.3, thp_max=4e5):
     vol[::7] = V
     thp = np.where(rng.random(3 * N) < thp_fraction, thp_max * rng.random(3 * N), 0.0)
     return t, z, vol, thp
+
+
+class RateWell:
+    """An auxiliary module for the tests (BaseAuxiliaryModule's interface): one
+    auxiliary DOF per well whose first unknown is the well's bottom-hole
+    pressure p_w; each perforated cell k takes q_k = rho WI (p_w - p_k) of
+    water (kg/s, into the cell), and the well's equation holds the total at
+    its target, sum_k q_k - Q = 0.  The DOF's other two unknowns are pinned
+    to zero (residual x, unit diagonal).  Everything is linear in p_w and the
+    cells' pressures, so the terms and their derivatives are closed forms."""
+
+    RHO = 1000.0
+
+    def __init__(self, N, wells):
+        """wells: list of (perforated cells, well indices, target rate Q)"""
+        self.N = N
+        self.wells = [(np.asarray(c, np.int64), np.asarray(wi, np.float64), float(q)) for c, wi, q in wells]
+        self.n_aux = len(self.wells)
+
+    def links(self):
+        rows, cols = [], []
+        for w, (cells, _, _) in enumerate(self.wells):
+            a = self.N + w
+            rows.append(a); cols.append(a)
+            for k in cells:
+                rows += [int(k), a]
+                cols += [a, int(k)]
+        return np.array(rows, np.int32), np.array(cols, np.int32)
+
+    def initial(self, pv):
+        x = np.zeros(3 * self.n_aux)
+        for w, (cells, _, _) in enumerate(self.wells):
+            x[3 * w] = float(np.mean(pv[3 * cells + 1])) + 1e5
+        return x
+
+    def terms(self, pv, mean, aux, iteration=0):
+        """(brow, bcol, blocks, rrow, rvals) of the module's linearize"""
+        brow, bcol, blocks, rrow, rvals = [], [], [], [], []
+        for w, (cells, wi, q) in enumerate(self.wells):
+            a = self.N + w
+            pw = aux[3 * w]
+            total = 0.0
+            dpw = 0.0
+            for k, wik in zip(cells, wi):
+                rwi = self.RHO * wik
+                qk = rwi * (pw - pv[3 * k + 1])
+                total += qk
+                dpw += rwi
+                # the cell's water equation (index 1): -q_k, a source
+                rrow.append(int(k)); rvals.append([0.0, -qk, 0.0])
+                d = np.zeros(9); d[3 * 1 + 1] = rwi          # d(-q_k)/dp_k
+                brow.append(int(k)); bcol.append(int(k)); blocks.append(d)
+                o = np.zeros(9); o[3 * 1 + 0] = -rwi          # d(-q_k)/dp_w
+                brow.append(int(k)); bcol.append(a); blocks.append(o)
+                c = np.zeros(9); c[0 * 3 + 1] = -rwi          # d(sum q)/dp_k
+                brow.append(a); bcol.append(int(k)); blocks.append(c)
+            rrow.append(a); rvals.append([total - q, aux[3 * w + 1], aux[3 * w + 2]])
+            dd = np.zeros(9); dd[0] = dpw; dd[4] = 1.0; dd[8] = 1.0
+            brow.append(a); bcol.append(a); blocks.append(dd)
+        return (np.array(brow, np.int32), np.array(bcol, np.int32), np.array(blocks).reshape(-1, 9),
+                np.array(rrow, np.int32), np.array(rvals).reshape(-1, 3))
