@@ -293,6 +293,18 @@ int omodels_model_set_directional_mobility(omodels_model* m, const double* multi
  * current solution(0); iteration_index 0 refreshes the storage cache of
  * time level 1 as recycleFirstIterationStorage() does (:786-812). */
 int omodels_linearizer_linearize_domain(omodels_model* m, double dt, int iteration_index);
+/* linearizeDomain(domain) (tpfalinearizer.hh:226-245) on the sub-domain of
+ * the n distinct cells `cells`: resetSystem_(domain) zeroes their residual
+ * rows and Jacobian rows (:370-379); linearize_ then evaluates only these
+ * cells (:711-842): their residual rows and diagonal blocks, and each
+ * cell's column blocks A[J][I] added into its neighbours' rows, which keep
+ * their previous values when the neighbour lies outside the domain; the
+ * storage cache of time level 1 is not refreshed (:793-803); the separate
+ * sparse source terms and the boundary terms are added for all cells, as
+ * the loops after the cell loop do (:843-865).  All cells: the same as
+ * linearize_domain.  Not with diffusion or dispersion. */
+int omodels_linearizer_linearize_subdomain(omodels_model* m, double dt, int iteration_index, int32_t n,
+                                           const int32_t* cells);
 /* jacobian() / residual() host views (Dune layouts) */
 int omodels_linearizer_get_jacobian(omodels_model* m, double* values);
 int omodels_linearizer_get_residual(omodels_model* m, double* residual);

@@ -82,7 +82,7 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
                                           double area, double dZg, bool upInTie, double r[3],
                                           double D[9], double* __restrict__ Jout, const BoDiff& dif,
                                           DiffFace df, const BoDisp& dsp, double& nv, double thp,
-                                          const FaceTM* tm, const FaceDM* dm)
+                                          const FaceTM* tm, const FaceDM* dm, bool acc = false)
 {
     E3 fl[3];
     double darcy[3];
@@ -102,8 +102,15 @@ __device__ __forceinline__ void applyFace(const BoFluid& f, const IqE& qi, const
         blk[3 * e + 1] = 0.0 + (-fl[e].d1);
         blk[3 * e + 2] = 0.0 + (-fl[e].d2);
     }
+    // a sub-domain's linearization adds the column block into the
+    // neighbour's row, which it does not clear (tpfalinearizer.hh:770)
+    if (acc) {
 #pragma unroll
-    for (int t = 0; t < 9; ++t) Jout[t] = blk[t];
+        for (int t = 0; t < 9; ++t) Jout[t] += blk[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < 9; ++t) Jout[t] = blk[t];
+    }
 }
 
 // One (x, y) tile of columns over the planes of one z chunk.  Shared by the
@@ -162,7 +169,12 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
         if (own && k + 1 < G.nz) loadIqV(f, pv, mean, I + nxy, above);
         __syncthreads();
 
-        if (own) {
+        // a sub-domain's linearization (Geo form with a domain mask) only
+        // evaluates the domain's cells
+        bool act = own;
+        if constexpr (Geo) act = act && (!geo.domain || geo.domain[I]);
+        const bool acc = Geo && geo.domain != nullptr;
+        if (act) {
             IqE qi;
             const double sw = pv[3 * I], po = pv[3 * I + 1], x2 = pv[3 * I + 2];
             const int mI = mean[I];
@@ -223,39 +235,39 @@ __device__ __forceinline__ void assembleTile(const BoFluid& f, const BoGrid& G,
                 IqV q; fromLds(plane[ty + 1][tx], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[I - 1], G.area[0], (zI - zOf(I - 1, 0)) * G.g, tieOf(I - 1), r, D,
                                 jac + 9L * nbOff[0 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - 1, q, I - 1), dsp,
-                                nvo, thpOf(I - 1), tmOf(q), dmOf(I - 1, 0));
+                                nvo, thpOf(I - 1), tmOf(q), dmOf(I - 1, 0), acc);
             }
             if (i + 1 < G.nx) {
                 IqV q; fromLds(plane[ty + 1][tx + 2], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[I], G.area[0], (zI - zOf(I + 1, 0)) * G.g, tieOf(I + 1), r, D,
                                 jac + 9L * nbOff[1 * N + I], dif, diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + 1, q, I), dsp,
-                                nvo, thpOf(I), tmOf(q), dmOf(I + 1, 0));
+                                nvo, thpOf(I), tmOf(q), dmOf(I + 1, 0), acc);
             }
             if (j > 0) {
                 IqV q; fromLds(plane[ty][tx + 1], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[N + I - G.nx], G.area[1], (zI - zOf(I - G.nx, 0)) * G.g,
                                 tieOf(I - G.nx), r, D, jac + 9L * nbOff[2 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - G.nx, q, N + I - G.nx), dsp, nvo,
-                                thpOf(N + I - G.nx), tmOf(q), dmOf(I - G.nx, 1));
+                                thpOf(N + I - G.nx), tmOf(q), dmOf(I - G.nx, 1), acc);
             }
             if (j + 1 < G.ny) {
                 IqV q; fromLds(plane[ty + 2][tx + 1], q);
                 applyFace<Diff, Disp>(f, qi, q, trans[N + I], G.area[1], (zI - zOf(I + G.nx, 0)) * G.g, tieOf(I + G.nx),
                                 r, D, jac + 9L * nbOff[3 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + G.nx, q, N + I), dsp, nvo, thpOf(N + I), tmOf(q),
-                                dmOf(I + G.nx, 1));
+                                dmOf(I + G.nx, 1), acc);
             }
             if (haveBelow) {
                 applyFace<Diff, Disp>(f, qi, below, trans[2 * N + I - nxy], G.area[2], (zI - zOf(I - nxy, -1)) * G.g,
                                 tieOf(I - nxy), r, D, jac + 9L * nbOff[4 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I - nxy, below, 2 * N + I - nxy), dsp, nvo,
-                                thpOf(2 * N + I - nxy), tmOf(below), dmOf(I - nxy, 2));
+                                thpOf(2 * N + I - nxy), tmOf(below), dmOf(I - nxy, 2), acc);
             }
             if (k + 1 < G.nz) {
                 applyFace<Diff, Disp>(f, qi, above, trans[2 * N + I], G.area[2], (zI - zOf(I + nxy, 1)) * G.g,
                                 tieOf(I + nxy), r, D, jac + 9L * nbOff[5 * N + I], dif,
                                 diffFace<Diff, Disp>(dif, dsp, pv, mean, I, I + nxy, above, 2 * N + I), dsp, nvo,
-                                thpOf(2 * N + I), tmOf(above), dmOf(I + nxy, 2));
+                                thpOf(2 * N + I), tmOf(above), dmOf(I + nxy, 2), acc);
             }
             // accumulation term (tpfalinearizer.hh:776-825)
             {
@@ -338,12 +350,13 @@ __global__ void k_bo_sources(BoFluid f, BoGrid G, const double* __restrict__ pv,
                              const int* __restrict__ type, const double* __restrict__ a,
                              const double* __restrict__ b, double* __restrict__ res,
                              double* __restrict__ jac, const int* __restrict__ diagOff,
-                             const double* __restrict__ vol)
+                             const double* __restrict__ vol, const uint8_t* __restrict__ domain)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
     const int2 grp = groups[g];
     const long I = cell[order[grp.x]];
+    if (domain && !domain[I]) return;      // a sub-domain: its cells' sources only
     const double V = vol ? vol[I] : G.V;
     IqE qi;
     iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
@@ -677,11 +690,11 @@ void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, co
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                      int ngroups, const int2* groups, const int* order, const int* cell, const int* type,
                      const double* a, const double* b, double* res, double* jac, const int* diagOff, hipStream_t s,
-                     const double* vol)
+                     const double* vol, const uint8_t* domain)
 {
     if (ngroups <= 0) return;
     hipLaunchKernelGGL(k_bo_sources, dim3((ngroups + 63) / 64), dim3(64), 0, s, f, G, pv, mean, ngroups, groups,
-                       order, cell, type, a, b, res, jac, diagOff, vol);
+                       order, cell, type, a, b, res, jac, diagOff, vol, domain);
 }
 
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
@@ -761,6 +774,24 @@ void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const
     hipLaunchKernelGGL(k_bo_flows, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, s, fdev, G, pv, mean, trans, rowPtr,
                        flows, flores, dif ? *dif : BoDiff{}, dsp ? *dsp : BoDisp{},
                        (geo && geo->depth) ? *geo : BoGeo{});
+}
+
+// resetSystem_(domain) (tpfalinearizer.hh:370-379): the domain's rows of the
+// residual and the Jacobian set to zero; one thread per row
+__global__ void k_clear_rows(int n, const int* __restrict__ cells, const int* __restrict__ rowPtr,
+                             double* __restrict__ jac, double* __restrict__ res)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long I = cells[t];
+    for (int e = 0; e < 3; ++e) res[3 * I + e] = 0.0;
+    for (long k = 9L * rowPtr[I]; k < 9L * rowPtr[I + 1]; ++k) jac[k] = 0.0;
+}
+
+void launchClearRows(int n, const int* cells, const int* rowPtr, double* jac, double* res, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_clear_rows, dim3((n + 255) / 256), dim3(256), 0, s, n, cells, rowPtr, jac, res);
 }
 
 void launchAddAuxiliary(int nb, const int* blockOff, const double* blocks, int nr, const int* resRow,

@@ -356,6 +356,16 @@ int omodels_linearizer_linearize_domain(omodels_model* h, double dt, int iterati
     });
 }
 
+int omodels_linearizer_linearize_subdomain(omodels_model* h, double dt, int iteration_index, int32_t n,
+                                           const int32_t* cells)
+{
+    return guarded([&] {
+        if (!(dt > 0)) throw std::invalid_argument("dt must be positive");
+        h->m->linearizeSubdomain(dt, iteration_index, n, cells);
+        return 0;
+    });
+}
+
 int omodels_linearizer_get_jacobian(omodels_model* h, double* values)
 {
     return guarded([&] {

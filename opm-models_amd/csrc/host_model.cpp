@@ -156,6 +156,60 @@ void Model::linearizeDomain(double dt, int iterationIndex)
     OM_HIP(hipGetLastError());
 }
 
+// linearizeDomain(domain) (tpfalinearizer.hh:226-245) on a part of the grid:
+// resetSystem_(domain) clears the domain's rows (:370-379), then linearize_
+// evaluates the domain's cells only (:711-842): their residual rows and
+// diagonal blocks, and the column blocks A[J][I] added into the neighbours'
+// rows (which are not cleared when the neighbour lies outside); the storage
+// cache is not refreshed on a sub-domain (:793-803).  The separate sparse
+// source terms and the boundary terms are added for all cells, as the
+// reference's loops after the cell loop do (:843-865).  All cells: the full
+// linearization.
+void Model::linearizeSubdomain(double dt, int iterationIndex, int n, const int* cells)
+{
+    if (n <= 0 || !cells) throw std::invalid_argument("sub-domain: at least one cell");
+    if (dispersion.disp || diffusion.diff) throw std::invalid_argument("sub-domain: not with diffusion or dispersion");
+    std::vector<uint8_t> mask(N, 0);
+    for (int k = 0; k < n; ++k) {
+        if (cells[k] < 0 || cells[k] >= N) throw std::invalid_argument("sub-domain: cell out of range");
+        if (mask[cells[k]]) throw std::invalid_argument("sub-domain: a cell given twice");
+        mask[cells[k]] = 1;
+    }
+    if (n == rows()) { linearizeDomain(dt, iterationIndex); return; }
+    ensureJacobian();
+    sync();
+    domMask.upload(mask.data(), mask.size(), stream);
+    domCells.upload(cells, n, stream);
+    launchClearRows(n, domCells.p, rowPtr.p, jac.p, res.p, stream);
+    BoGeo g = geo;
+    if (!g.depth) {
+        if (!ownDepth.p) {
+            const long nxy = (long)G.nx * G.ny;
+            std::vector<double> z(N), v(N, G.V), th(3 * (size_t)N, 0.0);
+            for (long I = 0; I < N; ++I) z[I] = G.dz * ((double)(I / nxy) + 0.5);
+            ownDepth.upload(z.data(), z.size(), stream);
+            ownVol.upload(v.data(), v.size(), stream);
+            ownThp.upload(th.data(), th.size(), stream);
+        }
+        g.depth = ownDepth.p; g.vol = ownVol.p; g.thp = ownThp.p;
+    }
+    g.dt = dt;
+    g.domain = domMask.p;
+    timer.begin(kClsAssemble, stream);
+    launchBoAssemble(fluidDev.p, G, pv0.p, mean0.p, poro.p, trans.p, st1.p, G.V / dt, res.p, jac.p, diagOff.p,
+                     nbOff.p, stream, nullptr, nullptr, &g);
+    timer.end(stream);
+    if (!separateSparseSources)
+        launchBoSources(fluid, G, pv0.p, mean0.p, nSrcGroups, srcGroups.p, srcOrder.p, srcCell.p, srcType.p, srcA.p,
+                        srcB.p, res.p, jac.p, diagOff.p, stream, geo.vol, domMask.p);
+    else
+        addBuiltinSparseSources();
+    launchBoBoundary(fluidDev.p, G, pv0.p, mean0.p, bc, geo.depth, res.p, jac.p, diagOff.p, stream);
+    (void)iterationIndex;
+    OM_HIP(hipGetLastError());
+    sync();
+}
+
 void Model::setBoundary(int n, const int* cell, const int* face, const int* type, const double* rate,
                         const double* state, const uint8_t* meaning, const double* btrans, const double* zface)
 {

@@ -116,6 +116,12 @@ public:
 
     // TpfaLinearizer
     void linearizeDomain(double dt, int iterationIndex);
+    // linearizeDomain(domain) of a sub-domain (tpfalinearizer.hh:226-245): n
+    // distinct cells
+    void linearizeSubdomain(double dt, int iterationIndex, int n, const int* cells);
+    DevBuf<uint8_t> domMask;
+    DevBuf<int> domCells;
+    DevBuf<double> ownDepth, ownVol, ownThp;   // the grid's own geometry for the extended form
     // SeparateSparseSourceTerms (tpfalinearizer.hh:56): when set, linearizeDomain
     // leaves the source terms out of the cell loop and adds them after it as
     // sparse blocks

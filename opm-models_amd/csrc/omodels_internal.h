@@ -42,6 +42,7 @@ struct BoGeo {
     double dt;
     BoRock rock;
     const double* dirMob;     // [3 axes][3 phases][N] directional mobility multipliers, or nullptr
+    const uint8_t* domain;    // a sub-domain's cells (nonzero), or nullptr: every cell
 };
 
 // Boundary faces with a condition, sorted by cell and face, grouped by cell
@@ -178,7 +179,9 @@ void launchBoAssemble(const BoFluid* fdev, const BoGrid& G, const double* pv, co
 void launchBoSources(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,
                      int ngroups, const int2* groups, const int* order, const int* cell, const int* type,
                      const double* a, const double* b, double* res, double* jac, const int* diagOff, hipStream_t s,
-                     const double* vol = nullptr);
+                     const double* vol = nullptr, const uint8_t* domain = nullptr);
+// resetSystem_(domain): zero the residual and Jacobian rows of n cells
+void launchClearRows(int n, const int* cells, const int* rowPtr, double* jac, double* res, hipStream_t s);
 void launchBoStorage(const BoFluid& f, long N, const double* pv, const uint8_t* mean,
                      const double* poro, double* st, hipStream_t s, const BoRock* rock = nullptr);
 void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, const uint8_t* mean,

@@ -115,6 +115,7 @@ SIGNATURES = {
     "omodels_linearizer_linearize_domain": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "omodels_linearizer_update_discretization": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     "omodels_model_set_rock": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int32, _dp]),
+    "omodels_linearizer_linearize_subdomain": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int32, _ip]),
     "omodels_linearizer_get_jacobian": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_get_residual": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_erase_matrix": (C.c_int, [C.c_void_p]),
@@ -546,6 +547,13 @@ class Model:
         check(lib().omodels_linearizer_get_jacobian(self.h, ptr(J)), "get_jacobian")
         check(lib().omodels_linearizer_get_residual(self.h, ptr(r)), "get_residual")
         return J, r
+
+    def linearize_subdomain(self, dt, cells, iteration_index=1):
+        """linearizeDomain(domain) on the given cells; returns the whole (J, r)"""
+        c = np.ascontiguousarray(cells, dtype=np.int32).ravel()
+        check(lib().omodels_linearizer_linearize_subdomain(self.h, dt, iteration_index, c.size, ptr(c, _ip)),
+              "linearize_subdomain")
+        return self.system()
 
     def system(self):
         """The linearizer's current Jacobian values and residual (no relinearization)."""

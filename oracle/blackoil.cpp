@@ -747,19 +747,29 @@ void boundaryFlux(const Pvt& pvt, const IQ<E3>& qi, const Problem::Bc& bc, doubl
  * blackoildispersionmodule.hh:355-383). */
 void linearize(const Problem& pr, const double* pv, const uint8_t* mean, const double* st1,
                double dt, double* res, double* jac, int nthreads, const double* nvIn = nullptr,
-               double* nvOut = nullptr)
+               double* nvOut = nullptr, const uint8_t* domain = nullptr)
 {
     const long N = pr.G.N;
-    /* resetSystem_ zeroes every row, the auxiliary ones included */
+    /* resetSystem_ zeroes every row, the auxiliary ones included; on a
+     * sub-domain resetSystem_(domain) only the domain's rows (:370-379) */
     const long R = (long)pr.pat.rowPtr.size() - 1;
     const long nnzb = pr.pat.rowPtr[R];
-    std::fill(res, res + 3 * R, 0.0);
-    std::fill(jac, jac + 9 * nnzb, 0.0);
+    if (!domain) {
+        std::fill(res, res + 3 * R, 0.0);
+        std::fill(jac, jac + 9 * nnzb, 0.0);
+    } else {
+        for (long I = 0; I < N; ++I) {
+            if (!domain[I]) continue;
+            std::fill(res + 3 * I, res + 3 * I + 3, 0.0);
+            std::fill(jac + 9L * pr.pat.rowPtr[I], jac + 9L * pr.pat.rowPtr[I + 1], 0.0);
+        }
+    }
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
     for (long I = 0; I < N; ++I) {
+        if (domain && !domain[I]) continue;     /* linearize_(domain): the domain's cells (:733-735) */
         IQ<E3> qi;
         pr.iq<E3>(pv, mean, I, qi);
         double* r = res + 3 * I;
@@ -1912,6 +1922,23 @@ int ob_flows_ext(const ob_problem* p, const ob_diffusion* dif, const ob_dispersi
                 }
             }
         }
+        return 0;
+    } catch (...) { return -1; }
+}
+
+int ob_linearize_domain(const ob_problem* p, const double* pv, const uint8_t* meaning, const double* storage1,
+                        double dt, int32_t n, const int32_t* cells, double* residual, double* jac)
+{
+    try {
+        Problem pr(p);
+        if (n <= 0 || !cells) return -1;
+        std::vector<uint8_t> dom(pr.G.N, 0);
+        for (int32_t k = 0; k < n; ++k) {
+            if (cells[k] < 0 || cells[k] >= pr.G.N || dom[cells[k]]) return -1;
+            dom[cells[k]] = 1;
+        }
+        const long R = (long)pr.pat.rowPtr.size() - 1;
+        linearize(pr, pv, meaning, storage1, dt, residual, jac, 1, nullptr, nullptr, n == R ? nullptr : dom.data());
         return 0;
     } catch (...) { return -1; }
 }

@@ -158,6 +158,11 @@ typedef struct ob_dispersion {
 
 /* ---- A1: pattern, neighbour info, block addresses ---- */
 int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2);
+/* linearizeDomain(domain) (tpfalinearizer.hh:226-245) on the n distinct
+ * cells: residual and jac are read and written (rows outside the domain keep
+ * their values but for the domain cells' column blocks added into them) */
+int ob_linearize_domain(const ob_problem* p, const double* pv, const uint8_t* meaning, const double* storage1,
+                        double dt, int32_t n, const int32_t* cells, double* residual, double* jac);
 /* linearizeAuxiliaryEquations (tpfalinearizer.hh:254-281): what the
  * auxiliary module's linearize(matrix, residual) adds, in order: block k
  * (row brow[k], column bcol[k], row-major 3x3) into J, entry k of the

@@ -134,6 +134,8 @@ def lib():
         L.ob_simulate_aux.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, OB_AUX_FN, C.c_void_p, C.c_int,
                                       C.c_double, C.POINTER(ObParams), _dp, _dp, C.POINTER(ObIterLog), C.c_int,
                                       _ip, C.c_int]
+        L.ob_linearize_domain.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, C.c_int32, _ip, _dp,
+                                          _dp]
         L.ob_add_auxiliary.argtypes = [C.POINTER(ObProblem), C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _dp, _dp,
                                        _dp]
         L.ob_bicgstab.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.POINTER(ObParams), _ip]
@@ -344,6 +346,17 @@ def linearize(prob, pv, mean, st1, dt, nthreads=1, diffusion=None, dispersion=No
                                 P(mean, _u8p), P(st1), dt, P(r), P(J), nthreads)
     assert rc == 0
     return (J, r) if dispersion is None else (J, r, nv_out)
+
+
+def linearize_domain(prob, pv, mean, st1, dt, cells, J, r):
+    """linearizeDomain(domain) on cells over the previous (J, r); returns the
+    new (J, r)"""
+    J = np.array(J, dtype=np.float64, copy=True)
+    r = np.array(r, dtype=np.float64, copy=True)
+    c = np.ascontiguousarray(cells, dtype=np.int32).ravel()
+    rc = lib().ob_linearize_domain(C.byref(prob), P(pv), P(mean, _u8p), P(st1), dt, c.size, P(c, _ip), P(r), P(J))
+    assert rc == 0
+    return J, r
 
 
 def linearize_times(prob, pv, mean, st1, dt, nthreads, reps=5):
