@@ -102,6 +102,26 @@ public:
         check(omodels_linearizer_get_residual(model_, residual_.data()), "get_residual");
     }
 
+    /* linearizeDomain(domain) (tpfalinearizer.hh:226-245): the cells of
+     * domain.cells only, over the previous Jacobian and residual */
+    template <class SubDomain>
+    void linearizeDomain(const SubDomain& domain)
+    {
+        std::vector<double> pv;
+        std::vector<uint8_t> meaning;
+        G::packSolution(*simulator_, pv, meaning);
+        if ((int64_t)pv.size() != 3 * n_ || (int64_t)meaning.size() != n_)
+            throw std::invalid_argument("packSolution: wrong sizes");
+        std::vector<int32_t> cells(domain.cells.begin(), domain.cells.end());
+        check(omodels_model_set_solution(model_, pv.data(), meaning.data(), 0), "set_solution");
+        check(omodels_linearizer_linearize_subdomain(model_, G::timeStepSize(*simulator_),
+                                                     G::newtonIteration(*simulator_), (int32_t)cells.size(),
+                                                     cells.data()),
+              "linearize_subdomain");
+        check(omodels_linearizer_get_jacobian(model_, jacobian_.data()), "get_jacobian");
+        check(omodels_linearizer_get_residual(model_, residual_.data()), "get_residual");
+    }
+
     /* updateStoredTransmissibilities (tpfalinearizer.hh:868-883), with the
      * depths, volumes and threshold pressures createMatrix_ reads (:443-452) */
     void updateStoredTransmissibilities()

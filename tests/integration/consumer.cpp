@@ -164,8 +164,15 @@ int main(int argc, char** argv)
         lin.linearizeDomain();
         writeBin(dir + "/jac1.out", lin.jacobian());
         writeBin(dir + "/res1.out", lin.residual());
-        std::printf("{\"converged\": %d, \"iterations\": %d, \"switched\": %d, \"cells\": %lld}\n", converged ? 1 : 0,
-                    backend.iterations(), switched, (long long)lin.numCells());
+        // linearizeDomain(domain) over every cell: the full linearization again
+        struct { std::vector<int> cells; } all;
+        for (long long c = 0; c < (long long)lin.numCells(); ++c) all.cells.push_back((int)c);
+        const std::vector<double> j1 = lin.jacobian(), r1 = lin.residual();
+        lin.linearizeDomain(all);
+        const bool same = lin.jacobian() == j1 && lin.residual() == r1;
+        std::printf("{\"converged\": %d, \"iterations\": %d, \"switched\": %d, \"cells\": %lld, "
+                    "\"subdomain_same\": %d}\n",
+                    converged ? 1 : 0, backend.iterations(), switched, (long long)lin.numCells(), same ? 1 : 0);
         return 0;
     } catch (const omodels_opm::NumericalProblem& e) {
         std::cerr << "numerical problem: " << e.what() << "\n";

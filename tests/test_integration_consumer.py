@@ -93,3 +93,4 @@ def test_consumer_newton_iteration_matches_the_binding(tmp_path, distorted):
         assert np.array_equal(_bits(out[k]), _bits(a)), k
     assert np.array_equal(out["mean1"], mean1)
     assert '"converged": %d' % (1 if rc == 0 else 0) in r.stdout
+    assert '"subdomain_same": 1' in r.stdout          # linearizeDomain(domain) over every cell
