@@ -299,6 +299,9 @@ __global__ __launch_bounds__(kGjThreads) void k_amg_dense_invert(int n, double* 
         if (tid == 0) atomicOr(fail, 2);
         return;
     }
+    // thread (rg, j) updates column j of rows rg, rg + ngroups, ...: the
+    // index arithmetic once, not per element
+    const int jc = tid % n, rg = tid / n, ngroups = kGjThreads / n;
     int cur = 0;
     for (int k = 0; k < n; ++k) {
         const double* A = S[cur];
@@ -328,14 +331,16 @@ __global__ __launch_bounds__(kGjThreads) void k_amg_dense_invert(int n, double* 
         }
         const int p = piv[k];
         const double r = 1.0 / A[p * kDenseLd + k];
-        for (int q = tid; q < nn; q += kGjThreads) {
-            const int i = q / n, j = q - i * n;
+        if (rg < ngroups) {
+            const int j = jc;
             const double rowk = (j == k ? 1.0 : A[p * kDenseLd + j]) * r;
-            if (i == k) {
-                B[i * kDenseLd + j] = rowk;
-            } else {
-                const int si = i == p ? k : i;
-                B[i * kDenseLd + j] = (j == k ? 0.0 : A[si * kDenseLd + j]) - rowk * A[si * kDenseLd + k];
+            for (int i = rg; i < n; i += ngroups) {
+                if (i == k) {
+                    B[i * kDenseLd + j] = rowk;
+                } else {
+                    const int si = i == p ? k : i;
+                    B[i * kDenseLd + j] = (j == k ? 0.0 : A[si * kDenseLd + j]) - rowk * A[si * kDenseLd + k];
+                }
             }
         }
         cur ^= 1;
