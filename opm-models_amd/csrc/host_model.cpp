@@ -679,7 +679,8 @@ void Model::addAuxiliary(int nb, const int* blockOff, const double* blocks, int 
 }
 
 // ---------------------------------------------------------------------------
-BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm) : m_(m), prm_(prm)
+BiCGStabBackend::BiCGStabBackend(LinearSystem& m, const omodels_params& prm)
+    : m_(m), builtAux_(m.nAux), builtNnzb_(m.nnzb), prm_(prm)
 {
     const long n = (long)m.bs * m.rows();
     if (prm_.precond < 0 || prm_.precond > 2) throw std::invalid_argument("precond must be 0, 1 or 2");
@@ -854,6 +855,10 @@ void BiCGStabBackend::prepare(const double* hostValues, const double* hostResidu
     hipStream_t s = m_.stream;
     const int bs = m_.bs, bb = bs * bs;
     const long n = (long)bs * m_.rows(), nnzb = m_.nnzb;
+    // a solver is sized for the system it was created on (auxiliary DOFs
+    // change the number of rows)
+    if ((long)b64_.n != n || m_.nAux != builtAux_ || m_.nnzb != builtNnzb_)
+        throw std::logic_error("the system's rows changed since the solver was created: create it again");
     if (hostValues) { Acopy_.upload(hostValues, bb * nnzb, s); A64_ = Acopy_.p; }
     else A64_ = m_.jac.p;
     if (hostResidual) b64_.upload(hostResidual, n, s);

@@ -253,6 +253,7 @@ private:
     IterGraphs graphs_[2];   // FP64, FP32 Krylov vectors
 
     LinearSystem& m_;
+    long builtAux_ = 0, builtNnzb_ = 0;   // the system it was created on (auxiliary DOFs, blocks)
     omodels_params prm_;
     int iters_ = 0;
     const double* A64_ = nullptr;        // FP64 matrix used by the solver

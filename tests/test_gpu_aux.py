@@ -154,7 +154,7 @@ def test_failing_module_halves_the_step_like_the_oracle():
     assert np.array_equal(_bits(m.get_auxiliary_solution()), _bits(auxo))
 
 
-@pytest.mark.parametrize("what", ["range", "diagonal", "outside", "no_module", "amg"])
+@pytest.mark.parametrize("what", ["range", "diagonal", "outside", "no_module", "amg", "stale_solver"])
 def test_invalid_use_is_rejected(what):
     c = case(3, 2, 2)
     N = c.N
@@ -180,3 +180,10 @@ def test_invalid_use_is_rejected(what):
         m.set_auxiliary(1, [N, N, 0], [N, 0, N])
         with pytest.raises(om.OmodelsError):
             om.Solver(m, om.default_params(precond=2))
+    elif what == "stale_solver":       # created before the system grew
+        s = om.Solver(m, om.default_params())
+        m.set_auxiliary(1, [N, N, 0], [N, 0, N])
+        m.set_solution(c.pv0, c.mean0)
+        m.linearize(86400.0)
+        with pytest.raises(om.OmodelsError):
+            s.prepare()
