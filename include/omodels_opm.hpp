@@ -21,6 +21,10 @@
  *     // vector empty for the structured grid's own values
  *     static void discretization(const MySimulator&, std::vector<double>& trans, std::vector<double>& depth,
  *                                std::vector<double>& volume, std::vector<double>& thpres);
+ *     // only for updateBoundaryConditionData: the faces with a condition
+ *     static void boundaryConditions(const MySimulator&, std::vector<int32_t>& cell, std::vector<int32_t>& face,
+ *                                    std::vector<int32_t>& type, std::vector<double>& massRate,
+ *                                    std::vector<double>& state, std::vector<uint8_t>& meaning);
  *   };
  *
  * Errors: a negative return code of the C ABI throws std::runtime_error with
@@ -32,6 +36,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "omodels.h"
@@ -52,9 +57,39 @@ inline int check(int rc, const char* what)
 template <class Simulator>
 struct Glue;    // specialized by the integrating code
 
-/* Linearizer: createMatrix_ (init), linearizeDomain, jacobian() / residual()
- * as host arrays in the Dune layouts (row-major 3x3 blocks in the sorted
- * BCRS pattern, residual [N][3]), eraseMatrix, updateStoredTransmissibilities. */
+/* An auxiliary module (BaseAuxiliaryModule, baseauxiliarymodule.hh:55-121):
+ * its DOFs follow the cells (setDofOffset), addNeighbors lists the pattern
+ * entries it needs as (row, column) pairs of global DOFs, applyInitial
+ * writes its initial solution ([numDofs][3]) and linearize adds its terms
+ * for the current cell state and its own solution. */
+struct AuxTerms {
+    std::vector<int32_t> blockRow, blockCol;   // blocks (row-major 3x3) added at (row, col)
+    std::vector<double> blocks;
+    std::vector<int32_t> resRow;               // residual rows (3 values each) added at row
+    std::vector<double> res;
+};
+class GpuAuxiliaryModule {
+public:
+    virtual ~GpuAuxiliaryModule() = default;
+    virtual unsigned numDofs() const = 0;
+    virtual void addNeighbors(std::vector<std::pair<int32_t, int32_t>>& entries) const = 0;
+    virtual void applyInitial(std::vector<double>& x) = 0;
+    /* pv / meaning: the cells' solution(0); x: this module's solution(0) */
+    virtual void linearize(const std::vector<double>& pv, const std::vector<uint8_t>& meaning,
+                           const std::vector<double>& x, AuxTerms& terms) = 0;
+    void setDofOffset(int value) { dofOffset_ = value; }
+    int dofOffset() const { return dofOffset_; }
+    int localToGlobalDof(unsigned localDofIdx) const { return dofOffset_ + (int)localDofIdx; }
+
+private:
+    int dofOffset_ = 0;
+};
+
+/* Linearizer: createMatrix_ (init), linearize, linearizeDomain (whole and
+ * sub-domain), linearizeAuxiliaryEquations with the auxiliary modules added,
+ * jacobian() / residual() as host arrays in the Dune layouts (row-major 3x3
+ * blocks in the sorted BCRS pattern, residual [rows][3]), eraseMatrix,
+ * updateStoredTransmissibilities, updateBoundaryConditionData, the flows. */
 template <class Simulator, class G = Glue<Simulator>>
 class GpuTpfaLinearizer {
 public:
@@ -76,6 +111,7 @@ public:
         check(omodels_model_num_cells(model_, &n, &nnzb), "num_cells");
         n_ = n;
         nnzb_ = nnzb;
+        gridNnzb_ = nnzb;
         rowPtr_.resize(n + 1);
         colIdx_.resize(nnzb);
         check(omodels_linearizer_pattern(model_, rowPtr_.data(), colIdx_.data(), nullptr, nullptr, nullptr,
@@ -140,19 +176,125 @@ public:
 
     void eraseMatrix() { check(omodels_linearizer_erase_matrix(model_), "erase_matrix"); }
 
+    /* linearize() (tpfalinearizer.hh:171-177): the domain, then the
+     * auxiliary equations; finalize() has nothing left to do here */
+    void linearize()
+    {
+        linearizeDomain();
+        linearizeAuxiliaryEquations();
+    }
+    void finalize() {}
+
+    /* model.addAuxiliaryModule: the module's DOFs follow the cells and the
+     * modules added before; the pattern grows by its entries (createMatrix_,
+     * :503-507) and its initial solution is set (applyInitial).  Create the
+     * solver backend after the last module. */
+    void addAuxiliaryModule(GpuAuxiliaryModule* module)
+    {
+        modules_.push_back(module);
+        int32_t off = (int32_t)n_;
+        std::vector<std::pair<int32_t, int32_t>> entries;
+        std::vector<double> x0;
+        for (GpuAuxiliaryModule* m : modules_) {
+            m->setDofOffset(off);
+            off += (int32_t)m->numDofs();
+            m->addNeighbors(entries);
+            std::vector<double> x(3 * (size_t)m->numDofs(), 0.0);
+            m->applyInitial(x);
+            x0.insert(x0.end(), x.begin(), x.end());
+        }
+        nAux_ = off - n_;
+        std::vector<int32_t> rows, cols;
+        for (const auto& e : entries) { rows.push_back(e.first); cols.push_back(e.second); }
+        check(omodels_model_set_auxiliary(model_, (int32_t)nAux_, (int32_t)rows.size(), rows.data(), cols.data()),
+              "set_auxiliary");
+        check(omodels_model_set_auxiliary_solution(model_, x0.data(), 1), "set_auxiliary_solution");
+        refreshPattern_();
+    }
+
+    /* linearizeAuxiliaryEquations (tpfalinearizer.hh:254-281): every
+     * module's terms added to the linearized system, in module order */
+    void linearizeAuxiliaryEquations()
+    {
+        if (modules_.empty()) return;
+        std::vector<double> pv, xall(3 * (size_t)nAux_);
+        std::vector<uint8_t> meaning;
+        G::packSolution(*simulator_, pv, meaning);
+        check(omodels_model_get_auxiliary_solution(model_, xall.data()), "get_auxiliary_solution");
+        for (GpuAuxiliaryModule* m : modules_) {
+            const size_t b = 3 * (size_t)(m->dofOffset() - n_);
+            std::vector<double> x(xall.begin() + b, xall.begin() + b + 3 * m->numDofs());
+            AuxTerms t;
+            m->linearize(pv, meaning, x, t);
+            check(omodels_linearizer_add_auxiliary(model_, (int32_t)t.blockRow.size(), t.blockRow.data(),
+                                                   t.blockCol.data(), t.blocks.data(), (int32_t)t.resRow.size(),
+                                                   t.resRow.data(), t.res.data()),
+                  "add_auxiliary");
+        }
+        check(omodels_linearizer_get_jacobian(model_, jacobian_.data()), "get_jacobian");
+        check(omodels_linearizer_get_residual(model_, residual_.data()), "get_residual");
+    }
+
+    /* updateFlowsInfo / getFlowsInfo / getFloresInfo (tpfalinearizer.hh:
+     * 647-705): per cell and neighbour in intersection order, [faces][3] */
+    void updateFlowsInfo()
+    {
+        const size_t nf = (size_t)(gridNnzb_ - n_);
+        flows_.assign(3 * nf, 0.0);
+        flores_.assign(3 * nf, 0.0);
+        check(omodels_linearizer_flows(model_, flows_.data(), flores_.data()), "flows");
+    }
+    const std::vector<double>& getFlowsInfo() const { return flows_; }
+    const std::vector<double>& getFloresInfo() const { return flores_; }
+
+    /* updateBoundaryConditionData: the faces with a condition from
+     * Glue::boundaryConditions(sim, cell, face, type, massRate, state,
+     * meaning) (see omodels_model_set_boundary_conditions) */
+    void updateBoundaryConditionData()
+    {
+        std::vector<int32_t> cell, face, type;
+        std::vector<double> rate, state;
+        std::vector<uint8_t> meaning;
+        G::boundaryConditions(*simulator_, cell, face, type, rate, state, meaning);
+        const size_t n = cell.size();
+        if (face.size() != n || type.size() != n)
+            throw std::invalid_argument("boundaryConditions: wrong sizes");
+        check(omodels_model_set_boundary_conditions(model_, (int32_t)n, cell.data(), face.data(), type.data(),
+                                                    rate.empty() ? nullptr : rate.data(),
+                                                    state.empty() ? nullptr : state.data(),
+                                                    meaning.empty() ? nullptr : meaning.data(), nullptr, nullptr),
+              "set_boundary_conditions");
+    }
+
     const std::vector<int32_t>& rowPtr() const { return rowPtr_; }
     const std::vector<int32_t>& colIdx() const { return colIdx_; }
     const std::vector<double>& jacobian() const { return jacobian_; }
     const std::vector<double>& residual() const { return residual_; }
     int64_t numCells() const { return n_; }
+    /* rows of the system: the cells and the auxiliary modules' DOFs */
+    int64_t numTotalDof() const { return n_ + nAux_; }
     omodels_model* handle() const { return model_; }
 
 private:
+    void refreshPattern_()
+    {
+        int64_t n = 0;
+        check(omodels_model_num_cells(model_, &n, &nnzb_), "num_cells");
+        rowPtr_.assign((size_t)(n_ + nAux_ + 1), 0);
+        colIdx_.assign((size_t)nnzb_, 0);
+        check(omodels_linearizer_pattern(model_, rowPtr_.data(), colIdx_.data(), nullptr, nullptr, nullptr, nullptr,
+                                         nullptr, nullptr),
+              "pattern");
+        jacobian_.assign(9 * (size_t)nnzb_, 0.0);
+        residual_.assign(3 * (size_t)(n_ + nAux_), 0.0);
+    }
+
     Simulator* simulator_ = nullptr;
     omodels_model* model_ = nullptr;
-    int64_t n_ = 0, nnzb_ = 0;
+    int64_t n_ = 0, nnzb_ = 0, gridNnzb_ = 0, nAux_ = 0;
     std::vector<int32_t> rowPtr_, colIdx_;
-    std::vector<double> jacobian_, residual_;
+    std::vector<double> jacobian_, residual_, flows_, flores_;
+    std::vector<GpuAuxiliaryModule*> modules_;
 };
 
 /* Linear-solver backend: prepare(M, b), setResidual, getResidual, solve(x),

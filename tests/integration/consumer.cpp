@@ -9,7 +9,8 @@
 //   <dir>/dims.txt: nx ny nz n_src n_oil n_gas dx dy dz gravity dt
 //   <dir>/<name>.bin: perm poro oil_tab gas_tab consts src_cell src_type src_a
 //                     src_b pv0 mean0 [trans depth volume thpres]
-// writes jac0 res0 x pv1 mean1 jac1 res1 (.out) and prints one summary line.
+// writes jac0 res0 x pv1 mean1 jac1 res1 flows jac2 res2 (.out) and prints one
+// summary line.
 // Exit codes: 0 done, 3 an error of the library (message on stderr).
 #include <cstdio>
 #include <cstdint>
@@ -56,6 +57,34 @@ struct StandInSimulator {
     std::vector<double> trans, depth, volume, thpres;
 };
 
+// an auxiliary module (BaseAuxiliaryModule's role): one DOF whose first
+// unknown follows cell 0's pressure, coupled into cell 0's water equation
+struct PressureProbe : omodels_opm::GpuAuxiliaryModule {
+    unsigned numDofs() const override { return 1; }
+    void addNeighbors(std::vector<std::pair<int32_t, int32_t>>& e) const override
+    {
+        const int32_t a = localToGlobalDof(0);
+        e.push_back({a, a});
+        e.push_back({0, a});
+        e.push_back({a, 0});
+    }
+    void applyInitial(std::vector<double>& x) override { x = {1.0e5, 0.0, 0.0}; }
+    void linearize(const std::vector<double>& pv, const std::vector<uint8_t>&, const std::vector<double>& x,
+                   omodels_opm::AuxTerms& t) override
+    {
+        const int32_t a = localToGlobalDof(0);
+        const double k = 1.0e-6, dp = x[0] - pv[1];
+        t.resRow = {a, 0};
+        t.res = {dp, x[1], x[2], 0.0, k * dp, 0.0};
+        t.blockRow = {a, a, 0, 0};
+        t.blockCol = {a, 0, 0, a};
+        t.blocks = {1, 0, 0, 0, 1, 0, 0, 0, 1,          // d(row a)/d(x)
+                    0, -1, 0, 0, 0, 0, 0, 0, 0,         // d(row a)/d(cell 0): -dp0
+                    0, 0, 0, 0, -k, 0, 0, 0, 0,         // d(cell 0 water)/d(p0)
+                    0, 0, 0, k, 0, 0, 0, 0, 0};         // d(cell 0 water)/d(x0)
+    }
+};
+
 }  // namespace
 
 template <>
@@ -83,6 +112,13 @@ struct omodels_opm::Glue<StandInSimulator> {
     {
         s.pv = pv;
         s.meaning = meaning;
+    }
+    // the reservoir problem's boundaries are closed: no face has a condition
+    static void boundaryConditions(const StandInSimulator&, std::vector<int32_t>& cell, std::vector<int32_t>& face,
+                                   std::vector<int32_t>& type, std::vector<double>& rate, std::vector<double>& state,
+                                   std::vector<uint8_t>& meaning)
+    {
+        cell.clear(); face.clear(); type.clear(); rate.clear(); state.clear(); meaning.clear();
     }
     static double timeStepSize(const StandInSimulator& s) { return s.dt; }
     static int newtonIteration(const StandInSimulator& s) { return s.newtonIteration; }
@@ -149,6 +185,7 @@ int main(int argc, char** argv)
         omodels_default_params(&prm);
         omodels_opm::GpuBiCGStabBackend backend(lin.handle(), prm);
 
+        lin.updateBoundaryConditionData();
         sim.newtonIteration = 0;
         lin.linearizeDomain();
         writeBin(dir + "/jac0.out", lin.jacobian());
@@ -170,6 +207,14 @@ int main(int argc, char** argv)
         const std::vector<double> j1 = lin.jacobian(), r1 = lin.residual();
         lin.linearizeDomain(all);
         const bool same = lin.jacobian() == j1 && lin.residual() == r1;
+        // FLOWS of the current state, then linearize() with an auxiliary module
+        lin.updateFlowsInfo();
+        writeBin(dir + "/flows.out", lin.getFlowsInfo());
+        PressureProbe probe;
+        lin.addAuxiliaryModule(&probe);
+        lin.linearize();
+        writeBin(dir + "/jac2.out", lin.jacobian());
+        writeBin(dir + "/res2.out", lin.residual());
         std::printf("{\"converged\": %d, \"iterations\": %d, \"switched\": %d, \"cells\": %lld, "
                     "\"subdomain_same\": %d}\n",
                     converged ? 1 : 0, backend.iterations(), switched, (long long)lin.numCells(), same ? 1 : 0);

@@ -94,3 +94,17 @@ def test_consumer_newton_iteration_matches_the_binding(tmp_path, distorted):
     assert np.array_equal(out["mean1"], mean1)
     assert '"converged": %d' % (1 if rc == 0 else 0) in r.stdout
     assert '"subdomain_same": 1' in r.stdout          # linearizeDomain(domain) over every cell
+    # updateFlowsInfo, then linearize() with the consumer's auxiliary module
+    fl, _ = m.flows()
+    assert np.array_equal(_bits(np.fromfile(os.path.join(d, "flows.out"))), _bits(fl.ravel()))
+    N = m.N
+    m.set_auxiliary(1, [N, 0, N], [N, N, 0])
+    m.set_auxiliary_solution([1.0e5, 0.0, 0.0])
+    J2, r2 = m.linearize(DT, iteration_index=1)
+    k, dp = 1.0e-6, 1.0e5 - pv1[1]
+    blocks = np.array([[1, 0, 0, 0, 1, 0, 0, 0, 1], [0, -1, 0, 0, 0, 0, 0, 0, 0],
+                       [0, 0, 0, 0, -k, 0, 0, 0, 0], [0, 0, 0, k, 0, 0, 0, 0, 0]], dtype=np.float64)
+    m.add_auxiliary([N, N, 0, 0], [N, 0, 0, N], blocks, [N, 0], [[dp, 0.0, 0.0], [0.0, k * dp, 0.0]])
+    J2, r2 = m.system()
+    assert np.array_equal(_bits(np.fromfile(os.path.join(d, "jac2.out"))), _bits(J2))
+    assert np.array_equal(_bits(np.fromfile(os.path.join(d, "res2.out"))), _bits(r2))
