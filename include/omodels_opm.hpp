@@ -34,6 +34,7 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -56,6 +57,15 @@ inline int check(int rc, const char* what)
 
 template <class Simulator>
 struct Glue;    // specialized by the integrating code
+
+/* LinearizationType (linearizationtype.hh:34-39): the linearizer stores it
+ * (tpfalinearizer.hh:300-306); its TPFA path linearizes fully implicitly
+ * whatever it holds, and so does this one. */
+struct LinearizationType {
+    enum VarType { implicit, pressure, seqtransport };
+    VarType type = implicit;
+    unsigned int time = 0;
+};
 
 /* An auxiliary module (BaseAuxiliaryModule, baseauxiliarymodule.hh:55-121):
  * its DOFs follow the cells (setDofOffset), addNeighbors lists the pattern
@@ -266,6 +276,12 @@ public:
               "set_boundary_conditions");
     }
 
+    void setLinearizationType(LinearizationType t) { linearizationType_ = t; }
+    const LinearizationType& getLinearizationType() const { return linearizationType_; }
+    /* the TPFA linearizer has no constrained DOFs (tpfalinearizer.hh:366-367) */
+    template <class Constraints = int>
+    std::map<unsigned, Constraints> constraintsMap() const { return {}; }
+
     const std::vector<int32_t>& rowPtr() const { return rowPtr_; }
     const std::vector<int32_t>& colIdx() const { return colIdx_; }
     const std::vector<double>& jacobian() const { return jacobian_; }
@@ -295,6 +311,7 @@ private:
     std::vector<int32_t> rowPtr_, colIdx_;
     std::vector<double> jacobian_, residual_, flows_, flores_;
     std::vector<GpuAuxiliaryModule*> modules_;
+    LinearizationType linearizationType_;
 };
 
 /* Linear-solver backend: prepare(M, b), setResidual, getResidual, solve(x),

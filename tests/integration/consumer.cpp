@@ -186,6 +186,11 @@ int main(int argc, char** argv)
         omodels_opm::GpuBiCGStabBackend backend(lin.handle(), prm);
 
         lin.updateBoundaryConditionData();
+        omodels_opm::LinearizationType lt;
+        lt.time = 1;
+        lin.setLinearizationType(lt);
+        if (lin.getLinearizationType().time != 1 || !lin.constraintsMap().empty())
+            throw std::runtime_error("linearization type / constraints");
         sim.newtonIteration = 0;
         lin.linearizeDomain();
         writeBin(dir + "/jac0.out", lin.jacobian());
