@@ -520,7 +520,7 @@ class Model:
         linearization of simulate (nonzero: the linearization failed)"""
         if fn is None:
             self._aux_fn = None
-            check(lib().omodels_model_set_auxiliary_module(self.h, None, None), "set_auxiliary_module")
+            check(lib().omodels_model_set_auxiliary_module(self.h, AUX_FN(), None), "set_auxiliary_module")
             return
 
         def tramp(user, handle, it):

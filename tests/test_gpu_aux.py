@@ -154,7 +154,8 @@ def test_failing_module_halves_the_step_like_the_oracle():
     assert np.array_equal(_bits(m.get_auxiliary_solution()), _bits(auxo))
 
 
-@pytest.mark.parametrize("what", ["range", "diagonal", "outside", "no_module", "amg", "stale_solver"])
+@pytest.mark.parametrize("what", ["range", "diagonal", "outside", "no_module", "cleared_module", "amg",
+                                  "stale_solver"])
 def test_invalid_use_is_rejected(what):
     c = case(3, 2, 2)
     N = c.N
@@ -174,6 +175,13 @@ def test_invalid_use_is_rejected(what):
     elif what == "no_module":
         m.set_auxiliary(1, [N, N, 0], [N, 0, N])
         m.set_solution(c.pv0, c.mean0)
+        with pytest.raises(om.OmodelsError):
+            m.simulate(om.default_params(), 1, 86400.0)
+    elif what == "cleared_module":
+        m.set_auxiliary(1, [N, N, 0], [N, 0, N])
+        m.set_solution(c.pv0, c.mean0)
+        m.set_auxiliary_module(lambda model, it: 0)
+        m.set_auxiliary_module(None)
         with pytest.raises(om.OmodelsError):
             m.simulate(om.default_params(), 1, 86400.0)
     elif what == "amg":
