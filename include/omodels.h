@@ -240,6 +240,15 @@ int omodels_model_set_rock(omodels_model* m, const double* compressibility, cons
 int omodels_model_set_boundary_conditions(omodels_model* m, int32_t n, const int32_t* cell, const int32_t* face,
                                           const int32_t* type, const double* mass_rate, const double* state,
                                           const uint8_t* meaning, const double* trans, const double* face_depth);
+/* The boundary faces' entries of FLOWS (updateFlowsInfo, tpfalinearizer.hh:
+ * 688-704: a face with a condition records its term, the values added to
+ * the residual; the rest of the reference's boundary entries stay zero, and
+ * FLORES records none) at solution(0): returns the number of faces set by
+ * omodels_model_set_boundary_conditions and writes, for each in cell and
+ * then face order, cell[b], face[b] and flows[b][3] (each may be NULL;
+ * type-0 faces give zeros).  omodels_linearizer_flows holds the interior
+ * faces. */
+int omodels_linearizer_boundary_flows(omodels_model* m, int32_t* cell, int32_t* face, double* flows);
 /* Auxiliary equations (BaseAuxiliaryModule, baseauxiliarymodule.hh:55-121):
  * n_dofs degrees of freedom after the N cells (rows N .. N + n_dofs - 1 of
  * the system, each with the model's three equations, setDofOffset) and the

@@ -711,7 +711,8 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
 // each face's term added to the residual and the diagonal block.
 __global__ void k_bo_boundary(const BoFluid* __restrict__ fp, BoGrid G, const double* __restrict__ pv,
                               const uint8_t* __restrict__ mean, BoBc bc, const double* __restrict__ depth,
-                              double* __restrict__ res, double* __restrict__ jac, const int* __restrict__ diagOff)
+                              double* __restrict__ res, double* __restrict__ jac, const int* __restrict__ diagOff,
+                              double* __restrict__ flows)
 {
     const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= bc.nGroups) return;
@@ -722,12 +723,15 @@ __global__ void k_bo_boundary(const BoFluid* __restrict__ fp, BoGrid G, const do
     const double zI = depth ? depth[I] : G.dz * ((double)(I / nxy) + 0.5);
     IqE qi;
     iqEval(f, pv[3 * I], pv[3 * I + 1], pv[3 * I + 2], mean[I], qi);
-    double r[3], D[9];
-    double* Dout = jac + 9L * diagOff[I];
+    double r[3] = {0.0, 0.0, 0.0}, D[9] = {};
+    double* Dout = nullptr;
+    if (!flows) {        // flows: only the faces' terms are written (res, jac, diagOff unused)
+        Dout = jac + 9L * diagOff[I];
 #pragma unroll
-    for (int e = 0; e < 3; ++e) r[e] = res[3 * I + e];
+        for (int e = 0; e < 3; ++e) r[e] = res[3 * I + e];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) D[q] = Dout[q];
+        for (int q = 0; q < 9; ++q) D[q] = Dout[q];
+    }
     for (int q = 0; q < grp.y; ++q) {
         const int b = grp.x + q;
         const int type = bc.type[b];
@@ -743,6 +747,11 @@ __global__ void k_bo_boundary(const BoFluid* __restrict__ fp, BoGrid G, const do
             iqValues(f, bc.state[3 * b], bc.state[3 * b + 1], bc.state[3 * b + 2], bc.meaning[b], qb);
             boundaryFreeFlux(f, qi, qb, bc.trans[b], area, (zI - bc.zFace[b]) * G.g, out);
         }
+        if (flows) {     // the face's term as FLOWS records it (tpfalinearizer.hh:700-703)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) flows[3L * b + e] = out[e].v;
+            continue;
+        }
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
             r[e] += out[e].v;
@@ -751,6 +760,7 @@ __global__ void k_bo_boundary(const BoFluid* __restrict__ fp, BoGrid G, const do
             D[3 * e + 2] += out[e].d2;
         }
     }
+    if (flows) return;
 #pragma unroll
     for (int e = 0; e < 3; ++e) res[3 * I + e] = r[e];
 #pragma unroll
@@ -758,11 +768,12 @@ __global__ void k_bo_boundary(const BoFluid* __restrict__ fp, BoGrid G, const do
 }
 
 void launchBoBoundary(const BoFluid* fp, const BoGrid& G, const double* pv, const uint8_t* mean, const BoBc& bc,
-                      const double* depth, double* res, double* jac, const int* diagOff, hipStream_t s)
+                      const double* depth, double* res, double* jac, const int* diagOff, hipStream_t s,
+                      double* flows)
 {
     if (bc.nGroups <= 0) return;
     hipLaunchKernelGGL(k_bo_boundary, dim3((unsigned)((bc.nGroups + 127) / 128)), dim3(128), 0, s, fp, G, pv, mean, bc,
-                       depth, res, jac, diagOff);
+                       depth, res, jac, diagOff, flows);
 }
 
 void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,

@@ -338,6 +338,11 @@ int omodels_linearizer_add_auxiliary(omodels_model* h, int32_t n_blocks, const i
     });
 }
 
+int omodels_linearizer_boundary_flows(omodels_model* h, int32_t* cell, int32_t* face, double* flows)
+{
+    return guarded([&] { return h->m->boundaryFlows(flows, cell, face); });
+}
+
 int omodels_model_set_directional_mobility(omodels_model* h, const double* multipliers)
 {
     return guarded([&] {

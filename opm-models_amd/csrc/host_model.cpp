@@ -287,6 +287,8 @@ void Model::setBoundary(int n, const int* cell, const int* face, const int* type
     }
     sync();
     bc = BoBc{};
+    bcCellHost = c;
+    bcFaceHost = fa;
     bcGroups.release(); bcCell.release(); bcFace.release(); bcType.release(); bcRate.release(); bcState.release();
     bcMean.release(); bcTrans.release(); bcZ.release();
     if (fs.empty()) return;
@@ -302,6 +304,26 @@ void Model::setBoundary(int n, const int* cell, const int* face, const int* type
     sync();
     bc = BoBc{(int)groups.size(), bcGroups.p, bcCell.p, bcFace.p, bcType.p, bcRate.p, bcState.p, bcMean.p,
               bcTrans.p, bcZ.p};
+}
+
+// the boundary faces' terms at solution(0) as FLOWS records them (the
+// faces in cell and face order; type-0 faces zero)
+int Model::boundaryFlows(double* hostFlows, int* hostCell, int* hostFace)
+{
+    const int n = (int)bcCellHost.size();
+    if (n == 0) return 0;
+    if (hostCell) std::copy(bcCellHost.begin(), bcCellHost.end(), hostCell);
+    if (hostFace) std::copy(bcFaceHost.begin(), bcFaceHost.end(), hostFace);
+    if (hostFlows) {
+        DevBuf<double> f;
+        f.alloc(3 * (size_t)n);
+        OM_HIP(hipMemsetAsync(f.p, 0, 3 * (size_t)n * sizeof(double), stream));
+        launchBoBoundary(fluidDev.p, G, pv0.p, mean0.p, bc, geo.depth, nullptr, nullptr, nullptr, stream, f.p);
+        OM_HIP(hipGetLastError());
+        f.download(hostFlows, 3 * (size_t)n, stream);
+        sync();
+    }
+    return n;
 }
 
 void Model::computeSourceBlocks()

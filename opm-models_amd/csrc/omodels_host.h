@@ -176,6 +176,8 @@ public:
     void setBoundary(int n, const int* cell, const int* face, const int* type, const double* rate,
                      const double* state, const uint8_t* meaning, const double* trans, const double* zface);
     BoBc bc{};
+    std::vector<int> bcCellHost, bcFaceHost;   // the faces in cell and face order
+    int boundaryFlows(double* hostFlows, int* hostCell, int* hostFace);
     DevBuf<int2> bcGroups;
     DevBuf<int> bcCell, bcFace, bcType;
     DevBuf<double> bcRate, bcState, bcTrans, bcZ;

@@ -188,7 +188,8 @@ void launchBoSourceBlocks(const BoFluid& f, const BoGrid& G, const double* pv, c
                           int nsrc, const int* cell, const int* type, const double* a, const double* b,
                           double* outRes, double* outDiag, hipStream_t s, const double* vol = nullptr);
 void launchBoBoundary(const BoFluid* fp, const BoGrid& G, const double* pv, const uint8_t* mean, const BoBc& bc,
-                      const double* depth, double* res, double* jac, const int* diagOff, hipStream_t s);
+                      const double* depth, double* res, double* jac, const int* diagOff, hipStream_t s,
+                      double* flows = nullptr);
 void launchBoFlows(const BoFluid* fdev, const BoGrid& G, const double* pv, const uint8_t* mean, const double* trans,
                    const int* rowPtr, double* flows, double* flores, hipStream_t s, const BoDiff* dif = nullptr,
                    const BoDisp* dsp = nullptr, const BoGeo* geo = nullptr);

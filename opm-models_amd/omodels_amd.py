@@ -154,6 +154,7 @@ SIGNATURES = {
     "omodels_model_get_auxiliary_solution": (C.c_int, [C.c_void_p, _dp]),
     "omodels_linearizer_add_auxiliary": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _dp]),
     "omodels_model_set_auxiliary_module": (C.c_int, [C.c_void_p, AUX_FN, C.c_void_p]),
+    "omodels_linearizer_boundary_flows": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
     "omodels_model_set_boundary_conditions": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip, _dp, _dp, _u8p, _dp, _dp]),
     "omodels_model_set_directional_mobility": (C.c_int, [C.c_void_p, _dp]),
     "omodels_solver_amg_levels": (C.c_int, [C.c_void_p, _ip, C.c_int]),
@@ -530,6 +531,18 @@ class Model:
                 return 1
         self._aux_fn = AUX_FN(tramp)
         check(lib().omodels_model_set_auxiliary_module(self.h, self._aux_fn, None), "set_auxiliary_module")
+
+    def boundary_flows(self):
+        """(cell, face, flows [n][3]) of the boundary faces with a condition,
+        in cell and face order"""
+        n = check(lib().omodels_linearizer_boundary_flows(self.h, None, None, None), "boundary_flows")
+        cell = np.zeros(n, np.int32)
+        face = np.zeros(n, np.int32)
+        fl = np.zeros((n, 3))
+        if n:
+            check(lib().omodels_linearizer_boundary_flows(self.h, ptr(cell, _ip), ptr(face, _ip), ptr(fl)),
+                  "boundary_flows")
+        return cell, face, fl
 
     def set_directional_mobility(self, mult=None):
         """directional relative permeability multipliers [3][3][N] (axis x, y,

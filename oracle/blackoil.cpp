@@ -1782,6 +1782,29 @@ int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2)
     } catch (...) { return -1; }
 }
 
+int ob_boundary_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cell, int32_t* face,
+                      double* flows)
+{
+    try {
+        Problem pr(p);
+        const int n = (int)pr.bcs.size();
+        for (int b = 0; b < n; ++b) {
+            const Problem::Bc& bc = pr.bcs[b];
+            if (cell) cell[b] = (int32_t)bc.cell;
+            if (face) face[b] = bc.face;
+            if (!flows) continue;
+            for (int e = 0; e < 3; ++e) flows[3 * b + e] = 0.0;
+            if (bc.type == 0) continue;
+            IQ<E3> qi;
+            pr.iq<E3>(pv, meaning, bc.cell, qi);
+            E3 f[3];
+            boundaryFlux(pr.pvt, qi, bc, pr.G.area(bc.face), pr.G.zc(bc.cell), pr.G.g, f);
+            for (int e = 0; e < 3; ++e) flows[3 * b + e] = f[e].v;
+        }
+        return n;
+    } catch (...) { return -1; }
+}
+
 int ob_add_auxiliary(const ob_problem* p, int32_t nb, const int32_t* brow, const int32_t* bcol,
                      const double* blocks, int32_t nr, const int32_t* rrow, const double* rvals, double* jac,
                      double* residual)

@@ -163,6 +163,10 @@ int ob_pattern_size(const ob_problem* p, int64_t* nnzb, int64_t* nfaces2);
  * their values but for the domain cells' column blocks added into them) */
 int ob_linearize_domain(const ob_problem* p, const double* pv, const uint8_t* meaning, const double* storage1,
                         double dt, int32_t n, const int32_t* cells, double* residual, double* jac);
+/* the boundary faces' FLOWS entries (tpfalinearizer.hh:688-704) in cell and
+ * face order; returns their number (cell, face, flows [n][3] may be NULL) */
+int ob_boundary_flows(const ob_problem* p, const double* pv, const uint8_t* meaning, int32_t* cell, int32_t* face,
+                      double* flows);
 /* linearizeAuxiliaryEquations (tpfalinearizer.hh:254-281): what the
  * auxiliary module's linearize(matrix, residual) adds, in order: block k
  * (row brow[k], column bcol[k], row-major 3x3) into J, entry k of the

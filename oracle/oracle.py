@@ -134,6 +134,7 @@ def lib():
         L.ob_simulate_aux.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, OB_AUX_FN, C.c_void_p, C.c_int,
                                       C.c_double, C.POINTER(ObParams), _dp, _dp, C.POINTER(ObIterLog), C.c_int,
                                       _ip, C.c_int]
+        L.ob_boundary_flows.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _ip, _ip, _dp]
         L.ob_linearize_domain.argtypes = [C.POINTER(ObProblem), _dp, _u8p, _dp, C.c_double, C.c_int32, _ip, _dp,
                                           _dp]
         L.ob_add_auxiliary.argtypes = [C.POINTER(ObProblem), C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _dp, _dp,
@@ -357,6 +358,17 @@ def linearize_domain(prob, pv, mean, st1, dt, cells, J, r):
     rc = lib().ob_linearize_domain(C.byref(prob), P(pv), P(mean, _u8p), P(st1), dt, c.size, P(c, _ip), P(r), P(J))
     assert rc == 0
     return J, r
+
+
+def boundary_flows(prob, pv, mean):
+    """(cell, face, flows [n][3]) of the boundary faces, cell and face order"""
+    n = lib().ob_boundary_flows(C.byref(prob), P(pv), P(mean, _u8p), None, None, None)
+    assert n >= 0
+    cell = np.zeros(max(n, 1), np.int32)
+    face = np.zeros(max(n, 1), np.int32)
+    fl = np.zeros((max(n, 1), 3))
+    assert lib().ob_boundary_flows(C.byref(prob), P(pv), P(mean, _u8p), P(cell, _ip), P(face, _ip), P(fl)) == n
+    return cell[:n], face[:n], fl[:n]
 
 
 def linearize_times(prob, pv, mean, st1, dt, nthreads, reps=5):

@@ -66,6 +66,10 @@ def test_linearize_parity(dims, seed, given):
     J, r = m.linearize(43200.0, iteration_index=1)
     Jo, ro = orc.linearize(prob, pv, mean, st1, 43200.0, nthreads=16)
     assert np.array_equal(_bits(J), _bits(Jo)) and np.array_equal(_bits(r), _bits(ro))
+    # the boundary faces' FLOWS entries
+    gc, gf, gfl = m.boundary_flows()
+    oc, of, ofl = orc.boundary_flows(prob, pv, mean)
+    assert np.array_equal(gc, oc) and np.array_equal(gf, of) and np.array_equal(_bits(gfl), _bits(ofl))
     # removed again: the plain bits
     m.set_boundary(None)
     J0, r0 = m.linearize(43200.0, iteration_index=1)

@@ -224,3 +224,29 @@ def test_invalid_boundaries_are_rejected(what):
     prob = orc.with_boundary(orc.problem_from_case(c), **kw)
     with pytest.raises(AssertionError):
         orc.flows(prob, c.pv0, c.mean0)
+
+
+def test_boundary_flows_are_the_terms_added_to_the_residual():
+    """One face per cell: each face's FLOWS entry is exactly what its cell's
+    residual gains; type-0 faces give zeros; the faces come in cell and face
+    order whatever the input order."""
+    c = case(4, 4, 2, seed=23)
+    pv, mean = mixed_state(c, seed=23)
+    plain = orc.problem_from_case(c)
+    st1 = orc.storage(plain, *mixed_state(c, seed=24))
+    _, r0 = _lin(plain, pv, mean, st1)
+    cells = np.array([15, 0, 5, 10, 3])
+    faces = np.array([3, 0, 4, 4, 1])
+    types = np.array([2, 1, 3, 0, 2])
+    rng = np.random.default_rng(23)
+    state = np.stack([pv[3 * I:3 * I + 3] * [1.0, 1.03, 1.0] for I in cells])
+    prob = orc.with_boundary(orc.problem_from_case(c), cell=cells, face=faces, type=types,
+                             rate=rng.uniform(-1e-3, 1e-3, (5, 3)), state=state, meaning=mean[cells])
+    _, r1 = _lin(prob, pv, mean, st1)
+    bc, bf, fl = orc.boundary_flows(prob, pv, mean)
+    assert list(bc) == sorted(cells) and len(bf) == 5
+    for k in range(5):
+        I = bc[k]
+        assert np.array_equal(_bits(r1[3 * I:3 * I + 3]), _bits(r0[3 * I:3 * I + 3] + fl[k]))
+        if types[list(cells).index(I)] == 0:
+            assert not np.any(fl[k])
