@@ -253,9 +253,29 @@ public:
         flows_.assign(3 * nf, 0.0);
         flores_.assign(3 * nf, 0.0);
         check(omodels_linearizer_flows(model_, flows_.data(), flores_.data()), "flows");
+        // getVelocityInfo: the reference's darcyFlux value (FLORES, the
+        // reservoir rate through the face) divided by the face area
+        // (tpfalinearizer.hh:759-762)
+        const omodels_blackoil_problem p = G::describe(*simulator_);
+        const double area[3] = {p.dy * p.dz, p.dx * p.dz, p.dx * p.dy};
+        std::vector<int32_t> nbPtr((size_t)n_ + 1), nbCell(nf);
+        check(omodels_linearizer_pattern(model_, nullptr, nullptr, nullptr, nbPtr.data(), nbCell.data(), nullptr,
+                                         nullptr, nullptr),
+              "pattern");
+        velocities_.assign(3 * nf, 0.0);
+        for (int64_t I = 0; I < n_; ++I)
+            for (int32_t k = nbPtr[I]; k < nbPtr[I + 1]; ++k) {
+                const int64_t d = nbCell[k] > I ? nbCell[k] - I : I - nbCell[k];
+                const int ax = (p.nx > 1 && d == 1) ? 0 : ((p.ny > 1 && d == p.nx) ? 1 : 2);
+                for (int e = 0; e < 3; ++e) velocities_[3 * (size_t)k + e] = flores_[3 * (size_t)k + e] / area[ax];
+            }
     }
     const std::vector<double>& getFlowsInfo() const { return flows_; }
     const std::vector<double>& getFloresInfo() const { return flores_; }
+    /* per neighbour entry and equation: FLORES / face area, the velocities
+     * the reference's dispersion reads (it fills them when dispersion is
+     * enabled; here after every updateFlowsInfo) */
+    const std::vector<double>& getVelocityInfo() const { return velocities_; }
 
     /* updateBoundaryConditionData: the faces with a condition from
      * Glue::boundaryConditions(sim, cell, face, type, massRate, state,
@@ -309,7 +329,7 @@ private:
     omodels_model* model_ = nullptr;
     int64_t n_ = 0, nnzb_ = 0, gridNnzb_ = 0, nAux_ = 0;
     std::vector<int32_t> rowPtr_, colIdx_;
-    std::vector<double> jacobian_, residual_, flows_, flores_;
+    std::vector<double> jacobian_, residual_, flows_, flores_, velocities_;
     std::vector<GpuAuxiliaryModule*> modules_;
     LinearizationType linearizationType_;
 };

@@ -215,6 +215,7 @@ int main(int argc, char** argv)
         // FLOWS of the current state, then linearize() with an auxiliary module
         lin.updateFlowsInfo();
         writeBin(dir + "/flows.out", lin.getFlowsInfo());
+        writeBin(dir + "/velocity.out", lin.getVelocityInfo());
         PressureProbe probe;
         lin.addAuxiliaryModule(&probe);
         lin.linearize();

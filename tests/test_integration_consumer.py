@@ -95,8 +95,18 @@ def test_consumer_newton_iteration_matches_the_binding(tmp_path, distorted):
     assert '"converged": %d' % (1 if rc == 0 else 0) in r.stdout
     assert '"subdomain_same": 1' in r.stdout          # linearizeDomain(domain) over every cell
     # updateFlowsInfo, then linearize() with the consumer's auxiliary module
-    fl, _ = m.flows()
+    fl, fr = m.flows()
     assert np.array_equal(_bits(np.fromfile(os.path.join(d, "flows.out"))), _bits(fl.ravel()))
+    # velocities: FLORES over the face's area (axis from the neighbour's offset)
+    pat = m.pattern()
+    sp = c.spec
+    area = np.array([sp.dy * sp.dz, sp.dx * sp.dz, sp.dx * sp.dy])
+    vel = np.zeros_like(fr)
+    for I in range(c.N):
+        for k in range(pat["nb_ptr"][I], pat["nb_ptr"][I + 1]):
+            dd = abs(int(pat["nb_cell"][k]) - I)
+            vel[k] = fr[k] / area[0 if dd == 1 else (1 if dd == sp.nx else 2)]
+    assert np.array_equal(_bits(np.fromfile(os.path.join(d, "velocity.out"))), _bits(vel.ravel()))
     N = m.N
     m.set_auxiliary(1, [N, 0, N], [N, N, 0])
     m.set_auxiliary_solution([1.0e5, 0.0, 0.0])
